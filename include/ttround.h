@@ -1,0 +1,169 @@
+/*
+ * ttround.h -- C-ABI of libttround, the MI355X (gfx950) TT-rounding library.
+ *
+ * Drop-in boundary for the rounding hot path of the Julia package
+ * TensorTrains (rodrigofj99/Tensor-Train-Julia).  Every entry point names the
+ * reference function it replaces (file:line in that repository).  A Julia
+ * caller reaches these with ccall; the host mirror in
+ * tensor-train-julia_amd/ttround/ reaches them with ctypes.
+ *
+ * Conventions
+ *  - All functions return a status code (TTR_OK == 0); on failure
+ *    ttr_last_error(h) describes it.  No C++ exception crosses this ABI.
+ *  - Host cores are passed in the reference layout: core k of a TT vector is a
+ *    column-major (n_k, r_{k-1}, r_k) array of doubles (TTvector.ttv_vec[k],
+ *    src/tt_tools.jl:24-34); core k of a TT operator is (n_k, n_k, R_{k-1}, R_k)
+ *    (TToperator.tto_vec[k], src/tt_tools.jl:61-71).  Ranks arrays have d+1
+ *    entries with r_0 = r_d = 1.  Host buffers are borrowed for the duration of
+ *    the call only.
+ *  - TT objects live in device memory owned by the handle and are named by
+ *    int64 ids.  Results are new objects; the library never frees caller memory
+ *    and callers release objects with ttr_tt_free.
+ *  - One handle per host thread; a handle owns one HIP stream and its
+ *    workspace.  Every call is synchronous on return.
+ */
+#ifndef TTROUND_H
+#define TTROUND_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  TTR_OK = 0,
+  TTR_INVALID_ARG = 1,       /* reference: ArgumentError / AssertionError       */
+  TTR_DIM_MISMATCH = 2,      /* reference: DimensionMismatch / AssertionError   */
+  TTR_OOM = 3,
+  TTR_NUMERICAL_FAILURE = 4, /* reference: LAPACKException                      */
+  TTR_DEVICE_ERROR = 5,
+  TTR_NOT_FOUND = 6          /* unknown object id                               */
+};
+
+enum { TTR_PREC_F64 = 0, TTR_PREC_F32 = 1, TTR_PREC_BF16 = 2 };
+enum { TTR_DIR_LEFT = 0, TTR_DIR_RIGHT = 1 };
+
+typedef struct ttr_handle_s* ttr_handle_t;
+
+/* library identification */
+const char* ttr_version(void);
+
+/* creates a handle on HIP device `device` (one stream, empty workspace) */
+int ttr_create(ttr_handle_t* h, int device);
+int ttr_destroy(ttr_handle_t h);
+/* message of the last failed call on this handle ("" if none) */
+const char* ttr_last_error(ttr_handle_t h);
+
+/* ---------------------------------------------------------------- objects */
+
+/* Uploads a TTvector.  ot may be NULL (all 0).
+ * Replaces: the TTvector constructor (src/tt_tools.jl:30-33). */
+int ttr_tt_upload(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, const int64_t* ot,
+                  const double* const* cores, int64_t* id);
+
+/* Metadata of a TT object; any output pointer may be NULL.
+ * dims: d entries, rks: d+1, ot: d. */
+int ttr_tt_info(ttr_handle_t h, int64_t id, int64_t* d, int64_t* dims, int64_t* rks, int64_t* ot);
+
+/* Copies core k (0-based) to dst in the reference layout (n_k, r_{k-1}, r_k). */
+int ttr_tt_download_core(ttr_handle_t h, int64_t id, int64_t k, double* dst);
+
+int ttr_tt_free(ttr_handle_t h, int64_t id);
+
+/* Seeded TT with i.i.d. N(0,1) cores drawn on the device, ranks capped as in
+ * rand_tt (src/tt_tools.jl:255-303, r_and_d_to_rks with rmax = max(rks)).
+ * Draw e of core k (reference order) is the library's Philox4x32-10 normal
+ * number (seed, k, e) of the input stream. */
+int ttr_tt_randn(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, uint64_t seed, int64_t* id);
+
+/* Uploads a TToperator (MPO).  Replaces: the TToperator constructor
+ * (src/tt_tools.jl:67-70). */
+int ttr_tto_upload(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, const double* const* cores,
+                   int64_t* id);
+
+/* ------------------------------------------------------- rank-growing ops */
+
+/* out = sum_j alpha_j * x_j with block-diagonal cores; the scalar multiplies
+ * the first core with ot == 0 of each term.
+ * Replaces: *(::Vector{TTvector}, ::Vector) (src/tt_operations.jl:252-258),
+ * +(::TTvector, ::TTvector) (:81-112), *(::Number, ::TTvector) (:428-441),
+ * -(::TTvector, ::TTvector) (:453-455). */
+int ttr_lincomb(ttr_handle_t h, int64_t nterms, const double* alpha, const int64_t* ids, int64_t* out_id);
+
+/* out = A * x (MPO times TT), bond index with the operator rank fastest.
+ * Replaces: *(::TToperator, ::TTvector) (src/tt_operations.jl:170-194). */
+int ttr_mpo_apply(ttr_handle_t h, int64_t op_id, int64_t x_id, int64_t* out_id);
+
+/* ------------------------------------------------- deterministic rounding */
+
+/* Orthogonalizes n TTs around root (1-based, as the reference) with
+ * Householder QR (left cores) and LQ (right cores), FP64.
+ * Replaces: orthogonalize(x; i) (src/tt_rounding.jl:65-97). */
+int ttr_orthogonalize(ttr_handle_t h, int64_t n, const int64_t* ids, int64_t root, int64_t* out_ids);
+
+/* Deterministic SVD rounding of n TTs (batched), FP64: orthogonalization
+ * sweep unless the input is already left- (right-) orthogonal, then the
+ * truncated-SVD sweep with the reference's floor rule and rmax.
+ * Replaces: tt_rounding(x; tol, rmax, direction) (src/tt_rounding.jl:146-154,
+ * 174-269) and tt_rounding!(x; ...) (:158-167). */
+int ttr_round_svd(ttr_handle_t h, int64_t n, const int64_t* ids, double tol, int64_t rmax, int direction,
+                  int64_t* out_ids);
+
+/* Frobenius norms of n TTs.  Replaces: norm(::TTvector) (src/tt_rounding.jl:107-114). */
+int ttr_norm(ttr_handle_t h, int64_t n, const int64_t* ids, double* out);
+
+/* ---------------------------------------------------- randomized rounding */
+
+typedef struct {
+  int64_t nterms;           /* number of terms of the linear combination            */
+  const double* alpha;      /* nterms coefficients                                  */
+  const int64_t* term_ids;  /* nterms TT ids (same dims)                            */
+  int64_t op_id;            /* reserved, must be -1 (explicit products: ttr_mpo_apply) */
+  const int64_t* rks;       /* d+1 target sketch ranks (rks[0] = rks[d] = 1)         */
+} ttr_rand_item;
+
+typedef struct {
+  int32_t orthogonal;   /* generate_sketch_blocks(...; orthogonal) (src/sketches.jl:44-84)   */
+  int32_t sketch_prec;  /* TTR_PREC_*: precision of the sketch contractions                   */
+  int64_t block_rks;    /* <= 0 means d (the reference default block_rks = N)                */
+  uint64_t seed;        /* the reference default is 1234 (src/tt_randtools.jl:112)           */
+  const int64_t* p;     /* optional d+1 block counts (tt_recursive_sketch kwarg p), or NULL   */
+  double oversampling;  /* block-count multiplier (kwarg oversampling; 1 = none)              */
+} ttr_rand_opts;
+
+/* Randomize-then-orthogonalize rounding of n linear combinations in one
+ * batch: per item a right-to-left block Gaussian TT sketch of each term (same
+ * draws for every term and item), then one left-to-right sweep of
+ * Z_k = sum_t Y_t W_t, Householder QR (FP64) and projection (FP64).  Output
+ * cores are left-orthogonal (ot = [1,...,1,0]) with ranks
+ * min(n_k r_{k-1}, sketch_rks_k), untruncated.
+ * Replaces: ttrand_rounding(y, rks; orthogonal, seed, block_rks)
+ * (src/tt_randtools.jl:112-154), ttrand_rounding(y, rmax) (:162-167),
+ * ttrand_rounding(alpha, y, rks) (:223-276) and its rmax form (:278-283). */
+int ttr_round_rand(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const ttr_rand_opts* opts,
+                   int64_t* out_ids);
+
+/* The reverse recursive sketch of one TT: W[k] (r_k x sketch_rks[k],
+ * column-major, k = 0..d) into W_out[k] (NULL entries skipped), the sketch
+ * ranks into sketch_rks (d+1).
+ * Replaces: tt_recursive_sketch(T, A, rks; reverse=true, ...)
+ * (src/sketches.jl:175-240,300-303). */
+int ttr_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts, double* const* W_out,
+               int64_t* sketch_rks);
+
+/* The sketch blocks of core k (0-based) exactly as the device uses them,
+ * upcast to FP64, in the reference order (bv_{k+1}, n_k, bv_k, p_k)
+ * column-major (the array generate_sketch_blocks returns, src/sketches.jl:83).
+ * Writes bv_{k+1}*n_k*bv_k*p_k doubles; shape4 (may be NULL) gets the 4 sizes. */
+int ttr_sketch_blocks(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, const ttr_rand_opts* opts,
+                      int64_t k, double* out, int64_t* shape4);
+
+/* Raw N(0,1) draws number 0..count-1 of core k of the sketch stream. */
+int ttr_sketch_draws(ttr_handle_t h, int64_t k, int64_t count, uint64_t seed, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TTROUND_H */

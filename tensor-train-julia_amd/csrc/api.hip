@@ -1,0 +1,1296 @@
+// libttround: TT rounding drivers and the C-ABI (include/ttround.h).
+//
+// Every driver works on a batch of TTs in lock-step over the core index, so a
+// sweep step over the whole batch is a handful of grouped launches (GEMM, QR
+// panels, Jacobi, copies) on the handle's stream.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "../../include/ttround.h"
+#include "ttr_internal.h"
+
+using namespace ttr;
+
+namespace {
+
+struct DevTT {
+  int d = 0;
+  std::vector<int64_t> dims, rks, ot;
+  std::vector<double*> cores;
+  std::vector<void*> allocs;
+  int64_t core_elems(int k) const { return rks[k] * dims[k] * rks[k + 1]; }
+};
+
+struct DevTTO {
+  int d = 0;
+  std::vector<int64_t> dims, rks;
+  std::vector<double*> cores;  // reference layout (n, n, R, R')
+  void* base = nullptr;
+};
+
+int64_t wrap_mul(int64_t a, int64_t b) { return int64_t(uint64_t(a) * uint64_t(b)); }
+
+int64_t wrap_prod(const std::vector<int64_t>& v, size_t lo, size_t hi) {
+  int64_t p = 1;
+  for (size_t i = lo; i < hi; ++i) p = wrap_mul(p, v[i]);
+  return p;
+}
+
+// r_and_d_to_rks (src/tt_rounding.jl:6-24), including its Int64 overflow guards
+std::vector<int64_t> r_and_d_to_rks(const std::vector<int64_t>& rks, const std::vector<int64_t>& dims, int64_t rmax) {
+  const size_t d = dims.size();
+  std::vector<int64_t> out(rks.size(), 1);
+  for (size_t i = 0; i < d; ++i) {
+    const int64_t right = wrap_prod(dims, i, d), left = wrap_prod(dims, 0, i);
+    int64_t v = std::min(rks[i], rmax);
+    if (right > 0) v = std::min(v, right);
+    if (left > 0) v = std::min(v, left);
+    out[i] = v;
+  }
+  return out;
+}
+
+// floor (src/tt_rounding.jl:297-315) on descending singular values
+int floor_rule(const std::vector<double>& s, double tol, double degen_tol = 1e-5) {
+  const int d = int(s.size());
+  if (tol == 0.0) return d;
+  int i = d;
+  double weight = 0.0, norm2 = 0.0;
+  for (double v : s) norm2 += v * v;
+  const double thr = tol * tol * norm2;
+  while (i > 0 && weight < thr) {
+    weight += s[i - 1] * s[i - 1];
+    --i;
+  }
+  ++i;
+  while (i < d) {
+    const double ratio = s[i] / s[i - 1];
+    if (std::isfinite(ratio) && std::fabs(1.0 - ratio) <= degen_tol) ++i;
+    else break;
+  }
+  return i;
+}
+
+Prec to_prec(int p) {
+  TTR_REQUIRE(p >= 0 && p <= 2, ST_INVALID_ARG, "sketch_prec must be 0 (f64), 1 (f32) or 2 (bf16)");
+  return Prec(p);
+}
+
+float bf16_bits_to_float(uint16_t h) {
+  uint32_t u = uint32_t(h) << 16;
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+}  // namespace
+
+struct ttr_handle_s {
+  Ctx ctx;
+  std::string err;
+  int64_t next_id = 1;
+  std::unordered_map<int64_t, std::unique_ptr<DevTT>> tts;
+  std::unordered_map<int64_t, std::unique_ptr<DevTTO>> ops;
+  double* one = nullptr;  // device constant 1.0
+
+  DevTT* get(int64_t id) {
+    auto it = tts.find(id);
+    TTR_REQUIRE(it != tts.end(), ST_NOT_FOUND, "unknown TT id " + std::to_string(id));
+    return it->second.get();
+  }
+  DevTTO* get_op(int64_t id) {
+    auto it = ops.find(id);
+    TTR_REQUIRE(it != ops.end(), ST_NOT_FOUND, "unknown TT operator id " + std::to_string(id));
+    return it->second.get();
+  }
+  // new TT with all cores allocated (contents undefined)
+  std::unique_ptr<DevTT> make_tt(const std::vector<int64_t>& dims, const std::vector<int64_t>& rks,
+                                 const std::vector<int64_t>& ot) {
+    auto t = std::make_unique<DevTT>();
+    t->d = int(dims.size());
+    t->dims = dims;
+    t->rks = rks;
+    t->ot = ot;
+    int64_t total = 0;
+    for (int k = 0; k < t->d; ++k) total += t->core_elems(k);
+    char* base = static_cast<char*>(ctx.cache.alloc(size_t(std::max<int64_t>(total, 1)) * 8));
+    t->allocs.push_back(base);
+    int64_t off = 0;
+    for (int k = 0; k < t->d; ++k) {
+      t->cores.push_back(reinterpret_cast<double*>(base) + off);
+      off += t->core_elems(k);
+    }
+    return t;
+  }
+  // TT whose cores are filled in later, one cache allocation per core
+  std::unique_ptr<DevTT> make_tt_lazy(const std::vector<int64_t>& dims) {
+    auto t = std::make_unique<DevTT>();
+    t->d = int(dims.size());
+    t->dims = dims;
+    t->rks.assign(dims.size() + 1, 1);
+    t->ot.assign(dims.size(), 0);
+    t->cores.assign(dims.size(), nullptr);
+    return t;
+  }
+  double* core_alloc(DevTT* t, int64_t elems) {
+    void* p = ctx.cache.alloc(size_t(std::max<int64_t>(elems, 1)) * 8);
+    t->allocs.push_back(p);
+    return static_cast<double*>(p);
+  }
+  int64_t add(std::unique_ptr<DevTT> t) {
+    const int64_t id = next_id++;
+    tts[id] = std::move(t);
+    return id;
+  }
+  void free_tt(DevTT* t) {
+    for (void* p : t->allocs) ctx.cache.free(p);
+    t->allocs.clear();
+  }
+};
+
+namespace {
+
+// ------------------------------------------------------------------ helpers
+
+void zeros_like_ranks(ttr_handle_s* h, DevTT* t) {
+  for (int k = 0; k < t->d; ++k) zero_fill(h->ctx, t->cores[k], size_t(t->core_elems(k)) * 8);
+}
+
+void check_dims_equal(const DevTT* a, const DevTT* b) {
+  TTR_REQUIRE(a->dims == b->dims, ST_DIM_MISMATCH, "Incompatible dimensions");
+}
+
+bool is_leftorth(const DevTT* x) {
+  for (int k = 0; k + 1 < x->d; ++k)
+    if (x->ot[k] != 1) return false;
+  return true;
+}
+bool is_rightorth(const DevTT* x) {
+  for (int k = 1; k < x->d; ++k)
+    if (x->ot[k] != -1) return false;
+  return true;
+}
+
+std::vector<double> frob_norms_host(ttr_handle_s* h, const std::vector<const double*>& ptrs,
+                                    const std::vector<int64_t>& counts) {
+  // |x|_F through a one-column GEMM: s = x^T x
+  Ctx& ctx = h->ctx;
+  double* out = ctx.ws.alloc_n<double>(std::max<size_t>(1, ptrs.size()));
+  GemmBatch gb;
+  for (size_t i = 0; i < ptrs.size(); ++i) {
+    if (counts[i] == 0) {
+      gb.prob(out + i, 1, 1, 1, 1, 0.0, 0.0);
+      gb.seg(h->one, 1, 1, h->one, 1, 1, 1, 1.0);
+      continue;
+    }
+    gb.add(1, 1, int(counts[i]), 1.0, ptrs[i], 1, 1, ptrs[i], 1, 1, 0.0, out + i, 1, 1);
+  }
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+  std::vector<double> res(ptrs.size());
+  ctx.sync();
+  TTR_HIP(hipMemcpy(res.data(), out, res.size() * 8, hipMemcpyDeviceToHost));
+  for (auto& v : res) v = std::sqrt(std::max(v, 0.0));
+  return res;
+}
+
+// ------------------------------------------------------------ orthogonalize
+// orthogonalize(x; i=root) (src/tt_rounding.jl:65-97) for a batch
+std::vector<std::unique_ptr<DevTT>> orthogonalize_batch(ttr_handle_s* h, const std::vector<DevTT*>& xs, int root) {
+  Ctx& ctx = h->ctx;
+  const size_t nb = xs.size();
+  std::vector<std::unique_ptr<DevTT>> outs(nb);
+  std::vector<std::vector<int64_t>> yr(nb);
+  int maxd = 0;
+  for (size_t q = 0; q < nb; ++q) {
+    const DevTT* x = xs[q];
+    TTR_REQUIRE(root >= 1 && root <= x->d, ST_DIM_MISMATCH, "Impossible orthogonalization");
+    const int d = x->d, i0 = root - 1;
+    std::vector<int64_t> r = r_and_d_to_rks(x->rks, x->dims, 1024);
+    for (int j = 0; j < i0; ++j) r[j + 1] = std::min(x->dims[j] * r[j], x->rks[j + 1]);
+    for (int j = d - 1; j > i0; --j) r[j] = std::min(x->dims[j] * r[j + 1], x->rks[j]);
+    std::vector<int64_t> ot(d, 0);
+    for (int j = 0; j < i0; ++j) ot[j] = 1;
+    for (int j = i0 + 1; j < d; ++j) ot[j] = -1;
+    outs[q] = h->make_tt(x->dims, r, ot);
+    yr[q] = r;
+    maxd = std::max(maxd, d);
+  }
+  // workspace: FR / FL factors and a working matrix per item
+  std::vector<double*> FR(nb), FL(nb), Abuf(nb), Qw(nb);
+  for (size_t q = 0; q < nb; ++q) {
+    const DevTT* x = xs[q];
+    int64_t mx_a = 1, mx_f = 1;
+    for (int j = 0; j < x->d; ++j) {
+      mx_a = std::max(mx_a, std::max(yr[q][j], x->rks[j]) * x->dims[j] * std::max(yr[q][j + 1], x->rks[j + 1]));
+      mx_f = std::max(mx_f, std::max(yr[q][j], x->rks[j]) * std::max(yr[q][j], x->rks[j]));
+    }
+    FR[q] = ctx.ws.alloc_n<double>(size_t(mx_f));
+    FL[q] = ctx.ws.alloc_n<double>(size_t(mx_f));
+    Abuf[q] = ctx.ws.alloc_n<double>(size_t(mx_a));
+    Qw[q] = ctx.ws.alloc_n<double>(size_t(mx_a));
+    zero_fill(ctx, FR[q], 8);
+    zero_fill(ctx, FL[q], 8);
+  }
+  GemmBatch gb;
+  // ---- left-to-right (cores 0..root-2): A = FR X_j, QR, core = Q, FR = R
+  for (int j = 0; j < root - 1; ++j) {
+    std::vector<QRJob> qj;
+    std::vector<Copy2D> cp;
+    for (size_t q = 0; q < nb; ++q) {
+      const DevTT* x = xs[q];
+      const int64_t n = x->dims[j], rp = yr[q][j], R0 = x->rks[j], R1 = x->rks[j + 1];
+      if (j == 0)
+        cp.push_back(copy_job(x->cores[0], 1, n, Abuf[q], 1, n, int(n), int(R1)));
+      else
+        gb.add(int(rp), int(n * R1), int(R0), 1.0, FR[q], 1, rp, x->cores[j], 1, R0, 0.0, Abuf[q], 1, rp);
+      qj.push_back(QRJob{Abuf[q], rp * n, int(rp * n), int(R1), outs[q]->cores[j], rp * n});
+    }
+    copy2d_batched(ctx, cp);
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    qr_batched(ctx, qj);
+    cp.clear();
+    for (size_t q = 0; q < nb; ++q) {
+      const int64_t n = xs[q]->dims[j], rp = yr[q][j], R1 = xs[q]->rks[j + 1], rn = yr[q][j + 1];
+      cp.push_back(copy_job(Abuf[q], 1, rp * n, FR[q], 1, rn, int(rn), int(R1), 1.0, 1));
+    }
+    copy2d_batched(ctx, cp);
+  }
+  // ---- right-to-left (cores d-1..root): T = (X_j FL)_R^T, QR(T) = Q1 R1, core = Q1^T, FL = R1^T
+  for (int j = maxd - 1; j >= root; --j) {
+    std::vector<QRJob> qj;
+    for (size_t q = 0; q < nb; ++q) {
+      const DevTT* x = xs[q];
+      if (j >= x->d) continue;
+      const int64_t n = x->dims[j], R0 = x->rks[j], R1 = x->rks[j + 1], rn = yr[q][j + 1];
+      const double* fl = (j == x->d - 1) ? h->one : FL[q];
+      const int64_t ldfl = (j == x->d - 1) ? 1 : R1;
+      const int64_t ldT = n * rn;
+      for (int64_t i = 0; i < n; ++i)  // T(i + n b, a) = sum_beta FL(beta, b) X(a, i, beta)
+        gb.add(int(rn), int(R0), int(R1), 1.0, fl, ldfl, 1, x->cores[j] + R0 * i, R0 * n, 1, 0.0, Abuf[q] + i, n, ldT);
+      qj.push_back(QRJob{Abuf[q], ldT, int(ldT), int(R0), Qw[q], ldT});
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    qr_batched(ctx, qj);
+    std::vector<Copy2D> cp;
+    for (size_t q = 0; q < nb; ++q) {
+      const DevTT* x = xs[q];
+      if (j >= x->d) continue;
+      const int64_t n = x->dims[j], R0 = x->rks[j], rn = yr[q][j + 1], k = yr[q][j];
+      const int64_t ldT = n * rn;
+      cp.push_back(copy_job(Qw[q], 1, ldT, outs[q]->cores[j], k, 1, int(ldT), int(k)));       // core = Q1^T
+      cp.push_back(copy_job(Abuf[q], 1, ldT, FL[q], R0, 1, int(k), int(R0), 1.0, 1));          // FL = R1^T
+    }
+    copy2d_batched(ctx, cp);
+  }
+  // ---- root core: FR X_root FL
+  for (size_t q = 0; q < nb; ++q) {
+    const DevTT* x = xs[q];
+    const int i0 = root - 1;
+    const int64_t n = x->dims[i0], R0 = x->rks[i0], R1 = x->rks[i0 + 1], r0 = yr[q][i0], r1 = yr[q][i0 + 1];
+    const double* fr = (i0 == 0) ? h->one : FR[q];
+    const double* fl = (i0 == x->d - 1) ? h->one : FL[q];
+    gb.add(int(r0), int(n * R1), int(R0), 1.0, fr, 1, (i0 == 0) ? 1 : r0, x->cores[i0], 1, R0, 0.0, Abuf[q], 1, r0);
+    (void)fl;
+  }
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+  for (size_t q = 0; q < nb; ++q) {
+    const DevTT* x = xs[q];
+    const int i0 = root - 1;
+    const int64_t n = x->dims[i0], R1 = x->rks[i0 + 1], r0 = yr[q][i0], r1 = yr[q][i0 + 1];
+    const double* fl = (i0 == x->d - 1) ? h->one : FL[q];
+    const int64_t ldfl = (i0 == x->d - 1) ? 1 : R1;
+    gb.add(int(r0 * n), int(r1), int(R1), 1.0, Abuf[q], 1, r0 * n, fl, 1, ldfl, 0.0, outs[q]->cores[i0], 1, r0 * n);
+  }
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+  return outs;
+}
+
+// ---------------------------------------------------------- SVD rounding sweep
+struct SweepState {
+  DevTT* y;                         // orthogonalised input (not modified)
+  std::unique_ptr<DevTT> out;       // result, cores filled as the sweep proceeds
+  std::vector<double*> cur;         // current core pointers
+  std::vector<int64_t> rk;          // current ranks
+  double* ping[2];
+  int pp = 0;
+  double *Mt, *Qb, *B, *J, *sv, *US;
+  int32_t* idx;
+  bool zero = false;
+};
+
+// _tt_rounding (src/tt_rounding.jl:174-269) on a batch of orthogonalised TTs
+void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t rmax, int direction) {
+  Ctx& ctx = h->ctx;
+  const size_t nb = st.size();
+  int maxd = 0;
+  for (auto& s : st) maxd = std::max(maxd, s.y->d);
+  // zero guard on the root core
+  {
+    std::vector<const double*> ptrs;
+    std::vector<int64_t> cnt;
+    for (auto& s : st) {
+      const int k = direction == TTR_DIR_LEFT ? s.y->d - 1 : 0;
+      ptrs.push_back(s.y->cores[k]);
+      cnt.push_back(s.y->core_elems(k));
+    }
+    std::vector<double> nr = frob_norms_host(h, ptrs, cnt);
+    for (size_t q = 0; q < nb; ++q) st[q].zero = nr[q] < tol;
+  }
+  for (auto& s : st) {
+    s.cur = s.y->cores;
+    s.rk = s.y->rks;
+    int64_t mx = 1, mxp = 1;
+    for (int k = 0; k < s.y->d; ++k) {
+      mx = std::max(mx, s.y->core_elems(k));
+      mxp = std::max(mxp, std::min(s.y->rks[k], s.y->dims[k] * s.y->rks[k + 1]));
+      mxp = std::max(mxp, std::min(s.y->rks[k] * s.y->dims[k], s.y->rks[k + 1]));
+    }
+    s.ping[0] = ctx.ws.alloc_n<double>(size_t(mx));
+    s.ping[1] = ctx.ws.alloc_n<double>(size_t(mx));
+    s.Mt = ctx.ws.alloc_n<double>(size_t(mx));
+    s.Qb = ctx.ws.alloc_n<double>(size_t(mx));
+    s.US = ctx.ws.alloc_n<double>(size_t(mx));
+    s.B = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
+    s.J = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
+    s.sv = ctx.ws.alloc_n<double>(size_t(mxp));
+    s.idx = ctx.ws.alloc_n<int32_t>(size_t(mxp));
+    s.out = h->make_tt_lazy(s.y->dims);
+  }
+  GemmBatch gb;
+  const bool left = direction == TTR_DIR_LEFT;
+  const int nsteps = maxd - 1;
+  for (int step = 0; step < nsteps; ++step) {
+    // core index being decomposed and its neighbour
+    std::vector<int> jcore(nb, -1);
+    std::vector<int64_t> P(nb), Qd(nb);
+    std::vector<char> tallcase(nb, 0);
+    std::vector<QRJob> qj;
+    std::vector<Copy2D> cp;
+    for (size_t q = 0; q < nb; ++q) {
+      SweepState& s = st[q];
+      const int d = s.y->d;
+      if (s.zero || step >= d - 1) continue;
+      const int j = left ? d - 1 - step : step;
+      jcore[q] = j;
+      const int64_t n = s.y->dims[j];
+      int64_t p, qq;
+      if (left) { p = s.rk[j]; qq = n * s.rk[j + 1]; }   // right unfolding p x q
+      else { p = s.rk[j] * n; qq = s.rk[j + 1]; }        // left unfolding p x q
+      P[q] = p;
+      Qd[q] = qq;
+      // left: wide (p <= q) -> QR of M^T;  right: tall (p >= q) -> QR of M
+      const bool qr_of_m = left ? (p > qq) : (p >= qq);
+      tallcase[q] = qr_of_m;
+      if (qr_of_m) {
+        cp.push_back(copy_job(s.cur[j], 1, p, s.Mt, 1, p, int(p), int(qq)));
+        qj.push_back(QRJob{s.Mt, p, int(p), int(qq), s.Qb, p});
+      } else {
+        cp.push_back(copy_job(s.cur[j], 1, p, s.Mt, qq, 1, int(p), int(qq)));
+        qj.push_back(QRJob{s.Mt, qq, int(qq), int(p), s.Qb, qq});
+      }
+    }
+    if (qj.empty()) break;
+    copy2d_batched(ctx, cp);
+    qr_batched(ctx, qj);
+    // B from the triangular factor: R (q x q) or R^T, see the per-case algebra below
+    cp.clear();
+    std::vector<SVDJob> sj;
+    std::vector<int64_t> pdim(nb, 0);
+    for (size_t q = 0; q < nb; ++q) {
+      if (jcore[q] < 0) continue;
+      SweepState& s = st[q];
+      const int64_t p = P[q], qq = Qd[q];
+      const bool t = tallcase[q];
+      const int64_t m = t ? qq : p;  // size of the square factor
+      const int64_t ldm = t ? p : qq;
+      pdim[q] = m;
+      // left&tall: B = R; left&wide: B = R^T; right&tall: B = R^T; right&wide: B = R
+      const bool transpose = left ? !t : t;
+      if (transpose) cp.push_back(copy_job(s.Mt, 1, ldm, s.B, m, 1, int(m), int(m), 1.0, 1));
+      else cp.push_back(copy_job(s.Mt, 1, ldm, s.B, 1, m, int(m), int(m), 1.0, 1));
+      sj.push_back(SVDJob{s.B, m, s.J, m, s.sv, int(m)});
+    }
+    copy2d_batched(ctx, cp);
+    jacobi_svd_batched(ctx, sj);
+    // rank decisions on the host
+    ctx.sync();
+    std::vector<int64_t> K(nb, 0);
+    std::vector<GatherCols> gc;
+    for (size_t q = 0; q < nb; ++q) {
+      if (jcore[q] < 0) continue;
+      SweepState& s = st[q];
+      const int64_t m = pdim[q];
+      std::vector<double> sv(m);
+      TTR_HIP(hipMemcpy(sv.data(), s.sv, size_t(m) * 8, hipMemcpyDeviceToHost));
+      std::vector<int32_t> order(m);
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return sv[a] > sv[b]; });
+      std::vector<double> ss(m);
+      int64_t npos = 0;
+      for (int64_t i = 0; i < m; ++i) {
+        ss[i] = sv[order[i]];
+        npos += ss[i] > 0.0;
+      }
+      if (!std::all_of(ss.begin(), ss.end(), [](double v) { return std::isfinite(v); }))
+        throw Error(ST_NUMERICAL_FAILURE, "SVD produced non-finite singular values");
+      int64_t k = floor_rule(ss, tol);
+      k = std::min<int64_t>({k, rmax, npos});
+      k = std::max<int64_t>(k, 1);
+      K[q] = k;
+      int32_t* didx = h->ctx.upload(order.data(), size_t(k));
+      // reorder columns of B*J and J in place order (into US / Mt scratch)
+      gc.push_back(GatherCols{s.B, s.US, m, m, didx, int(m), int(k)});
+      gc.push_back(GatherCols{s.J, s.Qb == nullptr ? nullptr : s.B, m, m, didx, int(m), int(k)});
+    }
+    // second gather must not overwrite B before the first reads it: run them in two launches
+    std::vector<GatherCols> g1, g2;
+    for (size_t i = 0; i < gc.size(); i += 2) {
+      g1.push_back(gc[i]);
+      g2.push_back(gc[i + 1]);
+    }
+    gather_cols_batched(ctx, g1);  // US <- (B J)[:, sel]        (m x k)
+    // J[:, sel] goes to a fresh buffer (B is free now)
+    gather_cols_batched(ctx, g2);  // B  <- J[:, sel]            (m x k)
+    // build the new cores
+    std::vector<Copy2D> cpo;
+    for (size_t q = 0; q < nb; ++q) {
+      if (jcore[q] < 0) continue;
+      SweepState& s = st[q];
+      const int j = jcore[q];
+      const int64_t p = P[q], qq = Qd[q], k = K[q], m = pdim[q];
+      const bool t = tallcase[q];
+      const double* BJk = s.US;  // (m x k): columns u_c s_c of the square factor
+      const double* Jk = s.B;    // (m x k): right rotations
+      if (left) {
+        const int64_t n = s.y->dims[j];
+        double* newj = h->core_alloc(s.out.get(), k * qq);
+        const int jl = j - 1;
+        const int64_t rl = s.rk[jl] * s.y->dims[jl];
+        double* dst = s.ping[s.pp];
+        if (!t) {
+          // M = (B J)(Q J)^T: core_j = (Q Jk)^T, core_{j-1} <- core_{j-1} (B J)_k
+          gb.add(int(k), int(qq), int(m), 1.0, Jk, m, 1, s.Qb, qq, 1, 0.0, newj, 1, k);
+          gb.add(int(rl), int(k), int(p), 1.0, s.cur[jl], 1, rl, BJk, 1, m, 0.0, dst, 1, rl);
+        } else {
+          // M = Q (R J) J^T: core_j = Jk^T, core_{j-1} <- core_{j-1} (Q (R J)_k)
+          cpo.push_back(copy_job(Jk, 1, m, newj, k, 1, int(m), int(k)));
+          gb.add(int(p), int(k), int(qq), 1.0, s.Qb, 1, p, BJk, 1, m, 0.0, s.Mt, 1, p);
+        }
+        (void)n;
+        s.cur[j] = newj;
+        s.out->cores[j] = newj;
+      } else {
+        const int jr = j + 1;
+        const int64_t nr = s.y->dims[jr] * s.rk[jr + 1];
+        double* newj = h->core_alloc(s.out.get(), p * k);
+        double* dst = s.ping[s.pp];
+        if (t) {
+          // M = Q J (R^T J)^T... : U = Q Jk, S V^T = (B J)_k^T
+          gb.add(int(p), int(k), int(qq), 1.0, s.Qb, 1, p, Jk, 1, m, 0.0, newj, 1, p);
+          gb.add(int(k), int(nr), int(qq), 1.0, BJk, m, 1, s.cur[jr], 1, qq, 0.0, dst, 1, k);
+        } else {
+          // M = J (Q (R J))^T: U = Jk, S V^T = (Q (B J)_k)^T
+          cpo.push_back(copy_job(Jk, 1, m, newj, 1, p, int(p), int(k)));
+          gb.add(int(k), int(qq), int(p), 1.0, BJk, m, 1, s.Qb, qq, 1, 0.0, s.Mt, 1, k);
+        }
+        s.cur[j] = newj;
+        s.out->cores[j] = newj;
+      }
+    }
+    copy2d_batched(ctx, cpo);
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    // second-stage products of the tall-left / wide-right cases
+    for (size_t q = 0; q < nb; ++q) {
+      if (jcore[q] < 0) continue;
+      SweepState& s = st[q];
+      const int j = jcore[q];
+      const int64_t p = P[q], qq = Qd[q], k = K[q];
+      const bool t = tallcase[q];
+      double* dst = s.ping[s.pp];
+      if (left && t) {
+        const int jl = j - 1;
+        const int64_t rl = s.rk[jl] * s.y->dims[jl];
+        gb.add(int(rl), int(k), int(p), 1.0, s.cur[jl], 1, rl, s.Mt, 1, p, 0.0, dst, 1, rl);
+      } else if (!left && !t) {
+        const int jr = j + 1;
+        const int64_t nr = s.y->dims[jr] * s.rk[jr + 1];
+        gb.add(int(k), int(nr), int(qq), 1.0, s.Mt, 1, k, s.cur[jr], 1, qq, 0.0, dst, 1, k);
+      }
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    for (size_t q = 0; q < nb; ++q) {
+      if (jcore[q] < 0) continue;
+      SweepState& s = st[q];
+      const int j = jcore[q];
+      if (left) {
+        s.cur[j - 1] = s.ping[s.pp];
+        s.rk[j] = K[q];
+        s.out->ot[j] = -1;
+      } else {
+        s.cur[j + 1] = s.ping[s.pp];
+        s.rk[j + 1] = K[q];
+        s.out->ot[j] = 1;
+      }
+      s.pp ^= 1;
+    }
+  }
+  // finish: copy the last working core, set metadata
+  std::vector<Copy2D> cp;
+  for (auto& s : st) {
+    const int d = s.y->d;
+    if (s.zero) {
+      s.out = h->make_tt(s.y->dims, s.y->rks, std::vector<int64_t>(d, 0));
+      zeros_like_ranks(h, s.out.get());
+      continue;
+    }
+    const int jf = left ? 0 : d - 1;
+    s.out->rks = s.rk;
+    const int64_t e = s.out->core_elems(jf);
+    double* c = h->core_alloc(s.out.get(), e);
+    cp.push_back(copy_job(s.cur[jf], 1, e, c, 1, e, int(e), 1));
+    s.out->cores[jf] = c;
+    if (left) s.out->ot[0] = 0;
+    else s.out->ot[d - 1] = 0;
+  }
+  copy2d_batched(ctx, cp);
+}
+
+// ------------------------------------------------------------ randomized
+struct SketchPlan {
+  int d = 0;
+  std::vector<int64_t> dims, bv, pmax;
+  Prec prec = Prec::F64;
+  bool orthogonal = true;
+  uint64_t seed = 1234;
+  std::vector<void*> S;  // per core, device layout (n, bvr, bvl, pmax)
+};
+
+std::vector<int64_t> block_counts(const std::vector<int64_t>& rks, const std::vector<int64_t>& bv, int d,
+                                  const ttr_rand_opts* o) {
+  std::vector<int64_t> p(d + 1, 1);
+  if (o->p) {
+    for (int k = 0; k <= d; ++k) p[k] = o->p[k];
+  } else {
+    // compute_sketch_blocks_heuristic(...; reverse=true) (src/sketches.jl:107-123)
+    for (int k = 0; k <= d; ++k) p[k] = int64_t(std::ceil(double(rks[k]) / double(bv[k])));
+    for (int k = 1; k < d; ++k) p[k] = std::max(p[k], p[k - 1]);
+    p[d] = 1;
+  }
+  if (o->oversampling != 1.0 && o->oversampling > 0.0)
+    for (int k = 0; k < d; ++k) p[k] = int64_t(std::ceil(o->oversampling * double(p[k])));
+  for (int k = 0; k <= d; ++k) TTR_REQUIRE(p[k] >= 1, ST_INVALID_ARG, "block counts must be >= 1");
+  return p;
+}
+
+std::vector<int64_t> block_ranks(const std::vector<int64_t>& dims, int64_t block_rks) {
+  const int d = int(dims.size());
+  std::vector<int64_t> bv(d + 1, 1);
+  for (int k = 0; k < d; ++k) bv[k] = block_rks;
+  for (int k = d - 1; k >= 0; --k) bv[k] = std::min(bv[k], dims[k] * bv[k + 1]);
+  return bv;
+}
+
+// generate_sketch_blocks (src/sketches.jl:44-84) for cores kmin..d-1
+void make_sketch(ttr_handle_s* h, SketchPlan& sp, int kmin) {
+  Ctx& ctx = h->ctx;
+  const size_t eb = prec_bytes(sp.prec);
+  sp.S.assign(sp.d, nullptr);
+  std::vector<SketchGen> gen;
+  for (int k = kmin; k < sp.d; ++k) {
+    const int64_t n = sp.dims[k], bvr = sp.bv[k + 1], bvl = sp.bv[k], p = sp.pmax[k];
+    const int64_t cnt = n * bvr * bvl * p;
+    sp.S[k] = ctx.ws.alloc(size_t(cnt) * eb);
+    if (!sp.orthogonal) {
+      gen.push_back(SketchGen{sp.S[k], nullptr, int(n), int(bvr), int(bvl), int(p), k, 1.0 / std::sqrt(double(bvl))});
+    }
+  }
+  if (!sp.orthogonal) {
+    sketch_generate(ctx, gen, sp.seed, sp.prec);
+    return;
+  }
+  // orthogonal: FP64 blocks in reference order, thin Householder Q per block (or identity), scale, pack
+  std::vector<double*> ref(sp.d, nullptr), qb(sp.d, nullptr);
+  std::vector<Fill2D> fills;
+  std::vector<QRJob> qj;
+  for (int k = kmin; k < sp.d; ++k) {
+    const int64_t n = sp.dims[k], bvr = sp.bv[k + 1], bvl = sp.bv[k], p = sp.pmax[k];
+    const int64_t m = bvr * n;
+    ref[k] = ctx.ws.alloc_n<double>(size_t(m * bvl * p));
+    if (bvl == m) {
+      for (int64_t j = 0; j < p; ++j) fills.push_back(Fill2D{ref[k] + j * m * bvl, m, int(m), int(bvl), 1.0});
+    } else {
+      gen.push_back(SketchGen{nullptr, ref[k], int(n), int(bvr), int(bvl), int(p), k, 1.0});
+      qb[k] = ctx.ws.alloc_n<double>(size_t(m * bvl * p));
+      for (int64_t j = 0; j < p; ++j)
+        qj.push_back(QRJob{ref[k] + j * m * bvl, m, int(m), int(bvl), qb[k] + j * m * bvl, m});
+    }
+  }
+  fill2d_batched(ctx, fills);
+  sketch_generate(ctx, gen, sp.seed, Prec::F64);
+  qr_batched(ctx, qj);
+  for (int k = kmin; k < sp.d; ++k) {
+    const int64_t n = sp.dims[k], bvr = sp.bv[k + 1], bvl = sp.bv[k], p = sp.pmax[k];
+    const int64_t m = bvr * n;
+    if (bvl == m) sketch_pack(ctx, ref[k], sp.S[k], int(n), int(bvr), int(bvl), int(p), 1.0, sp.prec);
+    else sketch_pack(ctx, qb[k], sp.S[k], int(n), int(bvr), int(bvl), int(p),
+                     std::sqrt(double(m) / double(bvl)), sp.prec);
+  }
+}
+
+struct RandTerm {
+  DevTT* x;
+  double alpha;
+  std::vector<void*> W;     // per core k: (R_k x s_k) in sketch precision
+  int32_t* E;               // per core exponent (d+1 ints, device)
+  double* Y[2];
+  double* Pb;
+};
+
+struct RandItem {
+  std::vector<RandTerm> terms;
+  std::vector<int64_t> rks, p, s;  // target ranks, block counts, sketch ranks (d+1)
+  double* Z = nullptr;
+};
+
+// right-to-left sketch phase of tt_recursive_sketch (src/sketches.jl:221-240)
+void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items, int kmin) {
+  Ctx& ctx = h->ctx;
+  const int d = sp.d;
+  const size_t eb = prec_bytes(sp.prec);
+  // buffers
+  int64_t t1max = 1;
+  std::vector<std::vector<void*>> T1(items.size());
+  for (size_t i = 0; i < items.size(); ++i) {
+    for (auto& t : items[i].terms) {
+      t.W.assign(d + 1, nullptr);
+      t.E = ctx.ws.alloc_n<int32_t>(size_t(d + 1));
+      zero_fill(ctx, t.E, size_t(d + 1) * 4);
+      int64_t mx = 1;
+      for (int k = kmin; k < d; ++k) {
+        t.W[k] = ctx.ws.alloc(size_t(t.x->rks[k] * items[i].s[k]) * eb);
+        if (k < d - 1) mx = std::max(mx, t.x->rks[k] * sp.dims[k] * sp.bv[k + 1] * items[i].p[k]);
+      }
+      t1max = std::max(t1max, mx);
+      T1[i].push_back(ctx.ws.alloc(size_t(mx) * eb));
+    }
+  }
+  GemmBatch gb;
+  for (int k = d - 1; k >= kmin; --k) {
+    const int64_t n = sp.dims[k], bvr = sp.bv[k + 1], bvl = sp.bv[k];
+    const char* S = static_cast<const char*>(sp.S[k]);
+    if (k == d - 1) {
+      for (auto& it : items)
+        for (auto& t : it.terms) {
+          const int64_t Rk = t.x->rks[k];
+          for (int64_t j = 0; j < it.p[k]; ++j)
+            gb.add(int(Rk), int(bvl), int(n), 1.0, t.x->cores[k], 1, Rk, S + size_t(n * bvl * j) * eb, 1, n, 0.0,
+                   static_cast<char*>(t.W[k]) + size_t(Rk * bvl * j) * eb, 1, Rk);
+        }
+      gemm_run(ctx, gb, Prec::F64, sp.prec, sp.prec, sp.prec);
+    } else {
+      for (size_t i = 0; i < items.size(); ++i)
+        for (size_t ti = 0; ti < items[i].terms.size(); ++ti) {
+          RandTerm& t = items[i].terms[ti];
+          const int64_t Rk = t.x->rks[k], Rk1 = t.x->rks[k + 1], p = items[i].p[k];
+          gb.add(int(Rk * n), int(bvr * p), int(Rk1), 1.0, t.x->cores[k], 1, Rk * n, t.W[k + 1], 1, Rk1, 0.0,
+                 T1[i][ti], 1, Rk * n);
+        }
+      gemm_run(ctx, gb, Prec::F64, sp.prec, sp.prec, sp.prec);
+      for (size_t i = 0; i < items.size(); ++i)
+        for (size_t ti = 0; ti < items[i].terms.size(); ++ti) {
+          RandTerm& t = items[i].terms[ti];
+          const int64_t Rk = t.x->rks[k];
+          for (int64_t j = 0; j < items[i].p[k]; ++j)
+            gb.add(int(Rk), int(bvl), int(n * bvr), 1.0, static_cast<char*>(T1[i][ti]) + size_t(Rk * n * bvr * j) * eb,
+                   1, Rk, S + size_t(n * bvr * bvl * j) * eb, 1, n * bvr, 0.0,
+                   static_cast<char*>(t.W[k]) + size_t(Rk * bvl * j) * eb, 1, Rk);
+        }
+      gemm_run(ctx, gb, sp.prec, sp.prec, sp.prec, sp.prec);
+    }
+    std::vector<Pow2Job> pj;
+    for (auto& it : items)
+      for (auto& t : it.terms)
+        pj.push_back(Pow2Job{t.W[k], t.x->rks[k], int(t.x->rks[k]), int(it.s[k]), k == d - 1 ? nullptr : t.E + k + 1,
+                             t.E + k});
+    pow2_normalize_batched(ctx, pj, sp.prec);
+  }
+}
+
+void prepare_plan(ttr_handle_s* h, const ttr_rand_opts* o, int d, const std::vector<int64_t>& dims,
+                  std::vector<RandItem>& items, SketchPlan& sp) {
+  TTR_REQUIRE(o != nullptr, ST_INVALID_ARG, "options must not be NULL");
+  sp.d = d;
+  sp.dims = dims;
+  sp.prec = to_prec(o->sketch_prec);
+  sp.orthogonal = o->orthogonal != 0;
+  sp.seed = o->seed;
+  sp.bv = block_ranks(dims, o->block_rks > 0 ? o->block_rks : d);
+  sp.pmax.assign(d + 1, 1);
+  for (auto& it : items) {
+    TTR_REQUIRE(int(it.rks.size()) == d + 1 && it.rks[d] == 1, ST_INVALID_ARG, "rks must have d+1 entries with rks[d] == 1");
+    it.p = block_counts(it.rks, sp.bv, d, o);
+    it.s.resize(d + 1);
+    for (int k = 0; k <= d; ++k) {
+      it.s[k] = sp.bv[k] * it.p[k];
+      sp.pmax[k] = std::max(sp.pmax[k], it.p[k]);
+    }
+    for (int k = 0; k + 1 < d; ++k)
+      TTR_REQUIRE(it.p[k] <= it.p[k + 1], ST_INVALID_ARG, "block counts must be non-decreasing towards the right");
+  }
+  (void)h;
+}
+
+// ttrand_rounding(alpha, y, rks) (src/tt_randtools.jl:223-276), batched
+std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vector<RandItem>& items,
+                                                     const ttr_rand_opts* o) {
+  Ctx& ctx = h->ctx;
+  const DevTT* x0 = items[0].terms[0].x;
+  const int d = x0->d;
+  for (auto& it : items)
+    for (auto& t : it.terms) TTR_REQUIRE(t.x->dims == x0->dims, ST_DIM_MISMATCH, "all terms must share dims");
+  SketchPlan sp;
+  prepare_plan(h, o, d, x0->dims, items, sp);
+  for (auto& it : items)
+    for (auto& t : it.terms) TTR_REQUIRE(t.x->rks[d] == 1, ST_INVALID_ARG, "rks[d] must be 1");
+  std::vector<std::unique_ptr<DevTT>> outs(items.size());
+  std::vector<std::vector<int64_t>> orr(items.size());
+  for (size_t i = 0; i < items.size(); ++i) {
+    std::vector<int64_t> r(d + 1, 1);
+    for (int k = 0; k + 1 < d; ++k) r[k + 1] = std::min(sp.dims[k] * r[k], items[i].s[k + 1]);
+    std::vector<int64_t> ot(d, 1);
+    ot[d - 1] = 0;
+    outs[i] = h->make_tt(sp.dims, r, ot);
+    orr[i] = r;
+  }
+  GemmBatch gb;
+  if (d == 1) {
+    for (size_t i = 0; i < items.size(); ++i) {
+      gb.prob(outs[i]->cores[0], 1, 1, 1, int(sp.dims[0]), 1.0, 0.0);
+      for (auto& t : items[i].terms) gb.seg(h->one, 1, 1, t.x->cores[0], 1, 1, 1, t.alpha);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    return outs;
+  }
+  make_sketch(h, sp, 1);
+  sketch_phase(h, sp, items, 1);
+  // buffers of the left-to-right sweep
+  for (size_t i = 0; i < items.size(); ++i) {
+    int64_t zmax = 1;
+    for (int k = 0; k + 1 < d; ++k) zmax = std::max(zmax, orr[i][k] * sp.dims[k] * items[i].s[k + 1]);
+    items[i].Z = ctx.ws.alloc_n<double>(size_t(zmax));
+    for (auto& t : items[i].terms) {
+      int64_t ymax = 1, pmax = 1;
+      for (int k = 0; k + 1 < d; ++k) {
+        ymax = std::max(ymax, orr[i][k + 1] * sp.dims[k + 1] * t.x->rks[k + 2]);
+        pmax = std::max(pmax, orr[i][k + 1] * t.x->rks[k + 1]);
+      }
+      t.Y[0] = ctx.ws.alloc_n<double>(size_t(ymax));
+      t.Y[1] = ctx.ws.alloc_n<double>(size_t(ymax));
+      t.Pb = ctx.ws.alloc_n<double>(size_t(pmax));
+    }
+  }
+  int yb = 0;
+  for (int k = 0; k + 1 < d; ++k) {
+    const int64_t n = sp.dims[k];
+    // Z_k = sum_t Y_t W_t (FP64 accumulate; terms rescaled by their sketch exponents)
+    for (size_t i = 0; i < items.size(); ++i) {
+      const int64_t r = orr[i][k], s = items[i].s[k + 1];
+      gb.prob(items[i].Z, 1, r * n, int(r * n), int(s), 1.0, 0.0);
+      for (auto& t : items[i].terms) {
+        const int64_t R1 = t.x->rks[k + 1];
+        const double* Y = k == 0 ? t.x->cores[0] : t.Y[yb];
+        gb.seg(Y, 1, r * n, t.W[k + 1], 1, R1, int(R1), k == 0 ? t.alpha : 1.0, AM_PLAIN, t.E + k + 1);
+      }
+    }
+    gemm_run(ctx, gb, Prec::F64, sp.prec, Prec::F64, Prec::F64);
+    std::vector<QRJob> qj;
+    for (size_t i = 0; i < items.size(); ++i) {
+      const int64_t r = orr[i][k], s = items[i].s[k + 1];
+      qj.push_back(QRJob{items[i].Z, r * n, int(r * n), int(s), outs[i]->cores[k], r * n});
+    }
+    qr_batched(ctx, qj);
+    // P_t = Q^T Y_t
+    for (size_t i = 0; i < items.size(); ++i) {
+      const int64_t r = orr[i][k], r1 = orr[i][k + 1];
+      for (auto& t : items[i].terms) {
+        const int64_t R1 = t.x->rks[k + 1];
+        const double* Y = k == 0 ? t.x->cores[0] : t.Y[yb];
+        gb.prob(t.Pb, 1, r1, int(r1), int(R1), 1.0, 0.0);
+        gb.seg(outs[i]->cores[k], r * n, 1, Y, 1, r * n, int(r * n), k == 0 ? t.alpha : 1.0);
+      }
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    // Y_t <- P_t X_{t,k+1}; the last core sums the terms
+    const int64_t n1 = sp.dims[k + 1];
+    for (size_t i = 0; i < items.size(); ++i) {
+      const int64_t r1 = orr[i][k + 1];
+      if (k + 2 < d) {
+        for (auto& t : items[i].terms) {
+          const int64_t R1 = t.x->rks[k + 1], R2 = t.x->rks[k + 2];
+          gb.add(int(r1), int(n1 * R2), int(R1), 1.0, t.Pb, 1, r1, t.x->cores[k + 1], 1, R1, 0.0, t.Y[yb ^ 1], 1, r1);
+        }
+      } else {
+        gb.prob(outs[i]->cores[k + 1], 1, r1, int(r1), int(n1), 1.0, 0.0);
+        for (auto& t : items[i].terms) {
+          const int64_t R1 = t.x->rks[k + 1];
+          gb.seg(t.Pb, 1, r1, t.x->cores[k + 1], 1, R1, int(R1), 1.0);
+        }
+      }
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    yb ^= 1;
+  }
+  return outs;
+}
+
+std::vector<RandItem> parse_items(ttr_handle_s* h, int64_t n, const ttr_rand_item* items) {
+  TTR_REQUIRE(n >= 1 && items != nullptr, ST_INVALID_ARG, "need at least one item");
+  std::vector<RandItem> out(static_cast<size_t>(n));
+  for (int64_t i = 0; i < n; ++i) {
+    const ttr_rand_item& it = items[i];
+    TTR_REQUIRE(it.nterms >= 1 && it.term_ids && it.alpha && it.rks, ST_INVALID_ARG, "bad randomized item");
+    TTR_REQUIRE(it.op_id < 0, ST_INVALID_ARG, "op_id must be -1 (use ttr_mpo_apply for products)");
+    for (int64_t t = 0; t < it.nterms; ++t) {
+      RandTerm rt{};
+      rt.x = h->get(it.term_ids[t]);
+      rt.alpha = it.alpha[t];
+      out[i].terms.push_back(rt);
+    }
+    const int d = out[i].terms[0].x->d;
+    out[i].rks.assign(it.rks, it.rks + d + 1);
+  }
+  return out;
+}
+
+// ------------------------------------------------------------------ C-ABI glue
+template <class F>
+int guarded(ttr_handle_t h, F&& f) {
+  if (!h) return TTR_INVALID_ARG;
+  try {
+    h->err.clear();
+    f();
+    h->ctx.end_call();
+    return TTR_OK;
+  } catch (const Error& e) {
+    h->err = e.what();
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(h->ctx.stream);
+    h->ctx.ws.reset();
+    h->ctx.pin.reset();
+    return e.status;
+  } catch (const std::exception& e) {
+    h->err = e.what();
+    (void)hipStreamSynchronize(h->ctx.stream);
+    h->ctx.ws.reset();
+    h->ctx.pin.reset();
+    return TTR_DEVICE_ERROR;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ttr_version(void) { return "libttround 0.1.0 (gfx950)"; }
+
+int ttr_create(ttr_handle_t* out, int device) {
+  if (!out) return TTR_INVALID_ARG;
+  *out = nullptr;
+  ttr_handle_s* h = new (std::nothrow) ttr_handle_s();
+  if (!h) return TTR_OOM;
+  try {
+    int count = 0;
+    TTR_HIP(hipGetDeviceCount(&count));
+    TTR_REQUIRE(device >= 0 && device < count, ST_INVALID_ARG, "no such HIP device");
+    TTR_HIP(hipSetDevice(device));
+    h->ctx.device = device;
+    TTR_HIP(hipStreamCreateWithFlags(&h->ctx.stream, hipStreamNonBlocking));
+    TTR_HIP(hipMalloc(&h->one, 64));
+    const double one = 1.0;
+    TTR_HIP(hipMemcpy(h->one, &one, 8, hipMemcpyHostToDevice));
+  } catch (const Error& e) {
+    int st = e.status;
+    if (h->ctx.stream) (void)hipStreamDestroy(h->ctx.stream);
+    delete h;
+    return st;
+  }
+  *out = h;
+  return TTR_OK;
+}
+
+int ttr_destroy(ttr_handle_t h) {
+  if (!h) return TTR_INVALID_ARG;
+  (void)hipSetDevice(h->ctx.device);
+  (void)hipStreamSynchronize(h->ctx.stream);
+  for (auto& kv : h->tts) h->free_tt(kv.second.get());
+  h->tts.clear();
+  for (auto& kv : h->ops) (void)hipFree(kv.second->base);
+  h->ops.clear();
+  (void)hipFree(h->one);
+  h->ctx.ws.release();
+  h->ctx.pin.release();
+  h->ctx.cache.release_cached();
+  (void)hipStreamDestroy(h->ctx.stream);
+  delete h;
+  return TTR_OK;
+}
+
+const char* ttr_last_error(ttr_handle_t h) { return h ? h->err.c_str() : "null handle"; }
+
+int ttr_tt_upload(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, const int64_t* ot,
+                  const double* const* cores, int64_t* id) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(d >= 1 && dims && rks && cores && id, ST_INVALID_ARG, "bad arguments");
+    std::vector<int64_t> vd(dims, dims + d), vr(rks, rks + d + 1), vo(size_t(d), 0);
+    if (ot) vo.assign(ot, ot + d);
+    for (int64_t k = 0; k < d; ++k) TTR_REQUIRE(vd[k] >= 1, ST_INVALID_ARG, "dims must be >= 1");
+    for (int64_t k = 0; k <= d; ++k) TTR_REQUIRE(vr[k] >= 1, ST_INVALID_ARG, "ranks must be >= 1");
+    TTR_REQUIRE(vr[0] == 1 && vr[d] == 1, ST_DIM_MISMATCH, "boundary ranks must be 1");
+    auto t = h->make_tt(vd, vr, vo);
+    int64_t total = 0;
+    for (int k = 0; k < d; ++k) total += t->core_elems(k);
+    double* stage = h->ctx.ws.alloc_n<double>(size_t(total));
+    std::vector<Copy2D> cp;
+    int64_t off = 0;
+    for (int k = 0; k < d; ++k) {
+      const int64_t n = vd[k], r = vr[k], r2 = vr[k + 1], e = t->core_elems(k);
+      TTR_REQUIRE(cores[k] != nullptr, ST_INVALID_ARG, "null core pointer");
+      TTR_HIP(hipMemcpy(stage + off, cores[k], size_t(e) * 8, hipMemcpyHostToDevice));
+      Copy2D c = copy_job(stage + off, 1, n, t->cores[k], r, 1, int(n), int(r));
+      c.ibs = n * r;
+      c.obs = r * n;
+      c.nb = int(r2);
+      cp.push_back(c);
+      off += e;
+    }
+    copy2d_batched(h->ctx, cp);
+    *id = h->add(std::move(t));
+  });
+}
+
+int ttr_tt_info(ttr_handle_t h, int64_t id, int64_t* d, int64_t* dims, int64_t* rks, int64_t* ot) {
+  return guarded(h, [&] {
+    DevTT* t = h->get(id);
+    if (d) *d = t->d;
+    if (dims) std::copy(t->dims.begin(), t->dims.end(), dims);
+    if (rks) std::copy(t->rks.begin(), t->rks.end(), rks);
+    if (ot) std::copy(t->ot.begin(), t->ot.end(), ot);
+  });
+}
+
+int ttr_tt_download_core(ttr_handle_t h, int64_t id, int64_t k, double* dst) {
+  return guarded(h, [&] {
+    DevTT* t = h->get(id);
+    TTR_REQUIRE(k >= 0 && k < t->d && dst, ST_INVALID_ARG, "bad core index");
+    const int64_t n = t->dims[k], r = t->rks[k], r2 = t->rks[k + 1], e = t->core_elems(int(k));
+    double* stage = h->ctx.ws.alloc_n<double>(size_t(std::max<int64_t>(e, 1)));
+    Copy2D c = copy_job(t->cores[k], r, 1, stage, 1, n, int(n), int(r));
+    c.ibs = r * n;
+    c.obs = n * r;
+    c.nb = int(r2);
+    copy2d_batched(h->ctx, {c});
+    h->ctx.sync();
+    TTR_HIP(hipMemcpy(dst, stage, size_t(e) * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+int ttr_tt_free(ttr_handle_t h, int64_t id) {
+  return guarded(h, [&] {
+    DevTT* t = h->get(id);
+    h->ctx.sync();
+    h->free_tt(t);
+    h->tts.erase(id);
+  });
+}
+
+int ttr_tt_randn(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, uint64_t seed, int64_t* id) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(d >= 1 && dims && rks && id, ST_INVALID_ARG, "bad arguments");
+    std::vector<int64_t> vd(dims, dims + d), vr(rks, rks + d + 1);
+    const int64_t rmax = *std::max_element(vr.begin(), vr.end());
+    vr = r_and_d_to_rks(vr, vd, rmax);
+    auto t = h->make_tt(vd, vr, std::vector<int64_t>(size_t(d), 0));
+    for (int k = 0; k < d; ++k) randn_core(h->ctx, t->cores[k], int(vd[k]), int(vr[k]), int(vr[k + 1]), k, seed);
+    *id = h->add(std::move(t));
+  });
+}
+
+int ttr_tto_upload(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, const double* const* cores,
+                   int64_t* id) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(d >= 1 && dims && rks && cores && id, ST_INVALID_ARG, "bad arguments");
+    auto op = std::make_unique<DevTTO>();
+    op->d = int(d);
+    op->dims.assign(dims, dims + d);
+    op->rks.assign(rks, rks + d + 1);
+    TTR_REQUIRE(op->rks[0] == 1 && op->rks[d] == 1, ST_DIM_MISMATCH, "boundary ranks must be 1");
+    int64_t total = 0;
+    for (int k = 0; k < d; ++k) total += op->dims[k] * op->dims[k] * op->rks[k] * op->rks[k + 1];
+    TTR_HIP(hipMalloc(&op->base, size_t(std::max<int64_t>(total, 1)) * 8));
+    int64_t off = 0;
+    for (int k = 0; k < d; ++k) {
+      const int64_t e = op->dims[k] * op->dims[k] * op->rks[k] * op->rks[k + 1];
+      double* p = static_cast<double*>(op->base) + off;
+      TTR_HIP(hipMemcpy(p, cores[k], size_t(e) * 8, hipMemcpyHostToDevice));
+      op->cores.push_back(p);
+      off += e;
+    }
+    const int64_t nid = h->next_id++;
+    h->ops[nid] = std::move(op);
+    *id = nid;
+  });
+}
+
+int ttr_lincomb(ttr_handle_t h, int64_t nterms, const double* alpha, const int64_t* ids, int64_t* out_id) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(nterms >= 1 && alpha && ids && out_id, ST_INVALID_ARG, "bad arguments");
+    std::vector<DevTT*> xs;
+    for (int64_t t = 0; t < nterms; ++t) {
+      xs.push_back(h->get(ids[t]));
+      check_dims_equal(xs[0], xs.back());
+    }
+    const int d = xs[0]->d;
+    TTR_REQUIRE(d >= 2 || nterms == 1, ST_INVALID_ARG, "TT sums need d >= 2");
+    // per term: ranks (a zero coefficient gives the rank-one zero TT, src/tt_operations.jl:432-433)
+    std::vector<std::vector<int64_t>> tr(static_cast<size_t>(nterms));
+    std::vector<int> scaled(static_cast<size_t>(nterms), 0);
+    for (int64_t t = 0; t < nterms; ++t) {
+      if (alpha[t] == 0.0) {
+        tr[t].assign(size_t(d + 1), 1);
+      } else {
+        tr[t] = xs[t]->rks;
+        auto it = std::find(xs[t]->ot.begin(), xs[t]->ot.end(), 0);
+        TTR_REQUIRE(it != xs[t]->ot.end(), ST_INVALID_ARG, "no core with ot == 0 to scale");
+        scaled[t] = int(it - xs[t]->ot.begin());
+      }
+    }
+    std::vector<int64_t> rks(static_cast<size_t>(d + 1), 0);
+    for (int k = 0; k <= d; ++k)
+      for (int64_t t = 0; t < nterms; ++t) rks[k] += tr[t][k];
+    rks[0] = 1;
+    rks[d] = 1;
+    if (nterms == 1) rks = tr[0];
+    auto out = h->make_tt(xs[0]->dims, rks, std::vector<int64_t>(size_t(d), nterms == 1 ? 0 : 0));
+    if (nterms == 1 && alpha[0] != 0.0) out->ot = xs[0]->ot;
+    zeros_like_ranks(h, out.get());
+    std::vector<BlockPut> bp;
+    std::vector<int64_t> a0(static_cast<size_t>(d + 1), 0);
+    for (int64_t t = 0; t < nterms; ++t) {
+      if (alpha[t] != 0.0)
+        for (int k = 0; k < d; ++k) {
+          const int64_t ra = tr[t][k], rb = tr[t][k + 1];
+          BlockPut b{};
+          b.in = xs[t]->cores[k];
+          b.out = out->cores[k];
+          b.ra = int(ra);
+          b.rb = int(rb);
+          b.n = int(xs[0]->dims[k]);
+          b.RA = int(rks[k]);
+          b.RB = int(rks[k + 1]);
+          b.a0 = k == 0 ? 0 : int(a0[k]);
+          b.b0 = k == d - 1 ? 0 : int(a0[k + 1]);
+          b.scale = (k == scaled[t]) ? alpha[t] : 1.0;
+          bp.push_back(b);
+        }
+      for (int k = 0; k <= d; ++k) a0[k] += tr[t][k];
+    }
+    block_put_batched(h->ctx, bp);
+    *out_id = h->add(std::move(out));
+  });
+}
+
+int ttr_mpo_apply(ttr_handle_t h, int64_t op_id, int64_t x_id, int64_t* out_id) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(out_id, ST_INVALID_ARG, "bad arguments");
+    DevTTO* A = h->get_op(op_id);
+    DevTT* x = h->get(x_id);
+    TTR_REQUIRE(A->dims == x->dims, ST_DIM_MISMATCH, "Incompatible dimensions");
+    std::vector<int64_t> rks(static_cast<size_t>(x->d + 1));
+    for (int k = 0; k <= x->d; ++k) rks[k] = A->rks[k] * x->rks[k];
+    auto y = h->make_tt(x->dims, rks, std::vector<int64_t>(size_t(x->d), 0));
+    std::vector<MpoCore> jobs;
+    for (int k = 0; k < x->d; ++k)
+      jobs.push_back(MpoCore{A->cores[k], x->cores[k], y->cores[k], int(x->dims[k]), int(A->rks[k]),
+                             int(A->rks[k + 1]), int(x->rks[k]), int(x->rks[k + 1])});
+    mpo_apply_batched(h->ctx, jobs);
+    *out_id = h->add(std::move(y));
+  });
+}
+
+int ttr_orthogonalize(ttr_handle_t h, int64_t n, const int64_t* ids, int64_t root, int64_t* out_ids) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(n >= 1 && ids && out_ids, ST_INVALID_ARG, "bad arguments");
+    std::vector<DevTT*> xs;
+    for (int64_t i = 0; i < n; ++i) xs.push_back(h->get(ids[i]));
+    auto outs = orthogonalize_batch(h, xs, int(root));
+    for (int64_t i = 0; i < n; ++i) out_ids[i] = h->add(std::move(outs[i]));
+  });
+}
+
+int ttr_round_svd(ttr_handle_t h, int64_t n, const int64_t* ids, double tol, int64_t rmax, int direction,
+                  int64_t* out_ids) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(n >= 1 && ids && out_ids, ST_INVALID_ARG, "bad arguments");
+    TTR_REQUIRE(direction == TTR_DIR_LEFT || direction == TTR_DIR_RIGHT, ST_INVALID_ARG,
+                "Invalid direction. Use direction=:right or direction=:left");
+    TTR_REQUIRE(rmax >= 1, ST_INVALID_ARG, "rmax must be >= 1");
+    std::vector<DevTT*> xs;
+    for (int64_t i = 0; i < n; ++i) xs.push_back(h->get(ids[i]));
+    // orthogonalise the ones that need it (grouped by d for the root)
+    std::vector<std::unique_ptr<DevTT>> tmp(static_cast<size_t>(n));
+    std::vector<DevTT*> ys(static_cast<size_t>(n));
+    std::map<int, std::vector<int64_t>> need;
+    for (int64_t i = 0; i < n; ++i) {
+      const bool ok = direction == TTR_DIR_LEFT ? is_leftorth(xs[i]) : is_rightorth(xs[i]);
+      if (ok) ys[i] = xs[i];
+      else need[xs[i]->d].push_back(i);
+    }
+    for (auto& kv : need) {
+      std::vector<DevTT*> grp;
+      for (int64_t i : kv.second) grp.push_back(xs[i]);
+      auto o = orthogonalize_batch(h, grp, direction == TTR_DIR_LEFT ? kv.first : 1);
+      for (size_t g = 0; g < kv.second.size(); ++g) {
+        ys[kv.second[g]] = o[g].get();
+        tmp[kv.second[g]] = std::move(o[g]);
+      }
+    }
+    std::vector<SweepState> st(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) st[i].y = ys[i];
+    svd_sweep(h, st, tol, rmax, direction);
+    h->ctx.sync();
+    for (auto& t : tmp)
+      if (t) h->free_tt(t.get());
+    for (int64_t i = 0; i < n; ++i) out_ids[i] = h->add(std::move(st[i].out));
+  });
+}
+
+int ttr_norm(ttr_handle_t h, int64_t n, const int64_t* ids, double* out) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(n >= 1 && ids && out, ST_INVALID_ARG, "bad arguments");
+    std::vector<DevTT*> xs;
+    for (int64_t i = 0; i < n; ++i) xs.push_back(h->get(ids[i]));
+    std::vector<const double*> ptrs(static_cast<size_t>(n));
+    std::vector<int64_t> cnt(static_cast<size_t>(n));
+    std::vector<std::unique_ptr<DevTT>> tmp(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) {
+      const DevTT* x = xs[i];
+      int zeros = 0, root = -1;
+      for (int k = 0; k < x->d; ++k)
+        if (x->ot[k] == 0) { ++zeros; root = k; }
+      if (zeros == 1) {
+        ptrs[i] = x->cores[root];
+        cnt[i] = x->core_elems(root);
+      } else {
+        auto o = orthogonalize_batch(h, {xs[i]}, x->d);
+        tmp[i] = std::move(o[0]);
+        ptrs[i] = tmp[i]->cores[x->d - 1];
+        cnt[i] = tmp[i]->core_elems(x->d - 1);
+      }
+    }
+    std::vector<double> r = frob_norms_host(h, ptrs, cnt);
+    for (int64_t i = 0; i < n; ++i) out[i] = r[i];
+    for (auto& t : tmp)
+      if (t) h->free_tt(t.get());
+  });
+}
+
+int ttr_round_rand(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const ttr_rand_opts* opts,
+                   int64_t* out_ids) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(out_ids, ST_INVALID_ARG, "bad arguments");
+    std::vector<RandItem> its = parse_items(h, n, items);
+    auto outs = round_rand_batch(h, its, opts);
+    for (int64_t i = 0; i < n; ++i) out_ids[i] = h->add(std::move(outs[i]));
+  });
+}
+
+int ttr_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts, double* const* W_out,
+               int64_t* sketch_rks) {
+  return guarded(h, [&] {
+    DevTT* x = h->get(tt_id);
+    TTR_REQUIRE(rks != nullptr, ST_INVALID_ARG, "rks must not be NULL");
+    const int d = x->d;
+    TTR_REQUIRE(x->rks[d] == 1, ST_INVALID_ARG, "rks[d] must be 1");
+    std::vector<RandItem> its(1);
+    RandTerm t{};
+    t.x = x;
+    t.alpha = 1.0;
+    its[0].terms.push_back(t);
+    its[0].rks.assign(rks, rks + d + 1);
+    SketchPlan sp;
+    prepare_plan(h, opts, d, x->dims, its, sp);
+    make_sketch(h, sp, 0);
+    sketch_phase(h, sp, its, 0);
+    h->ctx.sync();
+    std::vector<int32_t> E(static_cast<size_t>(d + 1));
+    TTR_HIP(hipMemcpy(E.data(), its[0].terms[0].E, size_t(d + 1) * 4, hipMemcpyDeviceToHost));
+    const size_t eb = prec_bytes(sp.prec);
+    for (int k = 0; k <= d; ++k) {
+      if (sketch_rks) sketch_rks[k] = its[0].s[k];
+      if (!W_out || !W_out[k]) continue;
+      const int64_t cnt = x->rks[k] * its[0].s[k];
+      const double scale = std::ldexp(1.0, k == d ? 0 : E[k]) / std::sqrt(double(its[0].p[k]));
+      if (k == d) {
+        for (int64_t e = 0; e < cnt; ++e) W_out[k][e] = 1.0 / std::sqrt(double(its[0].p[k]));
+        continue;
+      }
+      std::vector<char> raw(size_t(cnt) * eb);
+      TTR_HIP(hipMemcpy(raw.data(), its[0].terms[0].W[k], raw.size(), hipMemcpyDeviceToHost));
+      for (int64_t e = 0; e < cnt; ++e) {
+        double v;
+        if (sp.prec == Prec::F64) std::memcpy(&v, raw.data() + 8 * e, 8);
+        else if (sp.prec == Prec::F32) { float f; std::memcpy(&f, raw.data() + 4 * e, 4); v = f; }
+        else { uint16_t b; std::memcpy(&b, raw.data() + 2 * e, 2); v = bf16_bits_to_float(b); }
+        W_out[k][e] = v * scale;
+      }
+    }
+  });
+}
+
+int ttr_sketch_blocks(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, const ttr_rand_opts* opts,
+                      int64_t k, double* out, int64_t* shape4) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(d >= 1 && dims && rks && out && k >= 0 && k < d, ST_INVALID_ARG, "bad arguments");
+    std::vector<int64_t> vd(dims, dims + d);
+    std::vector<RandItem> its(1);
+    its[0].rks.assign(rks, rks + d + 1);
+    SketchPlan sp;
+    prepare_plan(h, opts, int(d), vd, its, sp);
+    sp.pmax = its[0].p;
+    make_sketch(h, sp, int(k));
+    h->ctx.sync();
+    const int64_t n = vd[k], bvr = sp.bv[k + 1], bvl = sp.bv[k], p = sp.pmax[k];
+    const int64_t cnt = n * bvr * bvl * p;
+    const size_t eb = prec_bytes(sp.prec);
+    std::vector<char> raw(size_t(cnt) * eb);
+    TTR_HIP(hipMemcpy(raw.data(), sp.S[k], raw.size(), hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < cnt; ++e) {
+      const int64_t beta = e % bvr, rest = e / bvr, z = rest % n, col = rest / n;
+      const int64_t src = z + n * (beta + bvr * col);
+      double v;
+      if (sp.prec == Prec::F64) std::memcpy(&v, raw.data() + 8 * src, 8);
+      else if (sp.prec == Prec::F32) { float f; std::memcpy(&f, raw.data() + 4 * src, 4); v = f; }
+      else { uint16_t b; std::memcpy(&b, raw.data() + 2 * src, 2); v = bf16_bits_to_float(b); }
+      out[e] = v;
+    }
+    if (shape4) {
+      shape4[0] = bvr;
+      shape4[1] = n;
+      shape4[2] = bvl;
+      shape4[3] = p;
+    }
+  });
+}
+
+int ttr_sketch_draws(ttr_handle_t h, int64_t k, int64_t count, uint64_t seed, double* out) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(count >= 0 && out && k >= 0, ST_INVALID_ARG, "bad arguments");
+    double* d = h->ctx.ws.alloc_n<double>(size_t(std::max<int64_t>(count, 1)));
+    sketch_draws(h->ctx, d, count, int(k), seed);
+    h->ctx.sync();
+    TTR_HIP(hipMemcpy(out, d, size_t(count) * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,330 @@
+// Grouped (variable-size, multi-segment) GEMM on CDNA4 MFMA.
+//
+// One workgroup computes one 64x64 tile of one problem.  The tile index of a
+// workgroup is mapped to its problem by a binary search over the problems'
+// first-tile prefix, so one launch covers every (item, term, block) product of
+// a sweep step, whatever their shapes.  Operands are staged through LDS with
+// the loader choosing the coalesced direction from the strides; A-operand
+// values are converted to the math precision (and scaled per segment) on load.
+//
+// Math precisions:
+//   F64  : v_mfma_f64_16x16x4_f64, FP64 accumulate
+//   F32  : v_mfma_f32_16x16x4_f32, FP32 accumulate
+//   BF16 : v_mfma_f32_16x16x32_bf16, FP32 accumulate
+#include "ttr_internal.h"
+
+namespace ttr {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8_t __attribute__((ext_vector_type(8)));
+
+struct bf16s {
+  uint16_t x;
+};
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40);  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
+template <class T> __device__ __forceinline__ double load_d(const void* p, int64_t i);
+template <> __device__ __forceinline__ double load_d<double>(const void* p, int64_t i) { return static_cast<const double*>(p)[i]; }
+template <> __device__ __forceinline__ double load_d<float>(const void* p, int64_t i) { return static_cast<const float*>(p)[i]; }
+template <> __device__ __forceinline__ double load_d<bf16s>(const void* p, int64_t i) { return bf16_to_f32(static_cast<const uint16_t*>(p)[i]); }
+
+template <class T> __device__ __forceinline__ void store_c(void* p, int64_t i, double v);
+template <> __device__ __forceinline__ void store_c<double>(void* p, int64_t i, double v) { static_cast<double*>(p)[i] = v; }
+template <> __device__ __forceinline__ void store_c<float>(void* p, int64_t i, double v) { static_cast<float*>(p)[i] = float(v); }
+template <> __device__ __forceinline__ void store_c<bf16s>(void* p, int64_t i, double v) { static_cast<uint16_t*>(p)[i] = f32_to_bf16(float(v)); }
+
+__device__ __forceinline__ int find_prob(const GemmProb* probs, int nprobs, int bid) {
+  int lo = 0, hi = nprobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].tile0 <= bid) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ double amode_value(int mode, int64_t gi, int64_t gl, double v) {
+  if (mode == AM_VT_UNIT) return gl < gi ? 0.0 : (gl == gi ? 1.0 : v);
+  if (mode == AM_V_UNIT) return gi < gl ? 0.0 : (gi == gl ? 1.0 : v);
+  return v;
+}
+
+constexpr int BM = 64, BN = 64;
+
+// ------------------------------------------------------------- F64 / F32 math
+template <class TA, class TB, class TC, class MT>
+__global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __restrict__ probs, int nprobs,
+                                                          const GemmSeg* __restrict__ segs) {
+  constexpr int BK = 16;
+  __shared__ MT As[BK][BM + 1];
+  __shared__ MT Bs[BK][BN + 1];
+  using acc_t = typename std::conditional<std::is_same<MT, double>::value, d4_t, f4_t>::type;
+
+  const int bid = blockIdx.x;
+  const GemmProb P = probs[find_prob(probs, nprobs, bid)];
+  const int t = bid - P.tile0;
+  const int row0 = (t / P.tiles_n) * BM, col0 = (t % P.tiles_n) * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  acc_t acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+
+  int emax = -(1 << 30);
+  for (int s = 0; s < P.nseg; ++s)
+    if (segs[P.seg0 + s].exp) emax = max(emax, segs[P.seg0 + s].exp[0]);
+
+  for (int s = 0; s < P.nseg; ++s) {
+    const GemmSeg S = segs[P.seg0 + s];
+    double scale = S.scale;
+    if (S.exp) scale = ldexp(scale, S.exp[0] - emax);
+    const bool a_row_fast = (S.a_rs == 1);
+    const bool b_row_fast = (S.b_rs == 1);
+    for (int k0 = 0; k0 < S.K; k0 += BK) {
+      // A tile BM x BK
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int i, l;
+        if (a_row_fast) { i = tid & 63; l = (tid >> 6) + 4 * q; }
+        else { i = (tid >> 4) + 16 * q; l = tid & 15; }
+        const int64_t gi = row0 + i, gl = k0 + l;
+        double v = 0.0;
+        if (gi < P.M && gl < S.K) {
+          v = (S.a_mode == AM_PLAIN || (S.a_mode == AM_VT_UNIT ? gl > gi : gi > gl))
+                  ? load_d<TA>(S.A, gi * S.a_rs + gl * S.a_cs) : 0.0;
+          v = amode_value(S.a_mode, gi, gl, v) * scale;
+        }
+        As[l][i] = MT(v);
+      }
+      // B tile BK x BN
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int l, j;
+        if (b_row_fast) { l = tid & 15; j = (tid >> 4) + 16 * q; }
+        else { j = tid & 63; l = (tid >> 6) + 4 * q; }
+        const int64_t gl = k0 + l, gj = col0 + j;
+        double v = 0.0;
+        if (gl < S.K && gj < P.N) v = load_d<TB>(S.B, gl * S.b_rs + gj * S.b_cs);
+        Bs[l][j] = MT(v);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        MT a0 = As[kk + (lane >> 4)][wm * 32 + (lane & 15)];
+        MT a1 = As[kk + (lane >> 4)][wm * 32 + 16 + (lane & 15)];
+        MT b0 = Bs[kk + (lane >> 4)][wn * 32 + (lane & 15)];
+        MT b1 = Bs[kk + (lane >> 4)][wn * 32 + 16 + (lane & 15)];
+        if constexpr (std::is_same<MT, double>::value) {
+          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        } else {
+          acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+          acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+          acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 block
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = row0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + r;
+        const int64_t gj = col0 + wn * 32 + bj * 16 + (lane & 15);
+        if (gi < P.M && gj < P.N) {
+          const int64_t ci = gi * P.c_rs + gj * P.c_cs;
+          double v = P.alpha * double(acc[bi][bj][r]);
+          if (P.beta != 0.0) v += P.beta * load_d<TC>(P.C, ci);
+          store_c<TC>(P.C, ci, v);
+        }
+      }
+}
+
+// ------------------------------------------------------------- BF16 math
+// LDS holds A as [m][k] and B as [n][k] (k contiguous) so each lane reads its
+// 8 consecutive k values of the 16x16x32 operand as one 16-byte vector.
+template <class TA, class TB, class TC>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmProb* __restrict__ probs, int nprobs,
+                                                        const GemmSeg* __restrict__ segs) {
+  constexpr int BK = 32;
+  constexpr int KP = BK + 8;  // padded row (elements)
+  __shared__ __attribute__((aligned(16))) uint16_t As[BM][KP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][KP];
+
+  const int bid = blockIdx.x;
+  const GemmProb P = probs[find_prob(probs, nprobs, bid)];
+  const int t = bid - P.tile0;
+  const int row0 = (t / P.tiles_n) * BM, col0 = (t % P.tiles_n) * BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  f4_t acc[2][2];
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) acc[a][b] = f4_t{0, 0, 0, 0};
+
+  int emax = -(1 << 30);
+  for (int s = 0; s < P.nseg; ++s)
+    if (segs[P.seg0 + s].exp) emax = max(emax, segs[P.seg0 + s].exp[0]);
+
+  for (int s = 0; s < P.nseg; ++s) {
+    const GemmSeg S = segs[P.seg0 + s];
+    double scale = S.scale;
+    if (S.exp) scale = ldexp(scale, S.exp[0] - emax);
+    const bool a_row_fast = (S.a_rs == 1);
+    const bool b_row_fast = (S.b_rs == 1);
+    for (int k0 = 0; k0 < S.K; k0 += BK) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        int i, l;
+        if (a_row_fast) { i = tid & 63; l = (tid >> 6) + 4 * q; }
+        else { i = (tid >> 5) + 8 * q; l = tid & 31; }
+        const int64_t gi = row0 + i, gl = k0 + l;
+        double v = 0.0;
+        if (gi < P.M && gl < S.K) {
+          v = (S.a_mode == AM_PLAIN || (S.a_mode == AM_VT_UNIT ? gl > gi : gi > gl))
+                  ? load_d<TA>(S.A, gi * S.a_rs + gl * S.a_cs) : 0.0;
+          v = amode_value(S.a_mode, gi, gl, v) * scale;
+        }
+        As[i][l] = f32_to_bf16(float(v));
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        int l, j;
+        if (b_row_fast) { l = tid & 31; j = (tid >> 5) + 8 * q; }
+        else { j = tid & 63; l = (tid >> 6) + 4 * q; }
+        const int64_t gl = k0 + l, gj = col0 + j;
+        double v = 0.0;
+        if (gl < S.K && gj < P.N) v = load_d<TB>(S.B, gl * S.b_rs + gj * S.b_cs);
+        Bs[j][l] = f32_to_bf16(float(v));
+      }
+      __syncthreads();
+      const int kq = 8 * (lane >> 4);
+      bf8_t a0 = *reinterpret_cast<const bf8_t*>(&As[wm * 32 + (lane & 15)][kq]);
+      bf8_t a1 = *reinterpret_cast<const bf8_t*>(&As[wm * 32 + 16 + (lane & 15)][kq]);
+      bf8_t b0 = *reinterpret_cast<const bf8_t*>(&Bs[wn * 32 + (lane & 15)][kq]);
+      bf8_t b1 = *reinterpret_cast<const bf8_t*>(&Bs[wn * 32 + 16 + (lane & 15)][kq]);
+      acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gi = row0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + r;
+        const int64_t gj = col0 + wn * 32 + bj * 16 + (lane & 15);
+        if (gi < P.M && gj < P.N) {
+          const int64_t ci = gi * P.c_rs + gj * P.c_cs;
+          double v = P.alpha * double(acc[bi][bj][r]);
+          if (P.beta != 0.0) v += P.beta * load_d<TC>(P.C, ci);
+          store_c<TC>(P.C, ci, v);
+        }
+      }
+}
+
+// ------------------------------------------------------------- host side
+void GemmBatch::prob(void* C, int64_t c_rs, int64_t c_cs, int M, int N, double alpha, double beta) {
+  GemmProb p{};
+  p.C = C;
+  p.c_rs = c_rs;
+  p.c_cs = c_cs;
+  p.M = M;
+  p.N = N;
+  p.alpha = alpha;
+  p.beta = beta;
+  p.seg0 = int32_t(segs_.size());
+  p.nseg = 0;
+  probs_.push_back(p);
+}
+
+void GemmBatch::seg(const void* A, int64_t a_rs, int64_t a_cs, const void* B, int64_t b_rs, int64_t b_cs,
+                    int K, double scale, int a_mode, const int32_t* exp) {
+  GemmSeg s{};
+  s.A = A;
+  s.B = B;
+  s.a_rs = a_rs;
+  s.a_cs = a_cs;
+  s.b_rs = b_rs;
+  s.b_cs = b_cs;
+  s.K = K;
+  s.a_mode = a_mode;
+  s.scale = scale;
+  s.exp = exp;
+  segs_.push_back(s);
+  probs_.back().nseg += 1;
+}
+
+namespace {
+template <class TA, class TB, class TC>
+void launch_math(Ctx& ctx, Prec math, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
+  dim3 grid(ntiles), block(256);
+  if (math == Prec::F64)
+    hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, double>), grid, block, 0, ctx.stream, dp, np, ds);
+  else if (math == Prec::F32)
+    hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, float>), grid, block, 0, ctx.stream, dp, np, ds);
+  else
+    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, TC>), grid, block, 0, ctx.stream, dp, np, ds);
+  TTR_HIP(hipGetLastError());
+}
+
+template <class TA, class TB>
+void launch_c(Ctx& ctx, Prec tc, Prec math, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
+  if (tc == Prec::F64) launch_math<TA, TB, double>(ctx, math, ntiles, dp, np, ds);
+  else if (tc == Prec::F32) launch_math<TA, TB, float>(ctx, math, ntiles, dp, np, ds);
+  else launch_math<TA, TB, bf16s>(ctx, math, ntiles, dp, np, ds);
+}
+
+template <class TA>
+void launch_b(Ctx& ctx, Prec tb, Prec tc, Prec math, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
+  if (tb == Prec::F64) launch_c<TA, double>(ctx, tc, math, ntiles, dp, np, ds);
+  else if (tb == Prec::F32) launch_c<TA, float>(ctx, tc, math, ntiles, dp, np, ds);
+  else launch_c<TA, bf16s>(ctx, tc, math, ntiles, dp, np, ds);
+}
+}  // namespace
+
+void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math) {
+  // drop empty problems, assign tile prefixes
+  std::vector<GemmProb> probs;
+  probs.reserve(b.probs_.size());
+  int64_t ntiles = 0;
+  for (auto p : b.probs_) {
+    if (p.M <= 0 || p.N <= 0) continue;
+    const int tm = (p.M + BM - 1) / BM, tn = (p.N + BN - 1) / BN;
+    p.tile0 = int32_t(ntiles);
+    p.tiles_n = tn;
+    ntiles += int64_t(tm) * tn;
+    probs.push_back(p);
+  }
+  std::vector<GemmSeg> segs = std::move(b.segs_);
+  b.clear();
+  if (probs.empty()) return;
+  TTR_REQUIRE(ntiles < (int64_t(1) << 31), ST_INVALID_ARG, "GEMM batch too large");
+  const GemmProb* dp = ctx.upload(probs);
+  const GemmSeg* ds = ctx.upload(segs);
+  const int np = int(probs.size());
+  if (ta == Prec::F64) launch_b<double>(ctx, tb, tc, math, int(ntiles), dp, np, ds);
+  else if (ta == Prec::F32) launch_b<float>(ctx, tb, tc, math, int(ntiles), dp, np, ds);
+  else launch_b<bf16s>(ctx, tb, tc, math, int(ntiles), dp, np, ds);
+}
+
+}  // namespace ttr

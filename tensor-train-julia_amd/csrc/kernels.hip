@@ -1,0 +1,374 @@
+// Data-movement, RNG and small elementwise kernels of libttround.
+#include "ttr_internal.h"
+
+namespace ttr {
+
+namespace {
+
+__device__ __forceinline__ float bf16_to_f32_(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+__device__ __forceinline__ uint16_t f32_to_bf16_(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return uint16_t((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
+__device__ __forceinline__ void store_prec(void* p, int64_t i, double v, int prec) {
+  if (prec == 0) static_cast<double*>(p)[i] = v;
+  else if (prec == 1) static_cast<float*>(p)[i] = float(v);
+  else static_cast<uint16_t*>(p)[i] = f32_to_bf16_(float(v));
+}
+
+template <class T>
+__device__ __forceinline__ double ld_as_d(const T* p, int64_t i);
+template <> __device__ __forceinline__ double ld_as_d<double>(const double* p, int64_t i) { return p[i]; }
+template <> __device__ __forceinline__ double ld_as_d<float>(const float* p, int64_t i) { return p[i]; }
+template <> __device__ __forceinline__ double ld_as_d<uint16_t>(const uint16_t* p, int64_t i) { return bf16_to_f32_(p[i]); }
+
+template <class T>
+__device__ __forceinline__ void st_from_d(T* p, int64_t i, double v);
+template <> __device__ __forceinline__ void st_from_d<double>(double* p, int64_t i, double v) { p[i] = v; }
+template <> __device__ __forceinline__ void st_from_d<float>(float* p, int64_t i, double v) { p[i] = float(v); }
+template <> __device__ __forceinline__ void st_from_d<uint16_t>(uint16_t* p, int64_t i, double v) { p[i] = f32_to_bf16_(float(v)); }
+
+__device__ __forceinline__ int find_job(const int32_t* prefix, int njobs, int bid) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= bid) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// ---------------------------------------------------------------- Philox
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), counter-based.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Standard normal draw number e of core k: counter (e_lo, e_hi, k, 0x5EED),
+// key = seed; two 53-bit uniforms, Box-Muller cosine branch, in FP64.
+__device__ __forceinline__ double rng_normal(uint64_t seed, int core, uint64_t e, uint32_t tag = 0x5EEDu) {
+  const uint4 x = philox4x32_10(make_uint4(uint32_t(e), uint32_t(e >> 32), uint32_t(core), tag),
+                                uint32_t(seed), uint32_t(seed >> 32));
+  const uint64_t a = ((uint64_t(x.x) << 32) | x.y) >> 11;
+  const uint64_t b = ((uint64_t(x.z) << 32) | x.w) >> 11;
+  const double u1 = (double(a) + 0.5) * 0x1.0p-53;
+  const double u2 = double(b) * 0x1.0p-53;
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+// ---------------------------------------------------------------- kernels
+
+struct Copy2DDev {
+  Copy2D j;
+  int32_t tile0, tiles_n, tiles_m;
+};
+
+__global__ __launch_bounds__(256) void copy2d_kernel(const Copy2DDev* __restrict__ jobs, const int32_t* __restrict__ prefix,
+                                                     int njobs) {
+  __shared__ double tile[32][33];
+  const int jid = find_job(prefix, njobs, blockIdx.x);
+  const Copy2DDev D = jobs[jid];
+  const Copy2D& J = D.j;
+  int t = blockIdx.x - D.tile0;
+  const int per_b = D.tiles_m * D.tiles_n;
+  const int b = t / per_b;
+  t -= b * per_b;
+  const int i0 = (t / D.tiles_n) * 32, j0 = (t % D.tiles_n) * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const double* in = J.in + b * J.ibs;
+  double* out = J.out + b * J.obs;
+  const bool in_i_fast = (J.irs == 1);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int ii = in_i_fast ? tx : ty + 8 * q;
+    const int jj = in_i_fast ? ty + 8 * q : tx;
+    const int gi = i0 + ii, gj = j0 + jj;
+    if (gi < J.M && gj < J.N) {
+      const bool zero = (J.tri == 1 && gi > gj) || (J.tri == 2 && gi < gj);
+      tile[ii][jj] = zero ? 0.0 : J.alpha * in[gi * J.irs + gj * J.ics];
+    }
+  }
+  __syncthreads();
+  const bool out_i_fast = (J.ors == 1);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int ii = out_i_fast ? tx : ty + 8 * q;
+    const int jj = out_i_fast ? ty + 8 * q : tx;
+    const int gi = i0 + ii, gj = j0 + jj;
+    if (gi < J.M && gj < J.N) out[gi * J.ors + gj * J.ocs] = tile[ii][jj];
+  }
+}
+
+__global__ void fill2d_kernel(const Fill2D* __restrict__ jobs) {
+  const Fill2D J = jobs[blockIdx.y];
+  const int64_t tot = int64_t(J.M) * J.N;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = e % J.M, c = e / J.M;
+    J.out[i + c * J.ld] = (i == c) ? J.diag : 0.0;
+  }
+}
+
+__global__ void gather_cols_kernel(const GatherCols* __restrict__ jobs) {
+  const GatherCols J = jobs[blockIdx.y];
+  const int64_t tot = int64_t(J.M) * J.ncol;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = e % J.M, c = e / J.M;
+    J.out[i + c * J.ldo] = J.in[i + int64_t(J.idx[c]) * J.ldi];
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void pow2_kernel(const Pow2Job* __restrict__ jobs) {
+  const Pow2Job J = jobs[blockIdx.x];
+  T* X = static_cast<T*>(J.X);
+  const int64_t tot = int64_t(J.M) * J.N;
+  double m = 0.0;
+  for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int64_t i = e % J.M, c = e / J.M;
+    m = fmax(m, fabs(ld_as_d<T>(X, i + c * J.ld)));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  int e2 = 0;
+  if (m > 0.0 && isfinite(m)) e2 = ilogb(m);
+  const double s = ldexp(1.0, -e2);
+  for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
+    const int64_t i = e % J.M, c = e / J.M;
+    st_from_d<T>(X, i + c * J.ld, ld_as_d<T>(X, i + c * J.ld) * s);
+  }
+  if (threadIdx.x == 0) J.exp_out[0] = (J.exp_in ? J.exp_in[0] : 0) + e2;
+}
+
+__global__ void sketch_gen_kernel(const SketchGen* __restrict__ jobs, uint64_t seed, int prec) {
+  const SketchGen J = jobs[blockIdx.y];
+  const int64_t tot = int64_t(J.n) * J.bvr * J.bvl * J.p;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
+    // e is the reference (randn!) order index: beta + bvr*(z + n*(b + bvl*j))
+    const double g = rng_normal(seed, J.core, uint64_t(e));
+    const int64_t beta = e % J.bvr;
+    const int64_t rest = e / J.bvr;
+    const int64_t z = rest % J.n;
+    const int64_t col = rest / J.n;  // b + bvl*j
+    if (J.out) store_prec(J.out, z + J.n * (beta + int64_t(J.bvr) * col), J.scale * g, prec);
+    if (J.ref_out) J.ref_out[e] = g;
+  }
+}
+
+__global__ void randn_core_kernel(double* out, int n, int r, int r2, int core, uint64_t seed) {
+  const int64_t tot = int64_t(n) * r * r2;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = e % n;
+    const int64_t rest = e / n;
+    const int64_t a = rest % r;
+    const int64_t b = rest / r;
+    out[a + int64_t(r) * (i + int64_t(n) * b)] = rng_normal(seed, core, uint64_t(e), 0x7A11u);
+  }
+}
+
+__global__ void sketch_draws_kernel(double* out, int64_t count, int core, uint64_t seed) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < count; e += int64_t(gridDim.x) * blockDim.x)
+    out[e] = rng_normal(seed, core, uint64_t(e));
+}
+
+__global__ void sketch_pack_kernel(const double* __restrict__ ref, void* out, int n, int bvr, int bvl, int p,
+                                   double scale, int prec) {
+  const int64_t tot = int64_t(n) * bvr * bvl * p;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t beta = e % bvr;
+    const int64_t rest = e / bvr;
+    const int64_t z = rest % n;
+    const int64_t col = rest / n;
+    store_prec(out, z + n * (beta + int64_t(bvr) * col), scale * ref[e], prec);
+  }
+}
+
+__global__ void zero_kernel(uint32_t* p, int64_t words) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < words; e += int64_t(gridDim.x) * blockDim.x)
+    p[e] = 0u;
+}
+
+struct SumJob {
+  const double* const* ins;
+  int nin;
+};
+
+__global__ void sum_kernel(double* out, const double* const* __restrict__ ins, int nin, int64_t count) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < count; e += int64_t(gridDim.x) * blockDim.x) {
+    double s = 0.0;
+    for (int t = 0; t < nin; ++t) s += ins[t][e];
+    out[e] = s;
+  }
+}
+
+__global__ void block_put_kernel(const BlockPut* __restrict__ jobs) {
+  const BlockPut J = jobs[blockIdx.y];
+  const int64_t tot = int64_t(J.ra) * J.n * J.rb;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t a = e % J.ra;
+    const int64_t rest = e / J.ra;
+    const int64_t i = rest % J.n;
+    const int64_t b = rest / J.n;
+    J.out[(J.a0 + a) + int64_t(J.RA) * (i + int64_t(J.n) * (J.b0 + b))] = J.scale * J.in[e];
+  }
+}
+
+__global__ void mpo_kernel(const MpoCore* __restrict__ jobs) {
+  const MpoCore J = jobs[blockIdx.y];
+  const int64_t R1 = int64_t(J.Ra) * J.rx, R2 = int64_t(J.Ra2) * J.rx2;
+  const int64_t tot = R1 * J.n * R2;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r1 = e % R1;
+    const int64_t rest = e / R1;
+    const int64_t i = rest % J.n;
+    const int64_t r2 = rest / J.n;
+    const int64_t al = r1 % J.Ra, nu = r1 / J.Ra;
+    const int64_t al2 = r2 % J.Ra2, nu2 = r2 / J.Ra2;
+    double s = 0.0;
+    for (int j = 0; j < J.n; ++j)
+      s += J.A[i + J.n * (j + int64_t(J.n) * (al + int64_t(J.Ra) * al2))] * J.x[nu + int64_t(J.rx) * (j + int64_t(J.n) * nu2)];
+    J.y[e] = s;
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- host launchers
+
+void copy2d_batched(Ctx& ctx, const std::vector<Copy2D>& jobs) {
+  std::vector<Copy2DDev> dj;
+  std::vector<int32_t> prefix;
+  int64_t tiles = 0;
+  for (const auto& j : jobs) {
+    if (j.M <= 0 || j.N <= 0 || j.nb <= 0) continue;
+    Copy2DDev d{};
+    d.j = j;
+    d.tiles_m = (j.M + 31) / 32;
+    d.tiles_n = (j.N + 31) / 32;
+    d.tile0 = int32_t(tiles);
+    prefix.push_back(int32_t(tiles));
+    tiles += int64_t(d.tiles_m) * d.tiles_n * j.nb;
+    dj.push_back(d);
+  }
+  if (dj.empty()) return;
+  TTR_REQUIRE(tiles < (int64_t(1) << 31), ST_INVALID_ARG, "copy too large");
+  const Copy2DDev* d = ctx.upload(dj);
+  const int32_t* pre = ctx.upload(prefix);
+  hipLaunchKernelGGL(copy2d_kernel, dim3(unsigned(tiles)), dim3(256), 0, ctx.stream, d, pre, int(dj.size()));
+  TTR_HIP(hipGetLastError());
+}
+
+template <class J>
+static void launch_y(Ctx& ctx, void (*kern)(const J*), const std::vector<J>& jobs, int64_t max_elems) {
+  if (jobs.empty()) return;
+  for (size_t s = 0; s < jobs.size(); s += 65535) {
+    const size_t cnt = std::min<size_t>(65535, jobs.size() - s);
+    const J* d = ctx.upload(jobs.data() + s, cnt);
+    int64_t bx = (max_elems + 255) / 256;
+    if (bx > 1024) bx = 1024;
+    if (bx < 1) bx = 1;
+    hipLaunchKernelGGL(kern, dim3(unsigned(bx), unsigned(cnt)), dim3(256), 0, ctx.stream, d);
+    TTR_HIP(hipGetLastError());
+  }
+}
+
+void fill2d_batched(Ctx& ctx, const std::vector<Fill2D>& jobs) {
+  int64_t mx = 0;
+  for (auto& j : jobs) mx = std::max<int64_t>(mx, int64_t(j.M) * j.N);
+  launch_y(ctx, fill2d_kernel, jobs, mx);
+}
+
+void gather_cols_batched(Ctx& ctx, const std::vector<GatherCols>& jobs) {
+  int64_t mx = 0;
+  for (auto& j : jobs) mx = std::max<int64_t>(mx, int64_t(j.M) * j.ncol);
+  launch_y(ctx, gather_cols_kernel, jobs, mx);
+}
+
+void block_put_batched(Ctx& ctx, const std::vector<BlockPut>& jobs) {
+  int64_t mx = 0;
+  for (auto& j : jobs) mx = std::max<int64_t>(mx, int64_t(j.ra) * j.n * j.rb);
+  launch_y(ctx, block_put_kernel, jobs, mx);
+}
+
+void mpo_apply_batched(Ctx& ctx, const std::vector<MpoCore>& jobs) {
+  int64_t mx = 0;
+  for (auto& j : jobs) mx = std::max<int64_t>(mx, int64_t(j.Ra) * j.rx * j.n * j.Ra2 * j.rx2);
+  launch_y(ctx, mpo_kernel, jobs, mx);
+}
+
+void pow2_normalize_batched(Ctx& ctx, const std::vector<Pow2Job>& jobs, Prec prec) {
+  if (jobs.empty()) return;
+  const Pow2Job* d = ctx.upload(jobs);
+  dim3 grid(unsigned(jobs.size())), block(256);
+  if (prec == Prec::F64) hipLaunchKernelGGL(pow2_kernel<double>, grid, block, 0, ctx.stream, d);
+  else if (prec == Prec::F32) hipLaunchKernelGGL(pow2_kernel<float>, grid, block, 0, ctx.stream, d);
+  else hipLaunchKernelGGL(pow2_kernel<uint16_t>, grid, block, 0, ctx.stream, d);
+  TTR_HIP(hipGetLastError());
+}
+
+void sketch_generate(Ctx& ctx, const std::vector<SketchGen>& jobs, uint64_t seed, Prec prec) {
+  if (jobs.empty()) return;
+  int64_t mx = 0;
+  for (auto& j : jobs) mx = std::max<int64_t>(mx, int64_t(j.n) * j.bvr * j.bvl * j.p);
+  const SketchGen* d = ctx.upload(jobs);
+  int64_t bx = std::min<int64_t>(1024, std::max<int64_t>(1, (mx + 255) / 256));
+  hipLaunchKernelGGL(sketch_gen_kernel, dim3(unsigned(bx), unsigned(jobs.size())), dim3(256), 0, ctx.stream, d, seed,
+                     int(prec));
+  TTR_HIP(hipGetLastError());
+}
+
+void sketch_draws(Ctx& ctx, double* out, int64_t count, int core, uint64_t seed) {
+  if (count <= 0) return;
+  int64_t bx = std::min<int64_t>(4096, (count + 255) / 256);
+  hipLaunchKernelGGL(sketch_draws_kernel, dim3(unsigned(bx)), dim3(256), 0, ctx.stream, out, count, core, seed);
+  TTR_HIP(hipGetLastError());
+}
+
+void randn_core(Ctx& ctx, double* out, int n, int r, int r2, int core, uint64_t seed) {
+  const int64_t tot = int64_t(n) * r * r2;
+  if (tot <= 0) return;
+  int64_t bx = std::min<int64_t>(8192, (tot + 255) / 256);
+  hipLaunchKernelGGL(randn_core_kernel, dim3(unsigned(bx)), dim3(256), 0, ctx.stream, out, n, r, r2, core, seed);
+  TTR_HIP(hipGetLastError());
+}
+
+void sketch_pack(Ctx& ctx, const double* ref, void* out, int n, int bvr, int bvl, int p, double scale, Prec prec) {
+  const int64_t tot = int64_t(n) * bvr * bvl * p;
+  if (tot <= 0) return;
+  int64_t bx = std::min<int64_t>(4096, (tot + 255) / 256);
+  hipLaunchKernelGGL(sketch_pack_kernel, dim3(unsigned(bx)), dim3(256), 0, ctx.stream, ref, out, n, bvr, bvl, p, scale,
+                     int(prec));
+  TTR_HIP(hipGetLastError());
+}
+
+void sum_arrays(Ctx& ctx, double* out, const std::vector<const double*>& ins, int64_t count) {
+  if (count <= 0) return;
+  const double* const* d = ctx.upload(ins.data(), ins.size());
+  int64_t bx = std::min<int64_t>(4096, (count + 255) / 256);
+  hipLaunchKernelGGL(sum_kernel, dim3(unsigned(bx)), dim3(256), 0, ctx.stream, out, d, int(ins.size()), count);
+  TTR_HIP(hipGetLastError());
+}
+
+void zero_fill(Ctx& ctx, void* p, size_t bytes) {
+  // a fill kernel rather than hipMemsetAsync, so the call is safe inside captured graphs
+  if (!bytes) return;
+  TTR_REQUIRE((reinterpret_cast<uintptr_t>(p) & 3) == 0 && (bytes & 3) == 0, ST_INVALID_ARG, "zero_fill alignment");
+  const int64_t words = int64_t(bytes / 4);
+  int64_t bx = std::min<int64_t>(4096, (words + 255) / 256);
+  hipLaunchKernelGGL(zero_kernel, dim3(unsigned(bx)), dim3(256), 0, ctx.stream, static_cast<uint32_t*>(p), words);
+  TTR_HIP(hipGetLastError());
+}
+
+}  // namespace ttr

@@ -1,0 +1,136 @@
+// Device and pinned-host memory pools of libttround.
+#include "ttr_internal.h"
+
+namespace ttr {
+
+namespace {
+constexpr size_t kAlign = 256;
+constexpr size_t kMinBlock = size_t(64) << 20;
+inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+}  // namespace
+
+DevArena::~DevArena() { release(); }
+
+void* DevArena::alloc(size_t bytes) {
+  bytes = align_up(bytes ? bytes : kAlign);
+  if (!blocks_.empty()) {
+    Block& b = blocks_.back();
+    if (b.used + bytes <= b.cap) {
+      void* p = b.p + b.used;
+      b.used += bytes;
+      return p;
+    }
+  }
+  size_t cap = bytes > kMinBlock ? bytes : kMinBlock;
+  if (!blocks_.empty() && blocks_.back().cap > cap) cap = blocks_.back().cap;
+  char* p = nullptr;
+  TTR_HIP(hipMalloc(&p, cap));
+  blocks_.push_back({p, cap, bytes});
+  return p;
+}
+
+void DevArena::reset() {
+  if (blocks_.size() > 1) {
+    size_t total = 0;
+    for (auto& b : blocks_) total += b.cap;
+    release();
+    char* p = nullptr;
+    TTR_HIP(hipMalloc(&p, total));
+    blocks_.push_back({p, total, 0});
+  } else if (!blocks_.empty()) {
+    blocks_[0].used = 0;
+  }
+}
+
+void DevArena::release() {
+  for (auto& b : blocks_) (void)hipFree(b.p);
+  blocks_.clear();
+}
+
+PinnedArena::~PinnedArena() { release(); }
+
+void* PinnedArena::alloc(size_t bytes) {
+  bytes = align_up(bytes ? bytes : kAlign);
+  if (!blocks_.empty()) {
+    Block& b = blocks_.back();
+    if (b.used + bytes <= b.cap) {
+      void* p = b.p + b.used;
+      b.used += bytes;
+      return p;
+    }
+  }
+  size_t cap = bytes > (size_t(8) << 20) ? bytes : (size_t(8) << 20);
+  if (!blocks_.empty() && blocks_.back().cap > cap) cap = blocks_.back().cap;
+  char* p = nullptr;
+  TTR_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), cap, hipHostMallocDefault));
+  blocks_.push_back({p, cap, bytes});
+  return p;
+}
+
+void PinnedArena::reset() {
+  if (blocks_.size() > 1) {
+    size_t total = 0;
+    for (auto& b : blocks_) total += b.cap;
+    release();
+    char* p = nullptr;
+    TTR_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), total, hipHostMallocDefault));
+    blocks_.push_back({p, total, 0});
+  } else if (!blocks_.empty()) {
+    blocks_[0].used = 0;
+  }
+}
+
+void PinnedArena::release() {
+  for (auto& b : blocks_) (void)hipHostFree(b.p);
+  blocks_.clear();
+}
+
+CachingAlloc::~CachingAlloc() {
+  release_cached();
+  for (auto& kv : live_) (void)hipFree(kv.first);
+  live_.clear();
+}
+
+size_t CachingAlloc::round_size(size_t b) {
+  if (b < (size_t(1) << 20)) return align_up(b < 512 ? 512 : b);
+  // round to 1/8 of the enclosing power of two
+  size_t p = 1;
+  while (p < b) p <<= 1;
+  size_t step = p >> 3;
+  return (b + step - 1) / step * step;
+}
+
+void* CachingAlloc::alloc(size_t bytes) {
+  size_t sz = round_size(bytes);
+  auto it = free_.find(sz);
+  void* p = nullptr;
+  if (it != free_.end()) {
+    p = it->second;
+    free_.erase(it);
+  } else {
+    hipError_t e = hipMalloc(&p, sz);
+    if (e == hipErrorOutOfMemory) {
+      (void)hipGetLastError();
+      release_cached();
+      e = hipMalloc(&p, sz);
+    }
+    TTR_HIP(e);
+  }
+  live_[p] = sz;
+  return p;
+}
+
+void CachingAlloc::free(void* p) {
+  if (!p) return;
+  auto it = live_.find(p);
+  if (it == live_.end()) return;
+  free_.emplace(it->second, p);
+  live_.erase(it);
+}
+
+void CachingAlloc::release_cached() {
+  for (auto& kv : free_) (void)hipFree(kv.second);
+  free_.clear();
+}
+
+}  // namespace ttr

@@ -1,0 +1,218 @@
+// Batched FP64 Householder QR with explicit thin Q.
+//
+// Blocked right-looking Householder (panel width 32, compact WY
+// I - V T V^T), following the conventions of LAPACK dgeqrf/dlarfg/dorgqr so
+// that Q agrees with the reference's LAPACK factorisations column by column:
+//   beta = -sign(alpha) * ||(alpha, x)||,  tau = (beta - alpha)/beta,
+//   v = (1, x/(alpha - beta)),  tau = 0 when x == 0.
+// The panel is factorised by one workgroup per matrix; the trailing update and
+// the formation of Q are grouped FP64 MFMA GEMMs over every matrix of the batch.
+#include "ttr_internal.h"
+
+namespace ttr {
+
+namespace {
+
+constexpr int NBP = 32;     // panel width
+constexpr int PT = 256;     // panel kernel threads
+constexpr int PW = PT / 64; // waves
+
+struct PanelJob {
+  double* A;
+  int64_t lda;
+  int32_t m, c0, nb;
+  double* tau;  // tau for columns c0..c0+nb-1 (indexed by column)
+  double* T;    // NBP x NBP, ld NBP, upper triangular
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__global__ __launch_bounds__(PT) void qr_panel_kernel(const PanelJob* __restrict__ jobs) {
+  const PanelJob J = jobs[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ double red[PW][NBP + 1];
+  __shared__ double dsum[NBP];
+  __shared__ double Ts[NBP][NBP + 1];
+  __shared__ double sh_alpha, sh_sigma;
+
+  for (int idx = tid; idx < NBP * (NBP + 1); idx += PT) (&Ts[0][0])[idx] = 0.0;
+
+  double* A = J.A;
+  const int64_t lda = J.lda;
+  const int m = J.m;
+  for (int jj = 0; jj < J.nb; ++jj) {
+    const int j = J.c0 + jj;
+    double* colj = A + int64_t(j) * lda;
+    // --- norm of the sub-column
+    double part = 0.0;
+    for (int i = j + 1 + tid; i < m; i += PT) {
+      const double x = colj[i];
+      part += x * x;
+    }
+    part = wave_sum(part);
+    if (lane == 0) red[wave][0] = part;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < PW; ++w) s += red[w][0];
+      sh_sigma = s;
+      sh_alpha = colj[j];
+    }
+    __syncthreads();
+    const double sigma = sh_sigma, alpha = sh_alpha;
+    double tau = 0.0, beta = alpha, scal = 1.0;
+    if (sigma != 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+      tau = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    if (sigma != 0.0)
+      for (int i = j + 1 + tid; i < m; i += PT) colj[i] *= scal;
+    __syncthreads();
+    if (tid == 0) {
+      colj[j] = beta;
+      J.tau[j] = tau;
+    }
+    // --- d_c = A[j,c] + sum_{i>j} v_i A[i,c] for every other panel column c:
+    //     c > j: w_c for the reflector application; c < j: V^T v_j for T
+    double acc[NBP];
+#pragma unroll
+    for (int c = 0; c < NBP; ++c) acc[c] = 0.0;
+    for (int i = j + 1 + tid; i < m; i += PT) {
+      const double v = colj[i];
+#pragma unroll
+      for (int c = 0; c < NBP; ++c)
+        if (c < J.nb && c != jj) acc[c] += v * A[i + int64_t(J.c0 + c) * lda];
+    }
+#pragma unroll
+    for (int c = 0; c < NBP; ++c) {
+      const double s = wave_sum(acc[c]);
+      if (lane == 0) red[wave][c] = s;
+    }
+    __syncthreads();
+    if (tid < J.nb && tid != jj) {
+      double s = A[j + int64_t(J.c0 + tid) * lda];
+      for (int w = 0; w < PW; ++w) s += red[w][tid];
+      dsum[tid] = s;
+    }
+    __syncthreads();
+    // --- apply H_j to the remaining panel columns (rows j..m-1)
+    if (tau != 0.0) {
+      for (int c = jj + 1; c < J.nb; ++c) {
+        const double w = tau * dsum[c];
+        double* colc = A + int64_t(J.c0 + c) * lda;
+        for (int i = j + tid; i < m; i += PT) {
+          const double v = (i == j) ? 1.0 : colj[i];
+          colc[i] -= v * w;
+        }
+      }
+    }
+    // --- T(:, jj): T(jj,jj) = tau, T(0:jj, jj) = -tau * T(0:jj,0:jj) z
+    if (tid < jj) {
+      double s = 0.0;
+      for (int q = tid; q < jj; ++q) s += Ts[tid][q] * dsum[q];
+      Ts[tid][jj] = -tau * s;
+    }
+    if (tid == 0) Ts[jj][jj] = tau;
+    __syncthreads();
+  }
+  for (int idx = tid; idx < NBP * NBP; idx += PT) {
+    const int r = idx % NBP, c = idx / NBP;
+    J.T[r + NBP * c] = (r < J.nb && c < J.nb) ? Ts[r][c] : 0.0;
+  }
+}
+
+}  // namespace
+
+void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
+  struct Aux {
+    double* tau;
+    double* T;   // npanel * NBP * NBP
+    double* W;   // NBP x n
+    double* VT;  // m x NBP
+    int kq, npanel;
+  };
+  std::vector<Aux> aux(jobs.size());
+  int max_panels = 0;
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    const QRJob& J = jobs[q];
+    TTR_REQUIRE(J.m >= 0 && J.n >= 0 && J.lda >= std::max(1, J.m), ST_INVALID_ARG, "qr: bad shape");
+    Aux& a = aux[q];
+    a.kq = std::min(J.m, J.n);
+    a.npanel = (a.kq + NBP - 1) / NBP;
+    max_panels = std::max(max_panels, a.npanel);
+    a.tau = ctx.ws.alloc_n<double>(std::max(1, J.n));
+    a.T = ctx.ws.alloc_n<double>(size_t(std::max(1, a.npanel)) * NBP * NBP);
+    a.W = ctx.ws.alloc_n<double>(size_t(NBP) * std::max(1, J.n));
+    a.VT = ctx.ws.alloc_n<double>(size_t(std::max(1, J.m)) * NBP);
+  }
+  GemmBatch gb;
+  // ---- factorisation
+  for (int p = 0; p < max_panels; ++p) {
+    std::vector<PanelJob> pj;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const Aux& a = aux[q];
+      if (p >= a.npanel) continue;
+      const int c0 = p * NBP, nb = std::min(NBP, a.kq - c0);
+      pj.push_back(PanelJob{jobs[q].A, jobs[q].lda, jobs[q].m, c0, nb, a.tau, a.T + size_t(p) * NBP * NBP});
+    }
+    if (pj.empty()) continue;
+    const PanelJob* d = ctx.upload(pj);
+    hipLaunchKernelGGL(qr_panel_kernel, dim3(unsigned(pj.size())), dim3(PT), 0, ctx.stream, d);
+    TTR_HIP(hipGetLastError());
+    // trailing update A2 -= (V T^T) (V^T A2)
+    for (int stage = 0; stage < 3; ++stage) {
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const QRJob& J = jobs[q];
+        const Aux& a = aux[q];
+        if (p >= a.npanel) continue;
+        const int c0 = p * NBP, nb = std::min(NBP, a.kq - c0);
+        const int m2 = J.m - c0, n2 = J.n - c0 - nb;
+        if (n2 <= 0) continue;
+        double* V = J.A + c0 + J.lda * c0;
+        double* A2 = J.A + c0 + J.lda * (c0 + nb);
+        const double* T = a.T + size_t(p) * NBP * NBP;
+        if (stage == 0)  // W = V^T A2  (nb x n2)
+          gb.add(nb, n2, m2, 1.0, V, J.lda, 1, A2, 1, J.lda, 0.0, a.W, 1, NBP, AM_VT_UNIT);
+        else if (stage == 1)  // VT = V T^T  (m2 x nb)
+          gb.add(m2, nb, nb, 1.0, V, 1, J.lda, T, NBP, 1, 0.0, a.VT, 1, m2, AM_V_UNIT);
+        else  // A2 -= VT W
+          gb.add(m2, n2, nb, -1.0, a.VT, 1, m2, a.W, 1, NBP, 1.0, A2, 1, J.lda);
+      }
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    }
+  }
+  // ---- thin Q by backward accumulation
+  std::vector<Fill2D> fills;
+  for (size_t q = 0; q < jobs.size(); ++q)
+    if (jobs[q].Q && aux[q].kq > 0) fills.push_back(Fill2D{jobs[q].Q, jobs[q].ldq, jobs[q].m, aux[q].kq, 1.0});
+  if (fills.empty()) return;
+  fill2d_batched(ctx, fills);
+  for (int p = max_panels - 1; p >= 0; --p) {
+    for (int stage = 0; stage < 3; ++stage) {
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const QRJob& J = jobs[q];
+        const Aux& a = aux[q];
+        if (!J.Q || p >= a.npanel) continue;
+        const int c0 = p * NBP, nb = std::min(NBP, a.kq - c0);
+        const int m2 = J.m - c0, n2 = a.kq - c0;
+        double* V = J.A + c0 + J.lda * c0;
+        double* Q2 = J.Q + c0 + J.ldq * c0;
+        const double* T = a.T + size_t(p) * NBP * NBP;
+        if (stage == 0)  // W = V^T Q2  (nb x n2)
+          gb.add(nb, n2, m2, 1.0, V, J.lda, 1, Q2, 1, J.ldq, 0.0, a.W, 1, NBP, AM_VT_UNIT);
+        else if (stage == 1)  // VT = V T  (m2 x nb)
+          gb.add(m2, nb, nb, 1.0, V, 1, J.lda, T, 1, NBP, 0.0, a.VT, 1, m2, AM_V_UNIT);
+        else  // Q2 -= VT W
+          gb.add(m2, n2, nb, -1.0, a.VT, 1, m2, a.W, 1, NBP, 1.0, Q2, 1, J.ldq);
+      }
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    }
+  }
+}
+
+}  // namespace ttr

@@ -1,0 +1,300 @@
+// Internal declarations of the MI355X TT-rounding library (libttround).
+//
+// Device layout of a TT core k: (r_{k-1}, n_k, r_k) column-major, i.e. element
+// (a, i, b) at a + r_{k-1}*(i + n_k*b).  Both unfoldings are plain column-major
+// matrices: the left unfolding (r_{k-1} n_k) x r_k and the right unfolding
+// r_{k-1} x (n_k r_k), each with leading dimension r_{k-1}*n_k resp. r_{k-1}.
+// Host-side (reference) cores are (n_k, r_{k-1}, r_k) column-major and are
+// permuted on upload/download.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace ttr {
+
+enum Status : int {
+  ST_OK = 0,
+  ST_INVALID_ARG = 1,
+  ST_DIM_MISMATCH = 2,
+  ST_OOM = 3,
+  ST_NUMERICAL_FAILURE = 4,
+  ST_DEVICE_ERROR = 5,
+  ST_NOT_FOUND = 6,
+};
+
+struct Error : std::runtime_error {
+  int status;
+  Error(int s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define TTR_HIP(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw ::ttr::Error(e_ == hipErrorOutOfMemory ? ::ttr::ST_OOM : ::ttr::ST_DEVICE_ERROR, \
+                         std::string(#call) + ": " + hipGetErrorString(e_) + " (" __FILE__ \
+                         ":" + std::to_string(__LINE__) + ")");                            \
+  } while (0)
+
+#define TTR_REQUIRE(cond, st, msg)                 \
+  do {                                             \
+    if (!(cond)) throw ::ttr::Error((st), (msg));  \
+  } while (0)
+
+enum class Prec : int { F64 = 0, F32 = 1, BF16 = 2 };
+
+inline size_t prec_bytes(Prec p) { return p == Prec::F64 ? 8 : (p == Prec::F32 ? 4 : 2); }
+
+// ---------------------------------------------------------------------------
+// memory
+// ---------------------------------------------------------------------------
+
+// Bump arena of device memory, reset after every library call.  When one call
+// needs more than the current block, further blocks are added and merged into
+// a single block on reset, so steady-state calls allocate nothing.
+class DevArena {
+ public:
+  ~DevArena();
+  void* alloc(size_t bytes);
+  template <class T>
+  T* alloc_n(size_t n) { return static_cast<T*>(alloc(n * sizeof(T))); }
+  void reset();  // caller must have synchronised the stream
+  void release();
+
+ private:
+  struct Block { char* p; size_t cap; size_t used; };
+  std::vector<Block> blocks_;
+};
+
+// Same scheme over pinned host memory (staging for descriptor arrays).
+class PinnedArena {
+ public:
+  ~PinnedArena();
+  void* alloc(size_t bytes);
+  void reset();
+  void release();
+
+ private:
+  struct Block { char* p; size_t cap; size_t used; };
+  std::vector<Block> blocks_;
+};
+
+// Caching allocator for long-lived objects (TT storage): frees go to per-size
+// free lists and are reused, so repeated calls of the same shape do not call
+// hipMalloc.
+class CachingAlloc {
+ public:
+  ~CachingAlloc();
+  void* alloc(size_t bytes);
+  void free(void* p);
+  void release_cached();
+
+ private:
+  static size_t round_size(size_t b);
+  std::multimap<size_t, void*> free_;
+  std::unordered_map<void*, size_t> live_;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DevArena ws;
+  PinnedArena pin;
+  CachingAlloc cache;
+  // copies a host array into fresh device workspace (asynchronously, through pinned memory)
+  template <class T>
+  T* upload(const T* host, size_t n) {
+    if (n == 0) return nullptr;
+    void* h = pin.alloc(n * sizeof(T));
+    std::memcpy(h, host, n * sizeof(T));
+    T* d = ws.alloc_n<T>(n);
+    TTR_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, stream));
+    return d;
+  }
+  template <class T>
+  T* upload(const std::vector<T>& v) { return upload(v.data(), v.size()); }
+  void sync() { TTR_HIP(hipStreamSynchronize(stream)); }
+  void end_call() { sync(); ws.reset(); pin.reset(); }
+};
+
+// ---------------------------------------------------------------------------
+// grouped GEMM: C_p = alpha_p * sum_s scale_s A_s B_s + beta_p C_p
+// Element (i,l) of A_s is at A_s[i*a_rs + l*a_cs] (same for B and C), so any
+// transpose is expressed through the strides.
+// ---------------------------------------------------------------------------
+
+enum AMode : int32_t {
+  AM_PLAIN = 0,
+  AM_VT_UNIT = 1,  // A(i,l) = V(l,i) with V unit lower trapezoidal (zero if l<i, one if l==i)
+  AM_V_UNIT = 2,   // A(i,l) = V(i,l) unit lower trapezoidal (zero if i<l, one if i==l)
+};
+
+struct GemmSeg {
+  const void* A;
+  const void* B;
+  int64_t a_rs, a_cs, b_rs, b_cs;
+  int32_t K;
+  int32_t a_mode;
+  double scale;
+  const int32_t* exp;  // optional: scale *= 2^(exp[0] - max over the problem's segments)
+};
+
+struct GemmProb {
+  void* C;
+  int64_t c_rs, c_cs;
+  int32_t M, N;
+  int32_t seg0, nseg;
+  double alpha, beta;
+  int32_t tile0, tiles_n;
+};
+
+class GemmBatch {
+ public:
+  // starts a problem; segments are added with seg()
+  void prob(void* C, int64_t c_rs, int64_t c_cs, int M, int N, double alpha, double beta);
+  void seg(const void* A, int64_t a_rs, int64_t a_cs, const void* B, int64_t b_rs, int64_t b_cs,
+           int K, double scale = 1.0, int a_mode = AM_PLAIN, const int32_t* exp = nullptr);
+  // convenience: single-segment problem
+  void add(int M, int N, int K, double alpha, const void* A, int64_t a_rs, int64_t a_cs,
+           const void* B, int64_t b_rs, int64_t b_cs, double beta, void* C, int64_t c_rs,
+           int64_t c_cs, int a_mode = AM_PLAIN) {
+    prob(C, c_rs, c_cs, M, N, alpha, beta);
+    seg(A, a_rs, a_cs, B, b_rs, b_cs, K, 1.0, a_mode);
+  }
+  bool empty() const { return probs_.empty(); }
+  void clear() { probs_.clear(); segs_.clear(); }
+  std::vector<GemmProb> probs_;
+  std::vector<GemmSeg> segs_;
+};
+
+// Launch all problems of the batch in one grouped launch.  ta/tb/tc are the
+// storage precisions of A, B, C; math is the MFMA precision (F64, F32 or BF16,
+// the latter two accumulating in FP32).
+void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math);
+
+// ---------------------------------------------------------------------------
+// Householder QR (LAPACK dgeqrf/dorgqr conventions), batched
+// ---------------------------------------------------------------------------
+struct QRJob {
+  double* A;       // m x n column-major, overwritten by R (upper) and V (strictly lower)
+  int64_t lda;
+  int m, n;
+  double* Q;       // optional output: thin Q, m x min(m,n)
+  int64_t ldq;
+};
+// factorizes every job; forms thin Q into job.Q when it is non-null
+void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs);
+
+// ---------------------------------------------------------------------------
+// one-sided Jacobi SVD of square matrices, batched
+// ---------------------------------------------------------------------------
+struct SVDJob {
+  double* B;   // p x p (ld ldb): overwritten by B*J (columns U_j s_j)
+  int64_t ldb;
+  double* J;   // p x p (ld ldj): accumulated rotations (right singular vectors)
+  int64_t ldj;
+  double* sv;  // p singular values (column norms of B*J), unsorted
+  int p;
+};
+void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs);
+
+// ---------------------------------------------------------------------------
+// elementwise / data-movement kernels
+// ---------------------------------------------------------------------------
+struct Copy2D {  // out[b*obs + i*ors + j*ocs] = alpha * in[b*ibs + i*irs + j*ics]
+  const double* in;
+  double* out;
+  int64_t irs, ics, ibs, ors, ocs, obs;
+  int32_t M, N, nb;
+  double alpha;
+  int32_t tri;  // 0: full copy; 1: keep i <= j (upper), zero below; 2: keep i >= j (lower), zero above
+};
+inline Copy2D copy_job(const double* in, int64_t irs, int64_t ics, double* out, int64_t ors, int64_t ocs, int M, int N,
+                       double alpha = 1.0, int tri = 0) {
+  Copy2D c{};
+  c.in = in; c.out = out; c.irs = irs; c.ics = ics; c.ors = ors; c.ocs = ocs;
+  c.ibs = 0; c.obs = 0; c.M = M; c.N = N; c.nb = 1; c.alpha = alpha; c.tri = tri;
+  return c;
+}
+void copy2d_batched(Ctx& ctx, const std::vector<Copy2D>& jobs);
+
+struct Fill2D {  // out[i + j*ld] = (i == j) ? diag : 0   for i < M, j < N
+  double* out;
+  int64_t ld;
+  int32_t M, N;
+  double diag;
+};
+void fill2d_batched(Ctx& ctx, const std::vector<Fill2D>& jobs);
+
+// gathers columns: out[:, c] = in[:, idx[c]] (M rows, ncol columns, optional scale per column)
+struct GatherCols {
+  const double* in;
+  double* out;
+  int64_t ldi, ldo;
+  const int32_t* idx;  // device
+  int32_t M, ncol;
+};
+void gather_cols_batched(Ctx& ctx, const std::vector<GatherCols>& jobs);
+
+// power-of-two normalisation of a matrix in place (storage precision prec):
+// e = exponent of max |x|; x *= 2^-e; exp_out = exp_in + e
+struct Pow2Job {
+  void* X;
+  int64_t ld;
+  int32_t M, N;
+  const int32_t* exp_in;  // may be null (treated as 0)
+  int32_t* exp_out;
+};
+void pow2_normalize_batched(Ctx& ctx, const std::vector<Pow2Job>& jobs, Prec prec);
+
+// sketch draws: Philox4x32-10 + Box-Muller (FP64), see rng_normal in kernels.hip
+struct SketchGen {
+  void* out;          // destination (precision prec), layout (n, bvr, bvl, p) with n fastest
+  double* ref_out;    // optional FP64 copy in the reference order (bvr*n, bvl, p) (before scaling)
+  int32_t n, bvr, bvl, p;  // bvr = left rank of the block (bv_{k+1}), bvl = right rank (bv_k)
+  int32_t core;       // core index k (0-based), part of the counter
+  double scale;
+};
+void sketch_generate(Ctx& ctx, const std::vector<SketchGen>& jobs, uint64_t seed, Prec prec);
+// raw draws in reference order for a given core (used by tests)
+void sketch_draws(Ctx& ctx, double* out, int64_t count, int core, uint64_t seed);
+// N(0,1) TT core (rand_tt semantics): draw e in reference order (i + n*(a + r*b)) of
+// core k goes to device position a + r*(i + n*b); counter stream distinct from sketches
+void randn_core(Ctx& ctx, double* out, int n, int r, int r2, int core, uint64_t seed);
+// converts FP64 blocks in reference order (bvr*n, bvl, p) to the device layout
+void sketch_pack(Ctx& ctx, const double* ref, void* out, int n, int bvr, int bvl, int p, double scale,
+                 Prec prec);
+
+// sum of nterm equally shaped arrays: out[e] = sum_t in_t[e]
+void sum_arrays(Ctx& ctx, double* out, const std::vector<const double*>& ins, int64_t count);
+
+// block-diagonal placement of a core into a larger core (TT sum)
+struct BlockPut {
+  const double* in;  // (ra, n, rb) device layout
+  double* out;       // (RA, n, RB) device layout
+  int32_t ra, rb, n, RA, RB, a0, b0;
+  double scale;
+};
+void block_put_batched(Ctx& ctx, const std::vector<BlockPut>& jobs);
+
+// MPO x TT core: y[(al,nu), i, (al',nu')] = sum_j A[i, j, al, al'] x[nu, j, nu']
+struct MpoCore {
+  const double* A;  // (n, n, Ra, Ra') column-major (reference layout)
+  const double* x;  // (rx, n, rx') device layout
+  double* y;        // (Ra*rx, n, Ra'*rx') device layout
+  int32_t n, Ra, Ra2, rx, rx2;
+};
+void mpo_apply_batched(Ctx& ctx, const std::vector<MpoCore>& jobs);
+
+void zero_fill(Ctx& ctx, void* p, size_t bytes);
+
+}  // namespace ttr

@@ -1,0 +1,15 @@
+"""ttround: MI355X (gfx950) TT-rounding library, Python host mirror.
+
+The numerical work runs in ``libttround.so`` (hand-written HIP kernels behind
+the C-ABI of ``include/ttround.h``).  This package mirrors the TensorTrains
+Julia API for the rounding path; see ``ttround.api``.
+"""
+from ._lib import (DimensionMismatch, NumericalFailure, DeviceError, Handle, default_handle, load,  # noqa: F401
+                   LIB_PATH, SIGNATURES)
+from .api import *  # noqa: F401,F403
+from .api import (DeviceTT, DeviceTTO, TTvector, TToperator, default_rank_heuristic, lincomb, lincomb_device,  # noqa: F401
+                  mpo_apply_device, norm, norm_batch, orthogonalize, orthogonalize_batch, r_and_d_to_rks, rand_tt,
+                  round_rand_batch, round_svd_batch, sketch_blocks, sketch_draws, tt_recursive_sketch, tt_rounding,
+                  tt_rounding_, tt_scale, ttrand_rounding, zeros_tt, is_leftorthogonal, is_rightorthogonal)
+
+__version__ = "0.1.0"

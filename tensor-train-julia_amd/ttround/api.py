@@ -1,0 +1,478 @@
+"""Host mirror of the TensorTrains rounding API over libttround.
+
+Names, argument meanings, defaults and error behaviour follow the Julia
+package (``src/tt_rounding.jl``, ``src/tt_randtools.jl``, ``src/sketches.jl``,
+``src/tt_operations.jl``, ``src/tt_tools.jl``); every numerical operation runs
+in the HIP library.  Two object kinds exist:
+
+* ``TTvector`` / ``TToperator``: host objects with the reference's fields
+  (``ttv_vec`` cores of shape ``(n_k, r_{k-1}, r_k)``, ``ttv_dims``,
+  ``ttv_rks``, ``ttv_ot``); each call uploads, computes on the GPU and
+  downloads the result, like the reference's value semantics.
+* ``DeviceTT``: a TT resident in device memory, for batched pipelines that keep
+  data in HBM between calls (``round_rand_batch``, ``round_svd_batch``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+# --------------------------------------------------------------------- helpers
+
+def _wrapped_prod(vals) -> int:
+    p = 1
+    for v in vals:
+        p = (p * int(v)) & 0xFFFFFFFFFFFFFFFF
+    return p - (1 << 64) if p >= (1 << 63) else p
+
+
+def r_and_d_to_rks(rks, dims, rmax: int = 1024) -> List[int]:
+    """``r_and_d_to_rks`` (``src/tt_rounding.jl:6-24``), integer bookkeeping on the host."""
+    d = len(dims)
+    out = [1] * len(rks)
+    for i in range(d):
+        right, left = _wrapped_prod(dims[i:]), _wrapped_prod(dims[:i])
+        v = min(int(rks[i]), int(rmax))
+        if right > 0:
+            v = min(v, right)
+        if left > 0:
+            v = min(v, left)
+        out[i] = v
+    return out
+
+
+def _opts(orthogonal=True, sketch_prec="f64", block_rks=None, seed=1234, p=None, oversampling=1.0):
+    o = L.RandOpts()
+    o.orthogonal = 1 if orthogonal else 0
+    o.sketch_prec = L.PREC[sketch_prec] if isinstance(sketch_prec, str) else int(sketch_prec)
+    o.block_rks = int(block_rks) if block_rks else 0
+    o.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+    keep = []
+    if p is not None:
+        pa = L.as_i64(p)
+        keep.append(pa)
+        o.p = L.ptr_i64(pa)
+    else:
+        o.p = C.cast(None, L.i64p)
+    o.oversampling = float(oversampling)
+    return o, keep
+
+
+# --------------------------------------------------------------------- device objects
+
+class DeviceTT:
+    """A TT held in device memory by a libttround handle."""
+
+    def __init__(self, handle: L.Handle, tid: int):
+        self.handle = handle
+        self.id = int(tid)
+        d = C.c_int64()
+        handle.check(handle.lib.ttr_tt_info(handle.h, self.id, C.byref(d), None, None, None))
+        self.d = d.value
+        dims = np.zeros(self.d, np.int64)
+        rks = np.zeros(self.d + 1, np.int64)
+        ot = np.zeros(self.d, np.int64)
+        handle.check(handle.lib.ttr_tt_info(handle.h, self.id, None, L.ptr_i64(dims), L.ptr_i64(rks), L.ptr_i64(ot)))
+        self.dims = tuple(int(x) for x in dims)
+        self.rks = [int(x) for x in rks]
+        self.ot = [int(x) for x in ot]
+
+    # construction
+    @classmethod
+    def upload(cls, x: "TTvector", handle: Optional[L.Handle] = None) -> "DeviceTT":
+        h = handle or L.default_handle()
+        d = x.N
+        cores = [np.asfortranarray(np.asarray(c, dtype=np.float64)) for c in x.ttv_vec]
+        for k, c in enumerate(cores):
+            exp = (x.ttv_dims[k], x.ttv_rks[k], x.ttv_rks[k + 1])
+            if c.shape != exp:
+                raise L.DimensionMismatch(f"core {k} has shape {c.shape}, expected {exp}")
+        arr = (L.f64p * d)(*[L.ptr_f64(c) for c in cores])
+        dims, rks, ot = L.as_i64(x.ttv_dims), L.as_i64(x.ttv_rks), L.as_i64(x.ttv_ot)
+        tid = C.c_int64()
+        h.check(h.lib.ttr_tt_upload(h.h, d, L.ptr_i64(dims), L.ptr_i64(rks), L.ptr_i64(ot), arr, C.byref(tid)))
+        return cls(h, tid.value)
+
+    @classmethod
+    def randn(cls, dims, rks, seed: int, handle: Optional[L.Handle] = None) -> "DeviceTT":
+        """Seeded N(0,1) TT drawn on the device (``rand_tt`` semantics, ``src/tt_tools.jl:255-303``)."""
+        h = handle or L.default_handle()
+        d = len(dims)
+        if isinstance(rks, (int, np.integer)):
+            rks = [int(rks)] * (d + 1)
+        da, ra = L.as_i64(dims), L.as_i64(rks)
+        tid = C.c_int64()
+        h.check(h.lib.ttr_tt_randn(h.h, d, L.ptr_i64(da), L.ptr_i64(ra), int(seed) & 0xFFFFFFFFFFFFFFFF, C.byref(tid)))
+        return cls(h, tid.value)
+
+    def core(self, k: int) -> np.ndarray:
+        shape = (self.dims[k], self.rks[k], self.rks[k + 1])
+        out = np.empty(shape, dtype=np.float64, order="F")
+        self.handle.check(self.handle.lib.ttr_tt_download_core(self.handle.h, self.id, int(k), L.ptr_f64(out)))
+        return out
+
+    def to_host(self) -> "TTvector":
+        return TTvector(self.d, [self.core(k) for k in range(self.d)], self.dims, list(self.rks), list(self.ot))
+
+    def free(self):
+        if self.id is not None and self.handle.h:
+            self.handle.check(self.handle.lib.ttr_tt_free(self.handle.h, self.id))
+        self.id = None
+
+    def norm(self) -> float:
+        out = np.zeros(1)
+        ids = L.as_i64([self.id])
+        self.handle.check(self.handle.lib.ttr_norm(self.handle.h, 1, L.ptr_i64(ids), L.ptr_f64(out)))
+        return float(out[0])
+
+
+class DeviceTTO:
+    """A TT operator held in device memory."""
+
+    def __init__(self, A: "TToperator", handle: Optional[L.Handle] = None):
+        h = handle or L.default_handle()
+        self.handle = h
+        cores = [np.asfortranarray(np.asarray(c, dtype=np.float64)) for c in A.tto_vec]
+        arr = (L.f64p * A.N)(*[L.ptr_f64(c) for c in cores])
+        dims, rks = L.as_i64(A.tto_dims), L.as_i64(A.tto_rks)
+        tid = C.c_int64()
+        h.check(h.lib.ttr_tto_upload(h.h, A.N, L.ptr_i64(dims), L.ptr_i64(rks), arr, C.byref(tid)))
+        self.id = tid.value
+        self.dims = tuple(A.tto_dims)
+        self.rks = list(A.tto_rks)
+
+
+def lincomb_device(coeffs: Sequence[float], xs: Sequence[DeviceTT]) -> DeviceTT:
+    h = xs[0].handle
+    a = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float64))
+    ids = L.as_i64([x.id for x in xs])
+    out = C.c_int64()
+    h.check(h.lib.ttr_lincomb(h.h, len(xs), L.ptr_f64(a), L.ptr_i64(ids), C.byref(out)))
+    return DeviceTT(h, out.value)
+
+
+def mpo_apply_device(A: DeviceTTO, x: DeviceTT) -> DeviceTT:
+    h = x.handle
+    out = C.c_int64()
+    h.check(h.lib.ttr_mpo_apply(h.h, A.id, x.id, C.byref(out)))
+    return DeviceTT(h, out.value)
+
+
+def orthogonalize_batch(xs: Sequence[DeviceTT], i: int = 1) -> List[DeviceTT]:
+    h = xs[0].handle
+    ids = L.as_i64([x.id for x in xs])
+    out = np.zeros(len(xs), np.int64)
+    h.check(h.lib.ttr_orthogonalize(h.h, len(xs), L.ptr_i64(ids), int(i), L.ptr_i64(out)))
+    return [DeviceTT(h, t) for t in out]
+
+
+def round_svd_batch(xs: Sequence[DeviceTT], tol: float = 1e-12, rmax: int = 2 ** 14,
+                    direction: str = "left") -> List[DeviceTT]:
+    h = xs[0].handle
+    if direction not in L.DIRECTION:
+        raise ValueError("Invalid direction. Use direction=:right or direction=:left")
+    ids = L.as_i64([x.id for x in xs])
+    out = np.zeros(len(xs), np.int64)
+    h.check(h.lib.ttr_round_svd(h.h, len(xs), L.ptr_i64(ids), float(tol), int(rmax), L.DIRECTION[direction],
+                                L.ptr_i64(out)))
+    return [DeviceTT(h, t) for t in out]
+
+
+def norm_batch(xs: Sequence[DeviceTT]) -> np.ndarray:
+    h = xs[0].handle
+    ids = L.as_i64([x.id for x in xs])
+    out = np.zeros(len(xs))
+    h.check(h.lib.ttr_norm(h.h, len(xs), L.ptr_i64(ids), L.ptr_f64(out)))
+    return out
+
+
+def round_rand_batch(items, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
+                     sketch_prec: str = "f64", p=None, oversampling: float = 1.0) -> List[DeviceTT]:
+    """Batched randomized rounding of linear combinations.
+
+    ``items`` is a list of ``(alphas, [DeviceTT...], rks)`` triples."""
+    h = items[0][1][0].handle
+    keep = []
+    arr = (L.RandItem * len(items))()
+    for i, (alpha, terms, rks) in enumerate(items):
+        a = np.ascontiguousarray(np.asarray(alpha, dtype=np.float64))
+        ids = L.as_i64([t.id for t in terms])
+        r = L.as_i64(rks)
+        keep += [a, ids, r]
+        arr[i].nterms = len(terms)
+        arr[i].alpha = L.ptr_f64(a)
+        arr[i].term_ids = L.ptr_i64(ids)
+        arr[i].op_id = -1
+        arr[i].rks = L.ptr_i64(r)
+    o, k2 = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling)
+    out = np.zeros(len(items), np.int64)
+    h.check(h.lib.ttr_round_rand(h.h, len(items), arr, C.byref(o), L.ptr_i64(out)))
+    return [DeviceTT(h, t) for t in out]
+
+
+def block_structure(dims, rks, block_rks: Optional[int] = None, p=None, oversampling: float = 1.0):
+    """Block ranks ``bv`` and block counts ``p`` of the reverse sketch
+    (``src/sketches.jl:107-123,186-198``)."""
+    d = len(dims)
+    bv = [1] * (d + 1)
+    for k in range(d):
+        bv[k] = int(block_rks) if block_rks else d
+    for k in range(d - 1, -1, -1):
+        bv[k] = min(bv[k], int(dims[k]) * bv[k + 1])
+    if p is None:
+        pp = [int(math.ceil(int(r) / b)) for r, b in zip(rks, bv)]
+        for k in range(1, d):
+            pp[k] = max(pp[k], pp[k - 1])
+        pp[d] = 1
+    else:
+        pp = [int(v) for v in p]
+    if oversampling != 1:
+        for k in range(d):
+            pp[k] = int(math.ceil(oversampling * pp[k]))
+    return bv, pp
+
+
+def sketch_blocks(dims, rks, k: int, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
+                  sketch_prec: str = "f64", p=None, oversampling: float = 1.0,
+                  handle: Optional[L.Handle] = None) -> np.ndarray:
+    """The sketch blocks of core ``k`` (0-based) as the device uses them, as the
+    ``(bv_{k+1}, n_k, bv_k, p_k)`` array of ``generate_sketch_blocks``."""
+    h = handle or L.default_handle()
+    d = len(dims)
+    o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling)
+    bv, pp = block_structure(dims, rks, block_rks, p, oversampling)
+    buf = np.zeros(int(bv[k + 1] * dims[k] * bv[k] * pp[k]))
+    shape = np.zeros(4, np.int64)
+    da, ra = L.as_i64(dims), L.as_i64(rks)
+    h.check(h.lib.ttr_sketch_blocks(h.h, d, L.ptr_i64(da), L.ptr_i64(ra), C.byref(o), int(k), L.ptr_f64(buf),
+                                    L.ptr_i64(shape)))
+    return np.reshape(buf, tuple(int(s) for s in shape), order="F").copy()
+
+
+def sketch_draws(k: int, count: int, seed: int = 1234, handle: Optional[L.Handle] = None) -> np.ndarray:
+    h = handle or L.default_handle()
+    out = np.zeros(int(count))
+    h.check(h.lib.ttr_sketch_draws(h.h, int(k), int(count), int(seed) & 0xFFFFFFFFFFFFFFFF, L.ptr_f64(out)))
+    return out
+
+
+# --------------------------------------------------------------------- host objects
+
+class TTvector:
+    """``TTvector{Float64,N}`` (``src/tt_tools.jl:24-34``)."""
+
+    def __init__(self, N, ttv_vec, ttv_dims, ttv_rks, ttv_ot=None):
+        self.N = int(N)
+        self.ttv_vec = [np.asarray(c, dtype=np.float64) for c in ttv_vec]
+        self.ttv_dims = tuple(int(x) for x in ttv_dims)
+        self.ttv_rks = [int(x) for x in ttv_rks]
+        self.ttv_ot = [int(x) for x in (ttv_ot if ttv_ot is not None else [0] * self.N)]
+
+    def copy(self) -> "TTvector":
+        return TTvector(self.N, [c.copy() for c in self.ttv_vec], self.ttv_dims, list(self.ttv_rks), list(self.ttv_ot))
+
+    def _dev(self) -> DeviceTT:
+        return DeviceTT.upload(self)
+
+    def __add__(self, other: "TTvector") -> "TTvector":
+        return _host_lincomb([1.0, 1.0], [self, other], plain_sum=True)
+
+    def __sub__(self, other: "TTvector") -> "TTvector":
+        """``A - B = (-1.0)*B + A`` (``src/tt_operations.jl:453-455``)."""
+        return tt_scale(-1.0, other) + self
+
+    def __rmul__(self, a) -> "TTvector":
+        return tt_scale(a, self)
+
+    def __truediv__(self, a) -> "TTvector":
+        return tt_scale(1.0 / a, self)
+
+
+class TToperator:
+    """``TToperator{Float64,N}`` (``src/tt_tools.jl:61-71``)."""
+
+    def __init__(self, N, tto_vec, tto_dims, tto_rks, tto_ot=None):
+        self.N = int(N)
+        self.tto_vec = [np.asarray(c, dtype=np.float64) for c in tto_vec]
+        self.tto_dims = tuple(int(x) for x in tto_dims)
+        self.tto_rks = [int(x) for x in tto_rks]
+        self.tto_ot = [int(x) for x in (tto_ot if tto_ot is not None else [0] * self.N)]
+
+    def __mul__(self, x: TTvector) -> TTvector:
+        """``A*x`` (``src/tt_operations.jl:170-194``)."""
+        dev = mpo_apply_device(DeviceTTO(self), x._dev())
+        out = dev.to_host()
+        dev.free()
+        return out
+
+
+def zeros_tt(dims, rks, ot=None) -> TTvector:
+    d = len(dims)
+    return TTvector(d, [np.zeros((dims[i], rks[i], rks[i + 1])) for i in range(d)], dims, rks, ot)
+
+
+def rand_tt(dims, rks, seed: int) -> TTvector:
+    """Seeded N(0,1) TT (drawn on the device with the library's Philox stream)."""
+    dev = DeviceTT.randn(dims, rks, seed)
+    out = dev.to_host()
+    dev.free()
+    return out
+
+
+def tt_scale(a, x: TTvector) -> TTvector:
+    """``*(a::Number, A::TTvector)`` (``src/tt_operations.jl:428-441``): scales the
+    first core with ``ot == 0`` (host arithmetic on one core; no GPU work)."""
+    if a == 0:
+        return zeros_tt(x.ttv_dims, [1] * (x.N + 1))
+    if 0 not in x.ttv_ot:
+        raise ValueError("no core with ot == 0 to scale")
+    i = x.ttv_ot.index(0)
+    vec = list(x.ttv_vec)
+    vec[i] = a * vec[i]
+    return TTvector(x.N, vec, x.ttv_dims, list(x.ttv_rks), list(x.ttv_ot))
+
+
+def _host_lincomb(coeffs, xs, plain_sum=False) -> TTvector:
+    devs = [x._dev() for x in xs]
+    out = lincomb_device(coeffs, devs)
+    res = out.to_host()
+    for d in devs + [out]:
+        d.free()
+    return res
+
+
+def lincomb(A: Sequence[TTvector], x: Sequence[float]) -> TTvector:
+    """``*(A::Vector{TTvector}, x::Vector)`` (``src/tt_operations.jl:252-258``)."""
+    return _host_lincomb(list(x), list(A))
+
+
+def is_leftorthogonal(x) -> bool:
+    return all(o == 1 for o in x.ttv_ot[: x.N - 1])
+
+
+def is_rightorthogonal(x) -> bool:
+    return all(o == -1 for o in x.ttv_ot[1:])
+
+
+def orthogonalize(x: TTvector, i: int = 1) -> TTvector:
+    """``orthogonalize(x; i=1)`` (``src/tt_rounding.jl:65-97``)."""
+    dev = x._dev()
+    out = orthogonalize_batch([dev], i)[0]
+    res = out.to_host()
+    dev.free()
+    out.free()
+    return res
+
+
+def norm(x: TTvector) -> float:
+    """``norm(::TTvector)`` (``src/tt_rounding.jl:107-114``)."""
+    dev = x._dev()
+    v = dev.norm()
+    dev.free()
+    return v
+
+
+def tt_rounding(x, tol: float = 1e-12, rmax: int = 2 ** 14, direction: str = "left"):
+    """``tt_rounding(x; tol, rmax, direction)`` (``src/tt_rounding.jl:146-154``);
+    for a ``TToperator`` (``:275-277``) the operator is rounded as the TT of its
+    ``(n^2, R, R')`` cores."""
+    if isinstance(x, TToperator):
+        v = TTvector(x.N, [np.reshape(c, (c.shape[0] * c.shape[1], c.shape[2], c.shape[3]), order="F") for c in x.tto_vec],
+                     tuple(n * n for n in x.tto_dims), x.tto_rks, x.tto_ot)
+        r = tt_rounding(v, tol=tol, rmax=rmax, direction=direction)
+        return TToperator(r.N, [np.reshape(c, (n, n, c.shape[1], c.shape[2]), order="F") for c, n in zip(r.ttv_vec, x.tto_dims)],
+                          x.tto_dims, r.ttv_rks, r.ttv_ot)
+    dev = x._dev()
+    out = round_svd_batch([dev], tol, rmax, direction)[0]
+    res = out.to_host()
+    dev.free()
+    out.free()
+    return res
+
+
+def tt_rounding_(x: TTvector, tol: float = 1e-12, rmax: int = 2 ** 14, direction: str = "left") -> None:
+    """``tt_rounding!``: replaces x's cores, ranks and orthogonality in place."""
+    r = tt_rounding(x, tol, rmax, direction)
+    x.ttv_vec, x.ttv_rks, x.ttv_ot = r.ttv_vec, r.ttv_rks, r.ttv_ot
+
+
+def default_rank_heuristic(y) -> List[int]:
+    """``default_rank_heuristic`` (``src/tt_randtools.jl:31-50``)."""
+    if isinstance(y, TTvector):
+        rks, N = list(y.ttv_rks), y.N
+    else:
+        rks, N = list(y[0].ttv_rks), y[0].N
+        for t in y[1:]:
+            rks = [max(a, b) for a, b in zip(rks, t.ttv_rks)]
+    ovs = int(math.ceil(0.5 * max(rks)))
+    for k in range(1, N):
+        rks[k] += ovs
+    return rks
+
+
+def _rks_arg(rks, N):
+    if isinstance(rks, (int, np.integer)):
+        r = [1] * (N + 1)
+        for k in range(1, N):
+            r[k] = int(rks)
+        return r
+    return [int(v) for v in rks]
+
+
+def ttrand_rounding(*args, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
+                    sketch_prec: str = "f64") -> TTvector:
+    """Randomized rounding (``src/tt_randtools.jl:112-167,223-283``).
+
+    Forms: ``ttrand_rounding(y)``, ``ttrand_rounding(y, rks_or_rmax)``,
+    ``ttrand_rounding(alpha, ys)``, ``ttrand_rounding(alpha, ys, rks_or_rmax)``.
+    The sketch draws come from the library's counter-based Philox stream keyed
+    by ``seed``; ``sketch_prec`` selects the precision of the sketch
+    contractions (the QR and projections are FP64)."""
+    if isinstance(args[0], TTvector):
+        y = args[0]
+        rks = _rks_arg(args[1], y.N) if len(args) > 1 else default_rank_heuristic(y)
+        alpha, ys = [1.0], [y]
+    else:
+        alpha, ys = list(args[0]), list(args[1])
+        if len(alpha) != len(ys) or any(t.ttv_dims != ys[0].ttv_dims for t in ys):
+            raise L.DimensionMismatch("length(alpha) must equal length(y) and all dims must agree")
+        rks = _rks_arg(args[2], ys[0].N) if len(args) > 2 else default_rank_heuristic(ys)
+    devs = [t._dev() for t in ys]
+    out = round_rand_batch([(alpha, devs, rks)], orthogonal=orthogonal, seed=seed, block_rks=block_rks,
+                           sketch_prec=sketch_prec)[0]
+    res = out.to_host()
+    for d in devs + [out]:
+        d.free()
+    return res
+
+
+def tt_recursive_sketch(A: TTvector, rks, orthogonal: bool = True, reverse: bool = True, seed: int = 1234,
+                        block_rks: Optional[int] = None, p=None, oversampling: float = 1.0,
+                        sketch_prec: str = "f64"):
+    """``tt_recursive_sketch(A, rks; reverse=true, ...)`` (``src/sketches.jl:175-315``).
+
+    Returns ``(W, sketch_rks)`` with ``W[k]`` of shape ``(r_k, sketch_rks[k])``."""
+    if not reverse:
+        raise NotImplementedError("only the reverse (right-to-left) sketch is provided")
+    if isinstance(rks, (int, np.integer)):
+        r = [int(rks)] * (A.N + 1)
+        r[A.N] = 1
+        rks = r
+    dev = A._dev()
+    h = dev.handle
+    o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling)
+    srk = np.zeros(A.N + 1, np.int64)
+    ra = L.as_i64(rks)
+    # first call: learn the sketch ranks
+    h.check(h.lib.ttr_sketch(h.h, dev.id, L.ptr_i64(ra), C.byref(o), C.cast(None, L.f64pp), L.ptr_i64(srk)))
+    Ws = [np.zeros((A.ttv_rks[k], int(srk[k])), order="F") for k in range(A.N + 1)]
+    arr = (L.f64p * (A.N + 1))(*[L.ptr_f64(w) for w in Ws])
+    h.check(h.lib.ttr_sketch(h.h, dev.id, L.ptr_i64(ra), C.byref(o), arr, L.ptr_i64(srk)))
+    dev.free()
+    return Ws, [int(s) for s in srk]

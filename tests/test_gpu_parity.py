@@ -1,0 +1,254 @@
+"""Parity of the HIP path (through the C-ABI) with the FP64 oracle.
+
+Tolerances: FP64 paths within 1e-12 relative (represented tensor, Frobenius
+norm); FP32 sketches 1e-6; BF16 sketches 1e-6 when the sketch rank covers the
+true rank (exact recovery), otherwise against the oracle run on the same
+rounded blocks with a looser bound stated in the test.  Integer outputs
+(ranks, orthogonality flags) are compared exactly.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import philox as P
+from helpers import dev_to_oracle, device_blocks_provider, rel_err_dense, to_api, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def T():
+    import ttround
+    ttround.default_handle()
+    return ttround
+
+
+def rand_tt(dims, rks, seed):
+    return O.rand_tt(dims, rks, np.random.default_rng(seed))
+
+
+def orth_ok(y, atol=1e-12):
+    for k in range(y.N):
+        c = y.cores[k]
+        if y.ot[k] == 1:
+            np.testing.assert_allclose(sum(c[i].T @ c[i] for i in range(c.shape[0])), np.eye(c.shape[2]), atol=atol)
+        elif y.ot[k] == -1:
+            np.testing.assert_allclose(sum(c[i] @ c[i].T for i in range(c.shape[0])), np.eye(c.shape[1]), atol=atol)
+
+
+# ------------------------------------------------------------------ data paths
+
+def test_upload_download_roundtrip(T):
+    say("case: upload/download d=5")
+    x = rand_tt((2, 3, 4, 3, 2), [1, 2, 5, 4, 3, 1], 1)
+    dx = T.DeviceTT.upload(to_api(x))
+    for k in range(x.N):
+        np.testing.assert_array_equal(dx.core(k), x.cores[k])
+    assert dx.rks == x.rks and dx.dims == x.dims
+    dx.free()
+
+
+def test_randn_matches_philox(T):
+    say("case: device randn vs numpy Philox")
+    dims, rks = (4, 4, 4, 4), [1, 4, 7, 4, 1]
+    dx = T.DeviceTT.randn(dims, [1, 7, 7, 7, 1], seed=99)
+    assert dx.rks == rks
+    ref = P.rand_tt_cores(dims, rks, 99)
+    for k in range(4):
+        np.testing.assert_allclose(dx.core(k), ref[k], rtol=1e-14, atol=1e-15)
+    dx.free()
+
+
+def test_sketch_draws_match_philox(T):
+    say("case: sketch draws")
+    g = T.sketch_draws(5, 20000, seed=1234)
+    np.testing.assert_allclose(g, P.normal_draws(1234, 5, 20000), rtol=1e-14, atol=1e-15)
+
+
+# ------------------------------------------------------------------ algebra
+
+def test_lincomb_exact(T):
+    say("case: lincomb")
+    dims = (3, 2, 4, 3)
+    xs = [rand_tt(dims, r, 10 + r) for r in (2, 3, 4)]
+    al = [0.5, -2.0, 3.0]
+    out = T.lincomb([to_api(x) for x in xs], al)
+    ref = O.tt_lincomb(xs, al)
+    assert out.ttv_rks == ref.rks and out.ttv_ot == ref.ot
+    for a, b in zip(out.ttv_vec, ref.cores):
+        np.testing.assert_array_equal(a, b)
+    s = to_api(xs[0]) - to_api(xs[1])
+    np.testing.assert_allclose(O.ttv_to_tensor(to_oracle(s)), O.ttv_to_tensor(O.tt_sub(xs[0], xs[1])), atol=1e-13)
+
+
+def test_mpo_apply(T):
+    say("case: mpo apply (Hubbard L=4)")
+    H = O.hubbard_1d_mpo(4, 1.0, 4.0)
+    y = rand_tt(H.dims, [1, 3, 5, 3, 1], 3)
+    A = T.TToperator(H.N, H.cores, H.dims, H.rks)
+    out = A * to_api(y)
+    ref = O.mpo_apply(H, y)
+    assert out.ttv_rks == ref.rks
+    for a, b in zip(out.ttv_vec, ref.cores):
+        np.testing.assert_allclose(a, b, rtol=1e-14, atol=1e-14)
+
+
+# ------------------------------------------------------------------ deterministic
+
+@pytest.mark.parametrize("root", [1, 4, 7])
+def test_orthogonalize(T, root):
+    say(f"case: orthogonalize root={root}")
+    dims = (2, 3, 2, 4, 2, 3, 2)
+    x = rand_tt(dims, [1, 2, 6, 9, 12, 6, 2, 1], 20 + root)
+    y = to_oracle(T.orthogonalize(to_api(x), i=root))
+    ref = O.orthogonalize(x, i=root)
+    assert y.rks == ref.rks and y.ot == ref.ot
+    assert rel_err_dense(x, y) < 1e-13
+    orth_ok(y)
+    # LAPACK sign conventions: the cores agree with the oracle's, not just the tensor
+    for a, b in zip(y.cores, ref.cores):
+        np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-11)
+
+
+def test_norm(T):
+    say("case: norm")
+    x = rand_tt((3, 4, 3, 4, 3), [1, 3, 8, 8, 3, 1], 5)
+    assert abs(T.norm(to_api(x)) - O.tt_norm(x)) < 1e-12 * O.tt_norm(x)
+
+
+@pytest.mark.parametrize("tol,rmax,direction", [(1e-12, 2 ** 14, "left"), (1e-3, 2 ** 14, "left"), (1e-12, 4, "left"),
+                                                (1e-12, 2 ** 14, "right"), (1e-3, 5, "right")])
+def test_tt_rounding(T, tol, rmax, direction):
+    say(f"case: tt_rounding tol={tol} rmax={rmax} {direction}")
+    dims = (2, 3, 4, 3, 2, 3)
+    x = O.tt_add(rand_tt(dims, 4, 30), O.tt_scale(1e-3, rand_tt(dims, 5, 31)))
+    y = to_oracle(T.tt_rounding(to_api(x), tol=tol, rmax=rmax, direction=direction))
+    ref = O.tt_rounding(x, tol=tol, rmax=rmax, direction=direction)
+    assert y.rks == ref.rks and y.ot == ref.ot
+    assert O.rel_err(ref, y) < 1e-12
+    orth_ok(y)
+
+
+def test_tt_rounding_closed_form(T):
+    say("case: round(y + y + 1e-10 z, 1e-8) = 2y")
+    dims = (4,) * 8
+    yv, z = rand_tt(dims, 6, 40), rand_tt(dims, 6, 41)
+    x = O.tt_add(O.tt_add(yv, yv), O.tt_scale(1e-10, z))
+    r = to_oracle(T.tt_rounding(to_api(x), tol=1e-8))
+    assert r.rks == yv.rks
+    assert O.rel_err(O.tt_scale(2.0, yv), r) < 1e-9
+
+
+def test_tt_rounding_batch_uneven(T):
+    say("case: round_svd_batch with uneven ranks")
+    dims = (4,) * 6
+    xs = [O.tt_add(rand_tt(dims, r, 50 + r), rand_tt(dims, r + 2, 60 + r)) for r in (2, 5, 9)]
+    devs = [T.DeviceTT.upload(to_api(x)) for x in xs]
+    outs = T.round_svd_batch(devs, tol=1e-10)
+    for x, o in zip(xs, outs):
+        y = dev_to_oracle(o)
+        ref = O.tt_rounding(x, tol=1e-10)
+        assert y.rks == ref.rks
+        assert O.rel_err(ref, y) < 1e-12
+    for d in devs + outs:
+        d.free()
+
+
+def test_tt_rounding_errors(T):
+    say("case: rounding error paths")
+    x = rand_tt((2, 2, 2), 2, 1)
+    with pytest.raises(ValueError):
+        T.tt_rounding(to_api(x), direction="up")
+
+
+# ------------------------------------------------------------------ sketch
+
+@pytest.mark.parametrize("orthogonal", [False, True])
+def test_sketch_blocks_match_generate(T, orthogonal):
+    say(f"case: sketch blocks orthogonal={orthogonal}")
+    dims = (3, 2, 4, 2, 3)
+    rks = [1, 7, 9, 9, 5, 1]
+    bv, p = O.sketch_structure(dims, rks, 4)
+    for k in range(len(dims)):
+        S = T.sketch_blocks(dims, rks, k, orthogonal=orthogonal, seed=77, block_rks=4)
+        g = P.normal_draws(77, k, bv[k + 1] * dims[k] * bv[k] * p[k])
+        ref = O.generate_sketch_blocks(g, bv[k + 1], dims[k], bv[k], p[k], orthogonal)
+        np.testing.assert_allclose(S, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6), ("bf16", 3e-2)])
+@pytest.mark.parametrize("orthogonal,block_rks", [(False, None), (True, 3)])
+def test_recursive_sketch(T, prec, tol, orthogonal, block_rks):
+    say(f"case: recursive sketch {prec} orthogonal={orthogonal} block_rks={block_rks}")
+    dims = (2, 3, 2, 4, 2, 3)
+    x = rand_tt(dims, [1, 2, 6, 8, 8, 3, 1], 70)
+    rks = [1, 5, 7, 7, 6, 3, 1]
+    W, s = T.tt_recursive_sketch(to_api(x), rks, orthogonal=orthogonal, seed=5, block_rks=block_rks, sketch_prec=prec)
+    prov = device_blocks_provider(dims, rks, orthogonal=orthogonal, seed=5, block_rks=block_rks, sketch_prec=prec)
+    Wr, sr = O.tt_recursive_sketch(x, rks, prov, orthogonal, block_rks)
+    assert s == sr
+    for a, b in zip(W, Wr):
+        assert a.shape == b.shape
+        assert np.linalg.norm(a - b) <= tol * np.linalg.norm(b)
+
+
+# ------------------------------------------------------------------ randomized rounding
+
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("orthogonal", [True, False])
+def test_ttrand_rounding_exact_recovery(T, prec, orthogonal):
+    say(f"case: ttrand exact recovery {prec} orthogonal={orthogonal}")
+    # test_ttrand.jl:67-88 pattern
+    dims = (2,) * 8
+    a = rand_tt(dims, [1, 2, 4, 4, 4, 4, 4, 2, 1], 80)
+    pert = O.tt_add(a, O.tt_scale(1e-8, rand_tt(dims, [1, 2, 3, 3, 3, 3, 3, 2, 1], 81)))
+    rks = [1, 2, 4, 8, 16, 8, 4, 2, 1]
+    y = to_oracle(T.ttrand_rounding(to_api(pert), rks, orthogonal=orthogonal, sketch_prec=prec))
+    assert y.ot == [1] * 7 + [0]
+    assert O.rel_err(pert, y) < 1e-11
+    orth_ok(y, atol=1e-11)
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5), ("bf16", 5e-2)])
+def test_ttrand_rounding_vs_oracle_truncating(T, prec, tol):
+    say(f"case: ttrand truncating vs oracle {prec}")
+    dims = (3,) * 7
+    ys = [rand_tt(dims, 6, 90 + j) for j in range(3)]
+    al = [1.0, 0.25, 0.0625]
+    rks = [1, 3, 6, 6, 6, 6, 3, 1]
+    y = to_oracle(T.ttrand_rounding(al, [to_api(t) for t in ys], rks, orthogonal=False, seed=42, sketch_prec=prec))
+    prov = device_blocks_provider(dims, rks, orthogonal=False, seed=42, sketch_prec=prec)
+    ref = O.ttrand_rounding_lincomb(al, ys, rks, prov, False)
+    assert y.rks == ref.rks
+    say(f"  rel err vs oracle: {O.rel_err(ref, y):.3e}")
+    assert O.rel_err(ref, y) < tol
+
+
+def test_round_rand_batch_uneven(T):
+    say("case: round_rand_batch uneven items")
+    dims = (4,) * 6
+    items, refs = [], []
+    devs = []
+    for i, r in enumerate((3, 7, 11)):
+        ys = [rand_tt(dims, r + j, 100 + 10 * i + j) for j in range(2)]
+        al = [1.0, 0.5]
+        d = [T.DeviceTT.upload(to_api(t)) for t in ys]
+        devs += d
+        rks = O.default_rank_heuristic(ys)
+        items.append((al, d, rks))
+        prov = device_blocks_provider(dims, rks, orthogonal=False, seed=7, sketch_prec="f64")
+        refs.append(O.ttrand_rounding_lincomb(al, ys, rks, prov, False))
+    outs = T.round_rand_batch(items, orthogonal=False, seed=7, sketch_prec="f64")
+    for o, ref in zip(outs, refs):
+        y = dev_to_oracle(o)
+        assert y.rks == ref.rks
+        assert O.rel_err(ref, y) < 1e-12
+    for d in devs + outs:
+        d.free()

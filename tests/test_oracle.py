@@ -1,0 +1,353 @@
+"""Pins of the FP64 oracle (CPU only).
+
+The reference ships no golden vectors for the rounding path (its tests are
+tolerance invariants on unseeded random data), so every oracle function is
+pinned here by something the mathematics fixes, chosen so that a dropped term,
+a wrong index or a transposed operand fails:
+
+  r_and_d_to_rks, block structure, rank heuristic  -> hand-worked integer cases
+  floor_rule          -> closed-form singular-value vectors (tol 0, tails, degeneracy)
+  tt_add, mpo_apply   -> dense brute force (test_tt_tools.jl:16-46 pattern)
+  orthogonalize       -> dense reconstruction + per-core orthogonality (test_tt_tools.jl:48-57)
+  tt_rounding         -> dense TT-SVD sweep on unfoldings (Oseledets 2011) + closed forms
+  generate_sketch_blocks -> scaling constants, identity case, Q^T Q = c I and Q^T G upper triangular
+  tt_recursive_sketch -> dense X_{k:N} Omega_{k:N} brute force; p=1 full sketch = dot
+  ttrand_rounding*    -> exact recovery when the sketch rank >= true rank (test_ttrand.jl:67-88)
+  hubbard_1d_mpo      -> dense fermionic Hamiltonian
+  Philox              -> Random123 known-answer vectors
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import philox as P
+
+RNG = np.random.default_rng(20240601)
+
+
+def rand_tt(dims, rks, seed):
+    return O.rand_tt(dims, rks, np.random.default_rng(seed))
+
+
+# ---------------------------------------------------------------- integers
+
+def test_r_and_d_to_rks_hand_cases():
+    assert O.r_and_d_to_rks([1, 100, 100, 100, 100, 1], (4, 4, 4, 4, 4), rmax=100) == [1, 4, 16, 16, 4, 1]
+    assert O.r_and_d_to_rks([1, 5, 5, 1], (2, 3, 4)) == [1, 2, 4, 1]
+    # d=40, n=4: prod(dims[1:i-1]) overflows Int64 to 0 for i >= 33 and is then ignored
+    r = O.r_and_d_to_rks([256] * 41, (4,) * 40, rmax=256)
+    assert r[:5] == [1, 4, 16, 64, 256] and r[-5:] == [256, 64, 16, 4, 1]
+    assert O.tt_oracle._wrapped_prod([4] * 32) == 0 and O.tt_oracle._wrapped_prod([2] * 63) == -(1 << 63)
+
+
+def test_block_structure_c1():
+    # configs C1 with block_rks = 144: s = [144, ..., 144, 192, 144, 144, 1]
+    d = 32
+    rks = [1] + [144] * (d - 1) + [1]
+    bv, p = O.sketch_structure((4,) * d, rks, 144)
+    s = [b * q for b, q in zip(bv, p)]
+    assert s[1:29] == [144] * 28 and s[29:] == [192, 144, 144, 1]
+    assert bv[-5:] == [144, 64, 16, 4, 1] and p[-5:] == [1, 3, 9, 36, 1]
+
+
+def test_block_structure_default_and_monotone():
+    dims = (2,) * 6
+    bv, p = O.sketch_structure(dims, [1, 2, 4, 6, 4, 2, 1], None)
+    assert bv == [6, 6, 6, 6, 4, 2, 1]
+    assert p == [1, 1, 1, 1, 1, 1, 1]
+    bv, p = O.sketch_structure(dims, [1, 9, 9, 9, 9, 9, 1], 2)
+    assert bv == [2, 2, 2, 2, 2, 2, 1] and p == [1, 5, 5, 5, 5, 5, 1]
+    bv, p = O.sketch_structure(dims, [1, 9, 9, 9, 9, 9, 1], 2, oversampling=1.5)
+    assert p == [2, 8, 8, 8, 8, 8, 1]
+
+
+def test_default_rank_heuristic():
+    a = O.zeros_tt((3, 3, 3, 3), [1, 3, 5, 4, 1])
+    b = O.zeros_tt((3, 3, 3, 3), [1, 4, 6, 3, 1])
+    assert O.default_rank_heuristic(a) == [1, 6, 8, 7, 1]
+    assert O.default_rank_heuristic([a, b]) == [1, 7, 9, 7, 1]
+
+
+# ---------------------------------------------------------------- floor
+
+def test_floor_rule_closed_forms():
+    s = np.array([4.0, 3.0, 2.0, 1.0])
+    assert O.floor_rule(s, 0.0)[1] == 4
+    # tail energy: drop s4 (1 < tol^2*30 = 1.2) but not s3 (4+1 = 5)
+    assert O.floor_rule(s, 0.2)[1] == 3
+    assert O.floor_rule(s, math.sqrt(5.0 / 30.0) + 1e-12)[1] == 2
+    assert O.floor_rule(s, 1e-12)[1] == 4
+    # degeneracy extension with atol 1e-5 on s[k+1]/s[k]
+    # (tol^2 = 1.44e-6 lies between s3^2 + s4^2 and s2^2 + s3^2 + s4^2, so the cut is k = 2)
+    s = np.array([1.0, 1e-3, 1e-3 * (1 - 5e-6), 1e-9])
+    assert O.floor_rule(s, 1.2e-3)[1] == 3
+    s = np.array([1.0, 1e-3, 1e-3 * (1 - 2e-5), 1e-9])
+    assert O.floor_rule(s, 1.2e-3)[1] == 2
+    s = np.array([1.0, 0.0, 0.0])
+    assert O.floor_rule(s, 1e-3)[1] == 1
+
+
+# ---------------------------------------------------------------- algebra
+
+def test_add_scale_lincomb_dense():
+    dims = (3, 2, 4, 3)
+    x, y, z = rand_tt(dims, 3, 1), rand_tt(dims, 2, 2), rand_tt(dims, 4, 3)
+    X, Y, Z = (O.ttv_to_tensor(t) for t in (x, y, z))
+    np.testing.assert_allclose(O.ttv_to_tensor(O.tt_add(x, y)), X + Y, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(O.ttv_to_tensor(O.tt_sub(x, y)), X - Y, rtol=1e-13, atol=1e-13)
+    lc = O.tt_lincomb([x, y, z], [0.5, -2.0, 3.0])
+    np.testing.assert_allclose(O.ttv_to_tensor(lc), 0.5 * X - 2 * Y + 3 * Z, rtol=1e-12, atol=1e-12)
+    assert lc.rks == [1, 3 + 2 + 3, 9, 9, 1] or lc.rks[1] == x.rks[1] + y.rks[1] + z.rks[1]
+    assert O.tt_scale(0.0, x).rks == [1] * 5
+
+
+def test_mpo_apply_dense():
+    dims = (2, 3, 2)
+    A = O.TTO([RNG.standard_normal((n, n, ra, rb)) for n, ra, rb in zip(dims, [1, 2, 3], [2, 3, 1])], dims, [1, 2, 3, 1])
+    x = rand_tt(dims, [1, 2, 2, 1], 4)
+    Ad = O.tto_to_tensor(A)
+    xd = O.ttv_to_tensor(x).reshape(-1, order="F")
+    y = O.mpo_apply(A, x)
+    assert y.rks == [1, 4, 6, 1]
+    np.testing.assert_allclose(O.ttv_to_tensor(y).reshape(-1, order="F"), Ad @ xd, rtol=1e-12, atol=1e-12)
+
+
+def test_hubbard_mpo_dense():
+    for L in (2, 3, 4):
+        H = O.hubbard_1d_mpo(L, 1.0, 4.0)
+        assert H.rks == [1] + [6] * (L - 1) + [1]
+        np.testing.assert_allclose(O.tto_to_tensor(H), O.hubbard_1d_dense(L, 1.0, 4.0), atol=1e-13)
+
+
+# ---------------------------------------------------------------- deterministic rounding
+
+@pytest.mark.parametrize("root", [1, 3, 6])
+def test_orthogonalize(root):
+    dims = (2, 3, 2, 2, 3, 2)
+    x = rand_tt(dims, [1, 2, 5, 7, 6, 4, 1], 11)
+    y = O.orthogonalize(x, i=root)
+    np.testing.assert_allclose(O.ttv_to_tensor(y), O.ttv_to_tensor(x), rtol=1e-12, atol=1e-12)
+    for k in range(x.N):
+        c = y.cores[k]
+        if k < root - 1:  # left-orthogonal: sum_i A_i^T A_i = I
+            G = sum(c[i].T @ c[i] for i in range(c.shape[0]))
+            np.testing.assert_allclose(G, np.eye(c.shape[2]), atol=1e-13)
+            assert y.ot[k] == 1
+        elif k > root - 1:  # right-orthogonal: sum_i A_i A_i^T = I
+            G = sum(c[i] @ c[i].T for i in range(c.shape[0]))
+            np.testing.assert_allclose(G, np.eye(c.shape[1]), atol=1e-13)
+            assert y.ot[k] == -1
+        else:
+            assert y.ot[k] == 0
+    assert abs(O.tt_norm(x) - np.linalg.norm(O.ttv_to_tensor(x))) < 1e-12 * np.linalg.norm(O.ttv_to_tensor(x))
+
+
+def dense_tt_svd_left(T, dims, tol, rmax):
+    """Right-to-left truncation of the dense unfoldings with the floor rule."""
+    T = T.copy()
+    d = len(dims)
+    ranks = [1] * (d + 1)
+    for j in range(d, 1, -1):  # bond between modes j-1 | j (1-based)
+        rows = int(np.prod(dims[: j - 1]))
+        M = np.reshape(T, (rows, -1), order="F")
+        u, s, vt = np.linalg.svd(M, full_matrices=False)
+        _, k = O.floor_rule(s, tol)
+        k = min(k, rmax, int(np.sum(s > 0)))
+        M = (u[:, :k] * s[:k]) @ vt[:k]
+        T = np.reshape(M, T.shape, order="F")
+        ranks[j - 1] = k
+    return T, ranks
+
+
+@pytest.mark.parametrize("tol,rmax", [(1e-12, 2 ** 14), (1e-2, 2 ** 14), (1e-12, 3), (0.3, 5)])
+def test_tt_rounding_matches_dense_svd(tol, rmax):
+    dims = (2, 3, 2, 3, 2)
+    x = O.tt_add(rand_tt(dims, 3, 5), O.tt_scale(0.05, rand_tt(dims, 4, 6)))
+    X = O.ttv_to_tensor(x)
+    Td, ranks = dense_tt_svd_left(X, dims, tol, rmax)
+    y = O.tt_rounding(x, tol=tol, rmax=rmax)
+    assert y.rks[1:-1] == ranks[1:-1]
+    np.testing.assert_allclose(O.ttv_to_tensor(y), Td, rtol=1e-11, atol=1e-11 * np.abs(X).max())
+    assert y.ot == [0] + [-1] * (len(dims) - 1)
+    yr = O.tt_rounding(x, tol=tol, rmax=rmax, direction="right")
+    assert yr.ot == [1] * (len(dims) - 1) + [0]
+
+
+def test_tt_rounding_closed_form():
+    # C0 pattern: round(y + y + 1e-10 z; 1e-8) = 2y with the ranks of y
+    dims = (4,) * 6
+    y, z = rand_tt(dims, 5, 7), rand_tt(dims, 5, 8)
+    x = O.tt_add(O.tt_add(y, y), O.tt_scale(1e-10, z))
+    r = O.tt_rounding(x, tol=1e-8)
+    assert r.rks == y.rks
+    assert O.rel_err(O.tt_scale(2.0, y), r) < 1e-9
+    with pytest.raises(ValueError):
+        O.tt_rounding(x, direction="up")
+
+
+# ---------------------------------------------------------------- sketches
+
+def test_generate_sketch_blocks():
+    g = RNG.standard_normal(3 * 2 * 4 * 2)
+    S = O.generate_sketch_blocks(g, 3, 2, 4, 2, orthogonal=False)
+    np.testing.assert_allclose(O.jreshape(S, -1), g / 2.0, rtol=1e-15)
+    S = O.generate_sketch_blocks(g, 3, 2, 4, 2, orthogonal=True)
+    G = O.jreshape(g, (6, 4, 2))
+    for j in range(2):
+        Q = O.jreshape(S[:, :, :, j], (6, 4))
+        np.testing.assert_allclose(Q.T @ Q, (6 / 4) * np.eye(4), atol=1e-13)
+        R = Q.T @ G[:, :, j] / math.sqrt(6 / 4)
+        np.testing.assert_allclose(np.tril(R, -1), 0.0, atol=1e-12)
+    S = O.generate_sketch_blocks(None, 2, 3, 6, 2, orthogonal=True)
+    for j in range(2):
+        np.testing.assert_array_equal(O.jreshape(S[:, :, :, j], (6, 6)), np.eye(6))
+
+
+def _dense_right(x, k):
+    """X_{k:N} as an (r_k x prod n) matrix (column-major multi-index)."""
+    acc = x.cores[k].transpose(1, 0, 2)  # (r, n, r')
+    for j in range(k + 1, x.N):
+        acc = np.tensordot(acc, x.cores[j], axes=([acc.ndim - 1], [1]))
+    return acc[..., 0].reshape(x.rks[k], -1, order="F")
+
+
+def _dense_sketch(Sblocks, dims, bv, p, k):
+    """Omega_{k:N} as (prod n) x (bv_k p_k) from the block chains."""
+    N = len(dims)
+    cols = []
+    for j in range(p[k]):
+        acc = Sblocks[k][:, :, :, j]  # (bvr, n, bvl)
+        acc = acc.transpose(1, 2, 0)  # (n, bvl, bvr)
+        # build Omega^{(j)}_{k:N}[(i_k..i_N), b]
+        om = None
+        for q in range(N - 1, k - 1, -1):
+            jj = j if q < N - 1 else 0
+            Sq = Sblocks[q][:, :, :, j if q > k else j]  # (bvr, n, bvl)
+            if q == N - 1:
+                om = Sq[0]  # (n, bvl)
+            else:
+                # om_new[(i_q, rest), b] = sum_beta S_q[beta, i_q, b] om[rest, beta]
+                t = np.einsum("bic,rb->irc", Sq, om)
+                om = t.reshape(-1, Sq.shape[2], order="F")
+            del jj
+        cols.append(om)
+    return np.concatenate(cols, axis=1)
+
+
+@pytest.mark.parametrize("orthogonal,block_rks", [(False, None), (True, None), (False, 2), (True, 3)])
+def test_recursive_sketch_dense(orthogonal, block_rks):
+    dims = (2, 3, 2, 2, 3)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 4, 5, 3, 1], 21)
+    rks = [1, 4, 5, 5, 4, 1]
+    bv, p = O.sketch_structure(dims, rks, block_rks)
+    draws = [RNG.standard_normal(bv[k + 1] * dims[k] * bv[k] * p[k]) for k in range(N)]
+    W, s = O.tt_recursive_sketch(x, rks, draws, orthogonal, block_rks)
+    Sb = [O.generate_sketch_blocks(draws[k], bv[k + 1], dims[k], bv[k], p[k], orthogonal) for k in range(N)]
+    for k in range(N):
+        Om = _dense_sketch(Sb, dims, bv, p, k)
+        ref = _dense_right(x, k) @ Om / math.sqrt(p[k])
+        np.testing.assert_allclose(W[k], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+        assert W[k].shape == (x.rks[k], s[k])
+
+
+def test_full_sketch_with_p1_is_dot():
+    # with block rank >= every bond the sketch is a Gaussian TT and W[1] = <x, omega>
+    dims = (2, 2, 2, 2)
+    x = rand_tt(dims, [1, 2, 4, 2, 1], 3)
+    rks = [1, 1, 1, 1, 1]
+    bv, p = O.sketch_structure(dims, rks, 1)
+    draws = [RNG.standard_normal(bv[k + 1] * dims[k] * bv[k] * p[k]) for k in range(4)]
+    W, _ = O.tt_recursive_sketch(x, rks, draws, False, 1)
+    om = O.TT([O.generate_sketch_blocks(draws[k], 1, 2, 1, 1, False)[:, :, :, 0].transpose(1, 0, 2) for k in range(4)],
+              dims, [1] * 5)
+    assert abs(W[0][0, 0] - O.tt_dot(x, om)) < 1e-12 * abs(O.tt_dot(x, om)) + 1e-14
+
+
+# ---------------------------------------------------------------- randomized rounding
+
+def _draws_for(dims, rks, block_rks, seed=1234):
+    bv, p = O.sketch_structure(dims, rks, block_rks)
+    return [P.normal_draws(seed, k, bv[k + 1] * dims[k] * bv[k] * p[k]) for k in range(len(dims))]
+
+
+@pytest.mark.parametrize("orthogonal", [False, True])
+def test_ttrand_rounding_exact_when_rank_sufficient(orthogonal):
+    dims = (2,) * 8
+    a = rand_tt(dims, [1, 2, 4, 4, 4, 4, 4, 2, 1], 31)
+    pert = O.tt_add(a, O.tt_scale(1e-8, rand_tt(dims, [1, 2, 3, 3, 3, 3, 3, 2, 1], 32)))
+    rks = [1, 2, 4, 8, 16, 8, 4, 2, 1]
+    y = O.ttrand_rounding(pert, rks, _draws_for(dims, rks, None), orthogonal)
+    assert O.rel_err(pert, y) < 1e-12
+    for k in range(len(dims) - 1):
+        c = y.cores[k]
+        G = sum(c[i].T @ c[i] for i in range(c.shape[0]))
+        np.testing.assert_allclose(G, np.eye(c.shape[2]), atol=1e-12)
+    assert y.ot == [1] * 7 + [0]
+
+
+def test_ttrand_rounding_lincomb_and_sum_agree_exactly_recovered():
+    dims = (2,) * 6
+    ys = [rand_tt(dims, 2, 40 + j) for j in range(3)]
+    al = [0.5, -1.0, 2.0]
+    rks = [1, 2, 4, 8, 4, 2, 1]
+    dr = _draws_for(dims, rks, None)
+    y1 = O.ttrand_rounding_lincomb(al, ys, rks, dr, False)
+    ref = O.tt_lincomb(ys, al)
+    assert O.rel_err(ref, y1) < 1e-12
+
+
+def test_ttrand_rounding_truncates_noise():
+    # test_ttrand.jl:164-175 pattern: |A - rand(A + eps B, r)| < 1e-3 |A|
+    dims = (6,) * 6
+    A = rand_tt(dims, 3, 50)
+    B = rand_tt(dims, 12, 51)
+    eps = 1e-4
+    x = O.tt_add(O.tt_scale(1.0 / O.tt_norm(A), A), O.tt_scale(eps / O.tt_norm(B), B))
+    rks = [1] + [6] * 5 + [1]
+    y = O.ttrand_rounding(x, rks, _draws_for(dims, rks, None), True)
+    assert O.rel_err(O.tt_scale(1.0 / O.tt_norm(A), A), y) < 1e-3
+
+
+def test_ttrand_rounding_operator_exact():
+    L = 4
+    H = O.hubbard_1d_mpo(L, 1.0, 4.0)
+    dims = H.dims
+    y = rand_tt(dims, [1, 3, 5, 3, 1], 60)
+    Hy = O.mpo_apply(H, y)
+    rks = [1, 4, 16, 4, 1]
+    dr = _draws_for(dims, rks, None)
+    r = O.ttrand_rounding_op([1.5], H, [y], rks, dr, False)
+    assert O.rel_err(O.tt_scale(1.5, Hy), r) < 1e-11
+    # operator sketch equals the plain sketch of the explicit product up to the bond order
+    Wop, s1 = O.tt_recursive_sketch_op(H, y, rks, dr, False)
+    Wex, s2 = O.tt_recursive_sketch(Hy, rks, dr, False)
+    assert s1 == s2
+    for k in range(L + 1):
+        ex = Wex[k].reshape(H.rks[k], y.rks[k], -1, order="F").transpose(1, 0, 2)
+        np.testing.assert_allclose(Wop[k], ex, rtol=1e-11, atol=1e-11 * np.abs(ex).max())
+    b = rand_tt(dims, [1, 2, 2, 2, 1], 61)
+    res = O.ttrand_rounding_residual(H, y, b, rks, dr, False)
+    assert O.rel_err(O.tt_sub(Hy, b), res) < 1e-11
+
+
+# ---------------------------------------------------------------- RNG
+
+def test_philox_known_answers():
+    def kat(c, k):
+        r = P.philox4x32_10(*[np.uint64(v) for v in c], np.uint64(k[0]), np.uint64(k[1]))
+        return [int(v) for v in r]
+    assert kat([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert kat([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert kat([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_normal_draws_statistics():
+    g = P.normal_draws(1234, 0, 400000)
+    assert abs(g.mean()) < 0.01 and abs(g.var() - 1.0) < 0.01
+    assert abs(np.mean(g ** 4) - 3.0) < 0.05
+    assert not np.array_equal(g[:100], P.normal_draws(1234, 1, 100))
+    np.testing.assert_array_equal(g[50:60], P.normal_draws(1234, 0, 10, start=50))
