@@ -162,6 +162,19 @@ int ttr_sketch_blocks(ttr_handle_t h, int64_t d, const int64_t* dims, const int6
 /* Raw N(0,1) draws number 0..count-1 of core k of the sketch stream. */
 int ttr_sketch_draws(ttr_handle_t h, int64_t k, int64_t count, uint64_t seed, double* out);
 
+/* ------------------------------------------------------ instrumentation */
+
+/* hipEvent timing of every kernel launch and sweep phase on the handle's
+ * stream, with the algorithmic flops and bytes of each launch.  Phase names
+ * ("phase:reverse_sketch", "phase:orthogonalization", ...) follow the
+ * reference's TimerOutputs sections (src/tt_randtools.jl:113-153,
+ * src/sketches.jl:176-304).  Statistics accumulate until ttr_profile_reset. */
+int ttr_profile_enable(ttr_handle_t h, int on);
+int ttr_profile_reset(ttr_handle_t h);
+int ttr_profile_count(ttr_handle_t h, int64_t* count);
+int ttr_profile_get(ttr_handle_t h, int64_t i, char* name, int64_t name_len, double* ms, int64_t* launches,
+                    double* flops, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -772,8 +772,12 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
     return outs;
   }
-  make_sketch(h, sp, 1);
-  sketch_phase(h, sp, items, 1);
+  {
+    ProfScope ps(ctx, "phase:reverse_sketch");
+    make_sketch(h, sp, 1);
+    sketch_phase(h, sp, items, 1);
+  }
+  ProfScope ps_orth(ctx, "phase:orthogonalization");
   // buffers of the left-to-right sweep
   for (size_t i = 0; i < items.size(); ++i) {
     int64_t zmax = 1;
@@ -1149,6 +1153,7 @@ int ttr_round_svd(ttr_handle_t h, int64_t n, const int64_t* ids, double tol, int
     for (auto& kv : need) {
       std::vector<DevTT*> grp;
       for (int64_t i : kv.second) grp.push_back(xs[i]);
+      ProfScope ps(h->ctx, "phase:orthogonalize");
       auto o = orthogonalize_batch(h, grp, direction == TTR_DIR_LEFT ? kv.first : 1);
       for (size_t g = 0; g < kv.second.size(); ++g) {
         ys[kv.second[g]] = o[g].get();
@@ -1157,7 +1162,10 @@ int ttr_round_svd(ttr_handle_t h, int64_t n, const int64_t* ids, double tol, int
     }
     std::vector<SweepState> st(static_cast<size_t>(n));
     for (int64_t i = 0; i < n; ++i) st[i].y = ys[i];
-    svd_sweep(h, st, tol, rmax, direction);
+    {
+      ProfScope ps(h->ctx, "phase:svd_sweep");
+      svd_sweep(h, st, tol, rmax, direction);
+    }
     h->ctx.sync();
     for (auto& t : tmp)
       if (t) h->free_tt(t.get());
@@ -1280,6 +1288,39 @@ int ttr_sketch_blocks(ttr_handle_t h, int64_t d, const int64_t* dims, const int6
       shape4[2] = bvl;
       shape4[3] = p;
     }
+  });
+}
+
+int ttr_profile_enable(ttr_handle_t h, int on) {
+  return guarded(h, [&] { h->ctx.prof.on = on != 0; });
+}
+
+int ttr_profile_reset(ttr_handle_t h) {
+  return guarded(h, [&] { h->ctx.prof.reset(); });
+}
+
+int ttr_profile_count(ttr_handle_t h, int64_t* count) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(count, ST_INVALID_ARG, "null count");
+    *count = int64_t(h->ctx.prof.stats.size());
+  });
+}
+
+int ttr_profile_get(ttr_handle_t h, int64_t i, char* name, int64_t name_len, double* ms, int64_t* launches,
+                    double* flops, double* bytes) {
+  return guarded(h, [&] {
+    auto& st = h->ctx.prof.stats;
+    TTR_REQUIRE(i >= 0 && i < int64_t(st.size()), ST_INVALID_ARG, "profile index out of range");
+    auto it = st.begin();
+    std::advance(it, i);
+    if (name && name_len > 0) {
+      std::strncpy(name, it->first.c_str(), size_t(name_len - 1));
+      name[name_len - 1] = 0;
+    }
+    if (ms) *ms = it->second.ms;
+    if (launches) *launches = it->second.launches;
+    if (flops) *flops = it->second.flops;
+    if (bytes) *bytes = it->second.bytes;
   });
 }
 

@@ -137,14 +137,16 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
       __syncthreads();
     }
   }
-  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 block
+  // epilogue: register r of a lane holds column lane%16 of each 16x16 block and
+  // row 4*(lane/16)+r for the f32 MFMA, row 4*r+lane/16 for the f64 MFMA
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
     for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gi = row0 + wm * 32 + bi * 16 + 4 * (lane >> 4) + r;
+        const int rr = std::is_same<MT, double>::value ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
+        const int64_t gi = row0 + wm * 32 + bi * 16 + rr;
         const int64_t gj = col0 + wn * 32 + bj * 16 + (lane & 15);
         if (gi < P.M && gj < P.N) {
           const int64_t ci = gi * P.c_rs + gj * P.c_cs;
@@ -322,6 +324,21 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math) {
   const GemmProb* dp = ctx.upload(probs);
   const GemmSeg* ds = ctx.upload(segs);
   const int np = int(probs.size());
+  double flops = 0.0, bytes = 0.0;
+  if (ctx.prof.on) {
+    for (const auto& p : probs) {
+      double k = 0.0;
+      for (int q = 0; q < p.nseg; ++q) {
+        const GemmSeg& sg = segs[p.seg0 + q];
+        k += sg.K;
+        bytes += double(p.M) * sg.K * prec_bytes(ta) + double(sg.K) * p.N * prec_bytes(tb);
+      }
+      flops += 2.0 * p.M * p.N * k;
+      bytes += double(p.M) * p.N * prec_bytes(tc) * (p.beta != 0.0 ? 2.0 : 1.0);
+    }
+  }
+  static const char* names[3] = {"gemm_f64", "gemm_f32", "gemm_bf16"};
+  ProfScope ps(ctx, names[int(math)], flops, bytes);
   if (ta == Prec::F64) launch_b<double>(ctx, tb, tc, math, int(ntiles), dp, np, ds);
   else if (ta == Prec::F32) launch_b<float>(ctx, tb, tc, math, int(ntiles), dp, np, ds);
   else launch_b<bf16s>(ctx, tb, tc, math, int(ntiles), dp, np, ds);

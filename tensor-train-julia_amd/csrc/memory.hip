@@ -133,4 +133,53 @@ void CachingAlloc::release_cached() {
   free_.clear();
 }
 
+Profiler::~Profiler() {
+  for (auto& p : pending_) {
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  for (auto e : pool_) (void)hipEventDestroy(e);
+}
+
+hipEvent_t Profiler::get_event() {
+  if (!pool_.empty()) {
+    hipEvent_t e = pool_.back();
+    pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  TTR_HIP(hipEventCreate(&e));
+  return e;
+}
+
+int Profiler::start(hipStream_t s, const char* name, double flops, double bytes) {
+  Pending p{get_event(), get_event(), name, flops, bytes};
+  TTR_HIP(hipEventRecord(p.a, s));
+  pending_.push_back(p);
+  return int(pending_.size()) - 1;
+}
+
+void Profiler::stop(hipStream_t s, int token) {
+  if (token < 0 || token >= int(pending_.size())) return;
+  (void)hipEventRecord(pending_[token].b, s);
+}
+
+void Profiler::collect() {
+  for (auto& p : pending_) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      ProfStat& st = stats[p.name];
+      st.ms += ms;
+      st.launches += 1;
+      st.flops += p.flops;
+      st.bytes += p.bytes;
+    } else {
+      (void)hipGetLastError();
+    }
+    pool_.push_back(p.a);
+    pool_.push_back(p.b);
+  }
+  pending_.clear();
+}
+
 }  // namespace ttr

@@ -162,6 +162,12 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     }
     if (pj.empty()) continue;
     const PanelJob* d = ctx.upload(pj);
+    double pfl = 0.0, pby = 0.0;
+    for (const auto& j : pj) {
+      pfl += 4.0 * double(j.m - j.c0) * j.nb * j.nb;  // Householder panel (2 m nb^2) + T
+      pby += 16.0 * double(j.m - j.c0) * j.nb;
+    }
+    ProfScope ps(ctx, "qr_panel", pfl, pby);
     hipLaunchKernelGGL(qr_panel_kernel, dim3(unsigned(pj.size())), dim3(PT), 0, ctx.stream, d);
     TTR_HIP(hipGetLastError());
     // trailing update A2 -= (V T^T) (V^T A2)

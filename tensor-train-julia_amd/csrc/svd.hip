@@ -105,6 +105,12 @@ void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs) {
     if (j.p > 0) live.push_back(j);
   if (live.empty()) return;
   const SVDJob* d = ctx.upload(live);
+  double fl = 0.0, by = 0.0;
+  for (const auto& j : live) {
+    fl += 9.0 * double(j.p) * j.p * j.p;  // one sweep of rotations (nominal)
+    by += 16.0 * double(j.p) * j.p;
+  }
+  ProfScope ps(ctx, "jacobi_svd", fl, by);
   hipLaunchKernelGGL(jacobi_kernel, dim3(unsigned(live.size())), dim3(JT), 0, ctx.stream, d, 60);
   TTR_HIP(hipGetLastError());
 }

@@ -104,12 +104,41 @@ class CachingAlloc {
   std::unordered_map<void*, size_t> live_;
 };
 
+// Per-kernel / per-phase timing with hipEvents on the handle's stream
+// (enabled by ttr_profile_enable).  Each record keeps the algorithmic flops and
+// bytes of the launch so achieved rates can be computed without counters.
+struct ProfStat {
+  double ms = 0.0;
+  int64_t launches = 0;
+  double flops = 0.0;
+  double bytes = 0.0;
+};
+
+class Profiler {
+ public:
+  ~Profiler();
+  bool on = false;
+  // returns a token; call stop(token) after enqueuing the launch
+  int start(hipStream_t s, const char* name, double flops, double bytes);
+  void stop(hipStream_t s, int token);
+  void collect();  // stream must be synchronised
+  void reset() { stats.clear(); }
+  std::map<std::string, ProfStat> stats;
+
+ private:
+  struct Pending { hipEvent_t a, b; std::string name; double flops, bytes; };
+  std::vector<Pending> pending_;
+  std::vector<hipEvent_t> pool_;
+  hipEvent_t get_event();
+};
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   DevArena ws;
   PinnedArena pin;
   CachingAlloc cache;
+  Profiler prof;
   // copies a host array into fresh device workspace (asynchronously, through pinned memory)
   template <class T>
   T* upload(const T* host, size_t n) {
@@ -123,7 +152,16 @@ struct Ctx {
   template <class T>
   T* upload(const std::vector<T>& v) { return upload(v.data(), v.size()); }
   void sync() { TTR_HIP(hipStreamSynchronize(stream)); }
-  void end_call() { sync(); ws.reset(); pin.reset(); }
+  void end_call() { sync(); if (prof.on) prof.collect(); ws.reset(); pin.reset(); }
+};
+
+// RAII timing scope around one or more launches on ctx.stream
+struct ProfScope {
+  Ctx& ctx;
+  int tok;
+  ProfScope(Ctx& c, const char* name, double flops = 0.0, double bytes = 0.0)
+      : ctx(c), tok(c.prof.on ? c.prof.start(c.stream, name, flops, bytes) : -1) {}
+  ~ProfScope() { if (tok >= 0) ctx.prof.stop(ctx.stream, tok); }
 };
 
 // ---------------------------------------------------------------------------

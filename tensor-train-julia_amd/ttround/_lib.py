@@ -76,6 +76,10 @@ SIGNATURES = {
     "ttr_sketch": (C.c_int, [C.c_void_p, i64, i64p, C.POINTER(RandOpts), f64pp, i64p]),
     "ttr_sketch_blocks": (C.c_int, [C.c_void_p, i64, i64p, i64p, C.POINTER(RandOpts), i64, f64p, i64p]),
     "ttr_sketch_draws": (C.c_int, [C.c_void_p, i64, i64, C.c_uint64, f64p]),
+    "ttr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "ttr_profile_reset": (C.c_int, [C.c_void_p]),
+    "ttr_profile_count": (C.c_int, [C.c_void_p, i64p]),
+    "ttr_profile_get": (C.c_int, [C.c_void_p, i64, C.c_char_p, i64, f64p, i64p, f64p, f64p]),
 }
 
 _lib = None
@@ -126,6 +130,25 @@ class Handle:
         if st != TTR_OK:
             msg = self.lib.ttr_last_error(self.h).decode(errors="replace")
             raise _EXC.get(st, DeviceError)(msg)
+
+    def profile(self, on: bool = True):
+        self.check(self.lib.ttr_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self):
+        self.check(self.lib.ttr_profile_reset(self.h))
+
+    def profile_stats(self) -> dict:
+        """{name: {"ms", "launches", "flops", "bytes"}} accumulated since the last reset."""
+        n = C.c_int64()
+        self.check(self.lib.ttr_profile_count(self.h, C.byref(n)))
+        out = {}
+        for i in range(n.value):
+            name = C.create_string_buffer(128)
+            ms, fl, by = C.c_double(), C.c_double(), C.c_double()
+            la = C.c_int64()
+            self.check(self.lib.ttr_profile_get(self.h, i, name, 128, C.byref(ms), C.byref(la), C.byref(fl), C.byref(by)))
+            out[name.value.decode()] = {"ms": ms.value, "launches": la.value, "flops": fl.value, "bytes": by.value}
+        return out
 
     def close(self):
         if getattr(self, "h", None):
