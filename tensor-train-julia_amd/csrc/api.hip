@@ -266,8 +266,10 @@ std::vector<std::unique_ptr<DevTT>> orthogonalize_batch(ttr_handle_s* h, const s
       const double* fl = (j == x->d - 1) ? h->one : FL[q];
       const int64_t ldfl = (j == x->d - 1) ? 1 : R1;
       const int64_t ldT = n * rn;
-      for (int64_t i = 0; i < n; ++i)  // T(i + n b, a) = sum_beta FL(beta, b) X(a, i, beta)
-        gb.add(int(rn), int(R0), int(R1), 1.0, fl, ldfl, 1, x->cores[j] + R0 * i, R0 * n, 1, 0.0, Abuf[q] + i, n, ldT);
+      // rows of T in the reference's order (src/tt_rounding.jl:85-86): T(b + rn i, a) = sum_g FL(g, b) X(a, i, g)
+      for (int64_t i = 0; i < n; ++i)
+        gb.add(int(rn), int(R0), int(R1), 1.0, fl, ldfl, 1, x->cores[j] + R0 * i, R0 * n, 1, 0.0, Abuf[q] + rn * i, 1,
+               ldT);
       qj.push_back(QRJob{Abuf[q], ldT, int(ldT), int(R0), Qw[q], ldT});
     }
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
@@ -278,7 +280,8 @@ std::vector<std::unique_ptr<DevTT>> orthogonalize_batch(ttr_handle_s* h, const s
       if (j >= x->d) continue;
       const int64_t n = x->dims[j], R0 = x->rks[j], rn = yr[q][j + 1], k = yr[q][j];
       const int64_t ldT = n * rn;
-      cp.push_back(copy_job(Qw[q], 1, ldT, outs[q]->cores[j], k, 1, int(ldT), int(k)));       // core = Q1^T
+      for (int64_t i = 0; i < n; ++i)  // core(c, i, b) = Q1(b + rn i, c)
+        cp.push_back(copy_job(Qw[q] + rn * i, 1, ldT, outs[q]->cores[j] + k * i, k * n, 1, int(rn), int(k)));
       cp.push_back(copy_job(Abuf[q], 1, ldT, FL[q], R0, 1, int(k), int(R0), 1.0, 1));          // FL = R1^T
     }
     copy2d_batched(ctx, cp);

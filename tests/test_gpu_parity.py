@@ -212,7 +212,10 @@ def test_ttrand_rounding_exact_recovery(T, prec, orthogonal):
     rks = [1, 2, 4, 8, 16, 8, 4, 2, 1]
     y = to_oracle(T.ttrand_rounding(to_api(pert), rks, orthogonal=orthogonal, sketch_prec=prec))
     assert y.ot == [1] * 7 + [0]
-    assert O.rel_err(pert, y) < 1e-11
+    # BF16 rounding of the stored sketches W (and of the core operands of the BF16
+    # contractions) blurs how the 1e-8 perturbation directions are resolved; storing
+    # W in BF16 alone already gives ~2e-11 in an FP64 emulation, so BF16 gets 1e-9.
+    assert O.rel_err(pert, y) < (1e-9 if prec == "bf16" else 1e-11)
     orth_ok(y, atol=1e-11)
 
 
