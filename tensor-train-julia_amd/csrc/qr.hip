@@ -126,6 +126,152 @@ __global__ __launch_bounds__(PT) void qr_panel_kernel(const PanelJob* __restrict
   }
 }
 
+
+// Register-resident panel factorisation: panel width NBR = 16, 512 threads,
+// thread t holds panel rows t, t+512, ... (R of them), so a Householder step is
+// two workgroup reductions and register updates, with no global traffic.
+constexpr int NBR = 16;
+constexpr int RT = 512;
+constexpr int RW = RT / 64;
+
+template <int R>
+__global__ __launch_bounds__(RT) void qr_panel_reg_kernel(const PanelJob* __restrict__ jobs) {
+  const PanelJob J = jobs[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ double red[RW][NBR + 1];
+  __shared__ double bc[NBR + 1];
+  __shared__ double Ts[NBR][NBR + 1];
+  double* A = J.A;
+  const int64_t lda = J.lda;
+  const int c0 = J.c0, nb = J.nb;
+  const int m2 = J.m - c0;
+  double P[R][NBR];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = tid + RT * q;
+#pragma unroll
+    for (int c = 0; c < NBR; ++c)
+      P[q][c] = (i < m2 && c < nb) ? A[(c0 + i) + lda * (c0 + c)] : 0.0;
+  }
+  for (int idx = tid; idx < NBR * (NBR + 1); idx += RT) (&Ts[0][0])[idx] = 0.0;
+#pragma unroll
+  for (int jj = 0; jj < NBR; ++jj) {
+    if (jj < nb) {
+      // sigma = sum_{i > jj} x_i^2 ; alpha = P[jj][jj] (held by thread jj, q = 0)
+      double part = 0.0;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = tid + RT * q;
+        if (i > jj && i < m2) part += P[q][jj] * P[q][jj];
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+      if (lane == 0) red[wave][0] = part;
+      if (tid == jj) bc[NBR] = P[0][jj];
+      __syncthreads();
+      double sigma = 0.0;
+#pragma unroll
+      for (int w = 0; w < RW; ++w) sigma += red[w][0];
+      const double alpha = bc[NBR];
+      double tau = 0.0, beta = alpha, scal = 1.0;
+      if (sigma != 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+        tau = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = tid + RT * q;
+        if (i > jj && i < m2) P[q][jj] *= scal;
+      }
+      if (tid == jj) P[0][jj] = beta;
+      if (tid == 0) J.tau[c0 + jj] = tau;
+      // d_c = sum_{i >= jj} v_i P[i][c] (v_jj = 1) for every other column c
+      double acc[NBR];
+#pragma unroll
+      for (int c = 0; c < NBR; ++c) acc[c] = 0.0;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = tid + RT * q;
+        if (i >= jj && i < m2) {
+          const double v = (i == jj) ? 1.0 : P[q][jj];
+#pragma unroll
+          for (int c = 0; c < NBR; ++c)
+            if (c != jj) acc[c] += v * P[q][c];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int c = 0; c < NBR; ++c) {
+        double sv = acc[c];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) sv += __shfl_xor(sv, off);
+        if (lane == 0) red[wave][c] = sv;
+      }
+      __syncthreads();
+      if (tid < NBR) {
+        double sv = 0.0;
+#pragma unroll
+        for (int w = 0; w < RW; ++w) sv += red[w][tid];
+        bc[tid] = sv;
+      }
+      __syncthreads();
+      // apply H_jj to columns c > jj
+      if (tau != 0.0) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int i = tid + RT * q;
+          if (i >= jj && i < m2) {
+            const double v = (i == jj) ? 1.0 : P[q][jj];
+#pragma unroll
+            for (int c = 0; c < NBR; ++c)
+              if (c > jj) P[q][c] -= tau * v * bc[c];
+          }
+        }
+      }
+      // T(0:jj, jj) = -tau T(0:jj, 0:jj) z,  z_c = d_c (c < jj)
+      if (tid < jj) {
+        double sv = 0.0;
+        for (int qq = tid; qq < jj; ++qq) sv += Ts[tid][qq] * bc[qq];
+        Ts[tid][jj] = -tau * sv;
+      }
+      if (tid == 0) Ts[jj][jj] = tau;
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = tid + RT * q;
+    if (i < m2) {
+#pragma unroll
+      for (int c = 0; c < NBR; ++c)
+        if (c < nb) A[(c0 + i) + lda * (c0 + c)] = P[q][c];
+    }
+  }
+  for (int idx = tid; idx < NBP * NBP; idx += RT) {
+    const int r = idx % NBP, c = idx / NBP;
+    J.T[r + NBP * c] = (r < nb && c < nb && r < NBR && c < NBR) ? Ts[r][c] : 0.0;
+  }
+}
+
+void launch_panel(Ctx& ctx, const PanelJob* d, int njobs, int max_rows, bool reg) {
+  if (!reg) {
+    hipLaunchKernelGGL(qr_panel_kernel, dim3(unsigned(njobs)), dim3(PT), 0, ctx.stream, d);
+  } else {
+    const int R = (max_rows + RT - 1) / RT;
+    switch (R) {
+      case 0:
+      case 1: hipLaunchKernelGGL(qr_panel_reg_kernel<1>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
+      case 2: hipLaunchKernelGGL(qr_panel_reg_kernel<2>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
+      case 3: hipLaunchKernelGGL(qr_panel_reg_kernel<3>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
+      case 4: hipLaunchKernelGGL(qr_panel_reg_kernel<4>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
+      case 5: hipLaunchKernelGGL(qr_panel_reg_kernel<5>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
+      default: hipLaunchKernelGGL(qr_panel_reg_kernel<6>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
+    }
+  }
+  TTR_HIP(hipGetLastError());
+}
+
 }  // namespace
 
 void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
@@ -138,12 +284,17 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
   };
   std::vector<Aux> aux(jobs.size());
   int max_panels = 0;
+  int max_m = 0;
+  for (const auto& J : jobs) max_m = std::max(max_m, J.m);
+  // register-resident panels (width 16) up to 6*512 rows, else the streaming panel (width 32)
+  const bool reg = max_m <= 6 * RT;
+  const int NB = reg ? NBR : NBP;
   for (size_t q = 0; q < jobs.size(); ++q) {
     const QRJob& J = jobs[q];
     TTR_REQUIRE(J.m >= 0 && J.n >= 0 && J.lda >= std::max(1, J.m), ST_INVALID_ARG, "qr: bad shape");
     Aux& a = aux[q];
     a.kq = std::min(J.m, J.n);
-    a.npanel = (a.kq + NBP - 1) / NBP;
+    a.npanel = (a.kq + NB - 1) / NB;
     max_panels = std::max(max_panels, a.npanel);
     a.tau = ctx.ws.alloc_n<double>(std::max(1, J.n));
     a.T = ctx.ws.alloc_n<double>(size_t(std::max(1, a.npanel)) * NBP * NBP);
@@ -157,7 +308,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     for (size_t q = 0; q < jobs.size(); ++q) {
       const Aux& a = aux[q];
       if (p >= a.npanel) continue;
-      const int c0 = p * NBP, nb = std::min(NBP, a.kq - c0);
+      const int c0 = p * NB, nb = std::min(NB, a.kq - c0);
       pj.push_back(PanelJob{jobs[q].A, jobs[q].lda, jobs[q].m, c0, nb, a.tau, a.T + size_t(p) * NBP * NBP});
     }
     if (pj.empty()) continue;
@@ -167,16 +318,19 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       pfl += 4.0 * double(j.m - j.c0) * j.nb * j.nb;  // Householder panel (2 m nb^2) + T
       pby += 16.0 * double(j.m - j.c0) * j.nb;
     }
-    ProfScope ps(ctx, "qr_panel", pfl, pby);
-    hipLaunchKernelGGL(qr_panel_kernel, dim3(unsigned(pj.size())), dim3(PT), 0, ctx.stream, d);
-    TTR_HIP(hipGetLastError());
+    int max_rows = 0;
+    for (const auto& j : pj) max_rows = std::max(max_rows, j.m - j.c0);
+    {
+      ProfScope ps(ctx, "qr_panel", pfl, pby);
+      launch_panel(ctx, d, int(pj.size()), max_rows, reg);
+    }
     // trailing update A2 -= (V T^T) (V^T A2)
     for (int stage = 0; stage < 3; ++stage) {
       for (size_t q = 0; q < jobs.size(); ++q) {
         const QRJob& J = jobs[q];
         const Aux& a = aux[q];
         if (p >= a.npanel) continue;
-        const int c0 = p * NBP, nb = std::min(NBP, a.kq - c0);
+        const int c0 = p * NB, nb = std::min(NB, a.kq - c0);
         const int m2 = J.m - c0, n2 = J.n - c0 - nb;
         if (n2 <= 0) continue;
         double* V = J.A + c0 + J.lda * c0;
@@ -204,7 +358,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
         const QRJob& J = jobs[q];
         const Aux& a = aux[q];
         if (!J.Q || p >= a.npanel) continue;
-        const int c0 = p * NBP, nb = std::min(NBP, a.kq - c0);
+        const int c0 = p * NB, nb = std::min(NB, a.kq - c0);
         const int m2 = J.m - c0, n2 = a.kq - c0;
         double* V = J.A + c0 + J.lda * c0;
         double* Q2 = J.Q + c0 + J.ldq * c0;
