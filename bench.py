@@ -123,6 +123,9 @@ def main():
     ms_per_step = 1e3 * dt / args.steps
     value = args.items * args.steps / dt
 
+    sweeps = stats.pop("jacobi_svd:sweeps", None)
+    jacobi = ({"svds": sweeps["launches"], "mean_sweeps": round(sweeps["flops"] / max(sweeps["launches"], 1), 2),
+               "max_sweeps": sweeps["bytes"]} if sweeps else None)
     kern = {k: v for k, v in stats.items() if not k.startswith("phase:")}
     phases = {k[6:]: round(v["ms"] / args.steps, 3) for k, v in stats.items() if k.startswith("phase:")}
     dom = max(kern, key=lambda k: kern[k]["ms"]) if kern else None
@@ -219,6 +222,7 @@ def main():
         "fp64_deterministic": det,
         "speedup_vs_fp64_deterministic": (round(value / det["rounded_tts_per_s"], 3) if det else None),
         "phases_ms_per_step": phases,
+        "jacobi": jacobi,
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -319,6 +319,7 @@ struct SweepState {
   int pp = 0;
   double *Mt, *Qb, *B, *J, *sv, *US;
   int32_t* idx;
+  int32_t* info;
   bool zero = false;
 };
 
@@ -358,6 +359,7 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     s.J = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
     s.sv = ctx.ws.alloc_n<double>(size_t(mxp));
     s.idx = ctx.ws.alloc_n<int32_t>(size_t(mxp));
+    s.info = ctx.ws.alloc_n<int32_t>(1);
     s.out = h->make_tt_lazy(s.y->dims);
   }
   GemmBatch gb;
@@ -412,7 +414,7 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       const bool transpose = left ? !t : t;
       if (transpose) cp.push_back(copy_job(s.Mt, 1, ldm, s.B, m, 1, int(m), int(m), 1.0, 1));
       else cp.push_back(copy_job(s.Mt, 1, ldm, s.B, 1, m, int(m), int(m), 1.0, 1));
-      sj.push_back(SVDJob{s.B, m, s.J, m, s.sv, int(m)});
+      sj.push_back(SVDJob{s.B, m, s.J, m, s.sv, int(m), s.info});
     }
     copy2d_batched(ctx, cp);
     jacobi_svd_batched(ctx, sj);
@@ -426,6 +428,14 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       const int64_t m = pdim[q];
       std::vector<double> sv(m);
       TTR_HIP(hipMemcpy(sv.data(), s.sv, size_t(m) * 8, hipMemcpyDeviceToHost));
+      if (ctx.prof.on) {
+        int32_t sw = 0;
+        TTR_HIP(hipMemcpy(&sw, s.info, 4, hipMemcpyDeviceToHost));
+        ProfStat& ps = ctx.prof.stats["jacobi_svd:sweeps"];
+        ps.launches += 1;
+        ps.flops += sw;
+        ps.bytes = std::max(ps.bytes, double(sw));
+      }
       std::vector<int32_t> order(m);
       std::iota(order.begin(), order.end(), 0);
       std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return sv[a] > sv[b]; });

@@ -7,7 +7,7 @@
 //      angles always come from fresh inner products, as in plain one-sided
 //      Jacobi);
 //   2. if some pair is not yet orthogonal (|G_ij| > tol sqrt(G_ii G_jj),
-//      tol = sqrt(p) eps), diagonalises G with a parallel cyclic Jacobi
+//      tol = p eps), diagonalises G with a parallel cyclic Jacobi
 //      eigen-iteration in LDS (32 disjoint rotations per round), accumulating
 //      the rotation V (64 x 64);
 //   3. applies V to the columns of B and of the accumulated right rotations J.
@@ -25,7 +25,7 @@ constexpr int WC = 2 * WB;      // columns per visit
 constexpr int GLD = WC + 1;     // padded LDS row
 constexpr int RCH = 16;         // row chunk for Gram / apply
 constexpr int MAX_SWEEPS = 40;
-constexpr int MAX_INNER = 12;
+constexpr int MAX_INNER = 4;
 
 __device__ __forceinline__ int rr_player(int pos, int r, int pp) {
   if (pos == 0) return 0;
@@ -46,18 +46,22 @@ __global__ __launch_bounds__(JT) void jacobi_block_kernel(const SVDJob* __restri
   __shared__ int cols[WC];        // global column of local index, -1 if padding
   __shared__ int partner[WC];
   __shared__ double cself[WC], cpart[WC];
-  __shared__ int any_rot, visit_rot, sweep_rot;
+  __shared__ int any_rot, visit_rot, sweep_rot, round_rot;
 
   for (int64_t idx = tid; idx < int64_t(p) * p; idx += JT) {
     const int64_t r = idx % p, c = idx / p;
     S.J[r + c * S.ldj] = (r == c) ? 1.0 : 0.0;
   }
-  const double tol = sqrt(double(p > 1 ? p : 1)) * 2.220446049250313e-16;
+  // inner products of length p carry rounding noise of order sqrt(p)..p eps, so a
+  // pair counts as orthogonal below p eps (at least 16 eps)
+  const double tol = double(p > 16 ? p : 16) * 2.220446049250313e-16;
+  int sweeps_done = 0;
   const int nblk = (p + WB - 1) / WB;
   const int pp = nblk + (nblk & 1);
   __syncthreads();
 
   for (int sweep = 0; sweep < MAX_SWEEPS && p > 1; ++sweep) {
+    sweeps_done = sweep + 1;
     if (tid == 0) sweep_rot = 0;
     __syncthreads();
     const int rounds = (nblk == 1) ? 1 : pp - 1;
@@ -115,6 +119,8 @@ __global__ __launch_bounds__(JT) void jacobi_block_kernel(const SVDJob* __restri
           if (tid == 0) any_rot = 0;
           __syncthreads();
           for (int ir = 0; ir < WC - 1; ++ir) {
+            if (tid == 0) round_rot = 0;
+            __syncthreads();
             if (tid < WC / 2) {
               int a = rr_player(tid, ir, WC), b = rr_player(WC - 1 - tid, ir, WC);
               if (a > b) { const int t = a; a = b; b = t; }
@@ -137,10 +143,10 @@ __global__ __launch_bounds__(JT) void jacobi_block_kernel(const SVDJob* __restri
               cpart[a] = -s;
               cself[b] = c;
               cpart[b] = s;
-              if (rot) any_rot = 1;
+              if (rot) { any_rot = 1; round_rot = 1; }
             }
             __syncthreads();
-            if (any_rot) {
+            if (round_rot) {
               double gn[WC * WC / JT], vn[WC * WC / JT];
 #pragma unroll
               for (int e = 0; e < WC * WC / JT; ++e) {
@@ -216,6 +222,7 @@ __global__ __launch_bounds__(JT) void jacobi_block_kernel(const SVDJob* __restri
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane == 0) S.sv[c] = sqrt(s);
   }
+  if (tid == 0 && S.info) S.info[0] = sweeps_done;
 }
 
 }  // namespace
@@ -232,6 +239,7 @@ void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs) {
     by += 16.0 * double(j.p) * j.p;
   }
   ProfScope ps(ctx, "jacobi_svd", fl, by);
+  // (sweep counts are read back by the caller through SVDJob::info)
   hipLaunchKernelGGL(jacobi_block_kernel, dim3(unsigned(live.size())), dim3(JT), 0, ctx.stream, d);
   TTR_HIP(hipGetLastError());
 }

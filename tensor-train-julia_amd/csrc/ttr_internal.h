@@ -242,6 +242,7 @@ struct SVDJob {
   int64_t ldj;
   double* sv;  // p singular values (column norms of B*J), unsorted
   int p;
+  int32_t* info;  // optional: number of sweeps used
 };
 void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs);
 
