@@ -126,6 +126,11 @@ def main():
     sweeps = stats.pop("jacobi_svd:sweeps", None)
     jacobi = ({"svds": sweeps["launches"], "mean_sweeps": round(sweeps["flops"] / max(sweeps["launches"], 1), 2),
                "max_sweeps": sweeps["bytes"]} if sweeps else None)
+    if jacobi:
+        for ph in ("gram", "inner", "apply"):
+            st = stats.pop("jacobi_svd:ticks_" + ph, None)
+            if st:
+                jacobi["ticks_" + ph] = st["flops"]
     kern = {k: v for k, v in stats.items() if not k.startswith("phase:")}
     phases = {k[6:]: round(v["ms"] / args.steps, 3) for k, v in stats.items() if k.startswith("phase:")}
     dom = max(kern, key=lambda k: kern[k]["ms"]) if kern else None

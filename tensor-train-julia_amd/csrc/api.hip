@@ -359,7 +359,7 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     s.J = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
     s.sv = ctx.ws.alloc_n<double>(size_t(mxp));
     s.idx = ctx.ws.alloc_n<int32_t>(size_t(mxp));
-    s.info = ctx.ws.alloc_n<int32_t>(1);
+    s.info = ctx.ws.alloc_n<int32_t>(4);
     s.out = h->make_tt_lazy(s.y->dims);
   }
   GemmBatch gb;
@@ -429,12 +429,16 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       std::vector<double> sv(m);
       TTR_HIP(hipMemcpy(sv.data(), s.sv, size_t(m) * 8, hipMemcpyDeviceToHost));
       if (ctx.prof.on) {
-        int32_t sw = 0;
-        TTR_HIP(hipMemcpy(&sw, s.info, 4, hipMemcpyDeviceToHost));
+        int32_t info[4] = {0, 0, 0, 0};
+        TTR_HIP(hipMemcpy(info, s.info, 16, hipMemcpyDeviceToHost));
         ProfStat& ps = ctx.prof.stats["jacobi_svd:sweeps"];
         ps.launches += 1;
-        ps.flops += sw;
-        ps.bytes = std::max(ps.bytes, double(sw));
+        ps.flops += info[0];
+        ps.bytes = std::max(ps.bytes, double(info[0]));
+        // wall-clock ticks (x1024) spent in the Gram, inner-iteration and apply phases
+        ctx.prof.stats["jacobi_svd:ticks_gram"].flops += 1024.0 * info[1];
+        ctx.prof.stats["jacobi_svd:ticks_inner"].flops += 1024.0 * info[2];
+        ctx.prof.stats["jacobi_svd:ticks_apply"].flops += 1024.0 * info[3];
       }
       std::vector<int32_t> order(m);
       std::iota(order.begin(), order.end(), 0);
