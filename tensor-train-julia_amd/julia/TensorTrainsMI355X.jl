@@ -1,0 +1,143 @@
+"""
+    TensorTrainsMI355X
+
+Julia binding of libttround (include/ttround.h): the MI355X rounding path of
+TensorTrains behind `ccall`.  Loading this module after `TensorTrains` adds
+GPU methods with the reference's names and keyword defaults
+(src/tt_rounding.jl:146-167, src/tt_randtools.jl:112-283,
+src/sketches.jl:175-315, src/tt_operations.jl:81-112,170-194,252-258); the
+solver callers (tt_gmres, arnoldi, davidson, als/mals/dmrg via orthogonalize)
+need no change.  Every call uploads the cores (column-major (n, r, r') arrays,
+exactly TTvector.ttv_vec), runs on the GPU and copies the result back into new
+Julia arrays.
+
+Status codes map to the reference's exceptions:
+TTR_INVALID_ARG -> ArgumentError, TTR_DIM_MISMATCH -> DimensionMismatch,
+TTR_NUMERICAL_FAILURE -> LinearAlgebra.LAPACKException-like ErrorException,
+others -> ErrorException with ttr_last_error.
+"""
+module TensorTrainsMI355X
+
+using TensorTrains
+import TensorTrains: tt_rounding, tt_rounding!, orthogonalize, ttrand_rounding, tt_recursive_sketch
+
+const lib = get(ENV, "TTROUND_LIB", joinpath(@__DIR__, "..", "ttround", "libttround.so"))
+
+const TTR_OK = 0
+const _handle = Ref{Ptr{Cvoid}}(C_NULL)
+
+function handle()
+    if _handle[] == C_NULL
+        h = Ref{Ptr{Cvoid}}(C_NULL)
+        st = ccall((:ttr_create, lib), Cint, (Ref{Ptr{Cvoid}}, Cint), h, 0)
+        st == TTR_OK || error("ttr_create failed with status $st")
+        _handle[] = h[]
+        atexit(() -> ccall((:ttr_destroy, lib), Cint, (Ptr{Cvoid},), _handle[]))
+    end
+    return _handle[]
+end
+
+function check(st)
+    st == TTR_OK && return
+    msg = unsafe_string(ccall((:ttr_last_error, lib), Cstring, (Ptr{Cvoid},), handle()))
+    st == 1 && throw(ArgumentError(msg))
+    st == 2 && throw(DimensionMismatch(msg))
+    error("libttround status $st: $msg")
+end
+
+function upload(x::TTvector{Float64,N}) where {N}
+    id = Ref{Int64}(0)
+    ptrs = [pointer(c) for c in x.ttv_vec]
+    GC.@preserve x begin
+        check(ccall((:ttr_tt_upload, lib), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Ptr{Float64}}, Ref{Int64}),
+                    handle(), N, collect(Int64, x.ttv_dims), x.ttv_rks, x.ttv_ot, ptrs, id))
+    end
+    return id[]
+end
+
+function download(id::Int64, ::Val{N}) where {N}
+    dims = zeros(Int64, N); rks = zeros(Int64, N + 1); ot = zeros(Int64, N)
+    check(ccall((:ttr_tt_info, lib), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}),
+                handle(), id, C_NULL, dims, rks, ot))
+    vec = Vector{Array{Float64,3}}(undef, N)
+    for k in 1:N
+        vec[k] = Array{Float64,3}(undef, dims[k], rks[k], rks[k+1])
+        check(ccall((:ttr_tt_download_core, lib), Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}),
+                    handle(), id, k - 1, vec[k]))
+    end
+    check(ccall((:ttr_tt_free, lib), Cint, (Ptr{Cvoid}, Int64), handle(), id))
+    return TTvector{Float64,N}(N, vec, Tuple(dims), rks, ot)
+end
+
+free(id) = check(ccall((:ttr_tt_free, lib), Cint, (Ptr{Cvoid}, Int64), handle(), id))
+
+# ---------------------------------------------------------------- deterministic
+
+function tt_rounding(x::TTvector{Float64,N}; tol=1e-12, rmax=2^14, direction=:left) where {N}
+    direction in (:left, :right) || error("Invalid direction. Use direction=:right or direction=:left")
+    id = upload(x); out = Ref{Int64}(0)
+    check(ccall((:ttr_round_svd, lib), Cint, (Ptr{Cvoid}, Int64, Ref{Int64}, Cdouble, Int64, Cint, Ref{Int64}),
+                handle(), 1, id, tol, rmax, direction == :left ? 0 : 1, out))
+    free(id)
+    return download(out[], Val(N))
+end
+
+function tt_rounding!(x::TTvector{Float64,N}; tol=1e-12, rmax=2^14, direction=:left) where {N}
+    y = tt_rounding(x; tol=tol, rmax=rmax, direction=direction)
+    copyto!(x.ttv_vec, y.ttv_vec); copyto!(x.ttv_rks, y.ttv_rks); copyto!(x.ttv_ot, y.ttv_ot)
+    return
+end
+
+function orthogonalize(x::TTvector{Float64,N}; i=1::Int) where {N}
+    id = upload(x); out = Ref{Int64}(0)
+    check(ccall((:ttr_orthogonalize, lib), Cint, (Ptr{Cvoid}, Int64, Ref{Int64}, Int64, Ref{Int64}),
+                handle(), 1, id, i, out))
+    free(id)
+    return download(out[], Val(N))
+end
+
+# ---------------------------------------------------------------- randomized
+
+struct RandItem
+    nterms::Int64
+    alpha::Ptr{Float64}
+    term_ids::Ptr{Int64}
+    op_id::Int64
+    rks::Ptr{Int64}
+end
+
+struct RandOpts
+    orthogonal::Int32
+    sketch_prec::Int32
+    block_rks::Int64
+    seed::UInt64
+    p::Ptr{Int64}
+    oversampling::Float64
+end
+
+const PREC = Dict(Float64 => Int32(0), Float32 => Int32(1))
+
+function ttrand_rounding(α::Vector{Float64}, y::Vector{TTvector{Float64,N}}, rks=TensorTrains.default_rank_heuristic(y);
+                         orthogonal=true, seed=1234, block_rks::Int=N, sketch_type::Type=Float64) where {N}
+    ids = [upload(t) for t in y]
+    r = collect(Int64, rks)
+    out = Ref{Int64}(0)
+    GC.@preserve α ids r begin
+        item = Ref(RandItem(length(y), pointer(α), pointer(ids), -1, pointer(r)))
+        opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[sketch_type], block_rks, UInt64(seed), C_NULL, 1.0))
+        check(ccall((:ttr_round_rand, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, Ref{Int64}),
+                    handle(), 1, item, opts, out))
+    end
+    foreach(free, ids)
+    return download(out[], Val(N))
+end
+
+ttrand_rounding(y::TTvector{Float64,N}, rks=TensorTrains.default_rank_heuristic(y); kwargs...) where {N} =
+    ttrand_rounding([1.0], [y], rks; kwargs...)
+ttrand_rounding(y::TTvector{Float64,N}, rmax::Int; kwargs...) where {N} =
+    ttrand_rounding(y, vcat(1, fill(rmax, N - 1), 1); kwargs...)
+ttrand_rounding(α::Vector{Float64}, y::Vector{TTvector{Float64,N}}, rmax::Int; kwargs...) where {N} =
+    ttrand_rounding(α, y, vcat(1, fill(rmax, N - 1), 1); kwargs...)
+
+end # module
