@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--cleanup-tol", type=float, default=1e-6)
     ap.add_argument("--det-items", type=int, default=2, help="items for the FP64 deterministic comparison (0: skip)")
     ap.add_argument("--cpu-items", type=int, default=1, help="items for the oracle CPU baseline / parity (0: skip)")
+    ap.add_argument("--no-events", action="store_true",
+                    help="do not record the library's per-launch hipEvents (for runs under an external profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -108,7 +110,7 @@ def main():
         for o in outs:
             o.free()
 
-    h.profile(True)
+    h.profile(not args.no_events)
     h.profile_reset()
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -126,12 +128,10 @@ def main():
     sweeps = stats.pop("jacobi_svd:sweeps", None)
     jacobi = ({"svds": sweeps["launches"], "mean_sweeps": round(sweeps["flops"] / max(sweeps["launches"], 1), 2),
                "max_sweeps": sweeps["bytes"]} if sweeps else None)
-    if jacobi:
-        for ph in ("gram", "inner", "apply"):
-            st = stats.pop("jacobi_svd:ticks_" + ph, None)
-            if st:
-                jacobi["ticks_" + ph] = st["flops"]
-    kern = {k: v for k, v in stats.items() if not k.startswith("phase:")}
+    sites = {k[5:]: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] // args.steps,
+                     "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3)}
+             for k, v in stats.items() if k.startswith("site:")}
+    kern = {k: v for k, v in stats.items() if not k.startswith("phase:") and not k.startswith("site:")}
     phases = {k[6:]: round(v["ms"] / args.steps, 3) for k, v in stats.items() if k.startswith("phase:")}
     dom = max(kern, key=lambda k: kern[k]["ms"]) if kern else None
     roof = None
@@ -150,6 +150,11 @@ def main():
                 "launches_per_step": st["launches"] / args.steps,
                 "share_of_kernel_time": round(st["ms"] / sum(v["ms"] for v in kern.values()), 4)}
 
+    if jacobi:
+        for ph in ("visits", "ticks_gram", "ticks_inner", "ticks_apply"):
+            st = stats.pop("jacobi_svd:" + ph, None)
+            if st:
+                jacobi[ph] = st["flops"]
     kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] // args.steps,
                    "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3)} for k, v in kern.items()}
 
@@ -229,6 +234,7 @@ def main():
         "phases_ms_per_step": phases,
         "jacobi": jacobi,
         "kernels": kernels,
+        "gemm_sites": sites,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

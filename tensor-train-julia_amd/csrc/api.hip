@@ -186,7 +186,7 @@ std::vector<double> frob_norms_host(ttr_handle_s* h, const std::vector<const dou
     }
     gb.add(1, 1, int(counts[i]), 1.0, ptrs[i], 1, 1, ptrs[i], 1, 1, 0.0, out + i, 1, 1);
   }
-  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "norm");
   std::vector<double> res(ptrs.size());
   ctx.sync();
   TTR_HIP(hipMemcpy(res.data(), out, res.size() * 8, hipMemcpyDeviceToHost));
@@ -247,7 +247,7 @@ std::vector<std::unique_ptr<DevTT>> orthogonalize_batch(ttr_handle_s* h, const s
       qj.push_back(QRJob{Abuf[q], rp * n, int(rp * n), int(R1), outs[q]->cores[j], rp * n});
     }
     copy2d_batched(ctx, cp);
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "orth_lr");
     qr_batched(ctx, qj);
     cp.clear();
     for (size_t q = 0; q < nb; ++q) {
@@ -272,7 +272,7 @@ std::vector<std::unique_ptr<DevTT>> orthogonalize_batch(ttr_handle_s* h, const s
                ldT);
       qj.push_back(QRJob{Abuf[q], ldT, int(ldT), int(R0), Qw[q], ldT});
     }
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "orth_rl");
     qr_batched(ctx, qj);
     std::vector<Copy2D> cp;
     for (size_t q = 0; q < nb; ++q) {
@@ -296,7 +296,7 @@ std::vector<std::unique_ptr<DevTT>> orthogonalize_batch(ttr_handle_s* h, const s
     gb.add(int(r0), int(n * R1), int(R0), 1.0, fr, 1, (i0 == 0) ? 1 : r0, x->cores[i0], 1, R0, 0.0, Abuf[q], 1, r0);
     (void)fl;
   }
-  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "orth_root");
   for (size_t q = 0; q < nb; ++q) {
     const DevTT* x = xs[q];
     const int i0 = root - 1;
@@ -305,7 +305,7 @@ std::vector<std::unique_ptr<DevTT>> orthogonalize_batch(ttr_handle_s* h, const s
     const int64_t ldfl = (i0 == x->d - 1) ? 1 : R1;
     gb.add(int(r0 * n), int(r1), int(R1), 1.0, Abuf[q], 1, r0 * n, fl, 1, ldfl, 0.0, outs[q]->cores[i0], 1, r0 * n);
   }
-  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "orth_root");
   return outs;
 }
 
@@ -359,9 +359,11 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     s.J = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
     s.sv = ctx.ws.alloc_n<double>(size_t(mxp));
     s.idx = ctx.ws.alloc_n<int32_t>(size_t(mxp));
-    s.info = ctx.ws.alloc_n<int32_t>(4);
     s.out = h->make_tt_lazy(s.y->dims);
   }
+  // Jacobi counters of all items in one array (zeroed before every SVD step)
+  int32_t* info_all = ctx.ws.alloc_n<int32_t>(size_t(8) * nb);
+  for (size_t q = 0; q < nb; ++q) st[q].info = info_all + 8 * q;
   GemmBatch gb;
   const bool left = direction == TTR_DIR_LEFT;
   const int nsteps = maxd - 1;
@@ -417,6 +419,7 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       sj.push_back(SVDJob{s.B, m, s.J, m, s.sv, int(m), s.info});
     }
     copy2d_batched(ctx, cp);
+    zero_fill(ctx, info_all, sizeof(int32_t) * 8 * nb);
     jacobi_svd_batched(ctx, sj);
     // rank decisions on the host
     ctx.sync();
@@ -429,16 +432,17 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       std::vector<double> sv(m);
       TTR_HIP(hipMemcpy(sv.data(), s.sv, size_t(m) * 8, hipMemcpyDeviceToHost));
       if (ctx.prof.on) {
-        int32_t info[4] = {0, 0, 0, 0};
-        TTR_HIP(hipMemcpy(info, s.info, 16, hipMemcpyDeviceToHost));
+        uint32_t info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        TTR_HIP(hipMemcpy(info, s.info, sizeof(info), hipMemcpyDeviceToHost));
         ProfStat& ps = ctx.prof.stats["jacobi_svd:sweeps"];
         ps.launches += 1;
         ps.flops += info[0];
         ps.bytes = std::max(ps.bytes, double(info[0]));
-        // wall-clock ticks (x1024) spent in the Gram, inner-iteration and apply phases
-        ctx.prof.stats["jacobi_svd:ticks_gram"].flops += 1024.0 * info[1];
-        ctx.prof.stats["jacobi_svd:ticks_inner"].flops += 1024.0 * info[2];
-        ctx.prof.stats["jacobi_svd:ticks_apply"].flops += 1024.0 * info[3];
+        // visits and wall-clock ticks (100 MHz) summed over visits: Gram, inner sweep, apply
+        ctx.prof.stats["jacobi_svd:visits"].flops += info[4];
+        ctx.prof.stats["jacobi_svd:ticks_gram"].flops += info[1];
+        ctx.prof.stats["jacobi_svd:ticks_inner"].flops += info[2];
+        ctx.prof.stats["jacobi_svd:ticks_apply"].flops += info[3];
       }
       std::vector<int32_t> order(m);
       std::iota(order.begin(), order.end(), 0);
@@ -516,7 +520,7 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       }
     }
     copy2d_batched(ctx, cpo);
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "svd_sweep_a");
     // second-stage products of the tall-left / wide-right cases
     for (size_t q = 0; q < nb; ++q) {
       if (jcore[q] < 0) continue;
@@ -535,7 +539,7 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
         gb.add(int(k), int(nr), int(qq), 1.0, s.Mt, 1, k, s.cur[jr], 1, qq, 0.0, dst, 1, k);
       }
     }
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "svd_sweep_b");
     for (size_t q = 0; q < nb; ++q) {
       if (jcore[q] < 0) continue;
       SweepState& s = st[q];
@@ -704,7 +708,7 @@ void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items,
             gb.add(int(Rk), int(bvl), int(n), 1.0, t.x->cores[k], 1, Rk, S + size_t(n * bvl * j) * eb, 1, n, 0.0,
                    static_cast<char*>(t.W[k]) + size_t(Rk * bvl * j) * eb, 1, Rk);
         }
-      gemm_run(ctx, gb, Prec::F64, sp.prec, sp.prec, sp.prec);
+      gemm_run(ctx, gb, Prec::F64, sp.prec, sp.prec, sp.prec, "sketch_last");
     } else {
       for (size_t i = 0; i < items.size(); ++i)
         for (size_t ti = 0; ti < items[i].terms.size(); ++ti) {
@@ -713,7 +717,7 @@ void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items,
           gb.add(int(Rk * n), int(bvr * p), int(Rk1), 1.0, t.x->cores[k], 1, Rk * n, t.W[k + 1], 1, Rk1, 0.0,
                  T1[i][ti], 1, Rk * n);
         }
-      gemm_run(ctx, gb, Prec::F64, sp.prec, sp.prec, sp.prec);
+      gemm_run(ctx, gb, Prec::F64, sp.prec, sp.prec, sp.prec, "sketch_t1");
       for (size_t i = 0; i < items.size(); ++i)
         for (size_t ti = 0; ti < items[i].terms.size(); ++ti) {
           RandTerm& t = items[i].terms[ti];
@@ -723,7 +727,7 @@ void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items,
                    1, Rk, S + size_t(n * bvr * bvl * j) * eb, 1, n * bvr, 0.0,
                    static_cast<char*>(t.W[k]) + size_t(Rk * bvl * j) * eb, 1, Rk);
         }
-      gemm_run(ctx, gb, sp.prec, sp.prec, sp.prec, sp.prec);
+      gemm_run(ctx, gb, sp.prec, sp.prec, sp.prec, sp.prec, "sketch_w");
     }
     std::vector<Pow2Job> pj;
     for (auto& it : items)
@@ -786,7 +790,7 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
       gb.prob(outs[i]->cores[0], 1, 1, 1, int(sp.dims[0]), 1.0, 0.0);
       for (auto& t : items[i].terms) gb.seg(h->one, 1, 1, t.x->cores[0], 1, 1, 1, t.alpha);
     }
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "rr_d1");
     return outs;
   }
   {
@@ -824,7 +828,7 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
         gb.seg(Y, 1, r * n, t.W[k + 1], 1, R1, int(R1), k == 0 ? t.alpha : 1.0, AM_PLAIN, t.E + k + 1);
       }
     }
-    gemm_run(ctx, gb, Prec::F64, sp.prec, Prec::F64, Prec::F64);
+    gemm_run(ctx, gb, Prec::F64, sp.prec, Prec::F64, Prec::F64, "rr_Z");
     std::vector<QRJob> qj;
     for (size_t i = 0; i < items.size(); ++i) {
       const int64_t r = orr[i][k], s = items[i].s[k + 1];
@@ -841,7 +845,7 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
         gb.seg(outs[i]->cores[k], r * n, 1, Y, 1, r * n, int(r * n), k == 0 ? t.alpha : 1.0);
       }
     }
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "rr_P");
     // Y_t <- P_t X_{t,k+1}; the last core sums the terms
     const int64_t n1 = sp.dims[k + 1];
     for (size_t i = 0; i < items.size(); ++i) {
@@ -859,7 +863,7 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
         }
       }
     }
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "rr_Y");
     yb ^= 1;
   }
   return outs;

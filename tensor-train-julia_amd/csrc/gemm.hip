@@ -88,8 +88,9 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
     if (S.exp) scale = ldexp(scale, S.exp[0] - emax);
     const bool a_row_fast = (S.a_rs == 1);
     const bool b_row_fast = (S.b_rs == 1);
-    for (int k0 = 0; k0 < S.K; k0 += BK) {
-      // A tile BM x BK
+    // the k-tile k0 + BK is loaded into registers while the MFMAs of tile k0 run
+    double ra[4], rb[4];
+    auto load_tiles = [&](int k0) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         int i, l;
@@ -102,20 +103,35 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
                   ? load_d<TA>(S.A, gi * S.a_rs + gl * S.a_cs) : 0.0;
           v = amode_value(S.a_mode, gi, gl, v) * scale;
         }
-        As[l][i] = MT(v);
+        ra[q] = v;
       }
-      // B tile BK x BN
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         int l, j;
         if (b_row_fast) { l = tid & 15; j = (tid >> 4) + 16 * q; }
         else { j = tid & 63; l = (tid >> 6) + 4 * q; }
         const int64_t gl = k0 + l, gj = col0 + j;
-        double v = 0.0;
-        if (gl < S.K && gj < P.N) v = load_d<TB>(S.B, gl * S.b_rs + gj * S.b_cs);
-        Bs[l][j] = MT(v);
+        rb[q] = (gl < S.K && gj < P.N) ? load_d<TB>(S.B, gl * S.b_rs + gj * S.b_cs) : 0.0;
+      }
+    };
+    load_tiles(0);
+    for (int k0 = 0; k0 < S.K; k0 += BK) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int i, l;
+        if (a_row_fast) { i = tid & 63; l = (tid >> 6) + 4 * q; }
+        else { i = (tid >> 4) + 16 * q; l = tid & 15; }
+        As[l][i] = MT(ra[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int l, j;
+        if (b_row_fast) { l = tid & 15; j = (tid >> 4) + 16 * q; }
+        else { j = tid & 63; l = (tid >> 6) + 4 * q; }
+        Bs[l][j] = MT(rb[q]);
       }
       __syncthreads();
+      if (k0 + BK < S.K) load_tiles(k0 + BK);
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         MT a0 = As[kk + (lane >> 4)][wm * 32 + (lane & 15)];
@@ -304,7 +320,7 @@ void launch_b(Ctx& ctx, Prec tb, Prec tc, Prec math, int ntiles, const GemmProb*
 }
 }  // namespace
 
-void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math) {
+void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, const char* site) {
   // drop empty problems, assign tile prefixes
   std::vector<GemmProb> probs;
   probs.reserve(b.probs_.size());
@@ -338,7 +354,8 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math) {
     }
   }
   static const char* names[3] = {"gemm_f64", "gemm_f32", "gemm_bf16"};
-  ProfScope ps(ctx, names[int(math)], flops, bytes);
+  const std::string site_name = site ? std::string("site:") + site : std::string();
+  ProfScope ps(ctx, names[int(math)], flops, bytes, site ? site_name.c_str() : nullptr);
   if (ta == Prec::F64) launch_b<double>(ctx, tb, tc, math, int(ntiles), dp, np, ds);
   else if (ta == Prec::F32) launch_b<float>(ctx, tb, tc, math, int(ntiles), dp, np, ds);
   else launch_b<bf16s>(ctx, tb, tc, math, int(ntiles), dp, np, ds);

@@ -94,8 +94,8 @@ __global__ __launch_bounds__(256) void copy2d_kernel(const Copy2DDev* __restrict
     const int jj = in_i_fast ? ty + 8 * q : tx;
     const int gi = i0 + ii, gj = j0 + jj;
     if (gi < J.M && gj < J.N) {
-      const bool zero = (J.tri == 1 && gi > gj) || (J.tri == 2 && gi < gj);
-      tile[ii][jj] = zero ? 0.0 : J.alpha * in[gi * J.irs + gj * J.ics];
+      const bool zero = (J.tri == 1 && gi > gj) || ((J.tri == 2 || J.tri == 3) && gi < gj);
+      tile[ii][jj] = zero ? 0.0 : (J.tri == 3 && gi == gj) ? 1.0 : J.alpha * in[gi * J.irs + gj * J.ics];
     }
   }
   __syncthreads();

@@ -134,9 +134,9 @@ void CachingAlloc::release_cached() {
 }
 
 Profiler::~Profiler() {
-  for (auto& p : pending_) {
-    pool_.push_back(p.a);
-    pool_.push_back(p.b);
+  for (auto& kv : pending_) {
+    pool_.push_back(kv.second.a);
+    pool_.push_back(kv.second.b);
   }
   for (auto e : pool_) (void)hipEventDestroy(e);
 }
@@ -152,34 +152,51 @@ hipEvent_t Profiler::get_event() {
   return e;
 }
 
-int Profiler::start(hipStream_t s, const char* name, double flops, double bytes) {
-  Pending p{get_event(), get_event(), name, flops, bytes};
+int Profiler::start(hipStream_t s, const char* name, double flops, double bytes, const char* name2) {
+  // bound the number of outstanding event records: drain the closed ones first
+  if (stopped_ >= 1024) {
+    TTR_HIP(hipStreamSynchronize(s));
+    collect();
+  }
+  Pending p{get_event(), get_event(), name, name2 ? name2 : "", flops, bytes, false};
   TTR_HIP(hipEventRecord(p.a, s));
-  pending_.push_back(p);
-  return int(pending_.size()) - 1;
+  pending_.emplace(next_, p);
+  return next_++;
 }
 
 void Profiler::stop(hipStream_t s, int token) {
-  if (token < 0 || token >= int(pending_.size())) return;
-  (void)hipEventRecord(pending_[token].b, s);
+  auto it = pending_.find(token);
+  if (it == pending_.end() || it->second.stopped) return;
+  (void)hipEventRecord(it->second.b, s);
+  it->second.stopped = true;
+  ++stopped_;
 }
 
 void Profiler::collect() {
-  for (auto& p : pending_) {
+  for (auto it = pending_.begin(); it != pending_.end();) {
+    Pending& p = it->second;
+    if (!p.stopped) {
+      ++it;
+      continue;
+    }
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
-      ProfStat& st = stats[p.name];
-      st.ms += ms;
-      st.launches += 1;
-      st.flops += p.flops;
-      st.bytes += p.bytes;
+      for (const std::string* nm : {&p.name, &p.name2}) {
+        if (nm->empty()) continue;
+        ProfStat& st = stats[*nm];
+        st.ms += ms;
+        st.launches += 1;
+        st.flops += p.flops;
+        st.bytes += p.bytes;
+      }
     } else {
       (void)hipGetLastError();
     }
     pool_.push_back(p.a);
     pool_.push_back(p.b);
+    it = pending_.erase(it);
   }
-  pending_.clear();
+  stopped_ = 0;
 }
 
 }  // namespace ttr

@@ -1,12 +1,18 @@
 // Batched FP64 Householder QR with explicit thin Q.
 //
-// Blocked right-looking Householder (panel width 32, compact WY
-// I - V T V^T), following the conventions of LAPACK dgeqrf/dlarfg/dorgqr so
-// that Q agrees with the reference's LAPACK factorisations column by column:
+// Two-level blocked right-looking Householder in compact WY form
+// (I - V T V^T), following the conventions of LAPACK dgeqrf/dlarfg/dlarft/
+// dorgqr so that Q agrees with the reference's LAPACK factorisations column by
+// column:
 //   beta = -sign(alpha) * ||(alpha, x)||,  tau = (beta - alpha)/beta,
 //   v = (1, x/(alpha - beta)),  tau = 0 when x == 0.
-// The panel is factorised by one workgroup per matrix; the trailing update and
-// the formation of Q are grouped FP64 MFMA GEMMs over every matrix of the batch.
+// Columns are processed in blocks of 64.  Inside a block, panels of 16 (32 for
+// very tall matrices) are factorised by one workgroup per matrix and applied to
+// the rest of the block; the block's reflectors are then merged into one
+// 64-wide compact WY factor (T from V^T V and tau, as dlarft), which updates the
+// trailing matrix and, backwards, forms Q.  Every product is a grouped FP64
+// MFMA GEMM over all matrices of the batch, so the trailing matrix is streamed
+// once per 64 columns.
 #include "ttr_internal.h"
 
 namespace ttr {
@@ -254,6 +260,41 @@ __global__ __launch_bounds__(RT) void qr_panel_reg_kernel(const PanelJob* __rest
   }
 }
 
+// T of a block of nb <= 64 reflectors from G = V^T V (64 x 64, ld 64) and tau:
+// T(0:j, j) = -tau_j T(0:j, 0:j) G(0:j, j), T(j, j) = tau_j (dlarft, forward,
+// columnwise).  One 64-thread workgroup per matrix.
+constexpr int NBO = 64;
+
+struct TBuildJob {
+  const double* G;
+  const double* tau;
+  double* T;  // 64 x 64, ld 64
+  int32_t nb;
+};
+
+__global__ __launch_bounds__(NBO) void build_t_kernel(const TBuildJob* __restrict__ jobs) {
+  const TBuildJob J = jobs[blockIdx.x];
+  const int t = threadIdx.x, nb = J.nb;
+  __shared__ double Gs[NBO][NBO + 1];
+  __shared__ double Ts[NBO][NBO + 1];
+  for (int c = 0; c < NBO; ++c) {
+    Gs[t][c] = (t < nb && c < nb) ? J.G[t + NBO * c] : 0.0;
+    Ts[t][c] = 0.0;
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    const double tau = J.tau[j];
+    if (t < j) {
+      double s = 0.0;
+      for (int q = t; q < j; ++q) s += Ts[t][q] * Gs[q][j];
+      Ts[t][j] = -tau * s;
+    }
+    if (t == j) Ts[j][j] = tau;
+    __syncthreads();
+  }
+  for (int c = 0; c < NBO; ++c) J.T[t + NBO * c] = Ts[t][c];
+}
+
 void launch_panel(Ctx& ctx, const PanelJob* d, int njobs, int max_rows, bool reg) {
   if (!reg) {
     hipLaunchKernelGGL(qr_panel_kernel, dim3(unsigned(njobs)), dim3(PT), 0, ctx.stream, d);
@@ -277,14 +318,16 @@ void launch_panel(Ctx& ctx, const PanelJob* d, int njobs, int max_rows, bool reg
 void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
   struct Aux {
     double* tau;
-    double* T;   // npanel * NBP * NBP
-    double* W;   // NBP x n
-    double* VT;  // m x NBP
-    int kq, npanel;
+    double* T;   // inner panels: npanel * NBP * NBP
+    double* TO;  // blocks: nblock * NBO * NBO
+    double* G;   // NBO x NBO
+    double* W;   // NBO x n
+    double* VT;  // m x NBO
+    double* VX;  // m x NBO: the block's V with explicit unit diagonal and zeros above
+    int kq, nblock;
   };
   std::vector<Aux> aux(jobs.size());
-  int max_panels = 0;
-  int max_m = 0;
+  int max_blocks = 0, max_m = 0;
   for (const auto& J : jobs) max_m = std::max(max_m, J.m);
   // register-resident panels (width 16) up to 6*512 rows, else the streaming panel (width 32)
   const bool reg = max_m <= 6 * RT;
@@ -294,83 +337,143 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     TTR_REQUIRE(J.m >= 0 && J.n >= 0 && J.lda >= std::max(1, J.m), ST_INVALID_ARG, "qr: bad shape");
     Aux& a = aux[q];
     a.kq = std::min(J.m, J.n);
-    a.npanel = (a.kq + NB - 1) / NB;
-    max_panels = std::max(max_panels, a.npanel);
+    a.nblock = (a.kq + NBO - 1) / NBO;
+    max_blocks = std::max(max_blocks, a.nblock);
+    const int npanel = (a.kq + NB - 1) / NB;
     a.tau = ctx.ws.alloc_n<double>(std::max(1, J.n));
-    a.T = ctx.ws.alloc_n<double>(size_t(std::max(1, a.npanel)) * NBP * NBP);
-    a.W = ctx.ws.alloc_n<double>(size_t(NBP) * std::max(1, J.n));
-    a.VT = ctx.ws.alloc_n<double>(size_t(std::max(1, J.m)) * NBP);
+    a.T = ctx.ws.alloc_n<double>(size_t(std::max(1, npanel)) * NBP * NBP);
+    a.TO = ctx.ws.alloc_n<double>(size_t(std::max(1, a.nblock)) * NBO * NBO);
+    a.G = ctx.ws.alloc_n<double>(size_t(NBO) * NBO);
+    a.W = ctx.ws.alloc_n<double>(size_t(NBO) * std::max(1, J.n));
+    a.VT = ctx.ws.alloc_n<double>(size_t(std::max(1, J.m)) * NBO);
+    a.VX = ctx.ws.alloc_n<double>(size_t(std::max(1, J.m)) * NBO);
   }
   GemmBatch gb;
-  // ---- factorisation
-  for (int p = 0; p < max_panels; ++p) {
-    std::vector<PanelJob> pj;
+  auto block_cols = [&](const Aux& a, int b) { return std::min(NBO, a.kq - b * NBO); };
+  // explicit V of block b (rows b*NBO.., ld m2) for every job that has it
+  auto copy_vx = [&](int b) {
+    std::vector<Copy2D> cp;
     for (size_t q = 0; q < jobs.size(); ++q) {
+      const QRJob& J = jobs[q];
       const Aux& a = aux[q];
-      if (p >= a.npanel) continue;
-      const int c0 = p * NB, nb = std::min(NB, a.kq - c0);
-      pj.push_back(PanelJob{jobs[q].A, jobs[q].lda, jobs[q].m, c0, nb, a.tau, a.T + size_t(p) * NBP * NBP});
+      if (b >= a.nblock) continue;
+      const int c0 = b * NBO, m2 = J.m - c0;
+      cp.push_back(copy_job(J.A + c0 + J.lda * c0, 1, J.lda, a.VX, 1, m2, m2, block_cols(a, b), 1.0, 3));
     }
-    if (pj.empty()) continue;
-    const PanelJob* d = ctx.upload(pj);
-    double pfl = 0.0, pby = 0.0;
-    for (const auto& j : pj) {
-      pfl += 4.0 * double(j.m - j.c0) * j.nb * j.nb;  // Householder panel (2 m nb^2) + T
-      pby += 16.0 * double(j.m - j.c0) * j.nb;
+    copy2d_batched(ctx, cp);
+  };
+  // ---- factorisation
+  for (int b = 0; b < max_blocks; ++b) {
+    for (int ip = 0; ip < NBO / NB; ++ip) {
+      std::vector<PanelJob> pj;
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const Aux& a = aux[q];
+        if (b >= a.nblock) continue;
+        const int c = b * NBO + ip * NB, cend = b * NBO + block_cols(a, b);
+        if (c >= cend) continue;
+        pj.push_back(PanelJob{jobs[q].A, jobs[q].lda, jobs[q].m, c, std::min(NB, cend - c), a.tau,
+                              a.T + size_t(c / NB) * NBP * NBP});
+      }
+      if (pj.empty()) continue;
+      const PanelJob* d = ctx.upload(pj);
+      double pfl = 0.0, pby = 0.0;
+      int max_rows = 0;
+      for (const auto& j : pj) {
+        pfl += 4.0 * double(j.m - j.c0) * j.nb * j.nb;  // Householder panel (2 m nb^2) + T
+        pby += 16.0 * double(j.m - j.c0) * j.nb;
+        max_rows = std::max(max_rows, j.m - j.c0);
+      }
+      {
+        ProfScope ps(ctx, "qr_panel", pfl, pby);
+        launch_panel(ctx, d, int(pj.size()), max_rows, reg);
+      }
+      // apply the panel to the rest of its block: A2 -= (V T^T) (V^T A2)
+      for (int stage = 0; stage < 3; ++stage) {
+        for (size_t q = 0; q < jobs.size(); ++q) {
+          const QRJob& J = jobs[q];
+          const Aux& a = aux[q];
+          if (b >= a.nblock) continue;
+          const int c = b * NBO + ip * NB, cend = b * NBO + block_cols(a, b);
+          if (c >= cend) continue;
+          const int nb = std::min(NB, cend - c);
+          const int m2 = J.m - c, n2 = cend - c - nb;
+          if (n2 <= 0) continue;
+          double* V = J.A + c + J.lda * c;
+          double* A2 = J.A + c + J.lda * (c + nb);
+          const double* T = a.T + size_t(c / NB) * NBP * NBP;
+          if (stage == 0)  // W = V^T A2  (nb x n2)
+            gb.add(nb, n2, m2, 1.0, V, J.lda, 1, A2, 1, J.lda, 0.0, a.W, 1, NBO, AM_VT_UNIT);
+          else if (stage == 1)  // VT = V T^T  (m2 x nb)
+            gb.add(m2, nb, nb, 1.0, V, 1, J.lda, T, NBP, 1, 0.0, a.VT, 1, m2, AM_V_UNIT);
+          else  // A2 -= VT W
+            gb.add(m2, n2, nb, -1.0, a.VT, 1, m2, a.W, 1, NBO, 1.0, A2, 1, J.lda);
+        }
+        gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_panel_upd");
+      }
     }
-    int max_rows = 0;
-    for (const auto& j : pj) max_rows = std::max(max_rows, j.m - j.c0);
-    {
-      ProfScope ps(ctx, "qr_panel", pfl, pby);
-      launch_panel(ctx, d, int(pj.size()), max_rows, reg);
+    // merge the block's reflectors: G = V^T V, T from (G, tau)
+    copy_vx(b);
+    std::vector<TBuildJob> tb;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const QRJob& J = jobs[q];
+      const Aux& a = aux[q];
+      if (b >= a.nblock) continue;
+      const int c0 = b * NBO, m2 = J.m - c0, nbo = block_cols(a, b);
+      gb.add(nbo, nbo, m2, 1.0, a.VX, m2, 1, a.VX, 1, m2, 0.0, a.G, 1, NBO);
+      tb.push_back(TBuildJob{a.G, a.tau + c0, a.TO + size_t(b) * NBO * NBO, nbo});
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
+    if (!tb.empty()) {
+      const TBuildJob* d = ctx.upload(tb);
+      hipLaunchKernelGGL(build_t_kernel, dim3(unsigned(tb.size())), dim3(NBO), 0, ctx.stream, d);
+      TTR_HIP(hipGetLastError());
     }
     // trailing update A2 -= (V T^T) (V^T A2)
     for (int stage = 0; stage < 3; ++stage) {
       for (size_t q = 0; q < jobs.size(); ++q) {
         const QRJob& J = jobs[q];
         const Aux& a = aux[q];
-        if (p >= a.npanel) continue;
-        const int c0 = p * NB, nb = std::min(NB, a.kq - c0);
-        const int m2 = J.m - c0, n2 = J.n - c0 - nb;
+        if (b >= a.nblock) continue;
+        const int c0 = b * NBO, nbo = block_cols(a, b);
+        const int m2 = J.m - c0, n2 = J.n - c0 - nbo;
         if (n2 <= 0) continue;
-        double* V = J.A + c0 + J.lda * c0;
-        double* A2 = J.A + c0 + J.lda * (c0 + nb);
-        const double* T = a.T + size_t(p) * NBP * NBP;
-        if (stage == 0)  // W = V^T A2  (nb x n2)
-          gb.add(nb, n2, m2, 1.0, V, J.lda, 1, A2, 1, J.lda, 0.0, a.W, 1, NBP, AM_VT_UNIT);
-        else if (stage == 1)  // VT = V T^T  (m2 x nb)
-          gb.add(m2, nb, nb, 1.0, V, 1, J.lda, T, NBP, 1, 0.0, a.VT, 1, m2, AM_V_UNIT);
+        double* A2 = J.A + c0 + J.lda * (c0 + nbo);
+        const double* T = a.TO + size_t(b) * NBO * NBO;
+        if (stage == 0)  // W = V^T A2  (nbo x n2)
+          gb.add(nbo, n2, m2, 1.0, a.VX, m2, 1, A2, 1, J.lda, 0.0, a.W, 1, NBO);
+        else if (stage == 1)  // VT = V T^T  (m2 x nbo)
+          gb.add(m2, nbo, nbo, 1.0, a.VX, 1, m2, T, NBO, 1, 0.0, a.VT, 1, m2);
         else  // A2 -= VT W
-          gb.add(m2, n2, nb, -1.0, a.VT, 1, m2, a.W, 1, NBP, 1.0, A2, 1, J.lda);
+          gb.add(m2, n2, nbo, -1.0, a.VT, 1, m2, a.W, 1, NBO, 1.0, A2, 1, J.lda);
       }
-      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_trail");
     }
   }
-  // ---- thin Q by backward accumulation
+  // ---- thin Q by backward accumulation over the blocks: Q2 -= (V T) (V^T Q2)
   std::vector<Fill2D> fills;
   for (size_t q = 0; q < jobs.size(); ++q)
     if (jobs[q].Q && aux[q].kq > 0) fills.push_back(Fill2D{jobs[q].Q, jobs[q].ldq, jobs[q].m, aux[q].kq, 1.0});
   if (fills.empty()) return;
   fill2d_batched(ctx, fills);
-  for (int p = max_panels - 1; p >= 0; --p) {
+  for (int b = max_blocks - 1; b >= 0; --b) {
+    copy_vx(b);
     for (int stage = 0; stage < 3; ++stage) {
       for (size_t q = 0; q < jobs.size(); ++q) {
         const QRJob& J = jobs[q];
         const Aux& a = aux[q];
-        if (!J.Q || p >= a.npanel) continue;
-        const int c0 = p * NB, nb = std::min(NB, a.kq - c0);
+        if (!J.Q || b >= a.nblock) continue;
+        const int c0 = b * NBO, nbo = block_cols(a, b);
         const int m2 = J.m - c0, n2 = a.kq - c0;
-        double* V = J.A + c0 + J.lda * c0;
         double* Q2 = J.Q + c0 + J.ldq * c0;
-        const double* T = a.T + size_t(p) * NBP * NBP;
-        if (stage == 0)  // W = V^T Q2  (nb x n2)
-          gb.add(nb, n2, m2, 1.0, V, J.lda, 1, Q2, 1, J.ldq, 0.0, a.W, 1, NBP, AM_VT_UNIT);
-        else if (stage == 1)  // VT = V T  (m2 x nb)
-          gb.add(m2, nb, nb, 1.0, V, 1, J.lda, T, 1, NBP, 0.0, a.VT, 1, m2, AM_V_UNIT);
+        const double* T = a.TO + size_t(b) * NBO * NBO;
+        if (stage == 0)  // W = V^T Q2  (nbo x n2)
+          gb.add(nbo, n2, m2, 1.0, a.VX, m2, 1, Q2, 1, J.ldq, 0.0, a.W, 1, NBO);
+        else if (stage == 1)  // VT = V T  (m2 x nbo)
+          gb.add(m2, nbo, nbo, 1.0, a.VX, 1, m2, T, 1, NBO, 0.0, a.VT, 1, m2);
         else  // Q2 -= VT W
-          gb.add(m2, n2, nb, -1.0, a.VT, 1, m2, a.W, 1, NBP, 1.0, Q2, 1, J.ldq);
+          gb.add(m2, n2, nbo, -1.0, a.VT, 1, m2, a.W, 1, NBO, 1.0, Q2, 1, J.ldq);
       }
-      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64);
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_formq");
     }
   }
 }

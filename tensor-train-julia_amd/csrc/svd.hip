@@ -1,19 +1,24 @@
 // Batched one-sided (Hestenes) Jacobi SVD in FP64, block variant.
 //
-// One workgroup per matrix B (p x p).  Columns are grouped in blocks of 32; a
-// sweep visits every pair of blocks once in round-robin order.  A visit
+// Columns of each p x p matrix B are grouped in blocks of 32.  A sweep visits
+// every pair of blocks once in round-robin order: each round is one launch in
+// which every disjoint block pair of every matrix is one workgroup, so a large
+// matrix is worked on by many CUs at once and a batch of uneven matrices keeps
+// the whole GPU busy.  A visit
 //   1. forms the Gram matrix G = C^T C of the (up to) 64 columns C of the
 //      block pair in LDS with FP64 MFMA, straight from the current columns
-//      (so rotation angles always come from fresh inner products, as in plain
+//      (rotation angles always come from fresh inner products, as in plain
 //      one-sided Jacobi);
-//   2. if some pair is not yet orthogonal (|G_ij| > tol sqrt(G_ii G_jj),
-//      tol = p eps), runs a parallel cyclic Jacobi eigen-iteration on G in LDS
-//      (32 disjoint rotations per round, each 2x2 block of G and each row pair
-//      of V updated in place by one thread), accumulating the rotation V;
-//   3. applies V to the visited columns of B and of the accumulated right
-//      rotations J with FP64 MFMA (C <- C V, 32-row chunks staged in LDS).
-// The sweep loop ends when a whole sweep found every pair orthogonal.  On exit
-// B_out = B_in J has orthogonal columns, B_out[:,c] = u_c s_c.
+//   2. runs one parallel cyclic Jacobi sweep on G in LDS (32 disjoint rotations
+//      per round, round-robin order; each 2x2 block of G and each row pair of V
+//      is updated in place by one thread), accumulating the rotation V; a pair
+//      is rotated when |G_ij| > tol sqrt(G_ii G_jj), tol = p eps;
+//   3. if anything rotated, applies V to the visited columns of B and of the
+//      accumulated right rotations J with FP64 MFMA (C <- C V, 32-row chunks
+//      staged in LDS) and marks the matrix as not yet converged.
+// A matrix whose previous sweep rotated nothing is converged: its workgroups
+// return at once.  On exit B_out = B_in J has orthogonal columns,
+// B_out[:,c] = u_c s_c, and sv[c] = |B_out[:,c]|.
 #include "ttr_internal.h"
 
 namespace ttr {
@@ -22,244 +27,366 @@ namespace {
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-constexpr int JT = 512;         // threads per matrix (8 waves)
+constexpr int JT = 512;         // threads per visit (8 waves)
 constexpr int WB = 32;          // column block width
 constexpr int WC = 2 * WB;      // columns per visit
 constexpr int GLD = WC + 1;     // padded LDS row
 constexpr int RCH = 32;         // row chunk for Gram / apply
-constexpr int MAX_SWEEPS = 40;
-constexpr int MAX_INNER = 1;
 constexpr int NPAIR = WC / 2;
+constexpr int IR = WC - 1;      // rounds of one inner sweep
+constexpr int MAX_SWEEPS = 40;
 
-__device__ __forceinline__ int rr_player(int pos, int r, int pp) {
-  if (pos == 0) return 0;
-  return ((pos - 1 + r) % (pp - 1)) + 1;
+struct JacJob {
+  double* B;
+  int64_t ldb;
+  double* J;
+  int64_t ldj;
+  int32_t* info;
+  int p, nblk, pp, rounds, slot0;
+};
+
+__host__ __device__ __forceinline__ int rr_player(int pos, int r, int pp) {
+  return pos == 0 ? 0 : ((pos - 1 + r) % (pp - 1)) + 1;
 }
-
-__global__ __launch_bounds__(JT) void jacobi_block_kernel(const SVDJob* __restrict__ jobs) {
-  const SVDJob S = jobs[blockIdx.x];
-  const int p = S.p;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  __shared__ double Gbuf[2][WC][GLD];  // ping-pong copies of the Gram block
-  __shared__ double V[WC][GLD];
-  __shared__ double chunk[RCH][GLD];
-  __shared__ int cols[WC];  // global column of local index, -1 if padding
-
-  for (int64_t idx = tid; idx < int64_t(p) * p; idx += JT) {
-    const int64_t r = idx % p, c = idx / p;
-    S.J[r + c * S.ldj] = (r == c) ? 1.0 : 0.0;
+__global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __restrict__ jobs, int njobs, int round,
+                                                             int sweep, int32_t* __restrict__ flags) {
+  const int bid = blockIdx.x;
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].slot0 <= bid) lo = mid; else hi = mid - 1;
   }
-  // inner products of length p carry rounding noise of order sqrt(p)..p eps, so a
-  // pair counts as orthogonal below p eps (at least 16 eps)
-  const double tol = double(p > 16 ? p : 16) * 2.220446049250313e-16;
-  int sweeps_done = 0;
-  uint64_t t_gram = 0, t_inner = 0, t_apply = 0, t0 = 0;
-  const int nblk = (p + WB - 1) / WB;
-  const int pp = nblk + (nblk & 1);
+  const JacJob S = jobs[lo];
+  if (round >= S.rounds) return;
+  if (sweep > 0 && flags[((sweep - 1) & 1) * njobs + lo] == 0) return;  // converged
+  const int k = bid - S.slot0;
+  const int p = S.p;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  if (k == 0 && round == 0 && tid == 0 && S.info) S.info[0] = sweep + 1;
+  uint64_t t0 = 0, t1 = 0, t2 = 0;
+  if (tid == 0) t0 = wall_clock64();
+
+  int ba, bb;
+  if (S.nblk == 1) { ba = 0; bb = -1; }
+  else {
+    ba = rr_player(k, round, S.pp);
+    bb = rr_player(S.pp - 1 - k, round, S.pp);
+    if (ba >= S.nblk) ba = -1;
+    if (bb >= S.nblk) bb = -1;
+    if (ba < 0 && bb < 0) return;
+  }
+
+  // G doubles as the row-chunk staging buffer of the Gram and apply phases
+  __shared__ double G[WC][GLD];
+  __shared__ double V[WC][GLD];
+  __shared__ double pdiag[2][WC];   // diagonal of G for the next round's parameters
+  __shared__ double pab[2][NPAIR];  // G(a_l, b_l) of the next round's pairs
+  __shared__ int cols[WC];          // global column of local index, -1 if padding
+  double (*chunk)[GLD] = G;
+
+  if (tid < WC) {
+    const int blk = tid < WB ? ba : bb;
+    const int c = blk < 0 ? -1 : blk * WB + (tid % WB);
+    cols[tid] = (c >= 0 && c < p) ? c : -1;
+  }
   __syncthreads();
 
-  for (int sweep = 0; sweep < MAX_SWEEPS && p > 1; ++sweep) {
-    sweeps_done = sweep + 1;
-    bool sweep_rot = false;
-    const int rounds = (nblk == 1) ? 1 : pp - 1;
-    for (int r = 0; r < rounds; ++r) {
-      for (int k = 0; k < ((nblk == 1) ? 1 : pp / 2); ++k) {
-        int ba, bb;
-        if (nblk == 1) { ba = 0; bb = -1; }
-        else {
-          ba = rr_player(k, r, pp);
-          bb = rr_player(pp - 1 - k, r, pp);
-          if (ba >= nblk) ba = -1;
-          if (bb >= nblk) bb = -1;
-          if (ba < 0 && bb < 0) continue;
-        }
-        if (tid < WC) {
-          const int blk = tid < WB ? ba : bb;
-          const int c = blk < 0 ? -1 : blk * WB + (tid % WB);
-          cols[tid] = (c >= 0 && c < p) ? c : -1;
-        }
-        __syncthreads();
-        // ---- Gram matrix G = C^T C (MFMA f64 16x16x4; wave w owns output blocks w and w+8)
-        if (tid == 0) t0 = wall_clock64();
-        d4v gacc[2] = {d4v{0, 0, 0, 0}, d4v{0, 0, 0, 0}};
-        constexpr int PER = RCH * WC / JT;  // chunk elements per thread
-        double pre[PER];
+  // ---- Gram matrix G = C^T C (MFMA f64 16x16x4; wave w owns output blocks w and w+8)
+  constexpr int PER = RCH * WC / JT;  // chunk elements per thread
+  {
+    d4v gacc[2] = {d4v{0, 0, 0, 0}, d4v{0, 0, 0, 0}};
+    double pre[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * JT, rr = e % RCH, gc = cols[e / RCH];
+      pre[q] = (gc >= 0 && rr < p) ? S.B[rr + int64_t(gc) * S.ldb] : 0.0;
+    }
+    for (int r0 = 0; r0 < p; r0 += RCH) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int e = tid + q * JT;
+        chunk[e % RCH][e / RCH] = pre[q];
+      }
+      __syncthreads();
+      if (r0 + RCH < p) {  // prefetch the next chunk while the MFMAs run
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-          const int e = tid + q * JT, rr = e % RCH, gc = cols[e / RCH];
+          const int e = tid + q * JT, rr = r0 + RCH + e % RCH, gc = cols[e / RCH];
           pre[q] = (gc >= 0 && rr < p) ? S.B[rr + int64_t(gc) * S.ldb] : 0.0;
         }
-        for (int r0 = 0; r0 < p; r0 += RCH) {
+      }
 #pragma unroll
-          for (int q = 0; q < PER; ++q) {
-            const int e = tid + q * JT;
-            chunk[e % RCH][e / RCH] = pre[q];
-          }
-          __syncthreads();
-          if (r0 + RCH < p) {  // prefetch the next chunk while the MFMAs run
+      for (int t = 0; t < 2; ++t) {
+        const int blk = wave + 8 * t, bi = blk >> 2, bj = blk & 3;
 #pragma unroll
-            for (int q = 0; q < PER; ++q) {
-              const int e = tid + q * JT, rr = r0 + RCH + e % RCH, gc = cols[e / RCH];
-              pre[q] = (gc >= 0 && rr < p) ? S.B[rr + int64_t(gc) * S.ldb] : 0.0;
-            }
-          }
-#pragma unroll
-          for (int t = 0; t < 2; ++t) {
-            const int blk = wave + 8 * t, bi = blk >> 2, bj = blk & 3;
-#pragma unroll
-            for (int kk = 0; kk < RCH; kk += 4) {
-              const double a = chunk[kk + (lane >> 4)][16 * bi + (lane & 15)];
-              const double b = chunk[kk + (lane >> 4)][16 * bj + (lane & 15)];
-              gacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, gacc[t], 0, 0, 0);
-            }
-          }
-          __syncthreads();
+        for (int kk = 0; kk < RCH; kk += 4) {
+          const double a = chunk[kk + (lane >> 4)][16 * bi + (lane & 15)];
+          const double b = chunk[kk + (lane >> 4)][16 * bj + (lane & 15)];
+          gacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, gacc[t], 0, 0, 0);
         }
+      }
+      __syncthreads();
+    }
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          const int blk = wave + 8 * t, bi = blk >> 2, bj = blk & 3;
+    for (int t = 0; t < 2; ++t) {
+      const int blk = wave + 8 * t, bi = blk >> 2, bj = blk & 3;
 #pragma unroll
-          for (int q = 0; q < 4; ++q) Gbuf[0][16 * bi + 4 * q + (lane >> 4)][16 * bj + (lane & 15)] = gacc[t][q];
-        }
-        for (int e = tid; e < WC * WC; e += JT) V[e % WC][e / WC] = (e % WC == e / WC) ? 1.0 : 0.0;
-        __syncthreads();
-        // scale G by a power of two (rotation angles are scale free) so that the
-        // squares below cannot overflow for unnormalised inputs
-        {
-          double mx = 0.0;
-          for (int i = lane; i < WC; i += 64) mx = fmax(mx, Gbuf[0][i][i]);
+      for (int q = 0; q < 4; ++q) G[16 * bi + 4 * q + (lane >> 4)][16 * bj + (lane & 15)] = gacc[t][q];
+    }
+  }
+  for (int e = tid; e < WC * WC; e += JT) V[e % WC][e / WC] = (e % WC == e / WC) ? 1.0 : 0.0;
+  __syncthreads();
+  if (tid == 0) t1 = wall_clock64();
+  // scale G by a power of two (rotation angles are scale free) so that the
+  // squares below cannot overflow for unnormalised inputs
+  {
+    double mx = 0.0;
+    for (int i = lane; i < WC; i += 64) mx = fmax(mx, G[i][i]);
 #pragma unroll
-          for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-          const double sc = (mx > 0.0 && isfinite(mx)) ? ldexp(1.0, -ilogb(mx)) : 1.0;
-          __syncthreads();
-          for (int e = tid; e < WC * WC; e += JT) Gbuf[0][e % WC][e / WC] *= sc;
-        }
-        __syncthreads();
-        if (tid == 0) { const uint64_t t1 = wall_clock64(); t_gram += t1 - t0; t0 = t1; }
-        // ---- inner parallel Jacobi on G.  Every wave computes the parameters of all
-        // 32 rotations of a round (lane l <-> pair l & 31) from the same LDS values, so
-        // the waves agree without shared flags and a round needs a single barrier.
-        bool visit_rot = false;
-        const double tol2 = tol * tol;
-        int cur = 0;
-        for (int inner = 0; inner < MAX_INNER; ++inner) {
-          bool any = false;
-          for (int ir = 0; ir < WC - 1; ++ir) {
-            const int pr = lane & (NPAIR - 1);
-            int a = rr_player(pr, ir, WC), b = rr_player(WC - 1 - pr, ir, WC);
-            if (a > b) { const int t = a; a = b; b = t; }
-            double c = 1.0, s = 0.0;
-            if (cols[a] >= 0 && cols[b] >= 0) {
-              const double aa = Gbuf[cur][a][a], bb2 = Gbuf[cur][b][b], ab = Gbuf[cur][a][b];
-              if (aa > 0.0 && bb2 > 0.0 && ab * ab > tol2 * aa * bb2) {
-                const double dd = bb2 - aa;
-                const double t = 2.0 * ab * copysign(1.0, dd) / (fabs(dd) + sqrt(dd * dd + 4.0 * ab * ab));
-                c = rsqrt(1.0 + t * t);
-                s = c * t;
-              }
-            }
-            const bool round_any = __any(s != 0.0);
-            any = any || round_any;
-            if (round_any) {
-              // G' = R^T G R, R = block diagonal of [[c, s], [-s, c]] on (a_k, b_k); each
-              // 2x2 block (pair kr rows, pair kc columns) is owned by one thread, which
-              // reads the current copy of G and writes the other one
-              for (int e = tid; e < NPAIR * NPAIR; e += JT) {
-                const int kr = e % NPAIR, kc = e / NPAIR;
-                const double sr = __shfl(s, kr), sc = __shfl(s, kc);
-                const double cr = __shfl(c, kr), cc = __shfl(c, kc);
-                const int ar = __shfl(a, kr), br = __shfl(b, kr), ac = __shfl(a, kc), bc = __shfl(b, kc);
-                const double g00 = Gbuf[cur][ar][ac], g01 = Gbuf[cur][ar][bc];
-                const double g10 = Gbuf[cur][br][ac], g11 = Gbuf[cur][br][bc];
-                const double t00 = cr * g00 - sr * g10, t01 = cr * g01 - sr * g11;
-                const double t10 = sr * g00 + cr * g10, t11 = sr * g01 + cr * g11;
-                Gbuf[cur ^ 1][ar][ac] = t00 * cc - t01 * sc;
-                Gbuf[cur ^ 1][ar][bc] = t00 * sc + t01 * cc;
-                Gbuf[cur ^ 1][br][ac] = t10 * cc - t11 * sc;
-                Gbuf[cur ^ 1][br][bc] = t10 * sc + t11 * cc;
-              }
-              // V <- V R: row x, pair l
-              for (int e = tid; e < WC * NPAIR; e += JT) {
-                const int x = e % WC, l = e / WC;
-                const double sl = __shfl(s, l), cl = __shfl(c, l);
-                const int al = __shfl(a, l), bl = __shfl(b, l);
-                if (sl == 0.0) continue;
-                const double va = V[x][al], vb = V[x][bl];
-                V[x][al] = cl * va - sl * vb;
-                V[x][bl] = sl * va + cl * vb;
-              }
-              __syncthreads();
-              cur ^= 1;
-            }
-          }
-          visit_rot = visit_rot || any;
-          if (!any) break;
-        }
-        if (tid == 0) { const uint64_t t1 = wall_clock64(); t_inner += t1 - t0; t0 = t1; }
-        if (!visit_rot) continue;
-        sweep_rot = true;
-        // ---- apply V to the visited columns of B and J: C <- C V (wave w: row block w>>2, column block w&3)
-        for (int which = 0; which < 2; ++which) {
-          double* M = which == 0 ? S.B : S.J;
-          const int64_t ld = which == 0 ? S.ldb : S.ldj;
-          constexpr int PER = RCH * WC / JT;
-          double pre[PER];
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    const double sc = (mx > 0.0 && isfinite(mx)) ? ldexp(1.0, -ilogb(mx)) : 1.0;
+    __syncthreads();
+    for (int e = tid; e < WC * WC; e += JT) G[e % WC][e / WC] *= sc;
+  }
+  __syncthreads();
+  if (tid < WC) pdiag[0][tid] = G[tid][tid];
+  if (tid < NPAIR) {
+    const int a = rr_player(tid, 0, WC), b = rr_player(WC - 1 - tid, 0, WC);
+    pab[0][tid] = G[min(a, b)][max(a, b)];
+  }
+  const double tol = double(p > 16 ? p : 16) * 2.220446049250313e-16;
+  const double tol2 = tol * tol;
+  // a visit whose columns are already pairwise orthogonal changes nothing (no
+  // round would rotate): skip the inner sweep and the apply
+  int need = 0;
+  for (int e = tid; e < WC * WC; e += JT) {
+    const int i = e % WC, j = e / WC;
+    if (i < j) {
+      const double aa = G[i][i], bb2 = G[j][j], ab = G[i][j];
+      need |= (aa > 0.0 && bb2 > 0.0 && ab * ab > tol2 * aa * bb2) ? 1 : 0;
+    }
+  }
+  if (!__syncthreads_or(need)) {
+    if (tid == 0 && S.info) {
+      atomicAdd(reinterpret_cast<unsigned*>(S.info) + 1, unsigned(t1 - t0));
+      atomicAdd(reinterpret_cast<unsigned*>(S.info) + 4, 1u);
+    }
+    return;
+  }
+
+  // ---- one inner Jacobi sweep on G (63 rounds of 32 disjoint rotations,
+  // round-robin order).  A round has two phases:
+  //   1. wave 0 computes the 32 rotation parameters from pdiag/pab and stores
+  //      them in pc/ps, while every thread applies the previous round's
+  //      rotation to its part of V (row lane, column slots 2w, 2w+1, 2w+16,
+  //      2w+17 of wave w);
+  //   2. thread (wave w, lane l) rotates its 2x2 blocks of G in place (row slot
+  //      l & 31, column slots 2w + h and 2w + 16 + h, h = l >> 5) and stores the
+  //      entries of G that the next round's parameters need into pdiag/pab.
+  // Each phase ends with a barrier.  Every slot keeps its position in the
+  // round-robin table, so the player at a position advances by one per round
+  // (63 wraps to 1, player 0 stays at position 0); the positions a player
+  // takes in the next round, and the partner there, are fixed per lane.
+  const int h = lane >> 5;
+  const int sl = lane & (NPAIR - 1);
+  __shared__ double pc[NPAIR], ps[NPAIR];
+  auto adv = [](int x) { return x == 0 ? 0 : (x == IR ? 1 : x + 1); };
+  // row slot sl: players at positions sl and 63 - sl; in the next round the
+  // player at position sl moves to nA (partner position 63 - nA, slot snA), the
+  // one at 63 - sl to 62 - sl (partner position sl + 1, slot snB)
+  int xA = rr_player(sl, 0, WC), xB = rr_player(IR - sl, 0, WC);
+  const int nA = sl == 0 ? 0 : (sl == 1 ? IR : sl - 1);
+  const int snA = min(nA, IR - nA), snB = min(62 - sl, sl + 1);
+  int yA = rr_player(IR - nA, 1, WC), yB = rr_player(sl + 1, 1, WC);
+  int xc[2], yc[2];
 #pragma unroll
-          for (int q = 0; q < PER; ++q) {
-            const int e = tid + q * JT, rr = e % RCH, gc = cols[e / RCH];
-            pre[q] = (gc >= 0 && rr < p) ? M[rr + int64_t(gc) * ld] : 0.0;
-          }
-          for (int r0 = 0; r0 < p; r0 += RCH) {
+  for (int j = 0; j < 2; ++j) {
+    const int kc = 2 * wave + 16 * j + h;
+    xc[j] = rr_player(kc, 0, WC);
+    yc[j] = rr_player(IR - kc, 0, WC);
+  }
+  const int qs[4] = {2 * wave, 2 * wave + 1, 2 * wave + 16, 2 * wave + 17};
+  int xv[4], yv[4];
 #pragma unroll
-            for (int q = 0; q < PER; ++q) {
-              const int e = tid + q * JT;
-              chunk[e % RCH][e / RCH] = pre[q];
-            }
-            __syncthreads();
-            if (r0 + RCH < p) {
+  for (int i = 0; i < 4; ++i) {
+    xv[i] = rr_player(qs[i], 0, WC);
+    yv[i] = rr_player(IR - qs[i], 0, WC);
+  }
+  double cV[4] = {1.0, 1.0, 1.0, 1.0}, sV[4] = {0.0, 0.0, 0.0, 0.0};
+  int aV[4] = {0, 0, 0, 0}, bV[4] = {0, 0, 0, 0};
+  bool visit_rot = false;
+  for (int ir = 0; ir < IR; ++ir) {
+    const int buf = ir & 1;
+    const int a = min(xA, xB), b = max(xA, xB);
+    // ---- phase 1
+    if (wave == 0 && lane < NPAIR) {
+      const double aa = pdiag[buf][a], bb2 = pdiag[buf][b], ab = pab[buf][sl];
+      double c = 1.0, s = 0.0;
+      // padding columns have a zero diagonal and never rotate
+      if (aa > 0.0 && bb2 > 0.0 && ab * ab > tol2 * aa * bb2) {
+        const double dd = bb2 - aa;
+        const double t = 2.0 * ab * copysign(1.0, dd) / (fabs(dd) + sqrt(dd * dd + 4.0 * ab * ab));
+        c = rsqrt(1.0 + t * t);
+        s = c * t;
+      }
+      pc[sl] = c;
+      ps[sl] = s;
+    }
+    if (ir > 0) {
+      double va[4], vb[4];
 #pragma unroll
-              for (int q = 0; q < PER; ++q) {
-                const int e = tid + q * JT, rr = r0 + RCH + e % RCH, gc = cols[e / RCH];
-                pre[q] = (gc >= 0 && rr < p) ? M[rr + int64_t(gc) * ld] : 0.0;
-              }
-            }
-            const int bi = wave >> 2, bj = wave & 3;
-            d4v acc = {0, 0, 0, 0};
+      for (int i = 0; i < 4; ++i) {
+        va[i] = V[lane][aV[i]];
+        vb[i] = V[lane][bV[i]];
+      }
 #pragma unroll
-            for (int kk = 0; kk < WC; kk += 4) {
-              const double a = chunk[16 * bi + (lane & 15)][kk + (lane >> 4)];
-              const double b = V[kk + (lane >> 4)][16 * bj + (lane & 15)];
-              acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const int row = r0 + 16 * bi + 4 * q + (lane >> 4);
-              const int gc = cols[16 * bj + (lane & 15)];
-              if (gc >= 0 && row < p) M[row + int64_t(gc) * ld] = acc[q];
-            }
-            __syncthreads();
-          }
-        }
-        if (tid == 0) t_apply += wall_clock64() - t0;
+      for (int i = 0; i < 4; ++i) {
+        if (sV[i] == 0.0) continue;
+        V[lane][aV[i]] = cV[i] * va[i] - sV[i] * vb[i];
+        V[lane][bV[i]] = sV[i] * va[i] + cV[i] * vb[i];
       }
     }
-    if (!sweep_rot) break;
+    __syncthreads();
+    // ---- phase 2
+    const double c = pc[sl], s = ps[sl];
+    visit_rot = visit_rot || __any(s != 0.0);
+    const bool nxt = ir + 1 < IR;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int kc = 2 * wave + 16 * j + h;
+      const double cc = pc[kc], scc = ps[kc];
+      const int ac = min(xc[j], yc[j]), bc = max(xc[j], yc[j]);
+      double g00, g01, g10, g11;
+      const bool rot = (s != 0.0) || (scc != 0.0);
+      if (rot) {
+        g00 = G[a][ac]; g01 = G[a][bc];
+        g10 = G[b][ac]; g11 = G[b][bc];
+        const double t00 = c * g00 - s * g10, t01 = c * g01 - s * g11;
+        const double t10 = s * g00 + c * g10, t11 = s * g01 + c * g11;
+        g00 = t00 * cc - t01 * scc;
+        g01 = t00 * scc + t01 * cc;
+        g10 = t10 * cc - t11 * scc;
+        g11 = t10 * scc + t11 * cc;
+        G[a][ac] = g00; G[a][bc] = g01;
+        G[b][ac] = g10; G[b][bc] = g11;
+      }
+      if (nxt) {
+        if (kc == sl) {  // diagonal block: G(a,a), G(b,b)
+          pdiag[buf ^ 1][a] = rot ? g00 : G[a][a];
+          pdiag[buf ^ 1][b] = rot ? g11 : G[b][b];
+        }
+        // next-round pairs (xA, yA) and (xB, yB): the thread holding row min and
+        // column max of a pair stores its entry
+        if (xA < yA && (yA == ac || yA == bc)) {
+          const bool ra = xA == a, ca = yA == ac;
+          pab[buf ^ 1][snA] = rot ? (ra ? (ca ? g00 : g01) : (ca ? g10 : g11)) : G[xA][yA];
+        }
+        if (xB < yB && (yB == ac || yB == bc)) {
+          const bool ra = xB == a, ca = yB == ac;
+          pab[buf ^ 1][snB] = rot ? (ra ? (ca ? g00 : g01) : (ca ? g10 : g11)) : G[xB][yB];
+        }
+      }
+    }
+    // this round's V rotation is applied in phase 1 of the next round
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      cV[i] = pc[qs[i]];
+      sV[i] = ps[qs[i]];
+      aV[i] = min(xv[i], yv[i]);
+      bV[i] = max(xv[i], yv[i]);
+      xv[i] = adv(xv[i]);
+      yv[i] = adv(yv[i]);
+    }
+    xA = adv(xA); xB = adv(xB); yA = adv(yA); yB = adv(yB);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      xc[j] = adv(xc[j]);
+      yc[j] = adv(yc[j]);
+    }
+    __syncthreads();
   }
-  // singular values: column norms
-  for (int c = wave; c < p; c += JT / 64) {
+  {
+    double va[4], vb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      va[i] = V[lane][aV[i]];
+      vb[i] = V[lane][bV[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (sV[i] == 0.0) continue;
+      V[lane][aV[i]] = cV[i] * va[i] - sV[i] * vb[i];
+      V[lane][bV[i]] = sV[i] * va[i] + cV[i] * vb[i];
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    t2 = wall_clock64();
+    if (S.info) {
+      atomicAdd(reinterpret_cast<unsigned*>(S.info) + 1, unsigned(t1 - t0));
+      atomicAdd(reinterpret_cast<unsigned*>(S.info) + 2, unsigned(t2 - t1));
+      atomicAdd(reinterpret_cast<unsigned*>(S.info) + 4, 1u);
+    }
+  }
+  if (!visit_rot) return;
+  if (tid == 0) flags[(sweep & 1) * njobs + lo] = 1;
+
+  // ---- apply V to the visited columns of B and J: C <- C V (wave w: row block w>>2, column block w&3)
+  for (int which = 0; which < 2; ++which) {
+    double* M = which == 0 ? S.B : S.J;
+    const int64_t ld = which == 0 ? S.ldb : S.ldj;
+    double pre[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * JT, rr = e % RCH, gc = cols[e / RCH];
+      pre[q] = (gc >= 0 && rr < p) ? M[rr + int64_t(gc) * ld] : 0.0;
+    }
+    for (int r0 = 0; r0 < p; r0 += RCH) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int e = tid + q * JT;
+        chunk[e % RCH][e / RCH] = pre[q];
+      }
+      __syncthreads();
+      if (r0 + RCH < p) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+          const int e = tid + q * JT, rr = r0 + RCH + e % RCH, gc = cols[e / RCH];
+          pre[q] = (gc >= 0 && rr < p) ? M[rr + int64_t(gc) * ld] : 0.0;
+        }
+      }
+      const int bi = wave >> 2, bj = wave & 3;
+      d4v acc = {0, 0, 0, 0};
+#pragma unroll
+      for (int kk = 0; kk < WC; kk += 4) {
+        const double av = chunk[16 * bi + (lane & 15)][kk + (lane >> 4)];
+        const double bv = V[kk + (lane >> 4)][16 * bj + (lane & 15)];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = r0 + 16 * bi + 4 * q + (lane >> 4);
+        const int gc = cols[16 * bj + (lane & 15)];
+        if (gc >= 0 && row < p) M[row + int64_t(gc) * ld] = acc[q];
+      }
+      __syncthreads();
+    }
+  }
+  if (tid == 0 && S.info) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 3, unsigned(wall_clock64() - t2));
+}
+
+// singular values: column norms of B (one workgroup per matrix, one wave per column)
+__global__ __launch_bounds__(256) void column_norms_kernel(const SVDJob* __restrict__ jobs) {
+  const SVDJob S = jobs[blockIdx.x];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int c = wave; c < S.p; c += 4) {
     const double* x = S.B + int64_t(c) * S.ldb;
     double s = 0.0;
-    for (int i = lane; i < p; i += 64) s += x[i] * x[i];
+    for (int i = lane; i < S.p; i += 64) s += x[i] * x[i];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if (lane == 0) S.sv[c] = sqrt(s);
-  }
-  if (tid == 0 && S.info) {
-    S.info[0] = sweeps_done;
-    // phase times in units of 1024 ticks of the constant-rate wall clock
-    S.info[1] = int32_t(t_gram >> 10);
-    S.info[2] = int32_t(t_inner >> 10);
-    S.info[3] = int32_t(t_apply >> 10);
   }
 }
 
@@ -270,14 +397,49 @@ void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs) {
   for (auto& j : jobs)
     if (j.p > 0) live.push_back(j);
   if (live.empty()) return;
-  const SVDJob* d = ctx.upload(live);
+  const int nj = int(live.size());
+  std::vector<JacJob> jj(nj);
+  std::vector<Fill2D> fills;
+  int slots = 0, max_rounds = 1;
   double fl = 0.0, by = 0.0;
-  for (const auto& j : live) {
-    fl += 9.0 * double(j.p) * j.p * j.p;  // one sweep of plain one-sided Jacobi (nominal)
-    by += 16.0 * double(j.p) * j.p;
+  for (int q = 0; q < nj; ++q) {
+    const SVDJob& s = live[q];
+    JacJob& J = jj[q];
+    J.B = s.B; J.ldb = s.ldb; J.J = s.J; J.ldj = s.ldj; J.info = s.info; J.p = s.p;
+    J.nblk = (s.p + WB - 1) / WB;
+    J.pp = J.nblk + (J.nblk & 1);
+    J.rounds = J.nblk == 1 ? 1 : J.pp - 1;
+    J.slot0 = slots;
+    slots += J.nblk == 1 ? 1 : J.pp / 2;
+    max_rounds = std::max(max_rounds, J.rounds);
+    fills.push_back(Fill2D{s.J, s.ldj, s.p, s.p, 1.0});
+    fl += 9.0 * double(s.p) * s.p * s.p;  // one sweep of plain one-sided Jacobi (nominal)
+    by += 16.0 * double(s.p) * s.p;
   }
+  const JacJob* dj = ctx.upload(jj);
+  const SVDJob* dl = ctx.upload(live);
+  int32_t* flags = ctx.ws.alloc_n<int32_t>(size_t(2) * nj);
+  int32_t* hflags = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * nj));
   ProfScope ps(ctx, "jacobi_svd", fl, by);
-  hipLaunchKernelGGL(jacobi_block_kernel, dim3(unsigned(live.size())), dim3(JT), 0, ctx.stream, d);
+  fill2d_batched(ctx, fills);
+  zero_fill(ctx, flags, sizeof(int32_t) * 2 * nj);
+  for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
+    int32_t* cur = flags + size_t(sweep & 1) * nj;
+    if (sweep >= 2) zero_fill(ctx, cur, sizeof(int32_t) * nj);
+    for (int r = 0; r < max_rounds; ++r) {
+      hipLaunchKernelGGL(jacobi_round_kernel, dim3(unsigned(slots)), dim3(JT), 0, ctx.stream, dj, nj, r, sweep, flags);
+      TTR_HIP(hipGetLastError());
+    }
+    // convergence check from the third sweep on (no matrix converges earlier in practice)
+    if (sweep >= 2) {
+      TTR_HIP(hipMemcpyAsync(hflags, cur, sizeof(int32_t) * nj, hipMemcpyDeviceToHost, ctx.stream));
+      ctx.sync();
+      bool any = false;
+      for (int q = 0; q < nj && !any; ++q) any = hflags[q] != 0;
+      if (!any) break;
+    }
+  }
+  hipLaunchKernelGGL(column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, ctx.stream, dl);
   TTR_HIP(hipGetLastError());
 }
 

@@ -119,15 +119,19 @@ class Profiler {
   ~Profiler();
   bool on = false;
   // returns a token; call stop(token) after enqueuing the launch
-  int start(hipStream_t s, const char* name, double flops, double bytes);
+  // name2 (optional): the record is also added to that second statistic
+  int start(hipStream_t s, const char* name, double flops, double bytes, const char* name2 = nullptr);
   void stop(hipStream_t s, int token);
-  void collect();  // stream must be synchronised
+  // adds every closed record to stats (the stream must be synchronised); open
+  // records (start without stop yet) stay pending
+  void collect();
   void reset() { stats.clear(); }
   std::map<std::string, ProfStat> stats;
 
  private:
-  struct Pending { hipEvent_t a, b; std::string name; double flops, bytes; };
-  std::vector<Pending> pending_;
+  struct Pending { hipEvent_t a, b; std::string name, name2; double flops, bytes; bool stopped; };
+  std::map<int, Pending> pending_;
+  int next_ = 0, stopped_ = 0;
   std::vector<hipEvent_t> pool_;
   hipEvent_t get_event();
 };
@@ -159,8 +163,8 @@ struct Ctx {
 struct ProfScope {
   Ctx& ctx;
   int tok;
-  ProfScope(Ctx& c, const char* name, double flops = 0.0, double bytes = 0.0)
-      : ctx(c), tok(c.prof.on ? c.prof.start(c.stream, name, flops, bytes) : -1) {}
+  ProfScope(Ctx& c, const char* name, double flops = 0.0, double bytes = 0.0, const char* name2 = nullptr)
+      : ctx(c), tok(c.prof.on && name ? c.prof.start(c.stream, name, flops, bytes, name2) : -1) {}
   ~ProfScope() { if (tok >= 0) ctx.prof.stop(ctx.stream, tok); }
 };
 
@@ -217,7 +221,9 @@ class GemmBatch {
 // Launch all problems of the batch in one grouped launch.  ta/tb/tc are the
 // storage precisions of A, B, C; math is the MFMA precision (F64, F32 or BF16,
 // the latter two accumulating in FP32).
-void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math);
+// site: optional call-site label; with profiling on, the launch is also
+// recorded under "site:<label>"
+void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, const char* site = nullptr);
 
 // ---------------------------------------------------------------------------
 // Householder QR (LAPACK dgeqrf/dorgqr conventions), batched
@@ -242,7 +248,8 @@ struct SVDJob {
   int64_t ldj;
   double* sv;  // p singular values (column norms of B*J), unsorted
   int p;
-  int32_t* info;  // optional: number of sweeps used
+  int32_t* info;  // optional, 8 counters zeroed by the caller: [0] sweeps used, [1..3] wall-clock
+                  // ticks of the Gram / inner / apply phases summed over visits, [4] visits
 };
 void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs);
 
@@ -255,7 +262,8 @@ struct Copy2D {  // out[b*obs + i*ors + j*ocs] = alpha * in[b*ibs + i*irs + j*ic
   int64_t irs, ics, ibs, ors, ocs, obs;
   int32_t M, N, nb;
   double alpha;
-  int32_t tri;  // 0: full copy; 1: keep i <= j (upper), zero below; 2: keep i >= j (lower), zero above
+  int32_t tri;  // 0: full copy; 1: keep i <= j (upper), zero below; 2: keep i >= j (lower), zero above;
+                // 3: unit lower (keep i > j, one on the diagonal, zero above)
 };
 inline Copy2D copy_job(const double* in, int64_t irs, int64_t ics, double* out, int64_t ors, int64_t ocs, int M, int N,
                        double alpha = 1.0, int tri = 0) {

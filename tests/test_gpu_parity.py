@@ -136,6 +136,24 @@ def test_tt_rounding(T, tol, rmax, direction):
     orth_ok(y)
 
 
+@pytest.mark.parametrize("tol", [1e-12, 1e-4])
+def test_tt_rounding_large_ranks(T, tol):
+    """Bond ranks 36..160: square factors of size 36..160 (1 to 5 column blocks of
+    the Jacobi SVD, odd block counts included) and QR/LQ of unfoldings up to
+    960 x 160 (many Householder panels)."""
+    say(f"case: tt_rounding large ranks tol={tol}")
+    dims = (6,) * 6
+    x = O.tt_add(rand_tt(dims, 100, 33), O.tt_scale(1e-3, rand_tt(dims, 60, 34)))
+    assert max(x.rks) == 160
+    y = to_oracle(T.tt_rounding(to_api(x), tol=tol))
+    ref = O.tt_rounding(x, tol=tol)
+    assert y.rks == ref.rks and y.ot == ref.ot
+    err = O.rel_err(ref, y)
+    say(f"  ranks {y.rks}, rel err vs oracle {err:.3e}")
+    assert err < 1e-12
+    orth_ok(y)
+
+
 def test_tt_rounding_closed_form(T):
     say("case: round(y + y + 1e-10 z, 1e-8) = 2y")
     dims = (4,) * 8
@@ -232,6 +250,25 @@ def test_ttrand_rounding_vs_oracle_truncating(T, prec, tol):
     assert y.rks == ref.rks
     say(f"  rel err vs oracle: {O.rel_err(ref, y):.3e}")
     assert O.rel_err(ref, y) < tol
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_ttrand_rounding_large_ranks(T, prec, tol):
+    """Sketch ranks up to 140 with term ranks 40..90: multi-panel Householder QR of
+    Z_k (up to 512 x 140) and multi-block sketch contractions."""
+    say(f"case: ttrand large ranks {prec}")
+    dims = (8,) * 6
+    ys = [rand_tt(dims, r, 120 + j) for j, r in enumerate((90, 40, 64))]
+    al = [1.0, -0.5, 0.25]
+    rks = O.r_and_d_to_rks([140] * 7, dims, rmax=140)
+    assert rks == [1, 8, 64, 140, 64, 8, 1]
+    y = to_oracle(T.ttrand_rounding(al, [to_api(t) for t in ys], rks, orthogonal=False, seed=43, sketch_prec=prec))
+    prov = device_blocks_provider(dims, rks, orthogonal=False, seed=43, sketch_prec=prec)
+    ref = O.ttrand_rounding_lincomb(al, ys, rks, prov, False)
+    assert y.rks == ref.rks
+    err = O.rel_err(ref, y)
+    say(f"  ranks {y.rks}, rel err vs oracle {err:.3e}")
+    assert err < tol
 
 
 def test_round_rand_batch_uneven(T):
