@@ -87,7 +87,8 @@ def main():
     ap.add_argument("--rmax", type=int, default=256)
     ap.add_argument("--sketch-prec", default="f32")
     ap.add_argument("--cleanup-tol", type=float, default=1e-6)
-    ap.add_argument("--det-items", type=int, default=2, help="items for the FP64 deterministic comparison (0: skip)")
+    ap.add_argument("--det-items", type=int, default=48,
+                    help="items for the FP64 deterministic comparison (0: skip); their materialised sums must fit in HBM")
     ap.add_argument("--cpu-items", type=int, default=1, help="items for the oracle CPU baseline / parity (0: skip)")
     ap.add_argument("--no-events", action="store_true",
                     help="do not record the library's per-launch hipEvents (for runs under an external profiler)")
@@ -197,6 +198,11 @@ def main():
     if args.det_items > 0:
         sub = items[: args.det_items]
         sums = [T.lincomb_device(al, terms) for al, terms, _ in sub]
+        # the materialised sums (1.28 GB each at these ranks) need the room of the terms
+        for _, terms, _ in items:
+            for t in terms:
+                t.free()
+        items = None
         r = T.round_svd_batch(sums, tol=args.cleanup_tol if args.cleanup_tol > 0 else 1e-12)
         for o in r:
             o.free()

@@ -89,7 +89,7 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
     const bool a_row_fast = (S.a_rs == 1);
     const bool b_row_fast = (S.b_rs == 1);
     // the k-tile k0 + BK is loaded into registers while the MFMAs of tile k0 run
-    double ra[4], rb[4];
+    MT ra[4], rb[4];
     auto load_tiles = [&](int k0) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
                   ? load_d<TA>(S.A, gi * S.a_rs + gl * S.a_cs) : 0.0;
           v = amode_value(S.a_mode, gi, gl, v) * scale;
         }
-        ra[q] = v;
+        ra[q] = MT(v);
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
         if (b_row_fast) { l = tid & 15; j = (tid >> 4) + 16 * q; }
         else { j = tid & 63; l = (tid >> 6) + 4 * q; }
         const int64_t gl = k0 + l, gj = col0 + j;
-        rb[q] = (gl < S.K && gj < P.N) ? load_d<TB>(S.B, gl * S.b_rs + gj * S.b_cs) : 0.0;
+        rb[q] = MT((gl < S.K && gj < P.N) ? load_d<TB>(S.B, gl * S.b_rs + gj * S.b_cs) : 0.0);
       }
     };
     load_tiles(0);
@@ -121,14 +121,14 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
         int i, l;
         if (a_row_fast) { i = tid & 63; l = (tid >> 6) + 4 * q; }
         else { i = (tid >> 4) + 16 * q; l = tid & 15; }
-        As[l][i] = MT(ra[q]);
+        As[l][i] = ra[q];
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         int l, j;
         if (b_row_fast) { l = tid & 15; j = (tid >> 4) + 16 * q; }
         else { j = tid & 63; l = (tid >> 6) + 4 * q; }
-        Bs[l][j] = MT(rb[q]);
+        Bs[l][j] = rb[q];
       }
       __syncthreads();
       if (k0 + BK < S.K) load_tiles(k0 + BK);
