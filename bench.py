@@ -29,10 +29,32 @@ import numpy as np  # noqa: E402
 
 PEAKS = {"gemm_f64": ("mfma", 78.6e12), "gemm_f32": ("mfma", 157.3e12), "gemm_bf16": ("mfma", 2516e12),
          "qr_panel": ("hbm", 8.0e12), "jacobi_svd": ("mfma", 78.6e12)}
+# device kernel behind each profiled class (for matching rocprofv3 summaries)
+KERNEL_SYMBOL = {"gemm_f64": "gemm_scalar_kernel<double, double, double, double>",
+                 "gemm_f32": "gemm_scalar_kernel<double, float, float, float>",
+                 "gemm_bf16": "gemm_bf16_kernel", "qr_panel": "qr_panel_reg_kernel",
+                 "jacobi_svd": "jacobi_round_kernel"}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(args, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by what
+    the counter reads for a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "c3_batched_rounding_hbm_traffic.json")
+    try:
+        with open(path) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        return None
+    cfg = rec.get("config", {})
+    if any(cfg.get(k) != getattr(args, k) for k in ("items", "d", "n", "terms", "sketch_prec")):
+        return None
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("hbm_bytes_per_launch")
 
 
 def term_ranks(dims, r):
@@ -90,6 +112,9 @@ def main():
     ap.add_argument("--det-items", type=int, default=48,
                     help="items for the FP64 deterministic comparison (0: skip); their materialised sums must fit in HBM")
     ap.add_argument("--cpu-items", type=int, default=1, help="items for the oracle CPU baseline / parity (0: skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first copy BYTES of device memory 3 times (a plain copy kernel of known size, "
+                         "the reference point for hardware traffic counters)")
     ap.add_argument("--no-events", action="store_true",
                     help="do not record the library's per-launch hipEvents (for runs under an external profiler)")
     args = ap.parse_args()
@@ -97,6 +122,17 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.copy_calibration > 0:
+        import torch
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {args.copy_calibration} bytes x3")
+        del src, dst
 
     import ttround as T
     h = T.default_handle()
@@ -129,6 +165,11 @@ def main():
     sweeps = stats.pop("jacobi_svd:sweeps", None)
     jacobi = ({"svds": sweeps["launches"], "mean_sweeps": round(sweeps["flops"] / max(sweeps["launches"], 1), 2),
                "max_sweeps": sweeps["bytes"]} if sweeps else None)
+    if jacobi:
+        for ph in ("visits", "ticks_gram", "ticks_inner", "ticks_apply"):
+            st = stats.pop("jacobi_svd:" + ph, None)
+            if st:
+                jacobi[ph] = st["flops"]
     sites = {k[5:]: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] // args.steps,
                      "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3)}
              for k, v in stats.items() if k.startswith("site:")}
@@ -146,16 +187,13 @@ def main():
         else:
             per_launch = st["bytes"] / st["launches"]
             ach, unit, pk = per_launch / avg_s / 1e9, "GB/s", peak / 1e9
-        roof = {"kernel": dom, "bound": bound, "achieved": round(ach, 3), "peak": pk, "unit": unit,
-                "frac": round(ach / pk, 5), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 2),
+        roof = {"kernel": dom, "device_kernel": KERNEL_SYMBOL.get(dom), "bound": bound, "achieved": round(ach, 3),
+                "peak": pk, "unit": unit, "frac": round(ach / pk, 5), "traffic": measured_traffic(args, dom),
+                "algorithmic_bytes_per_launch": round(st["bytes"] / st["launches"]),
+                "avg_launch_us": round(avg_s * 1e6, 2),
                 "launches_per_step": st["launches"] / args.steps,
                 "share_of_kernel_time": round(st["ms"] / sum(v["ms"] for v in kern.values()), 4)}
 
-    if jacobi:
-        for ph in ("visits", "ticks_gram", "ticks_inner", "ticks_apply"):
-            st = stats.pop("jacobi_svd:" + ph, None)
-            if st:
-                jacobi[ph] = st["flops"]
     kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] // args.steps,
                    "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3)} for k, v in kern.items()}
 

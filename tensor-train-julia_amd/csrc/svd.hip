@@ -330,7 +330,10 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
     }
   }
   if (!visit_rot) return;
-  if (tid == 0) flags[(sweep & 1) * njobs + lo] = 1;
+  if (tid == 0) {
+    flags[(sweep & 1) * njobs + lo] = 1;
+    if (S.info) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 5, 1u);
+  }
 
   // ---- apply V to the visited columns of B and J: C <- C V (wave w: row block w>>2, column block w&3)
   for (int which = 0; which < 2; ++which) {
@@ -401,7 +404,6 @@ void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs) {
   std::vector<JacJob> jj(nj);
   std::vector<Fill2D> fills;
   int slots = 0, max_rounds = 1;
-  double fl = 0.0, by = 0.0;
   for (int q = 0; q < nj; ++q) {
     const SVDJob& s = live[q];
     JacJob& J = jj[q];
@@ -413,20 +415,20 @@ void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs) {
     slots += J.nblk == 1 ? 1 : J.pp / 2;
     max_rounds = std::max(max_rounds, J.rounds);
     fills.push_back(Fill2D{s.J, s.ldj, s.p, s.p, 1.0});
-    fl += 9.0 * double(s.p) * s.p * s.p;  // one sweep of plain one-sided Jacobi (nominal)
-    by += 16.0 * double(s.p) * s.p;
   }
   const JacJob* dj = ctx.upload(jj);
   const SVDJob* dl = ctx.upload(live);
   int32_t* flags = ctx.ws.alloc_n<int32_t>(size_t(2) * nj);
   int32_t* hflags = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * nj));
-  ProfScope ps(ctx, "jacobi_svd", fl, by);
   fill2d_batched(ctx, fills);
   zero_fill(ctx, flags, sizeof(int32_t) * 2 * nj);
   for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
     int32_t* cur = flags + size_t(sweep & 1) * nj;
     if (sweep >= 2) zero_fill(ctx, cur, sizeof(int32_t) * nj);
     for (int r = 0; r < max_rounds; ++r) {
+      // timed per launch; the work depends on the sweeps taken, so the caller
+      // adds it from the counters
+      ProfScope ps(ctx, "jacobi_svd");
       hipLaunchKernelGGL(jacobi_round_kernel, dim3(unsigned(slots)), dim3(JT), 0, ctx.stream, dj, nj, r, sweep, flags);
       TTR_HIP(hipGetLastError());
     }

@@ -249,7 +249,8 @@ struct SVDJob {
   double* sv;  // p singular values (column norms of B*J), unsorted
   int p;
   int32_t* info;  // optional, 8 counters zeroed by the caller: [0] sweeps used, [1..3] wall-clock
-                  // ticks of the Gram / inner / apply phases summed over visits, [4] visits
+                  // ticks of the Gram / inner / apply phases summed over visits, [4] visits,
+                  // [5] visits that applied a rotation
 };
 void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs);
 
