@@ -69,6 +69,7 @@ int ttr_tt_info(ttr_handle_t h, int64_t id, int64_t* d, int64_t* dims, int64_t* 
 /* Copies core k (0-based) to dst in the reference layout (n_k, r_{k-1}, r_k). */
 int ttr_tt_download_core(ttr_handle_t h, int64_t id, int64_t k, double* dst);
 
+/* Releases a TT or a TT operator (the ids share one space). */
 int ttr_tt_free(ttr_handle_t h, int64_t id);
 
 /* Seeded TT with i.i.d. N(0,1) cores drawn on the device, ranks capped as in
@@ -119,7 +120,9 @@ typedef struct {
   int64_t nterms;           /* number of terms of the linear combination            */
   const double* alpha;      /* nterms coefficients                                  */
   const int64_t* term_ids;  /* nterms TT ids (same dims)                            */
-  int64_t op_id;            /* reserved, must be -1 (explicit products: ttr_mpo_apply) */
+  int64_t op_id;            /* -1, or an operator id: the first term is then         */
+                            /* alpha[0] * H * term_ids[0], sketched and projected    */
+                            /* without forming the product                           */
   const int64_t* rks;       /* d+1 target sketch ranks (rks[0] = rks[d] = 1)         */
 } ttr_rand_item;
 
@@ -140,7 +143,10 @@ typedef struct {
  * min(n_k r_{k-1}, sketch_rks_k), untruncated.
  * Replaces: ttrand_rounding(y, rks; orthogonal, seed, block_rks)
  * (src/tt_randtools.jl:112-154), ttrand_rounding(y, rmax) (:162-167),
- * ttrand_rounding(alpha, y, rks) (:223-276) and its rmax form (:278-283). */
+ * ttrand_rounding(alpha, y, rks) (:223-276) and its rmax form (:278-283);
+ * with op_id >= 0 also ttrand_rounding(alpha, A, y, rks) (:285-354), its rmax
+ * form (:356-361) and ttrand_rounding(A, y, b, rks) (:169-221, as alpha = (1, -1),
+ * terms (y, b)).  Operator bonds need n_k * R_k <= 64. */
 int ttr_round_rand(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const ttr_rand_opts* opts,
                    int64_t* out_ids);
 
@@ -151,6 +157,13 @@ int ttr_round_rand(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const 
  * (src/sketches.jl:175-240,300-303). */
 int ttr_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts, double* const* W_out,
                int64_t* sketch_rks);
+
+/* The reverse recursive sketch of H x without forming the product: W[k]
+ * ((r_k R_k) x sketch_rks[k], column-major, row index x rank + r_k * H rank).
+ * Replaces: tt_recursive_sketch(T, A::TToperator, x, rks; reverse=true, ...)
+ * (src/sketches.jl:317-389,456-459,463-472). */
+int ttr_sketch_op(ttr_handle_t h, int64_t op_id, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts,
+                  double* const* W_out, int64_t* sketch_rks);
 
 /* The sketch blocks of core k (0-based) exactly as the device uses them,
  * upcast to FP64, in the reference order (bv_{k+1}, n_k, bv_k, p_k)

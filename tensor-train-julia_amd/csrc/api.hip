@@ -27,6 +27,12 @@ struct DevTTO {
   std::vector<int64_t> dims, rks;
   std::vector<double*> cores;  // reference layout (n, n, R, R')
   void* base = nullptr;
+  DevTTO() = default;
+  DevTTO(const DevTTO&) = delete;
+  DevTTO& operator=(const DevTTO&) = delete;
+  ~DevTTO() {
+    if (base) (void)hipFree(base);
+  }
 };
 
 int64_t wrap_mul(int64_t a, int64_t b) { return int64_t(uint64_t(a) * uint64_t(b)); }
@@ -664,14 +670,44 @@ void make_sketch(ttr_handle_s* h, SketchPlan& sp, int kmin) {
   }
 }
 
+// One term alpha * x, or alpha * H x when H is set (implicit operator product:
+// its bond k is the pair (x rank, H rank), x rank fastest, of size rc(k)).
 struct RandTerm {
   DevTT* x;
   double alpha;
-  std::vector<void*> W;     // per core k: (R_k x s_k) in sketch precision
+  DevTTO* H = nullptr;
+  std::vector<void*> W;     // per core k: (rc(k) x s_k) in sketch precision
   int32_t* E;               // per core exponent (d+1 ints, device)
   double* Y[2];
   double* Pb;
+  double* U = nullptr;      // operator terms: P x_{k+1} for every H bond
+  int64_t rc(int k) const { return x->rks[k] * (H ? H->rks[k] : 1); }
 };
+
+// strides of H(out mode, out bond, in mode, in bond) in a reference-layout
+// operator core (n, n, R, R'): towards the left (sketch: the right bond is
+// contracted) or towards the right (projection: the left bond is contracted)
+OpApply op_job(const DevTTO* H, int k, bool left_out) {
+  OpApply j{};
+  const int64_t n = H->dims[k], Rl = H->rks[k], Rr = H->rks[k + 1];
+  j.H = H->cores[k];
+  j.n = int(n);
+  j.h_om = 1;
+  j.h_im = n;
+  if (left_out) {
+    j.h_ob = n * n;
+    j.h_ib = n * n * Rl;
+    j.rout = int(Rl);
+    j.rin = int(Rr);
+  } else {
+    j.h_ob = n * n * Rl;
+    j.h_ib = n * n;
+    j.rout = int(Rr);
+    j.rin = int(Rl);
+  }
+  j.scale = 1.0;
+  return j;
+}
 
 struct RandItem {
   std::vector<RandTerm> terms;
@@ -679,57 +715,102 @@ struct RandItem {
   double* Z = nullptr;
 };
 
-// right-to-left sketch phase of tt_recursive_sketch (src/sketches.jl:221-240)
+// right-to-left sketch phase of tt_recursive_sketch (src/sketches.jl:221-240);
+// operator terms follow the implicit operator sketch (src/sketches.jl:392-455,
+// contract_sketch_core_backwards! for operators :1648-1713): per core, T1 =
+// X_k W_{k+1} (GEMM), T2 = the operator core applied to the (mode, right H bond)
+// pairs of T1 (op_apply), W_k = T2 S_k (GEMM per block)
 void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items, int kmin) {
   Ctx& ctx = h->ctx;
   const int d = sp.d;
   const size_t eb = prec_bytes(sp.prec);
   // buffers
-  int64_t t1max = 1;
-  std::vector<std::vector<void*>> T1(items.size());
+  std::vector<std::vector<void*>> T1(items.size()), T2(items.size());
   for (size_t i = 0; i < items.size(); ++i) {
     for (auto& t : items[i].terms) {
       t.W.assign(d + 1, nullptr);
       t.E = ctx.ws.alloc_n<int32_t>(size_t(d + 1));
       zero_fill(ctx, t.E, size_t(d + 1) * 4);
-      int64_t mx = 1;
+      int64_t mx = 1, mx2 = 1;
       for (int k = kmin; k < d; ++k) {
-        t.W[k] = ctx.ws.alloc(size_t(t.x->rks[k] * items[i].s[k]) * eb);
-        if (k < d - 1) mx = std::max(mx, t.x->rks[k] * sp.dims[k] * sp.bv[k + 1] * items[i].p[k]);
+        t.W[k] = ctx.ws.alloc(size_t(t.rc(k) * items[i].s[k]) * eb);
+        const int64_t cols = sp.bv[k + 1] * items[i].p[k];
+        if (k < d - 1) mx = std::max(mx, t.x->rks[k] * sp.dims[k] * (t.H ? t.H->rks[k + 1] : 1) * cols);
+        if (t.H) mx2 = std::max(mx2, t.rc(k) * sp.dims[k] * cols);
       }
-      t1max = std::max(t1max, mx);
       T1[i].push_back(ctx.ws.alloc(size_t(mx) * eb));
+      T2[i].push_back(t.H ? ctx.ws.alloc(size_t(mx2) * eb) : nullptr);
     }
   }
   GemmBatch gb;
+  std::vector<OpApply> oj;
   for (int k = d - 1; k >= kmin; --k) {
     const int64_t n = sp.dims[k], bvr = sp.bv[k + 1], bvl = sp.bv[k];
     const char* S = static_cast<const char*>(sp.S[k]);
     if (k == d - 1) {
       for (auto& it : items)
         for (auto& t : it.terms) {
+          if (t.H) continue;
           const int64_t Rk = t.x->rks[k];
           for (int64_t j = 0; j < it.p[k]; ++j)
             gb.add(int(Rk), int(bvl), int(n), 1.0, t.x->cores[k], 1, Rk, S + size_t(n * bvl * j) * eb, 1, n, 0.0,
                    static_cast<char*>(t.W[k]) + size_t(Rk * bvl * j) * eb, 1, Rk);
         }
       gemm_run(ctx, gb, Prec::F64, sp.prec, sp.prec, sp.prec, "sketch_last");
+      // operator terms: T2 (r R x n) = H_{d-1} applied to the last core, then T2 S_j
+      for (size_t i = 0; i < items.size(); ++i)
+        for (size_t ti = 0; ti < items[i].terms.size(); ++ti) {
+          RandTerm& t = items[i].terms[ti];
+          if (!t.H) continue;
+          const int64_t r = t.x->rks[k], rR = t.rc(k);
+          OpApply o = op_job(t.H, k, true);
+          o.in = t.x->cores[k];
+          o.out = T2[i][ti];
+          o.X = int(r);
+          o.Y = 1;
+          o.is_m = r; o.is_b = 0; o.is_x = 1; o.is_y = 0;
+          o.os_m = rR; o.os_b = r; o.os_x = 1; o.os_y = 0;
+          oj.push_back(o);
+          for (int64_t j = 0; j < items[i].p[k]; ++j)
+            gb.add(int(rR), int(bvl), int(n), 1.0, T2[i][ti], 1, rR, S + size_t(n * bvl * j) * eb, 1, n, 0.0,
+                   static_cast<char*>(t.W[k]) + size_t(rR * bvl * j) * eb, 1, rR);
+        }
+      op_apply_batched(ctx, oj, Prec::F64, sp.prec);
+      oj.clear();
+      gemm_run(ctx, gb, sp.prec, sp.prec, sp.prec, sp.prec, "sketch_last_op");
     } else {
       for (size_t i = 0; i < items.size(); ++i)
         for (size_t ti = 0; ti < items[i].terms.size(); ++ti) {
           RandTerm& t = items[i].terms[ti];
           const int64_t Rk = t.x->rks[k], Rk1 = t.x->rks[k + 1], p = items[i].p[k];
-          gb.add(int(Rk * n), int(bvr * p), int(Rk1), 1.0, t.x->cores[k], 1, Rk * n, t.W[k + 1], 1, Rk1, 0.0,
+          const int64_t Hr1 = t.H ? t.H->rks[k + 1] : 1;
+          // W_{k+1} (Rk1*Hr1 x s) read as Rk1 x (Hr1 * s): columns (H bond, sketch column)
+          gb.add(int(Rk * n), int(Hr1 * bvr * p), int(Rk1), 1.0, t.x->cores[k], 1, Rk * n, t.W[k + 1], 1, Rk1, 0.0,
                  T1[i][ti], 1, Rk * n);
+          if (t.H) {
+            // T2[(a, b), (zeta, c)] = sum_{z, beta} H[zeta, z, b, beta] T1[(a, z), (beta, c)]
+            const int64_t rR = t.rc(k);
+            OpApply o = op_job(t.H, k, true);
+            o.in = T1[i][ti];
+            o.out = T2[i][ti];
+            o.X = int(Rk);
+            o.Y = int(bvr * p);
+            o.is_m = Rk; o.is_b = Rk * n; o.is_x = 1; o.is_y = Rk * n * Hr1;
+            o.os_m = rR; o.os_b = Rk; o.os_x = 1; o.os_y = rR * n;
+            oj.push_back(o);
+          }
         }
       gemm_run(ctx, gb, Prec::F64, sp.prec, sp.prec, sp.prec, "sketch_t1");
+      op_apply_batched(ctx, oj, sp.prec, sp.prec);
+      oj.clear();
       for (size_t i = 0; i < items.size(); ++i)
         for (size_t ti = 0; ti < items[i].terms.size(); ++ti) {
           RandTerm& t = items[i].terms[ti];
-          const int64_t Rk = t.x->rks[k];
+          const int64_t Rk = t.rc(k);
+          const char* A = static_cast<const char*>(t.H ? T2[i][ti] : T1[i][ti]);
           for (int64_t j = 0; j < items[i].p[k]; ++j)
-            gb.add(int(Rk), int(bvl), int(n * bvr), 1.0, static_cast<char*>(T1[i][ti]) + size_t(Rk * n * bvr * j) * eb,
-                   1, Rk, S + size_t(n * bvr * bvl * j) * eb, 1, n * bvr, 0.0,
+            gb.add(int(Rk), int(bvl), int(n * bvr), 1.0, A + size_t(Rk * n * bvr * j) * eb, 1, Rk,
+                   S + size_t(n * bvr * bvl * j) * eb, 1, n * bvr, 0.0,
                    static_cast<char*>(t.W[k]) + size_t(Rk * bvl * j) * eb, 1, Rk);
         }
       gemm_run(ctx, gb, sp.prec, sp.prec, sp.prec, sp.prec, "sketch_w");
@@ -737,7 +818,7 @@ void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items,
     std::vector<Pow2Job> pj;
     for (auto& it : items)
       for (auto& t : it.terms)
-        pj.push_back(Pow2Job{t.W[k], t.x->rks[k], int(t.x->rks[k]), int(it.s[k]), k == d - 1 ? nullptr : t.E + k + 1,
+        pj.push_back(Pow2Job{t.W[k], t.rc(k), int(t.rc(k)), int(it.s[k]), k == d - 1 ? nullptr : t.E + k + 1,
                              t.E + k});
     pow2_normalize_batched(ctx, pj, sp.prec);
   }
@@ -767,14 +848,19 @@ void prepare_plan(ttr_handle_s* h, const ttr_rand_opts* o, int d, const std::vec
   (void)h;
 }
 
-// ttrand_rounding(alpha, y, rks) (src/tt_randtools.jl:223-276), batched
+// ttrand_rounding(alpha, y, rks) (src/tt_randtools.jl:223-276), batched; a first
+// term with an operator is alpha_1 H y_1 with the implicit operator sketch and
+// projection of ttrand_rounding(alpha, A, y, rks) (:285-354)
 std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vector<RandItem>& items,
                                                      const ttr_rand_opts* o) {
   Ctx& ctx = h->ctx;
   const DevTT* x0 = items[0].terms[0].x;
   const int d = x0->d;
   for (auto& it : items)
-    for (auto& t : it.terms) TTR_REQUIRE(t.x->dims == x0->dims, ST_DIM_MISMATCH, "all terms must share dims");
+    for (auto& t : it.terms) {
+      TTR_REQUIRE(t.x->dims == x0->dims, ST_DIM_MISMATCH, "all terms must share dims");
+      if (t.H) TTR_REQUIRE(t.H->dims == x0->dims, ST_DIM_MISMATCH, "operator and TT dims differ");
+    }
   SketchPlan sp;
   prepare_plan(h, o, d, x0->dims, items, sp);
   for (auto& it : items)
@@ -790,10 +876,31 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
     orr[i] = r;
   }
   GemmBatch gb;
+  std::vector<OpApply> oj;
   if (d == 1) {
+    // a single core: sum_t alpha_t (H_t) x_t
+    for (size_t i = 0; i < items.size(); ++i)
+      for (auto& t : items[i].terms) {
+        if (!t.H) continue;
+        OpApply a = op_job(t.H, 0, false);
+        a.in = t.x->cores[0];
+        a.out = outs[i]->cores[0];
+        a.X = 1; a.Y = 1;
+        a.is_m = 1; a.os_m = 1;
+        a.scale = t.alpha;
+        oj.push_back(a);
+      }
+    op_apply_batched(ctx, oj, Prec::F64, Prec::F64);
     for (size_t i = 0; i < items.size(); ++i) {
-      gb.prob(outs[i]->cores[0], 1, 1, 1, int(sp.dims[0]), 1.0, 0.0);
-      for (auto& t : items[i].terms) gb.seg(h->one, 1, 1, t.x->cores[0], 1, 1, 1, t.alpha);
+      bool op = false;
+      for (auto& t : items[i].terms) op = op || t.H;
+      bool any = false;
+      for (auto& t : items[i].terms) {
+        if (t.H) continue;
+        if (!any) gb.prob(outs[i]->cores[0], 1, 1, 1, int(sp.dims[0]), 1.0, op ? 1.0 : 0.0);
+        any = true;
+        gb.seg(h->one, 1, 1, t.x->cores[0], 1, 1, 1, t.alpha);
+      }
     }
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "rr_d1");
     return outs;
@@ -810,27 +917,45 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
     for (int k = 0; k + 1 < d; ++k) zmax = std::max(zmax, orr[i][k] * sp.dims[k] * items[i].s[k + 1]);
     items[i].Z = ctx.ws.alloc_n<double>(size_t(zmax));
     for (auto& t : items[i].terms) {
-      int64_t ymax = 1, pmax = 1;
+      int64_t ymax = sp.dims[0] * t.rc(1), pmax = 1, umax = 1;
       for (int k = 0; k + 1 < d; ++k) {
-        ymax = std::max(ymax, orr[i][k + 1] * sp.dims[k + 1] * t.x->rks[k + 2]);
-        pmax = std::max(pmax, orr[i][k + 1] * t.x->rks[k + 1]);
+        ymax = std::max(ymax, orr[i][k + 1] * sp.dims[k + 1] * t.rc(k + 2));
+        pmax = std::max(pmax, orr[i][k + 1] * t.rc(k + 1));
+        if (t.H) umax = std::max(umax, orr[i][k + 1] * sp.dims[k + 1] * t.x->rks[k + 2] * t.H->rks[k + 1]);
       }
       t.Y[0] = ctx.ws.alloc_n<double>(size_t(ymax));
       t.Y[1] = ctx.ws.alloc_n<double>(size_t(ymax));
       t.Pb = ctx.ws.alloc_n<double>(size_t(pmax));
+      if (t.H) t.U = ctx.ws.alloc_n<double>(size_t(umax));
     }
   }
+  // operator terms: the first core of H x (1 x n x (r_1 R_1), x rank fastest)
+  for (size_t i = 0; i < items.size(); ++i)
+    for (auto& t : items[i].terms) {
+      if (!t.H) continue;
+      const int64_t n0 = sp.dims[0], r1 = t.x->rks[1];
+      OpApply a = op_job(t.H, 0, false);
+      a.in = t.x->cores[0];
+      a.out = t.Y[0];
+      a.X = 1;
+      a.Y = int(r1);
+      a.is_m = 1; a.is_b = 0; a.is_x = 0; a.is_y = n0;
+      a.os_m = 1; a.os_b = n0 * r1; a.os_x = 0; a.os_y = n0;
+      oj.push_back(a);
+    }
+  op_apply_batched(ctx, oj, Prec::F64, Prec::F64);
+  oj.clear();
   int yb = 0;
   for (int k = 0; k + 1 < d; ++k) {
     const int64_t n = sp.dims[k];
+    auto Ycur = [&](const RandTerm& t) -> const double* { return (k == 0 && !t.H) ? t.x->cores[0] : t.Y[yb]; };
     // Z_k = sum_t Y_t W_t (FP64 accumulate; terms rescaled by their sketch exponents)
     for (size_t i = 0; i < items.size(); ++i) {
       const int64_t r = orr[i][k], s = items[i].s[k + 1];
       gb.prob(items[i].Z, 1, r * n, int(r * n), int(s), 1.0, 0.0);
       for (auto& t : items[i].terms) {
-        const int64_t R1 = t.x->rks[k + 1];
-        const double* Y = k == 0 ? t.x->cores[0] : t.Y[yb];
-        gb.seg(Y, 1, r * n, t.W[k + 1], 1, R1, int(R1), k == 0 ? t.alpha : 1.0, AM_PLAIN, t.E + k + 1);
+        const int64_t R1 = t.rc(k + 1);
+        gb.seg(Ycur(t), 1, r * n, t.W[k + 1], 1, R1, int(R1), k == 0 ? t.alpha : 1.0, AM_PLAIN, t.E + k + 1);
       }
     }
     gemm_run(ctx, gb, Prec::F64, sp.prec, Prec::F64, Prec::F64, "rr_Z");
@@ -844,25 +969,54 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
     for (size_t i = 0; i < items.size(); ++i) {
       const int64_t r = orr[i][k], r1 = orr[i][k + 1];
       for (auto& t : items[i].terms) {
-        const int64_t R1 = t.x->rks[k + 1];
-        const double* Y = k == 0 ? t.x->cores[0] : t.Y[yb];
+        const int64_t R1 = t.rc(k + 1);
         gb.prob(t.Pb, 1, r1, int(r1), int(R1), 1.0, 0.0);
-        gb.seg(outs[i]->cores[k], r * n, 1, Y, 1, r * n, int(r * n), k == 0 ? t.alpha : 1.0);
+        gb.seg(outs[i]->cores[k], r * n, 1, Ycur(t), 1, r * n, int(r * n), k == 0 ? t.alpha : 1.0);
       }
     }
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "rr_P");
-    // Y_t <- P_t X_{t,k+1}; the last core sums the terms
+    // Y_t <- P_t X_{t,k+1}; the last core sums the terms.  Operator terms: U_beta =
+    // P_beta X_{k+1} for every left H bond beta, then H_{k+1} applied over
+    // (mode, beta) gives the next Y (or the term's part of the last core)
     const int64_t n1 = sp.dims[k + 1];
+    const bool last = k + 2 == d;
+    std::vector<char> has_op(items.size(), 0);
     for (size_t i = 0; i < items.size(); ++i) {
       const int64_t r1 = orr[i][k + 1];
-      if (k + 2 < d) {
+      for (auto& t : items[i].terms) {
+        if (!t.H) continue;
+        has_op[i] = 1;
+        const int64_t ry1 = t.x->rks[k + 1], ry2 = t.x->rks[k + 2], Hl = t.H->rks[k + 1];
+        for (int64_t b = 0; b < Hl; ++b)
+          gb.add(int(r1), int(n1 * ry2), int(ry1), 1.0, t.Pb + r1 * ry1 * b, 1, r1, t.x->cores[k + 1], 1, ry1, 0.0,
+                 t.U + r1 * n1 * ry2 * b, 1, r1);
+        OpApply a = op_job(t.H, k + 1, false);
+        a.in = t.U;
+        a.out = last ? outs[i]->cores[k + 1] : t.Y[yb ^ 1];
+        a.X = int(r1);
+        a.Y = int(ry2);
+        a.is_m = r1; a.is_b = r1 * n1 * ry2; a.is_x = 1; a.is_y = r1 * n1;
+        a.os_m = r1; a.os_b = r1 * n1 * ry2; a.os_x = 1; a.os_y = r1 * n1;
+        oj.push_back(a);
+      }
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "rr_U");
+    op_apply_batched(ctx, oj, Prec::F64, Prec::F64);
+    oj.clear();
+    for (size_t i = 0; i < items.size(); ++i) {
+      const int64_t r1 = orr[i][k + 1];
+      if (!last) {
         for (auto& t : items[i].terms) {
+          if (t.H) continue;
           const int64_t R1 = t.x->rks[k + 1], R2 = t.x->rks[k + 2];
           gb.add(int(r1), int(n1 * R2), int(R1), 1.0, t.Pb, 1, r1, t.x->cores[k + 1], 1, R1, 0.0, t.Y[yb ^ 1], 1, r1);
         }
       } else {
-        gb.prob(outs[i]->cores[k + 1], 1, r1, int(r1), int(n1), 1.0, 0.0);
+        bool any = false;
         for (auto& t : items[i].terms) {
+          if (t.H) continue;
+          if (!any) gb.prob(outs[i]->cores[k + 1], 1, r1, int(r1), int(n1), 1.0, has_op[i] ? 1.0 : 0.0);
+          any = true;
           const int64_t R1 = t.x->rks[k + 1];
           gb.seg(t.Pb, 1, r1, t.x->cores[k + 1], 1, R1, int(R1), 1.0);
         }
@@ -880,11 +1034,11 @@ std::vector<RandItem> parse_items(ttr_handle_s* h, int64_t n, const ttr_rand_ite
   for (int64_t i = 0; i < n; ++i) {
     const ttr_rand_item& it = items[i];
     TTR_REQUIRE(it.nterms >= 1 && it.term_ids && it.alpha && it.rks, ST_INVALID_ARG, "bad randomized item");
-    TTR_REQUIRE(it.op_id < 0, ST_INVALID_ARG, "op_id must be -1 (use ttr_mpo_apply for products)");
     for (int64_t t = 0; t < it.nterms; ++t) {
       RandTerm rt{};
       rt.x = h->get(it.term_ids[t]);
       rt.alpha = it.alpha[t];
+      if (t == 0 && it.op_id >= 0) rt.H = h->get_op(it.op_id);
       out[i].terms.push_back(rt);
     }
     const int d = out[i].terms[0].x->d;
@@ -1027,8 +1181,12 @@ int ttr_tt_download_core(ttr_handle_t h, int64_t id, int64_t k, double* dst) {
 
 int ttr_tt_free(ttr_handle_t h, int64_t id) {
   return guarded(h, [&] {
-    DevTT* t = h->get(id);
     h->ctx.sync();
+    if (h->ops.count(id)) {  // operators share the id space
+      h->ops.erase(id);
+      return;
+    }
+    DevTT* t = h->get(id);
     h->free_tt(t);
     h->tts.erase(id);
   });
@@ -1239,47 +1397,60 @@ int ttr_round_rand(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const 
   });
 }
 
+namespace {
+// reverse recursive sketch of x (or of H x without forming it) into host arrays
+void sketch_one(ttr_handle_s* h, DevTT* x, DevTTO* H, const int64_t* rks, const ttr_rand_opts* opts,
+                double* const* W_out, int64_t* sketch_rks) {
+  TTR_REQUIRE(rks != nullptr, ST_INVALID_ARG, "rks must not be NULL");
+  const int d = x->d;
+  TTR_REQUIRE(x->rks[d] == 1, ST_INVALID_ARG, "rks[d] must be 1");
+  if (H) TTR_REQUIRE(H->dims == x->dims, ST_DIM_MISMATCH, "operator and TT dims differ");
+  std::vector<RandItem> its(1);
+  RandTerm t{};
+  t.x = x;
+  t.H = H;
+  t.alpha = 1.0;
+  its[0].terms.push_back(t);
+  its[0].rks.assign(rks, rks + d + 1);
+  SketchPlan sp;
+  prepare_plan(h, opts, d, x->dims, its, sp);
+  make_sketch(h, sp, 0);
+  sketch_phase(h, sp, its, 0);
+  h->ctx.sync();
+  const RandTerm& tt = its[0].terms[0];
+  std::vector<int32_t> E(static_cast<size_t>(d + 1));
+  TTR_HIP(hipMemcpy(E.data(), tt.E, size_t(d + 1) * 4, hipMemcpyDeviceToHost));
+  const size_t eb = prec_bytes(sp.prec);
+  for (int k = 0; k <= d; ++k) {
+    if (sketch_rks) sketch_rks[k] = its[0].s[k];
+    if (!W_out || !W_out[k]) continue;
+    const int64_t cnt = tt.rc(k) * its[0].s[k];
+    const double scale = std::ldexp(1.0, k == d ? 0 : E[k]) / std::sqrt(double(its[0].p[k]));
+    if (k == d) {
+      for (int64_t e = 0; e < cnt; ++e) W_out[k][e] = 1.0 / std::sqrt(double(its[0].p[k]));
+      continue;
+    }
+    std::vector<char> raw(size_t(cnt) * eb);
+    TTR_HIP(hipMemcpy(raw.data(), tt.W[k], raw.size(), hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < cnt; ++e) {
+      double v;
+      if (sp.prec == Prec::F64) std::memcpy(&v, raw.data() + 8 * e, 8);
+      else if (sp.prec == Prec::F32) { float f; std::memcpy(&f, raw.data() + 4 * e, 4); v = f; }
+      else { uint16_t b; std::memcpy(&b, raw.data() + 2 * e, 2); v = bf16_bits_to_float(b); }
+      W_out[k][e] = v * scale;
+    }
+  }
+}
+}  // namespace
+
 int ttr_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts, double* const* W_out,
                int64_t* sketch_rks) {
-  return guarded(h, [&] {
-    DevTT* x = h->get(tt_id);
-    TTR_REQUIRE(rks != nullptr, ST_INVALID_ARG, "rks must not be NULL");
-    const int d = x->d;
-    TTR_REQUIRE(x->rks[d] == 1, ST_INVALID_ARG, "rks[d] must be 1");
-    std::vector<RandItem> its(1);
-    RandTerm t{};
-    t.x = x;
-    t.alpha = 1.0;
-    its[0].terms.push_back(t);
-    its[0].rks.assign(rks, rks + d + 1);
-    SketchPlan sp;
-    prepare_plan(h, opts, d, x->dims, its, sp);
-    make_sketch(h, sp, 0);
-    sketch_phase(h, sp, its, 0);
-    h->ctx.sync();
-    std::vector<int32_t> E(static_cast<size_t>(d + 1));
-    TTR_HIP(hipMemcpy(E.data(), its[0].terms[0].E, size_t(d + 1) * 4, hipMemcpyDeviceToHost));
-    const size_t eb = prec_bytes(sp.prec);
-    for (int k = 0; k <= d; ++k) {
-      if (sketch_rks) sketch_rks[k] = its[0].s[k];
-      if (!W_out || !W_out[k]) continue;
-      const int64_t cnt = x->rks[k] * its[0].s[k];
-      const double scale = std::ldexp(1.0, k == d ? 0 : E[k]) / std::sqrt(double(its[0].p[k]));
-      if (k == d) {
-        for (int64_t e = 0; e < cnt; ++e) W_out[k][e] = 1.0 / std::sqrt(double(its[0].p[k]));
-        continue;
-      }
-      std::vector<char> raw(size_t(cnt) * eb);
-      TTR_HIP(hipMemcpy(raw.data(), its[0].terms[0].W[k], raw.size(), hipMemcpyDeviceToHost));
-      for (int64_t e = 0; e < cnt; ++e) {
-        double v;
-        if (sp.prec == Prec::F64) std::memcpy(&v, raw.data() + 8 * e, 8);
-        else if (sp.prec == Prec::F32) { float f; std::memcpy(&f, raw.data() + 4 * e, 4); v = f; }
-        else { uint16_t b; std::memcpy(&b, raw.data() + 2 * e, 2); v = bf16_bits_to_float(b); }
-        W_out[k][e] = v * scale;
-      }
-    }
-  });
+  return guarded(h, [&] { sketch_one(h, h->get(tt_id), nullptr, rks, opts, W_out, sketch_rks); });
+}
+
+int ttr_sketch_op(ttr_handle_t h, int64_t op_id, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts,
+                  double* const* W_out, int64_t* sketch_rks) {
+  return guarded(h, [&] { sketch_one(h, h->get(tt_id), h->get_op(op_id), rks, opts, W_out, sketch_rks); });
 }
 
 int ttr_sketch_blocks(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, const ttr_rand_opts* opts,

@@ -243,6 +243,40 @@ __global__ void mpo_kernel(const MpoCore* __restrict__ jobs) {
   }
 }
 
+// Local action of an operator core on (mode, bond) pairs, one thread per fiber:
+// out[(om, ob)] = scale * sum_{im, ib} H(om, ob, im, ib) in[(im, ib)].  The
+// (n*rout) x (n*rin) matrix is staged in LDS; a thread keeps its MAXO outputs in
+// registers and reads each of its inputs once.
+template <int MAXO, class TI, class TO>
+__global__ __launch_bounds__(256) void op_apply_kernel(const OpApply* __restrict__ jobs) {
+  const OpApply J = jobs[blockIdx.y];
+  __shared__ double M[64 * 64];
+  const int n = J.n, din = J.n * J.rin, dout = J.n * J.rout;
+  for (int e = threadIdx.x; e < din * dout; e += blockDim.x) {
+    const int o = e % dout, i = e / dout;
+    M[e] = J.H[(o % n) * J.h_om + (o / n) * J.h_ob + (i % n) * J.h_im + (i / n) * J.h_ib];
+  }
+  __syncthreads();
+  const int64_t nf = int64_t(J.X) * J.Y;
+  for (int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; f < nf; f += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t x = f % J.X, y = f / J.X;
+    const TI* in = static_cast<const TI*>(J.in) + x * J.is_x + y * J.is_y;
+    double acc[MAXO];
+#pragma unroll
+    for (int o = 0; o < MAXO; ++o) acc[o] = 0.0;
+    for (int i = 0; i < din; ++i) {
+      const double v = ld_as_d<TI>(in, (i % n) * J.is_m + (i / n) * J.is_b);
+#pragma unroll
+      for (int o = 0; o < MAXO; ++o)
+        if (o < dout) acc[o] += M[o + dout * i] * v;
+    }
+    TO* out = static_cast<TO*>(J.out) + x * J.os_x + y * J.os_y;
+#pragma unroll
+    for (int o = 0; o < MAXO; ++o)
+      if (o < dout) st_from_d<TO>(out, (o % n) * J.os_m + (o / n) * J.os_b, acc[o] * J.scale);
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host launchers
@@ -306,6 +340,52 @@ void mpo_apply_batched(Ctx& ctx, const std::vector<MpoCore>& jobs) {
   int64_t mx = 0;
   for (auto& j : jobs) mx = std::max<int64_t>(mx, int64_t(j.Ra) * j.rx * j.n * j.Ra2 * j.rx2);
   launch_y(ctx, mpo_kernel, jobs, mx);
+}
+
+namespace {
+template <int MAXO, class TI, class TO>
+void launch_op_apply(Ctx& ctx, const OpApply* d, int64_t njobs, int64_t bx) {
+  hipLaunchKernelGGL((op_apply_kernel<MAXO, TI, TO>), dim3(unsigned(bx), unsigned(njobs)), dim3(256), 0, ctx.stream, d);
+}
+template <class TI, class TO>
+void launch_op_apply_o(Ctx& ctx, int maxo, const OpApply* d, int64_t njobs, int64_t bx) {
+  if (maxo <= 8) launch_op_apply<8, TI, TO>(ctx, d, njobs, bx);
+  else if (maxo <= 16) launch_op_apply<16, TI, TO>(ctx, d, njobs, bx);
+  else if (maxo <= 32) launch_op_apply<32, TI, TO>(ctx, d, njobs, bx);
+  else launch_op_apply<64, TI, TO>(ctx, d, njobs, bx);
+}
+template <class TI>
+void launch_op_apply_i(Ctx& ctx, Prec out, int maxo, const OpApply* d, int64_t njobs, int64_t bx) {
+  if (out == Prec::F64) launch_op_apply_o<TI, double>(ctx, maxo, d, njobs, bx);
+  else if (out == Prec::F32) launch_op_apply_o<TI, float>(ctx, maxo, d, njobs, bx);
+  else launch_op_apply_o<TI, uint16_t>(ctx, maxo, d, njobs, bx);
+}
+}  // namespace
+
+void op_apply_batched(Ctx& ctx, const std::vector<OpApply>& jobs, Prec in, Prec out) {
+  if (jobs.empty()) return;
+  int maxo = 1;
+  int64_t mx = 1;
+  double fl = 0.0, by = 0.0;
+  for (const auto& j : jobs) {
+    TTR_REQUIRE(j.n * j.rin <= 64 && j.n * j.rout <= 64, ST_INVALID_ARG,
+                "implicit operator products need n*R <= 64 on every bond");
+    maxo = std::max(maxo, j.n * j.rout);
+    const int64_t nf = int64_t(j.X) * j.Y;
+    mx = std::max(mx, nf);
+    fl += 2.0 * nf * j.n * j.rin * j.n * j.rout;
+    by += double(nf) * (j.n * j.rin * prec_bytes(in) + j.n * j.rout * prec_bytes(out));
+  }
+  ProfScope ps(ctx, "op_apply", fl, by);
+  for (size_t s = 0; s < jobs.size(); s += 65535) {
+    const size_t cnt = std::min<size_t>(65535, jobs.size() - s);
+    const OpApply* d = ctx.upload(jobs.data() + s, cnt);
+    const int64_t bx = std::min<int64_t>(1024, std::max<int64_t>(1, (mx + 255) / 256));
+    if (in == Prec::F64) launch_op_apply_i<double>(ctx, out, maxo, d, int64_t(cnt), bx);
+    else if (in == Prec::F32) launch_op_apply_i<float>(ctx, out, maxo, d, int64_t(cnt), bx);
+    else launch_op_apply_i<uint16_t>(ctx, out, maxo, d, int64_t(cnt), bx);
+    TTR_HIP(hipGetLastError());
+  }
 }
 
 void pow2_normalize_batched(Ctx& ctx, const std::vector<Pow2Job>& jobs, Prec prec) {

@@ -343,6 +343,24 @@ struct MpoCore {
 };
 void mpo_apply_batched(Ctx& ctx, const std::vector<MpoCore>& jobs);
 
+// Local action of an operator core on the (mode, bond) pairs of every fiber
+// (x, y) of a 2-D grid, without forming an MPO x TT product:
+//   out[(om, ob)] = scale * sum_{im, ib} H(om, ob, im, ib) in[(im, ib)],
+// H(om, ob, im, ib) = H[om*h_om + ob*h_ob + im*h_im + ib*h_ib] (H is a core of a
+// TToperator in the reference layout), in[(im, ib)] at in + x*is_x + y*is_y +
+// im*is_m + ib*is_b, out likewise.  n*rin and n*rout must not exceed 64.
+struct OpApply {
+  const void* in;
+  void* out;
+  const double* H;
+  int64_t h_om, h_ob, h_im, h_ib;
+  int32_t n, rin, rout, X, Y;
+  int64_t is_m, is_b, is_x, is_y;
+  int64_t os_m, os_b, os_x, os_y;
+  double scale;
+};
+void op_apply_batched(Ctx& ctx, const std::vector<OpApply>& jobs, Prec in, Prec out);
+
 void zero_fill(Ctx& ctx, void* p, size_t bytes);
 
 }  // namespace ttr

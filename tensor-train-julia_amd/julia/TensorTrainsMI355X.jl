@@ -118,20 +118,47 @@ end
 
 const PREC = Dict(Float64 => Int32(0), Float32 => Int32(1))
 
-function ttrand_rounding(α::Vector{Float64}, y::Vector{TTvector{Float64,N}}, rks=TensorTrains.default_rank_heuristic(y);
-                         orthogonal=true, seed=1234, block_rks::Int=N, sketch_type::Type=Float64) where {N}
+function upload(A::TToperator{Float64,N}) where {N}
+    id = Ref{Int64}(0)
+    ptrs = [pointer(c) for c in A.tto_vec]
+    GC.@preserve A begin
+        check(ccall((:ttr_tto_upload, lib), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Ptr{Float64}}, Ref{Int64}),
+                    handle(), N, collect(Int64, A.tto_dims), collect(Int64, A.tto_rks), ptrs, id))
+    end
+    return id[]
+end
+
+# alpha_1 (A) y_1 + sum_{j>1} alpha_j y_j; with A given the product is never formed
+function rand_round(α::Vector{Float64}, A, y::Vector{TTvector{Float64,N}}, rks;
+                    orthogonal=true, seed=1234, block_rks::Int=N, sketch_type::Type=Float64) where {N}
     ids = [upload(t) for t in y]
+    op = A === nothing ? -1 : upload(A)
     r = collect(Int64, rks)
     out = Ref{Int64}(0)
     GC.@preserve α ids r begin
-        item = Ref(RandItem(length(y), pointer(α), pointer(ids), -1, pointer(r)))
+        item = Ref(RandItem(length(y), pointer(α), pointer(ids), op, pointer(r)))
         opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[sketch_type], block_rks, UInt64(seed), C_NULL, 1.0))
         check(ccall((:ttr_round_rand, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, Ref{Int64}),
                     handle(), 1, item, opts, out))
     end
     foreach(free, ids)
+    op >= 0 && free(op)
     return download(out[], Val(N))
 end
+
+ttrand_rounding(α::Vector{Float64}, y::Vector{TTvector{Float64,N}}, rks=TensorTrains.default_rank_heuristic(y);
+                kwargs...) where {N} = rand_round(α, nothing, y, rks; kwargs...)
+
+ttrand_rounding(α::Vector{Float64}, A::TToperator{Float64,N}, y::Vector{TTvector{Float64,N}},
+                rks=TensorTrains.default_rank_heuristic(y); kwargs...) where {N} = rand_round(α, A, y, rks; kwargs...)
+ttrand_rounding(α::Vector{Float64}, A::TToperator{Float64,N}, y::Vector{TTvector{Float64,N}}, rmax::Int;
+                kwargs...) where {N} = rand_round(α, A, y, vcat(1, fill(rmax, N - 1), 1); kwargs...)
+
+# A y - b (src/tt_randtools.jl:169-214)
+ttrand_rounding(A::TToperator{Float64,N}, y::TTvector{Float64,N}, b::TTvector{Float64,N},
+                rks=TensorTrains.default_rank_heuristic(y); kwargs...) where {N} =
+    rand_round([1.0, -1.0], A, [y, b], rks; kwargs...)
 
 ttrand_rounding(y::TTvector{Float64,N}, rks=TensorTrains.default_rank_heuristic(y); kwargs...) where {N} =
     ttrand_rounding([1.0], [y], rks; kwargs...)

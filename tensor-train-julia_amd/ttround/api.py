@@ -147,6 +147,11 @@ class DeviceTTO:
         self.dims = tuple(A.tto_dims)
         self.rks = list(A.tto_rks)
 
+    def free(self):
+        if self.id is not None and self.handle.h:
+            self.handle.check(self.handle.lib.ttr_tt_free(self.handle.h, self.id))
+        self.id = None
+
 
 def lincomb_device(coeffs: Sequence[float], xs: Sequence[DeviceTT]) -> DeviceTT:
     h = xs[0].handle
@@ -196,11 +201,17 @@ def round_rand_batch(items, orthogonal: bool = True, seed: int = 1234, block_rks
                      sketch_prec: str = "f64", p=None, oversampling: float = 1.0) -> List[DeviceTT]:
     """Batched randomized rounding of linear combinations.
 
-    ``items`` is a list of ``(alphas, [DeviceTT...], rks)`` triples."""
-    h = items[0][1][0].handle
+    ``items`` is a list of ``(alphas, [DeviceTT...], rks)`` triples, or of
+    ``(alphas, DeviceTTO, [DeviceTT...], rks)`` quadruples for
+    ``alpha_1 H y_1 + sum_{j>1} alpha_j y_j`` (implicit operator product)."""
+    h = items[0][-2][0].handle
     keep = []
     arr = (L.RandItem * len(items))()
-    for i, (alpha, terms, rks) in enumerate(items):
+    for i, item in enumerate(items):
+        if len(item) == 4:
+            alpha, op, terms, rks = item
+        else:
+            (alpha, terms, rks), op = item, None
         a = np.ascontiguousarray(np.asarray(alpha, dtype=np.float64))
         ids = L.as_i64([t.id for t in terms])
         r = L.as_i64(rks)
@@ -208,7 +219,7 @@ def round_rand_batch(items, orthogonal: bool = True, seed: int = 1234, block_rks
         arr[i].nterms = len(terms)
         arr[i].alpha = L.ptr_f64(a)
         arr[i].term_ids = L.ptr_i64(ids)
-        arr[i].op_id = -1
+        arr[i].op_id = -1 if op is None else op.id
         arr[i].rks = L.ptr_i64(r)
     o, k2 = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling)
     out = np.zeros(len(items), np.int64)
@@ -306,9 +317,11 @@ class TToperator:
 
     def __mul__(self, x: TTvector) -> TTvector:
         """``A*x`` (``src/tt_operations.jl:170-194``)."""
-        dev = mpo_apply_device(DeviceTTO(self), x._dev())
+        op, xd = DeviceTTO(self), x._dev()
+        dev = mpo_apply_device(op, xd)
         out = dev.to_host()
-        dev.free()
+        for d in (dev, xd, op):
+            d.free()
         return out
 
 
@@ -427,37 +440,62 @@ def _rks_arg(rks, N):
 
 def ttrand_rounding(*args, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
                     sketch_prec: str = "f64") -> TTvector:
-    """Randomized rounding (``src/tt_randtools.jl:112-167,223-283``).
+    """Randomized rounding (``src/tt_randtools.jl:112-361``).
 
     Forms: ``ttrand_rounding(y)``, ``ttrand_rounding(y, rks_or_rmax)``,
-    ``ttrand_rounding(alpha, ys)``, ``ttrand_rounding(alpha, ys, rks_or_rmax)``.
-    The sketch draws come from the library's counter-based Philox stream keyed
-    by ``seed``; ``sketch_prec`` selects the precision of the sketch
-    contractions (the QR and projections are FP64)."""
+    ``ttrand_rounding(alpha, ys[, rks_or_rmax])``,
+    ``ttrand_rounding(alpha, A, ys[, rks_or_rmax])`` (``alpha_1 A y_1 +
+    sum_{j>1} alpha_j y_j``, the product never formed) and
+    ``ttrand_rounding(A, y, b[, rks_or_rmax])`` (``A y - b``).  The default
+    ranks follow ``default_rank_heuristic`` of ``y`` (or of ``ys``).  The sketch
+    draws come from the library's counter-based Philox stream keyed by ``seed``;
+    ``sketch_prec`` selects the precision of the sketch contractions (the QR and
+    projections are FP64)."""
+    op = None
     if isinstance(args[0], TTvector):
         y = args[0]
         rks = _rks_arg(args[1], y.N) if len(args) > 1 else default_rank_heuristic(y)
         alpha, ys = [1.0], [y]
+    elif isinstance(args[0], TToperator):
+        op, y, b = args[0], args[1], args[2]
+        if y.ttv_dims != b.ttv_dims or tuple(op.tto_dims) != tuple(y.ttv_dims):
+            raise L.DimensionMismatch("A, y and b must share dims")
+        rks = _rks_arg(args[3], y.N) if len(args) > 3 else default_rank_heuristic(y)
+        alpha, ys = [1.0, -1.0], [y, b]
     else:
-        alpha, ys = list(args[0]), list(args[1])
+        alpha = list(args[0])
+        if isinstance(args[1], TToperator):
+            op, ys, rest = args[1], list(args[2]), args[3:]
+        else:
+            ys, rest = list(args[1]), args[2:]
         if len(alpha) != len(ys) or any(t.ttv_dims != ys[0].ttv_dims for t in ys):
             raise L.DimensionMismatch("length(alpha) must equal length(y) and all dims must agree")
-        rks = _rks_arg(args[2], ys[0].N) if len(args) > 2 else default_rank_heuristic(ys)
+        if op is not None and tuple(op.tto_dims) != tuple(ys[0].ttv_dims):
+            raise L.DimensionMismatch("operator and TT dims differ")
+        rks = _rks_arg(rest[0], ys[0].N) if rest else default_rank_heuristic(ys)
     devs = [t._dev() for t in ys]
-    out = round_rand_batch([(alpha, devs, rks)], orthogonal=orthogonal, seed=seed, block_rks=block_rks,
+    dop = DeviceTTO(op) if op is not None else None
+    item = (alpha, dop, devs, rks) if dop is not None else (alpha, devs, rks)
+    out = round_rand_batch([item], orthogonal=orthogonal, seed=seed, block_rks=block_rks,
                            sketch_prec=sketch_prec)[0]
     res = out.to_host()
-    for d in devs + [out]:
+    for d in devs + [out] + ([dop] if dop is not None else []):
         d.free()
     return res
 
 
-def tt_recursive_sketch(A: TTvector, rks, orthogonal: bool = True, reverse: bool = True, seed: int = 1234,
+def tt_recursive_sketch(*args, orthogonal: bool = True, reverse: bool = True, seed: int = 1234,
                         block_rks: Optional[int] = None, p=None, oversampling: float = 1.0,
                         sketch_prec: str = "f64"):
-    """``tt_recursive_sketch(A, rks; reverse=true, ...)`` (``src/sketches.jl:175-315``).
+    """``tt_recursive_sketch([H,] A, rks; reverse=true, ...)`` (``src/sketches.jl:175-315``;
+    operator form ``:317-472``, the product ``H A`` never formed).
 
-    Returns ``(W, sketch_rks)`` with ``W[k]`` of shape ``(r_k, sketch_rks[k])``."""
+    Returns ``(W, sketch_rks)`` with ``W[k]`` of shape ``(r_k, sketch_rks[k])``, or
+    ``(r_k, R_k, sketch_rks[k])`` for the operator form."""
+    if isinstance(args[0], TToperator):
+        H, A, rks = args
+    else:
+        H, (A, rks) = None, args
     if not reverse:
         raise NotImplementedError("only the reverse (right-to-left) sketch is provided")
     if isinstance(rks, (int, np.integer)):
@@ -466,13 +504,22 @@ def tt_recursive_sketch(A: TTvector, rks, orthogonal: bool = True, reverse: bool
         rks = r
     dev = A._dev()
     h = dev.handle
+    dop = DeviceTTO(H, h) if H is not None else None
     o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling)
-    srk = np.zeros(A.N + 1, np.int64)
     ra = L.as_i64(rks)
-    # first call: learn the sketch ranks
-    h.check(h.lib.ttr_sketch(h.h, dev.id, L.ptr_i64(ra), C.byref(o), C.cast(None, L.f64pp), L.ptr_i64(srk)))
-    Ws = [np.zeros((A.ttv_rks[k], int(srk[k])), order="F") for k in range(A.N + 1)]
+    bv, pp = block_structure(A.ttv_dims, rks, block_rks, p, oversampling)
+    srk = [bv[k] * pp[k] for k in range(A.N + 1)]
+    Hr = list(H.tto_rks) if H is not None else [1] * (A.N + 1)
+    Ws = [np.zeros((A.ttv_rks[k] * Hr[k], srk[k]), order="F") for k in range(A.N + 1)]
     arr = (L.f64p * (A.N + 1))(*[L.ptr_f64(w) for w in Ws])
-    h.check(h.lib.ttr_sketch(h.h, dev.id, L.ptr_i64(ra), C.byref(o), arr, L.ptr_i64(srk)))
+    out_srk = np.zeros(A.N + 1, np.int64)
+    if dop is None:
+        h.check(h.lib.ttr_sketch(h.h, dev.id, L.ptr_i64(ra), C.byref(o), arr, L.ptr_i64(out_srk)))
+    else:
+        h.check(h.lib.ttr_sketch_op(h.h, dop.id, dev.id, L.ptr_i64(ra), C.byref(o), arr, L.ptr_i64(out_srk)))
+        dop.free()
     dev.free()
-    return Ws, [int(s) for s in srk]
+    assert [int(v) for v in out_srk] == srk
+    if H is not None:
+        Ws = [w.reshape((A.ttv_rks[k], Hr[k], srk[k]), order="F") for k, w in enumerate(Ws)]
+    return Ws, srk

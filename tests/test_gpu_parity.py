@@ -271,6 +271,111 @@ def test_ttrand_rounding_large_ranks(T, prec, tol):
     assert err < tol
 
 
+# ------------------------------------------------------------------ implicit operator products
+
+def to_api_op(H):
+    import ttround as T
+    return T.TToperator(H.N, [c.copy() for c in H.cores], H.dims, list(H.rks))
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6), ("bf16", 3e-2)])
+@pytest.mark.parametrize("orthogonal,block_rks", [(False, None), (True, 3)])
+def test_recursive_sketch_op(T, prec, tol, orthogonal, block_rks):
+    """Sketch of H x without forming it vs the oracle on the device's blocks."""
+    say(f"case: operator sketch {prec} orthogonal={orthogonal} block_rks={block_rks}")
+    H = O.hubbard_1d_mpo(5, 1.0, 4.0)
+    dims = H.dims
+    x = rand_tt(dims, [1, 3, 5, 6, 4, 1], 71)
+    rks = [1, 4, 9, 9, 4, 1]
+    W, s = T.tt_recursive_sketch(to_api_op(H), to_api(x), rks, orthogonal=orthogonal, seed=6, block_rks=block_rks,
+                                 sketch_prec=prec)
+    prov = device_blocks_provider(dims, rks, orthogonal=orthogonal, seed=6, block_rks=block_rks, sketch_prec=prec)
+    Wr, sr = O.tt_recursive_sketch_op(H, x, rks, prov, orthogonal, block_rks)
+    assert s == sr
+    for a, b in zip(W, Wr):
+        assert a.shape == b.shape
+        assert np.linalg.norm(a - b) <= tol * np.linalg.norm(b)
+
+
+def test_ttrand_rounding_op_exact(T):
+    """Sketch rank covering the rank of H y: the result represents 1.5 H y (the
+    explicit product, itself pinned against the dense Hamiltonian)."""
+    say("case: operator rounding exact recovery")
+    H = O.hubbard_1d_mpo(4, 1.0, 4.0)
+    y = rand_tt(H.dims, [1, 3, 5, 3, 1], 60)
+    rks = [1, 4, 16, 4, 1]
+    r = to_oracle(T.ttrand_rounding([1.5], to_api_op(H), [to_api(y)], rks, orthogonal=False, seed=9))
+    err = O.rel_err(O.tt_scale(1.5, O.mpo_apply(H, y)), r)
+    say(f"  rel err vs 1.5 H y: {err:.3e}")
+    assert err < 1e-12
+    assert r.ot == [1, 1, 1, 0]
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5), ("bf16", 5e-2)])
+def test_ttrand_rounding_op_vs_oracle(T, prec, tol):
+    """Truncating alpha_1 H y_1 + alpha_2 y_2 (H y_1 has rank 30) vs the oracle."""
+    say(f"case: operator rounding vs oracle {prec}")
+    H = O.hubbard_1d_mpo(6, 1.0, 4.0)
+    dims = H.dims
+    ys = [rand_tt(dims, 5, 90), rand_tt(dims, 4, 91)]
+    al = [1.0, 0.5]
+    rks = [1, 4, 12, 12, 12, 4, 1]
+    y = to_oracle(T.ttrand_rounding(al, to_api_op(H), [to_api(t) for t in ys], rks, orthogonal=False, seed=44,
+                                    sketch_prec=prec))
+    prov = device_blocks_provider(dims, rks, orthogonal=False, seed=44, sketch_prec=prec)
+    ref = O.ttrand_rounding_op(al, H, ys, rks, prov, False)
+    assert y.rks == ref.rks
+    err = O.rel_err(ref, y)
+    say(f"  rel err vs oracle: {err:.3e}")
+    assert err < tol
+
+
+def test_ttrand_rounding_residual(T):
+    """ttrand_rounding(A, y, b, rks) = rounding of A y - b, vs the oracle (truncating)
+    and vs the explicit A y - b (sketch rank covering the rank)."""
+    say("case: residual rounding")
+    H = O.hubbard_1d_mpo(5, 1.0, 4.0)
+    dims = H.dims
+    y, b = rand_tt(dims, 4, 92), rand_tt(dims, 3, 93)
+    rks = [1, 4, 10, 10, 4, 1]
+    r = to_oracle(T.ttrand_rounding(to_api_op(H), to_api(y), to_api(b), rks, orthogonal=True, seed=45))
+    prov = device_blocks_provider(dims, rks, orthogonal=True, seed=45)
+    ref = O.ttrand_rounding_residual(H, y, b, rks, prov, True)
+    assert r.rks == ref.rks
+    assert O.rel_err(ref, r) < 1e-12
+    full = [1, 4, 16, 16, 4, 1]
+    r = to_oracle(T.ttrand_rounding(to_api_op(H), to_api(y), to_api(b), full, orthogonal=False, seed=45))
+    assert O.rel_err(O.tt_sub(O.mpo_apply(H, y), b), r) < 1e-12
+
+
+def test_round_rand_batch_mixed_operator_items(T):
+    """One launch over items with and without an operator term, uneven ranks."""
+    say("case: round_rand_batch with operator items")
+    H = O.hubbard_1d_mpo(5, 1.0, 4.0)
+    dims = H.dims
+    dop = T.DeviceTTO(to_api_op(H))
+    specs = [(True, (3, 2)), (False, (5, 4)), (True, (6,))]
+    items, refs, devs = [], [], []
+    for i, (op, rs) in enumerate(specs):
+        ys = [rand_tt(dims, r, 200 + 10 * i + j) for j, r in enumerate(rs)]
+        al = [1.0, -0.25][: len(ys)]
+        d = [T.DeviceTT.upload(to_api(t)) for t in ys]
+        devs += d
+        rks = [1, 4, 11, 11, 4, 1]
+        items.append((al, dop, d, rks) if op else (al, d, rks))
+        prov = device_blocks_provider(dims, rks, orthogonal=False, seed=46, sketch_prec="f32")
+        refs.append(O.ttrand_rounding_op(al, H, ys, rks, prov, False) if op
+                    else O.ttrand_rounding_lincomb(al, ys, rks, prov, False))
+    outs = T.round_rand_batch(items, orthogonal=False, seed=46, sketch_prec="f32")
+    for o, ref in zip(outs, refs):
+        y = dev_to_oracle(o)
+        assert y.rks == ref.rks
+        assert O.rel_err(ref, y) < 1e-5
+    for d in devs + outs:
+        d.free()
+    dop.free()
+
+
 def test_round_rand_batch_uneven(T):
     say("case: round_rand_batch uneven items")
     dims = (4,) * 6
