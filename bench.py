@@ -1,18 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark: batched mixed-precision randomized TT rounding on one MI355X.
 
-Workload (default): 256 TT vectors, each a linear combination of 8 seeded
-N(0,1) TTs with d=40, n=4 and ranks drawn uniformly from 16..256, coefficients
-0.5^j.  One step rounds the whole batch: Gaussian TT sketch (FP32 contractions,
-block rank 40 = d, sketch ranks from the rank heuristic), FP64 Householder QR
-and projection sweep, then the FP64 SVD cleanup to tol 1e-6.  Inputs are
-generated on the device and resident in HBM before timing starts.
+Default workload (`--workload c3`): 256 TT vectors, each a linear combination
+of 8 seeded N(0,1) TTs with d=40, n=4 and ranks drawn uniformly from 16..256,
+coefficients 0.5^j.  One step rounds the whole batch: Gaussian TT sketch (FP32
+contractions, block rank 40 = d, sketch ranks from the rank heuristic), FP64
+Householder QR and projection sweep, then the FP64 SVD cleanup to tol 1e-6.
+Inputs are generated on the device and resident in HBM before timing starts.
 
-Prints one JSON line.  Besides the metric it reports the FP64 deterministic
-SVD rounding of the same TTs (on a subset, because 256 materialised sums do not
-fit in HBM), the relative error of a sample item against the FP64 oracle, the
-dominant kernel's achieved rate against its roofline, and the oracle's CPU time
-on a bounded sample.
+Other workloads (one JSON line each, same fields):
+  c1  6 N(0,1) TTs (d=32, n=4, r=100), weights 2^-k; FP64 Gaussian TT sketch of
+      rank 144 (block rank 144), cleanup to rmax 128; one TT.
+  c2  quantics: 8 N(0,1) TTs (d=64, n=2, r=128), weights 10^(-k/2); BF16 sketch
+      of rank 272 (block rank 272), cleanup to rmax 256; one TT.
+  c4  1D Hubbard MPO (L=32, local dim 4, t=1, U=4, bond dim 6) applied to 64
+      N(0,1) TTs of rank 128 without forming the products; BF16 sketch of rank
+      144 (block rank 144), cleanup to rmax 128.
+
+Prints one JSON line.  Besides the metric it reports the FP64 deterministic SVD
+rounding of the same inputs (materialised sums / products, on a subset where
+they do not fit in HBM), the relative error of a sample item against the FP64
+oracle run on the same sketch blocks, the dominant kernel's achieved rate
+against its roofline, and the oracle's CPU time on a bounded sample.
 """
 import argparse
 import json
@@ -28,72 +37,204 @@ sys.path.insert(0, os.path.join(ROOT, "tensor-train-julia_amd"))
 import numpy as np  # noqa: E402
 
 PEAKS = {"gemm_f64": ("mfma", 78.6e12), "gemm_f32": ("mfma", 157.3e12), "gemm_bf16": ("mfma", 2516e12),
-         "qr_panel": ("hbm", 8.0e12), "jacobi_svd": ("mfma", 78.6e12)}
+         "qr_panel": ("hbm", 8.0e12), "jacobi_svd": ("mfma", 78.6e12), "op_apply": ("hbm", 8.0e12)}
 # device kernel behind each profiled class (for matching rocprofv3 summaries)
 KERNEL_SYMBOL = {"gemm_f64": "gemm_scalar_kernel<double, double, double, double>",
                  "gemm_f32": "gemm_scalar_kernel<double, float, float, float>",
                  "gemm_bf16": "gemm_bf16_kernel", "qr_panel": "qr_panel_reg_kernel",
-                 "jacobi_svd": "jacobi_round_kernel"}
+                 "jacobi_svd": "jacobi_round_kernel", "op_apply": "op_apply_kernel"}
+
+WORKLOADS = {
+    "c3": dict(items=256, d=40, n=4, terms=8, rmin=16, rmax=256, weight="0.5^j", seed_base=3000, sketch="f32",
+               block_rks=None, rks=None, cleanup=dict(tol=1e-6), op=False, det_items=48,
+               text="batched randomized rounding: 256 TTs, each sum of 8 N(0,1) TTs d=40 n=4 r~U{16..256} "
+                    "coeff 0.5^j; FP32 Gaussian sketch, block_rks=40, heuristic ranks; FP64 QR/projection; "
+                    "SVD cleanup tol=1e-6"),
+    "c1": dict(items=1, d=32, n=4, terms=6, rmin=100, rmax=100, weight="2^-k", seed_base=1000, sketch="f64",
+               block_rks=144, rks=144, cleanup=dict(rmax=128), op=False, det_items=1,
+               text="randomized rounding of sum_k 2^-k y_k, 6 N(0,1) TTs d=32 n=4 r=100 (formal rank 600); "
+                    "FP64 Gaussian TT sketch rank 144 (block_rks=144); SVD cleanup rmax=128"),
+    "c2": dict(items=1, d=64, n=2, terms=8, rmin=128, rmax=128, weight="10^(-k/2)", seed_base=2000, sketch="bf16",
+               block_rks=272, rks=272, cleanup=dict(rmax=256), op=False, det_items=1,
+               text="quantics TT: sum_k 10^(-k/2) y_k, 8 N(0,1) TTs d=64 n=2 r=128 (formal rank 1024); "
+                    "sketch rank 272 (block_rks=272); SVD cleanup rmax=256"),
+    "c4": dict(items=64, d=32, n=4, terms=1, rmin=128, rmax=128, weight="1", seed_base=4000, sketch="bf16",
+               block_rks=144, rks=144, cleanup=dict(rmax=128), op=True, det_items=16,
+               text="1D Hubbard MPO (L=32, t=1, U=4, bond dim 6) times 64 N(0,1) TTs d=32 n=4 r=128, "
+                    "implicit product; sketch rank 144 (block_rks=144); SVD cleanup rmax=128"),
+}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def weight(wl, j):
+    return {"0.5^j": 0.5 ** j, "2^-k": 2.0 ** -j, "10^(-k/2)": 10.0 ** (-j / 2), "1": 1.0}[wl["weight"]]
+
+
 def measured_traffic(args, kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by what
     the counter reads for a plain copy of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "c3_batched_rounding_hbm_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{args.workload}_batched_rounding_hbm_traffic.json")
     try:
         with open(path) as f:
             rec = json.load(f)
     except (OSError, ValueError):
         return None
     cfg = rec.get("config", {})
-    if any(cfg.get(k) != getattr(args, k) for k in ("items", "d", "n", "terms", "sketch_prec")):
+    if any(cfg.get(k) != getattr(args, k) for k in ("items", "sketch_prec")):
         return None
     k = rec.get("kernels", {}).get(kernel)
     return None if k is None else k.get("hbm_bytes_per_launch")
 
 
-def term_ranks(dims, r):
-    import ttround as T
-    return T.r_and_d_to_rks([r] * (len(dims) + 1), dims, rmax=r)
-
-
-def build_workload(args):
+def build_workload(wl, args):
+    """Device-resident inputs: per item (alphas, [DeviceTT], target ranks)."""
     import ttround as T
     rng = np.random.default_rng(3)
-    dims = (args.n,) * args.d
+    d, n = wl["d"], wl["n"]
+    dims = (n,) * d
+    op = T.DeviceTTO(T.hubbard_1d_mpo(d, 1.0, 4.0)) if wl["op"] else None
     items = []
     for b in range(args.items):
         terms, alphas, rk = [], [], []
-        for j in range(1, args.terms + 1):
-            r = int(rng.integers(args.rmin, args.rmax + 1))
-            t = T.DeviceTT.randn(dims, r, seed=1000 * 3 + j + 100000 * b)
+        for j in range(1, wl["terms"] + 1):
+            r = int(rng.integers(wl["rmin"], wl["rmax"] + 1))
+            t = T.DeviceTT.randn(dims, r, seed=wl["seed_base"] + j + 100000 * b)
             terms.append(t)
-            alphas.append(0.5 ** j)
+            alphas.append(weight(wl, j))
             rk.append(t.rks)
-        rks = list(rk[0])
-        for q in rk[1:]:
-            rks = [max(a, c) for a, c in zip(rks, q)]
-        ovs = int(math.ceil(0.5 * max(rks)))
-        for k in range(1, args.d):
-            rks[k] += ovs
+        if wl["rks"] is None:
+            rks = list(rk[0])
+            for q in rk[1:]:
+                rks = [max(a, c) for a, c in zip(rks, q)]
+            ovs = int(math.ceil(0.5 * max(rks)))
+            for k in range(1, d):
+                rks[k] += ovs
+        else:
+            rks = [1] + [wl["rks"]] * (d - 1) + [1]
         items.append((alphas, terms, rks))
-    return dims, items
+    return dims, op, items
 
 
-def run_step(items, args):
+def run_step(items, op, wl, args, keep_raw=False):
+    """One pass over the batch; returns (rounded TTs, randomized-stage TTs or None)."""
     import ttround as T
-    outs = T.round_rand_batch(items, orthogonal=False, seed=1234, block_rks=None, sketch_prec=args.sketch_prec)
-    if args.cleanup_tol > 0:
-        clean = T.round_svd_batch(outs, tol=args.cleanup_tol)
-        for o in outs:
+    batch = [(al, op, terms, rks) if op is not None else (al, terms, rks) for al, terms, rks in items]
+    raw = T.round_rand_batch(batch, orthogonal=False, seed=1234, block_rks=wl["block_rks"],
+                             sketch_prec=args.sketch_prec)
+    if args.no_cleanup:
+        return raw, None
+    clean = T.round_svd_batch(raw, **wl["cleanup"])
+    if not keep_raw:
+        for o in raw:
             o.free()
-        outs = clean
-    return outs
+        raw = None
+    return clean, raw
+
+
+def parity_and_cpu(items, op, outs, raw, dims, wl, args):
+    """Item i of the batch through the FP64 oracle on the device's own sketch
+    blocks (and the same cleanup), timed on the host cores.  Returns the largest
+    relative error of the final result and of the randomized stage alone."""
+    import oracle as O
+    import ttround as T
+
+    def host(dx):
+        return O.TT([dx.core(k) for k in range(dx.d)], dx.dims, dx.rks, dx.ot)
+    errs, errs_raw, tcpu = [], [], 0.0
+    H = O.hubbard_1d_mpo(wl["d"], 1.0, 4.0) if op is not None else None
+    for i in range(args.cpu_items):
+        al, terms, rks = items[i]
+        ys = [O.TT([t.core(k) for k in range(t.d)], t.dims, t.rks, t.ot) for t in terms]
+        blocks = {}
+        for k in range(wl["d"]):
+            blocks[k] = T.sketch_blocks(dims, rks, k, orthogonal=False, seed=1234, block_rks=wl["block_rks"],
+                                        sketch_prec=args.sketch_prec)
+        tc = time.perf_counter()
+        if H is not None:
+            ref = O.ttrand_rounding_op(al, H, ys, rks, blocks.__getitem__, False, wl["block_rks"])
+        else:
+            ref = O.ttrand_rounding_lincomb(al, ys, rks, blocks.__getitem__, False, wl["block_rks"])
+        if raw is not None:
+            errs_raw.append(O.rel_err(ref, host(raw[i])))
+        if not args.no_cleanup:
+            ref = O.tt_rounding(ref, **wl["cleanup"])
+        tcpu += time.perf_counter() - tc
+        errs.append(O.rel_err(ref, host(outs[i])))
+        log(f"[bench] item {i}: rel err vs FP64 oracle {errs[-1]:.3e}"
+            f"{f' (randomized stage {errs_raw[-1]:.3e})' if errs_raw else ''} (oracle {tcpu:.1f}s)")
+    cpu = {"value": round(args.cpu_items / tcpu, 4), "unit": "rounded TTs/s", "cores": os.cpu_count(),
+           "kind": "port",
+           "sample": f"{args.cpu_items} item(s) of the same batch, FP64 numpy/OpenBLAS oracle "
+                     f"(randomized rounding{'' if args.no_cleanup else ' + SVD cleanup'}), "
+                     f"BLAS threads = {os.environ.get('OPENBLAS_NUM_THREADS', 'default')}"}
+    return max(errs), (max(errs_raw) if errs_raw else None), cpu
+
+
+def deterministic(items, op, wl, args):
+    """Plain FP64 SVD rounding (tt_rounding with the cleanup's tol/rmax) of the
+    materialised sums / products of the first det_items items."""
+    import ttround as T
+    nd = min(args.det_items, len(items))
+    sub = items[:nd]
+    if op is not None:
+        mats = [T.mpo_apply_device(op, terms[0]) for _, terms, _ in sub]
+    else:
+        mats = [T.lincomb_device(al, terms) for al, terms, _ in sub]
+    # the materialised inputs need the room of the terms
+    for _, terms, _ in items:
+        for t in terms:
+            t.free()
+    r = T.round_svd_batch(mats, **wl["cleanup"])
+    for o in r:
+        o.free()
+    tdet = time.perf_counter()
+    r = T.round_svd_batch(mats, **wl["cleanup"])
+    tdet = time.perf_counter() - tdet
+    det = {"items": nd, "rounded_tts_per_s": round(nd / tdet, 4), "ms": round(tdet * 1e3, 2),
+           "in_rank_max": max(max(m.rks) for m in mats), "out_rank_max": max(max(o.rks) for o in r)}
+    log(f"[bench] FP64 deterministic on {nd} items: {tdet:.2f}s")
+    for o in r + mats:
+        o.free()
+    return det
+
+
+def summarize(stats, args):
+    sweeps = stats.pop("jacobi_svd:sweeps", None)
+    jacobi = ({"svds": sweeps["launches"], "mean_sweeps": round(sweeps["flops"] / max(sweeps["launches"], 1), 2),
+               "max_sweeps": sweeps["bytes"]} if sweeps else None)
+    for ph in ("visits", "ticks_gram", "ticks_inner", "ticks_apply"):
+        st = stats.pop("jacobi_svd:" + ph, None)
+        if st and jacobi:
+            jacobi[ph] = st["flops"]
+    sites = {k[5:]: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] // args.steps,
+                     "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3)}
+             for k, v in stats.items() if k.startswith("site:")}
+    kern = {k: v for k, v in stats.items() if not k.startswith("phase:") and not k.startswith("site:")}
+    phases = {k[6:]: round(v["ms"] / args.steps, 3) for k, v in stats.items() if k.startswith("phase:")}
+    dom = max(kern, key=lambda k: kern[k]["ms"]) if kern else None
+    roof = None
+    if dom:
+        st = kern[dom]
+        bound, peak = PEAKS.get(dom, ("hbm", 8.0e12))
+        avg_s = st["ms"] / st["launches"] / 1e3
+        if bound == "mfma":
+            ach, unit, pk = st["flops"] / st["launches"] / avg_s / 1e12, "TFLOP/s", peak / 1e12
+        else:
+            ach, unit, pk = st["bytes"] / st["launches"] / avg_s / 1e9, "GB/s", peak / 1e9
+        roof = {"kernel": dom, "device_kernel": KERNEL_SYMBOL.get(dom), "bound": bound, "achieved": round(ach, 3),
+                "peak": pk, "unit": unit, "frac": round(ach / pk, 5), "traffic": measured_traffic(args, dom),
+                "algorithmic_bytes_per_launch": round(st["bytes"] / st["launches"]),
+                "avg_launch_us": round(avg_s * 1e6, 2),
+                "launches_per_step": st["launches"] / args.steps,
+                "share_of_kernel_time": round(st["ms"] / sum(v["ms"] for v in kern.values()), 4)}
+    kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] // args.steps,
+                   "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3),
+                   "gbps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in kern.items()}
+    return jacobi, sites, phases, roof, kernels
 
 
 def main():
@@ -101,16 +242,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--items", type=int, default=256)
-    ap.add_argument("--d", type=int, default=40)
-    ap.add_argument("--n", type=int, default=4)
-    ap.add_argument("--terms", type=int, default=8)
-    ap.add_argument("--rmin", type=int, default=16)
-    ap.add_argument("--rmax", type=int, default=256)
-    ap.add_argument("--sketch-prec", default="f32")
-    ap.add_argument("--cleanup-tol", type=float, default=1e-6)
-    ap.add_argument("--det-items", type=int, default=48,
-                    help="items for the FP64 deterministic comparison (0: skip); their materialised sums must fit in HBM")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c3")
+    ap.add_argument("--items", type=int, default=None, help="batch size (default: the workload's)")
+    ap.add_argument("--sketch-prec", default=None, help="f64 | f32 | bf16 (default: the workload's)")
+    ap.add_argument("--no-cleanup", action="store_true", help="skip the SVD cleanup after the randomized rounding")
+    ap.add_argument("--det-items", type=int, default=None,
+                    help="items for the FP64 deterministic comparison (0: skip; default: the workload's); their "
+                         "materialised inputs must fit in HBM")
     ap.add_argument("--cpu-items", type=int, default=1, help="items for the oracle CPU baseline / parity (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES of device memory 3 times (a plain copy kernel of known size, "
@@ -122,6 +260,10 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    wl = WORKLOADS[args.workload]
+    args.items = args.items or wl["items"]
+    args.sketch_prec = args.sketch_prec or wl["sketch"]
+    args.det_items = wl["det_items"] if args.det_items is None else args.det_items
 
     if args.copy_calibration > 0:
         import torch
@@ -136,13 +278,13 @@ def main():
 
     import ttround as T
     h = T.default_handle()
-    log(f"[bench] building workload: {args.items} items x {args.terms} terms, d={args.d}, n={args.n}")
+    log(f"[bench] workload {args.workload}: {args.items} items, d={wl['d']}, n={wl['n']}")
     t0 = time.time()
-    dims, items = build_workload(args)
+    dims, op, items = build_workload(wl, args)
     log(f"[bench] workload ready in {time.time() - t0:.1f}s")
 
     for w in range(args.warmup):
-        outs = run_step(items, args)
+        outs, _ = run_step(items, op, wl, args)
         log(f"[bench] warmup {w} done; out ranks (item 0, max) = {max(outs[0].rks)}")
         for o in outs:
             o.free()
@@ -151,107 +293,25 @@ def main():
     h.profile_reset()
     t0 = time.perf_counter()
     for s in range(args.steps):
-        outs = run_step(items, args)
-        if s < args.steps - 1:
+        last = s == args.steps - 1
+        outs, raw = run_step(items, op, wl, args, keep_raw=last and args.cpu_items > 0)
+        if not last:
             for o in outs:
                 o.free()
         log(f"[bench] step {s} done")
     dt = time.perf_counter() - t0
     h.profile(False)
-    stats = h.profile_stats()
+    jacobi, sites, phases, roof, kernels = summarize(h.profile_stats(), args)
     ms_per_step = 1e3 * dt / args.steps
     value = args.items * args.steps / dt
+    out_rks = max(outs[0].rks)
 
-    sweeps = stats.pop("jacobi_svd:sweeps", None)
-    jacobi = ({"svds": sweeps["launches"], "mean_sweeps": round(sweeps["flops"] / max(sweeps["launches"], 1), 2),
-               "max_sweeps": sweeps["bytes"]} if sweeps else None)
-    if jacobi:
-        for ph in ("visits", "ticks_gram", "ticks_inner", "ticks_apply"):
-            st = stats.pop("jacobi_svd:" + ph, None)
-            if st:
-                jacobi[ph] = st["flops"]
-    sites = {k[5:]: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] // args.steps,
-                     "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3)}
-             for k, v in stats.items() if k.startswith("site:")}
-    kern = {k: v for k, v in stats.items() if not k.startswith("phase:") and not k.startswith("site:")}
-    phases = {k[6:]: round(v["ms"] / args.steps, 3) for k, v in stats.items() if k.startswith("phase:")}
-    dom = max(kern, key=lambda k: kern[k]["ms"]) if kern else None
-    roof = None
-    if dom:
-        st = kern[dom]
-        bound, peak = PEAKS.get(dom, ("hbm", 8.0e12))
-        avg_s = st["ms"] / st["launches"] / 1e3
-        if bound == "mfma":
-            per_launch = st["flops"] / st["launches"]
-            ach, unit, pk = per_launch / avg_s / 1e12, "TFLOP/s", peak / 1e12
-        else:
-            per_launch = st["bytes"] / st["launches"]
-            ach, unit, pk = per_launch / avg_s / 1e9, "GB/s", peak / 1e9
-        roof = {"kernel": dom, "device_kernel": KERNEL_SYMBOL.get(dom), "bound": bound, "achieved": round(ach, 3),
-                "peak": pk, "unit": unit, "frac": round(ach / pk, 5), "traffic": measured_traffic(args, dom),
-                "algorithmic_bytes_per_launch": round(st["bytes"] / st["launches"]),
-                "avg_launch_us": round(avg_s * 1e6, 2),
-                "launches_per_step": st["launches"] / args.steps,
-                "share_of_kernel_time": round(st["ms"] / sum(v["ms"] for v in kern.values()), 4)}
-
-    kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] // args.steps,
-                   "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3)} for k, v in kern.items()}
-
-    # ------------------------------------------------------------ parity sample + CPU baseline
-    rel_err = None
-    cpu = None
+    rel_err, rel_err_raw, cpu = (None, None, None)
     if args.cpu_items > 0:
-        import oracle as O
-        errs, tcpu = [], 0.0
-        for i in range(args.cpu_items):
-            al, terms, rks = items[i]
-            ys = [O.TT([t.core(k) for k in range(t.d)], t.dims, t.rks, t.ot) for t in terms]
-            blocks = {}
-
-            def prov(k, rks=rks):
-                if k not in blocks:
-                    blocks[k] = T.sketch_blocks(dims, rks, k, orthogonal=False, seed=1234, sketch_prec=args.sketch_prec)
-                return blocks[k]
-            for k in range(args.d):
-                prov(k)
-            tc = time.perf_counter()
-            ref = O.ttrand_rounding_lincomb(al, ys, rks, prov, False)
-            if args.cleanup_tol > 0:
-                ref = O.tt_rounding(ref, tol=args.cleanup_tol)
-            tcpu += time.perf_counter() - tc
-            got = outs[i]
-            g = O.TT([got.core(k) for k in range(got.d)], got.dims, got.rks, got.ot)
-            errs.append(O.rel_err(ref, g))
-            log(f"[bench] item {i}: rel err vs FP64 oracle {errs[-1]:.3e} (oracle {tcpu:.1f}s)")
-        rel_err = max(errs)
-        cpu = {"value": round(args.cpu_items / tcpu, 4), "unit": "rounded TTs/s", "cores": os.cpu_count(),
-               "kind": "port",
-               "sample": f"{args.cpu_items} item(s) of the same batch, FP64 numpy/OpenBLAS oracle "
-                         f"(randomized rounding + SVD cleanup), BLAS threads = {os.environ.get('OPENBLAS_NUM_THREADS', 'default')}"}
-    for o in outs:
+        rel_err, rel_err_raw, cpu = parity_and_cpu(items, op, outs, raw, dims, wl, args)
+    for o in outs + (raw or []):
         o.free()
-
-    # ------------------------------------------------------------ FP64 deterministic comparison
-    det = None
-    if args.det_items > 0:
-        sub = items[: args.det_items]
-        sums = [T.lincomb_device(al, terms) for al, terms, _ in sub]
-        # the materialised sums (1.28 GB each at these ranks) need the room of the terms
-        for _, terms, _ in items:
-            for t in terms:
-                t.free()
-        items = None
-        r = T.round_svd_batch(sums, tol=args.cleanup_tol if args.cleanup_tol > 0 else 1e-12)
-        for o in r:
-            o.free()
-        tdet = time.perf_counter()
-        r = T.round_svd_batch(sums, tol=args.cleanup_tol if args.cleanup_tol > 0 else 1e-12)
-        tdet = time.perf_counter() - tdet
-        det = {"items": args.det_items, "rounded_tts_per_s": round(args.det_items / tdet, 4),
-               "ms": round(tdet * 1e3, 2), "out_rank_max": max(max(o.rks) for o in r)}
-        log(f"[bench] FP64 deterministic on {args.det_items} items: {tdet:.2f}s")
-        for o in r + sums:
-            o.free()
+    det = deterministic(items, op, wl, args) if args.det_items > 0 else None
 
     res = {
         "metric": "rounded TTs/sec, mixed-prec batched vs FP64 plain, at stated rel. error; HBM GB/s",
@@ -266,13 +326,12 @@ def main():
         "vs_baseline": None,
         "dtype": f"{args.sketch_prec}-sketch+f64",
         "data": "synthetic (seeded N(0,1) TT cores drawn on the device)",
-        "config": {"workload": f"batched randomized rounding: {args.items} TTs, each sum of {args.terms} N(0,1) TTs "
-                               f"d={args.d} n={args.n} r~U{{{args.rmin}..{args.rmax}}} coeff 0.5^j; "
-                               f"{args.sketch_prec} Gaussian sketch, block_rks={args.d}, heuristic ranks; "
-                               f"FP64 QR/projection; SVD cleanup tol={args.cleanup_tol}",
-                   "items": args.items, "d": args.d, "n": args.n, "terms": args.terms,
-                   "sketch_prec": args.sketch_prec, "cleanup_tol": args.cleanup_tol},
+        "config": {"workload": wl["text"], "name": args.workload, "items": args.items, "d": wl["d"], "n": wl["n"],
+                   "terms": wl["terms"], "sketch_prec": args.sketch_prec,
+                   "cleanup": None if args.no_cleanup else wl["cleanup"]},
         "rel_err_vs_oracle": rel_err,
+        "rel_err_vs_oracle_randomized_stage": rel_err_raw,
+        "out_rank_max_item0": out_rks,
         "fp64_deterministic": det,
         "speedup_vs_fp64_deterministic": (round(value / det["rounded_tts_per_s"], 3) if det else None),
         "phases_ms_per_step": phases,

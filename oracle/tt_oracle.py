@@ -219,7 +219,7 @@ def mpo_apply(A: TTO, v: TT) -> TT:
     rks = [a * b for a, b in zip(A.rks, v.rks)]
     cores = []
     for k in range(d):
-        t = np.einsum("ijab,jcd->iacbd", A.cores[k], v.cores[k])
+        t = np.einsum("ijab,jcd->iacbd", A.cores[k], v.cores[k], optimize=True)
         n = v.dims[k]
         cores.append(jreshape(t, (n, rks[k], rks[k + 1])))
     return TT(cores, v.dims, rks, [0] * d)
@@ -287,7 +287,7 @@ def orthogonalize(x: TT, i: Optional[int] = None) -> TT:
         j0 = j - 1
         ot[j0] = 1
         n = x.dims[j0]
-        t = np.einsum("ab,ibc->aic", FR, x.cores[j0])  # yleft[α_{j-1}, i_j, α_j]
+        t = np.einsum("ab,ibc->aic", FR, x.cores[j0], optimize=True)  # yleft[α_{j-1}, i_j, α_j]
         M = jreshape(t, (n * y_rks[j0], x.rks[j0 + 1]))
         Q, R = np.linalg.qr(M)
         y_rks[j0 + 1] = Q.shape[1]
@@ -298,7 +298,7 @@ def orthogonalize(x: TT, i: Optional[int] = None) -> TT:
         j0 = j - 1
         ot[j0] = -1
         n = x.dims[j0]
-        t = np.einsum("iab,bc->aci", x.cores[j0], FL)  # yright[α_{j-1}, β_j, i_j]
+        t = np.einsum("iab,bc->aci", x.cores[j0], FL, optimize=True)  # yright[α_{j-1}, β_j, i_j]
         M = jreshape(t, (x.rks[j0], y_rks[j0 + 1] * n))
         L, Q = _lq(M)
         y_rks[j0] = Q.shape[0]
@@ -528,9 +528,40 @@ def contract_sketch_core_backwards(A: np.ndarray, S: np.ndarray, V: np.ndarray) 
 
 
 def contract_sketch_core_backwards_op(A: np.ndarray, H: np.ndarray, S: np.ndarray, V: np.ndarray) -> np.ndarray:
-    """``W[a,b,c] = sum A[z,a,al] H[ze,z,b,be] S[ga,ze,c] V[al,be,ga]``
-    (``src/sketches.jl:1648-1713``; contraction order left to numpy)."""
-    return np.einsum("zax,wzby,gwc,xyg->abc", A, H, S, V, optimize=True)
+    """``W[a,b,c] = sum A[z,a,al] H[ze,z,b,be] S[ga,ze,c] V[al,be,ga]`` with the
+    reference's cost-chosen order, ((V A) H) S or ((V S) H) A
+    (``src/sketches.jl:1648-1713``)."""
+    z, a, al = A.shape
+    ze, _, b, be = H.shape
+    ga, _, c = S.shape
+    if a * ga * (z * be * (al + ze * b) + ze * b * c) < al * c * (ze * be * (ga + z * b) + z * a * b):
+        VA = jreshape(jreshape(A, (z * a, al)) @ jreshape(V, (al, be * ga)), (z, a, be, ga))
+        VAp = jreshape(np.transpose(VA, (1, 3, 0, 2)), (a * ga, z * be))
+        Bp = jreshape(np.transpose(H, (1, 3, 0, 2)), (z * be, ze * b))
+        VAB = jreshape(VAp @ Bp, (a, ga, ze, b))
+        VABp = jreshape(np.transpose(VAB, (0, 3, 1, 2)), (a * b, ga * ze))
+        return jreshape(VABp @ jreshape(S, (ga * ze, c)), (a, b, c))
+    VS = jreshape(jreshape(V, (al * be, ga)) @ jreshape(S, (ga, ze * c)), (al, be, ze, c))
+    VSp = jreshape(np.transpose(VS, (2, 1, 0, 3)), (ze * be, al * c))
+    Bp = jreshape(np.transpose(H, (1, 2, 0, 3)), (z * b, ze * be))
+    VSB = jreshape(Bp @ VSp, (z, b, al, c))
+    Ap = jreshape(np.transpose(A, (1, 0, 2)), (a, z * al))
+    VSBp = jreshape(np.transpose(VSB, (0, 2, 1, 3)), (z * al, b * c))
+    return jreshape(Ap @ VSBp, (a, b, c))
+
+
+def _op_project(P: np.ndarray, y: np.ndarray, H: np.ndarray) -> np.ndarray:
+    """``t[i,s,c,d] = sum_{a,b,j} P[s,a,b] y[j,a,c] H[i,j,b,d]`` as two products
+    (the left-part update of ``src/tt_randtools.jl:337-339``)."""
+    s_, a, b = P.shape
+    j, _, c = y.shape
+    i, _, _, d = H.shape
+    U = jreshape(np.transpose(P, (0, 2, 1)), (s_ * b, a)) @ jreshape(np.transpose(y, (1, 0, 2)), (a, j * c))
+    U = jreshape(U, (s_, b, j, c))
+    Up = jreshape(np.transpose(U, (0, 3, 1, 2)), (s_ * c, b * j))
+    Hp = jreshape(np.transpose(H, (2, 1, 0, 3)), (b * j, i * d))
+    t = jreshape(Up @ Hp, (s_, c, i, d))
+    return np.transpose(t, (2, 0, 1, 3))
 
 
 def tt_recursive_sketch(A: TT, rks, draws, orthogonal: bool = True, block_rks: Optional[int] = None,
@@ -626,8 +657,8 @@ def ttrand_rounding_lincomb(alpha, ys: Sequence[TT], rks, draws, orthogonal: boo
         vec[k] = jreshape(Q, (dims[k], out_rks[k], out_rks[k + 1]))
         ot[k] = 1
         for j in range(m):
-            P = np.einsum("ira,irs->sa", Y[j], vec[k])
-            Y[j] = np.einsum("sa,iab->isb", P, ys[j].cores[k + 1])
+            P = np.einsum("ira,irs->sa", Y[j], vec[k], optimize=True)
+            Y[j] = np.einsum("sa,iab->isb", P, ys[j].cores[k + 1], optimize=True)
     out_rks[N] = 1
     last = Y[0].copy()
     for j in range(1, m):
@@ -652,7 +683,7 @@ def ttrand_rounding_op(alpha, H: TTO, ys: Sequence[TT], rks, draws, orthogonal: 
     vec: List[Optional[np.ndarray]] = [None] * N
     out_rks = [1] * (N + 1)
     ot = [0] * N
-    Ay1 = np.einsum("jab,ijcd->iacbd", ys[0].cores[0], H.cores[0])  # (n,1,1,Ry2,RA2)
+    Ay1 = np.einsum("jab,ijcd->iacbd", ys[0].cores[0], H.cores[0], optimize=True)  # (n,1,1,Ry2,RA2)
     Ay1 = alpha[0] * Ay1
     Y = [jreshape(Ay1, (dims[0], 1, ys[0].rks[1] * H.rks[1]))]
     for j in range(1, m):
@@ -667,12 +698,12 @@ def ttrand_rounding_op(alpha, H: TTO, ys: Sequence[TT], rks, draws, orthogonal: 
         vec[k] = jreshape(Q, (dims[k], out_rks[k], out_rks[k + 1]))
         ot[k] = 1
         y1k = jreshape(Y[0], (dims[k], out_rks[k], ys[0].rks[k + 1], H.rks[k + 1]))
-        P = np.einsum("irab,irs->sab", y1k, vec[k])  # (rho', alpha, beta)
-        t = np.einsum("sab,jac,ijbd->iscd", P, ys[0].cores[k + 1], H.cores[k + 1], optimize=True)
+        P = np.einsum("irab,irs->sab", y1k, vec[k], optimize=True)  # (rho', alpha, beta)
+        t = _op_project(P, ys[0].cores[k + 1], H.cores[k + 1])
         Ynew = [jreshape(t, (dims[k + 1], out_rks[k + 1], ys[0].rks[k + 2] * H.rks[k + 2]))]
         for j in range(1, m):
-            Pj = np.einsum("ira,irs->sa", Y[j], vec[k])
-            Ynew.append(np.einsum("sa,iab->isb", Pj, ys[j].cores[k + 1]))
+            Pj = np.einsum("ira,irs->sa", Y[j], vec[k], optimize=True)
+            Ynew.append(np.einsum("sa,iab->isb", Pj, ys[j].cores[k + 1], optimize=True))
         Y = Ynew
     out_rks[N] = 1
     last = Y[0].copy()
@@ -693,19 +724,19 @@ def ttrand_rounding_residual(H: TTO, y: TT, b: TT, rks, draws, orthogonal: bool 
     vec: List[Optional[np.ndarray]] = [None] * N
     out_rks = [1] * (N + 1)
     ot = [0] * N
-    Ay = np.einsum("jab,ijcd->iacbd", y.cores[0], H.cores[0])[:, :, 0, :, :]  # (n,1,Ry2,RA2)
+    Ay = np.einsum("jab,ijcd->iacbd", y.cores[0], H.cores[0], optimize=True)[:, :, 0, :, :]  # (n,1,Ry2,RA2)
     bk = jreshape(b.cores[0], (dims[0], 1, b.rks[1]))
     for k in range(N - 1):
-        Z = np.einsum("irab,abs->irs", Ay, WAy[k + 1]) - np.tensordot(bk, Wb[k + 1], axes=([2], [0]))
+        Z = np.einsum("irab,abs->irs", Ay, WAy[k + 1], optimize=True) - np.tensordot(bk, Wb[k + 1], axes=([2], [0]))
         Z = jreshape(Z, (dims[k] * out_rks[k], srk[k + 1]))
         Q = _qr_thin(Z)
         out_rks[k + 1] = Q.shape[1]
         vec[k] = jreshape(Q, (dims[k], out_rks[k], out_rks[k + 1]))
         ot[k] = 1
-        P = np.einsum("irab,irs->sab", Ay, vec[k])
-        Ay = np.einsum("sab,jac,ijbd->iscd", P, y.cores[k + 1], H.cores[k + 1], optimize=True)
-        Pb = np.einsum("ira,irs->sa", bk, vec[k])
-        bk = np.einsum("sa,iab->isb", Pb, b.cores[k + 1])
+        P = np.einsum("irab,irs->sab", Ay, vec[k], optimize=True)
+        Ay = _op_project(P, y.cores[k + 1], H.cores[k + 1])
+        Pb = np.einsum("ira,irs->sa", bk, vec[k], optimize=True)
+        bk = np.einsum("sa,iab->isb", Pb, b.cores[k + 1], optimize=True)
     out_rks[N] = 1
     vec[N - 1] = jreshape(Ay, (dims[N - 1], out_rks[N - 1], out_rks[N])) - jreshape(bk, (dims[N - 1], out_rks[N - 1], out_rks[N]))
     return TT(vec, dims, out_rks, ot)

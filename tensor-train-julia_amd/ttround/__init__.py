@@ -13,3 +13,4 @@ from .api import (DeviceTT, DeviceTTO, TTvector, TToperator, default_rank_heuris
                   tt_rounding_, tt_scale, ttrand_rounding, zeros_tt, is_leftorthogonal, is_rightorthogonal)
 
 __version__ = "0.1.0"
+from .models import hubbard_1d_mpo  # noqa: F401,E402

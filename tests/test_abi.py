@@ -54,3 +54,18 @@ def test_create_without_gpu_reports_error():
         pass
     with pytest.raises(Exception):
         T.Handle(0)
+
+
+def test_hubbard_builder_matches_oracle():
+    """The product's Hubbard MPO (bench input) equals the oracle's, which is
+    pinned against the dense fermionic Hamiltonian."""
+    import ttround as T
+    for L in (2, 3, 6):
+        A = T.hubbard_1d_mpo(L, 1.0, 4.0)
+        H = O.hubbard_1d_mpo(L, 1.0, 4.0)
+        assert A.tto_rks == H.rks and A.tto_dims == H.dims
+        for a, b in zip(A.tto_vec, H.cores):
+            np.testing.assert_array_equal(a, b)
+    # a single site is the on-site term alone
+    A1 = T.hubbard_1d_mpo(1, 1.0, 4.0)
+    np.testing.assert_array_equal(A1.tto_vec[0][:, :, 0, 0], np.diag([0.0, 0.0, 0.0, 4.0]))
