@@ -8,6 +8,8 @@
 #include <numeric>
 
 #include "../../include/ttround.h"
+#include <cstdlib>
+
 #include "ttr_internal.h"
 
 using namespace ttr;
@@ -1093,6 +1095,9 @@ int ttr_create(ttr_handle_t* out, int device) {
     TTR_HIP(hipMalloc(&h->one, 64));
     const double one = 1.0;
     TTR_HIP(hipMemcpy(h->one, &one, 8, hipMemcpyHostToDevice));
+    // TTR_GEMM_TUNE=0: GEMM tiles by size only (no timing of the candidates on first use)
+    const char* tune = std::getenv("TTR_GEMM_TUNE");
+    h->ctx.gemm_autotune = !(tune && tune[0] == '0');
   } catch (const Error& e) {
     int st = e.status;
     if (h->ctx.stream) (void)hipStreamDestroy(h->ctx.stream);

@@ -11,6 +11,8 @@
 //   F64  : v_mfma_f64_16x16x4_f64, FP64 accumulate
 //   F32  : v_mfma_f32_16x16x4_f32, FP32 accumulate
 //   BF16 : v_mfma_f32_16x16x32_bf16, FP32 accumulate
+#include <cmath>
+
 #include "ttr_internal.h"
 
 namespace ttr {
@@ -36,6 +38,10 @@ template <> __device__ __forceinline__ double load_d<double>(const void* p, int6
 template <> __device__ __forceinline__ double load_d<float>(const void* p, int64_t i) { return static_cast<const float*>(p)[i]; }
 template <> __device__ __forceinline__ double load_d<bf16s>(const void* p, int64_t i) { return bf16_to_f32(static_cast<const uint16_t*>(p)[i]); }
 
+__device__ __forceinline__ double to_double(double v) { return v; }
+__device__ __forceinline__ double to_double(float v) { return v; }
+__device__ __forceinline__ double to_double(bf16s v) { return bf16_to_f32(v.x); }
+
 template <class T> __device__ __forceinline__ void store_c(void* p, int64_t i, double v);
 template <> __device__ __forceinline__ void store_c<double>(void* p, int64_t i, double v) { static_cast<double*>(p)[i] = v; }
 template <> __device__ __forceinline__ void store_c<float>(void* p, int64_t i, double v) { static_cast<float*>(p)[i] = float(v); }
@@ -56,27 +62,35 @@ __device__ __forceinline__ double amode_value(int mode, int64_t gi, int64_t gl, 
   return v;
 }
 
-constexpr int BM = 64, BN = 64;
+constexpr int BM = 64, BN = 64;  // tile of the BF16 kernel
 
 // ------------------------------------------------------------- F64 / F32 math
-template <class TA, class TB, class TC, class MT>
+// Tile TM x TN (64 or 128 each), BK = 16, 256 threads as 2 x 2 waves; wave (wm, wn)
+// owns a (TM/2) x (TN/2) block made of (TM/32) x (TN/32) MFMA 16x16 outputs.  The
+// accumulation order of every output element is the K order whatever the tile,
+// so all tile shapes give bitwise identical results.
+template <class TA, class TB, class TC, class MT, int TM, int TN>
 __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __restrict__ probs, int nprobs,
                                                           const GemmSeg* __restrict__ segs) {
   constexpr int BK = 16;
-  __shared__ MT As[BK][BM + 1];
-  __shared__ MT Bs[BK][BN + 1];
+  constexpr int MI = TM / 32, NI = TN / 32;
+  constexpr int NA = TM * BK / 256, NB = TN * BK / 256;  // tile elements per thread
+  __shared__ MT As[BK][TM + 1];
+  __shared__ MT Bs[BK][TN + 1];
   using acc_t = typename std::conditional<std::is_same<MT, double>::value, d4_t, f4_t>::type;
 
   const int bid = blockIdx.x;
   const GemmProb P = probs[find_prob(probs, nprobs, bid)];
   const int t = bid - P.tile0;
-  const int row0 = (t / P.tiles_n) * BM, col0 = (t % P.tiles_n) * BN;
+  const int row0 = (t / P.tiles_n) * TM, col0 = (t % P.tiles_n) * TN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
 
-  acc_t acc[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+  acc_t acc[MI][NI];
+#pragma unroll
+  for (int a = 0; a < MI; ++a)
+#pragma unroll
+    for (int b = 0; b < NI; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
 
   int emax = -(1 << 30);
   for (int s = 0; s < P.nseg; ++s)
@@ -88,67 +102,76 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
     if (S.exp) scale = ldexp(scale, S.exp[0] - emax);
     const bool a_row_fast = (S.a_rs == 1);
     const bool b_row_fast = (S.b_rs == 1);
-    // the k-tile k0 + BK is loaded into registers while the MFMAs of tile k0 run
-    MT ra[4], rb[4];
+    // element q of a thread: A (i, l) with i fastest when A is column-major in
+    // memory, l fastest otherwise; likewise for B
+    auto a_idx = [&](int q, int& i, int& l) {
+      const int e = tid + 256 * q;
+      if (a_row_fast) { i = e % TM; l = e / TM; } else { l = e % BK; i = e / BK; }
+    };
+    auto b_idx = [&](int q, int& l, int& j) {
+      const int e = tid + 256 * q;
+      if (b_row_fast) { l = e % BK; j = e / BK; } else { j = e % TN; l = e / TN; }
+    };
+    // The k-tile k0 + BK is loaded into registers while the MFMAs of tile k0 run.
+    // The loads are raw (no arithmetic on the loaded values, so nothing waits
+    // for them before the MFMAs); the unit diagonal of the implicit-V modes and
+    // the segment scale are applied when the registers are stored to LDS.
+    TA ra[NA];
+    TB rb[NB];
+    const TA* Ap = static_cast<const TA*>(S.A);
+    const TB* Bp = static_cast<const TB*>(S.B);
     auto load_tiles = [&](int k0) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NA; ++q) {
         int i, l;
-        if (a_row_fast) { i = tid & 63; l = (tid >> 6) + 4 * q; }
-        else { i = (tid >> 4) + 16 * q; l = tid & 15; }
+        a_idx(q, i, l);
         const int64_t gi = row0 + i, gl = k0 + l;
-        double v = 0.0;
-        if (gi < P.M && gl < S.K) {
-          v = (S.a_mode == AM_PLAIN || (S.a_mode == AM_VT_UNIT ? gl > gi : gi > gl))
-                  ? load_d<TA>(S.A, gi * S.a_rs + gl * S.a_cs) : 0.0;
-          v = amode_value(S.a_mode, gi, gl, v) * scale;
-        }
-        ra[q] = MT(v);
+        const bool in = gi < P.M && gl < S.K &&
+                        (S.a_mode == AM_PLAIN || (S.a_mode == AM_VT_UNIT ? gl > gi : gi > gl));
+        ra[q] = in ? Ap[gi * S.a_rs + gl * S.a_cs] : TA{};
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NB; ++q) {
         int l, j;
-        if (b_row_fast) { l = tid & 15; j = (tid >> 4) + 16 * q; }
-        else { j = tid & 63; l = (tid >> 6) + 4 * q; }
+        b_idx(q, l, j);
         const int64_t gl = k0 + l, gj = col0 + j;
-        rb[q] = MT((gl < S.K && gj < P.N) ? load_d<TB>(S.B, gl * S.b_rs + gj * S.b_cs) : 0.0);
+        rb[q] = (gl < S.K && gj < P.N) ? Bp[gl * S.b_rs + gj * S.b_cs] : TB{};
       }
     };
     load_tiles(0);
     for (int k0 = 0; k0 < S.K; k0 += BK) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NA; ++q) {
         int i, l;
-        if (a_row_fast) { i = tid & 63; l = (tid >> 6) + 4 * q; }
-        else { i = (tid >> 4) + 16 * q; l = tid & 15; }
-        As[l][i] = ra[q];
+        a_idx(q, i, l);
+        double v = to_double(ra[q]);
+        if (S.a_mode != AM_PLAIN && row0 + i == k0 + l && row0 + i < P.M && k0 + l < S.K) v = 1.0;
+        As[l][i] = MT(v * scale);
       }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NB; ++q) {
         int l, j;
-        if (b_row_fast) { l = tid & 15; j = (tid >> 4) + 16 * q; }
-        else { j = tid & 63; l = (tid >> 6) + 4 * q; }
-        Bs[l][j] = rb[q];
+        b_idx(q, l, j);
+        Bs[l][j] = MT(to_double(rb[q]));
       }
       __syncthreads();
       if (k0 + BK < S.K) load_tiles(k0 + BK);
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
-        MT a0 = As[kk + (lane >> 4)][wm * 32 + (lane & 15)];
-        MT a1 = As[kk + (lane >> 4)][wm * 32 + 16 + (lane & 15)];
-        MT b0 = Bs[kk + (lane >> 4)][wn * 32 + (lane & 15)];
-        MT b1 = Bs[kk + (lane >> 4)][wn * 32 + 16 + (lane & 15)];
-        if constexpr (std::is_same<MT, double>::value) {
-          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
-          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
-          acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
-          acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
-        } else {
-          acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-          acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-          acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-          acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
-        }
+        MT av[MI], bv[NI];
+#pragma unroll
+        for (int a = 0; a < MI; ++a) av[a] = As[kk + (lane >> 4)][wm * (TM / 2) + 16 * a + (lane & 15)];
+#pragma unroll
+        for (int b = 0; b < NI; ++b) bv[b] = Bs[kk + (lane >> 4)][wn * (TN / 2) + 16 * b + (lane & 15)];
+#pragma unroll
+        for (int a = 0; a < MI; ++a)
+#pragma unroll
+          for (int b = 0; b < NI; ++b) {
+            if constexpr (std::is_same<MT, double>::value)
+              acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+            else
+              acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+          }
       }
       __syncthreads();
     }
@@ -156,14 +179,14 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
   // epilogue: register r of a lane holds column lane%16 of each 16x16 block and
   // row 4*(lane/16)+r for the f32 MFMA, row 4*r+lane/16 for the f64 MFMA
 #pragma unroll
-  for (int bi = 0; bi < 2; ++bi)
+  for (int bi = 0; bi < MI; ++bi)
 #pragma unroll
-    for (int bj = 0; bj < 2; ++bj)
+    for (int bj = 0; bj < NI; ++bj)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int rr = std::is_same<MT, double>::value ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
-        const int64_t gi = row0 + wm * 32 + bi * 16 + rr;
-        const int64_t gj = col0 + wn * 32 + bj * 16 + (lane & 15);
+        const int64_t gi = row0 + wm * (TM / 2) + bi * 16 + rr;
+        const int64_t gj = col0 + wn * (TN / 2) + bj * 16 + (lane & 15);
         if (gi < P.M && gj < P.N) {
           const int64_t ci = gi * P.c_rs + gj * P.c_cs;
           double v = P.alpha * double(acc[bi][bj][r]);
@@ -293,72 +316,134 @@ void GemmBatch::seg(const void* A, int64_t a_rs, int64_t a_cs, const void* B, in
 }
 
 namespace {
-template <class TA, class TB, class TC>
-void launch_math(Ctx& ctx, Prec math, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
+// tile configurations of the scalar kernel: {TM, TN}
+constexpr int NCFG = 4;
+constexpr int CFG_TM[NCFG] = {64, 128, 64, 128};
+constexpr int CFG_TN[NCFG] = {64, 64, 128, 128};
+
+template <class TA, class TB, class TC, class MT>
+void launch_scalar(Ctx& ctx, int cfg, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
   dim3 grid(ntiles), block(256);
-  if (math == Prec::F64)
-    hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, double>), grid, block, 0, ctx.stream, dp, np, ds);
-  else if (math == Prec::F32)
-    hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, float>), grid, block, 0, ctx.stream, dp, np, ds);
-  else
-    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, TC>), grid, block, 0, ctx.stream, dp, np, ds);
+  switch (cfg) {
+    case 0: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 64>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    case 1: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 64>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    case 2: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 128>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    default: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 128>), grid, block, 0, ctx.stream, dp, np, ds); break;
+  }
+}
+
+// the precision combinations the library uses: (A, B, C, math)
+template <class TA, class TB, class TC>
+void launch_combo(Ctx& ctx, Prec math, int cfg, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
+  if (math == Prec::F64) launch_scalar<TA, TB, TC, double>(ctx, cfg, ntiles, dp, np, ds);
+  else if (math == Prec::F32) launch_scalar<TA, TB, TC, float>(ctx, cfg, ntiles, dp, np, ds);
+  else hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, TC>), dim3(ntiles), dim3(256), 0, ctx.stream, dp, np, ds);
+}
+
+void launch(Ctx& ctx, Prec ta, Prec tb, Prec tc, Prec math, int cfg, int ntiles, const GemmProb* dp, int np,
+            const GemmSeg* ds) {
+  using P = Prec;
+  if (ta == P::F64 && tb == P::F64 && tc == P::F64) launch_combo<double, double, double>(ctx, math, cfg, ntiles, dp, np, ds);
+  else if (ta == P::F64 && tb == P::F32 && tc == P::F32) launch_combo<double, float, float>(ctx, math, cfg, ntiles, dp, np, ds);
+  else if (ta == P::F64 && tb == P::BF16 && tc == P::BF16) launch_combo<double, bf16s, bf16s>(ctx, math, cfg, ntiles, dp, np, ds);
+  else if (ta == P::F32 && tb == P::F32 && tc == P::F32) launch_combo<float, float, float>(ctx, math, cfg, ntiles, dp, np, ds);
+  else if (ta == P::BF16 && tb == P::BF16 && tc == P::BF16) launch_combo<bf16s, bf16s, bf16s>(ctx, math, cfg, ntiles, dp, np, ds);
+  else if (ta == P::F64 && tb == P::F32 && tc == P::F64) launch_combo<double, float, double>(ctx, math, cfg, ntiles, dp, np, ds);
+  else if (ta == P::F64 && tb == P::BF16 && tc == P::F64) launch_combo<double, bf16s, double>(ctx, math, cfg, ntiles, dp, np, ds);
+  else throw Error(ST_INVALID_ARG, "unsupported GEMM precision combination");
   TTR_HIP(hipGetLastError());
 }
 
-template <class TA, class TB>
-void launch_c(Ctx& ctx, Prec tc, Prec math, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
-  if (tc == Prec::F64) launch_math<TA, TB, double>(ctx, math, ntiles, dp, np, ds);
-  else if (tc == Prec::F32) launch_math<TA, TB, float>(ctx, math, ntiles, dp, np, ds);
-  else launch_math<TA, TB, bf16s>(ctx, math, ntiles, dp, np, ds);
-}
-
-template <class TA>
-void launch_b(Ctx& ctx, Prec tb, Prec tc, Prec math, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
-  if (tb == Prec::F64) launch_c<TA, double>(ctx, tc, math, ntiles, dp, np, ds);
-  else if (tb == Prec::F32) launch_c<TA, float>(ctx, tc, math, ntiles, dp, np, ds);
-  else launch_c<TA, bf16s>(ctx, tc, math, ntiles, dp, np, ds);
-}
+int ilog2(double v) { return v < 1.0 ? 0 : int(std::floor(std::log2(v))); }
 }  // namespace
 
 void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, const char* site) {
-  // drop empty problems, assign tile prefixes
   std::vector<GemmProb> probs;
-  probs.reserve(b.probs_.size());
-  int64_t ntiles = 0;
-  for (auto p : b.probs_) {
-    if (p.M <= 0 || p.N <= 0) continue;
-    const int tm = (p.M + BM - 1) / BM, tn = (p.N + BN - 1) / BN;
-    p.tile0 = int32_t(ntiles);
-    p.tiles_n = tn;
-    ntiles += int64_t(tm) * tn;
-    probs.push_back(p);
-  }
   std::vector<GemmSeg> segs = std::move(b.segs_);
+  probs.reserve(b.probs_.size());
+  double sm = 0.0, sn = 0.0, sk = 0.0, flops = 0.0, bytes = 0.0;
+  bool beta0 = true;
+  for (const auto& p : b.probs_) {
+    if (p.M <= 0 || p.N <= 0) continue;
+    probs.push_back(p);
+    double k = 0.0;
+    for (int q = 0; q < p.nseg; ++q) {
+      const GemmSeg& sg = segs[p.seg0 + q];
+      k += sg.K;
+      bytes += double(p.M) * sg.K * prec_bytes(ta) + double(sg.K) * p.N * prec_bytes(tb);
+    }
+    sm += p.M;
+    sn += p.N;
+    sk += k;
+    flops += 2.0 * p.M * p.N * k;
+    bytes += double(p.M) * p.N * prec_bytes(tc) * (p.beta != 0.0 ? 2.0 : 1.0);
+    beta0 = beta0 && p.beta == 0.0;
+  }
   b.clear();
   if (probs.empty()) return;
-  TTR_REQUIRE(ntiles < (int64_t(1) << 31), ST_INVALID_ARG, "GEMM batch too large");
-  const GemmProb* dp = ctx.upload(probs);
-  const GemmSeg* ds = ctx.upload(segs);
   const int np = int(probs.size());
-  double flops = 0.0, bytes = 0.0;
-  if (ctx.prof.on) {
-    for (const auto& p : probs) {
-      double k = 0.0;
-      for (int q = 0; q < p.nseg; ++q) {
-        const GemmSeg& sg = segs[p.seg0 + q];
-        k += sg.K;
-        bytes += double(p.M) * sg.K * prec_bytes(ta) + double(sg.K) * p.N * prec_bytes(tb);
-      }
-      flops += 2.0 * p.M * p.N * k;
-      bytes += double(p.M) * p.N * prec_bytes(tc) * (p.beta != 0.0 ? 2.0 : 1.0);
-    }
+  const double am = sm / np, an = sn / np, ak = sk / np;
+  // tile choice: measured once per (site, precisions, size class) when the batch
+  // can be run more than once (beta == 0, C never aliases A or B at any call
+  // site), otherwise 64 x 64 (measured fastest for the accumulating updates)
+  int cfg = 0;
+  bool tune = false;
+  std::string key;
+  if (math != Prec::BF16) {
+    key = std::string(site ? site : "-") + "|" + std::to_string(int(ta)) + std::to_string(int(tb)) +
+          std::to_string(int(tc)) + std::to_string(int(math)) + "|" + std::to_string(ilog2(am)) + "," +
+          std::to_string(ilog2(an)) + "," + std::to_string(ilog2(ak)) + "," + std::to_string(ilog2(np));
+    auto it = ctx.gemm_cfg.find(key);
+    if (it != ctx.gemm_cfg.end()) cfg = it->second;
+    else if (ctx.gemm_autotune && beta0) tune = true;
+  } else {
+    cfg = 0;
   }
+  auto prepare = [&](int c, int64_t& ntiles) -> const GemmProb* {
+    const int TM = math == Prec::BF16 ? BM : CFG_TM[c], TN = math == Prec::BF16 ? BN : CFG_TN[c];
+    ntiles = 0;
+    for (auto& p : probs) {
+      const int tm = (p.M + TM - 1) / TM, tn = (p.N + TN - 1) / TN;
+      p.tile0 = int32_t(ntiles);
+      p.tiles_n = tn;
+      ntiles += int64_t(tm) * tn;
+    }
+    TTR_REQUIRE(ntiles < (int64_t(1) << 31), ST_INVALID_ARG, "GEMM batch too large");
+    return ctx.upload(probs);
+  };
+  const GemmSeg* ds = ctx.upload(segs);
   static const char* names[3] = {"gemm_f64", "gemm_f32", "gemm_bf16"};
   const std::string site_name = site ? std::string("site:") + site : std::string();
   ProfScope ps(ctx, names[int(math)], flops, bytes, site ? site_name.c_str() : nullptr);
-  if (ta == Prec::F64) launch_b<double>(ctx, tb, tc, math, int(ntiles), dp, np, ds);
-  else if (ta == Prec::F32) launch_b<float>(ctx, tb, tc, math, int(ntiles), dp, np, ds);
-  else launch_b<bf16s>(ctx, tb, tc, math, int(ntiles), dp, np, ds);
+  if (tune) {
+    hipEvent_t e0, e1;
+    TTR_HIP(hipEventCreate(&e0));
+    TTR_HIP(hipEventCreate(&e1));
+    float best = 0.0f;
+    int best_cfg = cfg;
+    for (int c = 0; c < NCFG; ++c) {
+      if ((CFG_TM[c] == 128 && am < 96) || (CFG_TN[c] == 128 && an < 96)) continue;
+      int64_t nt;
+      const GemmProb* dp = prepare(c, nt);
+      TTR_HIP(hipEventRecord(e0, ctx.stream));
+      launch(ctx, ta, tb, tc, math, c, int(nt), dp, np, ds);
+      TTR_HIP(hipEventRecord(e1, ctx.stream));
+      TTR_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      TTR_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (best == 0.0f || ms < best) {
+        best = ms;
+        best_cfg = c;
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx.gemm_cfg[key] = best_cfg;  // every candidate wrote the same C
+    return;
+  }
+  int64_t nt;
+  const GemmProb* dp = prepare(cfg, nt);
+  launch(ctx, ta, tb, tc, math, cfg, int(nt), dp, np, ds);
 }
 
 }  // namespace ttr

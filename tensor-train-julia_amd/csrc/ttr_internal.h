@@ -143,6 +143,9 @@ struct Ctx {
   PinnedArena pin;
   CachingAlloc cache;
   Profiler prof;
+  // GEMM tile choice per (call site, precisions, size class), measured on first use
+  std::unordered_map<std::string, int> gemm_cfg;
+  bool gemm_autotune = true;
   // copies a host array into fresh device workspace (asynchronously, through pinned memory)
   template <class T>
   T* upload(const T* host, size_t n) {
