@@ -38,6 +38,15 @@ template <> __device__ __forceinline__ double load_d<double>(const void* p, int6
 template <> __device__ __forceinline__ double load_d<float>(const void* p, int64_t i) { return static_cast<const float*>(p)[i]; }
 template <> __device__ __forceinline__ double load_d<bf16s>(const void* p, int64_t i) { return bf16_to_f32(static_cast<const uint16_t*>(p)[i]); }
 
+// operands live in global memory: global (not flat) loads
+template <class T> using gptr = const T __attribute__((address_space(1)))*;
+template <class T> __device__ __forceinline__ T ldg(const void* p, int64_t i) { return ((gptr<T>)p)[i]; }
+template <> __device__ __forceinline__ bf16s ldg<bf16s>(const void* p, int64_t i) {
+  bf16s r;
+  r.x = ((gptr<uint16_t>)p)[i];
+  return r;
+}
+
 __device__ __forceinline__ double to_double(double v) { return v; }
 __device__ __forceinline__ double to_double(float v) { return v; }
 __device__ __forceinline__ double to_double(bf16s v) { return bf16_to_f32(v.x); }
@@ -118,24 +127,35 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
     // the segment scale are applied when the registers are stored to LDS.
     TA ra[NA];
     TB rb[NB];
-    const TA* Ap = static_cast<const TA*>(S.A);
-    const TB* Bp = static_cast<const TB*>(S.B);
+    // tiles entirely inside the operands (the common case) load without predicates
+    const bool a_full = row0 + TM <= P.M && S.a_mode == AM_PLAIN;
+    const bool b_full = col0 + TN <= P.N;
     auto load_tiles = [&](int k0) {
+      const bool k_full = k0 + BK <= S.K;
 #pragma unroll
       for (int q = 0; q < NA; ++q) {
         int i, l;
         a_idx(q, i, l);
         const int64_t gi = row0 + i, gl = k0 + l;
-        const bool in = gi < P.M && gl < S.K &&
-                        (S.a_mode == AM_PLAIN || (S.a_mode == AM_VT_UNIT ? gl > gi : gi > gl));
-        ra[q] = in ? Ap[gi * S.a_rs + gl * S.a_cs] : TA{};
+        if (a_full && k_full) {
+          ra[q] = ldg<TA>(S.A, gi * S.a_rs + gl * S.a_cs);
+        } else {
+          const bool in = gi < P.M && gl < S.K &&
+                          (S.a_mode == AM_PLAIN || (S.a_mode == AM_VT_UNIT ? gl > gi : gi > gl));
+          ra[q] = TA{};
+          if (in) ra[q] = ldg<TA>(S.A, gi * S.a_rs + gl * S.a_cs);
+        }
       }
 #pragma unroll
       for (int q = 0; q < NB; ++q) {
         int l, j;
         b_idx(q, l, j);
         const int64_t gl = k0 + l, gj = col0 + j;
-        rb[q] = (gl < S.K && gj < P.N) ? Bp[gl * S.b_rs + gj * S.b_cs] : TB{};
+        if (b_full && k_full) rb[q] = ldg<TB>(S.B, gl * S.b_rs + gj * S.b_cs);
+        else {
+          rb[q] = TB{};
+          if (gl < S.K && gj < P.N) rb[q] = ldg<TB>(S.B, gl * S.b_rs + gj * S.b_cs);
+        }
       }
     };
     load_tiles(0);
