@@ -456,6 +456,8 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
         ctx.prof.stats["jacobi_svd:ticks_gram"].flops += info[1];
         ctx.prof.stats["jacobi_svd:ticks_inner"].flops += info[2];
         ctx.prof.stats["jacobi_svd:ticks_apply"].flops += info[3];
+        ctx.prof.stats["jacobi_svd:ticks_inner_params"].flops += info[6];
+        ctx.prof.stats["jacobi_svd:ticks_inner_rotate"].flops += info[7];
       }
       std::vector<int32_t> order(m);
       std::iota(order.begin(), order.end(), 0);

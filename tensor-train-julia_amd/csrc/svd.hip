@@ -48,6 +48,27 @@ struct JacJob {
 __host__ __device__ __forceinline__ int rr_player(int pos, int r, int pp) {
   return pos == 0 ? 0 : ((pos - 1 + r) % (pp - 1)) + 1;
 }
+// 1/sqrt(x) and 1/x for positive normal x: the hardware estimate refined by two
+// Newton-Raphson steps, y += y (1 - x y^2) / 2 and y += y (1 - x y)
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
+}
+__device__ __forceinline__ double rcp_nr(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+  }
+  return y;
+}
+
 __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __restrict__ jobs, int njobs, int round,
                                                              int sweep, int32_t* __restrict__ flags) {
   const int bid = blockIdx.x;
@@ -216,6 +237,8 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
   double cV[4] = {1.0, 1.0, 1.0, 1.0}, sV[4] = {0.0, 0.0, 0.0, 0.0};
   int aV[4] = {0, 0, 0, 0}, bV[4] = {0, 0, 0, 0};
   bool visit_rot = false;
+  uint64_t tp1 = 0, tp2 = 0, tr = 0;
+  if (tid == 0) tr = wall_clock64();
   for (int ir = 0; ir < IR; ++ir) {
     const int buf = ir & 1;
     const int a = min(xA, xB), b = max(xA, xB);
@@ -226,8 +249,17 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
       // padding columns have a zero diagonal and never rotate
       if (aa > 0.0 && bb2 > 0.0 && ab * ab > tol2 * aa * bb2) {
         const double dd = bb2 - aa;
-        const double t = 2.0 * ab * copysign(1.0, dd) / (fabs(dd) + sqrt(dd * dd + 4.0 * ab * ab));
-        c = rsqrt(1.0 + t * t);
+        const double q = dd * dd + 4.0 * ab * ab;
+        double t;
+        if (q > 0x1p-900) {
+          // G is scaled to max diagonal ~1, so q <= ~20: hardware estimates with two
+          // Newton steps each (rsqrt, reciprocal) are accurate to ~1 ulp here
+          const double rq = rsqrt_nr(q);
+          t = 2.0 * ab * copysign(1.0, dd) * rcp_nr(fabs(dd) + q * rq);
+        } else {
+          t = 2.0 * ab * copysign(1.0, dd) / (fabs(dd) + sqrt(q));
+        }
+        c = rsqrt_nr(1.0 + t * t);  // 1 + t^2 in [1, 2]
         s = c * t;
       }
       pc[sl] = c;
@@ -248,44 +280,53 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
       }
     }
     __syncthreads();
-    // ---- phase 2
+    if (tid == 0) {
+      const uint64_t t = wall_clock64();
+      tp1 += t - tr;
+      tr = t;
+    }
+    // ---- phase 2: all loads first, the rotations applied unconditionally (an
+    // identity rotation reproduces its operands exactly), stores only where a
+    // rotation acts
     const double c = pc[sl], s = ps[sl];
     visit_rot = visit_rot || __any(s != 0.0);
     const bool nxt = ir + 1 < IR;
+    double cc[2], scc[2], g[2][4];
+    int ac[2], bc[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int kc = 2 * wave + 16 * j + h;
-      const double cc = pc[kc], scc = ps[kc];
-      const int ac = min(xc[j], yc[j]), bc = max(xc[j], yc[j]);
-      double g00, g01, g10, g11;
-      const bool rot = (s != 0.0) || (scc != 0.0);
-      if (rot) {
-        g00 = G[a][ac]; g01 = G[a][bc];
-        g10 = G[b][ac]; g11 = G[b][bc];
-        const double t00 = c * g00 - s * g10, t01 = c * g01 - s * g11;
-        const double t10 = s * g00 + c * g10, t11 = s * g01 + c * g11;
-        g00 = t00 * cc - t01 * scc;
-        g01 = t00 * scc + t01 * cc;
-        g10 = t10 * cc - t11 * scc;
-        g11 = t10 * scc + t11 * cc;
-        G[a][ac] = g00; G[a][bc] = g01;
-        G[b][ac] = g10; G[b][bc] = g11;
+      cc[j] = pc[kc];
+      scc[j] = ps[kc];
+      ac[j] = min(xc[j], yc[j]);
+      bc[j] = max(xc[j], yc[j]);
+      g[j][0] = G[a][ac[j]]; g[j][1] = G[a][bc[j]];
+      g[j][2] = G[b][ac[j]]; g[j][3] = G[b][bc[j]];
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const double t00 = c * g[j][0] - s * g[j][2], t01 = c * g[j][1] - s * g[j][3];
+      const double t10 = s * g[j][0] + c * g[j][2], t11 = s * g[j][1] + c * g[j][3];
+      g[j][0] = t00 * cc[j] - t01 * scc[j];
+      g[j][1] = t00 * scc[j] + t01 * cc[j];
+      g[j][2] = t10 * cc[j] - t11 * scc[j];
+      g[j][3] = t10 * scc[j] + t11 * cc[j];
+      if (s != 0.0 || scc[j] != 0.0) {
+        G[a][ac[j]] = g[j][0]; G[a][bc[j]] = g[j][1];
+        G[b][ac[j]] = g[j][2]; G[b][bc[j]] = g[j][3];
       }
       if (nxt) {
+        const int kc = 2 * wave + 16 * j + h;
         if (kc == sl) {  // diagonal block: G(a,a), G(b,b)
-          pdiag[buf ^ 1][a] = rot ? g00 : G[a][a];
-          pdiag[buf ^ 1][b] = rot ? g11 : G[b][b];
+          pdiag[buf ^ 1][a] = g[j][0];
+          pdiag[buf ^ 1][b] = g[j][3];
         }
         // next-round pairs (xA, yA) and (xB, yB): the thread holding row min and
         // column max of a pair stores its entry
-        if (xA < yA && (yA == ac || yA == bc)) {
-          const bool ra = xA == a, ca = yA == ac;
-          pab[buf ^ 1][snA] = rot ? (ra ? (ca ? g00 : g01) : (ca ? g10 : g11)) : G[xA][yA];
-        }
-        if (xB < yB && (yB == ac || yB == bc)) {
-          const bool ra = xB == a, ca = yB == ac;
-          pab[buf ^ 1][snB] = rot ? (ra ? (ca ? g00 : g01) : (ca ? g10 : g11)) : G[xB][yB];
-        }
+        if (xA < yA && (yA == ac[j] || yA == bc[j]))
+          pab[buf ^ 1][snA] = xA == a ? (yA == ac[j] ? g[j][0] : g[j][1]) : (yA == ac[j] ? g[j][2] : g[j][3]);
+        if (xB < yB && (yB == ac[j] || yB == bc[j]))
+          pab[buf ^ 1][snB] = xB == a ? (yB == ac[j] ? g[j][0] : g[j][1]) : (yB == ac[j] ? g[j][2] : g[j][3]);
       }
     }
     // this round's V rotation is applied in phase 1 of the next round
@@ -305,6 +346,15 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
       yc[j] = adv(yc[j]);
     }
     __syncthreads();
+    if (tid == 0) {
+      const uint64_t t = wall_clock64();
+      tp2 += t - tr;
+      tr = t;
+    }
+  }
+  if (tid == 0 && S.info) {
+    atomicAdd(reinterpret_cast<unsigned*>(S.info) + 6, unsigned(tp1));
+    atomicAdd(reinterpret_cast<unsigned*>(S.info) + 7, unsigned(tp2));
   }
   {
     double va[4], vb[4];
