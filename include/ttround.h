@@ -114,6 +114,28 @@ int ttr_round_svd(ttr_handle_t h, int64_t n, const int64_t* ids, double tol, int
 /* Frobenius norms of n TTs.  Replaces: norm(::TTvector) (src/tt_rounding.jl:107-114). */
 int ttr_norm(ttr_handle_t h, int64_t n, const int64_t* ids, double* out);
 
+/* ------------------------------------------------------- inner products */
+
+/* Euclidean inner products <x_i, y_i> of n pairs (x_i and y_i of equal dims),
+ * FP64, folded left to right with every pair in the same launches.
+ * Replaces: dot(A, B) (src/tt_operations.jl:302-321) and dot_par(A, B)
+ * (:339-370), the Gram-Schmidt products of the Krylov solvers
+ * (src/tt_solvers.jl:69,90,279-280,315). */
+int ttr_dot(ttr_handle_t h, int64_t n, const int64_t* x_ids, const int64_t* y_ids, double* out);
+
+/* All partial contractions of x and y: W[k] (r^x_k x r^y_k, column-major,
+ * k = 0..d) into W_out[k] (NULL entries skipped); reverse != 0 contracts the
+ * cores k..d-1 into W[k] (W[0] = <x, y>, W[d] = 1), reverse == 0 the cores
+ * 0..k-1 (W[0] = 1, W[d] = <x, y>).
+ * Replaces: partial_contraction(A, B; reverse) (src/sketches.jl:1047-1070). */
+int ttr_partial_contraction(ttr_handle_t h, int64_t x_id, int64_t y_id, int reverse, double* const* W_out);
+
+/* Bilinear forms <psi_i|H_i|phi_i> of n triples without forming H phi, FP64.
+ * Replaces: dot_operator(psi, H, phi) (src/tt_operations.jl:385-421), the
+ * energy evaluations of src/tt_solvers.jl:586,607. */
+int ttr_dot_operator(ttr_handle_t h, int64_t n, const int64_t* psi_ids, const int64_t* op_ids, const int64_t* phi_ids,
+                     double* out);
+
 /* ---------------------------------------------------- randomized rounding */
 
 typedef struct {

@@ -33,7 +33,8 @@ __all__ = [
     "sketch_structure", "generate_sketch_blocks", "contract_sketch_core_backwards",
     "contract_sketch_core_backwards_op", "tt_recursive_sketch",
     "tt_recursive_sketch_op", "ttrand_rounding", "ttrand_rounding_lincomb",
-    "ttrand_rounding_op", "ttrand_rounding_residual", "rel_err", "tt_dot",
+    "ttrand_rounding_op", "ttrand_rounding_residual", "rel_err", "tt_dot", "dot",
+    "dot_par", "partial_contraction", "dot_operator",
     "hubbard_1d_mpo", "hubbard_1d_dense",
 ]
 
@@ -242,11 +243,82 @@ def ttv_to_tto(x: TT) -> TTO:
 def tt_dot(x: TT, y: TT) -> float:
     """Euclidean inner product via the right-to-left partial contraction
     (``src/sketches.jl:1047-1070``)."""
-    W = np.ones((1, 1))
-    for k in range(x.N - 1, -1, -1):
-        # W_k[a,b] = A[z,a,α] B[z,b,β] W[α,β]
-        W = np.einsum("zax,zby,xy->ab", x.cores[k], y.cores[k], W, optimize=True)
-    return float(W[0, 0])
+    return float(partial_contraction(x, y, reverse=True)[0][0, 0])
+
+
+def dot(x: TT, y: TT) -> float:
+    """``dot(A, B)`` (``src/tt_operations.jl:302-321``): the left-to-right fold
+    ``M[a,b] = A_k[z,α,a] B_k[z,β,b] out[α,β]`` starting from ``out = 1``."""
+    assert x.dims == y.dims, "TT dimensions are not compatible"
+    out = np.ones((1, 1))
+    for k in range(x.N):
+        out = np.einsum("zxa,zyb,xy->ab", x.cores[k], y.cores[k], out, optimize=True)
+    return float(out[0, 0])
+
+
+def dot_par(x: TT, y: TT) -> float:
+    """``dot_par(A, B)`` (``src/tt_operations.jl:339-370``): per-core transfer
+    matrices ``Y_k[(a,b),(c,d)] = A_k[z,a,c] B_k[z,b,d]`` multiplied from the
+    right, ``C = Y_1 (Y_2 (... Y_d))``."""
+    assert x.dims == y.dims, "TT dimensions are not compatible"
+    Y = []
+    for k in range(x.N):
+        M = np.einsum("zac,zbd->abcd", x.cores[k], y.cores[k])
+        Y.append(jreshape(M, (x.rks[k] * y.rks[k], x.rks[k + 1] * y.rks[k + 1])))
+    C = Y[-1].reshape(-1, order="F")
+    for k in range(x.N - 2, -1, -1):
+        C = Y[k] @ C
+    return float(C[0])
+
+
+def partial_contraction(x: TT, y: TT, reverse: bool = True) -> List[np.ndarray]:
+    """``partial_contraction(A, B; reverse)`` (``src/sketches.jl:1047-1070``):
+    ``W[k]`` of shape ``(r^A_k, r^B_k)``; right to left
+    ``W_k[a,b] = A_k[z,a,α] B_k[z,b,β] W_{k+1}[α,β]`` from ``W_{N+1} = 1``, left
+    to right ``W_{k+1}[a,b] = A_k[z,α,a] B_k[z,β,b] W_k[α,β]`` from ``W_1 = 1``."""
+    assert x.dims == y.dims, "TT dimensions are not compatible"
+    d = x.N
+    W = [np.zeros((x.rks[k], y.rks[k])) for k in range(d + 1)]
+    if reverse:
+        W[d] = np.ones((1, 1))
+        for k in range(d - 1, -1, -1):
+            W[k] = np.einsum("zax,zby,xy->ab", x.cores[k], y.cores[k], W[k + 1], optimize=True)
+    else:
+        W[0] = np.ones((1, 1))
+        for k in range(d):
+            W[k + 1] = np.einsum("zxa,zyb,xy->ab", x.cores[k], y.cores[k], W[k], optimize=True)
+    return W
+
+
+def dot_operator(psi: TT, H: TTO, phi: TT) -> float:
+    """``dot_operator(psi, H, phi)`` = ``<psi|H|phi>`` (``src/tt_operations.jl:385-421``):
+    pivot ``c`` = the later of the two orthogonality centres (first core with
+    ``ot == 0``, else the last core); left environment over cores ``1..c`` when
+    ``c > 1``, right environment over ``N..c+1`` (``N..1`` when ``c == 1``),
+    combined by a full contraction."""
+    N = psi.N
+    assert psi.dims == phi.dims and tuple(H.dims) == tuple(psi.dims), "TT dimensions are not compatible"
+
+    def center(v):
+        for k, o in enumerate(v.ot):
+            if o == 0:
+                return k + 1
+        return N
+
+    c = max(center(psi), center(phi))
+    Lenv = np.ones((1, 1, 1))
+    for k in range(c if c > 1 else 0):
+        Lenv = np.einsum("xyz,ixa,ijyb,jzc->abc", Lenv, psi.cores[k], H.cores[k], phi.cores[k], optimize=True)
+    Renv = np.ones((1, 1, 1))
+    r = c + 1 if c > 1 else 1
+    for k in range(N, r - 1, -1):
+        Renv = np.einsum("ixa,ijyb,jzc,abc->xyz", psi.cores[k - 1], H.cores[k - 1], phi.cores[k - 1], Renv,
+                         optimize=True)
+    if c == 1:
+        return float(Renv[0, 0, 0])
+    if c == N:
+        return float(Lenv[0, 0, 0])
+    return float(np.sum(Lenv * Renv))
 
 
 # ---------------------------------------------------------------------------

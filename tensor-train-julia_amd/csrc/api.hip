@@ -28,13 +28,20 @@ struct DevTTO {
   int d = 0;
   std::vector<int64_t> dims, rks;
   std::vector<double*> cores;  // reference layout (n, n, R, R')
+  // cores with n*max(R, R') > 64 also as the matrix Hm[(i + n b), (beta + R j)]
+  // = H[i, j, beta, b] ((n R') x (R n), column-major), for GEMM contractions of
+  // bonds too wide for the LDS-staged local action; nullptr elsewhere
+  std::vector<double*> hm;
   void* base = nullptr;
+  void* base_m = nullptr;
   DevTTO() = default;
   DevTTO(const DevTTO&) = delete;
   DevTTO& operator=(const DevTTO&) = delete;
   ~DevTTO() {
     if (base) (void)hipFree(base);
+    if (base_m) (void)hipFree(base_m);
   }
+  bool wide(int k) const { return dims[k] * std::max(rks[k], rks[k + 1]) > 64; }
 };
 
 int64_t wrap_mul(int64_t a, int64_t b) { return int64_t(uint64_t(a) * uint64_t(b)); }
@@ -1231,6 +1238,29 @@ int ttr_tto_upload(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t
       op->cores.push_back(p);
       off += e;
     }
+    // matrix copies of the wide cores, permuted on the host
+    int64_t total_m = 0;
+    for (int k = 0; k < d; ++k)
+      if (op->wide(k)) total_m += op->dims[k] * op->dims[k] * op->rks[k] * op->rks[k + 1];
+    op->hm.assign(size_t(d), nullptr);
+    if (total_m > 0) {
+      TTR_HIP(hipMalloc(&op->base_m, size_t(total_m) * 8));
+      int64_t offm = 0;
+      for (int k = 0; k < d; ++k) {
+        if (!op->wide(k)) continue;
+        const int64_t n = op->dims[k], R = op->rks[k], R1 = op->rks[k + 1];
+        std::vector<double> m(size_t(n * n * R * R1));
+        for (int64_t b = 0; b < R1; ++b)
+          for (int64_t be = 0; be < R; ++be)
+            for (int64_t j = 0; j < n; ++j)
+              for (int64_t i = 0; i < n; ++i)
+                m[size_t((i + n * b) + n * R1 * (be + R * j))] = cores[k][i + n * j + n * n * be + n * n * R * b];
+        double* p = static_cast<double*>(op->base_m) + offm;
+        TTR_HIP(hipMemcpy(p, m.data(), m.size() * 8, hipMemcpyHostToDevice));
+        op->hm[size_t(k)] = p;
+        offm += int64_t(m.size());
+      }
+    }
     const int64_t nid = h->next_id++;
     h->ops[nid] = std::move(op);
     *id = nid;
@@ -1391,6 +1421,216 @@ int ttr_norm(ttr_handle_t h, int64_t n, const int64_t* ids, double* out) {
     for (int64_t i = 0; i < n; ++i) out[i] = r[i];
     for (auto& t : tmp)
       if (t) h->free_tt(t.get());
+  });
+}
+
+namespace {
+// ------------------------------------------------------------ inner products
+
+// Environments of pairs of TTs (x_i, y_i) with equal dims, all pairs in the same
+// launches.  Left to right (dot, src/tt_operations.jl:302-321, and
+// partial_contraction(; reverse=false), src/sketches.jl:1063-1067), with
+// W_0 = 1 and W_k the rA_k x rB_k environment:
+//   T       = W_k Y_k      (Y_k read as rB_k x (n rB_{k+1}))
+//   W_{k+1} = X_k^T T      (X_k read as (rA_k n) x rA_{k+1}, T as (rA_k n) x rB_{k+1})
+// Right to left (partial_contraction(; reverse=true), :1058-1062), W_d = 1:
+//   T       = Y_k W_{k+1}^T   (Y_k read as (rB_k n) x rB_{k+1})
+//   W_k     = X_k T^T         (X_k read as rA_k x (n rA_{k+1}), T as rB_k x (n rA_{k+1}))
+// W[i][k] are device arrays (column-major) the caller provides for k = 0..d;
+// the boundary environment (W_0 left to right, W_d right to left) is never read,
+// the first core's product with it is skipped.
+void pair_environments(ttr_handle_s* h, const std::vector<DevTT*>& xs, const std::vector<DevTT*>& ys, bool reverse,
+                       const std::vector<std::vector<double*>>& W) {
+  Ctx& ctx = h->ctx;
+  const size_t np = xs.size();
+  int dmax = 0;
+  std::vector<double*> T(np);
+  for (size_t i = 0; i < np; ++i) {
+    const DevTT* x = xs[i];
+    const DevTT* y = ys[i];
+    int64_t mt = 1;
+    for (int k = 0; k < x->d; ++k)
+      mt = std::max(mt, reverse ? y->rks[k] * x->dims[k] * x->rks[k + 1] : x->rks[k] * x->dims[k] * y->rks[k + 1]);
+    T[i] = ctx.ws.alloc_n<double>(size_t(mt));
+    dmax = std::max(dmax, x->d);
+  }
+  GemmBatch gb;
+  for (int s = 0; s < dmax; ++s) {
+    // step s visits core k = s (left to right) or k = d-1-s (right to left)
+    for (size_t i = 0; i < np; ++i) {
+      const DevTT* x = xs[i];
+      const DevTT* y = ys[i];
+      if (s == 0 || s >= x->d) continue;
+      const int k = reverse ? x->d - 1 - s : s;
+      const int64_t n = x->dims[k], ra = x->rks[k], rb = y->rks[k], ra1 = x->rks[k + 1], rb1 = y->rks[k + 1];
+      if (reverse)
+        gb.add(int(rb * n), int(ra1), int(rb1), 1.0, y->cores[k], 1, rb * n, W[i][size_t(k + 1)], ra1, 1, 0.0, T[i],
+               1, rb * n);
+      else
+        gb.add(int(ra), int(n * rb1), int(rb), 1.0, W[i][size_t(k)], 1, ra, y->cores[k], 1, rb, 0.0, T[i], 1, ra);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "dot_env");
+    for (size_t i = 0; i < np; ++i) {
+      const DevTT* x = xs[i];
+      const DevTT* y = ys[i];
+      if (s >= x->d) continue;
+      const int k = reverse ? x->d - 1 - s : s;
+      const int64_t n = x->dims[k], ra = x->rks[k], rb = y->rks[k], ra1 = x->rks[k + 1], rb1 = y->rks[k + 1];
+      // the boundary environment is 1: the first product's result is Y_k itself
+      const double* Tk = s == 0 ? y->cores[k] : T[i];
+      if (reverse)
+        gb.add(int(ra), int(rb), int(n * ra1), 1.0, x->cores[k], 1, ra, Tk, rb, 1, 0.0, W[i][size_t(k)], 1, ra);
+      else
+        gb.add(int(ra1), int(rb1), int(ra * n), 1.0, x->cores[k], ra * n, 1, Tk, 1, ra * n, 0.0, W[i][size_t(k + 1)],
+               1, ra1);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "dot_core");
+  }
+}
+
+// <psi_i|H_i|phi_i> (dot_operator, src/tt_operations.jl:385-421) for a batch of
+// triples, folded left to right through the three-index environment
+// L[(a, b, c)] (psi bond fastest, then the H bond, then the phi bond), all
+// triples in the same launches:
+//   T1 = L B_k                         GEMM, L read as (rA R) x rB, B_k as rB x (n rB')
+//   T2[(a, i), (b', c)] = sum_{b, j} H_k[i, j, b, b'] T1[(a, b), (j, c)]
+//                                      op_apply, or for wide bonds one GEMM per c
+//                                      against the matrix copy Hm of H_k
+//   L' = A_k^T T2                      GEMM, A_k read as (rA n) x rA'
+// The reference pivots at the orthogonality centre and folds from both ends;
+// the contraction is the same sum, so results agree to rounding.
+void dot_operator_batch(ttr_handle_s* h, const std::vector<DevTT*>& ps, const std::vector<DevTTO*>& Hs,
+                        const std::vector<DevTT*>& fs, double* out) {
+  Ctx& ctx = h->ctx;
+  const size_t nt = ps.size();
+  int dmax = 0;
+  std::vector<double*> Lb(nt), T1(nt), T2(nt);
+  for (size_t i = 0; i < nt; ++i) {
+    const DevTT* a = ps[i];
+    const DevTTO* H = Hs[i];
+    const DevTT* b = fs[i];
+    int64_t ml = 1, m1 = 1, m2 = 1;
+    for (int k = 0; k < a->d; ++k) {
+      const int64_t n = a->dims[k];
+      ml = std::max(ml, a->rks[k + 1] * H->rks[k + 1] * b->rks[k + 1]);
+      m1 = std::max(m1, a->rks[k] * H->rks[k] * n * b->rks[k + 1]);
+      m2 = std::max(m2, a->rks[k] * n * H->rks[k + 1] * b->rks[k + 1]);
+    }
+    Lb[i] = ctx.ws.alloc_n<double>(size_t(ml));
+    T1[i] = ctx.ws.alloc_n<double>(size_t(m1));
+    T2[i] = ctx.ws.alloc_n<double>(size_t(m2));
+    dmax = std::max(dmax, a->d);
+  }
+  GemmBatch gb;
+  std::vector<OpApply> oj;
+  for (int k = 0; k < dmax; ++k) {
+    if (k > 0) {
+      for (size_t i = 0; i < nt; ++i) {
+        const DevTT* a = ps[i];
+        const DevTT* b = fs[i];
+        if (k >= a->d) continue;
+        const int64_t n = a->dims[k], rar = a->rks[k] * Hs[i]->rks[k], rb = b->rks[k], rb1 = b->rks[k + 1];
+        gb.add(int(rar), int(n * rb1), int(rb), 1.0, Lb[i], 1, rar, b->cores[k], 1, rb, 0.0, T1[i], 1, rar);
+      }
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "dotop_l");
+    }
+    for (size_t i = 0; i < nt; ++i) {
+      const DevTT* a = ps[i];
+      const DevTTO* H = Hs[i];
+      const DevTT* b = fs[i];
+      if (k >= a->d) continue;
+      const int64_t n = a->dims[k], ra = a->rks[k], R = H->rks[k], R1 = H->rks[k + 1], rb1 = b->rks[k + 1];
+      const double* in = k == 0 ? b->cores[0] : T1[i];
+      if (!H->wide(k)) {
+        OpApply o = op_job(H, k, false);
+        o.in = in;
+        o.out = T2[i];
+        o.X = int(ra);
+        o.Y = int(rb1);
+        o.is_x = 1; o.is_b = ra; o.is_m = ra * R; o.is_y = ra * R * n;
+        o.os_x = 1; o.os_m = ra; o.os_b = ra * n; o.os_y = ra * n * R1;
+        oj.push_back(o);
+      } else {
+        for (int64_t c = 0; c < rb1; ++c)
+          gb.add(int(ra), int(n * R1), int(R * n), 1.0, in + ra * R * n * c, 1, ra, H->hm[size_t(k)], n * R1, 1, 0.0,
+                 T2[i] + ra * n * R1 * c, 1, ra);
+      }
+    }
+    op_apply_batched(ctx, oj, Prec::F64, Prec::F64);
+    oj.clear();
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "dotop_h");
+    for (size_t i = 0; i < nt; ++i) {
+      const DevTT* a = ps[i];
+      const DevTT* b = fs[i];
+      if (k >= a->d) continue;
+      const int64_t n = a->dims[k], ra = a->rks[k], ra1 = a->rks[k + 1], cols = Hs[i]->rks[k + 1] * b->rks[k + 1];
+      double* C = k == a->d - 1 ? out + i : Lb[i];
+      gb.add(int(ra1), int(cols), int(ra * n), 1.0, a->cores[k], ra * n, 1, T2[i], 1, ra * n, 0.0, C, 1, ra1);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "dotop_r");
+  }
+}
+}  // namespace
+
+int ttr_dot(ttr_handle_t h, int64_t n, const int64_t* x_ids, const int64_t* y_ids, double* out) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(n >= 1 && x_ids && y_ids && out, ST_INVALID_ARG, "bad arguments");
+    std::vector<DevTT*> xs, ys;
+    std::vector<std::vector<double*>> W(static_cast<size_t>(n));
+    double* res = h->ctx.ws.alloc_n<double>(size_t(n));
+    for (int64_t i = 0; i < n; ++i) {
+      xs.push_back(h->get(x_ids[i]));
+      ys.push_back(h->get(y_ids[i]));
+      TTR_REQUIRE(xs.back()->dims == ys.back()->dims, ST_DIM_MISMATCH, "TT dimensions are not compatible");
+      const DevTT* x = xs.back();
+      const DevTT* y = ys.back();
+      W[i].assign(size_t(x->d + 1), nullptr);
+      for (int k = 1; k < x->d; ++k) W[i][size_t(k)] = h->ctx.ws.alloc_n<double>(size_t(x->rks[k] * y->rks[k]));
+      W[i][size_t(x->d)] = res + i;
+    }
+    pair_environments(h, xs, ys, false, W);
+    h->ctx.sync();
+    TTR_HIP(hipMemcpy(out, res, size_t(n) * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+int ttr_partial_contraction(ttr_handle_t h, int64_t x_id, int64_t y_id, int reverse, double* const* W_out) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(W_out, ST_INVALID_ARG, "bad arguments");
+    DevTT* x = h->get(x_id);
+    DevTT* y = h->get(y_id);
+    TTR_REQUIRE(x->dims == y->dims, ST_DIM_MISMATCH, "TT dimensions are not compatible");
+    const int d = x->d;
+    std::vector<std::vector<double*>> W(1, std::vector<double*>(size_t(d + 1)));
+    for (int k = 0; k <= d; ++k) W[0][size_t(k)] = h->ctx.ws.alloc_n<double>(size_t(x->rks[k] * y->rks[k]));
+    pair_environments(h, {x}, {y}, reverse != 0, W);
+    h->ctx.sync();
+    for (int k = 0; k <= d; ++k) {
+      if (!W_out[k]) continue;
+      const bool boundary = reverse ? k == d : k == 0;
+      if (boundary) W_out[k][0] = 1.0;
+      else TTR_HIP(hipMemcpy(W_out[k], W[0][size_t(k)], size_t(x->rks[k] * y->rks[k]) * 8, hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+int ttr_dot_operator(ttr_handle_t h, int64_t n, const int64_t* psi_ids, const int64_t* op_ids, const int64_t* phi_ids,
+                     double* out) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(n >= 1 && psi_ids && op_ids && phi_ids && out, ST_INVALID_ARG, "bad arguments");
+    std::vector<DevTT*> ps, fs;
+    std::vector<DevTTO*> Hs;
+    for (int64_t i = 0; i < n; ++i) {
+      ps.push_back(h->get(psi_ids[i]));
+      Hs.push_back(h->get_op(op_ids[i]));
+      fs.push_back(h->get(phi_ids[i]));
+      TTR_REQUIRE(ps.back()->dims == fs.back()->dims && Hs.back()->dims == fs.back()->dims, ST_DIM_MISMATCH,
+                  "TT dimensions are not compatible");
+    }
+    double* res = h->ctx.ws.alloc_n<double>(size_t(n));
+    dot_operator_batch(h, ps, Hs, fs, res);
+    h->ctx.sync();
+    TTR_HIP(hipMemcpy(out, res, size_t(n) * 8, hipMemcpyDeviceToHost));
   });
 }
 

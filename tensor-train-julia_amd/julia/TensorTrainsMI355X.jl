@@ -5,7 +5,8 @@ Julia binding of libttround (include/ttround.h): the MI355X rounding path of
 TensorTrains behind `ccall`.  Loading this module after `TensorTrains` adds
 GPU methods with the reference's names and keyword defaults
 (src/tt_rounding.jl:146-167, src/tt_randtools.jl:112-283,
-src/sketches.jl:175-315, src/tt_operations.jl:81-112,170-194,252-258); the
+src/sketches.jl:175-315,1047-1070, src/tt_operations.jl:81-112,170-194,252-258,
+302-421); the
 solver callers (tt_gmres, arnoldi, davidson, als/mals/dmrg via orthogonalize)
 need no change.  Every call uploads the cores (column-major (n, r, r') arrays,
 exactly TTvector.ttv_vec), runs on the GPU and copies the result back into new
@@ -19,7 +20,9 @@ others -> ErrorException with ttr_last_error.
 module TensorTrainsMI355X
 
 using TensorTrains
-import TensorTrains: tt_rounding, tt_rounding!, orthogonalize, ttrand_rounding, tt_recursive_sketch
+import TensorTrains: tt_rounding, tt_rounding!, orthogonalize, ttrand_rounding, tt_recursive_sketch,
+                     dot_par, dot_operator, partial_contraction
+import LinearAlgebra: dot
 
 const lib = get(ENV, "TTROUND_LIB", joinpath(@__DIR__, "..", "ttround", "libttround.so"))
 
@@ -95,6 +98,41 @@ function orthogonalize(x::TTvector{Float64,N}; i=1::Int) where {N}
                 handle(), 1, id, i, out))
     free(id)
     return download(out[], Val(N))
+end
+
+# ---------------------------------------------------------------- inner products
+
+# dot(A, B) (src/tt_operations.jl:302-321); dot_par (:339-370) is the same number
+function dot(A::TTvector{Float64,N}, B::TTvector{Float64,N}) where {N}
+    A.ttv_dims == B.ttv_dims || throw(DimensionMismatch("TT dimensions are not compatible"))
+    a, b = upload(A), upload(B); out = Ref{Float64}(0.0)
+    check(ccall((:ttr_dot, lib), Cint, (Ptr{Cvoid}, Int64, Ref{Int64}, Ref{Int64}, Ref{Float64}),
+                handle(), 1, a, b, out))
+    free(a); free(b)
+    return out[]
+end
+dot_par(A::TTvector{Float64,N}, B::TTvector{Float64,N}) where {N} = dot(A, B)
+
+# partial_contraction(A, B; reverse) (src/sketches.jl:1047-1070)
+function partial_contraction(A::TTvector{Float64,N}, B::TTvector{Float64,N}; reverse=true) where {N}
+    A.ttv_dims == B.ttv_dims || throw(DimensionMismatch("TT dimensions are not compatible"))
+    W = [zeros(Float64, A.ttv_rks[i], B.ttv_rks[i]) for i in 1:N+1]
+    a, b = upload(A), upload(B)
+    GC.@preserve W begin
+        check(ccall((:ttr_partial_contraction, lib), Cint, (Ptr{Cvoid}, Int64, Int64, Cint, Ptr{Ptr{Float64}}),
+                    handle(), a, b, reverse ? 1 : 0, [pointer(w) for w in W]))
+    end
+    free(a); free(b)
+    return W
+end
+
+# dot_operator(psi, H, phi) = <psi|H|phi> (src/tt_operations.jl:385-421)
+function dot_operator(ψ::TTvector{Float64,N}, H::TToperator{Float64,N}, φ::TTvector{Float64,N}) where {N}
+    a, op, b = upload(ψ), upload(H), upload(φ); out = Ref{Float64}(0.0)
+    check(ccall((:ttr_dot_operator, lib), Cint,
+                (Ptr{Cvoid}, Int64, Ref{Int64}, Ref{Int64}, Ref{Int64}, Ref{Float64}), handle(), 1, a, op, b, out))
+    free(a); free(op); free(b)
+    return out[]
 end
 
 # ---------------------------------------------------------------- randomized

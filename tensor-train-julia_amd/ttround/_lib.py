@@ -72,6 +72,9 @@ SIGNATURES = {
     "ttr_orthogonalize": (C.c_int, [C.c_void_p, i64, i64p, i64, i64p]),
     "ttr_round_svd": (C.c_int, [C.c_void_p, i64, i64p, C.c_double, i64, C.c_int, i64p]),
     "ttr_norm": (C.c_int, [C.c_void_p, i64, i64p, f64p]),
+    "ttr_dot": (C.c_int, [C.c_void_p, i64, i64p, i64p, f64p]),
+    "ttr_partial_contraction": (C.c_int, [C.c_void_p, i64, i64, C.c_int, f64pp]),
+    "ttr_dot_operator": (C.c_int, [C.c_void_p, i64, i64p, i64p, i64p, f64p]),
     "ttr_round_rand": (C.c_int, [C.c_void_p, i64, C.POINTER(RandItem), C.POINTER(RandOpts), i64p]),
     "ttr_sketch": (C.c_int, [C.c_void_p, i64, i64p, C.POINTER(RandOpts), f64pp, i64p]),
     "ttr_sketch_op": (C.c_int, [C.c_void_p, i64, i64, i64p, C.POINTER(RandOpts), f64pp, i64p]),

@@ -197,6 +197,32 @@ def norm_batch(xs: Sequence[DeviceTT]) -> np.ndarray:
     return out
 
 
+def dot_batch(xs: Sequence[DeviceTT], ys: Sequence[DeviceTT]) -> np.ndarray:
+    """``<xs[i], ys[i]>`` for every pair, all pairs in the same launches
+    (``dot``, ``src/tt_operations.jl:302-321``)."""
+    if len(xs) != len(ys) or not xs:
+        raise ValueError("xs and ys must be non-empty and of equal length")
+    h = xs[0].handle
+    a, b = L.as_i64([x.id for x in xs]), L.as_i64([y.id for y in ys])
+    out = np.zeros(len(xs))
+    h.check(h.lib.ttr_dot(h.h, len(xs), L.ptr_i64(a), L.ptr_i64(b), L.ptr_f64(out)))
+    return out
+
+
+def dot_operator_batch(psis: Sequence[DeviceTT], ops, phis: Sequence[DeviceTT]) -> np.ndarray:
+    """``<psis[i]|ops[i]|phis[i]>`` for every triple (``dot_operator``,
+    ``src/tt_operations.jl:385-421``); ``ops`` is one DeviceTTO or one per triple."""
+    if isinstance(ops, DeviceTTO):
+        ops = [ops] * len(psis)
+    if not (len(psis) == len(ops) == len(phis)) or not psis:
+        raise ValueError("psis, ops and phis must be non-empty and of equal length")
+    h = psis[0].handle
+    a, o, b = L.as_i64([x.id for x in psis]), L.as_i64([A.id for A in ops]), L.as_i64([y.id for y in phis])
+    out = np.zeros(len(psis))
+    h.check(h.lib.ttr_dot_operator(h.h, len(psis), L.ptr_i64(a), L.ptr_i64(o), L.ptr_i64(b), L.ptr_f64(out)))
+    return out
+
+
 def round_rand_batch(items, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
                      sketch_prec: str = "f64", p=None, oversampling: float = 1.0) -> List[DeviceTT]:
     """Batched randomized rounding of linear combinations.
@@ -389,6 +415,51 @@ def norm(x: TTvector) -> float:
     v = dev.norm()
     dev.free()
     return v
+
+
+def dot(A: TTvector, B: TTvector) -> float:
+    """``dot(A, B)`` (``src/tt_operations.jl:302-321``)."""
+    if A.ttv_dims != B.ttv_dims:
+        raise L.DimensionMismatch("TT dimensions are not compatible")
+    a, b = A._dev(), B._dev()
+    v = float(dot_batch([a], [b])[0])
+    a.free()
+    b.free()
+    return v
+
+
+def dot_par(A: TTvector, B: TTvector) -> float:
+    """``dot_par(A, B)`` (``src/tt_operations.jl:339-370``): the same inner
+    product; the device evaluates every core's contraction in parallel already."""
+    return dot(A, B)
+
+
+def dot_operator(psi: TTvector, H: TToperator, phi: TTvector) -> float:
+    """``dot_operator(psi, H, phi)`` = ``<psi|H|phi>`` without forming ``H phi``
+    (``src/tt_operations.jl:385-421``)."""
+    if psi.ttv_dims != phi.ttv_dims or tuple(H.tto_dims) != tuple(phi.ttv_dims):
+        raise L.DimensionMismatch("TT dimensions are not compatible")
+    a, b = psi._dev(), phi._dev()
+    op = DeviceTTO(H, a.handle)
+    v = float(dot_operator_batch([a], [op], [b])[0])
+    for t in (a, b, op):
+        t.free()
+    return v
+
+
+def partial_contraction(A: TTvector, B: TTvector, reverse: bool = True) -> List[np.ndarray]:
+    """``partial_contraction(A, B; reverse)`` (``src/sketches.jl:1047-1070``):
+    ``W[k]`` of shape ``(A.ttv_rks[k], B.ttv_rks[k])`` for ``k = 0..N``."""
+    if A.ttv_dims != B.ttv_dims:
+        raise L.DimensionMismatch("TT dimensions are not compatible")
+    a, b = A._dev(), B._dev()
+    h = a.handle
+    Ws = [np.zeros((A.ttv_rks[k], B.ttv_rks[k]), order="F") for k in range(A.N + 1)]
+    arr = (L.f64p * (A.N + 1))(*[L.ptr_f64(w) for w in Ws])
+    h.check(h.lib.ttr_partial_contraction(h.h, a.id, b.id, 1 if reverse else 0, arr))
+    a.free()
+    b.free()
+    return Ws
 
 
 def tt_rounding(x, tol: float = 1e-12, rmax: int = 2 ** 14, direction: str = "left"):

@@ -397,3 +397,137 @@ def test_round_rand_batch_uneven(T):
         assert O.rel_err(ref, y) < 1e-12
     for d in devs + outs:
         d.free()
+
+
+# ------------------------------------------------------------------ inner products
+
+def test_dot_batch_vs_oracle(T):
+    """dot (tt_operations.jl:302-321) for pairs of different lengths, dims and ranks
+    in one call; |got - ref| <= 1e-12 |x| |y| (FP64, the sum may cancel)."""
+    say("case: dot batch, mixed d")
+    cases = [((3, 2, 4, 2, 3), [1, 3, 5, 4, 2, 1], [1, 2, 6, 3, 3, 1]),
+             ((7,), [1, 1], [1, 1]),
+             ((4,) * 8, [1, 4, 9, 17, 23, 17, 9, 4, 1], [1, 3, 7, 12, 30, 12, 7, 3, 1]),
+             ((2, 5), [1, 2, 1], [1, 2, 1])]
+    xs, ys, refs, bounds, devs = [], [], [], [], []
+    for i, (dims, ra, rb) in enumerate(cases):
+        x, y = rand_tt(dims, ra, 200 + i), rand_tt(dims, rb, 300 + i)
+        dx, dy = T.DeviceTT.upload(to_api(x)), T.DeviceTT.upload(to_api(y))
+        xs.append(dx)
+        ys.append(dy)
+        devs += [dx, dy]
+        refs.append(O.dot(x, y))
+        bounds.append(1e-12 * O.tt_norm(x) * O.tt_norm(y))
+    got = T.dot_batch(xs, ys)
+    for g, r, b in zip(got, refs, bounds):
+        assert abs(g - r) <= b, (g, r)
+    # the same pairs in the other order give the same numbers
+    got2 = T.dot_batch(ys, xs)
+    for g, r, b in zip(got2, refs, bounds):
+        assert abs(g - r) <= b
+    for d in devs:
+        d.free()
+
+
+def test_dot_large_ranks_equals_norm_squared(T):
+    """At ranks to 200 the oracle is slow: <x, x> against |x|^2 from ttr_norm
+    and <x, y> against the oracle once."""
+    say("case: dot large ranks")
+    dims = (4,) * 10
+    x = rand_tt(dims, [1, 4, 16, 64, 200, 200, 200, 64, 16, 4, 1], 401)
+    y = rand_tt(dims, [1, 4, 16, 50, 120, 150, 120, 50, 16, 4, 1], 402)
+    dx, dy = T.DeviceTT.upload(to_api(x)), T.DeviceTT.upload(to_api(y))
+    g = T.dot_batch([dx, dx, dx], [dx, dy, dx])
+    n2 = T.norm_batch([dx])[0] ** 2
+    assert abs(g[0] - n2) <= 1e-12 * n2
+    assert g[0] == g[2]
+    nx, ny = O.tt_norm(x), O.tt_norm(y)
+    assert abs(g[1] - O.dot(x, y)) <= 1e-12 * nx * ny
+    assert abs(T.dot(to_api(x), to_api(y)) - g[1]) <= 1e-13 * nx * ny
+    dx.free()
+    dy.free()
+
+
+@pytest.mark.parametrize("reverse", [True, False])
+def test_partial_contraction_vs_oracle(T, reverse):
+    say(f"case: partial_contraction reverse={reverse}")
+    dims = (2, 3, 2, 4, 3)
+    x, y = rand_tt(dims, [1, 2, 5, 7, 3, 1], 411), rand_tt(dims, [1, 3, 4, 6, 2, 1], 412)
+    W = T.partial_contraction(to_api(x), to_api(y), reverse=reverse)
+    Wr = O.partial_contraction(x, y, reverse=reverse)
+    scale = O.tt_norm(x) * O.tt_norm(y)
+    for a, b in zip(W, Wr):
+        assert a.shape == b.shape
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-12 * max(1.0, np.abs(b).max()))
+    end = W[0] if reverse else W[-1]
+    assert abs(end[0, 0] - O.dot(x, y)) <= 1e-12 * scale
+
+
+def wide_operator(dims, R, seed):
+    rng = np.random.default_rng(seed)
+    Hr = [1] + [R] * (len(dims) - 1) + [1]
+    return O.TTO([rng.standard_normal((n, n, Hr[k], Hr[k + 1])) / np.sqrt(n * Hr[k])
+                  for k, n in enumerate(dims)], dims, Hr)
+
+
+def test_dot_operator_vs_oracle(T):
+    """<psi|H|phi> (tt_operations.jl:385-421): Hubbard (n R = 24, LDS-staged
+    local action) and a random operator with R = 20 (n R = 80, GEMM against the
+    permuted core) in one batch, against the oracle and against dot(psi, H phi)."""
+    say("case: dot_operator batch (Hubbard + wide operator)")
+    Hh = O.hubbard_1d_mpo(6, 1.0, 4.0)
+    Hw = wide_operator((4,) * 6, 20, 421)
+    trip, refs, bounds, devs = [], [], [], []
+    dh, dw = T.DeviceTTO(to_api_op(Hh)), T.DeviceTTO(to_api_op(Hw))
+    for i, (H, dH) in enumerate(((Hh, dh), (Hw, dw), (Hh, dh))):
+        psi = rand_tt(H.dims, [1, 4, 9, 12, 9, 4, 1], 430 + i)
+        phi = rand_tt(H.dims, [1, 3, 7, 10, 8, 4, 1], 440 + i)
+        if i == 2:
+            psi = O.orthogonalize(psi, 3)  # centre in the middle: the oracle pivots there
+        dp, df = T.DeviceTT.upload(to_api(psi)), T.DeviceTT.upload(to_api(phi))
+        devs += [dp, df]
+        trip.append((dp, dH, df))
+        refs.append(O.dot_operator(psi, H, phi))
+        bounds.append(1e-12 * O.tt_norm(psi) * O.tt_norm(O.mpo_apply(H, phi)))
+    got = T.dot_operator_batch([t[0] for t in trip], [t[1] for t in trip], [t[2] for t in trip])
+    for g, r, b in zip(got, refs, bounds):
+        say(f"  got {g:.15e} ref {r:.15e}")
+        assert abs(g - r) <= b
+    # implicit form equals the explicit product H phi
+    hp = T.mpo_apply_device(dw, trip[1][2])
+    assert abs(T.dot_batch([trip[1][0]], [hp])[0] - got[1]) <= bounds[1]
+    hp.free()
+    for d in devs + [dh, dw]:
+        d.free()
+
+
+def test_dot_operator_edge_cases(T):
+    """d = 1, a rank-one operator (identity: <psi|I|phi> = dot) and the
+    Hubbard energy of a normalised state against the dense Hamiltonian."""
+    say("case: dot_operator edge cases")
+    H1 = O.TTO([np.random.default_rng(5).standard_normal((5, 5, 1, 1))], (5,), [1, 1])
+    a, b = rand_tt((5,), [1, 1], 451), rand_tt((5,), [1, 1], 452)
+    ref = float(a.cores[0][:, 0, 0] @ H1.cores[0][:, :, 0, 0] @ b.cores[0][:, 0, 0])
+    assert abs(T.dot_operator(to_api(a), to_api_op(H1), to_api(b)) - ref) <= 1e-13 * abs(ref) + 1e-15
+    dims = (3, 4, 2, 3)
+    Id = O.TTO([np.eye(n).reshape(n, n, 1, 1) for n in dims], dims, [1] * 5)
+    x, y = rand_tt(dims, [1, 3, 5, 2, 1], 453), rand_tt(dims, [1, 2, 4, 3, 1], 454)
+    assert abs(T.dot_operator(to_api(x), to_api_op(Id), to_api(y)) - O.dot(x, y)) <= 1e-12 * O.tt_norm(x) * O.tt_norm(y)
+    Hh = O.hubbard_1d_mpo(4, 1.0, 4.0)
+    v = rand_tt(Hh.dims, [1, 4, 8, 4, 1], 455)
+    vd = O.ttv_to_tensor(v).reshape(-1, order="F")
+    e_dense = float(vd @ O.hubbard_1d_dense(4, 1.0, 4.0) @ vd)
+    e = T.dot_operator(to_api(v), to_api_op(Hh), to_api(v))
+    assert abs(e - e_dense) <= 1e-12 * abs(e_dense)
+
+
+def test_inner_product_errors(T):
+    say("case: inner product argument errors")
+    x, y = rand_tt((2, 3), [1, 2, 1], 461), rand_tt((3, 2), [1, 2, 1], 462)
+    dx, dy = T.DeviceTT.upload(to_api(x)), T.DeviceTT.upload(to_api(y))
+    with pytest.raises(T.DimensionMismatch):
+        T.dot_batch([dx], [dy])
+    with pytest.raises(T.DimensionMismatch):
+        T.dot(to_api(x), to_api(y))
+    dx.free()
+    dy.free()

@@ -14,6 +14,9 @@ a wrong index or a transposed operand fails:
   tt_recursive_sketch -> dense X_{k:N} Omega_{k:N} brute force; p=1 full sketch = dot
   ttrand_rounding*    -> exact recovery when the sketch rank >= true rank (test_ttrand.jl:67-88)
   hubbard_1d_mpo      -> dense fermionic Hamiltonian
+  dot, dot_par, partial_contraction, dot_operator
+                      -> dense inner products, dense left/right interface
+                         contractions, psi^T H phi with the dense operator
   Philox              -> Random123 known-answer vectors
 """
 import math
@@ -119,6 +122,73 @@ def test_hubbard_mpo_dense():
         H = O.hubbard_1d_mpo(L, 1.0, 4.0)
         assert H.rks == [1] + [6] * (L - 1) + [1]
         np.testing.assert_allclose(O.tto_to_tensor(H), O.hubbard_1d_dense(L, 1.0, 4.0), atol=1e-13)
+
+
+# ---------------------------------------------------------------- inner products
+
+def _left_part(x, k):
+    """Dense (n_1...n_k, r_k) left interface matrix of cores 1..k."""
+    acc = np.ones((1, 1))
+    for j in range(k):
+        c = x.cores[j]
+        acc = np.einsum("ma,iab->mib", acc, c).reshape(-1, c.shape[2], order="F")
+    return acc
+
+
+def _right_part(x, k):
+    """Dense (r_k, n_{k+1}...n_N) right interface matrix of cores k+1..N (0-based k)."""
+    acc = np.ones((1, 1))
+    for j in range(x.N - 1, k - 1, -1):
+        c = x.cores[j]
+        acc = np.einsum("iab,bm->aim", c, acc).reshape(c.shape[1], -1, order="F")
+    return acc
+
+
+def test_dot_and_dot_par_dense():
+    dims = (3, 2, 4, 2, 3)
+    x, y = rand_tt(dims, [1, 3, 5, 4, 2, 1], 21), rand_tt(dims, [1, 2, 6, 3, 3, 1], 22)
+    ref = float(np.vdot(O.ttv_to_tensor(x), O.ttv_to_tensor(y)))
+    for f in (O.dot, O.dot_par, O.tt_dot):
+        assert abs(f(x, y) - ref) <= 1e-12 * abs(ref)
+    x1, y1 = rand_tt((7,), [1, 1], 23), rand_tt((7,), [1, 1], 24)
+    assert abs(O.dot(x1, y1) - float(x1.cores[0].ravel() @ y1.cores[0].ravel())) < 1e-13
+
+
+@pytest.mark.parametrize("reverse", [True, False])
+def test_partial_contraction_dense(reverse):
+    dims = (2, 3, 2, 4)
+    x, y = rand_tt(dims, [1, 2, 5, 3, 1], 31), rand_tt(dims, [1, 3, 4, 2, 1], 32)
+    W = O.partial_contraction(x, y, reverse=reverse)
+    assert [w.shape for w in W] == [(x.rks[k], y.rks[k]) for k in range(5)]
+    for k in range(5):
+        if reverse:
+            ref = _right_part(x, k) @ _right_part(y, k).T
+        else:
+            ref = _left_part(x, k).T @ _left_part(y, k)
+        np.testing.assert_allclose(W[k], ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("ots", [(None, None), ((1, 0, -1, -1), None), (None, (0, -1, -1, -1)),
+                                 ((1, 1, 1, 0), (1, 0, -1, -1)), ((1, 1, 0, -1), (1, 1, 1, 0))])
+def test_dot_operator_dense(ots):
+    dims = (2, 3, 2, 2)
+    psi, phi = rand_tt(dims, [1, 2, 4, 3, 1], 41), rand_tt(dims, [1, 3, 3, 2, 1], 42)
+    # the pivot only changes the contraction order, so any ot labels are valid here
+    if ots[0] is not None:
+        psi.ot = list(ots[0])
+    if ots[1] is not None:
+        phi.ot = list(ots[1])
+    Hr = [1, 3, 2, 4, 1]
+    H = O.TTO([RNG.standard_normal((n, n, Hr[k], Hr[k + 1])) for k, n in enumerate(dims)], dims, Hr)
+    ref = float(O.ttv_to_tensor(psi).reshape(-1, order="F") @ O.tto_to_tensor(H)
+                @ O.ttv_to_tensor(phi).reshape(-1, order="F"))
+    assert abs(O.dot_operator(psi, H, phi) - ref) <= 1e-12 * abs(ref)
+    # Hubbard energy of a state against the dense Hamiltonian
+    Hh = O.hubbard_1d_mpo(3, 1.0, 4.0)
+    v = rand_tt((4, 4, 4), [1, 4, 4, 1], 43)
+    vd = O.ttv_to_tensor(v).reshape(-1, order="F")
+    e = O.dot_operator(v, Hh, v)
+    assert abs(e - vd @ O.hubbard_1d_dense(3, 1.0, 4.0) @ vd) <= 1e-12 * abs(e)
 
 
 # ---------------------------------------------------------------- deterministic rounding
