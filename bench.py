@@ -16,6 +16,9 @@ Other workloads (one JSON line each, same fields):
   c4  1D Hubbard MPO (L=32, local dim 4, t=1, U=4, bond dim 6) applied to 64
       N(0,1) TTs of rank 128 without forming the products; BF16 sketch of rank
       144 (block rank 144), cleanup to rmax 128.
+  dot    one Gram-Schmidt pass: 32 inner products of rank-128 TTs (d=40, n=4)
+         with one rank-128 TT, FP64.
+  dotop  16 Hubbard energies <psi|H|psi> (L=32, rank-128 states), FP64.
 
 Prints one JSON line.  Besides the metric it reports the FP64 deterministic SVD
 rounding of the same inputs (materialised sums / products, on a subset where
@@ -62,6 +65,16 @@ WORKLOADS = {
                block_rks=144, rks=144, cleanup=dict(rmax=128), op=True, det_items=16,
                text="1D Hubbard MPO (L=32, t=1, U=4, bond dim 6) times 64 N(0,1) TTs d=32 n=4 r=128, "
                     "implicit product; sketch rank 144 (block_rks=144); SVD cleanup rmax=128"),
+}
+
+# inner-product workloads: one step evaluates the whole batch
+INNER = {
+    "dot": dict(pairs=32, d=40, n=4, r=128, seed_base=5000,
+                text="Gram-Schmidt pass: <V_i, w> for a basis of 32 N(0,1) TTs d=40 n=4 r=128 against one TT "
+                     "r=128 (dot, FP64)"),
+    "dotop": dict(pairs=16, d=32, n=4, r=128, seed_base=6000,
+                  text="energies <psi_i|H|psi_i>: 1D Hubbard MPO (L=32, t=1, U=4, bond dim 6), 16 N(0,1) TTs "
+                       "d=32 n=4 r=128 (dot_operator, FP64, H psi never formed)"),
 }
 
 
@@ -237,12 +250,85 @@ def summarize(stats, args):
     return jacobi, sites, phases, roof, kernels
 
 
+def inner_main(args):
+    """Inner-product workloads: value = inner products per second."""
+    import oracle as O
+    import ttround as T
+    wl = INNER[args.workload]
+    h = T.default_handle()
+    d, n, r = wl["d"], wl["n"], wl["r"]
+    dims = (n,) * d
+    m = args.items or wl["pairs"]
+    op = T.DeviceTTO(T.hubbard_1d_mpo(d, 1.0, 4.0)) if args.workload == "dotop" else None
+    xs = [T.DeviceTT.randn(dims, r, seed=wl["seed_base"] + i) for i in range(m)]
+    if op is None:
+        w = T.DeviceTT.randn(dims, r, seed=wl["seed_base"] + 99999)
+        ys = [w] * m
+
+        def step():
+            return T.dot_batch(xs, ys)
+    else:
+        ys = xs
+
+        def step():
+            return T.dot_operator_batch(xs, op, xs)
+    log(f"[bench] workload {args.workload}: {m} inner products, d={d}, n={n}, r={r}")
+    for _ in range(args.warmup):
+        step()
+    h.profile(not args.no_events)
+    h.profile_reset()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        got = step()
+        log(f"[bench] step {s} done")
+    dt = time.perf_counter() - t0
+    h.profile(False)
+    args.steps = max(args.steps, 1)
+    _, sites, _, roof, kernels = summarize(h.profile_stats(), args)
+    value = m * args.steps / dt
+    rel, cpu = None, None
+    if args.cpu_items > 0:
+        args.cpu_items = m  # the whole batch: the oracle takes seconds at these sizes
+        H = O.hubbard_1d_mpo(d, 1.0, 4.0) if op is not None else None
+        host = [O.TT([t.core(k) for k in range(d)], dims, t.rks, t.ot) for t in xs[:args.cpu_items]]
+        wy = O.TT([ys[0].core(k) for k in range(d)], dims, ys[0].rks, ys[0].ot)
+        tc = time.perf_counter()
+        refs = [O.dot(x, wy) if H is None else O.dot_operator(x, H, x) for x in host]
+        tcpu = time.perf_counter() - tc
+        if H is None:
+            # relative to |x| |w|: a product of independent draws cancels
+            nx, nw = T.norm_batch(xs[:args.cpu_items]), T.norm_batch([ys[0]])[0]
+            rel = max(abs(g - f) / (a * nw) for g, f, a in zip(got, refs, nx))
+        else:
+            rel = max(abs(g - f) / abs(f) for g, f in zip(got, refs))
+        log(f"[bench] {args.cpu_items} oracle products in {tcpu:.2f}s, max rel err {rel:.3e}")
+        cpu = {"value": round(args.cpu_items / tcpu, 4), "unit": "inner products/s", "cores": os.cpu_count(),
+               "kind": "port",
+               "sample": f"{args.cpu_items} of the batch's products, FP64 numpy/OpenBLAS oracle "
+                         f"({'dot' if H is None else 'dot_operator'}), "
+                         f"BLAS threads = {os.environ.get('OPENBLAS_NUM_THREADS', 'default')}"}
+    res = {
+        "metric": "TT inner products/sec" if op is None else "TT operator expectation values/sec",
+        "value": round(value, 4), "unit": "inner products/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded N(0,1) TT cores drawn on the device)",
+        "config": {"workload": wl["text"], "name": args.workload, "items": m, "d": d, "n": n, "r": r},
+        "rel_err_vs_oracle": rel, "kernels": kernels, "gemm_sites": sites, "roofline": roof, "cpu_baseline": cpu,
+    }
+    for t in xs + ([ys[0]] if op is None else []):
+        t.free()
+    if op is not None:
+        op.free()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="c3")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(INNER), default="c3")
     ap.add_argument("--items", type=int, default=None, help="batch size (default: the workload's)")
     ap.add_argument("--sketch-prec", default=None, help="f64 | f32 | bf16 (default: the workload's)")
     ap.add_argument("--no-cleanup", action="store_true", help="skip the SVD cleanup after the randomized rounding")
@@ -260,6 +346,9 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload in INNER:
+        inner_main(args)
+        return
     wl = WORKLOADS[args.workload]
     args.items = args.items or wl["items"]
     args.sketch_prec = args.sketch_prec or wl["sketch"]

@@ -306,14 +306,21 @@ def dot_operator(psi: TT, H: TTO, phi: TT) -> float:
         return N
 
     c = max(center(psi), center(phi))
+    # each environment update L[a,b,c] = L[x,y,z] A[i,x,a] H[i,j,y,b] B[j,z,c]
+    # (and its mirror) as three pairwise BLAS contractions
     Lenv = np.ones((1, 1, 1))
     for k in range(c if c > 1 else 0):
-        Lenv = np.einsum("xyz,ixa,ijyb,jzc->abc", Lenv, psi.cores[k], H.cores[k], phi.cores[k], optimize=True)
+        A, Hk, B = psi.cores[k], H.cores[k], phi.cores[k]
+        T1 = np.tensordot(Lenv, B, axes=([2], [1]))          # (x, y, j, c)
+        T2 = np.tensordot(T1, Hk, axes=([1, 2], [2, 1]))     # (x, c, i, b)
+        Lenv = np.tensordot(T2, A, axes=([0, 2], [1, 0])).transpose(2, 1, 0)  # (a, b, c)
     Renv = np.ones((1, 1, 1))
     r = c + 1 if c > 1 else 1
     for k in range(N, r - 1, -1):
-        Renv = np.einsum("ixa,ijyb,jzc,abc->xyz", psi.cores[k - 1], H.cores[k - 1], phi.cores[k - 1], Renv,
-                         optimize=True)
+        A, Hk, B = psi.cores[k - 1], H.cores[k - 1], phi.cores[k - 1]
+        T1 = np.tensordot(B, Renv, axes=([2], [2]))          # (j, z, a, b)
+        T2 = np.tensordot(T1, Hk, axes=([0, 3], [1, 3]))     # (z, a, i, y)
+        Renv = np.tensordot(A, T2, axes=([0, 2], [2, 1])).transpose(0, 2, 1)  # (x, y, z)
     if c == 1:
         return float(Renv[0, 0, 0])
     if c == N:

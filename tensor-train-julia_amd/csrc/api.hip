@@ -1123,9 +1123,9 @@ int ttr_destroy(ttr_handle_t h) {
   (void)hipStreamSynchronize(h->ctx.stream);
   for (auto& kv : h->tts) h->free_tt(kv.second.get());
   h->tts.clear();
-  for (auto& kv : h->ops) (void)hipFree(kv.second->base);
-  h->ops.clear();
+  h->ops.clear();  // DevTTO frees its device copies
   (void)hipFree(h->one);
+  if (h->ctx.splitk) (void)hipFree(h->ctx.splitk);
   h->ctx.ws.release();
   h->ctx.pin.release();
   h->ctx.cache.release_cached();

@@ -90,10 +90,14 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
 
   const int bid = blockIdx.x;
   const GemmProb P = probs[find_prob(probs, nprobs, bid)];
-  const int t = bid - P.tile0;
+  const int t0 = bid - P.tile0;
+  const int chunk = t0 / P.tiles_mn, t = t0 - chunk * P.tiles_mn;
   const int row0 = (t / P.tiles_n) * TM, col0 = (t % P.tiles_n) * TN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
+  // k-tiles [kt_lo, kt_hi) of the problem's segment sequence belong to this workgroup
+  const int kt_lo = P.ksplit > 1 ? chunk * P.kc : 0;
+  const int kt_hi = P.ksplit > 1 ? kt_lo + P.kc : (1 << 30);
 
   acc_t acc[MI][NI];
 #pragma unroll
@@ -105,8 +109,14 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
   for (int s = 0; s < P.nseg; ++s)
     if (segs[P.seg0 + s].exp) emax = max(emax, segs[P.seg0 + s].exp[0]);
 
+  int kt_seg = 0;  // first k-tile of the segment in the problem's sequence
   for (int s = 0; s < P.nseg; ++s) {
     const GemmSeg S = segs[P.seg0 + s];
+    const int nkt = (S.K + BK - 1) / BK;
+    const int kb = max(kt_lo - kt_seg, 0) * BK, ke = min(kt_hi - kt_seg, nkt) * BK;
+    kt_seg += nkt;
+    if (kb >= ke) continue;
+    const int kend = min(ke, S.K);
     double scale = S.scale;
     if (S.exp) scale = ldexp(scale, S.exp[0] - emax);
     const bool a_row_fast = (S.a_rs == 1);
@@ -158,8 +168,8 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
         }
       }
     };
-    load_tiles(0);
-    for (int k0 = 0; k0 < S.K; k0 += BK) {
+    load_tiles(kb);
+    for (int k0 = kb; k0 < kend; k0 += BK) {
 #pragma unroll
       for (int q = 0; q < NA; ++q) {
         int i, l;
@@ -175,7 +185,7 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
         Bs[l][j] = MT(to_double(rb[q]));
       }
       __syncthreads();
-      if (k0 + BK < S.K) load_tiles(k0 + BK);
+      if (k0 + BK < kend) load_tiles(k0 + BK);
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         MT av[MI], bv[NI];
@@ -208,12 +218,34 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
         const int64_t gi = row0 + wm * (TM / 2) + bi * 16 + rr;
         const int64_t gj = col0 + wn * (TN / 2) + bj * 16 + (lane & 15);
         if (gi < P.M && gj < P.N) {
+          if (P.ksplit > 1) {
+            P.part[(int64_t(chunk) * P.N + gj) * P.M + gi] = double(acc[bi][bj][r]);
+            continue;
+          }
           const int64_t ci = gi * P.c_rs + gj * P.c_cs;
           double v = P.alpha * double(acc[bi][bj][r]);
           if (P.beta != 0.0) v += P.beta * load_d<TC>(P.C, ci);
           store_c<TC>(P.C, ci, v);
         }
       }
+}
+
+// C = alpha * (sum of the chunks in chunk order) + beta * C for the split problems
+// (blockIdx.y indexes the list of split problems)
+template <class TC>
+__global__ __launch_bounds__(256) void gemm_splitk_reduce(const GemmProb* __restrict__ probs,
+                                                          const int32_t* __restrict__ split_ids) {
+  const GemmProb P = probs[split_ids[blockIdx.y]];
+  const int64_t mn = int64_t(P.M) * P.N;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < mn; e += int64_t(gridDim.x) * blockDim.x) {
+    double v = 0.0;
+    for (int c = 0; c < P.ksplit; ++c) v += P.part[c * mn + e];
+    const int64_t gi = e % P.M, gj = e / P.M;
+    const int64_t ci = gi * P.c_rs + gj * P.c_cs;
+    v *= P.alpha;
+    if (P.beta != 0.0) v += P.beta * load_d<TC>(P.C, ci);
+    store_c<TC>(P.C, ci, v);
+  }
 }
 
 // ------------------------------------------------------------- BF16 math
@@ -419,6 +451,51 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
   } else {
     cfg = 0;
   }
+  // Split K when the batch has too few 64 x 64 output tiles to fill the chip.
+  // The split depends on the shapes only (not on the tile configuration), so
+  // every configuration the tuner tries still writes the same C.
+  std::vector<int32_t> split_ids;
+  for (auto& p : probs) {
+    p.ksplit = 1;
+    p.kc = 0;
+    p.part = nullptr;
+  }
+  if (math != Prec::BF16) {
+    int64_t t64 = 0;
+    for (const auto& p : probs) t64 += int64_t((p.M + 63) / 64) * ((p.N + 63) / 64);
+    if (t64 < 512) {
+      int want = int(std::min<int64_t>(16, (1024 + t64 - 1) / t64));
+      for (;; want /= 2) {
+        int64_t elems = 0;
+        for (auto& p : probs) {
+          int kt = 0;
+          for (int q = 0; q < p.nseg; ++q) kt += (segs[p.seg0 + q].K + 15) / 16;
+          int sp = std::min(want, kt / 8);  // chunks of at least 8 k-tiles
+          if (sp <= 1) { p.ksplit = 1; continue; }
+          p.kc = (kt + sp - 1) / sp;
+          p.ksplit = (kt + p.kc - 1) / p.kc;
+          elems += int64_t(p.ksplit) * p.M * p.N;
+        }
+        if (elems <= (int64_t(1) << 25) || want <= 2) {
+          if (elems > (int64_t(1) << 25))
+            for (auto& p : probs) p.ksplit = 1;
+          else if (elems > 0) {
+            double* base = ctx.splitk_buffer(size_t(elems));
+            int64_t off = 0;
+            for (size_t i = 0; i < probs.size(); ++i) {
+              auto& p = probs[i];
+              if (p.ksplit <= 1) continue;
+              p.part = base + off;
+              off += int64_t(p.ksplit) * p.M * p.N;
+              split_ids.push_back(int32_t(i));
+              bytes += 16.0 * p.ksplit * p.M * p.N;
+            }
+          }
+          break;
+        }
+      }
+    }
+  }
   auto prepare = [&](int c, int64_t& ntiles) -> const GemmProb* {
     const int TM = math == Prec::BF16 ? BM : CFG_TM[c], TN = math == Prec::BF16 ? BN : CFG_TN[c];
     ntiles = 0;
@@ -426,10 +503,22 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
       const int tm = (p.M + TM - 1) / TM, tn = (p.N + TN - 1) / TN;
       p.tile0 = int32_t(ntiles);
       p.tiles_n = tn;
-      ntiles += int64_t(tm) * tn;
+      p.tiles_mn = tm * tn;
+      ntiles += int64_t(tm) * tn * p.ksplit;
     }
     TTR_REQUIRE(ntiles < (int64_t(1) << 31), ST_INVALID_ARG, "GEMM batch too large");
     return ctx.upload(probs);
+  };
+  auto reduce = [&](const GemmProb* dp) {
+    if (split_ids.empty()) return;
+    const int32_t* di = ctx.upload(split_ids);
+    int64_t mx = 1;
+    for (int32_t i : split_ids) mx = std::max<int64_t>(mx, int64_t(probs[size_t(i)].M) * probs[size_t(i)].N);
+    dim3 grid(unsigned(std::min<int64_t>(256, (mx + 255) / 256)), unsigned(split_ids.size())), block(256);
+    if (tc == Prec::F64) hipLaunchKernelGGL(gemm_splitk_reduce<double>, grid, block, 0, ctx.stream, dp, di);
+    else if (tc == Prec::F32) hipLaunchKernelGGL(gemm_splitk_reduce<float>, grid, block, 0, ctx.stream, dp, di);
+    else hipLaunchKernelGGL(gemm_splitk_reduce<bf16s>, grid, block, 0, ctx.stream, dp, di);
+    TTR_HIP(hipGetLastError());
   };
   const GemmSeg* ds = ctx.upload(segs);
   static const char* names[3] = {"gemm_f64", "gemm_f32", "gemm_bf16"};
@@ -441,10 +530,11 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     TTR_HIP(hipEventCreate(&e1));
     float best = 0.0f;
     int best_cfg = cfg;
+    const GemmProb* dp = nullptr;
     for (int c = 0; c < NCFG; ++c) {
       if ((CFG_TM[c] == 128 && am < 96) || (CFG_TN[c] == 128 && an < 96)) continue;
       int64_t nt;
-      const GemmProb* dp = prepare(c, nt);
+      dp = prepare(c, nt);
       TTR_HIP(hipEventRecord(e0, ctx.stream));
       launch(ctx, ta, tb, tc, math, c, int(nt), dp, np, ds);
       TTR_HIP(hipEventRecord(e1, ctx.stream));
@@ -458,12 +548,14 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    ctx.gemm_cfg[key] = best_cfg;  // every candidate wrote the same C
+    ctx.gemm_cfg[key] = best_cfg;  // every candidate wrote the same C (or the same partial sums)
+    reduce(dp);
     return;
   }
   int64_t nt;
   const GemmProb* dp = prepare(cfg, nt);
   launch(ctx, ta, tb, tc, math, cfg, int(nt), dp, np, ds);
+  reduce(dp);
 }
 
 }  // namespace ttr

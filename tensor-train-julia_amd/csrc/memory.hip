@@ -200,3 +200,21 @@ void Profiler::collect() {
 }
 
 }  // namespace ttr
+
+namespace ttr {
+
+double* Ctx::splitk_buffer(size_t elems) {
+  if (elems > splitk_elems) {
+    // the previous buffer may still be read by queued launches
+    sync();
+    if (splitk) TTR_HIP(hipFree(splitk));
+    splitk = nullptr;
+    splitk_elems = 0;
+    const size_t n = std::max(elems, size_t(1) << 20);
+    TTR_HIP(hipMalloc(reinterpret_cast<void**>(&splitk), n * sizeof(double)));
+    splitk_elems = n;
+  }
+  return splitk;
+}
+
+}  // namespace ttr

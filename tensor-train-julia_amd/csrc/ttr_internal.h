@@ -146,6 +146,10 @@ struct Ctx {
   // GEMM tile choice per (call site, precisions, size class), measured on first use
   std::unordered_map<std::string, int> gemm_cfg;
   bool gemm_autotune = true;
+  // partial sums of split-K GEMMs, reused by every launch on the stream
+  double* splitk = nullptr;
+  size_t splitk_elems = 0;
+  double* splitk_buffer(size_t elems);
   // copies a host array into fresh device workspace (asynchronously, through pinned memory)
   template <class T>
   T* upload(const T* host, size_t n) {
@@ -199,7 +203,12 @@ struct GemmProb {
   int32_t M, N;
   int32_t seg0, nseg;
   double alpha, beta;
-  int32_t tile0, tiles_n;
+  int32_t tile0, tiles_n, tiles_mn;
+  // split K: ksplit chunks of kc k-tiles each (the k-tiles of all segments in
+  // order); chunk c writes its raw sum to part + c*M*N (i fastest) and a
+  // reduction adds the chunks in order, then applies alpha and beta
+  int32_t ksplit, kc;
+  double* part;
 };
 
 class GemmBatch {
