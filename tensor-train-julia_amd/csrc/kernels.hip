@@ -245,16 +245,26 @@ __global__ void mpo_kernel(const MpoCore* __restrict__ jobs) {
 
 // Local action of an operator core on (mode, bond) pairs, one thread per fiber:
 // out[(om, ob)] = scale * sum_{im, ib} H(om, ob, im, ib) in[(im, ib)].  The
-// (n*rout) x (n*rin) matrix is staged in LDS; a thread keeps its MAXO outputs in
-// registers and reads each of its inputs once.
+// (n*rout) x (n*rin) matrix is staged in LDS with row pitch MAXO, zero padded to
+// MAXO outputs and to a multiple of 8 inputs, so the accumulation loop carries
+// no guards and reads the matrix as aligned vectors; a thread keeps its MAXO
+// outputs in registers and issues its inputs 8 loads at a time.
 template <int MAXO, class TI, class TO>
 __global__ __launch_bounds__(256) void op_apply_kernel(const OpApply* __restrict__ jobs) {
   const OpApply J = jobs[blockIdx.y];
-  __shared__ double M[64 * 64];
+  __shared__ __attribute__((aligned(16))) double M[64 * MAXO];
+  __shared__ int64_t offi[64], offo[64];
   const int n = J.n, din = J.n * J.rin, dout = J.n * J.rout;
-  for (int e = threadIdx.x; e < din * dout; e += blockDim.x) {
-    const int o = e % dout, i = e / dout;
-    M[e] = J.H[(o % n) * J.h_om + (o / n) * J.h_ob + (i % n) * J.h_im + (i / n) * J.h_ib];
+  const int din8 = (din + 7) & ~7;
+  for (int e = threadIdx.x; e < din8 * MAXO; e += blockDim.x) {
+    const int o = e % MAXO, i = e / MAXO;
+    M[e] = (o < dout && i < din)
+               ? J.H[(o % n) * J.h_om + (o / n) * J.h_ob + (i % n) * J.h_im + (i / n) * J.h_ib]
+               : 0.0;
+  }
+  for (int e = threadIdx.x; e < 64; e += blockDim.x) {
+    offi[e] = e < din ? (e % n) * J.is_m + (e / n) * J.is_b : 0;
+    offo[e] = e < dout ? (e % n) * J.os_m + (e / n) * J.os_b : 0;
   }
   __syncthreads();
   const int64_t nf = int64_t(J.X) * J.Y;
@@ -264,16 +274,21 @@ __global__ __launch_bounds__(256) void op_apply_kernel(const OpApply* __restrict
     double acc[MAXO];
 #pragma unroll
     for (int o = 0; o < MAXO; ++o) acc[o] = 0.0;
-    for (int i = 0; i < din; ++i) {
-      const double v = ld_as_d<TI>(in, (i % n) * J.is_m + (i / n) * J.is_b);
+    for (int i0 = 0; i0 < din8; i0 += 8) {
+      double v[8];
 #pragma unroll
-      for (int o = 0; o < MAXO; ++o)
-        if (o < dout) acc[o] += M[o + dout * i] * v;
+      for (int u = 0; u < 8; ++u) v[u] = (i0 + u < din) ? ld_as_d<TI>(in, offi[i0 + u]) : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const double* m = M + (i0 + u) * MAXO;
+#pragma unroll
+        for (int o = 0; o < MAXO; ++o) acc[o] += m[o] * v[u];
+      }
     }
     TO* out = static_cast<TO*>(J.out) + x * J.os_x + y * J.os_y;
 #pragma unroll
     for (int o = 0; o < MAXO; ++o)
-      if (o < dout) st_from_d<TO>(out, (o % n) * J.os_m + (o / n) * J.os_b, acc[o] * J.scale);
+      if (o < dout) st_from_d<TO>(out, offo[o], acc[o] * J.scale);
   }
 }
 
