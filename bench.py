@@ -219,7 +219,7 @@ def summarize(stats, args):
     sweeps = stats.pop("jacobi_svd:sweeps", None)
     jacobi = ({"svds": sweeps["launches"], "mean_sweeps": round(sweeps["flops"] / max(sweeps["launches"], 1), 2),
                "max_sweeps": sweeps["bytes"]} if sweeps else None)
-    for ph in ("visits", "ticks_gram", "ticks_inner", "ticks_apply", "ticks_inner_params", "ticks_inner_rotate"):
+    for ph in ("visits", "ticks_gram", "ticks_inner", "ticks_apply"):
         st = stats.pop("jacobi_svd:" + ph, None)
         if st and jacobi:
             jacobi[ph] = st["flops"]

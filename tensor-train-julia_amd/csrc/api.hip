@@ -356,6 +356,20 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     std::vector<double> nr = frob_norms_host(h, ptrs, cnt);
     for (size_t q = 0; q < nb; ++q) st[q].zero = nr[q] < tol;
   }
+  // singular values of every item in one array (one download per step)
+  std::vector<int64_t> sv_off(nb + 1, 0);
+  for (size_t q = 0; q < nb; ++q) {
+    const DevTT* y = st[q].y;
+    int64_t mxp = 1;
+    for (int k = 0; k < y->d; ++k) {
+      mxp = std::max(mxp, std::min(y->rks[k], y->dims[k] * y->rks[k + 1]));
+      mxp = std::max(mxp, std::min(y->rks[k] * y->dims[k], y->rks[k + 1]));
+    }
+    sv_off[q + 1] = sv_off[q] + mxp;
+  }
+  double* sv_all = ctx.ws.alloc_n<double>(size_t(sv_off[nb]));
+  std::vector<double> sv_host(static_cast<size_t>(sv_off[nb]));
+  for (size_t q = 0; q < nb; ++q) st[q].sv = sv_all + sv_off[q];
   for (auto& s : st) {
     s.cur = s.y->cores;
     s.rk = s.y->rks;
@@ -372,7 +386,6 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     s.US = ctx.ws.alloc_n<double>(size_t(mx));
     s.B = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
     s.J = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
-    s.sv = ctx.ws.alloc_n<double>(size_t(mxp));
     s.idx = ctx.ws.alloc_n<int32_t>(size_t(mxp));
     s.out = h->make_tt_lazy(s.y->dims);
   }
@@ -438,17 +451,21 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     jacobi_svd_batched(ctx, sj);
     // rank decisions on the host
     ctx.sync();
+    TTR_HIP(hipMemcpy(sv_host.data(), sv_all, sv_host.size() * 8, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> info_host;
+    if (ctx.prof.on) {
+      info_host.resize(8 * nb);
+      TTR_HIP(hipMemcpy(info_host.data(), info_all, info_host.size() * 4, hipMemcpyDeviceToHost));
+    }
     std::vector<int64_t> K(nb, 0);
-    std::vector<GatherCols> gc;
+    std::vector<int32_t> orders;  // the kept columns of every item, one upload
+    std::vector<size_t> order_off(nb, 0);
     for (size_t q = 0; q < nb; ++q) {
       if (jcore[q] < 0) continue;
-      SweepState& s = st[q];
       const int64_t m = pdim[q];
-      std::vector<double> sv(m);
-      TTR_HIP(hipMemcpy(sv.data(), s.sv, size_t(m) * 8, hipMemcpyDeviceToHost));
+      const double* sv = sv_host.data() + sv_off[q];
       if (ctx.prof.on) {
-        uint32_t info[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        TTR_HIP(hipMemcpy(info, s.info, sizeof(info), hipMemcpyDeviceToHost));
+        const uint32_t* info = info_host.data() + 8 * q;
         ProfStat& ps = ctx.prof.stats["jacobi_svd:sweeps"];
         ps.launches += 1;
         ps.flops += info[0];
@@ -463,8 +480,6 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
         ctx.prof.stats["jacobi_svd:ticks_gram"].flops += info[1];
         ctx.prof.stats["jacobi_svd:ticks_inner"].flops += info[2];
         ctx.prof.stats["jacobi_svd:ticks_apply"].flops += info[3];
-        ctx.prof.stats["jacobi_svd:ticks_inner_params"].flops += info[6];
-        ctx.prof.stats["jacobi_svd:ticks_inner_rotate"].flops += info[7];
       }
       std::vector<int32_t> order(m);
       std::iota(order.begin(), order.end(), 0);
@@ -481,10 +496,19 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       k = std::min<int64_t>({k, rmax, npos});
       k = std::max<int64_t>(k, 1);
       K[q] = k;
-      int32_t* didx = h->ctx.upload(order.data(), size_t(k));
+      order_off[q] = orders.size();
+      orders.insert(orders.end(), order.begin(), order.begin() + k);
+    }
+    const int32_t* dorders = h->ctx.upload(orders);
+    std::vector<GatherCols> gc;
+    for (size_t q = 0; q < nb; ++q) {
+      if (jcore[q] < 0) continue;
+      SweepState& s = st[q];
+      const int64_t m = pdim[q], k = K[q];
+      int32_t* didx = const_cast<int32_t*>(dorders) + order_off[q];
       // reorder columns of B*J and J in place order (into US / Mt scratch)
       gc.push_back(GatherCols{s.B, s.US, m, m, didx, int(m), int(k)});
-      gc.push_back(GatherCols{s.J, s.Qb == nullptr ? nullptr : s.B, m, m, didx, int(m), int(k)});
+      gc.push_back(GatherCols{s.J, s.B, m, m, didx, int(m), int(k)});
     }
     // second gather must not overwrite B before the first reads it: run them in two launches
     std::vector<GatherCols> g1, g2;

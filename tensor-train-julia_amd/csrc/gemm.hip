@@ -12,6 +12,7 @@
 //   F32  : v_mfma_f32_16x16x4_f32, FP32 accumulate
 //   BF16 : v_mfma_f32_16x16x32_bf16, FP32 accumulate
 #include <cmath>
+#include <cstring>
 
 #include "ttr_internal.h"
 
@@ -496,22 +497,45 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
       }
     }
   }
-  auto prepare = [&](int c, int64_t& ntiles) -> const GemmProb* {
+  // the tile configurations to run: every admissible one when tuning
+  std::vector<int> cands;
+  if (tune) {
+    for (int c = 0; c < NCFG; ++c)
+      if (!((CFG_TM[c] == 128 && am < 96) || (CFG_TN[c] == 128 && an < 96))) cands.push_back(c);
+  } else {
+    cands.push_back(cfg);
+  }
+  // one upload for all descriptors: segments, split ids, then the problems as
+  // laid out for each candidate configuration
+  auto align16 = [](size_t x) { return (x + 15) & ~size_t(15); };
+  const size_t seg_bytes = align16(segs.size() * sizeof(GemmSeg));
+  const size_t ids_bytes = align16(split_ids.size() * sizeof(int32_t));
+  const size_t prob_bytes = align16(probs.size() * sizeof(GemmProb));
+  std::vector<char> pack(seg_bytes + ids_bytes + prob_bytes * cands.size());
+  std::memcpy(pack.data(), segs.data(), segs.size() * sizeof(GemmSeg));
+  if (!split_ids.empty()) std::memcpy(pack.data() + seg_bytes, split_ids.data(), split_ids.size() * sizeof(int32_t));
+  std::vector<int64_t> ntiles(cands.size());
+  for (size_t ci = 0; ci < cands.size(); ++ci) {
+    const int c = cands[ci];
     const int TM = math == Prec::BF16 ? BM : CFG_TM[c], TN = math == Prec::BF16 ? BN : CFG_TN[c];
-    ntiles = 0;
+    int64_t nt = 0;
     for (auto& p : probs) {
       const int tm = (p.M + TM - 1) / TM, tn = (p.N + TN - 1) / TN;
-      p.tile0 = int32_t(ntiles);
+      p.tile0 = int32_t(nt);
       p.tiles_n = tn;
       p.tiles_mn = tm * tn;
-      ntiles += int64_t(tm) * tn * p.ksplit;
+      nt += int64_t(tm) * tn * p.ksplit;
     }
-    TTR_REQUIRE(ntiles < (int64_t(1) << 31), ST_INVALID_ARG, "GEMM batch too large");
-    return ctx.upload(probs);
-  };
+    TTR_REQUIRE(nt < (int64_t(1) << 31), ST_INVALID_ARG, "GEMM batch too large");
+    ntiles[ci] = nt;
+    std::memcpy(pack.data() + seg_bytes + ids_bytes + prob_bytes * ci, probs.data(), probs.size() * sizeof(GemmProb));
+  }
+  const char* dpack = ctx.upload(pack);
+  const GemmSeg* ds = reinterpret_cast<const GemmSeg*>(dpack);
+  const int32_t* di = reinterpret_cast<const int32_t*>(dpack + seg_bytes);
+  auto dprobs = [&](size_t ci) { return reinterpret_cast<const GemmProb*>(dpack + seg_bytes + ids_bytes + prob_bytes * ci); };
   auto reduce = [&](const GemmProb* dp) {
     if (split_ids.empty()) return;
-    const int32_t* di = ctx.upload(split_ids);
     int64_t mx = 1;
     for (int32_t i : split_ids) mx = std::max<int64_t>(mx, int64_t(probs[size_t(i)].M) * probs[size_t(i)].N);
     dim3 grid(unsigned(std::min<int64_t>(256, (mx + 255) / 256)), unsigned(split_ids.size())), block(256);
@@ -520,7 +544,6 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     else hipLaunchKernelGGL(gemm_splitk_reduce<bf16s>, grid, block, 0, ctx.stream, dp, di);
     TTR_HIP(hipGetLastError());
   };
-  const GemmSeg* ds = ctx.upload(segs);
   static const char* names[3] = {"gemm_f64", "gemm_f32", "gemm_bf16"};
   const std::string site_name = site ? std::string("site:") + site : std::string();
   ProfScope ps(ctx, names[int(math)], flops, bytes, site ? site_name.c_str() : nullptr);
@@ -530,32 +553,26 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     TTR_HIP(hipEventCreate(&e1));
     float best = 0.0f;
     int best_cfg = cfg;
-    const GemmProb* dp = nullptr;
-    for (int c = 0; c < NCFG; ++c) {
-      if ((CFG_TM[c] == 128 && am < 96) || (CFG_TN[c] == 128 && an < 96)) continue;
-      int64_t nt;
-      dp = prepare(c, nt);
+    for (size_t ci = 0; ci < cands.size(); ++ci) {
       TTR_HIP(hipEventRecord(e0, ctx.stream));
-      launch(ctx, ta, tb, tc, math, c, int(nt), dp, np, ds);
+      launch(ctx, ta, tb, tc, math, cands[ci], int(ntiles[ci]), dprobs(ci), np, ds);
       TTR_HIP(hipEventRecord(e1, ctx.stream));
       TTR_HIP(hipEventSynchronize(e1));
       float ms = 0.0f;
       TTR_HIP(hipEventElapsedTime(&ms, e0, e1));
       if (best == 0.0f || ms < best) {
         best = ms;
-        best_cfg = c;
+        best_cfg = cands[ci];
       }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ctx.gemm_cfg[key] = best_cfg;  // every candidate wrote the same C (or the same partial sums)
-    reduce(dp);
+    reduce(dprobs(0));
     return;
   }
-  int64_t nt;
-  const GemmProb* dp = prepare(cfg, nt);
-  launch(ctx, ta, tb, tc, math, cfg, int(nt), dp, np, ds);
-  reduce(dp);
+  launch(ctx, ta, tb, tc, math, cfg, int(ntiles[0]), dprobs(0), np, ds);
+  reduce(dprobs(0));
 }
 
 }  // namespace ttr

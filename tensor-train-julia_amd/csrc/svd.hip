@@ -237,8 +237,6 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
   double cV[4] = {1.0, 1.0, 1.0, 1.0}, sV[4] = {0.0, 0.0, 0.0, 0.0};
   int aV[4] = {0, 0, 0, 0}, bV[4] = {0, 0, 0, 0};
   bool visit_rot = false;
-  uint64_t tp1 = 0, tp2 = 0, tr = 0;
-  if (tid == 0) tr = wall_clock64();
   for (int ir = 0; ir < IR; ++ir) {
     const int buf = ir & 1;
     const int a = min(xA, xB), b = max(xA, xB);
@@ -280,11 +278,6 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
       }
     }
     __syncthreads();
-    if (tid == 0) {
-      const uint64_t t = wall_clock64();
-      tp1 += t - tr;
-      tr = t;
-    }
     // ---- phase 2: all loads first, the rotations applied unconditionally (an
     // identity rotation reproduces its operands exactly), stores only where a
     // rotation acts
@@ -346,15 +339,6 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
       yc[j] = adv(yc[j]);
     }
     __syncthreads();
-    if (tid == 0) {
-      const uint64_t t = wall_clock64();
-      tp2 += t - tr;
-      tr = t;
-    }
-  }
-  if (tid == 0 && S.info) {
-    atomicAdd(reinterpret_cast<unsigned*>(S.info) + 6, unsigned(tp1));
-    atomicAdd(reinterpret_cast<unsigned*>(S.info) + 7, unsigned(tp2));
   }
   {
     double va[4], vb[4];

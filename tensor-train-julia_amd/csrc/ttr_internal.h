@@ -262,8 +262,7 @@ struct SVDJob {
   int p;
   int32_t* info;  // optional, 8 counters zeroed by the caller: [0] sweeps used, [1..3] wall-clock
                   // ticks of the Gram / inner / apply phases summed over visits, [4] visits,
-                  // [5] visits that applied a rotation, [6..7] ticks of the two phases
-                  // of the inner rounds (parameters + V update, rotation of G)
+                  // [5] visits that applied a rotation, [6..7] unused
 };
 void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs);
 
