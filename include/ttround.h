@@ -187,6 +187,20 @@ int ttr_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand
 int ttr_sketch_op(ttr_handle_t h, int64_t op_id, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts,
                   double* const* W_out, int64_t* sketch_rks);
 
+/* Boundary-only reverse sketch of n TTs (op_ids NULL or op_ids[i] < 0), or of
+ * H_i x_i without forming the product (op_ids[i] >= 0), all sharing one draw
+ * stream: the block count p = cld(s, bv_1) is the same at every core, and
+ * only vec(W_1) / sqrt(p) is kept, bv_1 * p = *s_actual values per item.
+ * Item i goes to out[i * s_actual ...]; out_len must be >= n * s_actual.
+ * out == NULL only computes *s_actual.  opts->p and opts->oversampling are
+ * ignored (p follows from s).  The extra sketch of tt_combined_sketch is this
+ * call with seed + 1.
+ * Replaces: tt_sketch(T, A, s; reverse=true, orthogonal, seed, block_rks)
+ * (src/sketches.jl:645-723) and tt_sketch(T, H, A, s; reverse=true, ...)
+ * (:729-813). */
+int ttr_tt_sketch(ttr_handle_t h, int64_t n, const int64_t* tt_ids, const int64_t* op_ids, int64_t s,
+                  const ttr_rand_opts* opts, double* out, int64_t out_len, int64_t* s_actual);
+
 /* The sketch blocks of core k (0-based) exactly as the device uses them,
  * upcast to FP64, in the reference order (bv_{k+1}, n_k, bv_k, p_k)
  * column-major (the array generate_sketch_blocks returns, src/sketches.jl:83).

@@ -32,7 +32,7 @@ __all__ = [
     "tto_rounding", "default_rank_heuristic", "sketch_blocks_heuristic",
     "sketch_structure", "generate_sketch_blocks", "contract_sketch_core_backwards",
     "contract_sketch_core_backwards_op", "tt_recursive_sketch",
-    "tt_recursive_sketch_op", "ttrand_rounding", "ttrand_rounding_lincomb",
+    "tt_recursive_sketch_op", "tt_sketch", "tt_combined_sketch", "ttrand_rounding", "ttrand_rounding_lincomb",
     "ttrand_rounding_op", "ttrand_rounding_residual", "rel_err", "tt_dot", "dot",
     "dot_par", "partial_contraction", "dot_operator",
     "hubbard_1d_mpo", "hubbard_1d_dense",
@@ -692,6 +692,61 @@ def tt_recursive_sketch_op(H: TTO, A: TT, rks, draws, orthogonal: bool = True,
         W[k] = W[k] / math.sqrt(p[k])
         out.append(jreshape(W[k], (A.rks[k], H.rks[k], bv[k] * p[k])))
     return out, [bv[k] * p[k] for k in range(N + 1)]
+
+
+def tt_sketch(A: TT, s: int, draws, orthogonal: bool = True, block_rks: Optional[int] = None,
+              H: Optional[TTO] = None):
+    """Boundary-only reverse sketch ``tt_sketch(T, [H,] A, s; reverse=true)``
+    (``src/sketches.jl:645-723``, operator form ``:729-813``).
+
+    The same block count ``p = cld(s, bv_1)`` is used at every core and only
+    the boundary ``W_1`` of shape ``(1, bv_1, p)`` (``(1, 1, bv_1, p)`` with an
+    operator) is kept, divided by ``sqrt(p)`` once.  Returns
+    ``(vec(W_1), bv_1 p)``."""
+    N = A.N
+    dims = A.dims
+    bv, _ = sketch_structure(dims, [1] * (N + 1), block_rks)
+    p = -(-int(s) // bv[0])
+    W = np.ones((1, 1, 1)) if H is None else np.ones((1, 1, 1, 1))
+    for k in range(N - 1, -1, -1):
+        S = _sketch_for(draws, k, bv[k + 1], dims[k], bv[k], p, orthogonal)
+        if H is None:
+            Wk = np.zeros((A.rks[k], bv[k], p))
+            for j in range(p):
+                Wk[:, :, j] = contract_sketch_core_backwards(A.cores[k], S[:, :, :, j],
+                                                             W[:, :, j if k < N - 1 else 0])
+        else:
+            Wk = np.zeros((A.rks[k], H.rks[k], bv[k], p))
+            for j in range(p):
+                Wk[:, :, :, j] = contract_sketch_core_backwards_op(A.cores[k], H.cores[k], S[:, :, :, j],
+                                                                   W[:, :, :, j if k < N - 1 else 0])
+        W = Wk
+    return jreshape(W, -1) / math.sqrt(p), bv[0] * p
+
+
+def tt_combined_sketch(A: TT, rks, s: int, draws, draws_extra, orthogonal: bool = True,
+                       block_rks: Optional[int] = None, H: Optional[TTO] = None):
+    """``tt_combined_sketch([T,] [H,] A, rks, s; reverse=true)`` (``src/sketches.jl:930-953``,
+    operator form ``:966-989``): the recursive sketch with the boundary rank raised
+    to ``max(rks_1, min(s, rks_2))``; its boundary sketch, extended by a boundary-only
+    sketch of size ``s - s_rec`` from the next seed (``draws_extra``) when ``s`` exceeds
+    it, the two parts weighted by ``sqrt(len/s)``.  Returns ``(sketch, W, sketch_rks)``
+    with ``W[1]`` replaced by ones and ``sketch_rks[1] = 1``."""
+    rks = list(rks)
+    rks[0] = max(rks[0], min(int(s), rks[1]))
+    if H is None:
+        W, srk = tt_recursive_sketch(A, rks, draws, orthogonal, block_rks)
+    else:
+        W, srk = tt_recursive_sketch_op(H, A, rks, draws, orthogonal, block_rks)
+    sketch = jreshape(W[0], -1)
+    s_rec = srk[0]
+    if s > s_rec:
+        extra, _ = tt_sketch(A, s - s_rec, draws_extra, orthogonal, block_rks, H)
+        s = s_rec + len(extra)
+        sketch = np.concatenate([math.sqrt(len(sketch) / s) * sketch, math.sqrt(len(extra) / s) * extra])
+    W[0] = np.ones((1, 1)) if H is None else np.ones((1, 1, 1))
+    srk[0] = 1
+    return sketch, W, srk
 
 
 def _qr_thin(Z: np.ndarray) -> np.ndarray:

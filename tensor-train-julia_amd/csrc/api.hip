@@ -1724,6 +1724,73 @@ int ttr_sketch_op(ttr_handle_t h, int64_t op_id, int64_t tt_id, const int64_t* r
   return guarded(h, [&] { sketch_one(h, h->get(tt_id), h->get_op(op_id), rks, opts, W_out, sketch_rks); });
 }
 
+// tt_sketch(T, [H,] A, s; reverse=true, ...) (src/sketches.jl:645-723, operator
+// form :729-813) for a batch: the right-to-left recursion of the recursive sketch
+// with the same block count p = cld(s, bv_1) at every core (the draws of core k
+// are those of the recursive sketch with p blocks), all items in the same
+// launches; only the boundary sketch vec(W_1) / sqrt(p) is returned.
+int ttr_tt_sketch(ttr_handle_t h, int64_t n, const int64_t* tt_ids, const int64_t* op_ids, int64_t s,
+                  const ttr_rand_opts* opts, double* out, int64_t out_len, int64_t* s_actual) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(n >= 1 && tt_ids && opts && s_actual && s >= 1, ST_INVALID_ARG, "bad arguments");
+    DevTT* x0 = h->get(tt_ids[0]);
+    const int d = x0->d;
+    const std::vector<int64_t> bv = block_ranks(x0->dims, opts->block_rks > 0 ? opts->block_rks : d);
+    const int64_t p = (s + bv[0] - 1) / bv[0];
+    const int64_t sa = bv[0] * p;
+    *s_actual = sa;
+    if (!out) return;  // size query
+    TTR_REQUIRE(out_len >= n * sa, ST_INVALID_ARG, "out holds fewer than n * s_actual values");
+    std::vector<RandItem> its(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) {
+      RandTerm t{};
+      t.x = h->get(tt_ids[i]);
+      t.alpha = 1.0;
+      t.H = (op_ids && op_ids[i] >= 0) ? h->get_op(op_ids[i]) : nullptr;
+      check_dims_equal(x0, t.x);
+      TTR_REQUIRE(t.x->rks[0] == 1 && t.x->rks[d] == 1, ST_INVALID_ARG, "boundary ranks must be 1");
+      if (t.H) {
+        TTR_REQUIRE(t.H->dims == x0->dims, ST_DIM_MISMATCH, "operator and TT dims differ");
+        TTR_REQUIRE(t.H->rks[0] == 1 && t.H->rks[d] == 1, ST_INVALID_ARG, "operator boundary ranks must be 1");
+      }
+      its[size_t(i)].terms.push_back(t);
+      its[size_t(i)].rks.assign(size_t(d) + 1, 1);
+    }
+    std::vector<int64_t> pv(size_t(d) + 1, p);
+    pv[size_t(d)] = 1;
+    ttr_rand_opts o2 = *opts;
+    o2.p = pv.data();
+    o2.oversampling = 1.0;
+    SketchPlan sp;
+    prepare_plan(h, &o2, d, x0->dims, its, sp);
+    make_sketch(h, sp, 0);
+    sketch_phase(h, sp, its, 0);
+    Ctx& ctx = h->ctx;
+    const size_t eb = prec_bytes(sp.prec);
+    // W_1 (1 x s) and its power-of-two exponent of every item, one sync
+    char* hw = static_cast<char*>(ctx.pin.alloc(size_t(n) * size_t(sa) * eb));
+    int32_t* he = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * size_t(n)));
+    for (int64_t i = 0; i < n; ++i) {
+      const RandTerm& tt = its[size_t(i)].terms[0];
+      TTR_HIP(hipMemcpyAsync(hw + size_t(i) * size_t(sa) * eb, tt.W[0], size_t(sa) * eb, hipMemcpyDeviceToHost,
+                             ctx.stream));
+      TTR_HIP(hipMemcpyAsync(he + i, tt.E, sizeof(int32_t), hipMemcpyDeviceToHost, ctx.stream));
+    }
+    ctx.sync();
+    for (int64_t i = 0; i < n; ++i) {
+      const double scale = std::ldexp(1.0, he[i]) / std::sqrt(double(p));
+      const char* raw = hw + size_t(i) * size_t(sa) * eb;
+      for (int64_t e = 0; e < sa; ++e) {
+        double v;
+        if (sp.prec == Prec::F64) std::memcpy(&v, raw + 8 * e, 8);
+        else if (sp.prec == Prec::F32) { float f; std::memcpy(&f, raw + 4 * e, 4); v = f; }
+        else { uint16_t b; std::memcpy(&b, raw + 2 * e, 2); v = bf16_bits_to_float(b); }
+        out[i * sa + e] = v * scale;
+      }
+    }
+  });
+}
+
 int ttr_sketch_blocks(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, const ttr_rand_opts* opts,
                       int64_t k, double* out, int64_t* shape4) {
   return guarded(h, [&] {

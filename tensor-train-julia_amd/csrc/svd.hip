@@ -69,6 +69,176 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return y;
 }
 
+// lane 0's value broadcast through scalar registers (the readfirstlane builtin
+// takes a 32-bit int: a double passed to it directly is converted, not copied)
+__device__ __forceinline__ double first_lane(double v) {
+  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(b & 0xffffffffu)));
+  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32)));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// ---- one inner Jacobi sweep on G: 63 rounds of 32 disjoint rotations in
+// round-robin order (pair t of round r joins the players at positions t and
+// 63 - t; every player but 0 advances one position per round).  G is
+// symmetric and only its upper triangle is read or written from here on:
+// entry (i, j) lives at G[min(i,j)][max(i,j)].  A round has two phases, each
+// ending with a barrier:
+//   1. lanes 0..31 of wave 0 compute the rotation (c, s) of pair `lane` from
+//      G into pcs[r & 1]; waves 1..7 meanwhile apply the previous round's
+//      rotations to V (row = lane; wave w owns pairs w - 1 + 7k), skipping
+//      identity rotations (their parameters are wave-uniform, read by one
+//      lane and broadcast through scalar registers);
+//   2. every thread rotates its upper-triangle 2x2 block(s) of G in place:
+//      threads 0..495 one off-diagonal block (pair positions P < Q), threads
+//      496..511 two diagonal blocks (P = Q).  A block whose two rotations
+//      are the identity is neither read nor written.
+// Each thread only touches its own blocks in phase 2, and every block is
+// owned by one thread, so the update is in place.
+__device__ __forceinline__ void inner_sweep(double (*G)[GLD], double (*V)[GLD], double tol2, int tid, int* vrot) {
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  __shared__ double2 pcs[2][NPAIR];
+  auto adv = [](int x) { return x == 0 ? 0 : (x == IR ? 1 : x + 1); };
+  // phase-1 players of the parameter lanes
+  int qa = rr_player(lane & (NPAIR - 1), 0, WC), qb = rr_player(IR - (lane & (NPAIR - 1)), 0, WC);
+  // phase-2 blocks
+  int bP[2], bQ[2];
+  const int nbk = tid < 496 ? 1 : 2;
+  if (tid < 496) {
+    int t = tid, P = 0;
+    while (t >= NPAIR - 1 - P) {
+      t -= NPAIR - 1 - P;
+      ++P;
+    }
+    bP[0] = P;
+    bQ[0] = P + 1 + t;
+    bP[1] = bQ[1] = 0;
+  } else {
+    bP[0] = bQ[0] = 2 * (tid - 496);
+    bP[1] = bQ[1] = 2 * (tid - 496) + 1;
+  }
+  int rx0[2], rx1[2], cx0[2], cx1[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    rx0[b] = rr_player(bP[b], 0, WC);
+    rx1[b] = rr_player(IR - bP[b], 0, WC);
+    cx0[b] = rr_player(bQ[b], 0, WC);
+    cx1[b] = rr_player(IR - bQ[b], 0, WC);
+  }
+  // V pairs of this wave (waves 1..7), players one round behind
+  constexpr int VK = 5;
+  int vt[VK], vx[VK], vy[VK];
+#pragma unroll
+  for (int k = 0; k < VK; ++k) {
+    vt[k] = wave - 1 + 7 * k;
+    const int t = (wave > 0 && vt[k] < NPAIR) ? vt[k] : 0;
+    vx[k] = rr_player(t, 0, WC);
+    vy[k] = rr_player(IR - t, 0, WC);
+  }
+  // (c, s) of this wave's V pairs for the pending V update, read at the end of
+  // the phase 2 that follows their computation
+  double vc[VK], vs[VK];
+#pragma unroll
+  for (int k = 0; k < VK; ++k) {
+    vc[k] = 1.0;
+    vs[k] = 0.0;
+  }
+  auto v_params = [&](const double2* cs) {
+#pragma unroll
+    for (int k = 0; k < VK; ++k) {
+      if (vt[k] < NPAIR) {
+        const double2 cv = cs[vt[k]];
+        vc[k] = cv.x;
+        vs[k] = cv.y;
+      }
+    }
+  };
+  // all loads first, then the rotations, stores only where a rotation acts
+  auto v_update = [&]() {
+    double va[VK], vb[VK];
+    int lo[VK], hi[VK];
+#pragma unroll
+    for (int k = 0; k < VK; ++k) {
+      lo[k] = min(vx[k], vy[k]);
+      hi[k] = max(vx[k], vy[k]);
+      va[k] = V[lane][lo[k]];
+      vb[k] = V[lane][hi[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < VK; ++k) {
+      if (vt[k] < NPAIR && vs[k] != 0.0) {
+        V[lane][lo[k]] = vc[k] * va[k] - vs[k] * vb[k];
+        V[lane][hi[k]] = vs[k] * va[k] + vc[k] * vb[k];
+      }
+      vx[k] = adv(vx[k]);
+      vy[k] = adv(vy[k]);
+    }
+  };
+  for (int ir = 0; ir < IR; ++ir) {
+    const int buf = ir & 1;
+    // ---- phase 1
+    if (wave == 0) {
+      if (lane < NPAIR) {
+        const int a = min(qa, qb), b = max(qa, qb);
+        const double aa = G[a][a], bb2 = G[b][b], ab = G[a][b];
+        double c = 1.0, s = 0.0;
+        // padding columns have a zero diagonal and never rotate
+        if (aa > 0.0 && bb2 > 0.0 && ab * ab > tol2 * aa * bb2) {
+          const double dd = bb2 - aa;
+          const double q = dd * dd + 4.0 * ab * ab;
+          double t;
+          if (q > 0x1p-900) {
+            // G is scaled to max diagonal ~1, so q <= ~20: hardware estimates with two
+            // Newton steps each (rsqrt, reciprocal) are accurate to ~1 ulp here
+            const double rq = rsqrt_nr(q);
+            t = 2.0 * ab * copysign(1.0, dd) * rcp_nr(fabs(dd) + q * rq);
+          } else {
+            t = 2.0 * ab * copysign(1.0, dd) / (fabs(dd) + sqrt(q));
+          }
+          c = rsqrt_nr(1.0 + t * t);  // 1 + t^2 in [1, 2]
+          s = c * t;
+          *vrot = 1;
+        }
+        pcs[buf][lane] = make_double2(c, s);
+        qa = adv(qa);
+        qb = adv(qb);
+      }
+    } else if (ir > 0) {
+      v_update();
+    }
+    __syncthreads();
+    // ---- phase 2: loads first (parameters and the four entries), then the
+    // rotation, stores only where a rotation acts
+    for (int b = 0; b < nbk; ++b) {
+      const int rl = min(rx0[b], rx1[b]), rh = max(rx0[b], rx1[b]);
+      const int cl = min(cx0[b], cx1[b]), ch = max(cx0[b], cx1[b]);
+      double* e00 = &G[min(rl, cl)][max(rl, cl)];
+      double* e01 = &G[min(rl, ch)][max(rl, ch)];
+      double* e10 = &G[min(rh, cl)][max(rh, cl)];
+      double* e11 = &G[min(rh, ch)][max(rh, ch)];
+      const double2 pr = pcs[buf][bP[b]], pq = pcs[buf][bQ[b]];
+      const double g0 = *e00, g1 = *e01, g2 = *e10, g3 = *e11;
+      const double t00 = pr.x * g0 - pr.y * g2, t01 = pr.x * g1 - pr.y * g3;
+      const double t10 = pr.y * g0 + pr.x * g2, t11 = pr.y * g1 + pr.x * g3;
+      if (pr.y != 0.0 || pq.y != 0.0) {
+        *e00 = t00 * pq.x - t01 * pq.y;
+        *e01 = t00 * pq.y + t01 * pq.x;
+        if (bP[b] != bQ[b]) *e10 = t10 * pq.x - t11 * pq.y;  // a diagonal block's (h, l) is its (l, h)
+        *e11 = t10 * pq.y + t11 * pq.x;
+      }
+      rx0[b] = adv(rx0[b]);
+      rx1[b] = adv(rx1[b]);
+      cx0[b] = adv(cx0[b]);
+      cx1[b] = adv(cx1[b]);
+    }
+    if (wave > 0) v_params(pcs[buf]);
+    __syncthreads();
+  }
+  if (wave > 0) v_update();
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __restrict__ jobs, int njobs, int round,
                                                              int sweep, int32_t* __restrict__ flags) {
   const int bid = blockIdx.x;
@@ -101,8 +271,6 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
   // G doubles as the row-chunk staging buffer of the Gram and apply phases
   __shared__ double G[WC][GLD];
   __shared__ double V[WC][GLD];
-  __shared__ double pdiag[2][WC];   // diagonal of G for the next round's parameters
-  __shared__ double pab[2][NPAIR];  // G(a_l, b_l) of the next round's pairs
   __shared__ int cols[WC];          // global column of local index, -1 if padding
   double (*chunk)[GLD] = G;
 
@@ -171,11 +339,6 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
     for (int e = tid; e < WC * WC; e += JT) G[e % WC][e / WC] *= sc;
   }
   __syncthreads();
-  if (tid < WC) pdiag[0][tid] = G[tid][tid];
-  if (tid < NPAIR) {
-    const int a = rr_player(tid, 0, WC), b = rr_player(WC - 1 - tid, 0, WC);
-    pab[0][tid] = G[min(a, b)][max(a, b)];
-  }
   const double tol = double(p > 16 ? p : 16) * 2.220446049250313e-16;
   const double tol2 = tol * tol;
   // a visit whose columns are already pairwise orthogonal changes nothing (no
@@ -188,6 +351,8 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
       need |= (aa > 0.0 && bb2 > 0.0 && ab * ab > tol2 * aa * bb2) ? 1 : 0;
     }
   }
+  __shared__ int vrot;
+  if (tid == 0) vrot = 0;
   if (!__syncthreads_or(need)) {
     if (tid == 0 && S.info) {
       atomicAdd(reinterpret_cast<unsigned*>(S.info) + 1, unsigned(t1 - t0));
@@ -196,165 +361,7 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
     return;
   }
 
-  // ---- one inner Jacobi sweep on G (63 rounds of 32 disjoint rotations,
-  // round-robin order).  A round has two phases:
-  //   1. wave 0 computes the 32 rotation parameters from pdiag/pab and stores
-  //      them in pc/ps, while every thread applies the previous round's
-  //      rotation to its part of V (row lane, column slots 2w, 2w+1, 2w+16,
-  //      2w+17 of wave w);
-  //   2. thread (wave w, lane l) rotates its 2x2 blocks of G in place (row slot
-  //      l & 31, column slots 2w + h and 2w + 16 + h, h = l >> 5) and stores the
-  //      entries of G that the next round's parameters need into pdiag/pab.
-  // Each phase ends with a barrier.  Every slot keeps its position in the
-  // round-robin table, so the player at a position advances by one per round
-  // (63 wraps to 1, player 0 stays at position 0); the positions a player
-  // takes in the next round, and the partner there, are fixed per lane.
-  const int h = lane >> 5;
-  const int sl = lane & (NPAIR - 1);
-  __shared__ double pc[NPAIR], ps[NPAIR];
-  auto adv = [](int x) { return x == 0 ? 0 : (x == IR ? 1 : x + 1); };
-  // row slot sl: players at positions sl and 63 - sl; in the next round the
-  // player at position sl moves to nA (partner position 63 - nA, slot snA), the
-  // one at 63 - sl to 62 - sl (partner position sl + 1, slot snB)
-  int xA = rr_player(sl, 0, WC), xB = rr_player(IR - sl, 0, WC);
-  const int nA = sl == 0 ? 0 : (sl == 1 ? IR : sl - 1);
-  const int snA = min(nA, IR - nA), snB = min(62 - sl, sl + 1);
-  int yA = rr_player(IR - nA, 1, WC), yB = rr_player(sl + 1, 1, WC);
-  int xc[2], yc[2];
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int kc = 2 * wave + 16 * j + h;
-    xc[j] = rr_player(kc, 0, WC);
-    yc[j] = rr_player(IR - kc, 0, WC);
-  }
-  const int qs[4] = {2 * wave, 2 * wave + 1, 2 * wave + 16, 2 * wave + 17};
-  int xv[4], yv[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    xv[i] = rr_player(qs[i], 0, WC);
-    yv[i] = rr_player(IR - qs[i], 0, WC);
-  }
-  double cV[4] = {1.0, 1.0, 1.0, 1.0}, sV[4] = {0.0, 0.0, 0.0, 0.0};
-  int aV[4] = {0, 0, 0, 0}, bV[4] = {0, 0, 0, 0};
-  bool visit_rot = false;
-  for (int ir = 0; ir < IR; ++ir) {
-    const int buf = ir & 1;
-    const int a = min(xA, xB), b = max(xA, xB);
-    // ---- phase 1
-    if (wave == 0 && lane < NPAIR) {
-      const double aa = pdiag[buf][a], bb2 = pdiag[buf][b], ab = pab[buf][sl];
-      double c = 1.0, s = 0.0;
-      // padding columns have a zero diagonal and never rotate
-      if (aa > 0.0 && bb2 > 0.0 && ab * ab > tol2 * aa * bb2) {
-        const double dd = bb2 - aa;
-        const double q = dd * dd + 4.0 * ab * ab;
-        double t;
-        if (q > 0x1p-900) {
-          // G is scaled to max diagonal ~1, so q <= ~20: hardware estimates with two
-          // Newton steps each (rsqrt, reciprocal) are accurate to ~1 ulp here
-          const double rq = rsqrt_nr(q);
-          t = 2.0 * ab * copysign(1.0, dd) * rcp_nr(fabs(dd) + q * rq);
-        } else {
-          t = 2.0 * ab * copysign(1.0, dd) / (fabs(dd) + sqrt(q));
-        }
-        c = rsqrt_nr(1.0 + t * t);  // 1 + t^2 in [1, 2]
-        s = c * t;
-      }
-      pc[sl] = c;
-      ps[sl] = s;
-    }
-    if (ir > 0) {
-      double va[4], vb[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        va[i] = V[lane][aV[i]];
-        vb[i] = V[lane][bV[i]];
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (sV[i] == 0.0) continue;
-        V[lane][aV[i]] = cV[i] * va[i] - sV[i] * vb[i];
-        V[lane][bV[i]] = sV[i] * va[i] + cV[i] * vb[i];
-      }
-    }
-    __syncthreads();
-    // ---- phase 2: all loads first, the rotations applied unconditionally (an
-    // identity rotation reproduces its operands exactly), stores only where a
-    // rotation acts
-    const double c = pc[sl], s = ps[sl];
-    visit_rot = visit_rot || __any(s != 0.0);
-    const bool nxt = ir + 1 < IR;
-    double cc[2], scc[2], g[2][4];
-    int ac[2], bc[2];
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int kc = 2 * wave + 16 * j + h;
-      cc[j] = pc[kc];
-      scc[j] = ps[kc];
-      ac[j] = min(xc[j], yc[j]);
-      bc[j] = max(xc[j], yc[j]);
-      g[j][0] = G[a][ac[j]]; g[j][1] = G[a][bc[j]];
-      g[j][2] = G[b][ac[j]]; g[j][3] = G[b][bc[j]];
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const double t00 = c * g[j][0] - s * g[j][2], t01 = c * g[j][1] - s * g[j][3];
-      const double t10 = s * g[j][0] + c * g[j][2], t11 = s * g[j][1] + c * g[j][3];
-      g[j][0] = t00 * cc[j] - t01 * scc[j];
-      g[j][1] = t00 * scc[j] + t01 * cc[j];
-      g[j][2] = t10 * cc[j] - t11 * scc[j];
-      g[j][3] = t10 * scc[j] + t11 * cc[j];
-      if (s != 0.0 || scc[j] != 0.0) {
-        G[a][ac[j]] = g[j][0]; G[a][bc[j]] = g[j][1];
-        G[b][ac[j]] = g[j][2]; G[b][bc[j]] = g[j][3];
-      }
-      if (nxt) {
-        const int kc = 2 * wave + 16 * j + h;
-        if (kc == sl) {  // diagonal block: G(a,a), G(b,b)
-          pdiag[buf ^ 1][a] = g[j][0];
-          pdiag[buf ^ 1][b] = g[j][3];
-        }
-        // next-round pairs (xA, yA) and (xB, yB): the thread holding row min and
-        // column max of a pair stores its entry
-        if (xA < yA && (yA == ac[j] || yA == bc[j]))
-          pab[buf ^ 1][snA] = xA == a ? (yA == ac[j] ? g[j][0] : g[j][1]) : (yA == ac[j] ? g[j][2] : g[j][3]);
-        if (xB < yB && (yB == ac[j] || yB == bc[j]))
-          pab[buf ^ 1][snB] = xB == a ? (yB == ac[j] ? g[j][0] : g[j][1]) : (yB == ac[j] ? g[j][2] : g[j][3]);
-      }
-    }
-    // this round's V rotation is applied in phase 1 of the next round
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      cV[i] = pc[qs[i]];
-      sV[i] = ps[qs[i]];
-      aV[i] = min(xv[i], yv[i]);
-      bV[i] = max(xv[i], yv[i]);
-      xv[i] = adv(xv[i]);
-      yv[i] = adv(yv[i]);
-    }
-    xA = adv(xA); xB = adv(xB); yA = adv(yA); yB = adv(yB);
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      xc[j] = adv(xc[j]);
-      yc[j] = adv(yc[j]);
-    }
-    __syncthreads();
-  }
-  {
-    double va[4], vb[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      va[i] = V[lane][aV[i]];
-      vb[i] = V[lane][bV[i]];
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (sV[i] == 0.0) continue;
-      V[lane][aV[i]] = cV[i] * va[i] - sV[i] * vb[i];
-      V[lane][bV[i]] = sV[i] * va[i] + cV[i] * vb[i];
-    }
-  }
-  __syncthreads();
+  inner_sweep(G, V, tol2, tid, &vrot);
   if (tid == 0) {
     t2 = wall_clock64();
     if (S.info) {
@@ -363,7 +370,7 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
       atomicAdd(reinterpret_cast<unsigned*>(S.info) + 4, 1u);
     }
   }
-  if (!visit_rot) return;
+  if (!vrot) return;
   if (tid == 0) {
     flags[(sweep & 1) * njobs + lo] = 1;
     if (S.info) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 5, 1u);

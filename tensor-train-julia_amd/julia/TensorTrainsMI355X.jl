@@ -5,7 +5,8 @@ Julia binding of libttround (include/ttround.h): the MI355X rounding path of
 TensorTrains behind `ccall`.  Loading this module after `TensorTrains` adds
 GPU methods with the reference's names and keyword defaults
 (src/tt_rounding.jl:146-167, src/tt_randtools.jl:112-283,
-src/sketches.jl:175-315,1047-1070, src/tt_operations.jl:81-112,170-194,252-258,
+src/sketches.jl:175-315,317-472,645-817,930-1001,1047-1070,
+src/tt_operations.jl:81-112,170-194,252-258,
 302-421); the
 solver callers (tt_gmres, arnoldi, davidson, als/mals/dmrg via orthogonalize)
 need no change.  Every call uploads the cores (column-major (n, r, r') arrays,
@@ -21,7 +22,7 @@ module TensorTrainsMI355X
 
 using TensorTrains
 import TensorTrains: tt_rounding, tt_rounding!, orthogonalize, ttrand_rounding, tt_recursive_sketch,
-                     dot_par, dot_operator, partial_contraction
+                     tt_sketch, tt_combined_sketch, dot_par, dot_operator, partial_contraction
 import LinearAlgebra: dot
 
 const lib = get(ENV, "TTROUND_LIB", joinpath(@__DIR__, "..", "ttround", "libttround.so"))
@@ -204,5 +205,121 @@ ttrand_rounding(y::TTvector{Float64,N}, rmax::Int; kwargs...) where {N} =
     ttrand_rounding(y, vcat(1, fill(rmax, N - 1), 1); kwargs...)
 ttrand_rounding(α::Vector{Float64}, y::Vector{TTvector{Float64,N}}, rmax::Int; kwargs...) where {N} =
     ttrand_rounding(α, y, vcat(1, fill(rmax, N - 1), 1); kwargs...)
+
+# ------------------------------------------------------------------ sketches (reverse = true only)
+
+# block ranks of the reverse sketch (src/sketches.jl:186-191)
+function _block_ranks(dims, block_rks::Int, N::Int)
+    bv = ones(Int, N + 1)
+    bv[1:N] .= block_rks
+    for k = N:-1:1
+        bv[k] = min(bv[k], dims[k] * bv[k + 1])
+    end
+    return bv
+end
+
+# tt_recursive_sketch(T, [H,] A, rks; reverse=true, ...) (src/sketches.jl:175-240,300-303,317-459)
+function _recursive_sketch(::Type{T}, H, A::TTvector{Float64,N}, rks::Vector{Int}; orthogonal=true, reverse=true,
+                           seed=1234, block_rks::Int=N, p=0, oversampling=1) where {T,N}
+    reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
+    bv = _block_ranks(A.ttv_dims, block_rks, N)
+    pp = p == 0 ? TensorTrains.compute_sketch_blocks_heuristic(rks, bv, N; reverse=true) : collect(Int, p)
+    oversampling != 1 && (pp[1:N] .= ceil.(Int, oversampling .* pp[1:N]))
+    s = bv .* pp
+    Hr = H === nothing ? ones(Int, N + 1) : collect(Int, H.tto_rks)
+    W = [zeros(Float64, A.ttv_rks[k] * Hr[k], s[k]) for k in 1:N+1]
+    id = upload(A)
+    op = H === nothing ? -1 : upload(H)
+    r, pv, srk = collect(Int64, rks), collect(Int64, pp), zeros(Int64, N + 1)
+    GC.@preserve W r pv begin
+        ptrs = [pointer(w) for w in W]
+        opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[T], block_rks, UInt64(seed), pointer(pv), 1.0))
+        if H === nothing
+            check(ccall((:ttr_sketch, lib), Cint,
+                        (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{RandOpts}, Ptr{Ptr{Float64}}, Ptr{Int64}),
+                        handle(), id, r, opts, ptrs, srk))
+        else
+            check(ccall((:ttr_sketch_op, lib), Cint,
+                        (Ptr{Cvoid}, Int64, Int64, Ptr{Int64}, Ref{RandOpts}, Ptr{Ptr{Float64}}, Ptr{Int64}),
+                        handle(), op, id, r, opts, ptrs, srk))
+        end
+    end
+    free(id)
+    op >= 0 && free(op)
+    H === nothing && return W, collect(Int, srk)
+    return [reshape(W[k], A.ttv_rks[k], Hr[k], s[k]) for k in 1:N+1], collect(Int, srk)
+end
+
+_rks_of(rmax::Int, N) = (r = fill(rmax, N + 1); r[N + 1] = 1; r)
+_rks_of(rks::Vector{Int}, N) = rks
+
+tt_recursive_sketch(::Type{T}, A::TTvector{Float64,N}, rks_or_rmax; kwargs...) where {T<:Union{Float64,Float32},N} =
+    _recursive_sketch(T, nothing, A, _rks_of(rks_or_rmax, N); kwargs...)
+tt_recursive_sketch(::Type{T}, H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax;
+                    kwargs...) where {T<:Union{Float64,Float32},N} =
+    _recursive_sketch(T, H, A, _rks_of(rks_or_rmax, N); kwargs...)
+tt_recursive_sketch(A::TTvector{Float64,N}, rks_or_rmax; kwargs...) where {N} =
+    tt_recursive_sketch(Float64, A, rks_or_rmax; kwargs...)
+tt_recursive_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax; kwargs...) where {N} =
+    tt_recursive_sketch(Float64, H, A, rks_or_rmax; kwargs...)
+
+# tt_sketch(T, [H,] A, s; reverse=true, ...) (src/sketches.jl:645-817): boundary sketch only
+function _tt_sketch(::Type{T}, H, A::TTvector{Float64,N}, s::Int; reverse=true, orthogonal=true, seed=1234,
+                    block_rks::Int=N) where {T,N}
+    reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
+    id = upload(A)
+    op = H === nothing ? -1 : upload(H)
+    sa = Ref{Int64}(0)
+    ids, ops = Int64[id], Int64[op]
+    opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[T], block_rks, UInt64(seed), C_NULL, 1.0))
+    check(ccall((:ttr_tt_sketch, lib), Cint,
+                (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ref{RandOpts}, Ptr{Float64}, Int64, Ref{Int64}),
+                handle(), 1, ids, ops, s, opts, C_NULL, 0, sa))
+    out = zeros(Float64, sa[])
+    check(ccall((:ttr_tt_sketch, lib), Cint,
+                (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ref{RandOpts}, Ptr{Float64}, Int64, Ref{Int64}),
+                handle(), 1, ids, ops, s, opts, out, length(out), sa))
+    free(id)
+    op >= 0 && free(op)
+    return out, Int(sa[])
+end
+
+tt_sketch(::Type{T}, A::TTvector{Float64,N}, s::Int; kwargs...) where {T<:Union{Float64,Float32},N} =
+    _tt_sketch(T, nothing, A, s; kwargs...)
+tt_sketch(::Type{T}, H::TToperator{Float64,N}, A::TTvector{Float64,N}, s::Int;
+          kwargs...) where {T<:Union{Float64,Float32},N} = _tt_sketch(T, H, A, s; kwargs...)
+tt_sketch(A::TTvector{Float64,N}, s::Int; kwargs...) where {N} = tt_sketch(Float64, A, s; kwargs...)
+tt_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, s::Int; kwargs...) where {N} =
+    tt_sketch(Float64, H, A, s; kwargs...)
+
+# tt_combined_sketch(T, [H,] A, rks_or_rmax, s; reverse=true, ...) (src/sketches.jl:930-1001)
+function _combined_sketch(::Type{T}, H, A::TTvector{Float64,N}, rks::Vector{Int}, s::Int; reverse=true,
+                          orthogonal=true, seed=1234, block_rks::Int=N, timer=nothing) where {T,N}
+    reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
+    rks = copy(rks)
+    rks[1] = max(rks[1], min(s, rks[2]))
+    W, sketch_rks = _recursive_sketch(T, H, A, rks; orthogonal=orthogonal, seed=seed, block_rks=block_rks)
+    sketch = vec(W[1])
+    s_rec = sketch_rks[1]
+    if s > s_rec
+        extra, _ = _tt_sketch(T, H, A, s - s_rec; orthogonal=orthogonal, seed=seed + 1, block_rks=block_rks)
+        s = s_rec + length(extra)
+        sketch = [sqrt(length(sketch) / s) .* sketch; sqrt(length(extra) / s) .* extra]
+    end
+    W[1] = H === nothing ? ones(Float64, 1, 1) : ones(Float64, 1, 1, 1)
+    sketch_rks[1] = 1
+    return sketch, W, sketch_rks
+end
+
+tt_combined_sketch(::Type{T}, A::TTvector{Float64,N}, rks_or_rmax, s::Int;
+                   kwargs...) where {T<:Union{Float64,Float32},N} =
+    _combined_sketch(T, nothing, A, _rks_of(rks_or_rmax, N), s; kwargs...)
+tt_combined_sketch(::Type{T}, H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax, s::Int;
+                   kwargs...) where {T<:Union{Float64,Float32},N} =
+    _combined_sketch(T, H, A, _rks_of(rks_or_rmax, N), s; kwargs...)
+tt_combined_sketch(A::TTvector{Float64,N}, rks_or_rmax, s::Int; kwargs...) where {N} =
+    tt_combined_sketch(Float64, A, rks_or_rmax, s; kwargs...)
+tt_combined_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax, s::Int; kwargs...) where {N} =
+    tt_combined_sketch(Float64, H, A, rks_or_rmax, s; kwargs...)
 
 end # module

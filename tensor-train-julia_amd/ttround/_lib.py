@@ -78,6 +78,7 @@ SIGNATURES = {
     "ttr_round_rand": (C.c_int, [C.c_void_p, i64, C.POINTER(RandItem), C.POINTER(RandOpts), i64p]),
     "ttr_sketch": (C.c_int, [C.c_void_p, i64, i64p, C.POINTER(RandOpts), f64pp, i64p]),
     "ttr_sketch_op": (C.c_int, [C.c_void_p, i64, i64, i64p, C.POINTER(RandOpts), f64pp, i64p]),
+    "ttr_tt_sketch": (C.c_int, [C.c_void_p, i64, i64p, i64p, i64, C.POINTER(RandOpts), f64p, i64, i64p]),
     "ttr_sketch_blocks": (C.c_int, [C.c_void_p, i64, i64p, i64p, C.POINTER(RandOpts), i64, f64p, i64p]),
     "ttr_sketch_draws": (C.c_int, [C.c_void_p, i64, i64, C.c_uint64, f64p]),
     "ttr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),

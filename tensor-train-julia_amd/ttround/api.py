@@ -275,6 +275,25 @@ def block_structure(dims, rks, block_rks: Optional[int] = None, p=None, oversamp
     return bv, pp
 
 
+def tt_sketch_batch(xs: Sequence[DeviceTT], s: int, ops=None, orthogonal: bool = True, seed: int = 1234,
+                    block_rks: Optional[int] = None, sketch_prec: str = "f64"):
+    """Boundary-only reverse sketches of a batch (``tt_sketch``, ``src/sketches.jl:645-813``)
+    in one library call: item ``i`` is ``xs[i]``, or ``ops[i] * xs[i]`` (never formed)
+    when ``ops[i]`` is a ``DeviceTTO``.  All items share the draws of ``seed``.
+    Returns ``(S, s_actual)`` with ``S[:, i]`` the sketch of item ``i``."""
+    h = xs[0].handle
+    ids = L.as_i64([x.id for x in xs])
+    oids = L.as_i64([-1 if ops is None or ops[i] is None else ops[i].id for i in range(len(xs))])
+    o, keep = _opts(orthogonal, sketch_prec, block_rks, seed)
+    sa = np.zeros(1, np.int64)
+    h.check(h.lib.ttr_tt_sketch(h.h, len(xs), L.ptr_i64(ids), L.ptr_i64(oids), int(s), C.byref(o),
+                                C.cast(None, L.f64p), 0, L.ptr_i64(sa)))
+    out = np.zeros((int(sa[0]), len(xs)), order="F")
+    h.check(h.lib.ttr_tt_sketch(h.h, len(xs), L.ptr_i64(ids), L.ptr_i64(oids), int(s), C.byref(o),
+                                L.ptr_f64(out), out.size, L.ptr_i64(sa)))
+    return out, int(sa[0])
+
+
 def sketch_blocks(dims, rks, k: int, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
                   sketch_prec: str = "f64", p=None, oversampling: float = 1.0,
                   handle: Optional[L.Handle] = None) -> np.ndarray:
@@ -594,3 +613,64 @@ def tt_recursive_sketch(*args, orthogonal: bool = True, reverse: bool = True, se
     if H is not None:
         Ws = [w.reshape((A.ttv_rks[k], Hr[k], srk[k]), order="F") for k, w in enumerate(Ws)]
     return Ws, srk
+
+
+def _split_op(args):
+    if isinstance(args[0], TToperator):
+        return args[0], args[1:]
+    return None, args
+
+
+def tt_sketch(*args, reverse: bool = True, orthogonal: bool = True, seed: int = 1234,
+              block_rks: Optional[int] = None, sketch_prec: str = "f64"):
+    """``tt_sketch([H,] A, s; reverse=true, orthogonal, seed, block_rks)``
+    (``src/sketches.jl:645-727``; operator form ``:729-817``, ``H A`` never formed).
+
+    Returns ``(sketch, s_actual)``: the boundary sketch of length
+    ``s_actual = bv_1 * cld(s, bv_1)``."""
+    H, (A, s) = _split_op(args)
+    if not reverse:
+        raise NotImplementedError("only the reverse (right-to-left) sketch is provided")
+    if H is not None and tuple(H.tto_dims) != tuple(A.ttv_dims):
+        raise L.DimensionMismatch("operator and TT dims differ")
+    dev = A._dev()
+    dop = DeviceTTO(H, dev.handle) if H is not None else None
+    try:
+        out, sa = tt_sketch_batch([dev], s, [dop] if dop is not None else None, orthogonal, seed, block_rks,
+                                  sketch_prec)
+    finally:
+        dev.free()
+        if dop is not None:
+            dop.free()
+    return out[:, 0].copy(), sa
+
+
+def tt_combined_sketch(*args, reverse: bool = True, orthogonal: bool = True, seed: int = 1234,
+                       block_rks: Optional[int] = None, sketch_prec: str = "f64"):
+    """``tt_combined_sketch([H,] A, rks_or_rmax, s; reverse=true, ...)``
+    (``src/sketches.jl:930-1001``): the recursive sketch with the boundary rank
+    raised to ``max(rks_1, min(s, rks_2))``, its boundary sketch extended to at
+    least ``s`` entries by ``tt_sketch`` with ``seed + 1`` (each part weighted
+    by ``sqrt(len/s)``).  Returns ``(sketch, W, sketch_rks)`` where ``W`` is the
+    recursive sketch with ``W[1]`` replaced by ones and ``sketch_rks[1] = 1``."""
+    H, (A, rks, s) = _split_op(args)
+    if not reverse:
+        raise NotImplementedError("only the reverse (right-to-left) sketch is provided")
+    N = A.N
+    if isinstance(rks, (int, np.integer)):
+        rks = [int(rks)] * (N + 1)
+        rks[N] = 1
+    rks = [int(v) for v in rks]
+    rks[0] = max(rks[0], min(int(s), rks[1]))
+    pre = [H] if H is not None else []
+    kw = dict(orthogonal=orthogonal, block_rks=block_rks, sketch_prec=sketch_prec)
+    W, srk = tt_recursive_sketch(*pre, A, rks, reverse=True, seed=seed, **kw)
+    sketch = np.reshape(W[0], -1, order="F").copy()
+    s_rec = srk[0]
+    if s > s_rec:
+        extra, _ = tt_sketch(*pre, A, int(s) - s_rec, seed=seed + 1, **kw)
+        s = s_rec + len(extra)
+        sketch = np.concatenate([math.sqrt(len(sketch) / s) * sketch, math.sqrt(len(extra) / s) * extra])
+    W[0] = np.ones((1, 1)) if H is None else np.ones((1, 1, 1))
+    srk[0] = 1
+    return sketch, W, srk

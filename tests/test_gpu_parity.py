@@ -531,3 +531,84 @@ def test_inner_product_errors(T):
         T.dot(to_api(x), to_api(y))
     dx.free()
     dy.free()
+
+
+# ------------------------------------------------------------------ boundary sketches (Krylov callers)
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6), ("bf16", 3e-2)])
+@pytest.mark.parametrize("orthogonal,block_rks", [(False, None), (True, 3)])
+def test_tt_sketch_batch_vs_oracle(T, prec, tol, orthogonal, block_rks):
+    """tt_sketch of a batch mixing plain TTs and implicit H x products (one
+    library call) vs the oracle on the device's blocks (same p at every core)."""
+    say(f"case: tt_sketch batch {prec} orthogonal={orthogonal} block_rks={block_rks}")
+    H = O.hubbard_1d_mpo(5, 1.0, 4.0)
+    dims, N = H.dims, 5
+    xs = [rand_tt(dims, [1, 3, 5, 6, 4, 1], 130 + i) for i in range(4)]
+    ops = [None, H, None, H]
+    s = 20
+    dx = [T.DeviceTT.upload(to_api(x)) for x in xs]
+    dH = T.DeviceTTO(to_api_op(H))
+    S, sa = T.tt_sketch_batch(dx, s, [None if o is None else dH for o in ops], orthogonal=orthogonal, seed=8,
+                              block_rks=block_rks, sketch_prec=prec)
+    bv, _ = O.sketch_structure(dims, [1] * (N + 1), block_rks)
+    p = -(-s // bv[0])
+    prov = device_blocks_provider(dims, [1] * (N + 1), orthogonal=orthogonal, seed=8, block_rks=block_rks,
+                                  sketch_prec=prec, p=[p] * N + [1])
+    assert S.shape == (sa, 4) and sa == bv[0] * p
+    for i, x in enumerate(xs):
+        ref, sr = O.tt_sketch(x, s, prov, orthogonal, block_rks, H=ops[i])
+        assert sr == sa
+        err = np.linalg.norm(S[:, i] - ref) / np.linalg.norm(ref)
+        say(f"  item {i} ({'H x' if ops[i] is not None else 'x'}): rel err {err:.3e}")
+        assert err <= tol
+    for d in dx + [dH]:
+        d.free()
+
+
+@pytest.mark.parametrize("op", [False, True])
+@pytest.mark.parametrize("s", [8, 30])
+def test_tt_combined_sketch_vs_oracle(T, op, s):
+    """tt_combined_sketch(H?, x, rmax, s): s = 8 fits the recursive sketch's
+    boundary rank, s = 30 needs the extra tt_sketch drawn from seed + 1."""
+    say(f"case: tt_combined_sketch op={op} s={s}")
+    H = O.hubbard_1d_mpo(5, 1.0, 4.0)
+    dims, N = H.dims, 5
+    x = rand_tt(dims, [1, 3, 5, 6, 4, 1], 140)
+    rmax = 6
+    args = ([to_api_op(H)] if op else []) + [to_api(x), rmax, s]
+    sk, W, srk = T.tt_combined_sketch(*args, orthogonal=False, seed=9)
+    rks0 = [rmax] * (N + 1)
+    rks0[N] = 1
+    rks = list(rks0)
+    rks[0] = max(rks[0], min(s, rks[1]))
+    prov = device_blocks_provider(dims, rks, orthogonal=False, seed=9)
+    bv, pr = O.sketch_structure(dims, rks, None)
+    bvs, _ = O.sketch_structure(dims, [1] * (N + 1), None)
+    pe = max(1, -(-(s - bv[0] * pr[0]) // bvs[0]))
+    prov_e = device_blocks_provider(dims, [1] * (N + 1), orthogonal=False, seed=10, p=[pe] * N + [1])
+    ref, Wr, sr = O.tt_combined_sketch(x, rks0, s, prov, prov_e, False, None, H=H if op else None)
+    assert srk == sr and len(sk) == len(ref) and len(sk) >= s
+    assert np.linalg.norm(sk - ref) <= 1e-12 * np.linalg.norm(ref)
+    for a, b in zip(W, Wr):
+        assert a.shape == b.shape
+        assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b)
+
+
+def test_tt_sketch_errors(T):
+    say("case: tt_sketch argument errors")
+    import ctypes as C
+    from ttround import _lib as L
+    x, y = rand_tt((2, 3, 2), [1, 2, 2, 1], 471), rand_tt((3, 2, 2), [1, 2, 2, 1], 472)
+    dx, dy = T.DeviceTT.upload(to_api(x)), T.DeviceTT.upload(to_api(y))
+    with pytest.raises(T.DimensionMismatch):
+        T.tt_sketch_batch([dx, dy], 4)
+    h = dx.handle
+    o, _ = T.api._opts(False)
+    ids = L.as_i64([dx.id])
+    sa = np.zeros(1, np.int64)
+    out = np.zeros(1)
+    st = h.lib.ttr_tt_sketch(h.h, 1, L.ptr_i64(ids), C.cast(None, L.i64p), 4, C.byref(o), L.ptr_f64(out), 1,
+                             L.ptr_i64(sa))
+    assert st == L.TTR_INVALID_ARG and sa[0] >= 4
+    dx.free()
+    dy.free()

@@ -12,6 +12,10 @@ a wrong index or a transposed operand fails:
   tt_rounding         -> dense TT-SVD sweep on unfoldings (Oseledets 2011) + closed forms
   generate_sketch_blocks -> scaling constants, identity case, Q^T Q = c I and Q^T G upper triangular
   tt_recursive_sketch -> dense X_{k:N} Omega_{k:N} brute force; p=1 full sketch = dot
+  tt_sketch           -> dense vec(x)^T Omega_{1:N} with constant p (TT and operator forms);
+                         E|sketch|^2 = |x|^2 over seeds
+  tt_combined_sketch  -> parts equal the recursive sketch's W_1 and tt_sketch(seed + 1) with
+                         the sqrt(len/s) weights; E|sketch|^2 = |x|^2 over seeds
   ttrand_rounding*    -> exact recovery when the sketch rank >= true rank (test_ttrand.jl:67-88)
   hubbard_1d_mpo      -> dense fermionic Hamiltonian
   dot, dot_par, partial_contraction, dot_operator
@@ -334,6 +338,88 @@ def test_full_sketch_with_p1_is_dot():
     om = O.TT([O.generate_sketch_blocks(draws[k], 1, 2, 1, 1, False)[:, :, :, 0].transpose(1, 0, 2) for k in range(4)],
               dims, [1] * 5)
     assert abs(W[0][0, 0] - O.tt_dot(x, om)) < 1e-12 * abs(O.tt_dot(x, om)) + 1e-14
+
+
+def _rand_tto(dims, rks, seed):
+    g = np.random.default_rng(seed)
+    return O.TTO([g.standard_normal((n, n, rks[k], rks[k + 1])) for k, n in enumerate(dims)], dims, rks)
+
+
+@pytest.mark.parametrize("orthogonal,block_rks,op", [(False, None, False), (True, None, False), (False, 2, True),
+                                                     (True, 3, True)])
+def test_tt_sketch_dense(orthogonal, block_rks, op):
+    # tt_sketch (sketches.jl:645-813): vec(x)^T Omega_{1:N} / sqrt(p), the dense Omega
+    # built from the block chains with the same p at every core
+    dims = (2, 3, 2, 2, 3)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 4, 5, 3, 1], 23)
+    H = _rand_tto(dims, [1, 2, 3, 2, 2, 1], 24) if op else None
+    target = O.mpo_apply(H, x) if op else x
+    s = 7
+    bv, _ = O.sketch_structure(dims, [1] * (N + 1), block_rks)
+    p = -(-s // bv[0])
+    draws = [RNG.standard_normal(bv[k + 1] * dims[k] * bv[k] * p) for k in range(N)]
+    got, sa = O.tt_sketch(x, s, draws, orthogonal, block_rks, H=H)
+    assert sa == bv[0] * p and got.shape == (sa,) and sa >= s
+    Sb = [O.generate_sketch_blocks(draws[k], bv[k + 1], dims[k], bv[k], p, orthogonal) for k in range(N)]
+    Om = _dense_sketch(Sb, dims, bv, [p] * N + [1], 0)
+    ref = (_dense_right(target, 0) @ Om)[0] / math.sqrt(p)
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_tt_sketch_and_combined_sketch_are_unbiased():
+    # Gaussian blocks scaled by 1/sqrt(bv_k), boundary divided by sqrt(p):
+    # E |sketch|^2 = |x|^2; the combined sketch keeps that only with the
+    # sqrt(len/s) weights of sketches.jl:947-948
+    dims = (2, 2, 3, 2)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 3, 2, 1], 25)
+    nx2 = O.tt_norm(x) ** 2
+    g = np.random.default_rng(99)
+    rks = [1, 3, 3, 3, 1]
+    acc_s, acc_c, trials = 0.0, 0.0, 600
+    bvr, pr = O.sketch_structure(dims, [max(rks[0], min(9, rks[1]))] + rks[1:], None)
+    bvs, _ = O.sketch_structure(dims, [1] * (N + 1), None)
+    for _ in range(trials):
+        dr = [g.standard_normal(bvr[k + 1] * dims[k] * bvr[k] * pr[k]) for k in range(N)]
+        de = [g.standard_normal(bvs[k + 1] * dims[k] * bvs[k] * 4) for k in range(N)]
+        sk, _ = O.tt_sketch(x, 5, de, False)
+        acc_s += float(sk @ sk)
+        cs, W, srk = O.tt_combined_sketch(x, rks, 9, dr, de, False)
+        assert len(cs) >= 9 and srk[0] == 1 and W[0].shape == (1, 1)
+        acc_c += float(cs @ cs)
+    assert abs(acc_s / trials / nx2 - 1.0) < 0.15
+    assert abs(acc_c / trials / nx2 - 1.0) < 0.15
+
+
+def test_tt_combined_sketch_parts():
+    dims = (2, 3, 2, 2)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 4, 3, 1], 26)
+    rks = [1, 3, 4, 3, 1]
+    # s within the recursive sketch's boundary rank: no extra sketch
+    rk1 = [max(1, min(2, rks[1]))] + rks[1:]
+    bv, p = O.sketch_structure(dims, rk1, None)
+    dr = [RNG.standard_normal(bv[k + 1] * dims[k] * bv[k] * p[k]) for k in range(N)]
+    cs, W, srk = O.tt_combined_sketch(x, rks, 2, dr, None, False)
+    Wr, sr = O.tt_recursive_sketch(x, rk1, dr, False)
+    np.testing.assert_array_equal(cs, O.jreshape(Wr[0], -1))
+    assert srk == [1] + sr[1:] and all(np.array_equal(a, b) for a, b in zip(W[1:], Wr[1:]))
+    # s above it: [sqrt(l/s) W_1 ; sqrt(le/s) tt_sketch(seed + 1)]
+    s = 11
+    rk2 = [max(1, min(s, rks[1]))] + rks[1:]
+    bv, p = O.sketch_structure(dims, rk2, None)
+    dr = [RNG.standard_normal(bv[k + 1] * dims[k] * bv[k] * p[k]) for k in range(N)]
+    bvs, _ = O.sketch_structure(dims, [1] * (N + 1), None)
+    Wr, sr = O.tt_recursive_sketch(x, rk2, dr, False)
+    pe = -(-(s - sr[0]) // bvs[0])
+    de = [RNG.standard_normal(bvs[k + 1] * dims[k] * bvs[k] * pe) for k in range(N)]
+    ex, le = O.tt_sketch(x, s - sr[0], de, False)
+    cs, W, srk = O.tt_combined_sketch(x, rks, s, dr, de, False)
+    tot = sr[0] + le
+    assert len(cs) == tot and tot >= s
+    np.testing.assert_allclose(cs[: sr[0]], math.sqrt(sr[0] / tot) * O.jreshape(Wr[0], -1), rtol=1e-15)
+    np.testing.assert_allclose(cs[sr[0]:], math.sqrt(le / tot) * ex, rtol=1e-15)
 
 
 # ---------------------------------------------------------------- randomized rounding
