@@ -448,7 +448,7 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     }
     copy2d_batched(ctx, cp);
     zero_fill(ctx, info_all, sizeof(int32_t) * 8 * nb);
-    jacobi_svd_batched(ctx, sj);
+    const double wc = jacobi_svd_batched(ctx, sj);
     // rank decisions on the host
     ctx.sync();
     TTR_HIP(hipMemcpy(sv_host.data(), sv_all, sv_host.size() * 8, hipMemcpyDeviceToHost));
@@ -472,11 +472,11 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
         ps.bytes = std::max(ps.bytes, double(info[0]));
         // visits and wall-clock ticks (100 MHz) summed over visits: Gram, inner sweep, apply
         ctx.prof.stats["jacobi_svd:visits"].flops += info[4];
-        // MFMA work: every visit forms a 64-column Gram (2*64*64*m flops), a rotating
-        // visit applies V to 64 columns of B and of J (2 * 2*m*64*64); bytes: the
+        // MFMA work: every visit forms a wc-column Gram (2*wc*wc*m flops), a rotating
+        // visit applies V to wc columns of B and of J (2 * 2*m*wc*wc); bytes: the
         // columns read by the Gram, read and written by the apply
-        ctx.prof.stats["jacobi_svd"].flops += 8192.0 * m * (double(info[4]) + 2.0 * info[5]);
-        ctx.prof.stats["jacobi_svd"].bytes += 512.0 * m * (double(info[4]) + 4.0 * info[5]);
+        ctx.prof.stats["jacobi_svd"].flops += 2.0 * wc * wc * m * (double(info[4]) + 2.0 * info[5]);
+        ctx.prof.stats["jacobi_svd"].bytes += 8.0 * wc * m * (double(info[4]) + 4.0 * info[5]);
         ctx.prof.stats["jacobi_svd:ticks_gram"].flops += info[1];
         ctx.prof.stats["jacobi_svd:ticks_inner"].flops += info[2];
         ctx.prof.stats["jacobi_svd:ticks_apply"].flops += info[3];

@@ -1,6 +1,6 @@
 // Batched one-sided (Hestenes) Jacobi SVD in FP64, block variant.
 //
-// Columns of each p x p matrix B are grouped in blocks of 32.  A sweep visits
+// Columns of each p x p matrix B are grouped in blocks of 32 (or 16).  A sweep visits
 // every pair of blocks once in round-robin order: each round is one launch in
 // which every disjoint block pair of every matrix is one workgroup, so a large
 // matrix is worked on by many CUs at once and a batch of uneven matrices keeps
@@ -9,10 +9,10 @@
 //      block pair in LDS with FP64 MFMA, straight from the current columns
 //      (rotation angles always come from fresh inner products, as in plain
 //      one-sided Jacobi);
-//   2. runs one parallel cyclic Jacobi sweep on G in LDS (32 disjoint rotations
-//      per round, round-robin order; each 2x2 block of G and each row pair of V
-//      is updated in place by one thread), accumulating the rotation V; a pair
-//      is rotated when |G_ij| > tol sqrt(G_ii G_jj), tol = p eps;
+//   2. runs one parallel cyclic Jacobi sweep on G (32 disjoint rotations per
+//      round, round-robin order, G in LDS and the accumulated rotation V in
+//      registers; see inner_sweep); a pair is rotated when
+//      |G_ij| > tol sqrt(G_ii G_jj), tol = p eps;
 //   3. if anything rotated, applies V to the visited columns of B and of the
 //      accumulated right rotations J with FP64 MFMA (C <- C V, 32-row chunks
 //      staged in LDS) and marks the matrix as not yet converged.
@@ -27,14 +27,26 @@ namespace {
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-constexpr int JT = 512;         // threads per visit (8 waves)
-constexpr int WB = 32;          // column block width
-constexpr int WC = 2 * WB;      // columns per visit
-constexpr int GLD = WC + 1;     // padded LDS row
 constexpr int RCH = 32;         // row chunk for Gram / apply
-constexpr int NPAIR = WC / 2;
-constexpr int IR = WC - 1;      // rounds of one inner sweep
 constexpr int MAX_SWEEPS = 40;
+
+// Shape of a visit for column blocks of width WB_ (16 or 32)
+template <int WB_>
+struct Jac {
+  static constexpr int WB = WB_;             // column block width
+  static constexpr int WC = 2 * WB;          // columns per visit
+  static constexpr int GLD = WC + 1;         // padded LDS row
+  static constexpr int JT = 16 * WB;         // threads per visit
+  static constexpr int NW = JT / 64;         // waves
+  static constexpr int NPAIR = WB;           // rotations per inner round
+  static constexpr int IR = WC - 1;          // rounds of one inner sweep
+  static constexpr int NT = WC / 16;         // 16x16 MFMA tiles per Gram row
+  static constexpr int TPW = NT * NT / NW;   // Gram tiles per wave
+  static constexpr int GBLK = WB / 16;       // 2x2 blocks of G per thread in the inner sweep
+  static constexpr int VROWS = WB / 8;       // rows of V per thread
+  static constexpr int PER = RCH * WC / JT;  // staged chunk elements per thread
+  static constexpr int MINB = WB == 16 ? 4 : 2;  // resident visits per CU the registers allow
+};
 
 struct JacJob {
   double* B;
@@ -69,178 +81,148 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return y;
 }
 
-// lane 0's value broadcast through scalar registers (the readfirstlane builtin
-// takes a 32-bit int: a double passed to it directly is converted, not copied)
-__device__ __forceinline__ double first_lane(double v) {
-  const unsigned long long b = static_cast<unsigned long long>(__double_as_longlong(v));
-  const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(b & 0xffffffffu)));
-  const unsigned hi = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32)));
-  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+// Cross-lane moves of a double on the VALU (DPP): lane i takes lane i - 1's
+// value (wave_shr:1; lane 0 gets 0) or lane i + 1's (wave_shl:1; lane 63 gets 0),
+// and lane l's value in every lane (readlane takes 32-bit operands).
+__device__ __forceinline__ double from_lower_lane(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_upper_lane(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l), __builtin_amdgcn_readlane(__double2loint(v), l));
 }
 
-// ---- one inner Jacobi sweep on G: 63 rounds of 32 disjoint rotations in
-// round-robin order (pair t of round r joins the players at positions t and
-// 63 - t; every player but 0 advances one position per round).  G is
-// symmetric and only its upper triangle is read or written from here on:
-// entry (i, j) lives at G[min(i,j)][max(i,j)].  A round has two phases, each
-// ending with a barrier:
-//   1. lanes 0..31 of wave 0 compute the rotation (c, s) of pair `lane` from
-//      G into pcs[r & 1]; waves 1..7 meanwhile apply the previous round's
-//      rotations to V (row = lane; wave w owns pairs w - 1 + 7k), skipping
-//      identity rotations (their parameters are wave-uniform, read by one
-//      lane and broadcast through scalar registers);
-//   2. every thread rotates its upper-triangle 2x2 block(s) of G in place:
-//      threads 0..495 one off-diagonal block (pair positions P < Q), threads
-//      496..511 two diagonal blocks (P = Q).  A block whose two rotations
-//      are the identity is neither read nor written.
-// Each thread only touches its own blocks in phase 2, and every block is
-// owned by one thread, so the update is in place.
-__device__ __forceinline__ void inner_sweep(double (*G)[GLD], double (*V)[GLD], double tol2, int tid, int* vrot) {
+// ---- one inner Jacobi sweep on the 64x64 Gram matrix: 63 rounds of 32
+// disjoint rotations in round-robin order.  Pair t of a round joins positions
+// t and 63 - t; after the round every value moves with its player, position p
+// to adv(p) (0 stays, 63 wraps to 1, others p + 1), so a player's data sits at
+// the position it takes in the next round, and after 63 rounds every player is
+// back at its starting position.  Working in these positions keeps every
+// thread's addresses fixed:
+//   * G (full, symmetric) ping-pongs between two LDS buffers: thread (wave w,
+//     lane l) rotates the 2x2 blocks (P, Q_j), P = l & 31, Q_j = 2w + (l >> 5) +
+//     16j, reading them at rows {P, 63-P} x cols {Q, 63-Q} of the current buffer
+//     (32 lanes of a half-wave read 32 different rows of one column: no bank
+//     conflicts with the odd row pitch) and writing them moved into the other;
+//   * V lives in registers: the same thread holds V at rows 2w + (l >> 5) + 16j
+//     (j = 0..3) and positions P, 63 - P; the move is a shift by one lane
+//     within each 32-lane half (DPP, plus the two ends of the round-robin
+//     table), so V never touches LDS during the sweep.
+// A round is two phases, each ending with a barrier: lanes 0..31 of wave 0
+// compute pair t's rotation (c, s) from the current buffer (position t first),
+// then every thread rotates its G blocks and its V rows.  On exit V, in player
+// order again, is stored in GA (GB is free); *vrot is set when any rotation
+// acted.
+template <int WB_>
+__device__ __forceinline__ void inner_sweep(double (*GA)[Jac<WB_>::GLD], double (*GB)[Jac<WB_>::GLD], double tol2,
+                                            int tid, int* vrot) {
+  using C = Jac<WB_>;
+  constexpr int GLD = C::GLD, NPAIR = C::NPAIR, IR = C::IR;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  __shared__ double2 pcs[2][NPAIR];
+  const int P = lane & (NPAIR - 1), slot = tid / NPAIR;
+  __shared__ double2 pcs[NPAIR];
   auto adv = [](int x) { return x == 0 ? 0 : (x == IR ? 1 : x + 1); };
-  // phase-1 players of the parameter lanes
-  int qa = rr_player(lane & (NPAIR - 1), 0, WC), qb = rr_player(IR - (lane & (NPAIR - 1)), 0, WC);
-  // phase-2 blocks
-  int bP[2], bQ[2];
-  const int nbk = tid < 496 ? 1 : 2;
-  if (tid < 496) {
-    int t = tid, P = 0;
-    while (t >= NPAIR - 1 - P) {
-      t -= NPAIR - 1 - P;
-      ++P;
-    }
-    bP[0] = P;
-    bQ[0] = P + 1 + t;
-    bP[1] = bQ[1] = 0;
-  } else {
-    bP[0] = bQ[0] = 2 * (tid - 496);
-    bP[1] = bQ[1] = 2 * (tid - 496) + 1;
+  const int x0 = P, x1 = IR - P;
+  const int ax0 = adv(x0), ax1 = adv(x1);
+  int Q[C::GBLK], y0[C::GBLK], y1[C::GBLK], ay0[C::GBLK], ay1[C::GBLK];
+#pragma unroll
+  for (int j = 0; j < C::GBLK; ++j) {
+    Q[j] = slot + 16 * j;
+    y0[j] = Q[j];
+    y1[j] = IR - Q[j];
+    ay0[j] = adv(y0[j]);
+    ay1[j] = adv(y1[j]);
   }
-  int rx0[2], rx1[2], cx0[2], cx1[2];
+  // V rows of this thread, identity at the start
+  double VL[C::VROWS], VR[C::VROWS];
 #pragma unroll
-  for (int b = 0; b < 2; ++b) {
-    rx0[b] = rr_player(bP[b], 0, WC);
-    rx1[b] = rr_player(IR - bP[b], 0, WC);
-    cx0[b] = rr_player(bQ[b], 0, WC);
-    cx1[b] = rr_player(IR - bQ[b], 0, WC);
+  for (int j = 0; j < C::VROWS; ++j) {
+    const int row = slot + 16 * j;
+    VL[j] = row == x0 ? 1.0 : 0.0;
+    VR[j] = row == x1 ? 1.0 : 0.0;
   }
-  // V pairs of this wave (waves 1..7), players one round behind
-  constexpr int VK = 5;
-  int vt[VK], vx[VK], vy[VK];
-#pragma unroll
-  for (int k = 0; k < VK; ++k) {
-    vt[k] = wave - 1 + 7 * k;
-    const int t = (wave > 0 && vt[k] < NPAIR) ? vt[k] : 0;
-    vx[k] = rr_player(t, 0, WC);
-    vy[k] = rr_player(IR - t, 0, WC);
-  }
-  // (c, s) of this wave's V pairs for the pending V update, read at the end of
-  // the phase 2 that follows their computation
-  double vc[VK], vs[VK];
-#pragma unroll
-  for (int k = 0; k < VK; ++k) {
-    vc[k] = 1.0;
-    vs[k] = 0.0;
-  }
-  auto v_params = [&](const double2* cs) {
-#pragma unroll
-    for (int k = 0; k < VK; ++k) {
-      if (vt[k] < NPAIR) {
-        const double2 cv = cs[vt[k]];
-        vc[k] = cv.x;
-        vs[k] = cv.y;
-      }
-    }
-  };
-  // all loads first, then the rotations, stores only where a rotation acts
-  auto v_update = [&]() {
-    double va[VK], vb[VK];
-    int lo[VK], hi[VK];
-#pragma unroll
-    for (int k = 0; k < VK; ++k) {
-      lo[k] = min(vx[k], vy[k]);
-      hi[k] = max(vx[k], vy[k]);
-      va[k] = V[lane][lo[k]];
-      vb[k] = V[lane][hi[k]];
-    }
-#pragma unroll
-    for (int k = 0; k < VK; ++k) {
-      if (vt[k] < NPAIR && vs[k] != 0.0) {
-        V[lane][lo[k]] = vc[k] * va[k] - vs[k] * vb[k];
-        V[lane][hi[k]] = vs[k] * va[k] + vc[k] * vb[k];
-      }
-      vx[k] = adv(vx[k]);
-      vy[k] = adv(vy[k]);
-    }
-  };
+  // sources of the move (groups of NPAIR lanes): position t takes t - 1 (t >= 2:
+  // the left slot of the lane below), IR (t = 1: the right slot of the group's
+  // first lane, which therefore shifts its right slot instead of its left) or
+  // itself (t = 0); position IR - t takes IR - 1 - t, the right slot of the lane
+  // above, or the own left slot (t = NPAIR - 1).  The lanes that receive a
+  // value from across a group boundary are exactly those that do not use it.
   for (int ir = 0; ir < IR; ++ir) {
-    const int buf = ir & 1;
-    // ---- phase 1
-    if (wave == 0) {
-      if (lane < NPAIR) {
-        const int a = min(qa, qb), b = max(qa, qb);
-        const double aa = G[a][a], bb2 = G[b][b], ab = G[a][b];
-        double c = 1.0, s = 0.0;
-        // padding columns have a zero diagonal and never rotate
-        if (aa > 0.0 && bb2 > 0.0 && ab * ab > tol2 * aa * bb2) {
-          const double dd = bb2 - aa;
-          const double q = dd * dd + 4.0 * ab * ab;
-          double t;
-          if (q > 0x1p-900) {
-            // G is scaled to max diagonal ~1, so q <= ~20: hardware estimates with two
-            // Newton steps each (rsqrt, reciprocal) are accurate to ~1 ulp here
-            const double rq = rsqrt_nr(q);
-            t = 2.0 * ab * copysign(1.0, dd) * rcp_nr(fabs(dd) + q * rq);
-          } else {
-            t = 2.0 * ab * copysign(1.0, dd) / (fabs(dd) + sqrt(q));
-          }
-          c = rsqrt_nr(1.0 + t * t);  // 1 + t^2 in [1, 2]
-          s = c * t;
-          *vrot = 1;
+    double(*cur)[GLD] = (ir & 1) ? GB : GA;
+    double(*nxt)[GLD] = (ir & 1) ? GA : GB;
+    // ---- phase 1: rotation parameters
+    if (wave == 0 && lane < NPAIR) {
+      const double aa = cur[x0][x0], bb2 = cur[x1][x1], ab = cur[x0][x1];
+      double c = 1.0, s = 0.0;
+      // padding columns have a zero diagonal and never rotate (bitwise & keeps
+      // the three loads ahead of a single wait)
+      if ((aa > 0.0) & (bb2 > 0.0) & (ab * ab > tol2 * aa * bb2)) {
+        const double dd = bb2 - aa;
+        const double q = dd * dd + 4.0 * ab * ab;
+        double t;
+        if (q > 0x1p-900) {
+          // G is scaled to max diagonal ~1, so q <= ~20: hardware estimates with two
+          // Newton steps each (rsqrt, reciprocal) are accurate to ~1 ulp here
+          const double rq = rsqrt_nr(q);
+          t = 2.0 * ab * copysign(1.0, dd) * rcp_nr(fabs(dd) + q * rq);
+        } else {
+          t = 2.0 * ab * copysign(1.0, dd) / (fabs(dd) + sqrt(q));
         }
-        pcs[buf][lane] = make_double2(c, s);
-        qa = adv(qa);
-        qb = adv(qb);
+        c = rsqrt_nr(1.0 + t * t);  // 1 + t^2 in [1, 2]
+        s = c * t;
+        *vrot = 1;
       }
-    } else if (ir > 0) {
-      v_update();
+      pcs[lane] = make_double2(c, s);
     }
     __syncthreads();
-    // ---- phase 2: loads first (parameters and the four entries), then the
-    // rotation, stores only where a rotation acts
-    for (int b = 0; b < nbk; ++b) {
-      const int rl = min(rx0[b], rx1[b]), rh = max(rx0[b], rx1[b]);
-      const int cl = min(cx0[b], cx1[b]), ch = max(cx0[b], cx1[b]);
-      double* e00 = &G[min(rl, cl)][max(rl, cl)];
-      double* e01 = &G[min(rl, ch)][max(rl, ch)];
-      double* e10 = &G[min(rh, cl)][max(rh, cl)];
-      double* e11 = &G[min(rh, ch)][max(rh, ch)];
-      const double2 pr = pcs[buf][bP[b]], pq = pcs[buf][bQ[b]];
-      const double g0 = *e00, g1 = *e01, g2 = *e10, g3 = *e11;
-      const double t00 = pr.x * g0 - pr.y * g2, t01 = pr.x * g1 - pr.y * g3;
-      const double t10 = pr.y * g0 + pr.x * g2, t11 = pr.y * g1 + pr.x * g3;
-      if (pr.y != 0.0 || pq.y != 0.0) {
-        *e00 = t00 * pq.x - t01 * pq.y;
-        *e01 = t00 * pq.y + t01 * pq.x;
-        if (bP[b] != bQ[b]) *e10 = t10 * pq.x - t11 * pq.y;  // a diagonal block's (h, l) is its (l, h)
-        *e11 = t10 * pq.y + t11 * pq.x;
-      }
-      rx0[b] = adv(rx0[b]);
-      rx1[b] = adv(rx1[b]);
-      cx0[b] = adv(cx0[b]);
-      cx1[b] = adv(cx1[b]);
+    // ---- phase 2: G blocks (current -> next buffer, moved) and V rows
+    const double2 pr = pcs[P];
+#pragma unroll
+    for (int j = 0; j < C::GBLK; ++j) {
+      const double2 pq = pcs[Q[j]];
+      const double g00 = cur[x0][y0[j]], g01 = cur[x0][y1[j]];
+      const double g10 = cur[x1][y0[j]], g11 = cur[x1][y1[j]];
+      const double t00 = pr.x * g00 - pr.y * g10, t01 = pr.x * g01 - pr.y * g11;
+      const double t10 = pr.y * g00 + pr.x * g10, t11 = pr.y * g01 + pr.x * g11;
+      nxt[ax0][ay0[j]] = t00 * pq.x - t01 * pq.y;
+      nxt[ax0][ay1[j]] = t00 * pq.y + t01 * pq.x;
+      nxt[ax1][ay0[j]] = t10 * pq.x - t11 * pq.y;
+      nxt[ax1][ay1[j]] = t10 * pq.y + t11 * pq.x;
     }
-    if (wave > 0) v_params(pcs[buf]);
+#pragma unroll
+    for (int j = 0; j < C::VROWS; ++j) {
+      const double nl = pr.x * VL[j] - pr.y * VR[j];
+      const double nr = pr.y * VL[j] + pr.x * VR[j];
+      const double fromL = from_lower_lane(P == 0 ? nr : nl);
+      const double fromR = from_upper_lane(nr);
+      VL[j] = P == 0 ? nl : fromL;
+      VR[j] = P == NPAIR - 1 ? nl : fromR;
+    }
     __syncthreads();
   }
-  if (wave > 0) v_update();
+  // V back in player order (positions = players after 63 moves) into GA
+#pragma unroll
+  for (int j = 0; j < C::VROWS; ++j) {
+    const int row = slot + 16 * j;
+    GA[row][x0] = VL[j];
+    GA[row][x1] = VR[j];
+  }
   __syncthreads();
 }
 
-__global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __restrict__ jobs, int njobs, int round,
-                                                             int sweep, int32_t* __restrict__ flags) {
+template <int WB_>
+__global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_kernel(const JacJob* __restrict__ jobs,
+                                                                                  int njobs, int round, int sweep,
+                                                                                  int32_t* __restrict__ flags) {
+  using C = Jac<WB_>;
+  constexpr int WB = C::WB, WC = C::WC, GLD = C::GLD, JT = C::JT, NW = C::NW, NT = C::NT;
   const int bid = blockIdx.x;
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
@@ -268,11 +250,14 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
     if (ba < 0 && bb < 0) return;
   }
 
-  // G doubles as the row-chunk staging buffer of the Gram and apply phases
-  __shared__ double G[WC][GLD];
-  __shared__ double V[WC][GLD];
+  // GA holds the Gram matrix (and stages the Gram's row chunks); the inner sweep
+  // ping-pongs G between GA and GB and leaves V in GA; GB then stages the
+  // apply's row chunks
+  __shared__ double GA[WC][GLD];
+  __shared__ double GB[WC][GLD];
   __shared__ int cols[WC];          // global column of local index, -1 if padding
-  double (*chunk)[GLD] = G;
+  double (*G)[GLD] = GA;
+  double (*chunk)[GLD] = GA;
 
   if (tid < WC) {
     const int blk = tid < WB ? ba : bb;
@@ -281,10 +266,12 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
   }
   __syncthreads();
 
-  // ---- Gram matrix G = C^T C (MFMA f64 16x16x4; wave w owns output blocks w and w+8)
-  constexpr int PER = RCH * WC / JT;  // chunk elements per thread
+  // ---- Gram matrix G = C^T C (MFMA f64 16x16x4; wave w owns output tiles w + NW t)
+  constexpr int PER = C::PER;
   {
-    d4v gacc[2] = {d4v{0, 0, 0, 0}, d4v{0, 0, 0, 0}};
+    d4v gacc[C::TPW];
+#pragma unroll
+    for (int t = 0; t < C::TPW; ++t) gacc[t] = d4v{0, 0, 0, 0};
     double pre[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
@@ -306,8 +293,8 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
         }
       }
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int blk = wave + 8 * t, bi = blk >> 2, bj = blk & 3;
+      for (int t = 0; t < C::TPW; ++t) {
+        const int blk = wave + NW * t, bi = blk / NT, bj = blk % NT;
 #pragma unroll
         for (int kk = 0; kk < RCH; kk += 4) {
           const double a = chunk[kk + (lane >> 4)][16 * bi + (lane & 15)];
@@ -318,13 +305,12 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
       __syncthreads();
     }
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int blk = wave + 8 * t, bi = blk >> 2, bj = blk & 3;
+    for (int t = 0; t < C::TPW; ++t) {
+      const int blk = wave + NW * t, bi = blk / NT, bj = blk % NT;
 #pragma unroll
       for (int q = 0; q < 4; ++q) G[16 * bi + 4 * q + (lane >> 4)][16 * bj + (lane & 15)] = gacc[t][q];
     }
   }
-  for (int e = tid; e < WC * WC; e += JT) V[e % WC][e / WC] = (e % WC == e / WC) ? 1.0 : 0.0;
   __syncthreads();
   if (tid == 0) t1 = wall_clock64();
   // scale G by a power of two (rotation angles are scale free) so that the
@@ -361,7 +347,9 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
     return;
   }
 
-  inner_sweep(G, V, tol2, tid, &vrot);
+  inner_sweep<WB_>(GA, GB, tol2, tid, &vrot);
+  double (*V)[GLD] = GA;
+  chunk = GB;
   if (tid == 0) {
     t2 = wall_clock64();
     if (S.info) {
@@ -376,7 +364,7 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
     if (S.info) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 5, 1u);
   }
 
-  // ---- apply V to the visited columns of B and J: C <- C V (wave w: row block w>>2, column block w&3)
+  // ---- apply V to the visited columns of B and J: C <- C V (wave w: row tile w / NT, column tile w % NT)
   for (int which = 0; which < 2; ++which) {
     double* M = which == 0 ? S.B : S.J;
     const int64_t ld = which == 0 ? S.ldb : S.ldj;
@@ -400,7 +388,7 @@ __global__ __launch_bounds__(JT, 2) void jacobi_round_kernel(const JacJob* __res
           pre[q] = (gc >= 0 && rr < p) ? M[rr + int64_t(gc) * ld] : 0.0;
         }
       }
-      const int bi = wave >> 2, bj = wave & 3;
+      const int bi = wave / NT, bj = wave % NT;
       d4v acc = {0, 0, 0, 0};
 #pragma unroll
       for (int kk = 0; kk < WC; kk += 4) {
@@ -436,7 +424,8 @@ __global__ __launch_bounds__(256) void column_norms_kernel(const SVDJob* __restr
 
 }  // namespace
 
-void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs) {
+template <int WB>
+void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   std::vector<SVDJob> live;
   for (auto& j : jobs)
     if (j.p > 0) live.push_back(j);
@@ -470,7 +459,8 @@ void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs) {
       // timed per launch; the work depends on the sweeps taken, so the caller
       // adds it from the counters
       ProfScope ps(ctx, "jacobi_svd");
-      hipLaunchKernelGGL(jacobi_round_kernel, dim3(unsigned(slots)), dim3(JT), 0, ctx.stream, dj, nj, r, sweep, flags);
+      hipLaunchKernelGGL(jacobi_round_kernel<WB>, dim3(unsigned(slots)), dim3(Jac<WB>::JT), 0, ctx.stream, dj, nj, r,
+                         sweep, flags);
       TTR_HIP(hipGetLastError());
     }
     // convergence check from the third sweep on (no matrix converges earlier in practice)
@@ -484,6 +474,22 @@ void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs) {
   }
   hipLaunchKernelGGL(column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, ctx.stream, dl);
   TTR_HIP(hipGetLastError());
+}
+
+
+// Column blocks of 32 (64-column visits, 512 threads).  TTR_JACOBI_WB=16
+// selects 32-column visits: the same number of sweeps and half the inner
+// rotation work per sweep, but four times the visits, whose Gram and apply
+// phases read every column twice as often (measured 11% slower on the square
+// factors of the rounding sweeps).
+int jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs) {
+  static const int wb = [] {
+    const char* e = std::getenv("TTR_JACOBI_WB");
+    return (e && std::atoi(e) == 16) ? 16 : 32;
+  }();
+  if (wb == 32) jacobi_run<32>(ctx, jobs);
+  else jacobi_run<16>(ctx, jobs);
+  return 2 * wb;
 }
 
 }  // namespace ttr

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -264,7 +265,8 @@ struct SVDJob {
                   // ticks of the Gram / inner / apply phases summed over visits, [4] visits,
                   // [5] visits that applied a rotation, [6..7] unused
 };
-void jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs);
+// returns the number of columns per visit (for the work counters)
+int jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs);
 
 // ---------------------------------------------------------------------------
 // elementwise / data-movement kernels
