@@ -32,7 +32,9 @@ __all__ = [
     "tto_rounding", "default_rank_heuristic", "sketch_blocks_heuristic",
     "sketch_structure", "generate_sketch_blocks", "contract_sketch_core_backwards",
     "contract_sketch_core_backwards_op", "tt_recursive_sketch",
-    "tt_recursive_sketch_op", "tt_sketch", "tt_combined_sketch", "ttrand_rounding", "ttrand_rounding_lincomb",
+    "tt_recursive_sketch_op", "tt_sketch", "tt_combined_sketch", "sketch_structure_forward",
+    "contract_sketch_core_forwards", "tt_recursive_sketch_forward", "stable_inverse", "stta_sketch",
+    "stta_sketch_tt", "stta_solve", "stta", "ttrand_rounding", "ttrand_rounding_lincomb",
     "ttrand_rounding_op", "ttrand_rounding_residual", "rel_err", "tt_dot", "dot",
     "dot_par", "partial_contraction", "dot_operator",
     "hubbard_1d_mpo", "hubbard_1d_dense",
@@ -747,6 +749,140 @@ def tt_combined_sketch(A: TT, rks, s: int, draws, draws_extra, orthogonal: bool 
     W[0] = np.ones((1, 1)) if H is None else np.ones((1, 1, 1))
     srk[0] = 1
     return sketch, W, srk
+
+
+def sketch_structure_forward(dims, rks, block_rks: Optional[int] = None, p=None, oversampling=1):
+    """Block ranks and counts of the forward (left-to-right) sketch
+    (``src/sketches.jl:241-257``, ``compute_sketch_blocks_heuristic(...; reverse=false)``
+    ``:113-121``): ``bv_1 = 1``, ``bv_{k+1} = min(block_rks, n_k bv_k)``; ``p_k =
+    ceil(rks_k / bv_k)`` made non-increasing towards the right, ``p_1 = 1``; the
+    oversampling multiplies ``p_2..p_{N+1}``."""
+    N = len(dims)
+    if block_rks is None:
+        block_rks = N
+    bv = [1] * (N + 1)
+    for k in range(1, N + 1):
+        bv[k] = block_rks
+    for k in range(N):
+        bv[k + 1] = min(bv[k + 1], dims[k] * bv[k])
+    if p is None or (isinstance(p, int) and p == 0):
+        p = [int(math.ceil(r / b)) for r, b in zip(rks, bv)]
+        for i in range(N - 1, 0, -1):
+            p[i] = max(p[i], p[i + 1])
+        p[0] = 1
+    else:
+        p = [int(v) for v in p]
+    if oversampling != 1:
+        for k in range(1, N + 1):
+            p[k] = int(math.ceil(oversampling * p[k]))
+    return bv, p
+
+
+def contract_sketch_core_forwards(A: np.ndarray, S: np.ndarray, V: np.ndarray) -> np.ndarray:
+    """``W[a,b] = sum A[z,al,a] S[be,z,b] V[al,be]`` with the reference's
+    cost-chosen order, ((V^T A) S) or ((V S) A) (``src/sketches.jl:1192-1230``)."""
+    z, al, a = A.shape
+    be, _, b = S.shape
+    if a * (be + 1) * (al + b) < al * b * (a + be + 1):
+        VA = V.T @ jreshape(np.transpose(A, (1, 0, 2)), (al, z * a))  # (be, z*a)
+        return jreshape(VA, (be * z, a)).T @ jreshape(S, (be * z, b))
+    VS = jreshape(V @ jreshape(S, (be, z * b)), (al, z, b))
+    VSp = jreshape(np.transpose(VS, (1, 0, 2)), (z * al, b))
+    return jreshape(A, (z * al, a)).T @ VSp
+
+
+def tt_recursive_sketch_forward(A: TT, rks, draws, orthogonal: bool = True, block_rks: Optional[int] = None,
+                                p=None, oversampling=1):
+    """Forward (left-to-right) recursive sketch, ``tt_recursive_sketch(...; reverse=false)``
+    (``src/sketches.jl:241-298,300-303``).  Core ``k`` uses blocks
+    ``(bv_k, n_k, bv_{k+1}, p_{k+1})``; ``W[k+1][:,:,j]`` contracts core ``k``
+    with block ``j`` and ``W[k][:,:,j]`` (``W[1] = 1``).  Returns ``W[k]`` of
+    shape ``(A.rks[k], bv_k p_k)`` divided by ``sqrt(p_k)``, and the sketch ranks."""
+    N = A.N
+    dims = A.dims
+    assert rks[0] == 1 and A.rks[0] == 1
+    bv, p = sketch_structure_forward(dims, rks, block_rks, p, oversampling)
+    W: List[Optional[np.ndarray]] = [None] * (N + 1)
+    W[0] = np.ones((1, 1, 1))
+    for k in range(N):
+        S = _sketch_for(draws, k, bv[k], dims[k], bv[k + 1], p[k + 1], orthogonal)
+        Wk = np.zeros((A.rks[k + 1], bv[k + 1], p[k + 1]))
+        for j in range(p[k + 1]):
+            Wk[:, :, j] = contract_sketch_core_forwards(A.cores[k], S[:, :, :, j], W[k][:, :, j if k > 0 else 0])
+        W[k + 1] = Wk
+    out = []
+    for k in range(N + 1):
+        out.append(jreshape(W[k] / math.sqrt(p[k]), (A.rks[k], bv[k] * p[k])))
+    return out, [bv[k] * p[k] for k in range(N + 1)]
+
+
+def stable_inverse(A: np.ndarray, eps: float = 1e-12) -> np.ndarray:
+    """Pseudo-inverse over the singular values above ``eps * max(s)``
+    (``src/tt_randtools.jl:485-488``)."""
+    u, s, vt = np.linalg.svd(A, full_matrices=False)
+    keep = s > s.max() * eps
+    return vt[keep].T @ np.diag(1.0 / s[keep]) @ u[:, keep].T
+
+
+def stta_sketch(x: TT, rks, draws_left, draws_right, orthogonal: bool = True, block_rks: Optional[int] = None):
+    """``stta_sketch(x, rks; seed_left, seed_right, orthogonal, block_rks)``
+    (``src/sketches.jl:1126-1156``): the forward sketch with oversampling 1.5 and
+    the reverse sketch, ``Omega[k] = W_L[k+1]^T W_R[k+1]`` (k = 1..N-1) and
+    ``Psi[k][i,a,b] = x_k[i,al,be] W_L[k][al,a] W_R[k+1][be,b]`` (k = 1..N)."""
+    N = x.N
+    WL, sl = tt_recursive_sketch_forward(x, rks, draws_left, orthogonal, block_rks, oversampling=1.5)
+    WR, sr = tt_recursive_sketch(x, rks, draws_right, orthogonal, block_rks)
+    Om = [WL[k + 1].T @ WR[k + 1] for k in range(N - 1)]
+    Ps = [np.einsum("iab,ac,bd->icd", x.cores[k], WL[k], WR[k + 1], optimize=True) for k in range(N)]
+    return Om, Ps, sl, sr
+
+
+def stta_sketch_tt(x: TT, L: TT, R: TT):
+    """``stta_sketch(x, L, R)`` with explicit sketch TTs (``src/sketches.jl:1112-1124``):
+    the left and right interface contractions of ``x`` with ``L`` and ``R``."""
+    lc = partial_contraction(x, L, reverse=False)
+    rc = partial_contraction(x, R, reverse=True)
+    N = x.N
+    Om = [lc[k + 1].T @ rc[k + 1] for k in range(N - 1)]
+    Ps = [np.einsum("iyz,ya,zb->iab", x.cores[k], lc[k], rc[k + 1], optimize=True) for k in range(N)]
+    return Om, Ps
+
+
+def stta_solve(Om, Ps, dims) -> TT:
+    """The local solves of ``stta`` (``src/tt_randtools.jl:562-575``): per bond,
+    ``Psi[k] pinv(Omega[k])`` when ``Omega[k]`` has fewer rows than columns,
+    else ``pinv(Omega[k]) Psi[k+1]`` (in the second index of ``Psi[k+1]``)."""
+    N = len(dims)
+    P = [c.copy() for c in Ps]
+    rks = [1] * (N + 1)
+    for k in range(N - 1):
+        O_ = Om[k]
+        if O_.shape[0] < O_.shape[1]:
+            n_, a, b = P[k].shape
+            P[k] = jreshape(jreshape(P[k], (n_ * a, b)) @ stable_inverse(O_), (n_, a, -1))
+            rks[k + 1] = P[k].shape[2]
+        else:
+            n_, a, b = P[k + 1].shape
+            t = stable_inverse(O_) @ jreshape(np.transpose(P[k + 1], (1, 0, 2)), (a, n_ * b))
+            P[k + 1] = np.transpose(jreshape(t, (-1, n_, b)), (1, 0, 2))
+            rks[k + 1] = P[k + 1].shape[1]
+    return TT(P, dims, rks, [0] * N)
+
+
+def stta(alpha, ys: Sequence[TT], rks, draws_left, draws_right, orthogonal: bool = True,
+         block_rks: Optional[int] = None) -> TT:
+    """``stta(alpha, y, rks; seed_left, seed_right, orthogonal, block_rks)``
+    (``src/tt_randtools.jl:552-638``; one term is ``stta(y, rks)``): the sketches
+    of ``sum_j alpha_j y_j`` (linear in the terms), then the local solves."""
+    Om, Ps = None, None
+    for a, y in zip(alpha, ys):
+        O_, P_, _, _ = stta_sketch(y, rks, draws_left, draws_right, orthogonal, block_rks)
+        if Om is None:
+            Om, Ps = [a * o for o in O_], [a * q for q in P_]
+        else:
+            Om = [o + a * o2 for o, o2 in zip(Om, O_)]
+            Ps = [q + a * q2 for q, q2 in zip(Ps, P_)]
+    return stta_solve(Om, Ps, ys[0].dims)
 
 
 def _qr_thin(Z: np.ndarray) -> np.ndarray:

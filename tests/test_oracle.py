@@ -14,6 +14,11 @@ a wrong index or a transposed operand fails:
   tt_recursive_sketch -> dense X_{k:N} Omega_{k:N} brute force; p=1 full sketch = dot
   tt_sketch           -> dense vec(x)^T Omega_{1:N} with constant p (TT and operator forms);
                          E|sketch|^2 = |x|^2 over seeds
+  forward sketch      -> dense L_k^T Omega_{1:k-1} brute force
+  stta_sketch         -> dense Omega_L^T X_(k) Omega_R and the three-index Psi_k; explicit-TT form
+                         against dense interface matrices
+  stable_inverse      -> A A^+ A = A, numpy pinv, the 1e-12 relative cutoff
+  stta                -> exact recovery when the sketch ranks cover the ranks (test_ttrand.jl:136-147)
   tt_combined_sketch  -> parts equal the recursive sketch's W_1 and tt_sketch(seed + 1) with
                          the sqrt(len/s) weights; E|sketch|^2 = |x|^2 over seeds
   ttrand_rounding*    -> exact recovery when the sketch rank >= true rank (test_ttrand.jl:67-88)
@@ -420,6 +425,121 @@ def test_tt_combined_sketch_parts():
     assert len(cs) == tot and tot >= s
     np.testing.assert_allclose(cs[: sr[0]], math.sqrt(sr[0] / tot) * O.jreshape(Wr[0], -1), rtol=1e-15)
     np.testing.assert_allclose(cs[sr[0]:], math.sqrt(le / tot) * ex, rtol=1e-15)
+
+
+def _dense_sketch_fwd(Sblocks, dims, p, k):
+    """Forward Omega_{1:k} as (n_1...n_k) x (bv_{k+1} p_{k+1}) from the block chains."""
+    cols = []
+    for j in range(p[k]):
+        om = Sblocks[0][0, :, :, j]  # (n_1, bv_2)
+        for q in range(1, k):
+            t = np.einsum("mb,bic->mic", om, Sblocks[q][:, :, :, j])
+            om = t.reshape(-1, Sblocks[q].shape[2], order="F")
+        cols.append(om)
+    return np.concatenate(cols, axis=1)
+
+
+@pytest.mark.parametrize("orthogonal,block_rks", [(False, None), (True, None), (False, 2), (True, 3)])
+def test_forward_sketch_dense(orthogonal, block_rks):
+    # tt_recursive_sketch(...; reverse=false) (sketches.jl:241-303): W_k = L_k^T Omega_{1:k-1} / sqrt(p_k)
+    dims = (2, 3, 2, 2, 3)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 4, 5, 3, 1], 27)
+    rks = [1, 4, 5, 5, 4, 1]
+    bv, p = O.sketch_structure_forward(dims, rks, block_rks, oversampling=1.5)
+    assert p[0] == 1 and all(p[k] >= p[k + 1] for k in range(1, N))
+    draws = [RNG.standard_normal(bv[k] * dims[k] * bv[k + 1] * p[k + 1]) for k in range(N)]
+    W, s = O.tt_recursive_sketch_forward(x, rks, draws, orthogonal, block_rks, oversampling=1.5)
+    Sb = [O.generate_sketch_blocks(draws[k], bv[k], dims[k], bv[k + 1], p[k + 1], orthogonal) for k in range(N)]
+    for k in range(1, N + 1):
+        ref = _left_part(x, k).T @ _dense_sketch_fwd(Sb, dims, p, k) / math.sqrt(p[k])
+        np.testing.assert_allclose(W[k], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+        assert W[k].shape == (x.rks[k], s[k])
+
+
+def _stta_draws(dims, rks, block_rks=None, g=None):
+    g = g or RNG
+    bvl, pl = O.sketch_structure_forward(dims, rks, block_rks, oversampling=1.5)
+    bvr, pr = O.sketch_structure(dims, rks, block_rks)
+    N = len(dims)
+    dl = [g.standard_normal(bvl[k] * dims[k] * bvl[k + 1] * pl[k + 1]) for k in range(N)]
+    dr = [g.standard_normal(bvr[k + 1] * dims[k] * bvr[k] * pr[k]) for k in range(N)]
+    return dl, dr, (bvl, pl), (bvr, pr)
+
+
+@pytest.mark.parametrize("orthogonal", [False, True])
+def test_stta_sketch_dense(orthogonal):
+    # Omega_k = Omega_L^T X_(k) Omega_R, Psi_k = Omega_L^T X Omega_R over the outer modes
+    dims = (2, 3, 2, 3)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 3, 2, 1], 28)
+    rks = [1, 2, 3, 2, 1]
+    dl, dr, (bvl, pl), (bvr, pr) = _stta_draws(dims, rks)
+    Om, Ps, sl, sr = O.stta_sketch(x, rks, dl, dr, orthogonal)
+    SL = [O.generate_sketch_blocks(dl[k], bvl[k], dims[k], bvl[k + 1], pl[k + 1], orthogonal) for k in range(N)]
+    SR = [O.generate_sketch_blocks(dr[k], bvr[k + 1], dims[k], bvr[k], pr[k], orthogonal) for k in range(N)]
+    X = O.ttv_to_tensor(x)
+    for k in range(N - 1):
+        nl = int(np.prod(dims[: k + 1]))
+        XL = X.reshape(nl, -1, order="F")
+        OL = _dense_sketch_fwd(SL, dims, pl, k + 1) / math.sqrt(pl[k + 1])
+        OR = _dense_sketch(SR, dims, bvr, pr, k + 1) / math.sqrt(pr[k + 1])
+        ref = OL.T @ XL @ OR
+        assert Om[k].shape == (sl[k + 1], sr[k + 1])
+        np.testing.assert_allclose(Om[k], ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+    for k in range(N):
+        m0 = int(np.prod(dims[:k]))
+        X3 = X.reshape(m0, dims[k], -1, order="F")
+        OL = (_dense_sketch_fwd(SL, dims, pl, k) / math.sqrt(pl[k])) if k > 0 else np.ones((1, 1))
+        OR = (_dense_sketch(SR, dims, bvr, pr, k + 1) / math.sqrt(pr[k + 1])) if k < N - 1 else np.ones((1, 1))
+        ref = np.einsum("ma,mib,bc->iac", OL, X3, OR, optimize=True)
+        assert Ps[k].shape == (dims[k], sl[k], sr[k + 1])
+        np.testing.assert_allclose(Ps[k], ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+
+
+def test_stta_sketch_with_explicit_tts_dense():
+    # stta_sketch(x, L, R) (sketches.jl:1112-1124): Omega_k = L_{<=k}^T X_(k) R_{>k}^T
+    dims = (2, 3, 2, 2)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 4, 2, 1], 29)
+    L = rand_tt(dims, [1, 2, 5, 3, 1], 30)
+    R = rand_tt(dims, [1, 2, 3, 2, 1], 31)
+    Om, Ps = O.stta_sketch_tt(x, L, R)
+    X = O.ttv_to_tensor(x)
+    for k in range(N - 1):
+        XL = X.reshape(int(np.prod(dims[: k + 1])), -1, order="F")
+        ref = _left_part(L, k + 1).T @ XL @ _right_part(R, k + 1).T
+        np.testing.assert_allclose(Om[k], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    for k in range(N):
+        X3 = X.reshape(int(np.prod(dims[:k])), dims[k], -1, order="F")
+        ref = np.einsum("ma,mib,cb->iac", _left_part(L, k), X3, _right_part(R, k + 1), optimize=True)
+        np.testing.assert_allclose(Ps[k], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
+def test_stable_inverse():
+    A = RNG.standard_normal((7, 4))
+    P = O.stable_inverse(A)
+    np.testing.assert_allclose(A @ P @ A, A, atol=1e-12)
+    np.testing.assert_allclose(P, np.linalg.pinv(A), atol=1e-12)
+    # singular values below 1e-12 max(s) are dropped (src/tt_randtools.jl:485-488)
+    u, _, vt = np.linalg.svd(RNG.standard_normal((6, 6)))
+    B = u @ np.diag([3.0, 2.0, 1.0, 0.5, 1e-13, 1e-14]) @ vt
+    np.testing.assert_allclose(O.stable_inverse(B), vt[:4].T @ np.diag(1 / np.array([3.0, 2.0, 1.0, 0.5])) @ u[:, :4].T,
+                               atol=1e-10)
+
+
+@pytest.mark.parametrize("orthogonal", [False, True])
+def test_stta_exact_when_rank_sufficient(orthogonal):
+    # test_ttrand.jl:136-147 pattern: STTA reproduces x when the sketch ranks cover its ranks
+    dims = (2,) * 7
+    x = rand_tt(dims, [1, 2, 4, 4, 4, 4, 2, 1], 32)
+    rks = [1, 2, 4, 6, 6, 4, 2, 1]
+    dl, dr, _, _ = _stta_draws(dims, rks)
+    y = O.stta([1.0], [x], rks, dl, dr, orthogonal)
+    assert O.rel_err(x, y) < 1e-10
+    ys = [rand_tt(dims, [1, 2, 2, 2, 2, 2, 2, 1], 33 + j) for j in range(2)]
+    y2 = O.stta([0.5, -2.0], ys, rks, dl, dr, orthogonal)
+    assert O.rel_err(O.tt_lincomb(ys, [0.5, -2.0]), y2) < 1e-10
 
 
 # ---------------------------------------------------------------- randomized rounding
