@@ -155,6 +155,8 @@ typedef struct {
   uint64_t seed;        /* the reference default is 1234 (src/tt_randtools.jl:112)           */
   const int64_t* p;     /* optional d+1 block counts (tt_recursive_sketch kwarg p), or NULL   */
   double oversampling;  /* block-count multiplier (kwarg oversampling; 1 = none)              */
+  int32_t forward;      /* 0: right-to-left sketch (kwarg reverse=true, the default); 1: the  */
+                        /* left-to-right sketch (reverse=false; ttr_sketch, ttr_sketch_blocks) */
 } ttr_rand_opts;
 
 /* Randomize-then-orthogonalize rounding of n linear combinations in one
@@ -172,11 +174,12 @@ typedef struct {
 int ttr_round_rand(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const ttr_rand_opts* opts,
                    int64_t* out_ids);
 
-/* The reverse recursive sketch of one TT: W[k] (r_k x sketch_rks[k],
- * column-major, k = 0..d) into W_out[k] (NULL entries skipped), the sketch
- * ranks into sketch_rks (d+1).
+/* The recursive sketch of one TT: W[k] (r_k x sketch_rks[k], column-major,
+ * k = 0..d) into W_out[k] (NULL entries skipped), the sketch ranks into
+ * sketch_rks (d+1).  opts->forward selects the left-to-right sketch (FP64 only).
  * Replaces: tt_recursive_sketch(T, A, rks; reverse=true, ...)
- * (src/sketches.jl:175-240,300-303). */
+ * (src/sketches.jl:175-240,300-303) and tt_recursive_sketch(T, A, rks;
+ * reverse=false, ...) (:241-303). */
 int ttr_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts, double* const* W_out,
                int64_t* sketch_rks);
 
@@ -203,10 +206,36 @@ int ttr_tt_sketch(ttr_handle_t h, int64_t n, const int64_t* tt_ids, const int64_
 
 /* The sketch blocks of core k (0-based) exactly as the device uses them,
  * upcast to FP64, in the reference order (bv_{k+1}, n_k, bv_k, p_k)
- * column-major (the array generate_sketch_blocks returns, src/sketches.jl:83).
- * Writes bv_{k+1}*n_k*bv_k*p_k doubles; shape4 (may be NULL) gets the 4 sizes. */
+ * column-major (the array generate_sketch_blocks returns, src/sketches.jl:83);
+ * with opts->forward the blocks of the left-to-right sketch, (bv_k, n_k,
+ * bv_{k+1}, p_{k+1}) (src/sketches.jl:279-281).  Writes the product of the 4
+ * sizes; shape4 (may be NULL) gets them. */
 int ttr_sketch_blocks(ttr_handle_t h, int64_t d, const int64_t* dims, const int64_t* rks, const ttr_rand_opts* opts,
                       int64_t k, double* out, int64_t* shape4);
+
+/* Streaming TT approximation of n linear combinations (two-sided sketches):
+ * per item the left-to-right sketch of every term (block counts of rks, times
+ * 1.5; seed opts->seed) and the right-to-left one (seed_right), Omega_k =
+ * W_L[k+1]^T W_R[k+1] and the sketched cores Psi_k summed over the terms with
+ * their alpha, then per bond the pseudo-inverse of Omega_k (singular values
+ * above 1e-12 max) applied to Psi_k from the right (Omega_k wide) or to
+ * Psi_{k+1} from the left.  FP64; op_id must be -1; opts->p must be NULL.
+ * Output ot = 0.
+ * Replaces: stta(y, rks; seed_left, seed_right, orthogonal, block_rks)
+ * (src/tt_randtools.jl:552-580), its rmax form (:582-589) and
+ * stta(alpha, y, rks; ...) (:591-638). */
+int ttr_stta(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const ttr_rand_opts* opts, uint64_t seed_right,
+             int64_t* out_ids);
+
+/* The sketch stage of ttr_stta for one TT: Omega_out[k] (sl_{k+1} x sr_{k+1},
+ * column-major, k = 0..d-2) and Psi_out[k] (reference layout (n_k, sl_k,
+ * sr_{k+1}), k = 0..d-1), the left and right sketch ranks into sl, sr (d+1).
+ * NULL output entries are skipped.
+ * Replaces: stta_sketch(x, rks; seed_left, seed_right, orthogonal, block_rks)
+ * (src/sketches.jl:1126-1156). */
+int ttr_stta_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts,
+                    uint64_t seed_right, double* const* Omega_out, double* const* Psi_out, int64_t* sl,
+                    int64_t* sr);
 
 /* Raw N(0,1) draws number 0..count-1 of core k of the sketch stream. */
 int ttr_sketch_draws(ttr_handle_t h, int64_t k, int64_t count, uint64_t seed, double* out);

@@ -862,6 +862,7 @@ void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items,
 void prepare_plan(ttr_handle_s* h, const ttr_rand_opts* o, int d, const std::vector<int64_t>& dims,
                   std::vector<RandItem>& items, SketchPlan& sp) {
   TTR_REQUIRE(o != nullptr, ST_INVALID_ARG, "options must not be NULL");
+  TTR_REQUIRE(o->forward == 0, ST_INVALID_ARG, "this entry point takes the right-to-left sketch only");
   sp.d = d;
   sp.dims = dims;
   sp.prec = to_prec(o->sketch_prec);
@@ -1078,6 +1079,393 @@ std::vector<RandItem> parse_items(ttr_handle_s* h, int64_t n, const ttr_rand_ite
     }
     const int d = out[i].terms[0].x->d;
     out[i].rks.assign(it.rks, it.rks + d + 1);
+  }
+  return out;
+}
+
+// ------------------------------------------------------------ forward sketch
+// Block ranks of the forward (left-to-right) sketch (src/sketches.jl:244-248):
+// bv_0 = 1, bv_{k+1} = min(block_rks, n_k bv_k)
+std::vector<int64_t> block_ranks_fwd(const std::vector<int64_t>& dims, int64_t block_rks) {
+  const int d = int(dims.size());
+  std::vector<int64_t> bv(d + 1, 1);
+  for (int k = 1; k <= d; ++k) bv[k] = block_rks;
+  for (int k = 0; k < d; ++k) bv[k + 1] = std::min(bv[k + 1], dims[k] * bv[k]);
+  return bv;
+}
+
+// compute_sketch_blocks_heuristic(...; reverse=false) (src/sketches.jl:113-121) and the
+// oversampling of p_2..p_{N+1} (:253-255)
+std::vector<int64_t> block_counts_fwd(const std::vector<int64_t>& rks, const std::vector<int64_t>& bv, int d,
+                                      const int64_t* p_in, double oversampling) {
+  std::vector<int64_t> p(d + 1, 1);
+  if (p_in) {
+    for (int k = 0; k <= d; ++k) p[k] = p_in[k];
+  } else {
+    for (int k = 0; k <= d; ++k) p[k] = int64_t(std::ceil(double(rks[k]) / double(bv[k])));
+    for (int k = d - 1; k >= 1; --k) p[k] = std::max(p[k], p[k + 1]);
+    p[0] = 1;
+  }
+  if (oversampling != 1.0 && oversampling > 0.0)
+    for (int k = 1; k <= d; ++k) p[k] = int64_t(std::ceil(oversampling * double(p[k])));
+  for (int k = 0; k <= d; ++k) TTR_REQUIRE(p[k] >= 1, ST_INVALID_ARG, "block counts must be >= 1");
+  TTR_REQUIRE(p[0] == 1, ST_INVALID_ARG, "the first block count of a forward sketch must be 1");
+  for (int k = 1; k < d; ++k)
+    TTR_REQUIRE(p[k] >= p[k + 1], ST_INVALID_ARG, "forward block counts must be non-increasing towards the right");
+  return p;
+}
+
+struct FwdPlan {
+  int d = 0;
+  std::vector<int64_t> dims, bv, pmax;
+  bool orthogonal = true;
+  uint64_t seed = 1234;
+  std::vector<double*> S;  // core k: (bv_k n_k) x bv_{k+1} x pmax_{k+1}, reference order, FP64
+};
+
+// One term of a forward sketch: W_k (r_k x bv_k p_k), unscaled (the 1/sqrt(p_k) is
+// applied where W_k is used), k = 1..kend; W_0 = 1
+struct FwdTerm {
+  DevTT* x = nullptr;
+  double alpha = 1.0;
+  std::vector<int64_t> p;  // block counts of its item
+  std::vector<double*> W;
+};
+
+// generate_sketch_blocks(rng, T, bv_k, n_k, bv_{k+1}, p_{k+1}, orthogonal) for cores
+// 0..kend-1 (src/sketches.jl:44-84,279-281): Gaussian blocks scaled by 1/sqrt(bv_{k+1});
+// orthogonal: thin Householder Q of each block scaled by sqrt(bv_k n_k / bv_{k+1}), or
+// identity blocks when bv_{k+1} = n_k bv_k
+void make_sketch_fwd(ttr_handle_s* h, FwdPlan& fp, int kend) {
+  Ctx& ctx = h->ctx;
+  fp.S.assign(fp.d, nullptr);
+  std::vector<SketchGen> gen;
+  std::vector<Fill2D> fills;
+  std::vector<QRJob> qj;
+  std::vector<Copy2D> cp;
+  for (int k = 0; k < kend; ++k) {
+    const int64_t n = fp.dims[k], left = fp.bv[k], right = fp.bv[k + 1], p = fp.pmax[k + 1], m = left * n;
+    fp.S[k] = ctx.ws.alloc_n<double>(size_t(m * right * p));
+    if (!fp.orthogonal) {
+      SketchGen g{nullptr, fp.S[k], int(n), int(left), int(right), int(p), k, 1.0};
+      g.ref_scale = 1.0 / std::sqrt(double(right));
+      gen.push_back(g);
+    } else if (right == m) {
+      for (int64_t j = 0; j < p; ++j) fills.push_back(Fill2D{fp.S[k] + j * m * right, m, int(m), int(right), 1.0});
+    } else {
+      double* raw = ctx.ws.alloc_n<double>(size_t(m * right * p));
+      double* q = ctx.ws.alloc_n<double>(size_t(m * right * p));
+      gen.push_back(SketchGen{nullptr, raw, int(n), int(left), int(right), int(p), k, 1.0});
+      for (int64_t j = 0; j < p; ++j) {
+        qj.push_back(QRJob{raw + j * m * right, m, int(m), int(right), q + j * m * right, m});
+        cp.push_back(copy_job(q + j * m * right, 1, m, fp.S[k] + j * m * right, 1, m, int(m), int(right),
+                              std::sqrt(double(m) / double(right))));
+      }
+    }
+  }
+  fill2d_batched(ctx, fills);
+  sketch_generate(ctx, gen, fp.seed, Prec::F64);
+  qr_batched(ctx, qj);
+  copy2d_batched(ctx, cp);
+}
+
+// forward recursive sketch of every term (src/sketches.jl:276-298), all terms in the same
+// launches: for core k and block j, T1 = W_k[:, block j]^T X_k (bv_k x n r_{k+1}), then
+// W_{k+1}[:, block j] = T1^T S_j with T1 read as (bv_k n) x r_{k+1}; W_0 = 1 serves every
+// block of core 0
+void sketch_phase_fwd(ttr_handle_s* h, const FwdPlan& fp, std::vector<FwdTerm>& terms, int kend) {
+  Ctx& ctx = h->ctx;
+  GemmBatch gb;
+  for (auto& t : terms) {
+    t.W.assign(size_t(fp.d) + 1, nullptr);
+    for (int k = 1; k <= kend; ++k) t.W[k] = ctx.ws.alloc_n<double>(size_t(t.x->rks[k] * fp.bv[k] * t.p[k]));
+  }
+  for (int k = 0; k < kend; ++k) {
+    const int64_t n = fp.dims[k], bl = fp.bv[k], br = fp.bv[k + 1];
+    std::vector<double*> T1(terms.size(), nullptr);
+    if (k > 0) {
+      for (size_t i = 0; i < terms.size(); ++i) {
+        const DevTT* x = terms[i].x;
+        const int64_t r = x->rks[k], r1 = x->rks[k + 1], pk1 = terms[i].p[k + 1];
+        T1[i] = ctx.ws.alloc_n<double>(size_t(bl * n * r1 * pk1));
+        for (int64_t j = 0; j < pk1; ++j)
+          gb.add(int(bl), int(n * r1), int(r), 1.0, terms[i].W[k] + r * bl * j, r, 1, x->cores[k], 1, r, 0.0,
+                 T1[i] + bl * n * r1 * j, 1, bl);
+      }
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "fwd_t1");
+    }
+    for (size_t i = 0; i < terms.size(); ++i) {
+      const DevTT* x = terms[i].x;
+      const int64_t r1 = x->rks[k + 1], pk1 = terms[i].p[k + 1];
+      for (int64_t j = 0; j < pk1; ++j) {
+        const double* A = k == 0 ? x->cores[0] : T1[i] + bl * n * r1 * j;  // (bl n) x r1
+        gb.add(int(r1), int(br), int(bl * n), 1.0, A, bl * n, 1, fp.S[k] + bl * n * br * j, 1, bl * n, 0.0,
+               terms[i].W[k + 1] + r1 * br * j, 1, r1);
+      }
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "fwd_w");
+  }
+}
+
+void prepare_fwd(const ttr_rand_opts* o, int d, const std::vector<int64_t>& dims, FwdPlan& fp) {
+  fp.d = d;
+  fp.dims = dims;
+  fp.orthogonal = o->orthogonal != 0;
+  fp.seed = o->seed;
+  fp.bv = block_ranks_fwd(dims, o->block_rks > 0 ? o->block_rks : d);
+  fp.pmax.assign(size_t(d) + 1, 1);
+}
+
+// ------------------------------------------------------------ STTA
+// One item of stta(alpha, y, rks) (src/tt_randtools.jl:552-638): the sketches of every
+// term, then Omega_k = sum_t alpha_t W_L,t[k+1]^T W_R,t[k+1] and the sketched cores
+// Psi_k = sum_t alpha_t X_t,k x_1 W_L,t[k] x_3 W_R,t[k+1] (device layout (sl_k, n_k, sr_{k+1}))
+struct SttaItem {
+  std::vector<FwdTerm> left;  // per term
+  RandItem right;             // reverse sketch of the same terms
+  std::vector<int64_t> pl, sl, sr;
+  std::vector<double*> Om;    // d-1: sl_{k+1} x sr_{k+1}
+  std::vector<double*> Psi;   // d: (sl_k, n_k, sr_{k+1})
+};
+
+void stta_sketch_batch(ttr_handle_s* h, std::vector<SttaItem>& its, const ttr_rand_opts* o, uint64_t seed_right) {
+  Ctx& ctx = h->ctx;
+  TTR_REQUIRE(o != nullptr, ST_INVALID_ARG, "options must not be NULL");
+  TTR_REQUIRE(to_prec(o->sketch_prec) == Prec::F64, ST_INVALID_ARG, "STTA sketches are FP64");
+  TTR_REQUIRE(o->p == nullptr, ST_INVALID_ARG, "STTA takes its block counts from the ranks");
+  const DevTT* x0 = its[0].right.terms[0].x;
+  const int d = x0->d;
+  for (auto& it : its)
+    for (auto& t : it.right.terms) {
+      TTR_REQUIRE(t.x->dims == x0->dims, ST_DIM_MISMATCH, "all terms must share dims");
+      TTR_REQUIRE(t.x->rks[0] == 1 && t.x->rks[d] == 1, ST_INVALID_ARG, "boundary ranks must be 1");
+      TTR_REQUIRE(t.H == nullptr, ST_INVALID_ARG, "STTA takes no operator");
+    }
+  // right: the reverse sketch (no oversampling), seed_right
+  ttr_rand_opts orr = *o;
+  orr.seed = seed_right;
+  orr.oversampling = 1.0;
+  orr.forward = 0;
+  std::vector<RandItem> ritems;
+  for (auto& it : its) ritems.push_back(it.right);
+  SketchPlan sp;
+  prepare_plan(h, &orr, d, x0->dims, ritems, sp);
+  // left: the forward sketch with oversampling 1.5 (src/sketches.jl:1131-1133), seed_left
+  FwdPlan fp;
+  prepare_fwd(o, d, x0->dims, fp);
+  std::vector<FwdTerm> all;
+  for (auto& it : its) {
+    it.pl = block_counts_fwd(it.right.rks, fp.bv, d, nullptr, 1.5);
+    it.sl.resize(size_t(d) + 1);
+    for (int k = 0; k <= d; ++k) {
+      it.sl[k] = fp.bv[k] * it.pl[k];
+      fp.pmax[k] = std::max(fp.pmax[k], it.pl[k]);
+    }
+  }
+  {
+    ProfScope ps(ctx, "phase:stta_sketch");
+    make_sketch_fwd(h, fp, d - 1);
+    for (auto& it : its)
+      for (auto& t : it.right.terms) {
+        FwdTerm ft;
+        ft.x = t.x;
+        ft.alpha = t.alpha;
+        ft.p = it.pl;
+        all.push_back(ft);
+      }
+    sketch_phase_fwd(h, fp, all, d - 1);
+    make_sketch(h, sp, 1);
+    sketch_phase(h, sp, ritems, 1);
+  }
+  size_t ti = 0;
+  for (size_t i = 0; i < its.size(); ++i) {
+    its[i].right = ritems[i];
+    its[i].sr = ritems[i].s;
+    its[i].left.assign(all.begin() + ti, all.begin() + ti + ritems[i].terms.size());
+    ti += ritems[i].terms.size();
+  }
+  // the power-of-two exponents of the right sketches (the GEMM's per-segment exponents are
+  // relative to the problem's largest, so the absolute factors go into the host scales)
+  std::vector<int32_t*> eh;
+  for (auto& it : its)
+    for (auto& t : it.right.terms) {
+      int32_t* e = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * (size_t(d) + 1)));
+      TTR_HIP(hipMemcpyAsync(e, t.E, sizeof(int32_t) * (size_t(d) + 1), hipMemcpyDeviceToHost, ctx.stream));
+      eh.push_back(e);
+    }
+  ctx.sync();
+  std::vector<std::vector<const int32_t*>> E(its.size());
+  ti = 0;
+  for (size_t i = 0; i < its.size(); ++i)
+    for (size_t t = 0; t < its[i].right.terms.size(); ++t) E[i].push_back(eh[ti++]);
+  GemmBatch gb;
+  // Omega_k, k = 0..d-2
+  for (size_t i = 0; i < its.size(); ++i) {
+    auto& it = its[i];
+    it.Om.assign(size_t(std::max(d - 1, 0)), nullptr);
+    for (int k = 0; k + 1 < d; ++k) {
+      const int64_t sl = it.sl[k + 1], sr = it.sr[k + 1];
+      it.Om[k] = ctx.ws.alloc_n<double>(size_t(sl * sr));
+      gb.prob(it.Om[k], 1, sl, int(sl), int(sr), 1.0, 0.0);
+      for (size_t t = 0; t < it.left.size(); ++t) {
+        const RandTerm& rt = it.right.terms[t];
+        const int64_t r = rt.x->rks[k + 1];
+        const double sc = std::ldexp(it.left[t].alpha, E[i][t][k + 1]) /
+                          std::sqrt(double(it.pl[k + 1]) * double(it.right.p[k + 1]));
+        gb.seg(it.left[t].W[k + 1], r, 1, rt.W[k + 1], 1, r, int(r), sc);
+      }
+    }
+  }
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "stta_omega");
+  // T_k,t = W_L,t[k]^T X_t,k (sl_k x n r_{k+1}) for k >= 1
+  std::vector<std::vector<std::vector<double*>>> T(its.size());
+  for (size_t i = 0; i < its.size(); ++i) {
+    auto& it = its[i];
+    T[i].assign(it.left.size(), std::vector<double*>(size_t(d), nullptr));
+    for (size_t t = 0; t < it.left.size(); ++t) {
+      const DevTT* x = it.left[t].x;
+      for (int k = 1; k < d; ++k) {
+        const int64_t sl = it.sl[k], r = x->rks[k], n = x->dims[k], r1 = x->rks[k + 1];
+        T[i][t][k] = ctx.ws.alloc_n<double>(size_t(sl * n * r1));
+        gb.add(int(sl), int(n * r1), int(r), 1.0, it.left[t].W[k], r, 1, x->cores[k], 1, r, 0.0, T[i][t][k], 1, sl);
+      }
+    }
+  }
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "stta_t");
+  // Psi_k = sum_t alpha_t T_k,t W_R,t[k+1] / sqrt(pl_k pr_{k+1}); W_L[0] = W_R[d] = 1
+  for (size_t i = 0; i < its.size(); ++i) {
+    auto& it = its[i];
+    it.Psi.assign(size_t(d), nullptr);
+    for (int k = 0; k < d; ++k) {
+      const int64_t sl = it.sl[k], n = x0->dims[k], sr = it.sr[k + 1];
+      it.Psi[k] = ctx.ws.alloc_n<double>(size_t(sl * n * sr));
+      gb.prob(it.Psi[k], 1, sl * n, int(sl * n), int(sr), 1.0, 0.0);
+      for (size_t t = 0; t < it.left.size(); ++t) {
+        const RandTerm& rt = it.right.terms[t];
+        const int64_t r1 = rt.x->rks[k + 1];
+        const double* A = k == 0 ? rt.x->cores[0] : T[i][t][k];
+        const double sc = it.left[t].alpha / std::sqrt(double(it.pl[k]) * double(it.right.p[k + 1]));
+        if (k + 1 < d) gb.seg(A, 1, sl * n, rt.W[k + 1], 1, r1, int(r1), std::ldexp(sc, E[i][t][k + 1]));
+        else gb.seg(A, 1, sl * n, h->one, 1, 1, 1, sc);
+      }
+    }
+  }
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "stta_psi");
+}
+
+// stable_inverse(A; eps = 1e-12) (src/tt_randtools.jl:485-488) of every Omega_k and the
+// local solves of stta (:562-575): per bond, Psi_k <- Psi_k pinv(Omega_k) when Omega_k has
+// fewer rows than columns, else Psi_{k+1} <- pinv(Omega_k) Psi_{k+1}.  pinv of an m x nn
+// matrix M (m >= nn; Omega_k or its transpose): M = Q R, R J = U S (Jacobi), pinv(M) =
+// J diag(1/s^2) (Q R J)^T over s > eps max(s).  The left products act on the left index
+// and the right products on the right index of a core, so all of them are batched.
+std::vector<std::unique_ptr<DevTT>> stta_solve_batch(ttr_handle_s* h, std::vector<SttaItem>& its) {
+  Ctx& ctx = h->ctx;
+  const int d = its[0].right.terms[0].x->d;
+  const std::vector<int64_t>& dims = its[0].right.terms[0].x->dims;
+  struct Bond {
+    bool wide;      // sl < sr: Psi_k <- Psi_k pinv
+    int64_t m, nn;  // M = m x nn
+    double *M, *Q, *B, *J, *sv, *QB, *P;  // P = pinv(M), nn x m
+  };
+  std::vector<std::vector<Bond>> bonds(its.size());
+  std::vector<Copy2D> cp;
+  std::vector<QRJob> qj;
+  for (size_t i = 0; i < its.size(); ++i)
+    for (int k = 0; k + 1 < d; ++k) {
+      const int64_t sl = its[i].sl[k + 1], sr = its[i].sr[k + 1];
+      Bond b{};
+      b.wide = sl < sr;
+      b.m = b.wide ? sr : sl;
+      b.nn = b.wide ? sl : sr;
+      b.M = ctx.ws.alloc_n<double>(size_t(b.m * b.nn));
+      b.Q = ctx.ws.alloc_n<double>(size_t(b.m * b.nn));
+      b.B = ctx.ws.alloc_n<double>(size_t(b.nn * b.nn));
+      b.J = ctx.ws.alloc_n<double>(size_t(b.nn * b.nn));
+      b.sv = ctx.ws.alloc_n<double>(size_t(b.nn));
+      b.QB = ctx.ws.alloc_n<double>(size_t(b.m * b.nn));
+      b.P = ctx.ws.alloc_n<double>(size_t(b.nn * b.m));
+      if (b.wide) cp.push_back(copy_job(its[i].Om[k], sl, 1, b.M, 1, b.m, int(b.m), int(b.nn)));
+      else cp.push_back(copy_job(its[i].Om[k], 1, sl, b.M, 1, b.m, int(b.m), int(b.nn)));
+      qj.push_back(QRJob{b.M, b.m, int(b.m), int(b.nn), b.Q, b.m});
+      bonds[i].push_back(b);
+    }
+  copy2d_batched(ctx, cp);
+  qr_batched(ctx, qj);
+  cp.clear();
+  std::vector<SVDJob> sj;
+  for (auto& bb : bonds)
+    for (auto& b : bb) {
+      cp.push_back(copy_job(b.M, 1, b.m, b.B, 1, b.nn, int(b.nn), int(b.nn), 1.0, 1));
+      sj.push_back(SVDJob{b.B, b.nn, b.J, b.nn, b.sv, int(b.nn), nullptr});
+    }
+  copy2d_batched(ctx, cp);
+  if (!sj.empty()) jacobi_svd_batched(ctx, sj);
+  std::vector<PinvWeights> pw;
+  for (auto& bb : bonds)
+    for (auto& b : bb) pw.push_back(PinvWeights{b.B, b.nn, b.sv, int(b.nn), 1e-12});
+  pinv_weights_batched(ctx, pw);
+  GemmBatch gb;
+  for (auto& bb : bonds)
+    for (auto& b : bb) gb.add(int(b.m), int(b.nn), int(b.nn), 1.0, b.Q, 1, b.m, b.B, 1, b.nn, 0.0, b.QB, 1, b.m);
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "stta_pinv_a");
+  for (auto& bb : bonds)
+    for (auto& b : bb) gb.add(int(b.nn), int(b.m), int(b.nn), 1.0, b.J, 1, b.nn, b.QB, b.m, 1, 0.0, b.P, 1, b.nn);
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "stta_pinv_b");
+  // output ranks: bond k keeps sl_{k+1} (wide) or sr_{k+1}
+  std::vector<std::unique_ptr<DevTT>> outs(its.size());
+  std::vector<std::vector<double*>> mid(its.size());
+  for (size_t i = 0; i < its.size(); ++i) {
+    std::vector<int64_t> r(size_t(d) + 1, 1);
+    for (int k = 0; k + 1 < d; ++k) r[k + 1] = bonds[i][k].wide ? its[i].sl[k + 1] : its[i].sr[k + 1];
+    outs[i] = h->make_tt(dims, r, std::vector<int64_t>(size_t(d), 0));
+    mid[i].assign(size_t(d), nullptr);
+  }
+  // left products: core k+1 <- pinv_k Psi_{k+1} for the tall bonds (into the output core
+  // when the core takes no right product, else into a scratch core)
+  for (size_t i = 0; i < its.size(); ++i)
+    for (int k = 0; k + 1 < d; ++k) {
+      const Bond& b = bonds[i][k];
+      if (b.wide) continue;
+      const int c = k + 1;
+      const int64_t sl = its[i].sl[c], n = dims[c], sr = its[i].sr[c + 1];
+      const bool right_next = c + 1 < d && bonds[i][c].wide;
+      double* dst = right_next ? ctx.ws.alloc_n<double>(size_t(b.nn * n * sr)) : outs[i]->cores[c];
+      mid[i][c] = dst;
+      gb.add(int(b.nn), int(n * sr), int(sl), 1.0, b.P, 1, b.nn, its[i].Psi[c], 1, sl, 0.0, dst, 1, b.nn);
+    }
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "stta_left");
+  // right products: core k <- core k pinv_k = core k P^T for the wide bonds; cores with
+  // neither product are copied
+  cp.clear();
+  for (size_t i = 0; i < its.size(); ++i)
+    for (int c = 0; c < d; ++c) {
+      const bool left = c > 0 && !bonds[i][c - 1].wide;
+      const bool right = c + 1 < d && bonds[i][c].wide;
+      const int64_t lr = outs[i]->rks[c], n = dims[c];
+      const double* src = left ? mid[i][c] : its[i].Psi[c];
+      if (right) {
+        const Bond& b = bonds[i][c];
+        gb.add(int(lr * n), int(b.nn), int(b.m), 1.0, src, 1, lr * n, b.P, b.nn, 1, 0.0, outs[i]->cores[c], 1, lr * n);
+      } else if (!left) {
+        const int64_t rr = outs[i]->rks[c + 1];
+        cp.push_back(copy_job(src, 1, lr * n, outs[i]->cores[c], 1, lr * n, int(lr * n), int(rr)));
+      }
+    }
+  gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "stta_right");
+  copy2d_batched(ctx, cp);
+  return outs;
+}
+
+std::vector<SttaItem> parse_stta_items(ttr_handle_s* h, int64_t n, const ttr_rand_item* items) {
+  std::vector<RandItem> ri = parse_items(h, n, items);
+  std::vector<SttaItem> out(ri.size());
+  for (size_t i = 0; i < ri.size(); ++i) {
+    TTR_REQUIRE(items[i].op_id < 0, ST_INVALID_ARG, "STTA takes no operator");
+    out[i].right = ri[i];
+    TTR_REQUIRE(out[i].right.rks.size() == out[i].right.terms[0].x->dims.size() + 1, ST_INVALID_ARG, "bad rks");
+    TTR_REQUIRE(out[i].right.rks.front() == 1 && out[i].right.rks.back() == 1, ST_INVALID_ARG,
+                "rks must have rks[0] == rks[d] == 1");
   }
   return out;
 }
@@ -1714,9 +2102,92 @@ void sketch_one(ttr_handle_s* h, DevTT* x, DevTTO* H, const int64_t* rks, const 
 }
 }  // namespace
 
+namespace {
+// forward recursive sketch of one TT into host arrays (W_0 = 1)
+void sketch_one_fwd(ttr_handle_s* h, DevTT* x, const int64_t* rks, const ttr_rand_opts* opts, double* const* W_out,
+                    int64_t* sketch_rks) {
+  TTR_REQUIRE(rks != nullptr, ST_INVALID_ARG, "rks must not be NULL");
+  TTR_REQUIRE(to_prec(opts->sketch_prec) == Prec::F64, ST_INVALID_ARG, "the forward sketch is FP64");
+  const int d = x->d;
+  TTR_REQUIRE(rks[0] == 1 && x->rks[0] == 1, ST_INVALID_ARG, "rks[0] must be 1");
+  FwdPlan fp;
+  prepare_fwd(opts, d, x->dims, fp);
+  std::vector<int64_t> r(rks, rks + d + 1);
+  FwdTerm t;
+  t.x = x;
+  t.p = block_counts_fwd(r, fp.bv, d, opts->p, opts->oversampling);
+  fp.pmax = t.p;
+  std::vector<FwdTerm> ts{t};
+  make_sketch_fwd(h, fp, d);
+  sketch_phase_fwd(h, fp, ts, d);
+  h->ctx.sync();
+  for (int k = 0; k <= d; ++k) {
+    const int64_t s = fp.bv[k] * ts[0].p[k];
+    if (sketch_rks) sketch_rks[k] = s;
+    if (!W_out || !W_out[k]) continue;
+    const int64_t cnt = x->rks[k] * s;
+    const double scale = 1.0 / std::sqrt(double(ts[0].p[k]));
+    if (k == 0) {
+      for (int64_t e = 0; e < cnt; ++e) W_out[k][e] = scale;
+      continue;
+    }
+    TTR_HIP(hipMemcpy(W_out[k], ts[0].W[k], size_t(cnt) * 8, hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < cnt; ++e) W_out[k][e] *= scale;
+  }
+}
+}  // namespace
+
 int ttr_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts, double* const* W_out,
                int64_t* sketch_rks) {
-  return guarded(h, [&] { sketch_one(h, h->get(tt_id), nullptr, rks, opts, W_out, sketch_rks); });
+  return guarded(h, [&] {
+    TTR_REQUIRE(opts != nullptr, ST_INVALID_ARG, "options must not be NULL");
+    if (opts->forward) sketch_one_fwd(h, h->get(tt_id), rks, opts, W_out, sketch_rks);
+    else sketch_one(h, h->get(tt_id), nullptr, rks, opts, W_out, sketch_rks);
+  });
+}
+
+int ttr_stta(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const ttr_rand_opts* opts, uint64_t seed_right,
+             int64_t* out_ids) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(out_ids, ST_INVALID_ARG, "bad arguments");
+    std::vector<SttaItem> its = parse_stta_items(h, n, items);
+    stta_sketch_batch(h, its, opts, seed_right);
+    auto outs = stta_solve_batch(h, its);
+    for (int64_t i = 0; i < n; ++i) out_ids[i] = h->add(std::move(outs[size_t(i)]));
+  });
+}
+
+int ttr_stta_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts,
+                    uint64_t seed_right, double* const* Omega_out, double* const* Psi_out, int64_t* sl,
+                    int64_t* sr) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(rks != nullptr, ST_INVALID_ARG, "rks must not be NULL");
+    DevTT* x = h->get(tt_id);
+    const int d = x->d;
+    const double one = 1.0;
+    ttr_rand_item it{1, &one, &tt_id, -1, rks};
+    std::vector<SttaItem> its = parse_stta_items(h, 1, &it);
+    stta_sketch_batch(h, its, opts, seed_right);
+    h->ctx.sync();
+    const SttaItem& S = its[0];
+    for (int k = 0; k <= d; ++k) {
+      if (sl) sl[k] = S.sl[k];
+      if (sr) sr[k] = S.sr[k];
+    }
+    for (int k = 0; k + 1 < d; ++k)
+      if (Omega_out && Omega_out[k])
+        TTR_HIP(hipMemcpy(Omega_out[k], S.Om[k], size_t(S.sl[k + 1] * S.sr[k + 1]) * 8, hipMemcpyDeviceToHost));
+    for (int k = 0; k < d; ++k) {
+      if (!Psi_out || !Psi_out[k]) continue;
+      const int64_t a = S.sl[k], nk = x->dims[k], b = S.sr[k + 1];
+      std::vector<double> raw(size_t(a * nk * b));
+      TTR_HIP(hipMemcpy(raw.data(), S.Psi[k], raw.size() * 8, hipMemcpyDeviceToHost));
+      // device (a, n, b) -> reference (n, a, b)
+      for (int64_t bb = 0; bb < b; ++bb)
+        for (int64_t z = 0; z < nk; ++z)
+          for (int64_t aa = 0; aa < a; ++aa) Psi_out[k][z + nk * (aa + a * bb)] = raw[size_t(aa + a * (z + nk * bb))];
+    }
+  });
 }
 
 int ttr_sketch_op(ttr_handle_t h, int64_t op_id, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts,
@@ -1796,6 +2267,24 @@ int ttr_sketch_blocks(ttr_handle_t h, int64_t d, const int64_t* dims, const int6
   return guarded(h, [&] {
     TTR_REQUIRE(d >= 1 && dims && rks && out && k >= 0 && k < d, ST_INVALID_ARG, "bad arguments");
     std::vector<int64_t> vd(dims, dims + d);
+    if (opts && opts->forward) {
+      TTR_REQUIRE(to_prec(opts->sketch_prec) == Prec::F64, ST_INVALID_ARG, "the forward sketch is FP64");
+      FwdPlan fp;
+      prepare_fwd(opts, int(d), vd, fp);
+      std::vector<int64_t> r(rks, rks + d + 1);
+      fp.pmax = block_counts_fwd(r, fp.bv, int(d), opts->p, opts->oversampling);
+      make_sketch_fwd(h, fp, int(k) + 1);
+      h->ctx.sync();
+      const int64_t left = fp.bv[k], right = fp.bv[k + 1], p = fp.pmax[k + 1], nk = vd[k];
+      TTR_HIP(hipMemcpy(out, fp.S[k], size_t(left * nk * right * p) * 8, hipMemcpyDeviceToHost));
+      if (shape4) {
+        shape4[0] = left;
+        shape4[1] = nk;
+        shape4[2] = right;
+        shape4[3] = p;
+      }
+      return;
+    }
     std::vector<RandItem> its(1);
     its[0].rks.assign(rks, rks + d + 1);
     SketchPlan sp;

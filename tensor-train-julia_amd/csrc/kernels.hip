@@ -163,7 +163,7 @@ __global__ void sketch_gen_kernel(const SketchGen* __restrict__ jobs, uint64_t s
     const int64_t z = rest % J.n;
     const int64_t col = rest / J.n;  // b + bvl*j
     if (J.out) store_prec(J.out, z + J.n * (beta + int64_t(J.bvr) * col), J.scale * g, prec);
-    if (J.ref_out) J.ref_out[e] = g;
+    if (J.ref_out) J.ref_out[e] = J.ref_scale * g;
   }
 }
 

@@ -422,7 +422,35 @@ __global__ __launch_bounds__(256) void column_norms_kernel(const SVDJob* __restr
   }
 }
 
+// pseudo-inverse weights: one workgroup per matrix, the maximum singular value by
+// a workgroup reduction, then column c of BJ scaled by 1/sv_c^2 or zeroed
+__global__ __launch_bounds__(256) void pinv_weights_kernel(const PinvWeights* __restrict__ jobs) {
+  const PinvWeights J = jobs[blockIdx.x];
+  __shared__ double red[4];
+  double mx = 0.0;
+  for (int c = threadIdx.x; c < J.p; c += 256) mx = fmax(mx, J.sv[c]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  const double cut = J.eps * mx;
+  const int64_t tot = int64_t(J.p) * J.p;
+  for (int64_t e = threadIdx.x; e < tot; e += 256) {
+    const int64_t i = e % J.p, c = e / J.p;
+    const double s = J.sv[c];
+    J.BJ[i + c * J.ld] *= s > cut ? 1.0 / (s * s) : 0.0;
+  }
+}
+
 }  // namespace
+
+void pinv_weights_batched(Ctx& ctx, const std::vector<PinvWeights>& jobs) {
+  if (jobs.empty()) return;
+  const PinvWeights* d = ctx.upload(jobs);
+  hipLaunchKernelGGL(pinv_weights_kernel, dim3(unsigned(jobs.size())), dim3(256), 0, ctx.stream, d);
+  TTR_HIP(hipGetLastError());
+}
 
 template <int WB>
 void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {

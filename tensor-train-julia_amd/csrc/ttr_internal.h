@@ -267,6 +267,16 @@ struct SVDJob {
 };
 // returns the number of columns per visit (for the work counters)
 int jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs);
+// pseudo-inverse weights after a Jacobi SVD: column c of BJ (p x p, ld) is scaled
+// by 1/sv_c^2 when sv_c > eps * max(sv), else zeroed
+struct PinvWeights {
+  double* BJ;
+  int64_t ld;
+  const double* sv;
+  int p;
+  double eps;
+};
+void pinv_weights_batched(Ctx& ctx, const std::vector<PinvWeights>& jobs);
 
 // ---------------------------------------------------------------------------
 // elementwise / data-movement kernels
@@ -321,10 +331,11 @@ void pow2_normalize_batched(Ctx& ctx, const std::vector<Pow2Job>& jobs, Prec pre
 // sketch draws: Philox4x32-10 + Box-Muller (FP64), see rng_normal in kernels.hip
 struct SketchGen {
   void* out;          // destination (precision prec), layout (n, bvr, bvl, p) with n fastest
-  double* ref_out;    // optional FP64 copy in the reference order (bvr*n, bvl, p) (before scaling)
-  int32_t n, bvr, bvl, p;  // bvr = left rank of the block (bv_{k+1}), bvl = right rank (bv_k)
+  double* ref_out;    // optional FP64 copy in the reference order (bvr*n, bvl, p), times ref_scale
+  int32_t n, bvr, bvl, p;  // bvr = left rank of the block, bvl = right rank
   int32_t core;       // core index k (0-based), part of the counter
   double scale;
+  double ref_scale = 1.0;
 };
 void sketch_generate(Ctx& ctx, const std::vector<SketchGen>& jobs, uint64_t seed, Prec prec);
 // raw draws in reference order for a given core (used by tests)

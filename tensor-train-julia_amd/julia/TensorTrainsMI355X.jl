@@ -5,7 +5,8 @@ Julia binding of libttround (include/ttround.h): the MI355X rounding path of
 TensorTrains behind `ccall`.  Loading this module after `TensorTrains` adds
 GPU methods with the reference's names and keyword defaults
 (src/tt_rounding.jl:146-167, src/tt_randtools.jl:112-283,
-src/sketches.jl:175-315,317-472,645-817,930-1001,1047-1070,
+src/sketches.jl:175-315,317-472,645-817,930-1001,1047-1070,1126-1156,
+src/tt_randtools.jl:552-638,
 src/tt_operations.jl:81-112,170-194,252-258,
 302-421); the
 solver callers (tt_gmres, arnoldi, davidson, als/mals/dmrg via orthogonalize)
@@ -22,7 +23,7 @@ module TensorTrainsMI355X
 
 using TensorTrains
 import TensorTrains: tt_rounding, tt_rounding!, orthogonalize, ttrand_rounding, tt_recursive_sketch,
-                     tt_sketch, tt_combined_sketch, dot_par, dot_operator, partial_contraction
+                     tt_sketch, tt_combined_sketch, stta, stta_sketch, dot_par, dot_operator, partial_contraction
 import LinearAlgebra: dot
 
 const lib = get(ENV, "TTROUND_LIB", joinpath(@__DIR__, "..", "ttround", "libttround.so"))
@@ -153,6 +154,7 @@ struct RandOpts
     seed::UInt64
     p::Ptr{Int64}
     oversampling::Float64
+    forward::Int32       # 0: reverse = true (right-to-left), 1: reverse = false
 end
 
 const PREC = Dict(Float64 => Int32(0), Float32 => Int32(1))
@@ -177,7 +179,7 @@ function rand_round(α::Vector{Float64}, A, y::Vector{TTvector{Float64,N}}, rks;
     out = Ref{Int64}(0)
     GC.@preserve α ids r begin
         item = Ref(RandItem(length(y), pointer(α), pointer(ids), op, pointer(r)))
-        opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[sketch_type], block_rks, UInt64(seed), C_NULL, 1.0))
+        opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[sketch_type], block_rks, UInt64(seed), C_NULL, 1.0, Int32(0)))
         check(ccall((:ttr_round_rand, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, Ref{Int64}),
                     handle(), 1, item, opts, out))
     end
@@ -221,10 +223,13 @@ end
 # tt_recursive_sketch(T, [H,] A, rks; reverse=true, ...) (src/sketches.jl:175-240,300-303,317-459)
 function _recursive_sketch(::Type{T}, H, A::TTvector{Float64,N}, rks::Vector{Int}; orthogonal=true, reverse=true,
                            seed=1234, block_rks::Int=N, p=0, oversampling=1) where {T,N}
-    reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
-    bv = _block_ranks(A.ttv_dims, block_rks, N)
-    pp = p == 0 ? TensorTrains.compute_sketch_blocks_heuristic(rks, bv, N; reverse=true) : collect(Int, p)
-    oversampling != 1 && (pp[1:N] .= ceil.(Int, oversampling .* pp[1:N]))
+    reverse || H === nothing || throw(ArgumentError("the GPU forward sketch takes no operator"))
+    bv = reverse ? _block_ranks(A.ttv_dims, block_rks, N) : _block_ranks_fwd(A.ttv_dims, block_rks, N)
+    pp = p == 0 ? TensorTrains.compute_sketch_blocks_heuristic(rks, bv, N; reverse=reverse) : collect(Int, p)
+    if oversampling != 1
+        rng = reverse ? (1:N) : (2:N+1)
+        pp[rng] .= ceil.(Int, oversampling .* pp[rng])
+    end
     s = bv .* pp
     Hr = H === nothing ? ones(Int, N + 1) : collect(Int, H.tto_rks)
     W = [zeros(Float64, A.ttv_rks[k] * Hr[k], s[k]) for k in 1:N+1]
@@ -233,7 +238,8 @@ function _recursive_sketch(::Type{T}, H, A::TTvector{Float64,N}, rks::Vector{Int
     r, pv, srk = collect(Int64, rks), collect(Int64, pp), zeros(Int64, N + 1)
     GC.@preserve W r pv begin
         ptrs = [pointer(w) for w in W]
-        opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[T], block_rks, UInt64(seed), pointer(pv), 1.0))
+        opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[T], block_rks, UInt64(seed), pointer(pv), 1.0,
+                            Int32(reverse ? 0 : 1)))
         if H === nothing
             check(ccall((:ttr_sketch, lib), Cint,
                         (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{RandOpts}, Ptr{Ptr{Float64}}, Ptr{Int64}),
@@ -250,11 +256,26 @@ function _recursive_sketch(::Type{T}, H, A::TTvector{Float64,N}, rks::Vector{Int
     return [reshape(W[k], A.ttv_rks[k], Hr[k], s[k]) for k in 1:N+1], collect(Int, srk)
 end
 
+# block ranks of the forward sketch (src/sketches.jl:244-248)
+function _block_ranks_fwd(dims, block_rks::Int, N::Int)
+    bv = ones(Int, N + 1)
+    bv[2:N+1] .= block_rks
+    for k = 1:N
+        bv[k + 1] = min(bv[k + 1], dims[k] * bv[k])
+    end
+    return bv
+end
+
 _rks_of(rmax::Int, N) = (r = fill(rmax, N + 1); r[N + 1] = 1; r)
 _rks_of(rks::Vector{Int}, N) = rks
 
-tt_recursive_sketch(::Type{T}, A::TTvector{Float64,N}, rks_or_rmax; kwargs...) where {T<:Union{Float64,Float32},N} =
-    _recursive_sketch(T, nothing, A, _rks_of(rks_or_rmax, N); kwargs...)
+_rks_of_fwd(rmax::Int, N) = (r = fill(rmax, N + 1); r[1] = 1; r)
+_rks_of_fwd(rks::Vector{Int}, N) = rks
+
+tt_recursive_sketch(::Type{T}, A::TTvector{Float64,N}, rks_or_rmax; reverse=true,
+                    kwargs...) where {T<:Union{Float64,Float32},N} =
+    _recursive_sketch(T, nothing, A, reverse ? _rks_of(rks_or_rmax, N) : _rks_of_fwd(rks_or_rmax, N);
+                      reverse=reverse, kwargs...)
 tt_recursive_sketch(::Type{T}, H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax;
                     kwargs...) where {T<:Union{Float64,Float32},N} =
     _recursive_sketch(T, H, A, _rks_of(rks_or_rmax, N); kwargs...)
@@ -271,7 +292,7 @@ function _tt_sketch(::Type{T}, H, A::TTvector{Float64,N}, s::Int; reverse=true, 
     op = H === nothing ? -1 : upload(H)
     sa = Ref{Int64}(0)
     ids, ops = Int64[id], Int64[op]
-    opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[T], block_rks, UInt64(seed), C_NULL, 1.0))
+    opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[T], block_rks, UInt64(seed), C_NULL, 1.0, Int32(0)))
     check(ccall((:ttr_tt_sketch, lib), Cint,
                 (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ref{RandOpts}, Ptr{Float64}, Int64, Ref{Int64}),
                 handle(), 1, ids, ops, s, opts, C_NULL, 0, sa))
@@ -321,5 +342,54 @@ tt_combined_sketch(A::TTvector{Float64,N}, rks_or_rmax, s::Int; kwargs...) where
     tt_combined_sketch(Float64, A, rks_or_rmax, s; kwargs...)
 tt_combined_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax, s::Int; kwargs...) where {N} =
     tt_combined_sketch(Float64, H, A, rks_or_rmax, s; kwargs...)
+
+# ------------------------------------------------------------------ STTA (src/tt_randtools.jl:552-638)
+
+function _stta(α::Vector{Float64}, y::Vector{TTvector{Float64,N}}, rks::Vector{Int}; seed_left=1234, seed_right=5678,
+               orthogonal=true, block_rks::Int=N, timer=nothing) where {N}
+    ids = [upload(t) for t in y]
+    r = collect(Int64, rks)
+    out = Ref{Int64}(0)
+    GC.@preserve α ids r begin
+        item = Ref(RandItem(length(y), pointer(α), pointer(ids), -1, pointer(r)))
+        opts = Ref(RandOpts(orthogonal ? 1 : 0, Int32(0), block_rks, UInt64(seed_left), C_NULL, 1.0, Int32(0)))
+        check(ccall((:ttr_stta, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, UInt64, Ref{Int64}),
+                    handle(), 1, item, opts, UInt64(seed_right), out))
+    end
+    foreach(free, ids)
+    return download(out[], Val(N))
+end
+
+_stta_rks(rmax::Int, N) = (r = ones(Int, N + 1); r[2:N] .= rmax; r)
+_stta_rks(rks::Vector{Int}, N) = rks
+
+stta(y::TTvector{Float64,N}, rks=TensorTrains.default_rank_heuristic(y); kwargs...) where {N} =
+    _stta([1.0], [y], _stta_rks(rks, N); kwargs...)
+stta(α::Vector{Float64}, y::Vector{TTvector{Float64,N}}, rks=TensorTrains.default_rank_heuristic(y);
+     kwargs...) where {N} = _stta(α, y, _stta_rks(rks, N); kwargs...)
+
+# stta_sketch(x, rks; seed_left, seed_right, orthogonal, block_rks) (src/sketches.jl:1126-1156)
+function stta_sketch(x::TTvector{Float64,N}, rks::Vector{Int}; seed_left=1234, seed_right=5678, orthogonal=true,
+                     block_rks::Int=N, timer=nothing) where {N}
+    sl, sr = zeros(Int64, N + 1), zeros(Int64, N + 1)
+    id = upload(x)
+    r = collect(Int64, rks)
+    opts = Ref(RandOpts(orthogonal ? 1 : 0, Int32(0), block_rks, UInt64(seed_left), C_NULL, 1.0, Int32(0)))
+    nullp = fill(Ptr{Float64}(C_NULL), N)
+    # the sketch ranks first (no outputs), then the arrays
+    check(ccall((:ttr_stta_sketch, lib), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{RandOpts}, UInt64,
+                Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ptr{Int64}, Ptr{Int64}),
+                handle(), id, r, opts, UInt64(seed_right), nullp, nullp, sl, sr))
+    Ω = [zeros(Float64, sl[k + 1], sr[k + 1]) for k in 1:N-1]
+    Ψ = [zeros(Float64, x.ttv_dims[k], sl[k], sr[k + 1]) for k in 1:N]
+    GC.@preserve Ω Ψ begin
+        check(ccall((:ttr_stta_sketch, lib), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{RandOpts}, UInt64,
+                    Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ptr{Int64}, Ptr{Int64}),
+                    handle(), id, r, opts, UInt64(seed_right), [pointer(o) for o in Ω], [pointer(q) for q in Ψ],
+                    sl, sr))
+    end
+    free(id)
+    return Ω, Ψ
+end
 
 end # module

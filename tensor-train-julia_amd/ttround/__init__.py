@@ -10,7 +10,8 @@ from .api import *  # noqa: F401,F403
 from .api import (DeviceTT, DeviceTTO, TTvector, TToperator, default_rank_heuristic, lincomb, lincomb_device,  # noqa: F401
                   mpo_apply_device, norm, norm_batch, orthogonalize, orthogonalize_batch, r_and_d_to_rks, rand_tt,
                   round_rand_batch, round_svd_batch, sketch_blocks, sketch_draws, tt_recursive_sketch, tt_rounding,
-                  tt_sketch, tt_sketch_batch, tt_combined_sketch,
+                  tt_sketch, tt_sketch_batch, tt_combined_sketch, stta, stta_batch, stta_sketch,
+                  block_structure_forward,
                   tt_rounding_, tt_scale, ttrand_rounding, zeros_tt, is_leftorthogonal, is_rightorthogonal,
                   dot, dot_par, dot_operator, dot_batch, dot_operator_batch, partial_contraction)
 

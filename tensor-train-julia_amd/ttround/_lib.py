@@ -52,7 +52,7 @@ class RandItem(C.Structure):
 
 class RandOpts(C.Structure):
     _fields_ = [("orthogonal", C.c_int32), ("sketch_prec", C.c_int32), ("block_rks", i64), ("seed", C.c_uint64),
-                ("p", i64p), ("oversampling", C.c_double)]
+                ("p", i64p), ("oversampling", C.c_double), ("forward", C.c_int32)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/ttround.h
@@ -80,6 +80,8 @@ SIGNATURES = {
     "ttr_sketch_op": (C.c_int, [C.c_void_p, i64, i64, i64p, C.POINTER(RandOpts), f64pp, i64p]),
     "ttr_tt_sketch": (C.c_int, [C.c_void_p, i64, i64p, i64p, i64, C.POINTER(RandOpts), f64p, i64, i64p]),
     "ttr_sketch_blocks": (C.c_int, [C.c_void_p, i64, i64p, i64p, C.POINTER(RandOpts), i64, f64p, i64p]),
+    "ttr_stta": (C.c_int, [C.c_void_p, i64, C.POINTER(RandItem), C.POINTER(RandOpts), C.c_uint64, i64p]),
+    "ttr_stta_sketch": (C.c_int, [C.c_void_p, i64, i64p, C.POINTER(RandOpts), C.c_uint64, f64pp, f64pp, i64p, i64p]),
     "ttr_sketch_draws": (C.c_int, [C.c_void_p, i64, i64, C.c_uint64, f64p]),
     "ttr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "ttr_profile_reset": (C.c_int, [C.c_void_p]),

@@ -47,8 +47,9 @@ def r_and_d_to_rks(rks, dims, rmax: int = 1024) -> List[int]:
     return out
 
 
-def _opts(orthogonal=True, sketch_prec="f64", block_rks=None, seed=1234, p=None, oversampling=1.0):
+def _opts(orthogonal=True, sketch_prec="f64", block_rks=None, seed=1234, p=None, oversampling=1.0, forward=False):
     o = L.RandOpts()
+    o.forward = 1 if forward else 0
     o.orthogonal = 1 if orthogonal else 0
     o.sketch_prec = L.PREC[sketch_prec] if isinstance(sketch_prec, str) else int(sketch_prec)
     o.block_rks = int(block_rks) if block_rks else 0
@@ -294,16 +295,42 @@ def tt_sketch_batch(xs: Sequence[DeviceTT], s: int, ops=None, orthogonal: bool =
     return out, int(sa[0])
 
 
+def block_structure_forward(dims, rks, block_rks: Optional[int] = None, p=None, oversampling: float = 1.0):
+    """Block ranks and counts of the forward sketch (``src/sketches.jl:113-121,244-255``)."""
+    d = len(dims)
+    bv = [1] * (d + 1)
+    for k in range(1, d + 1):
+        bv[k] = int(block_rks) if block_rks else d
+    for k in range(d):
+        bv[k + 1] = min(bv[k + 1], int(dims[k]) * bv[k])
+    if p is None:
+        pp = [int(math.ceil(int(r) / b)) for r, b in zip(rks, bv)]
+        for k in range(d - 1, 0, -1):
+            pp[k] = max(pp[k], pp[k + 1])
+        pp[0] = 1
+    else:
+        pp = [int(v) for v in p]
+    if oversampling != 1:
+        for k in range(1, d + 1):
+            pp[k] = int(math.ceil(oversampling * pp[k]))
+    return bv, pp
+
+
 def sketch_blocks(dims, rks, k: int, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
-                  sketch_prec: str = "f64", p=None, oversampling: float = 1.0,
+                  sketch_prec: str = "f64", p=None, oversampling: float = 1.0, forward: bool = False,
                   handle: Optional[L.Handle] = None) -> np.ndarray:
     """The sketch blocks of core ``k`` (0-based) as the device uses them, as the
-    ``(bv_{k+1}, n_k, bv_k, p_k)`` array of ``generate_sketch_blocks``."""
+    ``(bv_{k+1}, n_k, bv_k, p_k)`` array of ``generate_sketch_blocks`` (forward:
+    ``(bv_k, n_k, bv_{k+1}, p_{k+1})``)."""
     h = handle or L.default_handle()
     d = len(dims)
-    o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling)
-    bv, pp = block_structure(dims, rks, block_rks, p, oversampling)
-    buf = np.zeros(int(bv[k + 1] * dims[k] * bv[k] * pp[k]))
+    o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling, forward)
+    if forward:
+        bv, pp = block_structure_forward(dims, rks, block_rks, p, oversampling)
+        buf = np.zeros(int(bv[k] * dims[k] * bv[k + 1] * pp[k + 1]))
+    else:
+        bv, pp = block_structure(dims, rks, block_rks, p, oversampling)
+        buf = np.zeros(int(bv[k + 1] * dims[k] * bv[k] * pp[k]))
     shape = np.zeros(4, np.int64)
     da, ra = L.as_i64(dims), L.as_i64(rks)
     h.check(h.lib.ttr_sketch_blocks(h.h, d, L.ptr_i64(da), L.ptr_i64(ra), C.byref(o), int(k), L.ptr_f64(buf),
@@ -586,12 +613,14 @@ def tt_recursive_sketch(*args, orthogonal: bool = True, reverse: bool = True, se
         H, A, rks = args
     else:
         H, (A, rks) = None, args
-    if not reverse:
-        raise NotImplementedError("only the reverse (right-to-left) sketch is provided")
+    if not reverse and H is not None:
+        raise NotImplementedError("the forward sketch is provided for TT vectors only")
     if isinstance(rks, (int, np.integer)):
         r = [int(rks)] * (A.N + 1)
-        r[A.N] = 1
+        r[A.N if reverse else 0] = 1
         rks = r
+    if not reverse:
+        return _forward_sketch(A, rks, orthogonal, seed, block_rks, p, oversampling)
     dev = A._dev()
     h = dev.handle
     dop = DeviceTTO(H, h) if H is not None else None
@@ -674,3 +703,110 @@ def tt_combined_sketch(*args, reverse: bool = True, orthogonal: bool = True, see
     W[0] = np.ones((1, 1)) if H is None else np.ones((1, 1, 1))
     srk[0] = 1
     return sketch, W, srk
+
+
+def _forward_sketch(A, rks, orthogonal, seed, block_rks, p, oversampling):
+    """``tt_recursive_sketch(A, rks; reverse=false)`` (``src/sketches.jl:241-303``), FP64."""
+    dev = A._dev()
+    h = dev.handle
+    o, keep = _opts(orthogonal, "f64", block_rks, seed, p, oversampling, forward=True)
+    ra = L.as_i64(rks)
+    bv, pp = block_structure_forward(A.ttv_dims, rks, block_rks, p, oversampling)
+    srk = [bv[k] * pp[k] for k in range(A.N + 1)]
+    Ws = [np.zeros((A.ttv_rks[k], srk[k]), order="F") for k in range(A.N + 1)]
+    arr = (L.f64p * (A.N + 1))(*[L.ptr_f64(w) for w in Ws])
+    out_srk = np.zeros(A.N + 1, np.int64)
+    try:
+        h.check(h.lib.ttr_sketch(h.h, dev.id, L.ptr_i64(ra), C.byref(o), arr, L.ptr_i64(out_srk)))
+    finally:
+        dev.free()
+    assert [int(v) for v in out_srk] == srk
+    return Ws, srk
+
+
+def _rks_stta(rks, N):
+    if isinstance(rks, (int, np.integer)):
+        r = [1] * (N + 1)
+        for k in range(1, N):
+            r[k] = int(rks)
+        return r
+    return [int(v) for v in rks]
+
+
+def stta_sketch(x: TTvector, rks, seed_left: int = 1234, seed_right: int = 5678, orthogonal: bool = True,
+                block_rks: Optional[int] = None):
+    """``stta_sketch(x, rks; seed_left, seed_right, orthogonal, block_rks)``
+    (``src/sketches.jl:1126-1156``).  Returns ``(Omega, Psi)``: ``Omega[k]`` of shape
+    ``(sl_{k+1}, sr_{k+1})`` and ``Psi[k]`` of shape ``(n_k, sl_k, sr_{k+1})``."""
+    N = x.N
+    rks = _rks_stta(rks, N)
+    bvl, pl = block_structure_forward(x.ttv_dims, rks, block_rks, None, 1.5)
+    bvr, pr = block_structure(x.ttv_dims, rks, block_rks)
+    sl = [bvl[k] * pl[k] for k in range(N + 1)]
+    sr = [bvr[k] * pr[k] for k in range(N + 1)]
+    Om = [np.zeros((sl[k + 1], sr[k + 1]), order="F") for k in range(N - 1)]
+    Ps = [np.zeros((x.ttv_dims[k], sl[k], sr[k + 1]), order="F") for k in range(N)]
+    dev = x._dev()
+    h = dev.handle
+    o, keep = _opts(orthogonal, "f64", block_rks, seed_left)
+    ra = L.as_i64(rks)
+    om_arr = (L.f64p * max(N - 1, 1))(*[L.ptr_f64(w) for w in Om])
+    ps_arr = (L.f64p * N)(*[L.ptr_f64(w) for w in Ps])
+    osl, osr = np.zeros(N + 1, np.int64), np.zeros(N + 1, np.int64)
+    try:
+        h.check(h.lib.ttr_stta_sketch(h.h, dev.id, L.ptr_i64(ra), C.byref(o), int(seed_right) & 0xFFFFFFFFFFFFFFFF,
+                                      om_arr, ps_arr, L.ptr_i64(osl), L.ptr_i64(osr)))
+    finally:
+        dev.free()
+    assert [int(v) for v in osl] == sl and [int(v) for v in osr] == sr
+    return Om, Ps
+
+
+def stta_batch(items, seed_left: int = 1234, seed_right: int = 5678, orthogonal: bool = True,
+               block_rks: Optional[int] = None) -> List[DeviceTT]:
+    """Batched ``stta``: ``items`` is a list of ``(alphas, [DeviceTT...], rks)``."""
+    h = items[0][1][0].handle
+    keep = []
+    arr = (L.RandItem * len(items))()
+    for i, (alpha, terms, rks) in enumerate(items):
+        a = np.ascontiguousarray(np.asarray(alpha, dtype=np.float64))
+        ids = L.as_i64([t.id for t in terms])
+        r = L.as_i64(_rks_stta(rks, terms[0].d))
+        keep += [a, ids, r]
+        arr[i].nterms = len(terms)
+        arr[i].alpha = L.ptr_f64(a)
+        arr[i].term_ids = L.ptr_i64(ids)
+        arr[i].op_id = -1
+        arr[i].rks = L.ptr_i64(r)
+    o, k2 = _opts(orthogonal, "f64", block_rks, seed_left)
+    out = np.zeros(len(items), np.int64)
+    h.check(h.lib.ttr_stta(h.h, len(items), arr, C.byref(o), int(seed_right) & 0xFFFFFFFFFFFFFFFF, L.ptr_i64(out)))
+    return [DeviceTT(h, t) for t in out]
+
+
+def stta(*args, seed_left: int = 1234, seed_right: int = 5678, orthogonal: bool = True,
+         block_rks: Optional[int] = None) -> TTvector:
+    """Streaming TT approximation (``src/tt_randtools.jl:552-638``).
+
+    Forms: ``stta(y[, rks_or_rmax])`` and ``stta(alpha, ys[, rks_or_rmax])``; the
+    default ranks follow ``default_rank_heuristic`` of ``y`` (or ``ys``).  The
+    left sketch draws from ``seed_left``, the right one from ``seed_right``
+    (the library's counter-based streams)."""
+    if isinstance(args[0], TTvector):
+        y = args[0]
+        alpha, ys = [1.0], [y]
+        rks = _rks_stta(args[1], y.N) if len(args) > 1 else default_rank_heuristic(y)
+    else:
+        alpha, ys = list(args[0]), list(args[1])
+        if len(alpha) != len(ys) or any(t.ttv_dims != ys[0].ttv_dims for t in ys):
+            raise L.DimensionMismatch("length(alpha) must equal length(y) and all dims must agree")
+        rks = _rks_stta(args[2], ys[0].N) if len(args) > 2 else default_rank_heuristic(ys)
+    devs = [t._dev() for t in ys]
+    try:
+        out = stta_batch([(alpha, devs, rks)], seed_left, seed_right, orthogonal, block_rks)[0]
+        res = out.to_host()
+        out.free()
+    finally:
+        for d in devs:
+            d.free()
+    return res

@@ -612,3 +612,129 @@ def test_tt_sketch_errors(T):
     assert st == L.TTR_INVALID_ARG and sa[0] >= 4
     dx.free()
     dy.free()
+
+
+# ------------------------------------------------------------------ STTA (two-sided sketches)
+
+@pytest.mark.parametrize("orthogonal,block_rks", [(False, None), (True, None), (True, 3)])
+def test_forward_sketch_vs_oracle(T, orthogonal, block_rks):
+    say(f"case: forward sketch orthogonal={orthogonal} block_rks={block_rks}")
+    dims = (2, 3, 2, 4, 2, 3)
+    x = rand_tt(dims, [1, 2, 6, 8, 8, 3, 1], 150)
+    rks = [1, 5, 7, 7, 6, 3, 1]
+    W, s = T.tt_recursive_sketch(to_api(x), rks, reverse=False, orthogonal=orthogonal, seed=21, block_rks=block_rks,
+                                 oversampling=1.5)
+    prov = device_blocks_provider(dims, rks, orthogonal=orthogonal, seed=21, block_rks=block_rks, oversampling=1.5,
+                                  forward=True)
+    Wr, sr = O.tt_recursive_sketch_forward(x, rks, prov, orthogonal, block_rks, oversampling=1.5)
+    assert s == sr
+    for a, b in zip(W, Wr):
+        assert a.shape == b.shape
+        assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b)
+
+
+def _stta_provs(dims, rks, orthogonal, block_rks, sl, sr_):
+    pl = device_blocks_provider(dims, rks, orthogonal=orthogonal, seed=sl, block_rks=block_rks, oversampling=1.5,
+                                forward=True)
+    pr = device_blocks_provider(dims, rks, orthogonal=orthogonal, seed=sr_, block_rks=block_rks)
+    return pl, pr
+
+
+@pytest.mark.parametrize("orthogonal", [False, True])
+def test_stta_sketch_vs_oracle(T, orthogonal):
+    say(f"case: stta_sketch orthogonal={orthogonal}")
+    dims = (3, 2, 4, 2, 3)
+    x = rand_tt(dims, [1, 3, 5, 6, 3, 1], 151)
+    rks = [1, 3, 4, 5, 3, 1]
+    Om, Ps = T.stta_sketch(to_api(x), rks, seed_left=31, seed_right=32, orthogonal=orthogonal)
+    pl, pr = _stta_provs(dims, rks, orthogonal, None, 31, 32)
+    Omr, Psr, _, _ = O.stta_sketch(x, rks, pl, pr, orthogonal)
+    for a, b in zip(Om, Omr):
+        assert a.shape == b.shape and np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b)
+    for a, b in zip(Ps, Psr):
+        assert a.shape == b.shape and np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("orthogonal", [False, True])
+def test_stta_exact_recovery(T, orthogonal):
+    # test_ttrand.jl:136-160 pattern.  With sketch ranks above the true ranks a low-rank
+    # input is recovered to rounding; with a 1e-8 perturbation of higher rank the
+    # pseudo-inverses (1e-12 cutoff) resolve those directions only to ~1e-7 (the FP64
+    # oracle gives 4e-8..1e-6 on this case), so that case gets the reference's quality
+    # bound instead (isapprox with atol 1e-6 on a unit-size tensor)
+    say(f"case: stta exact recovery orthogonal={orthogonal}")
+    dims = (2,) * 8
+    a = rand_tt(dims, [1, 2, 4, 4, 4, 4, 4, 2, 1], 152)
+    rks = [1, 2, 4, 8, 16, 8, 4, 2, 1]
+    y = to_oracle(T.stta(to_api(a), rks, seed_left=7777, seed_right=8888, orthogonal=orthogonal))
+    err = O.rel_err(a, y)
+    say(f"  ranks {y.rks}, rel err vs input {err:.3e}")
+    assert err < 1e-10 and y.ot == [0] * 8
+    pl, pr = _stta_provs(dims, rks, orthogonal, None, 7777, 8888)
+    ref = O.stta([1.0], [a], rks, pl, pr, orthogonal)
+    assert y.rks == ref.rks and O.rel_err(ref, y) < 1e-10
+    pert = O.tt_add(a, O.tt_scale(1e-8, rand_tt(dims, [1, 2, 3, 3, 3, 3, 3, 2, 1], 153)))
+    yp = to_oracle(T.stta(to_api(pert), rks, seed_left=7777, seed_right=8888, orthogonal=orthogonal))
+    say(f"  perturbed input: rel err {O.rel_err(pert, yp):.3e}")
+    assert O.rel_err(pert, yp) < 1e-5
+
+
+def test_stta_truncating_vs_oracle(T):
+    say("case: stta truncating vs oracle")
+    dims = (3,) * 6
+    ys = [rand_tt(dims, 5, 154 + j) for j in range(3)]
+    al = [1.0, 0.3, -0.1]
+    rks = [1, 3, 4, 4, 4, 3, 1]
+    y = to_oracle(T.stta(al, [to_api(t) for t in ys], rks, seed_left=41, seed_right=42, orthogonal=False))
+    pl, pr = _stta_provs(dims, rks, False, None, 41, 42)
+    ref = O.stta(al, ys, rks, pl, pr, False)
+    assert y.rks == ref.rks
+    err = O.rel_err(ref, y)
+    say(f"  ranks {y.rks}, rel err vs oracle {err:.3e}, vs input {O.rel_err(O.tt_lincomb(ys, al), y):.3e}")
+    assert err < 1e-10
+
+
+def test_stta_batch_uneven(T):
+    say("case: stta batch of uneven items")
+    dims = (2, 3, 2, 3, 2)
+    items, refs = [], []
+    for i, (r, m) in enumerate(((2, 1), (4, 2), (3, 3))):
+        ys = [rand_tt(dims, r, 160 + 10 * i + j) for j in range(m)]
+        al = [0.5 ** j for j in range(m)]
+        rks = O.default_rank_heuristic(ys)
+        items.append((al, [T.DeviceTT.upload(to_api(t)) for t in ys], rks))
+        pl, pr = _stta_provs(dims, rks, True, None, 51, 52)
+        refs.append(O.stta(al, ys, rks, pl, pr, True))
+    outs = T.stta_batch(items, seed_left=51, seed_right=52, orthogonal=True)
+    for o, ref in zip(outs, refs):
+        got = dev_to_oracle(o)
+        assert got.rks == ref.rks
+        assert O.rel_err(ref, got) < 1e-10
+        o.free()
+    for _, ds, _ in items:
+        for d_ in ds:
+            d_.free()
+
+
+def test_stta_errors(T):
+    say("case: stta argument errors")
+    x = rand_tt((2, 3, 2), [1, 2, 2, 1], 170)
+    dx = T.DeviceTT.upload(to_api(x))
+    import ctypes as C
+    from ttround import _lib as L
+    h = dx.handle
+    arr = (L.RandItem * 1)()
+    ids, al, rk = L.as_i64([dx.id]), np.ones(1), L.as_i64([1, 2, 2, 1])
+    arr[0].nterms, arr[0].alpha, arr[0].term_ids, arr[0].op_id, arr[0].rks = 1, L.ptr_f64(al), L.ptr_i64(ids), -1, \
+        L.ptr_i64(rk)
+    out = np.zeros(1, np.int64)
+    o, _ = T.api._opts(False, "f32")
+    assert h.lib.ttr_stta(h.h, 1, arr, C.byref(o), 5, L.ptr_i64(out)) == L.TTR_INVALID_ARG
+    o, _ = T.api._opts(False)
+    arr[0].op_id = 12345
+    assert h.lib.ttr_stta(h.h, 1, arr, C.byref(o), 5, L.ptr_i64(out)) != L.TTR_OK
+    # the randomized rounding takes the right-to-left sketch only
+    arr[0].op_id = -1
+    o, _ = T.api._opts(False, forward=True)
+    assert h.lib.ttr_round_rand(h.h, 1, arr, C.byref(o), L.ptr_i64(out)) == L.TTR_INVALID_ARG
+    dx.free()
