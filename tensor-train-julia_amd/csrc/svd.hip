@@ -35,14 +35,13 @@ template <int WB_>
 struct Jac {
   static constexpr int WB = WB_;             // column block width
   static constexpr int WC = 2 * WB;          // columns per visit
-  static constexpr int GLD = WC + 1;         // padded LDS row
+  static constexpr int GLD = WC + 2;         // LDS row pitch (2 mod 32 doubles: see inner_sweep)
   static constexpr int JT = 16 * WB;         // threads per visit
   static constexpr int NW = JT / 64;         // waves
   static constexpr int NPAIR = WB;           // rotations per inner round
   static constexpr int IR = WC - 1;          // rounds of one inner sweep
   static constexpr int NT = WC / 16;         // 16x16 MFMA tiles per Gram row
   static constexpr int TPW = NT * NT / NW;   // Gram tiles per wave
-  static constexpr int GBLK = WB / 16;       // 2x2 blocks of G per thread in the inner sweep
   static constexpr int VROWS = WB / 8;       // rows of V per thread
   static constexpr int PER = RCH * WC / JT;  // staged chunk elements per thread
   static constexpr int MINB = WB == 16 ? 4 : 2;  // resident visits per CU the registers allow
@@ -105,11 +104,10 @@ __device__ __forceinline__ double lane_value(double v, int l) {
 // the position it takes in the next round, and after 63 rounds every player is
 // back at its starting position.  Working in these positions keeps every
 // thread's addresses fixed:
-//   * G (full, symmetric) ping-pongs between two LDS buffers: thread (wave w,
-//     lane l) rotates the 2x2 blocks (P, Q_j), P = l & 31, Q_j = 2w + (l >> 5) +
-//     16j, reading them at rows {P, 63-P} x cols {Q, 63-Q} of the current buffer
-//     (32 lanes of a half-wave read 32 different rows of one column: no bank
-//     conflicts with the odd row pitch) and writing them moved into the other;
+//   * G (symmetric, upper triangle only) ping-pongs between two LDS buffers:
+//     every 2x2 block of G (rows of one pair, columns of another) is owned by
+//     one thread, read at its positions in the current buffer and written
+//     moved into the other, at addresses fixed for the whole sweep;
 //   * V lives in registers: the same thread holds V at rows 2w + (l >> 5) + 16j
 //     (j = 0..3) and positions P, 63 - P; the move is a shift by one lane
 //     within each 32-lane half (DPP, plus the two ends of the round-robin
@@ -130,15 +128,40 @@ __device__ __forceinline__ void inner_sweep(double (*GA)[Jac<WB_>::GLD], double 
   __shared__ double2 pcs[NPAIR];
   auto adv = [](int x) { return x == 0 ? 0 : (x == IR ? 1 : x + 1); };
   const int x0 = P, x1 = IR - P;
-  const int ax0 = adv(x0), ax1 = adv(x1);
-  int Q[C::GBLK], y0[C::GBLK], y1[C::GBLK], ay0[C::GBLK], ay1[C::GBLK];
+  // G blocks of this thread in the upper-triangle storage: block (bP, bQ) holds the
+  // rows of pair bP and the columns of pair bQ, entry (i, j) at [min][max].  Group g
+  // of NPAIR lanes takes the blocks {P, P + g + 1 mod NPAIR} (g < NPAIR/2 - 1); the
+  // last group the blocks {P, P + NPAIR/2} (P < NPAIR/2) and two diagonal blocks per
+  // lane (P >= NPAIR/2); every unordered pair of pairs once.  With a row pitch of 2
+  // mod 32 doubles the 32 lanes of a group touch 32 different bank pairs (up to the
+  // wrap of P + g + 1) for each of the four entries.
+  constexpr int HALF = NPAIR / 2;
+  const int g = tid / NPAIR;
+  int nbk = 0, bP[2] = {0, 0}, bQ[2] = {0, 0};
+  if (g < HALF - 1) {
+    nbk = 1;
+    bP[0] = P;
+    bQ[0] = (P + g + 1) & (NPAIR - 1);
+  } else if (g == HALF - 1) {
+    if (P < HALF) {
+      nbk = 1;
+      bP[0] = P;
+      bQ[0] = P + HALF;
+    } else {
+      nbk = 2;
+      bP[0] = bQ[0] = 2 * (P - HALF);
+      bP[1] = bQ[1] = 2 * (P - HALF) + 1;
+    }
+  }
+  auto canon = [](int i, int j) { return i < j ? i * GLD + j : j * GLD + i; };
+  int rd[2][4], wr[2][4];
 #pragma unroll
-  for (int j = 0; j < C::GBLK; ++j) {
-    Q[j] = slot + 16 * j;
-    y0[j] = Q[j];
-    y1[j] = IR - Q[j];
-    ay0[j] = adv(y0[j]);
-    ay1[j] = adv(y1[j]);
+  for (int b = 0; b < 2; ++b) {
+    const int r0 = bP[b], r1 = IR - bP[b], c0 = bQ[b], c1 = IR - bQ[b];
+    rd[b][0] = canon(r0, c0); rd[b][1] = canon(r0, c1);
+    rd[b][2] = canon(r1, c0); rd[b][3] = canon(r1, c1);
+    wr[b][0] = canon(adv(r0), adv(c0)); wr[b][1] = canon(adv(r0), adv(c1));
+    wr[b][2] = canon(adv(r1), adv(c0)); wr[b][3] = canon(adv(r1), adv(c1));
   }
   // V rows of this thread, identity at the start
   double VL[C::VROWS], VR[C::VROWS];
@@ -184,17 +207,20 @@ __device__ __forceinline__ void inner_sweep(double (*GA)[Jac<WB_>::GLD], double 
     __syncthreads();
     // ---- phase 2: G blocks (current -> next buffer, moved) and V rows
     const double2 pr = pcs[P];
+    const double* cb = &cur[0][0];
+    double* nb = &nxt[0][0];
 #pragma unroll
-    for (int j = 0; j < C::GBLK; ++j) {
-      const double2 pq = pcs[Q[j]];
-      const double g00 = cur[x0][y0[j]], g01 = cur[x0][y1[j]];
-      const double g10 = cur[x1][y0[j]], g11 = cur[x1][y1[j]];
-      const double t00 = pr.x * g00 - pr.y * g10, t01 = pr.x * g01 - pr.y * g11;
-      const double t10 = pr.y * g00 + pr.x * g10, t11 = pr.y * g01 + pr.x * g11;
-      nxt[ax0][ay0[j]] = t00 * pq.x - t01 * pq.y;
-      nxt[ax0][ay1[j]] = t00 * pq.y + t01 * pq.x;
-      nxt[ax1][ay0[j]] = t10 * pq.x - t11 * pq.y;
-      nxt[ax1][ay1[j]] = t10 * pq.y + t11 * pq.x;
+    for (int b = 0; b < 2; ++b) {
+      if (b < nbk) {
+        const double2 pa = pcs[bP[b]], pq = pcs[bQ[b]];
+        const double g00 = cb[rd[b][0]], g01 = cb[rd[b][1]], g10 = cb[rd[b][2]], g11 = cb[rd[b][3]];
+        const double t00 = pa.x * g00 - pa.y * g10, t01 = pa.x * g01 - pa.y * g11;
+        const double t10 = pa.y * g00 + pa.x * g10, t11 = pa.y * g01 + pa.x * g11;
+        nb[wr[b][0]] = t00 * pq.x - t01 * pq.y;
+        nb[wr[b][1]] = t00 * pq.y + t01 * pq.x;
+        if (bP[b] != bQ[b]) nb[wr[b][2]] = t10 * pq.x - t11 * pq.y;  // a diagonal block's (h, l) is its (l, h)
+        nb[wr[b][3]] = t10 * pq.y + t11 * pq.x;
+      }
     }
 #pragma unroll
     for (int j = 0; j < C::VROWS; ++j) {
