@@ -339,7 +339,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     double* TO;  // blocks: nblock * NBO * NBO
     double* G;   // NBO x NBO
     double* W;   // NBO x n
-    double* VT;  // m x NBO
+    double* W2;  // NBO x n: T W (or T^T W)
     double* VX;  // m x NBO: the block's V with explicit unit diagonal and zeros above
     int kq, nblock;
   };
@@ -362,7 +362,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     a.TO = ctx.ws.alloc_n<double>(size_t(std::max(1, a.nblock)) * NBO * NBO);
     a.G = ctx.ws.alloc_n<double>(size_t(NBO) * NBO);
     a.W = ctx.ws.alloc_n<double>(size_t(NBO) * std::max(1, J.n));
-    a.VT = ctx.ws.alloc_n<double>(size_t(std::max(1, J.m)) * NBO);
+    a.W2 = ctx.ws.alloc_n<double>(size_t(NBO) * std::max(1, J.n));
     a.VX = ctx.ws.alloc_n<double>(size_t(std::max(1, J.m)) * NBO);
   }
   GemmBatch gb;
@@ -404,7 +404,8 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
         ProfScope ps(ctx, "qr_panel", pfl, pby);
         launch_panel(ctx, d, int(pj.size()), max_rows, reg);
       }
-      // apply the panel to the rest of its block: A2 -= (V T^T) (V^T A2)
+      // apply the panel to the rest of its block: A2 -= V (T^T (V^T A2)); the
+      // triangular factor multiplies the short side (nb x n2), not V (m2 x nb)
       for (int stage = 0; stage < 3; ++stage) {
         for (size_t q = 0; q < jobs.size(); ++q) {
           const QRJob& J = jobs[q];
@@ -420,10 +421,10 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
           const double* T = a.T + size_t(c / NB) * NBP * NBP;
           if (stage == 0)  // W = V^T A2  (nb x n2)
             gb.add(nb, n2, m2, 1.0, V, J.lda, 1, A2, 1, J.lda, 0.0, a.W, 1, NBO, AM_VT_UNIT);
-          else if (stage == 1)  // VT = V T^T  (m2 x nb)
-            gb.add(m2, nb, nb, 1.0, V, 1, J.lda, T, NBP, 1, 0.0, a.VT, 1, m2, AM_V_UNIT);
-          else  // A2 -= VT W
-            gb.add(m2, n2, nb, -1.0, a.VT, 1, m2, a.W, 1, NBO, 1.0, A2, 1, J.lda);
+          else if (stage == 1)  // W2 = T^T W  (nb x n2)
+            gb.add(nb, n2, nb, 1.0, T, NBP, 1, a.W, 1, NBO, 0.0, a.W2, 1, NBO);
+          else  // A2 -= V W2
+            gb.add(m2, n2, nb, -1.0, V, 1, J.lda, a.W2, 1, NBO, 1.0, A2, 1, J.lda, AM_V_UNIT);
         }
         gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_panel_upd");
       }
@@ -445,7 +446,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       hipLaunchKernelGGL(build_t_kernel, dim3(unsigned(tb.size())), dim3(NBO), 0, ctx.stream, d);
       TTR_HIP(hipGetLastError());
     }
-    // trailing update A2 -= (V T^T) (V^T A2)
+    // trailing update A2 -= V (T^T (V^T A2))
     for (int stage = 0; stage < 3; ++stage) {
       for (size_t q = 0; q < jobs.size(); ++q) {
         const QRJob& J = jobs[q];
@@ -458,15 +459,15 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
         const double* T = a.TO + size_t(b) * NBO * NBO;
         if (stage == 0)  // W = V^T A2  (nbo x n2)
           gb.add(nbo, n2, m2, 1.0, a.VX, m2, 1, A2, 1, J.lda, 0.0, a.W, 1, NBO);
-        else if (stage == 1)  // VT = V T^T  (m2 x nbo)
-          gb.add(m2, nbo, nbo, 1.0, a.VX, 1, m2, T, NBO, 1, 0.0, a.VT, 1, m2);
-        else  // A2 -= VT W
-          gb.add(m2, n2, nbo, -1.0, a.VT, 1, m2, a.W, 1, NBO, 1.0, A2, 1, J.lda);
+        else if (stage == 1)  // W2 = T^T W  (nbo x n2)
+          gb.add(nbo, n2, nbo, 1.0, T, NBO, 1, a.W, 1, NBO, 0.0, a.W2, 1, NBO);
+        else  // A2 -= V W2
+          gb.add(m2, n2, nbo, -1.0, a.VX, 1, m2, a.W2, 1, NBO, 1.0, A2, 1, J.lda);
       }
       gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_trail");
     }
   }
-  // ---- thin Q by backward accumulation over the blocks: Q2 -= (V T) (V^T Q2)
+  // ---- thin Q by backward accumulation over the blocks: Q2 -= V (T (V^T Q2))
   std::vector<Fill2D> fills;
   for (size_t q = 0; q < jobs.size(); ++q)
     if (jobs[q].Q && aux[q].kq > 0) fills.push_back(Fill2D{jobs[q].Q, jobs[q].ldq, jobs[q].m, aux[q].kq, 1.0});
@@ -485,10 +486,10 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
         const double* T = a.TO + size_t(b) * NBO * NBO;
         if (stage == 0)  // W = V^T Q2  (nbo x n2)
           gb.add(nbo, n2, m2, 1.0, a.VX, m2, 1, Q2, 1, J.ldq, 0.0, a.W, 1, NBO);
-        else if (stage == 1)  // VT = V T  (m2 x nbo)
-          gb.add(m2, nbo, nbo, 1.0, a.VX, 1, m2, T, 1, NBO, 0.0, a.VT, 1, m2);
-        else  // Q2 -= VT W
-          gb.add(m2, n2, nbo, -1.0, a.VT, 1, m2, a.W, 1, NBO, 1.0, Q2, 1, J.ldq);
+        else if (stage == 1)  // W2 = T W  (nbo x n2)
+          gb.add(nbo, n2, nbo, 1.0, T, 1, NBO, a.W, 1, NBO, 0.0, a.W2, 1, NBO);
+        else  // Q2 -= V W2
+          gb.add(m2, n2, nbo, -1.0, a.VX, 1, m2, a.W2, 1, NBO, 1.0, Q2, 1, J.ldq);
       }
       gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_formq");
     }
