@@ -96,6 +96,20 @@ int ttr_lincomb(ttr_handle_t h, int64_t nterms, const double* alpha, const int64
  * Replaces: *(::TToperator, ::TTvector) (src/tt_operations.jl:170-194). */
 int ttr_mpo_apply(ttr_handle_t h, int64_t op_id, int64_t x_id, int64_t* out_id);
 
+/* out = x_1 .* x_2 .* ... .* x_m (entrywise product of m <= 8 TTs with equal
+ * dims): core k of out holds, per mode index, the Kronecker product of the
+ * factors' core-k slices, ranks prod_f r_f(k), ot all zero.  first_fastest == 0:
+ * the LAST factor's rank index runs fastest (kron(x, y) per slice).
+ * Replaces: *(::TTvector, ::TTvector) (src/tt_operations.jl:273-290).
+ * first_fastest == 1: the FIRST factor's index runs fastest, the layout of the
+ * implicit product in the NTuple methods ttrand_rounding
+ * (src/tt_randtools.jl:364-466) and tt_recursive_sketch / tt_sketch /
+ * tt_combined_sketch (src/sketches.jl:474-604, 819-909, 1003-1040): rounding or
+ * sketching the product built this way gives their results (their sketch
+ * tensors W_k are the rows of the product's W_k in this order).  Every product
+ * core must stay below 2^31 entries. */
+int ttr_hadamard(ttr_handle_t h, int64_t m, const int64_t* ids, int32_t first_fastest, int64_t* out_id);
+
 /* ------------------------------------------------- deterministic rounding */
 
 /* Orthogonalizes n TTs around root (1-based, as the reference) with

@@ -37,7 +37,8 @@ __all__ = [
     "stta_sketch_tt", "stta_solve", "stta", "ttrand_rounding", "ttrand_rounding_lincomb",
     "ttrand_rounding_op", "ttrand_rounding_residual", "rel_err", "tt_dot", "dot",
     "dot_par", "partial_contraction", "dot_operator",
-    "hubbard_1d_mpo", "hubbard_1d_dense",
+    "hubbard_1d_mpo", "hubbard_1d_dense", "hadamard", "contract_sketch_core_kronecker_backwards",
+    "tt_recursive_sketch_tuple", "tt_sketch_tuple", "ttrand_rounding_tuple",
 ]
 
 
@@ -883,6 +884,118 @@ def stta(alpha, ys: Sequence[TT], rks, draws_left, draws_right, orthogonal: bool
             Om = [o + a * o2 for o, o2 in zip(Om, O_)]
             Ps = [q + a * q2 for q, q2 in zip(Ps, P_)]
     return stta_solve(Om, Ps, ys[0].dims)
+
+
+# ------------------------------------------------------- entrywise (Hadamard) products
+
+def hadamard(xs: Sequence[TT], first_fastest: bool = False) -> TT:
+    """Entrywise product of TTs: slice ``i`` of core ``k`` is the Kronecker product
+    of the factors' slices.  ``first_fastest=False``: ``x * y`` of
+    ``src/tt_operations.jl:273-290`` (``kron(x[i,:,:], y[i,:,:])``, so the last
+    factor's rank index runs fastest), ot all zero; ``True``: the first factor's
+    index fastest, the order of the NTuple methods (``src/tt_randtools.jl:378-382``,
+    ``src/sketches.jl:574-575``)."""
+    xs = list(xs)
+    dims, N = xs[0].dims, xs[0].N
+    assert all(t.dims == dims for t in xs)
+    order = xs if not first_fastest else xs[::-1]  # np.kron: the later operand fastest
+    cores = []
+    for k in range(N):
+        c = order[0].cores[k]
+        for t in order[1:]:
+            c = np.stack([np.kron(c[i], t.cores[k][i]) for i in range(dims[k])])
+        cores.append(c)
+    rks = [int(np.prod([t.rks[k] for t in xs])) for k in range(N + 1)]
+    return TT(cores, dims, rks, [0] * N)
+
+
+def contract_sketch_core_kronecker_backwards(As: Sequence[np.ndarray], S: np.ndarray, V: np.ndarray) -> np.ndarray:
+    """``W[a_1..a_M, b] = sum_{z, al, be} prod_i A_i[z, a_i, al_i] S[be, z, b]
+    V[al_1..al_M, be]`` (``src/sketches.jl:1735-1881``), factor by factor per mode
+    index, the product core never formed."""
+    M = len(As)
+    out = None
+    for z in range(As[0].shape[0]):
+        T = V
+        for i in range(M):
+            T = np.moveaxis(np.tensordot(As[i][z], T, axes=([1], [i])), 0, i)
+        T = np.tensordot(T, S[:, z, :], axes=([M], [0]))
+        out = T if out is None else out + T
+    return out
+
+
+def tt_recursive_sketch_tuple(ys: Sequence[TT], rks, draws, orthogonal: bool = True,
+                              block_rks: Optional[int] = None, p=None, oversampling=1):
+    """Reverse sketch of the entrywise product of ``ys`` without forming it
+    (``src/sketches.jl:474-535,600-604``).  ``W[k]`` has shape
+    ``(r_1k, ..., r_Mk, bv_k p_k)``."""
+    ys = list(ys)
+    M, N, dims = len(ys), ys[0].N, ys[0].dims
+    assert rks[N] == 1 and all(t.rks[N] == 1 for t in ys)
+    bv, p = sketch_structure(dims, rks, block_rks, p, oversampling)
+    W: List[Optional[np.ndarray]] = [None] * (N + 1)
+    W[N] = np.ones((1,) * (M + 2))
+    for k in range(N - 1, -1, -1):
+        S = _sketch_for(draws, k, bv[k + 1], dims[k], bv[k], p[k], orthogonal)
+        Wk = np.zeros(tuple(t.rks[k] for t in ys) + (bv[k], p[k]))
+        for j in range(p[k]):
+            Vn = W[k + 1][..., j if k < N - 1 else 0]
+            Wk[..., j] = contract_sketch_core_kronecker_backwards([t.cores[k] for t in ys], S[:, :, :, j], Vn)
+        W[k] = Wk
+    out = []
+    for k in range(N + 1):
+        out.append(jreshape(W[k] / math.sqrt(p[k]), tuple(t.rks[k] for t in ys) + (bv[k] * p[k],)))
+    return out, [bv[k] * p[k] for k in range(N + 1)]
+
+
+def tt_sketch_tuple(ys: Sequence[TT], s: int, draws, orthogonal: bool = True, block_rks: Optional[int] = None):
+    """Boundary sketch of the entrywise product (``src/sketches.jl:819-900``):
+    the recursive sketch with ``p = cld(s, bv_1)`` blocks at every core."""
+    ys = list(ys)
+    N, dims = ys[0].N, ys[0].dims
+    bv, _ = sketch_structure(dims, [1] * (N + 1), block_rks)
+    pc = -(-int(s) // bv[0])
+    W, srk = tt_recursive_sketch_tuple(ys, [1] * (N + 1), draws, orthogonal, block_rks, p=[pc] * N + [1])
+    return np.reshape(W[0], -1, order="F"), bv[0] * pc
+
+
+def ttrand_rounding_tuple(ys: Sequence[TT], rks, draws, orthogonal: bool = True,
+                          block_rks: Optional[int] = None) -> TT:
+    """Randomized rounding of the entrywise product of ``ys``, never formed
+    (``src/tt_randtools.jl:364-458``): the left part ``y_k`` carries the factors'
+    rank indices (first fastest) and is advanced one factor at a time."""
+    ys = list(ys)
+    M, N, dims = len(ys), ys[0].N, ys[0].dims
+    W, srk = tt_recursive_sketch_tuple(ys, rks, draws, orthogonal, block_rks)
+    vec: List[Optional[np.ndarray]] = [None] * N
+    out_rks = [1] * (N + 1)
+    ot = [0] * N
+    # y_1[i, 1, al_1..al_M] = prod_f y_f[i, 1, al_f]
+    Y = np.ones((dims[0], 1) + (1,) * M)
+    for f, t in enumerate(ys):
+        shp = [dims[0], 1] + [1] * M
+        shp[2 + f] = t.rks[1]
+        Y = Y * t.cores[0].reshape(shp)
+    for k in range(N - 1):
+        R = [t.rks[k + 1] for t in ys]
+        Yf = jreshape(Y, (dims[k] * out_rks[k], int(np.prod(R))))
+        Z = Yf @ jreshape(W[k + 1], (int(np.prod(R)), srk[k + 1]))
+        Q = _qr_thin(Z)
+        out_rks[k + 1] = Q.shape[1]
+        vec[k] = jreshape(Q, (dims[k], out_rks[k], out_rks[k + 1]))
+        ot[k] = 1
+        # Qy[al_1..al_M, rho] = sum_{i, r} y_k[i, r, al] Q[i, r, rho], then each factor's
+        # next core applied to its own index
+        T = jreshape(Q.T @ Yf, (out_rks[k + 1],) + tuple(R))  # (rho, al_1..al_M)
+        n1 = dims[k + 1]
+        T = np.broadcast_to(T[None], (n1,) + T.shape).copy()  # (i, rho, al_1..al_M)
+        for f, t in enumerate(ys):
+            # contract al_f (axis 2 + f) with core[i, al_f, be_f] for every i
+            T = np.einsum(T, list(range(M + 2)), t.cores[k + 1], [0, 2 + f, M + 2],
+                          [x if x != 2 + f else M + 2 for x in range(M + 2)], optimize=True)
+        Y = T
+    vec[N - 1] = jreshape(Y, (dims[N - 1], out_rks[N - 1], 1))
+    return TT(vec, dims, out_rks, ot)
 
 
 def _qr_thin(Z: np.ndarray) -> np.ndarray:

@@ -1755,6 +1755,48 @@ int ttr_mpo_apply(ttr_handle_t h, int64_t op_id, int64_t x_id, int64_t* out_id) 
   });
 }
 
+int ttr_hadamard(ttr_handle_t h, int64_t m, const int64_t* ids, int32_t first_fastest, int64_t* out_id) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(m >= 1 && m <= KRON_MAX && ids && out_id, ST_INVALID_ARG, "bad arguments (1 <= m <= 8)");
+    std::vector<DevTT*> xs;
+    for (int64_t f = 0; f < m; ++f) {
+      xs.push_back(h->get(ids[f]));
+      check_dims_equal(xs[0], xs.back());
+    }
+    // factor order of the kernel: fastest first (src/tt_operations.jl:273-290 takes
+    // kron(x, y) per slice, so there the last factor runs fastest)
+    if (!first_fastest) std::reverse(xs.begin(), xs.end());
+    const int d = xs[0]->d;
+    std::vector<int64_t> rks(static_cast<size_t>(d + 1), 1);
+    for (int k = 0; k <= d; ++k)
+      for (auto* x : xs) {
+        rks[k] *= x->rks[k];
+        TTR_REQUIRE(rks[k] < (int64_t(1) << 31), ST_INVALID_ARG, "product rank too large");
+      }
+    for (int k = 0; k < d; ++k)
+      TTR_REQUIRE(rks[k] * xs[0]->dims[k] * rks[k + 1] < (int64_t(1) << 31), ST_INVALID_ARG,
+                  "product core exceeds 2^31 entries");
+    auto y = h->make_tt(xs[0]->dims, rks, std::vector<int64_t>(size_t(d), 0));
+    std::vector<KronCore> jobs;
+    for (int k = 0; k < d; ++k) {
+      KronCore J{};
+      J.out = y->cores[k];
+      J.m = int32_t(m);
+      J.n = int32_t(xs[0]->dims[k]);
+      J.RA = int32_t(rks[k]);
+      J.RB = int32_t(rks[k + 1]);
+      for (int64_t f = 0; f < m; ++f) {
+        J.in[f] = xs[size_t(f)]->cores[k];
+        J.ra[f] = int32_t(xs[size_t(f)]->rks[k]);
+        J.rb[f] = int32_t(xs[size_t(f)]->rks[k + 1]);
+      }
+      jobs.push_back(J);
+    }
+    kron_cores_batched(h->ctx, jobs);
+    *out_id = h->add(std::move(y));
+  });
+}
+
 int ttr_orthogonalize(ttr_handle_t h, int64_t n, const int64_t* ids, int64_t root, int64_t* out_ids) {
   return guarded(h, [&] {
     TTR_REQUIRE(n >= 1 && ids && out_ids, ST_INVALID_ARG, "bad arguments");

@@ -213,6 +213,30 @@ __global__ void sum_kernel(double* out, const double* const* __restrict__ ins, i
   }
 }
 
+// one thread per output entry (consecutive threads: consecutive left ranks, so
+// the stores are contiguous); 32-bit index arithmetic (the host bounds every
+// core below 2^31 entries)
+__global__ void kron_core_kernel(const KronCore* __restrict__ jobs) {
+  const KronCore J = jobs[blockIdx.y];
+  const uint32_t RA = uint32_t(J.RA), n = uint32_t(J.n);
+  const uint32_t tot = RA * n * uint32_t(J.RB);
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += gridDim.x * blockDim.x) {
+    uint32_t a = e % RA;
+    const uint32_t rest = e / RA;
+    const uint32_t i = rest % n;
+    uint32_t b = rest / n;
+    double v = 1.0;
+    for (int f = 0; f < J.m; ++f) {
+      const uint32_t ra = uint32_t(J.ra[f]), rb = uint32_t(J.rb[f]);
+      const uint32_t af = a % ra, bf = b % rb;
+      a /= ra;
+      b /= rb;
+      v *= J.in[f][af + ra * (i + n * bf)];
+    }
+    J.out[e] = v;
+  }
+}
+
 __global__ void block_put_kernel(const BlockPut* __restrict__ jobs) {
   const BlockPut J = jobs[blockIdx.y];
   const int64_t tot = int64_t(J.ra) * J.n * J.rb;
@@ -349,6 +373,17 @@ void block_put_batched(Ctx& ctx, const std::vector<BlockPut>& jobs) {
   int64_t mx = 0;
   for (auto& j : jobs) mx = std::max<int64_t>(mx, int64_t(j.ra) * j.n * j.rb);
   launch_y(ctx, block_put_kernel, jobs, mx);
+}
+
+void kron_cores_batched(Ctx& ctx, const std::vector<KronCore>& jobs) {
+  int64_t mx = 0, by = 0;
+  for (auto& j : jobs) {
+    const int64_t e = int64_t(j.RA) * j.n * j.RB;
+    mx = std::max<int64_t>(mx, e);
+    by += 8 * e;
+  }
+  ProfScope ps(ctx, "kron_core", 0.0, double(by));
+  launch_y(ctx, kron_core_kernel, jobs, mx);
 }
 
 void mpo_apply_batched(Ctx& ctx, const std::vector<MpoCore>& jobs) {

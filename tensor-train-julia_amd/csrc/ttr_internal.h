@@ -368,6 +368,18 @@ struct MpoCore {
 };
 void mpo_apply_batched(Ctx& ctx, const std::vector<MpoCore>& jobs);
 
+// Entrywise (Hadamard) product of m TTs, one core: out[(a, i, b)] =
+// prod_f in_f[(a_f, i, b_f)] with a = a_0 + ra_0 (a_1 + ra_1 (...)) (factor 0
+// fastest), likewise b; every index below 2^31
+constexpr int KRON_MAX = 8;
+struct KronCore {
+  const double* in[KRON_MAX];  // (ra_f, n, rb_f) device layout
+  double* out;                 // (RA, n, RB) device layout
+  int32_t ra[KRON_MAX], rb[KRON_MAX];
+  int32_t m, n, RA, RB;
+};
+void kron_cores_batched(Ctx& ctx, const std::vector<KronCore>& jobs);
+
 // Local action of an operator core on the (mode, bond) pairs of every fiber
 // (x, y) of a 2-D grid, without forming an MPO x TT product:
 //   out[(om, ob)] = scale * sum_{im, ib} H(om, ob, im, ib) in[(im, ib)],

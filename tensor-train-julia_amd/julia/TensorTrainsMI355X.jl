@@ -7,8 +7,9 @@ GPU methods with the reference's names and keyword defaults
 (src/tt_rounding.jl:146-167, src/tt_randtools.jl:112-283,
 src/sketches.jl:175-315,317-472,645-817,930-1001,1047-1070,1126-1156,
 src/tt_randtools.jl:552-638,
-src/tt_operations.jl:81-112,170-194,252-258,
-302-421); the
+src/tt_operations.jl:81-112,170-194,252-258,273-290,
+302-421, and the NTuple methods src/tt_randtools.jl:364-473, src/sketches.jl:474-615,
+819-909,1003-1040); the
 solver callers (tt_gmres, arnoldi, davidson, als/mals/dmrg via orthogonalize)
 need no change.  Every call uploads the cores (column-major (n, r, r') arrays,
 exactly TTvector.ttv_vec), runs on the GPU and copies the result back into new
@@ -25,6 +26,7 @@ using TensorTrains
 import TensorTrains: tt_rounding, tt_rounding!, orthogonalize, ttrand_rounding, tt_recursive_sketch,
                      tt_sketch, tt_combined_sketch, stta, stta_sketch, dot_par, dot_operator, partial_contraction
 import LinearAlgebra: dot
+import Base: *
 
 const lib = get(ENV, "TTROUND_LIB", joinpath(@__DIR__, "..", "ttround", "libttround.so"))
 
@@ -76,6 +78,32 @@ function download(id::Int64, ::Val{N}) where {N}
 end
 
 free(id) = check(ccall((:ttr_tt_free, lib), Cint, (Ptr{Cvoid}, Int64), handle(), id))
+
+# entrywise product of TTs on the device (ttr_hadamard); first_fastest: the first
+# factor's rank index runs fastest (the NTuple methods), else the last (x * y)
+function _hadamard(y, first_fastest::Bool)
+    ids = Int64[upload(t) for t in y]
+    out = Ref{Int64}(0)
+    check(ccall((:ttr_hadamard, lib), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Int32, Ref{Int64}),
+                handle(), length(ids), ids, Int32(first_fastest ? 1 : 0), out))
+    foreach(free, ids)
+    return out[]
+end
+
+# x * y (src/tt_operations.jl:273-290): slices kron(x_k[i,:,:], y_k[i,:,:])
+*(x::TTvector{Float64,N}, y::TTvector{Float64,N}) where {N} = download(_hadamard((x, y), false), Val(N))
+
+# an NTuple argument of the sketches: the product, built on the device at upload
+struct _Product{N}
+    factors::Vector{TTvector{Float64,N}}
+    ttv_dims::NTuple{N,Int}
+    ttv_rks::Vector{Int}
+end
+_Product(y::NTuple{M,TTvector{Float64,N}}) where {M,N} =
+    _Product{N}(collect(y), y[1].ttv_dims, [prod(t.ttv_rks[k] for t in y) for k in 1:N+1])
+upload(p::_Product) = _hadamard(p.factors, true)
+_bond_shape(A::TTvector, k) = (A.ttv_rks[k],)
+_bond_shape(A::_Product, k) = Tuple(t.ttv_rks[k] for t in A.factors)
 
 # ---------------------------------------------------------------- deterministic
 
@@ -221,7 +249,8 @@ function _block_ranks(dims, block_rks::Int, N::Int)
 end
 
 # tt_recursive_sketch(T, [H,] A, rks; reverse=true, ...) (src/sketches.jl:175-240,300-303,317-459)
-function _recursive_sketch(::Type{T}, H, A::TTvector{Float64,N}, rks::Vector{Int}; orthogonal=true, reverse=true,
+function _recursive_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}}, rks::Vector{Int};
+                           orthogonal=true, reverse=true,
                            seed=1234, block_rks::Int=N, p=0, oversampling=1) where {T,N}
     reverse || H === nothing || throw(ArgumentError("the GPU forward sketch takes no operator"))
     bv = reverse ? _block_ranks(A.ttv_dims, block_rks, N) : _block_ranks_fwd(A.ttv_dims, block_rks, N)
@@ -252,7 +281,7 @@ function _recursive_sketch(::Type{T}, H, A::TTvector{Float64,N}, rks::Vector{Int
     end
     free(id)
     op >= 0 && free(op)
-    H === nothing && return W, collect(Int, srk)
+    H === nothing && return [reshape(W[k], _bond_shape(A, k)..., s[k]) for k in 1:N+1], collect(Int, srk)
     return [reshape(W[k], A.ttv_rks[k], Hr[k], s[k]) for k in 1:N+1], collect(Int, srk)
 end
 
@@ -285,7 +314,8 @@ tt_recursive_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rma
     tt_recursive_sketch(Float64, H, A, rks_or_rmax; kwargs...)
 
 # tt_sketch(T, [H,] A, s; reverse=true, ...) (src/sketches.jl:645-817): boundary sketch only
-function _tt_sketch(::Type{T}, H, A::TTvector{Float64,N}, s::Int; reverse=true, orthogonal=true, seed=1234,
+function _tt_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}}, s::Int; reverse=true, orthogonal=true,
+                    seed=1234,
                     block_rks::Int=N) where {T,N}
     reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
     id = upload(A)
@@ -314,7 +344,8 @@ tt_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, s::Int; kwargs...) w
     tt_sketch(Float64, H, A, s; kwargs...)
 
 # tt_combined_sketch(T, [H,] A, rks_or_rmax, s; reverse=true, ...) (src/sketches.jl:930-1001)
-function _combined_sketch(::Type{T}, H, A::TTvector{Float64,N}, rks::Vector{Int}, s::Int; reverse=true,
+function _combined_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}}, rks::Vector{Int}, s::Int;
+                          reverse=true,
                           orthogonal=true, seed=1234, block_rks::Int=N, timer=nothing) where {T,N}
     reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
     rks = copy(rks)
@@ -327,10 +358,46 @@ function _combined_sketch(::Type{T}, H, A::TTvector{Float64,N}, rks::Vector{Int}
         s = s_rec + length(extra)
         sketch = [sqrt(length(sketch) / s) .* sketch; sqrt(length(extra) / s) .* extra]
     end
-    W[1] = H === nothing ? ones(Float64, 1, 1) : ones(Float64, 1, 1, 1)
+    W[1] = H === nothing ? ones(Float64, ntuple(_ -> 1, length(_bond_shape(A, 1)) + 1)...) : ones(Float64, 1, 1, 1)
     sketch_rks[1] = 1
     return sketch, W, sketch_rks
 end
+
+# NTuple forms (src/tt_randtools.jl:364-473, src/sketches.jl:474-615,819-909,1003-1040): the
+# entrywise product of the tuple, formed on the device with the first factor's rank index
+# fastest; target ranks are required, as in the reference
+function ttrand_rounding(y::NTuple{M,TTvector{Float64,N}}, rks::Vector{Int}; orthogonal=true, seed=1234,
+                         block_rks::Int=N, sketch_type::Type=Float64, timer=nothing) where {M,N}
+    M == 1 && return ttrand_rounding(y[1], rks; orthogonal=orthogonal, seed=seed, block_rks=block_rks)
+    id = upload(_Product(y))
+    α, ids, r = [1.0], Int64[id], collect(Int64, rks)
+    out = Ref{Int64}(0)
+    GC.@preserve α ids r begin
+        item = Ref(RandItem(1, pointer(α), pointer(ids), -1, pointer(r)))
+        opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[sketch_type], block_rks, UInt64(seed), C_NULL, 1.0, Int32(0)))
+        check(ccall((:ttr_round_rand, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, Ref{Int64}),
+                    handle(), 1, item, opts, out))
+    end
+    free(id)
+    return download(out[], Val(N))
+end
+ttrand_rounding(y::NTuple{M,TTvector{Float64,N}}, rmax::Int; kwargs...) where {M,N} =
+    ttrand_rounding(y, vcat(1, fill(rmax, N - 1), 1); kwargs...)
+
+tt_recursive_sketch(::Type{T}, A::NTuple{M,TTvector{Float64,N}}, rks_or_rmax; reverse=true,
+                    kwargs...) where {T<:Union{Float64,Float32},M,N} =
+    _recursive_sketch(T, nothing, _Product(A), reverse ? _rks_of(rks_or_rmax, N) : _rks_of_fwd(rks_or_rmax, N);
+                      reverse=reverse, kwargs...)
+tt_recursive_sketch(A::NTuple{M,TTvector{Float64,N}}, rks_or_rmax; kwargs...) where {M,N} =
+    tt_recursive_sketch(Float64, A, rks_or_rmax; kwargs...)
+tt_sketch(::Type{T}, A::NTuple{M,TTvector{Float64,N}}, s::Int; kwargs...) where {T<:Union{Float64,Float32},M,N} =
+    _tt_sketch(T, nothing, _Product(A), s; kwargs...)
+tt_sketch(A::NTuple{M,TTvector{Float64,N}}, s::Int; kwargs...) where {M,N} = tt_sketch(Float64, A, s; kwargs...)
+tt_combined_sketch(::Type{T}, A::NTuple{M,TTvector{Float64,N}}, rks_or_rmax, s::Int;
+                   kwargs...) where {T<:Union{Float64,Float32},M,N} =
+    _combined_sketch(T, nothing, _Product(A), _rks_of(rks_or_rmax, N), s; kwargs...)
+tt_combined_sketch(A::NTuple{M,TTvector{Float64,N}}, rks_or_rmax, s::Int; kwargs...) where {M,N} =
+    tt_combined_sketch(Float64, A, rks_or_rmax, s; kwargs...)
 
 tt_combined_sketch(::Type{T}, A::TTvector{Float64,N}, rks_or_rmax, s::Int;
                    kwargs...) where {T<:Union{Float64,Float32},N} =

@@ -69,6 +69,7 @@ SIGNATURES = {
     "ttr_tto_upload": (C.c_int, [C.c_void_p, i64, i64p, i64p, f64pp, i64p]),
     "ttr_lincomb": (C.c_int, [C.c_void_p, i64, f64p, i64p, i64p]),
     "ttr_mpo_apply": (C.c_int, [C.c_void_p, i64, i64, i64p]),
+    "ttr_hadamard": (C.c_int, [C.c_void_p, i64, i64p, C.c_int32, i64p]),
     "ttr_orthogonalize": (C.c_int, [C.c_void_p, i64, i64p, i64, i64p]),
     "ttr_round_svd": (C.c_int, [C.c_void_p, i64, i64p, C.c_double, i64, C.c_int, i64p]),
     "ttr_norm": (C.c_int, [C.c_void_p, i64, i64p, f64p]),

@@ -170,6 +170,55 @@ def mpo_apply_device(A: DeviceTTO, x: DeviceTT) -> DeviceTT:
     return DeviceTT(h, out.value)
 
 
+def hadamard_device(xs: Sequence[DeviceTT], first_fastest: bool = False) -> DeviceTT:
+    """Entrywise product of device TTs (``ttr_hadamard``): per mode index the
+    Kronecker product of the core slices.  ``first_fastest=False`` is the
+    reference's ``x * y`` (``src/tt_operations.jl:273-290``, the last factor's
+    rank index fastest); ``True`` is the layout of the NTuple methods."""
+    h = xs[0].handle
+    ids = L.as_i64([x.id for x in xs])
+    out = C.c_int64()
+    h.check(h.lib.ttr_hadamard(h.h, len(xs), L.ptr_i64(ids), 1 if first_fastest else 0, C.byref(out)))
+    return DeviceTT(h, out.value)
+
+
+def _hadamard_dev(ys) -> DeviceTT:
+    """The implicit product of an NTuple argument, built on the device with the
+    first factor's rank index fastest (the reference's order for these methods)."""
+    ys = list(ys)
+    if not ys or any(not isinstance(t, TTvector) for t in ys):
+        raise TypeError("a tuple argument must hold TTvectors")
+    if any(t.ttv_dims != ys[0].ttv_dims for t in ys):
+        raise L.DimensionMismatch("all TTs of the product must share dims")
+    devs = [t._dev() for t in ys]
+    if len(devs) == 1:
+        return devs[0]
+    try:
+        return hadamard_device(devs, first_fastest=True)
+    finally:
+        for d in devs:
+            d.free()
+
+
+class _Product:
+    """An NTuple argument of the sketch functions: the entrywise product of the
+    factors, built on the device at each use (``_dev``) and never downloaded."""
+
+    def __init__(self, ys):
+        self.factors = tuple(ys)
+        self.N = self.factors[0].N
+        self.ttv_dims = self.factors[0].ttv_dims
+        self.factor_rks = [[t.ttv_rks[k] for t in self.factors] for k in range(self.N + 1)]
+        self.ttv_rks = [int(np.prod(r)) for r in self.factor_rks]
+
+    def _dev(self) -> DeviceTT:
+        return _hadamard_dev(self.factors)
+
+    def split(self, W, k):
+        """Sketch rows of bond ``k`` as the reference's ``(r_1, ..., r_M, s)`` array."""
+        return W.reshape(tuple(self.factor_rks[k]) + (W.shape[-1],), order="F")
+
+
 def orthogonalize_batch(xs: Sequence[DeviceTT], i: int = 1) -> List[DeviceTT]:
     h = xs[0].handle
     ids = L.as_i64([x.id for x in xs])
@@ -373,6 +422,20 @@ class TTvector:
     def __rmul__(self, a) -> "TTvector":
         return tt_scale(a, self)
 
+    def __mul__(self, other):
+        """``x * y`` (``src/tt_operations.jl:273-290``): the entrywise product,
+        core slices ``kron(x_k[i], y_k[i])``; ``x * a`` for a number scales."""
+        if not isinstance(other, TTvector):
+            return tt_scale(other, self)
+        if other.ttv_dims != self.ttv_dims:
+            raise L.DimensionMismatch("Incompatible dimensions")
+        xs = [self._dev(), other._dev()]
+        out = hadamard_device(xs)
+        res = out.to_host()
+        for d in xs + [out]:
+            d.free()
+        return res
+
     def __truediv__(self, a) -> "TTvector":
         return tt_scale(1.0 / a, self)
 
@@ -395,6 +458,68 @@ class TToperator:
         for d in (dev, xd, op):
             d.free()
         return out
+
+
+# --------------------------------------------------------------------- JSON I/O
+
+def _json_obj(x):
+    if isinstance(x, (str, bytes, bytearray)):
+        import json
+        return json.loads(x)
+    return x
+
+
+def _json_cores(flat, shapes, what):
+    cores = []
+    for k, (c, shp) in enumerate(zip(flat, shapes)):
+        a = np.asarray(c, dtype=np.float64).reshape(-1)
+        if a.size != int(np.prod(shp)):
+            raise L.DimensionMismatch(f"{what} core {k}: {a.size} entries, expected {shp}")
+        cores.append(a.reshape(shp, order="F"))
+    return cores
+
+
+def json_to_mps(x) -> TTvector:
+    """``json_to_mps`` (``src/tt_tools.jl:765-774``): a TTvector from its JSON
+    object (``N``, ``ttv_dims``, ``ttv_rks``, ``ttv_ot`` and ``ttv_vec``, each core
+    a flat column-major list of shape ``(n_k, r_k, r_{k+1})``), as written by
+    ``JSON3.write`` of a TTvector.  ``x`` is the parsed object or its text."""
+    x = _json_obj(x)
+    N = int(x["N"])
+    dims = [int(v) for v in x["ttv_dims"]]
+    rks = [int(v) for v in x["ttv_rks"]]
+    ot = [int(v) for v in x["ttv_ot"]]
+    if len(dims) != N or len(rks) != N + 1 or len(ot) != N or len(x["ttv_vec"]) != N:
+        raise L.DimensionMismatch("inconsistent N, ttv_dims, ttv_rks, ttv_ot, ttv_vec")
+    cores = _json_cores(x["ttv_vec"], [(dims[k], rks[k], rks[k + 1]) for k in range(N)], "ttv_vec")
+    return TTvector(N, cores, dims, rks, ot)
+
+
+def json_to_mpo(x) -> TToperator:
+    """``json_to_mpo`` (``src/tt_tools.jl:776-785``): a TToperator from its JSON
+    object (``tto_*`` fields, cores of shape ``(n_k, n_k, r_k, r_{k+1})``)."""
+    x = _json_obj(x)
+    N = int(x["N"])
+    dims = [int(v) for v in x["tto_dims"]]
+    rks = [int(v) for v in x["tto_rks"]]
+    ot = [int(v) for v in x["tto_ot"]]
+    if len(dims) != N or len(rks) != N + 1 or len(ot) != N or len(x["tto_vec"]) != N:
+        raise L.DimensionMismatch("inconsistent N, tto_dims, tto_rks, tto_ot, tto_vec")
+    cores = _json_cores(x["tto_vec"], [(dims[k], dims[k], rks[k], rks[k + 1]) for k in range(N)], "tto_vec")
+    return TToperator(N, cores, dims, rks, ot)
+
+
+def mps_to_json(x: TTvector) -> dict:
+    """The JSON object ``JSON3.write`` makes of a TTvector (fields in struct
+    order, cores flattened column-major): the inverse of ``json_to_mps``."""
+    return {"N": x.N, "ttv_vec": [np.asarray(c).reshape(-1, order="F").tolist() for c in x.ttv_vec],
+            "ttv_dims": list(x.ttv_dims), "ttv_rks": list(x.ttv_rks), "ttv_ot": list(x.ttv_ot)}
+
+
+def mpo_to_json(A: TToperator) -> dict:
+    """The JSON object of a TToperator: the inverse of ``json_to_mpo``."""
+    return {"N": A.N, "tto_vec": [np.asarray(c).reshape(-1, order="F").tolist() for c in A.tto_vec],
+            "tto_dims": list(A.tto_dims), "tto_rks": list(A.tto_rks), "tto_ot": list(A.tto_ot)}
 
 
 def zeros_tt(dims, rks, ot=None) -> TTvector:
@@ -569,6 +694,23 @@ def ttrand_rounding(*args, orthogonal: bool = True, seed: int = 1234, block_rks:
     ``sketch_prec`` selects the precision of the sketch contractions (the QR and
     projections are FP64)."""
     op = None
+    if isinstance(args[0], tuple):
+        # NTuple form (src/tt_randtools.jl:364-466): the entrywise product of the
+        # factors, never formed by the reference; target ranks are required
+        # (the reference defines no default heuristic for a tuple)
+        if len(args) < 2:
+            raise TypeError("ttrand_rounding(y::Tuple, rks_or_rmax): ranks are required")
+        if len(args[0]) == 1:
+            return ttrand_rounding(args[0][0], args[1], orthogonal=orthogonal, seed=seed, block_rks=block_rks,
+                                   sketch_prec=sketch_prec)
+        dev = _hadamard_dev(args[0])
+        rks = _rks_arg(args[1], dev.d)
+        out = round_rand_batch([([1.0], [dev], rks)], orthogonal=orthogonal, seed=seed, block_rks=block_rks,
+                               sketch_prec=sketch_prec)[0]
+        res = out.to_host()
+        for d in (dev, out):
+            d.free()
+        return res
     if isinstance(args[0], TTvector):
         y = args[0]
         rks = _rks_arg(args[1], y.N) if len(args) > 1 else default_rank_heuristic(y)
@@ -605,10 +747,17 @@ def tt_recursive_sketch(*args, orthogonal: bool = True, reverse: bool = True, se
                         block_rks: Optional[int] = None, p=None, oversampling: float = 1.0,
                         sketch_prec: str = "f64"):
     """``tt_recursive_sketch([H,] A, rks; reverse=true, ...)`` (``src/sketches.jl:175-315``;
-    operator form ``:317-472``, the product ``H A`` never formed).
+    operator form ``:317-472``, the product ``H A`` never formed; NTuple form
+    ``:474-615``, the entrywise product of the tuple's TTs).
 
-    Returns ``(W, sketch_rks)`` with ``W[k]`` of shape ``(r_k, sketch_rks[k])``, or
-    ``(r_k, R_k, sketch_rks[k])`` for the operator form."""
+    Returns ``(W, sketch_rks)`` with ``W[k]`` of shape ``(r_k, sketch_rks[k])``,
+    ``(r_k, R_k, sketch_rks[k])`` for the operator form, or
+    ``(r_1k, ..., r_Mk, sketch_rks[k])`` for a tuple of M TTs."""
+    if isinstance(args[0], tuple):
+        P = _Product(args[0])
+        W, srk = tt_recursive_sketch(P, *args[1:], orthogonal=orthogonal, reverse=reverse, seed=seed,
+                                     block_rks=block_rks, p=p, oversampling=oversampling, sketch_prec=sketch_prec)
+        return [P.split(w, k) for k, w in enumerate(W)], srk
     if isinstance(args[0], TToperator):
         H, A, rks = args
     else:
@@ -658,6 +807,8 @@ def tt_sketch(*args, reverse: bool = True, orthogonal: bool = True, seed: int = 
     Returns ``(sketch, s_actual)``: the boundary sketch of length
     ``s_actual = bv_1 * cld(s, bv_1)``."""
     H, (A, s) = _split_op(args)
+    if isinstance(A, tuple):  # NTuple form (src/sketches.jl:819-909): the entrywise product
+        A = _Product(A)
     if not reverse:
         raise NotImplementedError("only the reverse (right-to-left) sketch is provided")
     if H is not None and tuple(H.tto_dims) != tuple(A.ttv_dims):
@@ -685,7 +836,10 @@ def tt_combined_sketch(*args, reverse: bool = True, orthogonal: bool = True, see
     H, (A, rks, s) = _split_op(args)
     if not reverse:
         raise NotImplementedError("only the reverse (right-to-left) sketch is provided")
-    N = A.N
+    M = len(A) if isinstance(A, tuple) else 0  # NTuple form (src/sketches.jl:1003-1040)
+    if M and H is not None:
+        raise TypeError("no operator form for a tuple of TTs")
+    N = A[0].N if M else A.N
     if isinstance(rks, (int, np.integer)):
         rks = [int(rks)] * (N + 1)
         rks[N] = 1
@@ -700,7 +854,7 @@ def tt_combined_sketch(*args, reverse: bool = True, orthogonal: bool = True, see
         extra, _ = tt_sketch(*pre, A, int(s) - s_rec, seed=seed + 1, **kw)
         s = s_rec + len(extra)
         sketch = np.concatenate([math.sqrt(len(sketch) / s) * sketch, math.sqrt(len(extra) / s) * extra])
-    W[0] = np.ones((1, 1)) if H is None else np.ones((1, 1, 1))
+    W[0] = np.ones((1,) * (M + 1)) if M else (np.ones((1, 1)) if H is None else np.ones((1, 1, 1)))
     srk[0] = 1
     return sketch, W, srk
 

@@ -1,0 +1,121 @@
+"""Entrywise (Hadamard) products on the device and the NTuple methods that
+round or sketch them, against the oracle (FP64: 1e-12 relative, products of
+two factors bit-exact).  The oracle's NTuple sketch and rounding work factor by
+factor, never forming the product; the device builds the product core by core
+(``ttr_hadamard``) and runs the plain TT path on it."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import device_blocks_provider, to_api, to_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def T():
+    import ttround
+    ttround.default_handle()
+    return ttround
+
+
+def rand_tt(dims, rks, seed):
+    return O.rand_tt(dims, rks, np.random.default_rng(seed))
+
+
+def test_hadamard_vs_oracle(T):
+    say("case: x * y (kron order) and a 3-factor product (tuple order)")
+    dims = (3, 4, 2, 3, 2)
+    x = rand_tt(dims, [1, 3, 7, 5, 2, 1], 300)
+    y = rand_tt(dims, [1, 2, 6, 4, 2, 1], 301)
+    z = rand_tt(dims, [1, 3, 3, 3, 2, 1], 302)
+    p = to_oracle(to_api(x) * to_api(y))
+    ref = O.hadamard([x, y], first_fastest=False)
+    assert p.rks == ref.rks and p.ot == [0] * 5
+    for k in range(5):
+        np.testing.assert_array_equal(p.cores[k], ref.cores[k])
+    dev = [T.DeviceTT.upload(to_api(t)) for t in (x, y, z)]
+    d3 = T.hadamard_device(dev, first_fastest=True)
+    ref3 = O.hadamard([x, y, z], first_fastest=True)
+    assert d3.rks == ref3.rks
+    for k in range(5):
+        np.testing.assert_allclose(d3.core(k), ref3.cores[k], rtol=1e-15, atol=0)
+    for d in dev + [d3]:
+        d.free()
+
+
+@pytest.mark.parametrize("orthogonal", [True, False])
+def test_ttrand_rounding_tuple_vs_oracle(T, orthogonal):
+    say(f"case: ttrand_rounding((x, y), rks) orthogonal={orthogonal}")
+    dims = (2, 3, 2, 3, 2, 2)
+    x = rand_tt(dims, [1, 2, 4, 4, 3, 2, 1], 310)
+    y = rand_tt(dims, [1, 2, 3, 4, 4, 2, 1], 311)
+    rks = [1, 3, 8, 10, 8, 3, 1]  # truncating
+    out = to_oracle(T.ttrand_rounding((to_api(x), to_api(y)), rks, orthogonal=orthogonal, seed=17))
+    prov = device_blocks_provider(dims, rks, orthogonal=orthogonal, seed=17)
+    ref = O.ttrand_rounding_tuple([x, y], rks, prov, orthogonal)
+    assert out.rks == ref.rks and out.ot == ref.ot
+    err = O.rel_err(ref, out)
+    say(f"  rel err vs factored oracle: {err:.3e}")
+    assert err < 1e-12
+    # ranks that cover the product: exact recovery of x .* y
+    full = [1] + [a * b for a, b in zip(x.rks[1:-1], y.rks[1:-1])] + [1]
+    ex = to_oracle(T.ttrand_rounding((to_api(x), to_api(y)), full, orthogonal=orthogonal, seed=18))
+    assert O.rel_err(O.hadamard([x, y]), ex) < 1e-11
+    # a one-element tuple is the TT itself
+    one = to_oracle(T.ttrand_rounding((to_api(x),), rks, seed=17, orthogonal=orthogonal))
+    ref1 = to_oracle(T.ttrand_rounding(to_api(x), rks, seed=17, orthogonal=orthogonal))
+    assert one.rks == ref1.rks
+    for a, b in zip(one.cores, ref1.cores):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_sketches_of_tuple_vs_oracle(T):
+    say("case: tt_recursive_sketch / tt_sketch / tt_combined_sketch of (x, y, z)")
+    dims = (2, 3, 2, 2, 3)
+    N = len(dims)
+    xs = [rand_tt(dims, [1, 2, 3, 3, 2, 1], 320 + i) for i in range(3)]
+    api = tuple(to_api(t) for t in xs)
+    rks = [1, 6, 9, 9, 5, 1]
+    W, srk = T.tt_recursive_sketch(api, rks, seed=5)
+    prov = device_blocks_provider(dims, rks, orthogonal=True, seed=5)
+    Wr, sr = O.tt_recursive_sketch_tuple(xs, rks, prov, True)
+    assert srk == sr
+    for k in range(N + 1):
+        assert W[k].shape == Wr[k].shape == tuple(t.rks[k] for t in xs) + (sr[k],)
+        np.testing.assert_allclose(W[k], Wr[k], rtol=1e-12, atol=1e-12 * np.abs(Wr[k]).max())
+    s = 11
+    S, sa = T.tt_sketch(api, s, seed=9)
+    bv, _ = O.sketch_structure(dims, [1] * (N + 1))
+    p = -(-s // bv[0])
+    prov = device_blocks_provider(dims, [1] * (N + 1), orthogonal=True, seed=9, p=[p] * N + [1])
+    Sr, sb = O.tt_sketch_tuple(xs, s, prov, True)
+    assert sa == sb and S.shape == (sa,)
+    np.testing.assert_allclose(S, Sr, rtol=1e-12, atol=1e-12 * np.abs(Sr).max())
+    sk, Wc, sc = T.tt_combined_sketch(api, rks, 30, seed=5)
+    assert len(sk) >= 30 and sc[0] == 1 and Wc[0].shape == (1, 1, 1, 1)
+    for k in range(1, N + 1):
+        assert Wc[k].shape == tuple(t.rks[k] for t in xs) + (sc[k],)
+
+
+def test_hadamard_errors(T):
+    say("case: ttr_hadamard argument errors")
+    x = rand_tt((2, 3, 2), [1, 2, 2, 1], 330)
+    y = rand_tt((2, 2, 2), [1, 2, 2, 1], 331)
+    with pytest.raises(T.DimensionMismatch):
+        to_api(x) * to_api(y)
+    with pytest.raises(T.DimensionMismatch):
+        T.ttrand_rounding((to_api(x), to_api(y)), 4)
+    with pytest.raises(TypeError):
+        T.ttrand_rounding((to_api(x), to_api(x)))
+    dx = T.DeviceTT.upload(to_api(x))
+    with pytest.raises(Exception):
+        T.hadamard_device([dx] * 9)
+    dx.free()

@@ -23,6 +23,9 @@ a wrong index or a transposed operand fails:
                          the sqrt(len/s) weights; E|sketch|^2 = |x|^2 over seeds
   ttrand_rounding*    -> exact recovery when the sketch rank >= true rank (test_ttrand.jl:67-88)
   hubbard_1d_mpo      -> dense fermionic Hamiltonian
+  hadamard            -> dense entrywise product (2 and 3 factors) and the bond order of each
+                         layout; the factored NTuple sketch / rounding equal those of the
+                         explicit product, which the rounding recovers exactly
   dot, dot_par, partial_contraction, dot_operator
                       -> dense inner products, dense left/right interface
                          contractions, psi^T H phi with the dense operator
@@ -607,6 +610,67 @@ def test_ttrand_rounding_operator_exact():
     b = rand_tt(dims, [1, 2, 2, 2, 1], 61)
     res = O.ttrand_rounding_residual(H, y, b, rks, dr, False)
     assert O.rel_err(O.tt_sub(Hy, b), res) < 1e-11
+
+
+# ---------------------------------------------------------------- entrywise products
+
+@pytest.mark.parametrize("first_fastest", [False, True])
+def test_hadamard_dense(first_fastest):
+    dims = (2, 3, 2, 3)
+    x = rand_tt(dims, [1, 2, 3, 2, 1], 70)
+    y = rand_tt(dims, [1, 3, 2, 4, 1], 71)
+    z = rand_tt(dims, [1, 2, 2, 2, 1], 72)
+    p2 = O.hadamard([x, y], first_fastest)
+    np.testing.assert_allclose(O.ttv_to_tensor(p2), O.ttv_to_tensor(x) * O.ttv_to_tensor(y), rtol=1e-12, atol=1e-13)
+    assert p2.rks == [a * b for a, b in zip(x.rks, y.rks)] and p2.ot == [0, 0, 0, 0]
+    p3 = O.hadamard([x, y, z], first_fastest)
+    np.testing.assert_allclose(O.ttv_to_tensor(p3), O.ttv_to_tensor(x) * O.ttv_to_tensor(y) * O.ttv_to_tensor(z),
+                               rtol=1e-12, atol=1e-13)
+    # bond order: kron(x, y) puts y's rank index fastest (src/tt_operations.jl:283);
+    # the tuple order puts x's fastest
+    rxa, rxb, rya, ryb = x.rks[1], x.rks[2], y.rks[1], y.rks[2]
+    for a in range(rxa * rya):
+        for b in range(rxb * ryb):
+            if first_fastest:
+                da, db = (a % rxa, a // rxa), (b % rxb, b // rxb)
+            else:
+                da, db = (a // rya, a % rya), (b // ryb, b % ryb)
+            ref = x.cores[1][1, da[0], db[0]] * y.cores[1][1, da[1], db[1]]
+            assert p2.cores[1][1, a, b] == ref
+
+
+def test_hadamard_sketch_and_rounding_factored():
+    """The factored NTuple sketch and rounding (product never formed) equal the
+    plain sketch and rounding of the explicit product in tuple order; the
+    rounding recovers the product exactly when the ranks cover it."""
+    dims = (2, 3, 2, 2)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 3, 2, 1], 73)
+    y = rand_tt(dims, [1, 2, 2, 3, 1], 74)
+    xy = O.hadamard([x, y], first_fastest=True)
+    rks = [1, 4, 6, 6, 1]
+    for orth in (False, True):
+        dr = _draws_for(dims, rks, None)
+        Wt, st = O.tt_recursive_sketch_tuple([x, y], rks, dr, orth)
+        We, se = O.tt_recursive_sketch(xy, rks, dr, orth)
+        assert st == se
+        for k in range(N + 1):
+            assert Wt[k].shape == (x.rks[k], y.rks[k], se[k])
+            ex = We[k].reshape(x.rks[k], y.rks[k], -1, order="F")
+            np.testing.assert_allclose(Wt[k], ex, rtol=1e-11, atol=1e-12 * np.abs(ex).max())
+        rt = O.ttrand_rounding_tuple([x, y], rks, dr, orth)
+        re = O.ttrand_rounding(xy, rks, dr, orth)
+        assert rt.rks == re.rks
+        np.testing.assert_allclose(O.ttv_to_tensor(rt), O.ttv_to_tensor(re), rtol=1e-10, atol=1e-11)
+        assert O.rel_err(xy, rt) < 1e-11
+    # boundary sketch of the product
+    bv, _ = O.sketch_structure(dims, [1] * (N + 1))
+    pc = -(-7 // bv[0])
+    dr = [RNG.standard_normal(bv[k + 1] * dims[k] * bv[k] * pc) for k in range(N)]
+    st, sa = O.tt_sketch_tuple([x, y], 7, dr, True)
+    se, sb = O.tt_sketch(xy, 7, dr, True)
+    assert sa == sb
+    np.testing.assert_allclose(st, se, rtol=1e-11, atol=1e-12 * np.abs(se).max())
 
 
 # ---------------------------------------------------------------- RNG
