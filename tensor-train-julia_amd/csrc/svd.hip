@@ -28,6 +28,7 @@ namespace {
 typedef double d4v __attribute__((ext_vector_type(4)));
 
 constexpr int RCH = 32;         // row chunk for Gram / apply
+constexpr int RING = 3;         // row chunks in flight (registers) ahead of the MFMAs
 constexpr int MAX_SWEEPS = 40;
 
 // Shape of a visit for column blocks of width WB_ (16 or 32)
@@ -81,8 +82,7 @@ __device__ __forceinline__ double rcp_nr(double x) {
 }
 
 // Cross-lane moves of a double on the VALU (DPP): lane i takes lane i - 1's
-// value (wave_shr:1; lane 0 gets 0) or lane i + 1's (wave_shl:1; lane 63 gets 0),
-// and lane l's value in every lane (readlane takes 32-bit operands).
+// value (wave_shr:1; lane 0 gets 0) or lane i + 1's (wave_shl:1; lane 63 gets 0).
 __device__ __forceinline__ double from_lower_lane(double v) {
   const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, false);
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, false);
@@ -92,9 +92,6 @@ __device__ __forceinline__ double from_upper_lane(double v) {
   const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, false);
   const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, false);
   return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double lane_value(double v, int l) {
-  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l), __builtin_amdgcn_readlane(__double2loint(v), l));
 }
 
 // ---- one inner Jacobi sweep on the 64x64 Gram matrix: 63 rounds of 32
@@ -112,11 +109,9 @@ __device__ __forceinline__ double lane_value(double v, int l) {
 //     (j = 0..3) and positions P, 63 - P; the move is a shift by one lane
 //     within each 32-lane half (DPP, plus the two ends of the round-robin
 //     table), so V never touches LDS during the sweep.
-// A round is two phases, each ending with a barrier: lanes 0..31 of wave 0
-// compute pair t's rotation (c, s) from the current buffer (position t first),
-// then every thread rotates its G blocks and its V rows.  On exit V, in player
-// order again, is stored in GA (GB is free); *vrot is set when any rotation
-// acted.
+// Lanes 0..31 of wave 0 compute pair t's rotation (c, s) (position t first);
+// the phases of a round are described at the loop.  On exit V, in player order
+// again, is stored in GA (GB is free); *vrot is set when any rotation acted.
 template <int WB_>
 __device__ __forceinline__ void inner_sweep(double (*GA)[Jac<WB_>::GLD], double (*GB)[Jac<WB_>::GLD], double tol2,
                                             int tid, int* vrot) {
@@ -177,12 +172,10 @@ __device__ __forceinline__ void inner_sweep(double (*GA)[Jac<WB_>::GLD], double 
   // itself (t = 0); position IR - t takes IR - 1 - t, the right slot of the lane
   // above, or the own left slot (t = NPAIR - 1).  The lanes that receive a
   // value from across a group boundary are exactly those that do not use it.
-  for (int ir = 0; ir < IR; ++ir) {
-    double(*cur)[GLD] = (ir & 1) ? GB : GA;
-    double(*nxt)[GLD] = (ir & 1) ? GA : GB;
-    // ---- phase 1: rotation parameters
-    if (wave == 0 && lane < NPAIR) {
-      const double aa = cur[x0][x0], bb2 = cur[x1][x1], ab = cur[x0][x1];
+  // rotation parameters of one round from the G buffer g (lanes 0..NPAIR-1 of wave 0)
+  auto rotations = [&](const double (*g)[GLD]) {
+    if (lane < NPAIR) {
+      const double aa = g[x0][x0], bb2 = g[x1][x1], ab = g[x0][x1];
       double c = 1.0, s = 0.0;
       // padding columns have a zero diagonal and never rotate (bitwise & keeps
       // the three loads ahead of a single wait)
@@ -204,8 +197,32 @@ __device__ __forceinline__ void inner_sweep(double (*GA)[Jac<WB_>::GLD], double 
       }
       pcs[lane] = make_double2(c, s);
     }
-    __syncthreads();
-    // ---- phase 2: G blocks (current -> next buffer, moved) and V rows
+  };
+  // one round's rotation and move of this thread's V rows
+  auto v_update = [&](double2 pr) {
+#pragma unroll
+    for (int j = 0; j < C::VROWS; ++j) {
+      const double nl = pr.x * VL[j] - pr.y * VR[j];
+      const double nr = pr.y * VL[j] + pr.x * VR[j];
+      const double fromL = from_lower_lane(P == 0 ? nr : nl);
+      const double fromR = from_upper_lane(nr);
+      VL[j] = P == 0 ? nl : fromL;
+      VR[j] = P == NPAIR - 1 ? nl : fromR;
+    }
+  };
+  // A round is two phases, each ending with a barrier.  Phase A: every thread
+  // rotates its G blocks (current -> next buffer, moved).  Phase B: wave 0
+  // computes the next round's rotations from the new buffer while the other
+  // waves rotate their V rows; wave 0 rotates its own V rows one round late,
+  // in the next phase A (its rows are touched by no other wave), so no wave
+  // does both the rotation parameters and V work between the same barriers.
+  if (wave == 0) rotations(GA);
+  __syncthreads();
+  double2 held = make_double2(1.0, 0.0);
+  for (int ir = 0; ir < IR; ++ir) {
+    double(*cur)[GLD] = (ir & 1) ? GB : GA;
+    double(*nxt)[GLD] = (ir & 1) ? GA : GB;
+    // ---- phase A
     const double2 pr = pcs[P];
     const double* cb = &cur[0][0];
     double* nb = &nxt[0][0];
@@ -222,17 +239,18 @@ __device__ __forceinline__ void inner_sweep(double (*GA)[Jac<WB_>::GLD], double 
         nb[wr[b][3]] = t10 * pq.y + t11 * pq.x;
       }
     }
-#pragma unroll
-    for (int j = 0; j < C::VROWS; ++j) {
-      const double nl = pr.x * VL[j] - pr.y * VR[j];
-      const double nr = pr.y * VL[j] + pr.x * VR[j];
-      const double fromL = from_lower_lane(P == 0 ? nr : nl);
-      const double fromR = from_upper_lane(nr);
-      VL[j] = P == 0 ? nl : fromL;
-      VR[j] = P == NPAIR - 1 ? nl : fromR;
+    if (wave == 0 && ir > 0) v_update(held);
+    __syncthreads();
+    // ---- phase B
+    if (wave == 0) {
+      if (ir + 1 < IR) rotations(nxt);
+      held = pr;
+    } else {
+      v_update(pr);
     }
     __syncthreads();
   }
+  if (wave == 0) v_update(held);
   // V back in player order (positions = players after 63 moves) into GA
 #pragma unroll
   for (int j = 0; j < C::VROWS; ++j) {
@@ -293,42 +311,63 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
   __syncthreads();
 
   // ---- Gram matrix G = C^T C (MFMA f64 16x16x4; wave w owns output tiles w + NW t)
+  // Row chunks of RCH rows pass through LDS; the global loads run RING chunks
+  // ahead in registers, so each load has RING chunks of MFMA work to arrive.
   constexpr int PER = C::PER;
+  const int nch = (p + RCH - 1) / RCH;
+  auto load_chunk = [&](const double* M, int64_t ld, int r0, double (&dst)[PER]) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * JT, rr = r0 + e % RCH, gc = cols[e / RCH];
+      dst[q] = (gc >= 0 && rr < p) ? M[rr + int64_t(gc) * ld] : 0.0;
+    }
+  };
+  auto stage_chunk = [&](double (*buf)[GLD], const double (&src)[PER]) {
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * JT;
+      buf[e % RCH][e / RCH] = src[q];
+    }
+  };
   {
+    // the Gram's chunk is staged column-major ([column][row], pitch 2 mod 32
+    // doubles): the MFMA operands are then 16 columns x 4 rows per read, which
+    // fall in distinct banks; the staging stores are contiguous
+    constexpr int CP = RCH + 2;
+    static_assert(WC * CP <= WC * GLD, "the Gram chunk must fit in GA");
+    double (*gchunk)[CP] = reinterpret_cast<double (*)[CP]>(&GA[0][0]);
     d4v gacc[C::TPW];
 #pragma unroll
     for (int t = 0; t < C::TPW; ++t) gacc[t] = d4v{0, 0, 0, 0};
-    double pre[PER];
+    double pre[RING][PER];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int e = tid + q * JT, rr = e % RCH, gc = cols[e / RCH];
-      pre[q] = (gc >= 0 && rr < p) ? S.B[rr + int64_t(gc) * S.ldb] : 0.0;
-    }
-    for (int r0 = 0; r0 < p; r0 += RCH) {
+    for (int u = 0; u < RING; ++u)
+      if (u < nch) load_chunk(S.B, S.ldb, u * RCH, pre[u]);
+    for (int c0 = 0; c0 < nch; c0 += RING) {
 #pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        const int e = tid + q * JT;
-        chunk[e % RCH][e / RCH] = pre[q];
-      }
-      __syncthreads();
-      if (r0 + RCH < p) {  // prefetch the next chunk while the MFMAs run
+      for (int u = 0; u < RING; ++u) {
+        const int c = c0 + u;
+        if (c < nch) {  // uniform over the workgroup
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-          const int e = tid + q * JT, rr = r0 + RCH + e % RCH, gc = cols[e / RCH];
-          pre[q] = (gc >= 0 && rr < p) ? S.B[rr + int64_t(gc) * S.ldb] : 0.0;
+          for (int q = 0; q < PER; ++q) {
+            const int e = tid + q * JT;
+            gchunk[e / RCH][e % RCH] = pre[u][q];
+          }
+          __syncthreads();
+          if (c + RING < nch) load_chunk(S.B, S.ldb, (c + RING) * RCH, pre[u]);
+#pragma unroll
+          for (int t = 0; t < C::TPW; ++t) {
+            const int blk = wave + NW * t, bi = blk / NT, bj = blk % NT;
+#pragma unroll
+            for (int kk = 0; kk < RCH; kk += 4) {
+              const double a = gchunk[16 * bi + (lane & 15)][kk + (lane >> 4)];
+              const double b = gchunk[16 * bj + (lane & 15)][kk + (lane >> 4)];
+              gacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, gacc[t], 0, 0, 0);
+            }
+          }
+          __syncthreads();
         }
       }
-#pragma unroll
-      for (int t = 0; t < C::TPW; ++t) {
-        const int blk = wave + NW * t, bi = blk / NT, bj = blk % NT;
-#pragma unroll
-        for (int kk = 0; kk < RCH; kk += 4) {
-          const double a = chunk[kk + (lane >> 4)][16 * bi + (lane & 15)];
-          const double b = chunk[kk + (lane >> 4)][16 * bj + (lane & 15)];
-          gacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, gacc[t], 0, 0, 0);
-        }
-      }
-      __syncthreads();
     }
 #pragma unroll
     for (int t = 0; t < C::TPW; ++t) {
@@ -390,45 +429,48 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
     if (S.info) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 5, 1u);
   }
 
-  // ---- apply V to the visited columns of B and J: C <- C V (wave w: row tile w / NT, column tile w % NT)
-  for (int which = 0; which < 2; ++which) {
-    double* M = which == 0 ? S.B : S.J;
-    const int64_t ld = which == 0 ? S.ldb : S.ldj;
-    double pre[PER];
+  // ---- apply V to the visited columns of B and J: C <- C V (wave w: row tile w / NT,
+  // column tile w % NT); the chunks of B and then of J form one sequence, so the
+  // loads of J's first chunks overlap B's last MFMAs
+  {
+    const int tot = 2 * nch;
+    auto mat = [&](int c) { return c < nch ? S.B : S.J; };
+    auto mld = [&](int c) { return c < nch ? S.ldb : S.ldj; };
+    double pre[RING][PER];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-      const int e = tid + q * JT, rr = e % RCH, gc = cols[e / RCH];
-      pre[q] = (gc >= 0 && rr < p) ? M[rr + int64_t(gc) * ld] : 0.0;
-    }
-    for (int r0 = 0; r0 < p; r0 += RCH) {
+    for (int u = 0; u < RING; ++u)
+      if (u < tot) load_chunk(mat(u), mld(u), (u % nch) * RCH, pre[u]);
+    const int bi = wave / NT, bj = wave % NT;
+    // this wave's MFMA B operands (column tile bj of V) for every k, read once
+    double vb[WC / 4];
 #pragma unroll
-      for (int q = 0; q < PER; ++q) {
-        const int e = tid + q * JT;
-        chunk[e % RCH][e / RCH] = pre[q];
-      }
-      __syncthreads();
-      if (r0 + RCH < p) {
+    for (int kk = 0; kk < WC; kk += 4) vb[kk / 4] = V[kk + (lane >> 4)][16 * bj + (lane & 15)];
+    for (int c0 = 0; c0 < tot; c0 += RING) {
 #pragma unroll
-        for (int q = 0; q < PER; ++q) {
-          const int e = tid + q * JT, rr = r0 + RCH + e % RCH, gc = cols[e / RCH];
-          pre[q] = (gc >= 0 && rr < p) ? M[rr + int64_t(gc) * ld] : 0.0;
+      for (int u = 0; u < RING; ++u) {
+        const int c = c0 + u;
+        if (c < tot) {  // uniform over the workgroup
+          double* M = mat(c);
+          const int64_t ld = mld(c);
+          const int r0 = (c % nch) * RCH;
+          stage_chunk(chunk, pre[u]);
+          __syncthreads();
+          if (c + RING < tot) load_chunk(mat(c + RING), mld(c + RING), ((c + RING) % nch) * RCH, pre[u]);
+          d4v acc = {0, 0, 0, 0};
+#pragma unroll
+          for (int kk = 0; kk < WC; kk += 4) {
+            const double av = chunk[16 * bi + (lane & 15)][kk + (lane >> 4)];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, vb[kk / 4], acc, 0, 0, 0);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int row = r0 + 16 * bi + 4 * q + (lane >> 4);
+            const int gc = cols[16 * bj + (lane & 15)];
+            if (gc >= 0 && row < p) M[row + int64_t(gc) * ld] = acc[q];
+          }
+          __syncthreads();
         }
       }
-      const int bi = wave / NT, bj = wave % NT;
-      d4v acc = {0, 0, 0, 0};
-#pragma unroll
-      for (int kk = 0; kk < WC; kk += 4) {
-        const double av = chunk[16 * bi + (lane & 15)][kk + (lane >> 4)];
-        const double bv = V[kk + (lane >> 4)][16 * bj + (lane & 15)];
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = r0 + 16 * bi + 4 * q + (lane >> 4);
-        const int gc = cols[16 * bj + (lane & 15)];
-        if (gc >= 0 && row < p) M[row + int64_t(gc) * ld] = acc[q];
-      }
-      __syncthreads();
     }
   }
   if (tid == 0 && S.info) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 3, unsigned(wall_clock64() - t2));
