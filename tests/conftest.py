@@ -8,4 +8,4 @@ for p in (ROOT, os.path.join(ROOT, "tensor-train-julia_amd")):
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X GPU (runs on the GPU box)")
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU")
