@@ -105,6 +105,24 @@ def test_sketches_of_tuple_vs_oracle(T):
         assert Wc[k].shape == tuple(t.rks[k] for t in xs) + (sc[k],)
 
 
+def test_forward_sketch_of_tuple(T):
+    """tt_recursive_sketch((x, y), rks; reverse=false): the forward sketch of the
+    product in tuple order, rows reshaped to (r_xk, r_yk, s_k)."""
+    say("case: forward sketch of (x, y)")
+    dims = (2, 3, 2, 3)
+    x = rand_tt(dims, [1, 2, 3, 2, 1], 340)
+    y = rand_tt(dims, [1, 2, 2, 3, 1], 341)
+    rks = [1, 4, 6, 5, 1]
+    W, s = T.tt_recursive_sketch((to_api(x), to_api(y)), rks, reverse=False, seed=12)
+    prov = device_blocks_provider(dims, rks, orthogonal=True, seed=12, forward=True)
+    Wr, sr = O.tt_recursive_sketch_forward(O.hadamard([x, y], first_fastest=True), rks, prov, True)
+    assert s == sr
+    for k in range(len(dims) + 1):
+        ref = Wr[k].reshape(x.rks[k], y.rks[k], -1, order="F")
+        assert W[k].shape == ref.shape
+        np.testing.assert_allclose(W[k], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+
+
 def test_hadamard_errors(T):
     say("case: ttr_hadamard argument errors")
     x = rand_tt((2, 3, 2), [1, 2, 2, 1], 330)
