@@ -221,9 +221,10 @@ __device__ __forceinline__ void inner_sweep(double (*GA)[Jac<WB_>::GLD], double 
   if (wave == 0) rotations(GA);
   __syncthreads();
   double2 held = make_double2(1.0, 0.0);
-  for (int ir = 0; ir < IR; ++ir) {
-    double(*cur)[GLD] = (ir & 1) ? GB : GA;
-    double(*nxt)[GLD] = (ir & 1) ? GA : GB;
+  // one round from buffer cur into buffer nxt; the loop below is unrolled by two
+  // so that cur and nxt are fixed arrays at each call site and every LDS address
+  // is a loop-invariant register plus an immediate offset
+  auto round_step = [&](const double (*cur)[GLD], double (*nxt)[GLD], int ir) {
     // ---- phase A
     const double2 pr = pcs[P];
     const double* cb = &cur[0][0];
@@ -251,6 +252,10 @@ __device__ __forceinline__ void inner_sweep(double (*GA)[Jac<WB_>::GLD], double 
       v_update(pr);
     }
     __syncthreads();
+  };
+  for (int ir = 0; ir < IR; ir += 2) {
+    round_step(GA, GB, ir);
+    if (ir + 1 < IR) round_step(GB, GA, ir + 1);
   }
   if (wave == 0) v_update(held);
   // V back in player order (positions = players after 63 moves) into GA
