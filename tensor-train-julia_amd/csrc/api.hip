@@ -472,10 +472,13 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
         ps.bytes = std::max(ps.bytes, double(info[0]));
         // visits and wall-clock ticks (100 MHz) summed over visits: Gram, inner sweep, apply
         ctx.prof.stats["jacobi_svd:visits"].flops += info[4];
-        // MFMA work: every visit forms a wc-column Gram (2*wc*wc*m flops), a rotating
+        // MFMA work: every visit forms the upper triangle of a wc-column Gram (its
+        // wc/16 (wc/16 + 1) / 2 tiles of 16 x 16, 2*256*m flops each), a rotating
         // visit applies V to wc columns of B and of J (2 * 2*m*wc*wc); bytes: the
         // columns read by the Gram, read and written by the apply
-        ctx.prof.stats["jacobi_svd"].flops += 2.0 * wc * wc * m * (double(info[4]) + 2.0 * info[5]);
+        const double gram_tiles = (wc / 16) * (wc / 16 + 1) / 2;
+        ctx.prof.stats["jacobi_svd"].flops +=
+            2.0 * m * (256.0 * gram_tiles * double(info[4]) + 2.0 * wc * wc * info[5]);
         ctx.prof.stats["jacobi_svd"].bytes += 8.0 * wc * m * (double(info[4]) + 4.0 * info[5]);
         ctx.prof.stats["jacobi_svd:ticks_gram"].flops += info[1];
         ctx.prof.stats["jacobi_svd:ticks_inner"].flops += info[2];

@@ -42,7 +42,19 @@ struct Jac {
   static constexpr int NPAIR = WB;           // rotations per inner round
   static constexpr int IR = WC - 1;          // rounds of one inner sweep
   static constexpr int NT = WC / 16;         // 16x16 MFMA tiles per Gram row
-  static constexpr int TPW = NT * NT / NW;   // Gram tiles per wave
+  static constexpr int TPW = NT * NT / NW;   // Gram tiles per wave (at most)
+  // The inner sweep reads only the upper triangle of G: the NGT tiles bi <= bj.
+  // Tile k goes to wave k % NW; its (bi, bj) are bits 4k.. and 4k+2.. of GTILES.
+  // For 4 x 4 tiles the order puts (0,3),(1,3) on wave 0 and (2,3),(3,3) on
+  // wave 1, so a wave with two tiles reads one column operand for both.
+  static constexpr int NGT = NT * (NT + 1) / 2;
+  static constexpr unsigned long long GTILES =
+      NT == 4 ? (0x0ull | (0x3ull << 2)) | ((0x2ull | (0x3ull << 2)) << 4) | ((0x0ull | (0x0ull << 2)) << 8) |
+                    ((0x0ull | (0x1ull << 2)) << 12) | ((0x1ull | (0x1ull << 2)) << 16) |
+                    ((0x0ull | (0x2ull << 2)) << 20) | ((0x1ull | (0x2ull << 2)) << 24) |
+                    ((0x2ull | (0x2ull << 2)) << 28) | ((0x1ull | (0x3ull << 2)) << 32) |
+                    ((0x3ull | (0x3ull << 2)) << 36)
+              : (0x0ull | (0x0ull << 2)) | ((0x0ull | (0x1ull << 2)) << 4) | ((0x1ull | (0x1ull << 2)) << 8);
   static constexpr int VROWS = WB / 8;       // rows of V per thread
   static constexpr int PER = RCH * WC / JT;  // staged chunk elements per thread
   static constexpr int MINB = WB == 16 ? 4 : 2;  // resident visits per CU the registers allow
@@ -317,7 +329,7 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
   }
   __syncthreads();
 
-  // ---- Gram matrix G = C^T C (MFMA f64 16x16x4; wave w owns output tiles w + NW t)
+  // ---- Gram matrix G = C^T C, upper-triangle tiles (MFMA f64 16x16x4)
   // Row chunks of RCH rows pass through LDS; the global loads run RING chunks
   // ahead in registers, so each load has RING chunks of MFMA work to arrive.
   constexpr int PER = C::PER;
@@ -364,12 +376,15 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
           if (c + RING < nch) load_chunk(S.B, S.ldb, (c + RING) * RCH, pre[u]);
 #pragma unroll
           for (int t = 0; t < C::TPW; ++t) {
-            const int blk = wave + NW * t, bi = blk / NT, bj = blk % NT;
+            const int k = wave + NW * t;
+            if (k < C::NGT) {  // uniform over the wave
+              const int bi = int(C::GTILES >> (4 * k)) & 3, bj = int(C::GTILES >> (4 * k + 2)) & 3;
 #pragma unroll
-            for (int kk = 0; kk < RCH; kk += 4) {
-              const double a = gchunk[16 * bi + (lane & 15)][kk + (lane >> 4)];
-              const double b = gchunk[16 * bj + (lane & 15)][kk + (lane >> 4)];
-              gacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, gacc[t], 0, 0, 0);
+              for (int kk = 0; kk < RCH; kk += 4) {
+                const double a = gchunk[16 * bi + (lane & 15)][kk + (lane >> 4)];
+                const double b = gchunk[16 * bj + (lane & 15)][kk + (lane >> 4)];
+                gacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, gacc[t], 0, 0, 0);
+              }
             }
           }
           __syncthreads();
@@ -378,9 +393,12 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
     }
 #pragma unroll
     for (int t = 0; t < C::TPW; ++t) {
-      const int blk = wave + NW * t, bi = blk / NT, bj = blk % NT;
+      const int k = wave + NW * t;
+      if (k < C::NGT) {
+        const int bi = int(C::GTILES >> (4 * k)) & 3, bj = int(C::GTILES >> (4 * k + 2)) & 3;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) G[16 * bi + 4 * q + (lane >> 4)][16 * bj + (lane & 15)] = gacc[t][q];
+        for (int q = 0; q < 4; ++q) G[16 * bi + 4 * q + (lane >> 4)][16 * bj + (lane & 15)] = gacc[t][q];
+      }
     }
   }
   __syncthreads();
