@@ -5,10 +5,10 @@
 // which every disjoint block pair of every matrix is one workgroup, so a large
 // matrix is worked on by many CUs at once and a batch of uneven matrices keeps
 // the whole GPU busy.  A visit
-//   1. forms the Gram matrix G = C^T C of the (up to) 64 columns C of the
-//      block pair in LDS with FP64 MFMA, straight from the current columns
-//      (rotation angles always come from fresh inner products, as in plain
-//      one-sided Jacobi);
+//   1. forms the upper triangle of the Gram matrix G = C^T C of the (up to) 64
+//      columns C of the block pair in LDS with FP64 MFMA, straight from the
+//      current columns (rotation angles always come from fresh inner products,
+//      as in plain one-sided Jacobi);
 //   2. runs one parallel cyclic Jacobi sweep on G (32 disjoint rotations per
 //      round, round-robin order, G in LDS and the accumulated rotation V in
 //      registers; see inner_sweep); a pair is rotated when
