@@ -354,7 +354,17 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
     // fall in distinct banks; the staging stores are contiguous
     constexpr int CP = RCH + 2;
     static_assert(WC * CP <= WC * GLD, "the Gram chunk must fit in GA");
-    double (*gchunk)[CP] = reinterpret_cast<double (*)[CP]>(&GA[0][0]);
+    // two chunk buffers (GA, GB; GB is free until the inner sweep): chunk c + 1 is
+    // staged while the MFMAs of chunk c run, one barrier per chunk
+    double (*gbuf0)[CP] = reinterpret_cast<double (*)[CP]>(&GA[0][0]);
+    double (*gbuf1)[CP] = reinterpret_cast<double (*)[CP]>(&GB[0][0]);
+    auto gstage = [&](double (*buf)[CP], const double (&src)[PER]) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int e = tid + q * JT;
+        buf[e / RCH][e % RCH] = src[q];
+      }
+    };
     d4v gacc[C::TPW];
 #pragma unroll
     for (int t = 0; t < C::TPW; ++t) gacc[t] = d4v{0, 0, 0, 0};
@@ -362,18 +372,20 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
 #pragma unroll
     for (int u = 0; u < RING; ++u)
       if (u < nch) load_chunk(S.B, S.ldb, u * RCH, pre[u]);
+    if (nch > 0) gstage(gbuf0, pre[0]);
+    if (RING < nch) load_chunk(S.B, S.ldb, RING * RCH, pre[0]);
+    __syncthreads();
     for (int c0 = 0; c0 < nch; c0 += RING) {
 #pragma unroll
       for (int u = 0; u < RING; ++u) {
         const int c = c0 + u;
         if (c < nch) {  // uniform over the workgroup
-#pragma unroll
-          for (int q = 0; q < PER; ++q) {
-            const int e = tid + q * JT;
-            gchunk[e / RCH][e % RCH] = pre[u][q];
+          const int un = (u + 1) % RING;
+          if (c + 1 < nch) {
+            gstage((c & 1) ? gbuf0 : gbuf1, pre[un]);
+            if (c + 1 + RING < nch) load_chunk(S.B, S.ldb, (c + 1 + RING) * RCH, pre[un]);
           }
-          __syncthreads();
-          if (c + RING < nch) load_chunk(S.B, S.ldb, (c + RING) * RCH, pre[u]);
+          const double (*gch)[CP] = (c & 1) ? gbuf1 : gbuf0;
 #pragma unroll
           for (int t = 0; t < C::TPW; ++t) {
             const int k = wave + NW * t;
@@ -381,8 +393,8 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
               const int bi = int(C::GTILES >> (4 * k)) & 3, bj = int(C::GTILES >> (4 * k + 2)) & 3;
 #pragma unroll
               for (int kk = 0; kk < RCH; kk += 4) {
-                const double a = gchunk[16 * bi + (lane & 15)][kk + (lane >> 4)];
-                const double b = gchunk[16 * bj + (lane & 15)][kk + (lane >> 4)];
+                const double a = gch[16 * bi + (lane & 15)][kk + (lane >> 4)];
+                const double b = gch[16 * bj + (lane & 15)][kk + (lane >> 4)];
                 gacc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, gacc[t], 0, 0, 0);
               }
             }
@@ -465,32 +477,44 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
 #pragma unroll
     for (int u = 0; u < RING; ++u)
       if (u < tot) load_chunk(mat(u), mld(u), (u % nch) * RCH, pre[u]);
+    // wave w: row tile bi = w / NT, column tile bj = w % NT of each chunk
     const int bi = wave / NT, bj = wave % NT;
-    // this wave's MFMA B operands (column tile bj of V) for every k, read once
+    // this wave's MFMA B operands (column tile bj of V) for every k, read once;
+    // V's buffer (GA) then becomes the second chunk buffer beside GB
     double vb[WC / 4];
 #pragma unroll
     for (int kk = 0; kk < WC; kk += 4) vb[kk / 4] = V[kk + (lane >> 4)][16 * bj + (lane & 15)];
+    __syncthreads();
+    double(*abuf0)[GLD] = GB;
+    double(*abuf1)[GLD] = GA;
+    if (tot > 0) stage_chunk(abuf0, pre[0]);
+    if (RING < tot) load_chunk(mat(RING), mld(RING), (RING % nch) * RCH, pre[0]);
+    __syncthreads();
+    const int gc = cols[16 * bj + (lane & 15)];
     for (int c0 = 0; c0 < tot; c0 += RING) {
 #pragma unroll
       for (int u = 0; u < RING; ++u) {
         const int c = c0 + u;
         if (c < tot) {  // uniform over the workgroup
+          const int un = (u + 1) % RING;
+          if (c + 1 < tot) {
+            stage_chunk((c & 1) ? abuf0 : abuf1, pre[un]);
+            const int cn = c + 1 + RING;
+            if (cn < tot) load_chunk(mat(cn), mld(cn), (cn % nch) * RCH, pre[un]);
+          }
+          const double(*ch)[GLD] = (c & 1) ? abuf1 : abuf0;
           double* M = mat(c);
           const int64_t ld = mld(c);
           const int r0 = (c % nch) * RCH;
-          stage_chunk(chunk, pre[u]);
-          __syncthreads();
-          if (c + RING < tot) load_chunk(mat(c + RING), mld(c + RING), ((c + RING) % nch) * RCH, pre[u]);
           d4v acc = {0, 0, 0, 0};
 #pragma unroll
           for (int kk = 0; kk < WC; kk += 4) {
-            const double av = chunk[16 * bi + (lane & 15)][kk + (lane >> 4)];
+            const double av = ch[16 * bi + (lane & 15)][kk + (lane >> 4)];
             acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, vb[kk / 4], acc, 0, 0, 0);
           }
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int row = r0 + 16 * bi + 4 * q + (lane >> 4);
-            const int gc = cols[16 * bj + (lane & 15)];
             if (gc >= 0 && row < p) M[row + int64_t(gc) * ld] = acc[q];
           }
           __syncthreads();
