@@ -75,14 +75,16 @@ __device__ __forceinline__ double amode_value(int mode, int64_t gi, int64_t gl, 
 constexpr int BM = 64, BN = 64;  // tile of the BF16 kernel
 
 // ------------------------------------------------------------- F64 / F32 math
-// Tile TM x TN (64 or 128 each), BK = 16, 256 threads as 2 x 2 waves; wave (wm, wn)
-// owns a (TM/2) x (TN/2) block made of (TM/32) x (TN/32) MFMA 16x16 outputs.  The
-// accumulation order of every output element is the K order whatever the tile,
-// so all tile shapes give bitwise identical results.
-template <class TA, class TB, class TC, class MT, int TM, int TN>
+// Tile TM x TN (64 or 128 each), k-step BK (8 or 16), 256 threads as 2 x 2 waves;
+// wave (wm, wn) owns a (TM/2) x (TN/2) block made of (TM/32) x (TN/32) MFMA 16x16
+// outputs.  The accumulation order of every output element is the K order whatever
+// the tile, and the split-K chunks are counted in k-tiles of KS = 16 for every BK,
+// so all tile configurations give bitwise identical results.
+constexpr int KS = 16;
+template <class TA, class TB, class TC, class MT, int TM, int TN, int BK>
 __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __restrict__ probs, int nprobs,
                                                           const GemmSeg* __restrict__ segs) {
-  constexpr int BK = 16;
+  static_assert(KS % BK == 0, "a split-K k-tile holds whole k-steps");
   constexpr int MI = TM / 32, NI = TN / 32;
   constexpr int NA = TM * BK / 256, NB = TN * BK / 256;  // tile elements per thread
   __shared__ MT As[BK][TM + 1];
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
   const int row0 = (t / P.tiles_n) * TM, col0 = (t % P.tiles_n) * TN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
-  // k-tiles [kt_lo, kt_hi) of the problem's segment sequence belong to this workgroup
+  // k-tiles (of KS) [kt_lo, kt_hi) of the problem's segment sequence belong to this workgroup
   const int kt_lo = P.ksplit > 1 ? chunk * P.kc : 0;
   const int kt_hi = P.ksplit > 1 ? kt_lo + P.kc : (1 << 30);
 
@@ -113,8 +115,8 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
   int kt_seg = 0;  // first k-tile of the segment in the problem's sequence
   for (int s = 0; s < P.nseg; ++s) {
     const GemmSeg S = segs[P.seg0 + s];
-    const int nkt = (S.K + BK - 1) / BK;
-    const int kb = max(kt_lo - kt_seg, 0) * BK, ke = min(kt_hi - kt_seg, nkt) * BK;
+    const int nkt = (S.K + KS - 1) / KS;
+    const int kb = max(kt_lo - kt_seg, 0) * KS, ke = min(kt_hi - kt_seg, nkt) * KS;
     kt_seg += nkt;
     if (kb >= ke) continue;
     const int kend = min(ke, S.K);
@@ -369,19 +371,23 @@ void GemmBatch::seg(const void* A, int64_t a_rs, int64_t a_cs, const void* B, in
 }
 
 namespace {
-// tile configurations of the scalar kernel: {TM, TN}
-constexpr int NCFG = 4;
-constexpr int CFG_TM[NCFG] = {64, 128, 64, 128};
-constexpr int CFG_TN[NCFG] = {64, 64, 128, 128};
+// tile configurations of the scalar kernel: {TM, TN, BK}
+constexpr int NCFG = 8;
+constexpr int CFG_TM[NCFG] = {64, 128, 64, 128, 64, 128, 64, 128};
+constexpr int CFG_TN[NCFG] = {64, 64, 128, 128, 64, 64, 128, 128};
 
 template <class TA, class TB, class TC, class MT>
 void launch_scalar(Ctx& ctx, int cfg, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
   dim3 grid(ntiles), block(256);
   switch (cfg) {
-    case 0: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 64>), grid, block, 0, ctx.stream, dp, np, ds); break;
-    case 1: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 64>), grid, block, 0, ctx.stream, dp, np, ds); break;
-    case 2: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 128>), grid, block, 0, ctx.stream, dp, np, ds); break;
-    default: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 128>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    case 0: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 64, 16>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    case 1: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 64, 16>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    case 2: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 128, 16>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    case 3: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 128, 16>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    case 4: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 64, 8>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    case 5: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 64, 8>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    case 6: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 128, 8>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    default: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 128, 8>), grid, block, 0, ctx.stream, dp, np, ds); break;
   }
 }
 
