@@ -49,7 +49,10 @@ typedef struct ttr_handle_s* ttr_handle_t;
 /* library identification */
 const char* ttr_version(void);
 
-/* creates a handle on HIP device `device` (one stream, empty workspace) */
+/* creates a handle on HIP device `device` (one stream, empty workspace).  The
+   handle times the GEMM tile configurations on the first call at each site and
+   keeps the fastest (every configuration gives identical bits); with the
+   environment variable TTR_GEMM_TUNE=0 set at creation it uses fixed tiles. */
 int ttr_create(ttr_handle_t* h, int device);
 int ttr_destroy(ttr_handle_t h);
 /* message of the last failed call on this handle ("" if none) */
