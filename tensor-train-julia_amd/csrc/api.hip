@@ -391,6 +391,9 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
   }
   // Jacobi counters of all items in one array (zeroed before every SVD step)
   int32_t* info_all = ctx.ws.alloc_n<int32_t>(size_t(8) * nb);
+  // certificate flags of all items (device and host copies)
+  int32_t* cflags = ctx.ws.alloc_n<int32_t>(nb);
+  int32_t* hcflags = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * nb));
   for (size_t q = 0; q < nb; ++q) st[q].info = info_all + 8 * q;
   GemmBatch gb;
   const bool left = direction == TTR_DIR_LEFT;
@@ -430,7 +433,6 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     qr_batched(ctx, qj);
     // B from the triangular factor: R (q x q) or R^T, see the per-case algebra below
     cp.clear();
-    std::vector<SVDJob> sj;
     std::vector<int64_t> pdim(nb, 0);
     for (size_t q = 0; q < nb; ++q) {
       if (jcore[q] < 0) continue;
@@ -444,16 +446,57 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       const bool transpose = left ? !t : t;
       if (transpose) cp.push_back(copy_job(s.Mt, 1, ldm, s.B, m, 1, int(m), int(m), 1.0, 1));
       else cp.push_back(copy_job(s.Mt, 1, ldm, s.B, 1, m, int(m), int(m), 1.0, 1));
-      sj.push_back(SVDJob{s.B, m, s.J, m, s.sv, int(m), s.info});
     }
     copy2d_batched(ctx, cp);
-    zero_fill(ctx, info_all, sizeof(int32_t) * 8 * nb);
-    const double wc = jacobi_svd_batched(ctx, sj);
+    // Bonds whose floor rule provably keeps every singular value (Cholesky
+    // certificate on B^T B, written into J's buffer) need no SVD: the QR/LQ factor
+    // already is an orthonormal basis of the kept space, and B is absorbed as it is.
+    // Only bonds with m <= rmax qualify (rmax truncates the others).
+    std::vector<char> cert(nb, 0);
+    if (ctx.svd_certify) {
+      std::vector<CholJob> cj;
+      for (size_t q = 0; q < nb; ++q) {
+        if (jcore[q] < 0) continue;
+        const int64_t m = pdim[q];
+        if (m > rmax || m > CHOL_MAX_M) continue;
+        SweepState& s = st[q];
+        gb.add(int(m), int(m), int(m), 1.0, s.B, m, 1, s.B, 1, m, 0.0, s.J, 1, m);
+        cj.push_back(CholJob{s.J, m, int(m), tol, cflags + q});
+      }
+      if (!cj.empty()) {
+        gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "svd_cert_gram");
+        zero_fill(ctx, cflags, sizeof(int32_t) * nb);
+        chol_certify_batched(ctx, cj);
+        TTR_HIP(hipMemcpyAsync(hcflags, cflags, sizeof(int32_t) * nb, hipMemcpyDeviceToHost, ctx.stream));
+        ctx.sync();
+        int64_t ncert = 0;
+        for (size_t q = 0; q < nb; ++q) {
+          cert[q] = jcore[q] >= 0 && hcflags[q] == 1;
+          ncert += cert[q];
+        }
+        if (ctx.prof.on) {
+          ProfStat& ps = ctx.prof.stats["svd_certify:bonds"];
+          ps.launches += int64_t(cj.size());
+          ps.flops += double(ncert);
+        }
+      }
+    }
+    std::vector<SVDJob> sj;
+    for (size_t q = 0; q < nb; ++q) {
+      if (jcore[q] < 0 || cert[q]) continue;
+      SweepState& s = st[q];
+      sj.push_back(SVDJob{s.B, pdim[q], s.J, pdim[q], s.sv, int(pdim[q]), s.info});
+    }
+    double wc = 64.0;
+    if (!sj.empty()) {
+      zero_fill(ctx, info_all, sizeof(int32_t) * 8 * nb);
+      wc = jacobi_svd_batched(ctx, sj);
+    }
     // rank decisions on the host
     ctx.sync();
-    TTR_HIP(hipMemcpy(sv_host.data(), sv_all, sv_host.size() * 8, hipMemcpyDeviceToHost));
+    if (!sj.empty()) TTR_HIP(hipMemcpy(sv_host.data(), sv_all, sv_host.size() * 8, hipMemcpyDeviceToHost));
     std::vector<uint32_t> info_host;
-    if (ctx.prof.on) {
+    if (ctx.prof.on && !sj.empty()) {
       info_host.resize(8 * nb);
       TTR_HIP(hipMemcpy(info_host.data(), info_all, info_host.size() * 4, hipMemcpyDeviceToHost));
     }
@@ -463,6 +506,12 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     for (size_t q = 0; q < nb; ++q) {
       if (jcore[q] < 0) continue;
       const int64_t m = pdim[q];
+      if (cert[q]) {  // every column kept, in place
+        K[q] = m;
+        order_off[q] = orders.size();
+        for (int64_t i = 0; i < m; ++i) orders.push_back(int32_t(i));
+        continue;
+      }
       const double* sv = sv_host.data() + sv_off[q];
       if (ctx.prof.on) {
         const uint32_t* info = info_host.data() + 8 * q;
@@ -503,25 +552,23 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       orders.insert(orders.end(), order.begin(), order.begin() + k);
     }
     const int32_t* dorders = h->ctx.upload(orders);
-    std::vector<GatherCols> gc;
+    // reorder columns of B*J and J in place order (into US / Mt scratch); the
+    // second gather must not overwrite B before the first reads it: two launches
+    std::vector<GatherCols> g1, g2;
+    std::vector<Fill2D> eye;
     for (size_t q = 0; q < nb; ++q) {
       if (jcore[q] < 0) continue;
       SweepState& s = st[q];
       const int64_t m = pdim[q], k = K[q];
       int32_t* didx = const_cast<int32_t*>(dorders) + order_off[q];
-      // reorder columns of B*J and J in place order (into US / Mt scratch)
-      gc.push_back(GatherCols{s.B, s.US, m, m, didx, int(m), int(k)});
-      gc.push_back(GatherCols{s.J, s.B, m, m, didx, int(m), int(k)});
-    }
-    // second gather must not overwrite B before the first reads it: run them in two launches
-    std::vector<GatherCols> g1, g2;
-    for (size_t i = 0; i < gc.size(); i += 2) {
-      g1.push_back(gc[i]);
-      g2.push_back(gc[i + 1]);
+      g1.push_back(GatherCols{s.B, s.US, m, m, didx, int(m), int(k)});
+      if (cert[q]) eye.push_back(Fill2D{s.B, m, int(m), int(m), 1.0});  // J = I
+      else g2.push_back(GatherCols{s.J, s.B, m, m, didx, int(m), int(k)});
     }
     gather_cols_batched(ctx, g1);  // US <- (B J)[:, sel]        (m x k)
     // J[:, sel] goes to a fresh buffer (B is free now)
     gather_cols_batched(ctx, g2);  // B  <- J[:, sel]            (m x k)
+    fill2d_batched(ctx, eye);
     // build the new cores
     std::vector<Copy2D> cpo;
     for (size_t q = 0; q < nb; ++q) {
@@ -540,7 +587,8 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
         double* dst = s.ping[s.pp];
         if (!t) {
           // M = (B J)(Q J)^T: core_j = (Q Jk)^T, core_{j-1} <- core_{j-1} (B J)_k
-          gb.add(int(k), int(qq), int(m), 1.0, Jk, m, 1, s.Qb, qq, 1, 0.0, newj, 1, k);
+          if (cert[q]) cpo.push_back(copy_job(s.Qb, qq, 1, newj, 1, k, int(k), int(qq)));  // J = I
+          else gb.add(int(k), int(qq), int(m), 1.0, Jk, m, 1, s.Qb, qq, 1, 0.0, newj, 1, k);
           gb.add(int(rl), int(k), int(p), 1.0, s.cur[jl], 1, rl, BJk, 1, m, 0.0, dst, 1, rl);
         } else {
           // M = Q (R J) J^T: core_j = Jk^T, core_{j-1} <- core_{j-1} (Q (R J)_k)
@@ -557,7 +605,8 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
         double* dst = s.ping[s.pp];
         if (t) {
           // M = Q J (R^T J)^T... : U = Q Jk, S V^T = (B J)_k^T
-          gb.add(int(p), int(k), int(qq), 1.0, s.Qb, 1, p, Jk, 1, m, 0.0, newj, 1, p);
+          if (cert[q]) cpo.push_back(copy_job(s.Qb, 1, p, newj, 1, p, int(p), int(k)));  // J = I
+          else gb.add(int(p), int(k), int(qq), 1.0, s.Qb, 1, p, Jk, 1, m, 0.0, newj, 1, p);
           gb.add(int(k), int(nr), int(qq), 1.0, BJk, m, 1, s.cur[jr], 1, qq, 0.0, dst, 1, k);
         } else {
           // M = J (Q (R J))^T: U = Jk, S V^T = (Q (B J)_k)^T
@@ -1522,6 +1571,20 @@ int ttr_create(ttr_handle_t* out, int device) {
     // TTR_GEMM_TUNE=0: GEMM tiles by size only (no timing of the candidates on first use)
     const char* tune = std::getenv("TTR_GEMM_TUNE");
     h->ctx.gemm_autotune = !(tune && tune[0] == '0');
+    // TTR_GEMM_CFG=c (0..7): every GEMM uses tile configuration c (tests of each configuration)
+    if (const char* c = std::getenv("TTR_GEMM_CFG")) {
+      const int v = std::atoi(c);
+      TTR_REQUIRE(v >= 0 && v < 8, ST_INVALID_ARG, "TTR_GEMM_CFG must be 0..7");
+      h->ctx.gemm_force_cfg = v;
+    }
+    // TTR_SVD_CERTIFY=0: run the SVD at every bond (no Cholesky certificate)
+    if (const char* c = std::getenv("TTR_SVD_CERTIFY")) h->ctx.svd_certify = c[0] != '0';
+    // TTR_JACOBI_MAX_SWEEPS=s (1..40): Jacobi sweeps before a non-converged SVD fails
+    if (const char* c = std::getenv("TTR_JACOBI_MAX_SWEEPS")) {
+      const int v = std::atoi(c);
+      TTR_REQUIRE(v >= 1 && v <= 40, ST_INVALID_ARG, "TTR_JACOBI_MAX_SWEEPS must be 1..40");
+      h->ctx.jacobi_max_sweeps = v;
+    }
   } catch (const Error& e) {
     int st = e.status;
     if (h->ctx.stream) (void)hipStreamDestroy(h->ctx.stream);

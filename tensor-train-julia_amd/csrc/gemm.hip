@@ -453,7 +453,8 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
           std::to_string(int(tc)) + std::to_string(int(math)) + "|" + std::to_string(ilog2(am)) + "," +
           std::to_string(ilog2(an)) + "," + std::to_string(ilog2(ak)) + "," + std::to_string(ilog2(np));
     auto it = ctx.gemm_cfg.find(key);
-    if (it != ctx.gemm_cfg.end()) cfg = it->second;
+    if (ctx.gemm_force_cfg >= 0) cfg = ctx.gemm_force_cfg;
+    else if (it != ctx.gemm_cfg.end()) cfg = it->second;
     else if (ctx.gemm_autotune && beta0) tune = true;
   } else {
     cfg = 0;

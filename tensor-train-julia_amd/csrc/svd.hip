@@ -597,7 +597,9 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   int32_t* hflags = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * nj));
   fill2d_batched(ctx, fills);
   zero_fill(ctx, flags, sizeof(int32_t) * 2 * nj);
-  for (int sweep = 0; sweep < MAX_SWEEPS; ++sweep) {
+  const int max_sweeps = std::max(1, std::min(MAX_SWEEPS, ctx.jacobi_max_sweeps));
+  bool converged = false;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     int32_t* cur = flags + size_t(sweep & 1) * nj;
     if (sweep >= 2) zero_fill(ctx, cur, sizeof(int32_t) * nj);
     for (int r = 0; r < max_rounds; ++r) {
@@ -608,15 +610,24 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
                          sweep, flags);
       TTR_HIP(hipGetLastError());
     }
-    // convergence check from the third sweep on (no matrix converges earlier in practice)
-    if (sweep >= 2) {
+    // convergence check from the third sweep on (no matrix converges earlier in
+    // practice) and after the last sweep allowed
+    if (sweep >= 2 || sweep == max_sweeps - 1) {
       TTR_HIP(hipMemcpyAsync(hflags, cur, sizeof(int32_t) * nj, hipMemcpyDeviceToHost, ctx.stream));
       ctx.sync();
       bool any = false;
       for (int q = 0; q < nj && !any; ++q) any = hflags[q] != 0;
-      if (!any) break;
+      if (!any) {
+        converged = true;
+        break;
+      }
     }
   }
+  // a matrix that still rotated in the last sweep has no converged SVD: report it
+  // (the reference's LAPACK driver either converges or throws, src/tt_rounding.jl:191-195)
+  if (!converged)
+    throw Error(ST_NUMERICAL_FAILURE,
+                "Jacobi SVD did not converge in " + std::to_string(max_sweeps) + " sweeps");
   hipLaunchKernelGGL(column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, ctx.stream, dl);
   TTR_HIP(hipGetLastError());
 }

@@ -147,6 +147,13 @@ struct Ctx {
   // GEMM tile choice per (call site, precisions, size class), measured on first use
   std::unordered_map<std::string, int> gemm_cfg;
   bool gemm_autotune = true;
+  // GEMM tile configuration forced for every call (-1: choose per site)
+  int gemm_force_cfg = -1;
+  // skip the SVD of a bond whose singular values a Cholesky certificate shows
+  // the floor rule keeps (see chol_certify_batched)
+  bool svd_certify = true;
+  // Jacobi sweeps after which a matrix that still rotates is a numerical failure
+  int jacobi_max_sweeps = 40;
   // partial sums of split-K GEMMs, reused by every launch on the stream
   double* splitk = nullptr;
   size_t splitk_elems = 0;
@@ -267,6 +274,20 @@ struct SVDJob {
 };
 // returns the number of columns per visit (for the work counters)
 int jacobi_svd_batched(Ctx& ctx, std::vector<SVDJob>& jobs);
+// Certificate that a truncated SVD of B keeps all its singular values: attempts
+// the Cholesky factorisation of G - tau' I for the Gram G = B^T B (m x m, ld,
+// overwritten), tau' = tol^2 trace(G) (1 + 2^-6) + 8 (m + 1) 2^-53 trace(G).
+// *flag = 1 when it succeeds (every singular value s of B has
+// s^2 > tol^2 sum s^2 with margin), 0 otherwise (nothing is decided).
+constexpr int CHOL_MAX_M = 1024;
+struct CholJob {
+  double* G;
+  int64_t ld;
+  int m;
+  double tol;
+  int32_t* flag;  // device
+};
+void chol_certify_batched(Ctx& ctx, const std::vector<CholJob>& jobs);
 // pseudo-inverse weights after a Jacobi SVD: column c of BJ (p x p, ld) is scaled
 // by 1/sv_c^2 when sv_c > eps * max(sv), else zeroed
 struct PinvWeights {

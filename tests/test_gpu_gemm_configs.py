@@ -65,3 +65,23 @@ def test_gemm_configs_bitwise(monkeypatch):
     _same(first, plain, "tuned vs 64x64/k16 tiles")
     tuned.close()
     fixed.close()
+
+
+@pytest.mark.parametrize("cfg", range(1, 8))
+def test_each_gemm_config_bitwise(monkeypatch, cfg):
+    """TTR_GEMM_CFG forces one tile configuration for every GEMM of a handle
+    (split-K batches and the unit-trapezoidal operand modes of the Householder
+    updates included): each must give the bits of configuration 0."""
+    import ttround as T
+    from ttround import _lib as L
+
+    say(f"case: GEMM tile configuration {cfg} forced vs configuration 0")
+    runs = {}
+    for c in (0, cfg):
+        monkeypatch.setenv("TTR_GEMM_CFG", str(c))
+        h = L.Handle(0)
+        monkeypatch.setattr(L, "_default", h)
+        runs[c] = _workload(T)
+        h.close()
+    monkeypatch.delenv("TTR_GEMM_CFG")
+    _same(runs[0], runs[cfg], f"configuration {cfg} vs 0")
