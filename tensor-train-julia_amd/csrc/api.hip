@@ -333,6 +333,7 @@ struct SweepState {
   double* ping[2];
   int pp = 0;
   double *Mt, *Qb, *B, *J, *sv, *US;
+  double *Jh, *USh;  // re-orthonormalised kept rotations and the matching (B J) columns
   int32_t* idx;
   int32_t* info;
   bool zero = false;
@@ -370,6 +371,7 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
   double* sv_all = ctx.ws.alloc_n<double>(size_t(sv_off[nb]));
   std::vector<double> sv_host(static_cast<size_t>(sv_off[nb]));
   for (size_t q = 0; q < nb; ++q) st[q].sv = sv_all + sv_off[q];
+  int64_t mxp_all = 1;
   for (auto& s : st) {
     s.cur = s.y->cores;
     s.rk = s.y->rks;
@@ -386,9 +388,15 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     s.US = ctx.ws.alloc_n<double>(size_t(mx));
     s.B = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
     s.J = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
+    s.Jh = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
+    s.USh = ctx.ws.alloc_n<double>(size_t(mxp * mxp));
+    mxp_all = std::max(mxp_all, mxp);
     s.idx = ctx.ws.alloc_n<int32_t>(size_t(mxp));
     s.out = h->make_tt_lazy(s.y->dims);
   }
+  // an identity matrix (ld mxp_all) whose leading k x k block serves every item
+  double* eyeK = ctx.ws.alloc_n<double>(size_t(mxp_all * mxp_all));
+  fill2d_batched(ctx, {Fill2D{eyeK, mxp_all, int(mxp_all), int(mxp_all), 1.0}});
   // Jacobi counters of all items in one array (zeroed before every SVD step)
   int32_t* info_all = ctx.ws.alloc_n<int32_t>(size_t(8) * nb);
   // certificate flags of all items (device and host copies)
@@ -569,6 +577,31 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     // J[:, sel] goes to a fresh buffer (B is free now)
     gather_cols_batched(ctx, g2);  // B  <- J[:, sel]            (m x k)
     fill2d_batched(ctx, eye);
+    // The accumulated Jacobi rotations drift from orthogonality by about 64 eps per
+    // applied visit (tens of visits per column); across a sweep of truncating bonds
+    // that drift, not the SVD's accuracy, dominates the error.  One Newton-Schulz
+    // step C = (3 I - J_k^T J_k) / 2 makes Jh = J_k C orthonormal to rounding (the
+    // deviation is squared), and the same C on (B J)_k keeps the represented
+    // product (B J_k C)(J_k C)^T = B Jh Jh^T an orthogonal projection of B.
+    for (size_t q = 0; q < nb; ++q) {
+      if (jcore[q] < 0 || cert[q]) continue;
+      SweepState& s = st[q];
+      const int64_t m = pdim[q], k = K[q];
+      gb.add(int(k), int(k), int(m), 1.0, s.B, m, 1, s.B, 1, m, 0.0, s.J, 1, k);  // G = J_k^T J_k
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "svd_orth_gram");
+    for (size_t q = 0; q < nb; ++q) {
+      if (jcore[q] < 0 || cert[q]) continue;
+      SweepState& s = st[q];
+      const int64_t m = pdim[q], k = K[q];
+      for (int w = 0; w < 2; ++w) {  // Jh = J_k C, USh = (B J)_k C
+        const double* src = w == 0 ? s.B : s.US;
+        gb.prob(w == 0 ? s.Jh : s.USh, 1, m, int(m), int(k), 1.0, 0.0);
+        gb.seg(src, 1, m, eyeK, 1, mxp_all, int(k), 1.5);
+        gb.seg(src, 1, m, s.J, 1, k, int(k), -0.5);
+      }
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "svd_orth_fix");
     // build the new cores
     std::vector<Copy2D> cpo;
     for (size_t q = 0; q < nb; ++q) {
@@ -577,8 +610,9 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       const int j = jcore[q];
       const int64_t p = P[q], qq = Qd[q], k = K[q], m = pdim[q];
       const bool t = tallcase[q];
-      const double* BJk = s.US;  // (m x k): columns u_c s_c of the square factor
-      const double* Jk = s.B;    // (m x k): right rotations
+      // (m x k): columns u_c s_c of the square factor, and the right rotations
+      const double* BJk = cert[q] ? s.US : s.USh;
+      const double* Jk = cert[q] ? s.B : s.Jh;
       if (left) {
         const int64_t n = s.y->dims[j];
         double* newj = h->core_alloc(s.out.get(), k * qq);
@@ -1579,6 +1613,12 @@ int ttr_create(ttr_handle_t* out, int device) {
     }
     // TTR_SVD_CERTIFY=0: run the SVD at every bond (no Cholesky certificate)
     if (const char* c = std::getenv("TTR_SVD_CERTIFY")) h->ctx.svd_certify = c[0] != '0';
+    // TTR_JACOBI_TOL_SCALE=f (1e-3..1): scales the Jacobi rotation threshold max(p, 16) eps
+    if (const char* c = std::getenv("TTR_JACOBI_TOL_SCALE")) {
+      const double v = std::atof(c);
+      TTR_REQUIRE(v >= 1e-3 && v <= 1.0, ST_INVALID_ARG, "TTR_JACOBI_TOL_SCALE must be in [1e-3, 1]");
+      h->ctx.jacobi_tol_scale = v;
+    }
     // TTR_JACOBI_MAX_SWEEPS=s (1..40): Jacobi sweeps before a non-converged SVD fails
     if (const char* c = std::getenv("TTR_JACOBI_MAX_SWEEPS")) {
       const int v = std::atoi(c);

@@ -66,6 +66,7 @@ struct JacJob {
   double* J;
   int64_t ldj;
   int32_t* info;
+  double tol;  // rotate a pair when |G_ij| > tol sqrt(G_ii G_jj)
   int p, nblk, pp, rounds, slot0;
 };
 
@@ -427,8 +428,7 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
     for (int e = tid; e < WC * WC; e += JT) G[e % WC][e / WC] *= sc;
   }
   __syncthreads();
-  const double tol = double(p > 16 ? p : 16) * 2.220446049250313e-16;
-  const double tol2 = tol * tol;
+  const double tol2 = S.tol * S.tol;
   // a visit whose columns are already pairwise orthogonal changes nothing (no
   // round would rotate): skip the inner sweep and the apply
   int need = 0;
@@ -583,6 +583,7 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
     const SVDJob& s = live[q];
     JacJob& J = jj[q];
     J.B = s.B; J.ldb = s.ldb; J.J = s.J; J.ldj = s.ldj; J.info = s.info; J.p = s.p;
+    J.tol = double(std::max(s.p, 16)) * 2.220446049250313e-16 * ctx.jacobi_tol_scale;
     J.nblk = (s.p + WB - 1) / WB;
     J.pp = J.nblk + (J.nblk & 1);
     J.rounds = J.nblk == 1 ? 1 : J.pp - 1;

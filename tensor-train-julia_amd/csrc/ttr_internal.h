@@ -154,6 +154,8 @@ struct Ctx {
   bool svd_certify = true;
   // Jacobi sweeps after which a matrix that still rotates is a numerical failure
   int jacobi_max_sweeps = 40;
+  // Jacobi rotation threshold: |G_ij| > scale * max(p, 16) eps sqrt(G_ii G_jj)
+  double jacobi_tol_scale = 1.0;
   // partial sums of split-K GEMMs, reused by every launch on the stream
   double* splitk = nullptr;
   size_t splitk_elems = 0;
