@@ -140,6 +140,20 @@ int ttr_norm(ttr_handle_t h, int64_t n, const int64_t* ids, double* out);
  * (src/tt_solvers.jl:69,90,279-280,315). */
 int ttr_dot(ttr_handle_t h, int64_t n, const int64_t* x_ids, const int64_t* y_ids, double* out);
 
+/* Same results as ttr_dot, computed without allocating anything: no hipMalloc,
+ * hipFree, pinned-memory allocation or allocating library call, on the first
+ * call as on every later one.  The environments, temporaries, split-K partial
+ * sums and GEMM descriptors go into a static device array of 64 MiB (one per GPU,
+ * held until the process ends) and a pinned host staging buffer of 4 MiB that
+ * ttr_create allocates; a call whose environments need more than 32 MiB
+ * (2 r_x r_y + r_x n r_y doubles per pair at the largest bond) fails with
+ * TTR_INVALID_ARG.  Calls on one GPU share that array: calls that overlap on
+ * one GPU (handles used concurrently from several host threads) race.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight solver or serving loops such as
+ * the Gram-Schmidt passes of src/tt_solvers.jl:69,90,279-280,315). */
+int ttr_dot_noalloc(ttr_handle_t h, int64_t n, const int64_t* x_ids, const int64_t* y_ids, double* out);
+
 /* All partial contractions of x and y: W[k] (r^x_k x r^y_k, column-major,
  * k = 0..d) into W_out[k] (NULL entries skipped); reverse != 0 contracts the
  * cores k..d-1 into W[k] (W[0] = <x, y>, W[d] = 1), reverse == 0 the cores

@@ -5,6 +5,7 @@
 // panels, Jacobi, copies) on the handle's stream.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <numeric>
 
 #include "../../include/ttround.h"
@@ -14,7 +15,35 @@
 
 using namespace ttr;
 
+// Device scratch of the entry points that allocate nothing (ttr_dot_noalloc):
+// environments, temporaries, split-K partial sums and GEMM descriptors of one
+// call.  Each GPU the library runs on holds its own copy until the process
+// ends; every such call on a GPU uses that GPU's copy, so calls that overlap on
+// one GPU (two handles on two host threads) race.  Host code never passes the
+// array's name to a kernel: it uses the address hipGetSymbolAddress returned
+// for the device (scratch_base), taken once per device in ttr_create.
+constexpr size_t TTR_SCRATCH_BYTES = size_t(64) << 20;
+static __device__ __align__(256) char scratch[TTR_SCRATCH_BYTES];
+// pinned host staging of the descriptors, allocated per handle in ttr_create
+constexpr size_t TTR_STAGE_BYTES = size_t(4) << 20;
+
 namespace {
+
+std::mutex scratch_mu;
+std::vector<void*> scratch_base;  // per device: the address of `scratch` there
+
+// the address of this device's scratch (the device must be current)
+void* scratch_for_device(int device) {
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (scratch_base.size() <= size_t(device)) scratch_base.resize(size_t(device) + 1, nullptr);
+  if (!scratch_base[size_t(device)]) {
+    void* p = nullptr;
+    TTR_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TTR_REQUIRE(p != nullptr, ST_DEVICE_ERROR, "hipGetSymbolAddress returned no address for the scratch array");
+    scratch_base[size_t(device)] = p;
+  }
+  return scratch_base[size_t(device)];
+}
 
 struct DevTT {
   int d = 0;
@@ -108,6 +137,8 @@ struct ttr_handle_s {
   std::unordered_map<int64_t, std::unique_ptr<DevTT>> tts;
   std::unordered_map<int64_t, std::unique_ptr<DevTTO>> ops;
   double* one = nullptr;  // device constant 1.0
+  void* scratch = nullptr;  // this device's scratch array (set in ttr_create)
+  void* stage = nullptr;    // pinned host staging of the non-allocating entry points
 
   DevTT* get(int64_t id) {
     auto it = tts.find(id);
@@ -1600,6 +1631,8 @@ int ttr_create(ttr_handle_t* out, int device) {
     h->ctx.device = device;
     TTR_HIP(hipStreamCreateWithFlags(&h->ctx.stream, hipStreamNonBlocking));
     TTR_HIP(hipMalloc(&h->one, 64));
+    h->scratch = scratch_for_device(device);
+    TTR_HIP(hipHostMalloc(&h->stage, TTR_STAGE_BYTES, hipHostMallocDefault));
     const double one = 1.0;
     TTR_HIP(hipMemcpy(h->one, &one, 8, hipMemcpyHostToDevice));
     // TTR_GEMM_TUNE=0: GEMM tiles by size only (no timing of the candidates on first use)
@@ -1643,6 +1676,7 @@ int ttr_destroy(ttr_handle_t h) {
   h->tts.clear();
   h->ops.clear();  // DevTTO frees its device copies
   (void)hipFree(h->one);
+  if (h->stage) (void)hipHostFree(h->stage);
   if (h->ctx.splitk) (void)hipFree(h->ctx.splitk);
   h->ctx.ws.release();
   h->ctx.pin.release();
@@ -2146,6 +2180,69 @@ int ttr_dot(ttr_handle_t h, int64_t n, const int64_t* x_ids, const int64_t* y_id
       const DevTT* y = ys.back();
       W[i].assign(size_t(x->d + 1), nullptr);
       for (int k = 1; k < x->d; ++k) W[i][size_t(k)] = h->ctx.ws.alloc_n<double>(size_t(x->rks[k] * y->rks[k]));
+      W[i][size_t(x->d)] = res + i;
+    }
+    pair_environments(h, xs, ys, false, W);
+    h->ctx.sync();
+    TTR_HIP(hipMemcpy(out, res, size_t(n) * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+int ttr_dot_noalloc(ttr_handle_t h, int64_t n, const int64_t* x_ids, const int64_t* y_ids, double* out) {
+  // the workspace and staging arenas point at the scratch array and the handle's
+  // pinned staging for this call only
+  struct FixedScope {
+    Ctx& c;
+    FixedScope(Ctx& ctx, void* dev, size_t dcap, void* host, size_t hcap) : c(ctx) {
+      c.ws.set_fixed(dev, dcap);
+      c.pin.set_fixed(host, hcap);
+      c.fsplit = nullptr;
+      c.fsplit_elems = 0;
+    }
+    ~FixedScope() {
+      (void)hipStreamSynchronize(c.stream);  // nothing may still read the scratch
+      c.ws.clear_fixed();
+      c.pin.clear_fixed();
+      c.fsplit = nullptr;
+      c.fsplit_elems = 0;
+    }
+  };
+  return guarded(h, [&] {
+    TTR_REQUIRE(n >= 1 && x_ids && y_ids && out, ST_INVALID_ARG, "bad arguments");
+    TTR_REQUIRE(h->scratch && h->stage, ST_DEVICE_ERROR, "the handle has no scratch");
+    std::vector<DevTT*> xs, ys;
+    // environments: two per pair (W_k and W_{k+1}, alternating), the product
+    // temporary and the result; 256-byte granules as the arena hands them out
+    auto gran = [](int64_t elems) { return (size_t(std::max<int64_t>(elems, 1)) * 8 + 255) & ~size_t(255); };
+    size_t need = gran(n);
+    for (int64_t i = 0; i < n; ++i) {
+      xs.push_back(h->get(x_ids[i]));
+      ys.push_back(h->get(y_ids[i]));
+      TTR_REQUIRE(xs.back()->dims == ys.back()->dims, ST_DIM_MISMATCH, "TT dimensions are not compatible");
+      const DevTT* x = xs.back();
+      const DevTT* y = ys.back();
+      int64_t mw = 1, mt = 1;
+      for (int k = 0; k < x->d; ++k) {
+        mw = std::max(mw, x->rks[k] * y->rks[k]);
+        mt = std::max(mt, x->rks[k] * x->dims[k] * y->rks[k + 1]);
+      }
+      need += 2 * gran(mw) + gran(mt);
+    }
+    TTR_REQUIRE(need <= TTR_SCRATCH_BYTES / 2, ST_INVALID_ARG,
+                "ttr_dot_noalloc: the environments need " + std::to_string(need >> 20) + " MiB; the limit is " +
+                    std::to_string(TTR_SCRATCH_BYTES >> 21) + " MiB of the " + std::to_string(TTR_SCRATCH_BYTES >> 20) +
+                    " MiB device scratch (the rest holds descriptors and split-K sums)");
+    FixedScope fs(h->ctx, h->scratch, TTR_SCRATCH_BYTES, h->stage, TTR_STAGE_BYTES);
+    double* res = h->ctx.ws.alloc_n<double>(size_t(n));
+    std::vector<std::vector<double*>> W(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) {
+      const DevTT* x = xs[i];
+      const DevTT* y = ys[i];
+      int64_t mw = 1;
+      for (int k = 0; k < x->d; ++k) mw = std::max(mw, x->rks[k] * y->rks[k]);
+      double* pp[2] = {h->ctx.ws.alloc_n<double>(size_t(mw)), h->ctx.ws.alloc_n<double>(size_t(mw))};
+      W[i].assign(size_t(x->d + 1), nullptr);
+      for (int k = 1; k < x->d; ++k) W[i][size_t(k)] = pp[k & 1];  // step k reads W_k, writes W_{k+1}
       W[i][size_t(x->d)] = res + i;
     }
     pair_environments(h, xs, ys, false, W);

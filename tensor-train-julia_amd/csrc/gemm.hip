@@ -455,7 +455,7 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     auto it = ctx.gemm_cfg.find(key);
     if (ctx.gemm_force_cfg >= 0) cfg = ctx.gemm_force_cfg;
     else if (it != ctx.gemm_cfg.end()) cfg = it->second;
-    else if (ctx.gemm_autotune && beta0) tune = true;
+    else if (ctx.gemm_autotune && beta0 && !ctx.ws.fixed()) tune = true;
   } else {
     cfg = 0;
   }
@@ -489,6 +489,10 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
             for (auto& p : probs) p.ksplit = 1;
           else if (elems > 0) {
             double* base = ctx.splitk_buffer(size_t(elems));
+            if (!base) {  // no room in a fixed workspace: K is not split
+              for (auto& p : probs) p.ksplit = 1;
+              break;
+            }
             int64_t off = 0;
             for (size_t i = 0; i < probs.size(); ++i) {
               auto& p = probs[i];

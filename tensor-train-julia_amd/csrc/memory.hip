@@ -13,6 +13,14 @@ DevArena::~DevArena() { release(); }
 
 void* DevArena::alloc(size_t bytes) {
   bytes = align_up(bytes ? bytes : kAlign);
+  if (fixed_) {
+    TTR_REQUIRE(fixed_used_ + bytes <= fixed_cap_, ST_INVALID_ARG,
+                "the call needs more than the " + std::to_string(fixed_cap_ >> 20) +
+                    " MiB of device scratch of the non-allocating entry points");
+    void* p = fixed_ + fixed_used_;
+    fixed_used_ += bytes;
+    return p;
+  }
   if (!blocks_.empty()) {
     Block& b = blocks_.back();
     if (b.used + bytes <= b.cap) {
@@ -30,6 +38,10 @@ void* DevArena::alloc(size_t bytes) {
 }
 
 void DevArena::reset() {
+  if (fixed_) {
+    fixed_used_ = 0;
+    return;
+  }
   if (blocks_.size() > 1) {
     size_t total = 0;
     for (auto& b : blocks_) total += b.cap;
@@ -51,6 +63,14 @@ PinnedArena::~PinnedArena() { release(); }
 
 void* PinnedArena::alloc(size_t bytes) {
   bytes = align_up(bytes ? bytes : kAlign);
+  if (fixed_) {
+    TTR_REQUIRE(fixed_used_ + bytes <= fixed_cap_, ST_INVALID_ARG,
+                "the call needs more than the " + std::to_string(fixed_cap_ >> 20) +
+                    " MiB of host staging of the non-allocating entry points");
+    void* p = fixed_ + fixed_used_;
+    fixed_used_ += bytes;
+    return p;
+  }
   if (!blocks_.empty()) {
     Block& b = blocks_.back();
     if (b.used + bytes <= b.cap) {
@@ -68,6 +88,10 @@ void* PinnedArena::alloc(size_t bytes) {
 }
 
 void PinnedArena::reset() {
+  if (fixed_) {
+    fixed_used_ = 0;
+    return;
+  }
   if (blocks_.size() > 1) {
     size_t total = 0;
     for (auto& b : blocks_) total += b.cap;
@@ -204,6 +228,14 @@ void Profiler::collect() {
 namespace ttr {
 
 double* Ctx::splitk_buffer(size_t elems) {
+  if (ws.fixed()) {
+    // one region reused by every launch of the call (they run in stream order)
+    if (elems <= fsplit_elems) return fsplit;
+    if (elems * sizeof(double) + 256 > ws.fixed_free()) return nullptr;
+    fsplit = ws.alloc_n<double>(elems);
+    fsplit_elems = elems;
+    return fsplit;
+  }
   if (elems > splitk_elems) {
     // the previous buffer may still be read by queued launches
     sync();

@@ -62,6 +62,9 @@ inline size_t prec_bytes(Prec p) { return p == Prec::F64 ? 8 : (p == Prec::F32 ?
 // Bump arena of device memory, reset after every library call.  When one call
 // needs more than the current block, further blocks are added and merged into
 // a single block on reset, so steady-state calls allocate nothing.
+// With a fixed region set (set_fixed), allocations come from that region only
+// and a request beyond it throws: nothing is allocated (the entry points that
+// must not allocate run this way).
 class DevArena {
  public:
   ~DevArena();
@@ -70,10 +73,16 @@ class DevArena {
   T* alloc_n(size_t n) { return static_cast<T*>(alloc(n * sizeof(T))); }
   void reset();  // caller must have synchronised the stream
   void release();
+  void set_fixed(void* base, size_t cap) { fixed_ = static_cast<char*>(base); fixed_cap_ = cap; fixed_used_ = 0; }
+  void clear_fixed() { fixed_ = nullptr; fixed_cap_ = fixed_used_ = 0; }
+  bool fixed() const { return fixed_ != nullptr; }
+  size_t fixed_free() const { return fixed_cap_ - fixed_used_; }
 
  private:
   struct Block { char* p; size_t cap; size_t used; };
   std::vector<Block> blocks_;
+  char* fixed_ = nullptr;
+  size_t fixed_cap_ = 0, fixed_used_ = 0;
 };
 
 // Same scheme over pinned host memory (staging for descriptor arrays).
@@ -83,10 +92,14 @@ class PinnedArena {
   void* alloc(size_t bytes);
   void reset();
   void release();
+  void set_fixed(void* base, size_t cap) { fixed_ = static_cast<char*>(base); fixed_cap_ = cap; fixed_used_ = 0; }
+  void clear_fixed() { fixed_ = nullptr; fixed_cap_ = fixed_used_ = 0; }
 
  private:
   struct Block { char* p; size_t cap; size_t used; };
   std::vector<Block> blocks_;
+  char* fixed_ = nullptr;
+  size_t fixed_cap_ = 0, fixed_used_ = 0;
 };
 
 // Caching allocator for long-lived objects (TT storage): frees go to per-size
@@ -159,6 +172,10 @@ struct Ctx {
   // partial sums of split-K GEMMs, reused by every launch on the stream
   double* splitk = nullptr;
   size_t splitk_elems = 0;
+  double* fsplit = nullptr;  // the same inside the fixed workspace region
+  size_t fsplit_elems = 0;
+  // with the workspace in fixed mode: taken from it, or nullptr when it does
+  // not fit (the GEMM then runs without splitting K)
   double* splitk_buffer(size_t elems);
   // copies a host array into fresh device workspace (asynchronously, through pinned memory)
   template <class T>
