@@ -188,6 +188,21 @@ typedef struct {
   double oversampling;  /* block-count multiplier (kwarg oversampling; 1 = none)              */
   int32_t forward;      /* 0: right-to-left sketch (kwarg reverse=true, the default); 1: the  */
                         /* left-to-right sketch (reverse=false; ttr_sketch, ttr_sketch_blocks) */
+  /* Optional caller-supplied sketch blocks, or NULL for the library's draws of `seed`.
+   * blocks[k] (k = 0..d-1) holds core k's blocks exactly as the reference's
+   * generate_sketch_blocks returns them (src/sketches.jl:44-84, already scaled or
+   * orthogonalised), column-major: (bv_{k+1} n_k) x bv_k x p_k for the
+   * right-to-left sketch, (bv_k n_k) x bv_{k+1} x p_{k+1} for the left-to-right
+   * one (src/sketches.jl:232-237, 279-281), FP64 (cast to sketch_prec on the
+   * device).  A Julia caller passes the arrays that seeding default_rng with
+   * `seed` and calling generate_sketch_blocks core by core in the reference's
+   * order yields, so the rounding uses the reference's own draws; `orthogonal`
+   * and `seed` are then not used for the blocks.  Every item of a batch must
+   * have the same block counts.  Read during the call only. */
+  const double* const* blocks;
+  /* ttr_stta / ttr_stta_sketch only: the right-to-left sketch's blocks (seed_right);
+   * `blocks` then holds the left-to-right sketch's.  NULL elsewhere. */
+  const double* const* blocks_right;
 } ttr_rand_opts;
 
 /* Randomize-then-orthogonalize rounding of n linear combinations in one

@@ -747,6 +747,7 @@ struct SketchPlan {
   Prec prec = Prec::F64;
   bool orthogonal = true;
   uint64_t seed = 1234;
+  const double* const* blocks = nullptr;  // caller-supplied blocks (reference layout), or null
   std::vector<void*> S;  // per core, device layout (n, bvr, bvl, pmax)
 };
 
@@ -780,6 +781,17 @@ void make_sketch(ttr_handle_s* h, SketchPlan& sp, int kmin) {
   Ctx& ctx = h->ctx;
   const size_t eb = prec_bytes(sp.prec);
   sp.S.assign(sp.d, nullptr);
+  if (sp.blocks) {  // the caller's blocks: upload and pack into the device layout
+    for (int k = kmin; k < sp.d; ++k) {
+      TTR_REQUIRE(sp.blocks[k] != nullptr, ST_INVALID_ARG, "blocks[" + std::to_string(k) + "] is NULL");
+      const int64_t n = sp.dims[k], bvr = sp.bv[k + 1], bvl = sp.bv[k], p = sp.pmax[k];
+      const int64_t cnt = n * bvr * bvl * p;
+      sp.S[k] = ctx.ws.alloc(size_t(cnt) * eb);
+      const double* dref = ctx.upload(sp.blocks[k], size_t(cnt));
+      sketch_pack(ctx, dref, sp.S[k], int(n), int(bvr), int(bvl), int(p), 1.0, sp.prec);
+    }
+    return;
+  }
   std::vector<SketchGen> gen;
   for (int k = kmin; k < sp.d; ++k) {
     const int64_t n = sp.dims[k], bvr = sp.bv[k + 1], bvl = sp.bv[k], p = sp.pmax[k];
@@ -985,6 +997,7 @@ void prepare_plan(ttr_handle_s* h, const ttr_rand_opts* o, int d, const std::vec
   sp.prec = to_prec(o->sketch_prec);
   sp.orthogonal = o->orthogonal != 0;
   sp.seed = o->seed;
+  sp.blocks = o->blocks;
   sp.bv = block_ranks(dims, o->block_rks > 0 ? o->block_rks : d);
   sp.pmax.assign(d + 1, 1);
   for (auto& it : items) {
@@ -997,6 +1010,8 @@ void prepare_plan(ttr_handle_s* h, const ttr_rand_opts* o, int d, const std::vec
     }
     for (int k = 0; k + 1 < d; ++k)
       TTR_REQUIRE(it.p[k] <= it.p[k + 1], ST_INVALID_ARG, "block counts must be non-decreasing towards the right");
+    TTR_REQUIRE(!sp.blocks || it.p == items[0].p, ST_INVALID_ARG,
+                "with caller-supplied blocks every item must have the same block counts");
   }
   (void)h;
 }
@@ -1237,6 +1252,7 @@ struct FwdPlan {
   std::vector<int64_t> dims, bv, pmax;
   bool orthogonal = true;
   uint64_t seed = 1234;
+  const double* const* blocks = nullptr;  // caller-supplied blocks (reference layout), or null
   std::vector<double*> S;  // core k: (bv_k n_k) x bv_{k+1} x pmax_{k+1}, reference order, FP64
 };
 
@@ -1256,6 +1272,14 @@ struct FwdTerm {
 void make_sketch_fwd(ttr_handle_s* h, FwdPlan& fp, int kend) {
   Ctx& ctx = h->ctx;
   fp.S.assign(fp.d, nullptr);
+  if (fp.blocks) {  // the caller's blocks are in this layout already
+    for (int k = 0; k < kend; ++k) {
+      TTR_REQUIRE(fp.blocks[k] != nullptr, ST_INVALID_ARG, "blocks[" + std::to_string(k) + "] is NULL");
+      const int64_t cnt = fp.bv[k] * fp.dims[k] * fp.bv[k + 1] * fp.pmax[k + 1];
+      fp.S[k] = const_cast<double*>(ctx.upload(fp.blocks[k], size_t(cnt)));
+    }
+    return;
+  }
   std::vector<SketchGen> gen;
   std::vector<Fill2D> fills;
   std::vector<QRJob> qj;
@@ -1329,6 +1353,7 @@ void prepare_fwd(const ttr_rand_opts* o, int d, const std::vector<int64_t>& dims
   fp.dims = dims;
   fp.orthogonal = o->orthogonal != 0;
   fp.seed = o->seed;
+  fp.blocks = o->blocks;
   fp.bv = block_ranks_fwd(dims, o->block_rks > 0 ? o->block_rks : d);
   fp.pmax.assign(size_t(d) + 1, 1);
 }
@@ -1363,6 +1388,9 @@ void stta_sketch_batch(ttr_handle_s* h, std::vector<SttaItem>& its, const ttr_ra
   orr.seed = seed_right;
   orr.oversampling = 1.0;
   orr.forward = 0;
+  orr.blocks = o->blocks_right;
+  TTR_REQUIRE((o->blocks == nullptr) == (o->blocks_right == nullptr), ST_INVALID_ARG,
+              "STTA takes caller-supplied blocks for both sketches or for neither");
   std::vector<RandItem> ritems;
   for (auto& it : its) ritems.push_back(it.right);
   SketchPlan sp;
@@ -1378,6 +1406,8 @@ void stta_sketch_batch(ttr_handle_s* h, std::vector<SttaItem>& its, const ttr_ra
       it.sl[k] = fp.bv[k] * it.pl[k];
       fp.pmax[k] = std::max(fp.pmax[k], it.pl[k]);
     }
+    TTR_REQUIRE(!fp.blocks || it.pl == its[0].pl, ST_INVALID_ARG,
+                "with caller-supplied blocks every item must have the same block counts");
   }
   {
     ProfScope ps(ctx, "phase:stta_sketch");

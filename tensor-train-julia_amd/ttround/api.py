@@ -47,7 +47,21 @@ def r_and_d_to_rks(rks, dims, rmax: int = 1024) -> List[int]:
     return out
 
 
-def _opts(orthogonal=True, sketch_prec="f64", block_rks=None, seed=1234, p=None, oversampling=1.0, forward=False):
+def _block_ptrs(blocks, keep):
+    """Pointer array over per-core FP64 block arrays (column-major copies kept alive in ``keep``)."""
+    if blocks is None:
+        return C.cast(None, L.f64pp)
+    arrs = [np.asfortranarray(np.asarray(b, dtype=np.float64)) for b in blocks]
+    keep.extend(arrs)
+    ptrs = (L.f64p * len(arrs))(*[a.ctypes.data_as(L.f64p) for a in arrs])
+    keep.append(ptrs)
+    return C.cast(ptrs, L.f64pp)
+
+
+def _opts(orthogonal=True, sketch_prec="f64", block_rks=None, seed=1234, p=None, oversampling=1.0, forward=False,
+          blocks=None, blocks_right=None):
+    """``blocks``: optional per-core sketch blocks exactly as ``generate_sketch_blocks``
+    returns them (``src/sketches.jl:44-84``), used instead of the library's draws."""
     o = L.RandOpts()
     o.forward = 1 if forward else 0
     o.orthogonal = 1 if orthogonal else 0
@@ -62,6 +76,8 @@ def _opts(orthogonal=True, sketch_prec="f64", block_rks=None, seed=1234, p=None,
     else:
         o.p = C.cast(None, L.i64p)
     o.oversampling = float(oversampling)
+    o.blocks = _block_ptrs(blocks, keep)
+    o.blocks_right = _block_ptrs(blocks_right, keep)
     return o, keep
 
 
@@ -274,7 +290,7 @@ def dot_operator_batch(psis: Sequence[DeviceTT], ops, phis: Sequence[DeviceTT]) 
 
 
 def round_rand_batch(items, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
-                     sketch_prec: str = "f64", p=None, oversampling: float = 1.0) -> List[DeviceTT]:
+                     sketch_prec: str = "f64", p=None, oversampling: float = 1.0, blocks=None) -> List[DeviceTT]:
     """Batched randomized rounding of linear combinations.
 
     ``items`` is a list of ``(alphas, [DeviceTT...], rks)`` triples, or of
@@ -297,7 +313,7 @@ def round_rand_batch(items, orthogonal: bool = True, seed: int = 1234, block_rks
         arr[i].term_ids = L.ptr_i64(ids)
         arr[i].op_id = -1 if op is None else op.id
         arr[i].rks = L.ptr_i64(r)
-    o, k2 = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling)
+    o, k2 = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling, blocks=blocks)
     out = np.zeros(len(items), np.int64)
     h.check(h.lib.ttr_round_rand(h.h, len(items), arr, C.byref(o), L.ptr_i64(out)))
     return [DeviceTT(h, t) for t in out]
@@ -326,7 +342,7 @@ def block_structure(dims, rks, block_rks: Optional[int] = None, p=None, oversamp
 
 
 def tt_sketch_batch(xs: Sequence[DeviceTT], s: int, ops=None, orthogonal: bool = True, seed: int = 1234,
-                    block_rks: Optional[int] = None, sketch_prec: str = "f64"):
+                    block_rks: Optional[int] = None, sketch_prec: str = "f64", blocks=None):
     """Boundary-only reverse sketches of a batch (``tt_sketch``, ``src/sketches.jl:645-813``)
     in one library call: item ``i`` is ``xs[i]``, or ``ops[i] * xs[i]`` (never formed)
     when ``ops[i]`` is a ``DeviceTTO``.  All items share the draws of ``seed``.
@@ -334,7 +350,7 @@ def tt_sketch_batch(xs: Sequence[DeviceTT], s: int, ops=None, orthogonal: bool =
     h = xs[0].handle
     ids = L.as_i64([x.id for x in xs])
     oids = L.as_i64([-1 if ops is None or ops[i] is None else ops[i].id for i in range(len(xs))])
-    o, keep = _opts(orthogonal, sketch_prec, block_rks, seed)
+    o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, blocks=blocks)
     sa = np.zeros(1, np.int64)
     h.check(h.lib.ttr_tt_sketch(h.h, len(xs), L.ptr_i64(ids), L.ptr_i64(oids), int(s), C.byref(o),
                                 C.cast(None, L.f64p), 0, L.ptr_i64(sa)))
@@ -681,7 +697,7 @@ def _rks_arg(rks, N):
 
 
 def ttrand_rounding(*args, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
-                    sketch_prec: str = "f64") -> TTvector:
+                    sketch_prec: str = "f64", blocks=None) -> TTvector:
     """Randomized rounding (``src/tt_randtools.jl:112-361``).
 
     Forms: ``ttrand_rounding(y)``, ``ttrand_rounding(y, rks_or_rmax)``,
@@ -692,7 +708,8 @@ def ttrand_rounding(*args, orthogonal: bool = True, seed: int = 1234, block_rks:
     ranks follow ``default_rank_heuristic`` of ``y`` (or of ``ys``).  The sketch
     draws come from the library's counter-based Philox stream keyed by ``seed``;
     ``sketch_prec`` selects the precision of the sketch contractions (the QR and
-    projections are FP64)."""
+    projections are FP64).  ``blocks`` (optional, per core) are sketch blocks as
+    ``generate_sketch_blocks`` returns them, used instead of the library's draws."""
     op = None
     if isinstance(args[0], tuple):
         # NTuple form (src/tt_randtools.jl:364-466): the entrywise product of the
@@ -736,7 +753,7 @@ def ttrand_rounding(*args, orthogonal: bool = True, seed: int = 1234, block_rks:
     dop = DeviceTTO(op) if op is not None else None
     item = (alpha, dop, devs, rks) if dop is not None else (alpha, devs, rks)
     out = round_rand_batch([item], orthogonal=orthogonal, seed=seed, block_rks=block_rks,
-                           sketch_prec=sketch_prec)[0]
+                           sketch_prec=sketch_prec, blocks=blocks)[0]
     res = out.to_host()
     for d in devs + [out] + ([dop] if dop is not None else []):
         d.free()
@@ -745,7 +762,7 @@ def ttrand_rounding(*args, orthogonal: bool = True, seed: int = 1234, block_rks:
 
 def tt_recursive_sketch(*args, orthogonal: bool = True, reverse: bool = True, seed: int = 1234,
                         block_rks: Optional[int] = None, p=None, oversampling: float = 1.0,
-                        sketch_prec: str = "f64"):
+                        sketch_prec: str = "f64", blocks=None):
     """``tt_recursive_sketch([H,] A, rks; reverse=true, ...)`` (``src/sketches.jl:175-315``;
     operator form ``:317-472``, the product ``H A`` never formed; NTuple form
     ``:474-615``, the entrywise product of the tuple's TTs).
@@ -769,11 +786,11 @@ def tt_recursive_sketch(*args, orthogonal: bool = True, reverse: bool = True, se
         r[A.N if reverse else 0] = 1
         rks = r
     if not reverse:
-        return _forward_sketch(A, rks, orthogonal, seed, block_rks, p, oversampling)
+        return _forward_sketch(A, rks, orthogonal, seed, block_rks, p, oversampling, blocks)
     dev = A._dev()
     h = dev.handle
     dop = DeviceTTO(H, h) if H is not None else None
-    o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling)
+    o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling, blocks=blocks)
     ra = L.as_i64(rks)
     bv, pp = block_structure(A.ttv_dims, rks, block_rks, p, oversampling)
     srk = [bv[k] * pp[k] for k in range(A.N + 1)]
@@ -859,11 +876,11 @@ def tt_combined_sketch(*args, reverse: bool = True, orthogonal: bool = True, see
     return sketch, W, srk
 
 
-def _forward_sketch(A, rks, orthogonal, seed, block_rks, p, oversampling):
+def _forward_sketch(A, rks, orthogonal, seed, block_rks, p, oversampling, blocks=None):
     """``tt_recursive_sketch(A, rks; reverse=false)`` (``src/sketches.jl:241-303``), FP64."""
     dev = A._dev()
     h = dev.handle
-    o, keep = _opts(orthogonal, "f64", block_rks, seed, p, oversampling, forward=True)
+    o, keep = _opts(orthogonal, "f64", block_rks, seed, p, oversampling, forward=True, blocks=blocks)
     ra = L.as_i64(rks)
     bv, pp = block_structure_forward(A.ttv_dims, rks, block_rks, p, oversampling)
     srk = [bv[k] * pp[k] for k in range(A.N + 1)]
@@ -888,7 +905,7 @@ def _rks_stta(rks, N):
 
 
 def stta_sketch(x: TTvector, rks, seed_left: int = 1234, seed_right: int = 5678, orthogonal: bool = True,
-                block_rks: Optional[int] = None):
+                block_rks: Optional[int] = None, blocks_left=None, blocks_right=None):
     """``stta_sketch(x, rks; seed_left, seed_right, orthogonal, block_rks)``
     (``src/sketches.jl:1126-1156``).  Returns ``(Omega, Psi)``: ``Omega[k]`` of shape
     ``(sl_{k+1}, sr_{k+1})`` and ``Psi[k]`` of shape ``(n_k, sl_k, sr_{k+1})``."""
@@ -902,7 +919,7 @@ def stta_sketch(x: TTvector, rks, seed_left: int = 1234, seed_right: int = 5678,
     Ps = [np.zeros((x.ttv_dims[k], sl[k], sr[k + 1]), order="F") for k in range(N)]
     dev = x._dev()
     h = dev.handle
-    o, keep = _opts(orthogonal, "f64", block_rks, seed_left)
+    o, keep = _opts(orthogonal, "f64", block_rks, seed_left, blocks=blocks_left, blocks_right=blocks_right)
     ra = L.as_i64(rks)
     om_arr = (L.f64p * max(N - 1, 1))(*[L.ptr_f64(w) for w in Om])
     ps_arr = (L.f64p * N)(*[L.ptr_f64(w) for w in Ps])
@@ -917,7 +934,7 @@ def stta_sketch(x: TTvector, rks, seed_left: int = 1234, seed_right: int = 5678,
 
 
 def stta_batch(items, seed_left: int = 1234, seed_right: int = 5678, orthogonal: bool = True,
-               block_rks: Optional[int] = None) -> List[DeviceTT]:
+               block_rks: Optional[int] = None, blocks_left=None, blocks_right=None) -> List[DeviceTT]:
     """Batched ``stta``: ``items`` is a list of ``(alphas, [DeviceTT...], rks)``."""
     h = items[0][1][0].handle
     keep = []
@@ -932,14 +949,14 @@ def stta_batch(items, seed_left: int = 1234, seed_right: int = 5678, orthogonal:
         arr[i].term_ids = L.ptr_i64(ids)
         arr[i].op_id = -1
         arr[i].rks = L.ptr_i64(r)
-    o, k2 = _opts(orthogonal, "f64", block_rks, seed_left)
+    o, k2 = _opts(orthogonal, "f64", block_rks, seed_left, blocks=blocks_left, blocks_right=blocks_right)
     out = np.zeros(len(items), np.int64)
     h.check(h.lib.ttr_stta(h.h, len(items), arr, C.byref(o), int(seed_right) & 0xFFFFFFFFFFFFFFFF, L.ptr_i64(out)))
     return [DeviceTT(h, t) for t in out]
 
 
 def stta(*args, seed_left: int = 1234, seed_right: int = 5678, orthogonal: bool = True,
-         block_rks: Optional[int] = None) -> TTvector:
+         block_rks: Optional[int] = None, blocks_left=None, blocks_right=None) -> TTvector:
     """Streaming TT approximation (``src/tt_randtools.jl:552-638``).
 
     Forms: ``stta(y[, rks_or_rmax])`` and ``stta(alpha, ys[, rks_or_rmax])``; the
@@ -957,7 +974,8 @@ def stta(*args, seed_left: int = 1234, seed_right: int = 5678, orthogonal: bool 
         rks = _rks_stta(args[2], ys[0].N) if len(args) > 2 else default_rank_heuristic(ys)
     devs = [t._dev() for t in ys]
     try:
-        out = stta_batch([(alpha, devs, rks)], seed_left, seed_right, orthogonal, block_rks)[0]
+        out = stta_batch([(alpha, devs, rks)], seed_left, seed_right, orthogonal, block_rks, blocks_left,
+                         blocks_right)[0]
         res = out.to_host()
         out.free()
     finally:
