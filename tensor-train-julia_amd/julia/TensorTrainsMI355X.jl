@@ -15,6 +15,15 @@ need no change.  Every call uploads the cores (column-major (n, r, r') arrays,
 exactly TTvector.ttv_vec), runs on the GPU and copies the result back into new
 Julia arrays.
 
+Sketch draws: by default (`reference_draws=true`) the binding seeds
+`Random.default_rng()` with `seed` and calls the reference's own
+`generate_sketch_blocks` core by core in the reference's order
+(src/sketches.jl:177-178,222-225,279-281), then hands those blocks to the library
+(`ttr_rand_opts.blocks`), so a given `seed` gives the reference's sketch and
+result; `reference_draws=false` uses the library's counter-based draws instead
+(no host work).  Every method accepts the reference's `timer` keyword (TimerOutputs
+timing of the host code only; the GPU work is one call).
+
 Status codes map to the reference's exceptions:
 TTR_INVALID_ARG -> ArgumentError, TTR_DIM_MISMATCH -> DimensionMismatch,
 TTR_NUMERICAL_FAILURE -> LinearAlgebra.LAPACKException-like ErrorException,
@@ -23,6 +32,7 @@ others -> ErrorException with ttr_last_error.
 module TensorTrainsMI355X
 
 using TensorTrains
+using Random
 import TensorTrains: tt_rounding, tt_rounding!, orthogonalize, ttrand_rounding, tt_recursive_sketch,
                      tt_sketch, tt_combined_sketch, stta, stta_sketch, dot_par, dot_operator, partial_contraction
 import LinearAlgebra: dot
@@ -49,6 +59,7 @@ function check(st)
     msg = unsafe_string(ccall((:ttr_last_error, lib), Cstring, (Ptr{Cvoid},), handle()))
     st == 1 && throw(ArgumentError(msg))
     st == 2 && throw(DimensionMismatch(msg))
+    st == 4 && error("LAPACK-like failure (libttround): $msg")
     error("libttround status $st: $msg")
 end
 
@@ -116,11 +127,21 @@ function tt_rounding(x::TTvector{Float64,N}; tol=1e-12, rmax=2^14, direction=:le
     return download(out[], Val(N))
 end
 
+# The reference rounds x in place only when it is already orthogonal in the sweep's
+# sense; otherwise it rounds an orthogonalised copy and x stays as it was
+# (src/tt_rounding.jl:158-167)
 function tt_rounding!(x::TTvector{Float64,N}; tol=1e-12, rmax=2^14, direction=:left) where {N}
+    orth = direction == :left ? TensorTrains.is_leftorthogonal(x) : TensorTrains.is_rightorthogonal(x)
     y = tt_rounding(x; tol=tol, rmax=rmax, direction=direction)
+    orth || return
     copyto!(x.ttv_vec, y.ttv_vec); copyto!(x.ttv_rks, y.ttv_rks); copyto!(x.ttv_ot, y.ttv_ot)
     return
 end
+
+# tt_rounding(A::TToperator; tol, rmax) (src/tt_rounding.jl:275-277): the rounding of the
+# (n^2, R, R') reshape of the cores
+tt_rounding(A::TToperator{Float64,N}; tol=1e-12, rmax=2^14) where {N} =
+    TensorTrains.ttv_to_tto(tt_rounding(TensorTrains.tto_to_ttv(A); tol=tol, rmax=rmax))
 
 function orthogonalize(x::TTvector{Float64,N}; i=1::Int) where {N}
     id = upload(x); out = Ref{Int64}(0)
@@ -182,10 +203,47 @@ struct RandOpts
     seed::UInt64
     p::Ptr{Int64}
     oversampling::Float64
-    forward::Int32       # 0: reverse = true (right-to-left), 1: reverse = false
+    forward::Int32                    # 0: reverse = true (right-to-left), 1: reverse = false
+    blocks::Ptr{Ptr{Float64}}         # caller's blocks per core, or C_NULL (library draws)
+    blocks_right::Ptr{Ptr{Float64}}   # stta: the right sketch's blocks
 end
 
-const PREC = Dict(Float64 => Int32(0), Float32 => Int32(1))
+# sketch precision: Float64, Float32, or BF16 (BFloat16s.BFloat16, or the symbol :bf16)
+const PREC = Dict{Any,Int32}(Float64 => Int32(0), Float32 => Int32(1), :bf16 => Int32(2))
+_prec(T) = haskey(PREC, T) ? PREC[T] :
+           (T isa DataType && nameof(T) == :BFloat16) ? Int32(2) :
+           throw(ArgumentError("sketch precision must be Float64, Float32 or BFloat16"))
+
+_opts(orthogonal, T, block_rks, seed; p=Ptr{Int64}(C_NULL), forward=false, blocks=Ptr{Ptr{Float64}}(C_NULL),
+      blocks_right=Ptr{Ptr{Float64}}(C_NULL)) =
+    Ref(RandOpts(orthogonal ? 1 : 0, _prec(T), block_rks, UInt64(seed), p, 1.0, Int32(forward ? 1 : 0), blocks,
+                 blocks_right))
+
+# The reference's sketch blocks for `seed`: default_rng seeded, then
+# generate_sketch_blocks core by core in the reference's order (right to left for the
+# reverse sketch, left to right for the forward one), each array as it returns it
+# (src/sketches.jl:177-178,222-225,279-281,700-715).  Drawn in the element type TD of
+# the reference's call (Float64 for ttrand_rounding of Float64 TTs) and passed as FP64.
+function _reference_blocks(::Type{TD}, dims, bv, pp, orthogonal::Bool, seed, reverse::Bool) where {TD}
+    rng = Random.default_rng()
+    Random.seed!(rng, seed)
+    N = length(dims)
+    B = Vector{Array{Float64,4}}(undef, N)
+    for k in (reverse ? (N:-1:1) : (1:N))
+        S = reverse ? TensorTrains.generate_sketch_blocks(rng, TD, bv[k+1], dims[k], bv[k], pp[k], orthogonal) :
+                      TensorTrains.generate_sketch_blocks(rng, TD, bv[k], dims[k], bv[k+1], pp[k+1], orthogonal)
+        B[k] = Array{Float64,4}(S)
+    end
+    return B
+end
+
+# the reverse sketch's structure of the default block counts (src/sketches.jl:186-198)
+function _reverse_structure(dims, rks, block_rks, N; p=0, oversampling=1)
+    bv = _block_ranks(dims, block_rks, N)
+    pp = p == 0 ? TensorTrains.compute_sketch_blocks_heuristic(collect(Int, rks), bv, N; reverse=true) : collect(Int, p)
+    oversampling != 1 && (pp[1:N] .= ceil.(Int, oversampling .* pp[1:N]))
+    return bv, pp
+end
 
 function upload(A::TToperator{Float64,N}) where {N}
     id = Ref{Int64}(0)
@@ -200,14 +258,22 @@ end
 
 # alpha_1 (A) y_1 + sum_{j>1} alpha_j y_j; with A given the product is never formed
 function rand_round(α::Vector{Float64}, A, y::Vector{TTvector{Float64,N}}, rks;
-                    orthogonal=true, seed=1234, block_rks::Int=N, sketch_type::Type=Float64) where {N}
+                    orthogonal=true, seed=1234, block_rks::Int=N, sketch_type=Float64, reference_draws::Bool=true,
+                    timer=nothing) where {N}
     ids = [upload(t) for t in y]
     op = A === nothing ? -1 : upload(A)
     r = collect(Int64, rks)
     out = Ref{Int64}(0)
-    GC.@preserve α ids r begin
+    B = Array{Float64,4}[]
+    if reference_draws  # the draws of tt_recursive_sketch(Float64, y, rks; seed) (src/tt_randtools.jl:117)
+        bv, pp = _reverse_structure(y[1].ttv_dims, rks, block_rks, N)
+        B = _reference_blocks(Float64, y[1].ttv_dims, bv, pp, orthogonal, seed, true)
+    end
+    bptr = [pointer(b) for b in B]
+    GC.@preserve α ids r B bptr begin
         item = Ref(RandItem(length(y), pointer(α), pointer(ids), op, pointer(r)))
-        opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[sketch_type], block_rks, UInt64(seed), C_NULL, 1.0, Int32(0)))
+        opts = _opts(orthogonal, sketch_type, block_rks, seed;
+                     blocks=reference_draws ? pointer(bptr) : Ptr{Ptr{Float64}}(C_NULL))
         check(ccall((:ttr_round_rand, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, Ref{Int64}),
                     handle(), 1, item, opts, out))
     end
@@ -251,7 +317,8 @@ end
 # tt_recursive_sketch(T, [H,] A, rks; reverse=true, ...) (src/sketches.jl:175-240,300-303,317-459)
 function _recursive_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}}, rks::Vector{Int};
                            orthogonal=true, reverse=true,
-                           seed=1234, block_rks::Int=N, p=0, oversampling=1) where {T,N}
+                           seed=1234, block_rks::Int=N, p=0, oversampling=1, reference_draws::Bool=true,
+                           timer=nothing) where {T,N}
     reverse || H === nothing || throw(ArgumentError("the GPU forward sketch takes no operator"))
     bv = reverse ? _block_ranks(A.ttv_dims, block_rks, N) : _block_ranks_fwd(A.ttv_dims, block_rks, N)
     pp = p == 0 ? TensorTrains.compute_sketch_blocks_heuristic(rks, bv, N; reverse=reverse) : collect(Int, p)
@@ -265,10 +332,12 @@ function _recursive_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N
     id = upload(A)
     op = H === nothing ? -1 : upload(H)
     r, pv, srk = collect(Int64, rks), collect(Int64, pp), zeros(Int64, N + 1)
-    GC.@preserve W r pv begin
+    B = reference_draws ? _reference_blocks(T, A.ttv_dims, bv, pp, orthogonal, seed, reverse) : Array{Float64,4}[]
+    bptr = [pointer(b) for b in B]
+    GC.@preserve W r pv B bptr begin
         ptrs = [pointer(w) for w in W]
-        opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[T], block_rks, UInt64(seed), pointer(pv), 1.0,
-                            Int32(reverse ? 0 : 1)))
+        opts = _opts(orthogonal, T, block_rks, seed; p=pointer(pv), forward=!reverse,
+                     blocks=reference_draws ? pointer(bptr) : Ptr{Ptr{Float64}}(C_NULL))
         if H === nothing
             check(ccall((:ttr_sketch, lib), Cint,
                         (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{RandOpts}, Ptr{Ptr{Float64}}, Ptr{Int64}),
@@ -315,21 +384,29 @@ tt_recursive_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rma
 
 # tt_sketch(T, [H,] A, s; reverse=true, ...) (src/sketches.jl:645-817): boundary sketch only
 function _tt_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}}, s::Int; reverse=true, orthogonal=true,
-                    seed=1234,
-                    block_rks::Int=N) where {T,N}
+                    seed=1234, block_rks::Int=N, reference_draws::Bool=true, timer=nothing) where {T,N}
     reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
+    # the same block count p = cld(s, bv_1) at every core (src/sketches.jl:665-668)
+    bv = _block_ranks(A.ttv_dims, block_rks, N)
+    pc = cld(s, bv[1])
+    B = reference_draws ? _reference_blocks(T, A.ttv_dims, bv, fill(pc, N + 1), orthogonal, seed, true) :
+                          Array{Float64,4}[]
+    bptr = [pointer(b) for b in B]
     id = upload(A)
     op = H === nothing ? -1 : upload(H)
     sa = Ref{Int64}(0)
     ids, ops = Int64[id], Int64[op]
-    opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[T], block_rks, UInt64(seed), C_NULL, 1.0, Int32(0)))
-    check(ccall((:ttr_tt_sketch, lib), Cint,
-                (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ref{RandOpts}, Ptr{Float64}, Int64, Ref{Int64}),
-                handle(), 1, ids, ops, s, opts, C_NULL, 0, sa))
-    out = zeros(Float64, sa[])
-    check(ccall((:ttr_tt_sketch, lib), Cint,
-                (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ref{RandOpts}, Ptr{Float64}, Int64, Ref{Int64}),
-                handle(), 1, ids, ops, s, opts, out, length(out), sa))
+    out = Float64[]
+    GC.@preserve B bptr begin
+        opts = _opts(orthogonal, T, block_rks, seed; blocks=reference_draws ? pointer(bptr) : Ptr{Ptr{Float64}}(C_NULL))
+        check(ccall((:ttr_tt_sketch, lib), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ref{RandOpts}, Ptr{Float64}, Int64, Ref{Int64}),
+                    handle(), 1, ids, ops, s, opts, C_NULL, 0, sa))
+        out = zeros(Float64, sa[])
+        check(ccall((:ttr_tt_sketch, lib), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Ref{RandOpts}, Ptr{Float64}, Int64, Ref{Int64}),
+                    handle(), 1, ids, ops, s, opts, out, length(out), sa))
+    end
     free(id)
     op >= 0 && free(op)
     return out, Int(sa[])
@@ -346,15 +423,18 @@ tt_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, s::Int; kwargs...) w
 # tt_combined_sketch(T, [H,] A, rks_or_rmax, s; reverse=true, ...) (src/sketches.jl:930-1001)
 function _combined_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}}, rks::Vector{Int}, s::Int;
                           reverse=true,
-                          orthogonal=true, seed=1234, block_rks::Int=N, timer=nothing) where {T,N}
+                          orthogonal=true, seed=1234, block_rks::Int=N, reference_draws::Bool=true,
+                          timer=nothing) where {T,N}
     reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
     rks = copy(rks)
     rks[1] = max(rks[1], min(s, rks[2]))
-    W, sketch_rks = _recursive_sketch(T, H, A, rks; orthogonal=orthogonal, seed=seed, block_rks=block_rks)
+    W, sketch_rks = _recursive_sketch(T, H, A, rks; orthogonal=orthogonal, seed=seed, block_rks=block_rks,
+                                      reference_draws=reference_draws)
     sketch = vec(W[1])
     s_rec = sketch_rks[1]
     if s > s_rec
-        extra, _ = _tt_sketch(T, H, A, s - s_rec; orthogonal=orthogonal, seed=seed + 1, block_rks=block_rks)
+        extra, _ = _tt_sketch(T, H, A, s - s_rec; orthogonal=orthogonal, seed=seed + 1, block_rks=block_rks,
+                              reference_draws=reference_draws)
         s = s_rec + length(extra)
         sketch = [sqrt(length(sketch) / s) .* sketch; sqrt(length(extra) / s) .* extra]
     end
@@ -367,14 +447,23 @@ end
 # entrywise product of the tuple, formed on the device with the first factor's rank index
 # fastest; target ranks are required, as in the reference
 function ttrand_rounding(y::NTuple{M,TTvector{Float64,N}}, rks::Vector{Int}; orthogonal=true, seed=1234,
-                         block_rks::Int=N, sketch_type::Type=Float64, timer=nothing) where {M,N}
-    M == 1 && return ttrand_rounding(y[1], rks; orthogonal=orthogonal, seed=seed, block_rks=block_rks)
+                         block_rks::Int=N, sketch_type=Float64, reference_draws::Bool=true, timer=nothing) where {M,N}
+    M == 1 && return ttrand_rounding(y[1], rks; orthogonal=orthogonal, seed=seed, block_rks=block_rks,
+                                     sketch_type=sketch_type, reference_draws=reference_draws)
+    # the product's sketch uses the draws of a single TT with the same dims (src/sketches.jl:474-540)
+    B = Array{Float64,4}[]
+    if reference_draws
+        bv, pp = _reverse_structure(y[1].ttv_dims, rks, block_rks, N)
+        B = _reference_blocks(Float64, y[1].ttv_dims, bv, pp, orthogonal, seed, true)
+    end
+    bptr = [pointer(b) for b in B]
     id = upload(_Product(y))
     α, ids, r = [1.0], Int64[id], collect(Int64, rks)
     out = Ref{Int64}(0)
-    GC.@preserve α ids r begin
+    GC.@preserve α ids r B bptr begin
         item = Ref(RandItem(1, pointer(α), pointer(ids), -1, pointer(r)))
-        opts = Ref(RandOpts(orthogonal ? 1 : 0, PREC[sketch_type], block_rks, UInt64(seed), C_NULL, 1.0, Int32(0)))
+        opts = _opts(orthogonal, sketch_type, block_rks, seed;
+                     blocks=reference_draws ? pointer(bptr) : Ptr{Ptr{Float64}}(C_NULL))
         check(ccall((:ttr_round_rand, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, Ref{Int64}),
                     handle(), 1, item, opts, out))
     end
@@ -412,14 +501,30 @@ tt_combined_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax
 
 # ------------------------------------------------------------------ STTA (src/tt_randtools.jl:552-638)
 
+# the reference's left (forward, oversampling 1.5, seed_left) and right (reverse,
+# seed_right) blocks of stta_sketch (src/sketches.jl:1126-1140)
+function _stta_blocks(dims, rks, block_rks, N, orthogonal, seed_left, seed_right)
+    bvl = _block_ranks_fwd(dims, block_rks, N)
+    pl = TensorTrains.compute_sketch_blocks_heuristic(collect(Int, rks), bvl, N; reverse=false)
+    pl[2:N+1] .= ceil.(Int, 1.5 .* pl[2:N+1])
+    bvr, pr = _reverse_structure(dims, rks, block_rks, N)
+    return _reference_blocks(Float64, dims, bvl, pl, orthogonal, seed_left, false),
+           _reference_blocks(Float64, dims, bvr, pr, orthogonal, seed_right, true)
+end
+
 function _stta(α::Vector{Float64}, y::Vector{TTvector{Float64,N}}, rks::Vector{Int}; seed_left=1234, seed_right=5678,
-               orthogonal=true, block_rks::Int=N, timer=nothing) where {N}
+               orthogonal=true, block_rks::Int=N, reference_draws::Bool=true, timer=nothing) where {N}
+    BL, BR = reference_draws ? _stta_blocks(y[1].ttv_dims, rks, block_rks, N, orthogonal, seed_left, seed_right) :
+                               (Array{Float64,4}[], Array{Float64,4}[])
+    pl, pr = [pointer(b) for b in BL], [pointer(b) for b in BR]
     ids = [upload(t) for t in y]
     r = collect(Int64, rks)
     out = Ref{Int64}(0)
-    GC.@preserve α ids r begin
+    GC.@preserve α ids r BL BR pl pr begin
         item = Ref(RandItem(length(y), pointer(α), pointer(ids), -1, pointer(r)))
-        opts = Ref(RandOpts(orthogonal ? 1 : 0, Int32(0), block_rks, UInt64(seed_left), C_NULL, 1.0, Int32(0)))
+        nb = Ptr{Ptr{Float64}}(C_NULL)
+        opts = _opts(orthogonal, Float64, block_rks, seed_left; blocks=reference_draws ? pointer(pl) : nb,
+                     blocks_right=reference_draws ? pointer(pr) : nb)
         check(ccall((:ttr_stta, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, UInt64, Ref{Int64}),
                     handle(), 1, item, opts, UInt64(seed_right), out))
     end
@@ -437,11 +542,16 @@ stta(α::Vector{Float64}, y::Vector{TTvector{Float64,N}}, rks=TensorTrains.defau
 
 # stta_sketch(x, rks; seed_left, seed_right, orthogonal, block_rks) (src/sketches.jl:1126-1156)
 function stta_sketch(x::TTvector{Float64,N}, rks::Vector{Int}; seed_left=1234, seed_right=5678, orthogonal=true,
-                     block_rks::Int=N, timer=nothing) where {N}
+                     block_rks::Int=N, reference_draws::Bool=true, timer=nothing) where {N}
     sl, sr = zeros(Int64, N + 1), zeros(Int64, N + 1)
+    BL, BR = reference_draws ? _stta_blocks(x.ttv_dims, rks, block_rks, N, orthogonal, seed_left, seed_right) :
+                               (Array{Float64,4}[], Array{Float64,4}[])
+    pl, pr = [pointer(b) for b in BL], [pointer(b) for b in BR]
     id = upload(x)
     r = collect(Int64, rks)
-    opts = Ref(RandOpts(orthogonal ? 1 : 0, Int32(0), block_rks, UInt64(seed_left), C_NULL, 1.0, Int32(0)))
+    nb = Ptr{Ptr{Float64}}(C_NULL)
+    opts = _opts(orthogonal, Float64, block_rks, seed_left; blocks=reference_draws ? pointer(pl) : nb,
+                 blocks_right=reference_draws ? pointer(pr) : nb)
     nullp = fill(Ptr{Float64}(C_NULL), N)
     # the sketch ranks first (no outputs), then the arrays
     check(ccall((:ttr_stta_sketch, lib), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{RandOpts}, UInt64,
@@ -449,7 +559,7 @@ function stta_sketch(x::TTvector{Float64,N}, rks::Vector{Int}; seed_left=1234, s
                 handle(), id, r, opts, UInt64(seed_right), nullp, nullp, sl, sr))
     Ω = [zeros(Float64, sl[k + 1], sr[k + 1]) for k in 1:N-1]
     Ψ = [zeros(Float64, x.ttv_dims[k], sl[k], sr[k + 1]) for k in 1:N]
-    GC.@preserve Ω Ψ begin
+    GC.@preserve Ω Ψ BL BR pl pr begin
         check(ccall((:ttr_stta_sketch, lib), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{RandOpts}, UInt64,
                     Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ptr{Int64}, Ptr{Int64}),
                     handle(), id, r, opts, UInt64(seed_right), [pointer(o) for o in Ω], [pointer(q) for q in Ψ],
