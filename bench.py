@@ -49,19 +49,23 @@ KERNEL_SYMBOL = {"gemm_f64": "gemm_scalar_kernel<double, double, double, double>
 
 WORKLOADS = {
     "c3": dict(items=256, d=40, n=4, terms=8, rmin=16, rmax=256, weight="0.5^j", seed_base=3000, sketch="f32",
+               accum="f64",
                block_rks=None, rks=None, cleanup=dict(tol=1e-6), op=False, det_items=48,
                text="batched randomized rounding: 256 TTs, each sum of 8 N(0,1) TTs d=40 n=4 r~U{16..256} "
                     "coeff 0.5^j; FP32 Gaussian sketch, block_rks=40, heuristic ranks; FP64 QR/projection; "
                     "SVD cleanup tol=1e-6"),
     "c1": dict(items=1, d=32, n=4, terms=6, rmin=100, rmax=100, weight="2^-k", seed_base=1000, sketch="f64",
+               accum="sketch",
                block_rks=144, rks=144, cleanup=dict(rmax=128), op=False, det_items=1,
                text="randomized rounding of sum_k 2^-k y_k, 6 N(0,1) TTs d=32 n=4 r=100 (formal rank 600); "
                     "FP64 Gaussian TT sketch rank 144 (block_rks=144); SVD cleanup rmax=128"),
     "c2": dict(items=1, d=64, n=2, terms=8, rmin=128, rmax=128, weight="10^(-k/2)", seed_base=2000, sketch="bf16",
+               accum="sketch",
                block_rks=272, rks=272, cleanup=dict(rmax=256), op=False, det_items=1,
                text="quantics TT: sum_k 10^(-k/2) y_k, 8 N(0,1) TTs d=64 n=2 r=128 (formal rank 1024); "
                     "sketch rank 272 (block_rks=272); SVD cleanup rmax=256"),
     "c4": dict(items=64, d=32, n=4, terms=1, rmin=128, rmax=128, weight="1", seed_base=4000, sketch="bf16",
+               accum="sketch",
                block_rks=144, rks=144, cleanup=dict(rmax=128), op=True, det_items=16,
                text="1D Hubbard MPO (L=32, t=1, U=4, bond dim 6) times 64 N(0,1) TTs d=32 n=4 r=128, "
                     "implicit product; sketch rank 144 (block_rks=144); SVD cleanup rmax=128"),
@@ -110,15 +114,17 @@ def build_workload(wl, args):
     d, n = wl["d"], wl["n"]
     dims = (n,) * d
     op = T.DeviceTTO(T.hubbard_1d_mpo(d, 1.0, 4.0)) if wl["op"] else None
-    items = []
+    items, specs = [], []
     for b in range(args.items):
-        terms, alphas, rk = [], [], []
+        terms, alphas, rk, seeds = [], [], [], []
         for j in range(1, wl["terms"] + 1):
             r = int(rng.integers(wl["rmin"], wl["rmax"] + 1))
-            t = T.DeviceTT.randn(dims, r, seed=wl["seed_base"] + j + 100000 * b)
+            seeds.append(wl["seed_base"] + j + 100000 * b)
+            t = T.DeviceTT.randn(dims, r, seed=seeds[-1])
             terms.append(t)
             alphas.append(weight(wl, j))
             rk.append(t.rks)
+        specs.append((seeds, [list(r) for r in rk]))
         if wl["rks"] is None:
             rks = list(rk[0])
             for q in rk[1:]:
@@ -129,7 +135,7 @@ def build_workload(wl, args):
         else:
             rks = [1] + [wl["rks"]] * (d - 1) + [1]
         items.append((alphas, terms, rks))
-    return dims, op, items
+    return dims, op, items, specs
 
 
 def run_step(items, op, wl, args, keep_raw=False):
@@ -137,7 +143,7 @@ def run_step(items, op, wl, args, keep_raw=False):
     import ttround as T
     batch = [(al, op, terms, rks) if op is not None else (al, terms, rks) for al, terms, rks in items]
     raw = T.round_rand_batch(batch, orthogonal=False, seed=1234, block_rks=wl["block_rks"],
-                             sketch_prec=args.sketch_prec)
+                             sketch_prec=args.sketch_prec, sketch_accum=args.sketch_accum)
     if args.no_cleanup:
         return raw, None
     clean = T.round_svd_batch(raw, **wl["cleanup"])
@@ -148,43 +154,91 @@ def run_step(items, op, wl, args, keep_raw=False):
     return clean, raw
 
 
-def parity_and_cpu(items, op, outs, raw, dims, wl, args):
-    """Item i of the batch through the FP64 oracle on the device's own sketch
-    blocks (and the same cleanup), timed on the host cores.  Returns the largest
-    relative error of the final result and of the randomized stage alone."""
+def _oracle_item(task):
+    """One item through the FP64 oracle in a worker process (one BLAS thread): the
+    terms are regenerated from their seeds (the device's N(0,1) stream, bit for
+    bit), the device's sketch blocks are given, and the errors of the device's
+    results are returned with the oracle's time."""
+    import oracle as O
+    from oracle import philox as P
+    dims, seeds, term_rks, al, rks, blocks, block_rks, hub, cleanup, got, got_raw = task
+    ys = [O.TT(P.rand_tt_cores(dims, r, s), dims, list(r), [0] * len(dims)) for s, r in zip(seeds, term_rks)]
+    H = O.hubbard_1d_mpo(len(dims), 1.0, 4.0) if hub else None
+    tc = time.perf_counter()
+    if H is not None:
+        ref = O.ttrand_rounding_op(al, H, ys, rks, blocks.__getitem__, False, block_rks)
+    else:
+        ref = O.ttrand_rounding_lincomb(al, ys, rks, blocks.__getitem__, False, block_rks)
+    e_raw = O.rel_err(ref, got_raw) if got_raw is not None else None
+    if cleanup is not None:
+        ref = O.tt_rounding(ref, **cleanup)
+    t = time.perf_counter() - tc
+    return O.rel_err(ref, got), e_raw, t
+
+
+def parity_and_cpu(items, specs, op, outs, raw, dims, wl, args):
+    """The first cpu_items items of the batch through the FP64 oracle on the
+    device's own sketch blocks (and the same cleanup), which is also the CPU
+    baseline: the items run batch-parallel in worker processes with one BLAS
+    thread each (the oracle is not BLAS-bound: one C3 item takes about as long on
+    one thread as on eight), and item 0 runs again in this process with every
+    BLAS thread (the serial loop of the reference's benchmarks).  Returns the
+    largest relative errors (final result, randomized stage) and the baseline."""
+    import multiprocessing as mp
+
     import oracle as O
     import ttround as T
 
     def host(dx):
         return O.TT([dx.core(k) for k in range(dx.d)], dx.dims, dx.rks, dx.ot)
-    errs, errs_raw, tcpu = [], [], 0.0
-    H = O.hubbard_1d_mpo(wl["d"], 1.0, 4.0) if op is not None else None
+    cleanup = None if args.no_cleanup else wl["cleanup"]
+    tasks = []
     for i in range(args.cpu_items):
         al, terms, rks = items[i]
-        ys = [O.TT([t.core(k) for k in range(t.d)], t.dims, t.rks, t.ot) for t in terms]
-        blocks = {}
-        for k in range(wl["d"]):
-            blocks[k] = T.sketch_blocks(dims, rks, k, orthogonal=False, seed=1234, block_rks=wl["block_rks"],
-                                        sketch_prec=args.sketch_prec)
-        tc = time.perf_counter()
-        if H is not None:
-            ref = O.ttrand_rounding_op(al, H, ys, rks, blocks.__getitem__, False, wl["block_rks"])
-        else:
-            ref = O.ttrand_rounding_lincomb(al, ys, rks, blocks.__getitem__, False, wl["block_rks"])
-        if raw is not None:
-            errs_raw.append(O.rel_err(ref, host(raw[i])))
-        if not args.no_cleanup:
-            ref = O.tt_rounding(ref, **wl["cleanup"])
-        tcpu += time.perf_counter() - tc
-        errs.append(O.rel_err(ref, host(outs[i])))
-        log(f"[bench] item {i}: rel err vs FP64 oracle {errs[-1]:.3e}"
-            f"{f' (randomized stage {errs_raw[-1]:.3e})' if errs_raw else ''} (oracle {tcpu:.1f}s)")
-    cpu = {"value": round(args.cpu_items / tcpu, 4), "unit": "rounded TTs/s", "cores": os.cpu_count(),
+        blocks = {k: T.sketch_blocks(dims, rks, k, orthogonal=False, seed=1234, block_rks=wl["block_rks"],
+                                     sketch_prec=args.sketch_prec) for k in range(wl["d"])}
+        seeds, trks = specs[i]
+        tasks.append((dims, seeds, trks, al, rks, blocks, wl["block_rks"], op is not None, cleanup, host(outs[i]),
+                      host(raw[i]) if raw is not None else None))
+    nproc = max(1, min(args.cpu_items, args.cpu_procs))
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"  # the workers' BLAS (this process keeps its own)
+    t0 = time.perf_counter()
+    with mp.get_context("spawn").Pool(nproc) as pool:
+        res = pool.map(_oracle_item, tasks, chunksize=1)
+    wall = time.perf_counter() - t0
+    for i, (e, er, t) in enumerate(res):
+        log(f"[bench] item {i}: rel err vs FP64 oracle {e:.3e}"
+            f"{f' (randomized stage {er:.3e})' if er is not None else ''} (oracle {t:.1f}s, 1 thread)")
+    errs = [r[0] for r in res]
+    errs_raw = [r[1] for r in res if r[1] is not None]
+    item_s = sorted(r[2] for r in res)
+    med = item_s[len(item_s) // 2]
+    # serial: item 0 in this process with all BLAS threads
+    try:
+        from threadpoolctl import threadpool_info
+        threads = str(max(i["num_threads"] for i in threadpool_info() if i.get("internal_api") == "openblas"))
+    except (ImportError, ValueError):
+        threads = os.environ.get("OMP_NUM_THREADS", str(os.cpu_count()))
+    tc = time.perf_counter()
+    _oracle_item(tasks[0])
+    t_serial = time.perf_counter() - tc
+    log(f"[bench] batch-parallel oracle: {args.cpu_items} items on {nproc} processes in {wall:.1f}s "
+        f"(median item {med:.1f}s); serial item 0 with all BLAS threads {t_serial:.1f}s")
+    par = args.cpu_items / wall
+    ser = 1.0 / t_serial
+    cpu = {"value": round(max(par, ser), 4), "unit": "rounded TTs/s", "cores": nproc if par >= ser else None,
            "kind": "port",
-           "sample": f"{args.cpu_items} item(s) of the same batch, FP64 numpy/OpenBLAS oracle "
-                     f"(randomized rounding{'' if args.no_cleanup else ' + SVD cleanup'}), "
-                     f"BLAS threads = {os.environ.get('OPENBLAS_NUM_THREADS', 'default')}"}
-    return max(errs), (max(errs_raw) if errs_raw else None), cpu
+           "sample": f"{args.cpu_items} item(s) of the same batch through the FP64 numpy/OpenBLAS oracle "
+                     f"(randomized rounding{'' if args.no_cleanup else ' + SVD cleanup'}) on the device's sketch "
+                     f"blocks; value = the faster of the batch-parallel loop ({nproc} processes x 1 BLAS thread) "
+                     f"and the serial loop (all BLAS threads)",
+           "batch_parallel": {"items": args.cpu_items, "processes": nproc, "blas_threads": 1, "wall_s": round(wall, 2),
+                              "value": round(par, 4), "median_item_s": round(med, 2),
+                              "item_s": [round(v, 2) for v in item_s]},
+           "serial": {"items": 1, "blas_threads": threads, "item_s": round(t_serial, 2), "value": round(ser, 4)}}
+    if cpu["cores"] is None:
+        cpu["cores"] = int(threads) if threads.isdigit() else os.cpu_count()
+    return max(errs), (max(errs_raw) if errs_raw else None), cpu, errs
 
 
 def deterministic(items, op, wl, args):
@@ -216,6 +270,14 @@ def deterministic(items, op, wl, args):
 
 
 def summarize(stats, args):
+    cert = stats.pop("svd_certify:bonds", None)
+    svdb = stats.pop("svd_sweep:svd_bonds", None)
+    cleanup = None
+    if cert or svdb:
+        cleanup = {"bonds_certified_per_step": round((cert or {}).get("flops", 0.0) / args.steps, 1),
+                   "bonds_tried_per_step": round((cert or {}).get("launches", 0) / args.steps, 1),
+                   "svd_bonds_per_step": round((svdb or {}).get("launches", 0) / args.steps, 1),
+                   "svd_bonds_truncating_per_step": round((svdb or {}).get("flops", 0.0) / args.steps, 1)}
     sweeps = stats.pop("jacobi_svd:sweeps", None)
     jacobi = ({"svds": sweeps["launches"], "mean_sweeps": round(sweeps["flops"] / max(sweeps["launches"], 1), 2),
                "max_sweeps": sweeps["bytes"]} if sweeps else None)
@@ -247,7 +309,7 @@ def summarize(stats, args):
     kernels = {k: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] // args.steps,
                    "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3),
                    "gbps": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in kern.items()}
-    return jacobi, sites, phases, roof, kernels
+    return jacobi, sites, phases, roof, kernels, cleanup
 
 
 def inner_main(args):
@@ -284,7 +346,7 @@ def inner_main(args):
     dt = time.perf_counter() - t0
     h.profile(False)
     args.steps = max(args.steps, 1)
-    _, sites, _, roof, kernels = summarize(h.profile_stats(), args)
+    _, sites, _, roof, kernels, _ = summarize(h.profile_stats(), args)
     value = m * args.steps / dt
     rel, cpu = None, None
     if args.cpu_items > 0:
@@ -331,11 +393,18 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS) + sorted(INNER), default="c3")
     ap.add_argument("--items", type=int, default=None, help="batch size (default: the workload's)")
     ap.add_argument("--sketch-prec", default=None, help="f64 | f32 | bf16 (default: the workload's)")
+    ap.add_argument("--sketch-accum", default=None, choices=["sketch", "f64"],
+                    help="sketch: contractions in the sketch precision (FP32 accumulation on FP32/BF16 MFMA); "
+                         "f64: the blocks rounded to the sketch precision, contractions in FP64 (default: the "
+                         "workload's; C3 uses f64, the mode that meets its 1e-6 error bar on every item)")
     ap.add_argument("--no-cleanup", action="store_true", help="skip the SVD cleanup after the randomized rounding")
     ap.add_argument("--det-items", type=int, default=None,
                     help="items for the FP64 deterministic comparison (0: skip; default: the workload's); their "
                          "materialised inputs must fit in HBM")
-    ap.add_argument("--cpu-items", type=int, default=1, help="items for the oracle CPU baseline / parity (0: skip)")
+    ap.add_argument("--cpu-items", type=int, default=8,
+                    help="items checked against the oracle, which is also the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-procs", type=int, default=16,
+                    help="worker processes of the batch-parallel CPU oracle (one BLAS thread each)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first copy BYTES of device memory 3 times (a plain copy kernel of known size, "
                          "the reference point for hardware traffic counters)")
@@ -352,6 +421,7 @@ def main():
     wl = WORKLOADS[args.workload]
     args.items = args.items or wl["items"]
     args.sketch_prec = args.sketch_prec or wl["sketch"]
+    args.sketch_accum = args.sketch_accum or wl["accum"]
     args.det_items = wl["det_items"] if args.det_items is None else args.det_items
 
     if args.copy_calibration > 0:
@@ -369,7 +439,7 @@ def main():
     h = T.default_handle()
     log(f"[bench] workload {args.workload}: {args.items} items, d={wl['d']}, n={wl['n']}")
     t0 = time.time()
-    dims, op, items = build_workload(wl, args)
+    dims, op, items, specs = build_workload(wl, args)
     log(f"[bench] workload ready in {time.time() - t0:.1f}s")
 
     for w in range(args.warmup):
@@ -390,14 +460,15 @@ def main():
         log(f"[bench] step {s} done")
     dt = time.perf_counter() - t0
     h.profile(False)
-    jacobi, sites, phases, roof, kernels = summarize(h.profile_stats(), args)
+    jacobi, sites, phases, roof, kernels, cleanup_stats = summarize(h.profile_stats(), args)
     ms_per_step = 1e3 * dt / args.steps
     value = args.items * args.steps / dt
     out_rks = max(outs[0].rks)
 
-    rel_err, rel_err_raw, cpu = (None, None, None)
+    rel_err, rel_err_raw, cpu, errs = (None, None, None, None)
     if args.cpu_items > 0:
-        rel_err, rel_err_raw, cpu = parity_and_cpu(items, op, outs, raw, dims, wl, args)
+        args.cpu_items = min(args.cpu_items, args.items)
+        rel_err, rel_err_raw, cpu, errs = parity_and_cpu(items, specs, op, outs, raw, dims, wl, args)
     for o in outs + (raw or []):
         o.free()
     det = deterministic(items, op, wl, args) if args.det_items > 0 else None
@@ -413,17 +484,21 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.sketch_prec}-sketch+f64",
+        "dtype": ("f64" if args.sketch_accum == "f64" or args.sketch_prec == "f64"
+                  else f"{args.sketch_prec}-sketch+f64"),
         "data": "synthetic (seeded N(0,1) TT cores drawn on the device)",
         "config": {"workload": wl["text"], "name": args.workload, "items": args.items, "d": wl["d"], "n": wl["n"],
                    "terms": wl["terms"], "sketch_prec": args.sketch_prec,
+                   "sketch_accum": args.sketch_accum,
                    "cleanup": None if args.no_cleanup else wl["cleanup"]},
         "rel_err_vs_oracle": rel_err,
         "rel_err_vs_oracle_randomized_stage": rel_err_raw,
+        "rel_err_vs_oracle_items": None if errs is None else [float(f"{e:.3e}") for e in errs],
         "out_rank_max_item0": out_rks,
         "fp64_deterministic": det,
         "speedup_vs_fp64_deterministic": (round(value / det["rounded_tts_per_s"], 3) if det else None),
         "phases_ms_per_step": phases,
+        "svd_sweep_bonds": cleanup_stats,
         "jacobi": jacobi,
         "kernels": kernels,
         "gemm_sites": sites,

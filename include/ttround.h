@@ -203,6 +203,11 @@ typedef struct {
   /* ttr_stta / ttr_stta_sketch only: the right-to-left sketch's blocks (seed_right);
    * `blocks` then holds the left-to-right sketch's.  NULL elsewhere. */
   const double* const* blocks_right;
+  /* 0: the sketch tensors W_k and the contractions that form them run in
+   * sketch_prec (FP32 / BF16 MFMA with FP32 accumulation); 1: only the blocks are
+   * rounded to sketch_prec, W_k and the contractions are FP64 (the computation
+   * the FP64 oracle performs on the same rounded blocks). */
+  int32_t sketch_accum;
 } ttr_rand_opts;
 
 /* Randomize-then-orthogonalize rounding of n linear combinations in one

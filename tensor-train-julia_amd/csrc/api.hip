@@ -587,6 +587,11 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       k = std::min<int64_t>({k, rmax, npos});
       k = std::max<int64_t>(k, 1);
       K[q] = k;
+      if (ctx.prof.on) {  // SVD bonds, and those whose rank decision dropped something
+        ProfStat& ps = ctx.prof.stats["svd_sweep:svd_bonds"];
+        ps.launches += 1;
+        ps.flops += k < m ? 1.0 : 0.0;
+      }
       order_off[q] = orders.size();
       orders.insert(orders.end(), order.begin(), order.begin() + k);
     }
@@ -744,7 +749,8 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
 struct SketchPlan {
   int d = 0;
   std::vector<int64_t> dims, bv, pmax;
-  Prec prec = Prec::F64;
+  Prec prec = Prec::F64;   // the blocks
+  Prec wprec = Prec::F64;  // the sketch tensors W_k, their temporaries and the contractions' math
   bool orthogonal = true;
   uint64_t seed = 1234;
   const double* const* blocks = nullptr;  // caller-supplied blocks (reference layout), or null
@@ -887,7 +893,8 @@ struct RandItem {
 void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items, int kmin) {
   Ctx& ctx = h->ctx;
   const int d = sp.d;
-  const size_t eb = prec_bytes(sp.prec);
+  const size_t eb = prec_bytes(sp.wprec), es = prec_bytes(sp.prec);  // W / temporaries, blocks
+  const Prec wp = sp.wprec;
   // buffers
   std::vector<std::vector<void*>> T1(items.size()), T2(items.size());
   for (size_t i = 0; i < items.size(); ++i) {
@@ -917,10 +924,10 @@ void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items,
           if (t.H) continue;
           const int64_t Rk = t.x->rks[k];
           for (int64_t j = 0; j < it.p[k]; ++j)
-            gb.add(int(Rk), int(bvl), int(n), 1.0, t.x->cores[k], 1, Rk, S + size_t(n * bvl * j) * eb, 1, n, 0.0,
+            gb.add(int(Rk), int(bvl), int(n), 1.0, t.x->cores[k], 1, Rk, S + size_t(n * bvl * j) * es, 1, n, 0.0,
                    static_cast<char*>(t.W[k]) + size_t(Rk * bvl * j) * eb, 1, Rk);
         }
-      gemm_run(ctx, gb, Prec::F64, sp.prec, sp.prec, sp.prec, "sketch_last");
+      gemm_run(ctx, gb, Prec::F64, sp.prec, wp, wp, "sketch_last");
       // operator terms: T2 (r R x n) = H_{d-1} applied to the last core, then T2 S_j
       for (size_t i = 0; i < items.size(); ++i)
         for (size_t ti = 0; ti < items[i].terms.size(); ++ti) {
@@ -936,12 +943,12 @@ void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items,
           o.os_m = rR; o.os_b = r; o.os_x = 1; o.os_y = 0;
           oj.push_back(o);
           for (int64_t j = 0; j < items[i].p[k]; ++j)
-            gb.add(int(rR), int(bvl), int(n), 1.0, T2[i][ti], 1, rR, S + size_t(n * bvl * j) * eb, 1, n, 0.0,
+            gb.add(int(rR), int(bvl), int(n), 1.0, T2[i][ti], 1, rR, S + size_t(n * bvl * j) * es, 1, n, 0.0,
                    static_cast<char*>(t.W[k]) + size_t(rR * bvl * j) * eb, 1, rR);
         }
-      op_apply_batched(ctx, oj, Prec::F64, sp.prec);
+      op_apply_batched(ctx, oj, Prec::F64, wp);
       oj.clear();
-      gemm_run(ctx, gb, sp.prec, sp.prec, sp.prec, sp.prec, "sketch_last_op");
+      gemm_run(ctx, gb, wp, sp.prec, wp, wp, "sketch_last_op");
     } else {
       for (size_t i = 0; i < items.size(); ++i)
         for (size_t ti = 0; ti < items[i].terms.size(); ++ti) {
@@ -964,8 +971,8 @@ void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items,
             oj.push_back(o);
           }
         }
-      gemm_run(ctx, gb, Prec::F64, sp.prec, sp.prec, sp.prec, "sketch_t1");
-      op_apply_batched(ctx, oj, sp.prec, sp.prec);
+      gemm_run(ctx, gb, Prec::F64, wp, wp, wp, "sketch_t1");
+      op_apply_batched(ctx, oj, wp, wp);
       oj.clear();
       for (size_t i = 0; i < items.size(); ++i)
         for (size_t ti = 0; ti < items[i].terms.size(); ++ti) {
@@ -974,17 +981,17 @@ void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items,
           const char* A = static_cast<const char*>(t.H ? T2[i][ti] : T1[i][ti]);
           for (int64_t j = 0; j < items[i].p[k]; ++j)
             gb.add(int(Rk), int(bvl), int(n * bvr), 1.0, A + size_t(Rk * n * bvr * j) * eb, 1, Rk,
-                   S + size_t(n * bvr * bvl * j) * eb, 1, n * bvr, 0.0,
+                   S + size_t(n * bvr * bvl * j) * es, 1, n * bvr, 0.0,
                    static_cast<char*>(t.W[k]) + size_t(Rk * bvl * j) * eb, 1, Rk);
         }
-      gemm_run(ctx, gb, sp.prec, sp.prec, sp.prec, sp.prec, "sketch_w");
+      gemm_run(ctx, gb, wp, sp.prec, wp, wp, "sketch_w");
     }
     std::vector<Pow2Job> pj;
     for (auto& it : items)
       for (auto& t : it.terms)
         pj.push_back(Pow2Job{t.W[k], t.rc(k), int(t.rc(k)), int(it.s[k]), k == d - 1 ? nullptr : t.E + k + 1,
                              t.E + k});
-    pow2_normalize_batched(ctx, pj, sp.prec);
+    pow2_normalize_batched(ctx, pj, wp);
   }
 }
 
@@ -995,6 +1002,8 @@ void prepare_plan(ttr_handle_s* h, const ttr_rand_opts* o, int d, const std::vec
   sp.d = d;
   sp.dims = dims;
   sp.prec = to_prec(o->sketch_prec);
+  TTR_REQUIRE(o->sketch_accum == 0 || o->sketch_accum == 1, ST_INVALID_ARG, "sketch_accum must be 0 or 1");
+  sp.wprec = o->sketch_accum ? Prec::F64 : sp.prec;
   sp.orthogonal = o->orthogonal != 0;
   sp.seed = o->seed;
   sp.blocks = o->blocks;
@@ -1126,7 +1135,7 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
         gb.seg(Ycur(t), 1, r * n, t.W[k + 1], 1, R1, int(R1), k == 0 ? t.alpha : 1.0, AM_PLAIN, t.E + k + 1);
       }
     }
-    gemm_run(ctx, gb, Prec::F64, sp.prec, Prec::F64, Prec::F64, "rr_Z");
+    gemm_run(ctx, gb, Prec::F64, sp.wprec, Prec::F64, Prec::F64, "rr_Z");
     std::vector<QRJob> qj;
     for (size_t i = 0; i < items.size(); ++i) {
       const int64_t r = orr[i][k], s = items[i].s[k + 1];
@@ -1682,6 +1691,8 @@ int ttr_create(ttr_handle_t* out, int device) {
       TTR_REQUIRE(v >= 1e-3 && v <= 1.0, ST_INVALID_ARG, "TTR_JACOBI_TOL_SCALE must be in [1e-3, 1]");
       h->ctx.jacobi_tol_scale = v;
     }
+    // TTR_JACOBI_SORT=0: no column sorting between Jacobi sweeps
+    if (const char* c = std::getenv("TTR_JACOBI_SORT")) h->ctx.jacobi_sort = c[0] != '0';
     // TTR_JACOBI_MAX_SWEEPS=s (1..40): Jacobi sweeps before a non-converged SVD fails
     if (const char* c = std::getenv("TTR_JACOBI_MAX_SWEEPS")) {
       const int v = std::atoi(c);
@@ -2354,7 +2365,7 @@ void sketch_one(ttr_handle_s* h, DevTT* x, DevTTO* H, const int64_t* rks, const 
   const RandTerm& tt = its[0].terms[0];
   std::vector<int32_t> E(static_cast<size_t>(d + 1));
   TTR_HIP(hipMemcpy(E.data(), tt.E, size_t(d + 1) * 4, hipMemcpyDeviceToHost));
-  const size_t eb = prec_bytes(sp.prec);
+  const size_t eb = prec_bytes(sp.wprec);
   for (int k = 0; k <= d; ++k) {
     if (sketch_rks) sketch_rks[k] = its[0].s[k];
     if (!W_out || !W_out[k]) continue;
@@ -2368,8 +2379,8 @@ void sketch_one(ttr_handle_s* h, DevTT* x, DevTTO* H, const int64_t* rks, const 
     TTR_HIP(hipMemcpy(raw.data(), tt.W[k], raw.size(), hipMemcpyDeviceToHost));
     for (int64_t e = 0; e < cnt; ++e) {
       double v;
-      if (sp.prec == Prec::F64) std::memcpy(&v, raw.data() + 8 * e, 8);
-      else if (sp.prec == Prec::F32) { float f; std::memcpy(&f, raw.data() + 4 * e, 4); v = f; }
+      if (sp.wprec == Prec::F64) std::memcpy(&v, raw.data() + 8 * e, 8);
+      else if (sp.wprec == Prec::F32) { float f; std::memcpy(&f, raw.data() + 4 * e, 4); v = f; }
       else { uint16_t b; std::memcpy(&b, raw.data() + 2 * e, 2); v = bf16_bits_to_float(b); }
       W_out[k][e] = v * scale;
     }
@@ -2512,7 +2523,7 @@ int ttr_tt_sketch(ttr_handle_t h, int64_t n, const int64_t* tt_ids, const int64_
     make_sketch(h, sp, 0);
     sketch_phase(h, sp, its, 0);
     Ctx& ctx = h->ctx;
-    const size_t eb = prec_bytes(sp.prec);
+    const size_t eb = prec_bytes(sp.wprec);
     // W_1 (1 x s) and its power-of-two exponent of every item, one sync
     char* hw = static_cast<char*>(ctx.pin.alloc(size_t(n) * size_t(sa) * eb));
     int32_t* he = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * size_t(n)));
@@ -2528,8 +2539,8 @@ int ttr_tt_sketch(ttr_handle_t h, int64_t n, const int64_t* tt_ids, const int64_
       const char* raw = hw + size_t(i) * size_t(sa) * eb;
       for (int64_t e = 0; e < sa; ++e) {
         double v;
-        if (sp.prec == Prec::F64) std::memcpy(&v, raw + 8 * e, 8);
-        else if (sp.prec == Prec::F32) { float f; std::memcpy(&f, raw + 4 * e, 4); v = f; }
+        if (sp.wprec == Prec::F64) std::memcpy(&v, raw + 8 * e, 8);
+        else if (sp.wprec == Prec::F32) { float f; std::memcpy(&f, raw + 4 * e, 4); v = f; }
         else { uint16_t b; std::memcpy(&b, raw + 2 * e, 2); v = bf16_bits_to_float(b); }
         out[i * sa + e] = v * scale;
       }

@@ -12,7 +12,9 @@
 //   2. runs one parallel cyclic Jacobi sweep on G (32 disjoint rotations per
 //      round, round-robin order, G in LDS and the accumulated rotation V in
 //      registers; see inner_sweep); a pair is rotated when
-//      |G_ij| > tol sqrt(G_ii G_jj), tol = p eps;
+//      |G_ij| > tol sqrt(G_ii G_jj), tol = p eps (a relative test: the small
+//      singular values stay relatively accurate, which the pseudo-inverses of
+//      STTA need);
 //   3. if anything rotated, applies V to the visited columns of B and of the
 //      accumulated right rotations J with FP64 MFMA (C <- C V, 32-row chunks
 //      staged in LDS) and marks the matrix as not yet converged.
@@ -599,10 +601,63 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   fill2d_batched(ctx, fills);
   zero_fill(ctx, flags, sizeof(int32_t) * 2 * nj);
   const int max_sweeps = std::max(1, std::min(MAX_SWEEPS, ctx.jacobi_max_sweeps));
+  // de Rijk's pivoting: before every sweep the columns of B (and, to keep
+  // B = B_in J, of J) are sorted by decreasing norm, so each column block holds
+  // columns of similar size.  On graded matrices (terms whose norms differ by
+  // many orders of magnitude) the unsorted iteration needed more than 40 sweeps.
+  std::vector<int64_t> off(size_t(nj) + 1, 0);
+  for (int q = 0; q < nj; ++q) off[q + 1] = off[q] + live[q].p;
+  double* nrm = nullptr;
+  int32_t* perm = nullptr;
+  double* hn = nullptr;
+  int32_t* hp = nullptr;
+  const SVDJob* dn = nullptr;
+  std::vector<double*> tB(nj), tJ(nj);
+  if (ctx.jacobi_sort) {
+    nrm = ctx.ws.alloc_n<double>(size_t(off[nj]));
+    perm = ctx.ws.alloc_n<int32_t>(size_t(off[nj]));
+    hn = static_cast<double*>(ctx.pin.alloc(sizeof(double) * size_t(off[nj])));
+    hp = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * size_t(off[nj])));
+    std::vector<SVDJob> nj_jobs = live;
+    for (int q = 0; q < nj; ++q) {
+      nj_jobs[q].sv = nrm + off[q];
+      tB[q] = ctx.ws.alloc_n<double>(size_t(live[q].p) * live[q].p);
+      tJ[q] = ctx.ws.alloc_n<double>(size_t(live[q].p) * live[q].p);
+    }
+    dn = ctx.upload(nj_jobs);
+  }
+  auto sort_columns = [&]() {
+    hipLaunchKernelGGL(column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, ctx.stream, dn);
+    TTR_HIP(hipGetLastError());
+    TTR_HIP(hipMemcpyAsync(hn, nrm, sizeof(double) * size_t(off[nj]), hipMemcpyDeviceToHost, ctx.stream));
+    ctx.sync();
+    std::vector<GatherCols> g;
+    std::vector<Copy2D> cp;
+    for (int q = 0; q < nj; ++q) {
+      const int p = live[q].p;
+      int32_t* o = hp + off[q];
+      const double* v = hn + off[q];
+      for (int c = 0; c < p; ++c) o[c] = c;
+      std::stable_sort(o, o + p, [&](int32_t a, int32_t b) { return v[a] > v[b]; });
+      bool ident = true;
+      for (int c = 0; c < p && ident; ++c) ident = o[c] == c;
+      if (ident) continue;
+      g.push_back(GatherCols{live[q].B, tB[q], live[q].ldb, p, perm + off[q], p, p});
+      g.push_back(GatherCols{live[q].J, tJ[q], live[q].ldj, p, perm + off[q], p, p});
+      cp.push_back(copy_job(tB[q], 1, p, live[q].B, 1, live[q].ldb, p, p));
+      cp.push_back(copy_job(tJ[q], 1, p, live[q].J, 1, live[q].ldj, p, p));
+    }
+    if (g.empty()) return;
+    TTR_HIP(hipMemcpyAsync(perm, hp, sizeof(int32_t) * size_t(off[nj]), hipMemcpyHostToDevice, ctx.stream));
+    gather_cols_batched(ctx, g);
+    copy2d_batched(ctx, cp);
+  };
   bool converged = false;
+  std::vector<int> rotating;  // matrices still rotating after each checked sweep
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     int32_t* cur = flags + size_t(sweep & 1) * nj;
     if (sweep >= 2) zero_fill(ctx, cur, sizeof(int32_t) * nj);
+    if (ctx.jacobi_sort) sort_columns();
     for (int r = 0; r < max_rounds; ++r) {
       // timed per launch; the work depends on the sweeps taken, so the caller
       // adds it from the counters
@@ -616,8 +671,10 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
     if (sweep >= 2 || sweep == max_sweeps - 1) {
       TTR_HIP(hipMemcpyAsync(hflags, cur, sizeof(int32_t) * nj, hipMemcpyDeviceToHost, ctx.stream));
       ctx.sync();
-      bool any = false;
-      for (int q = 0; q < nj && !any; ++q) any = hflags[q] != 0;
+      int cnt = 0;
+      for (int q = 0; q < nj; ++q) cnt += hflags[q] != 0;
+      rotating.push_back(cnt);
+      const bool any = cnt > 0;
       if (!any) {
         converged = true;
         break;
@@ -626,9 +683,19 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   }
   // a matrix that still rotated in the last sweep has no converged SVD: report it
   // (the reference's LAPACK driver either converges or throws, src/tt_rounding.jl:191-195)
-  if (!converged)
-    throw Error(ST_NUMERICAL_FAILURE,
-                "Jacobi SVD did not converge in " + std::to_string(max_sweeps) + " sweeps");
+  if (!converged) {
+    std::string msg = "Jacobi SVD did not converge in " + std::to_string(max_sweeps) +
+                      " sweeps (matrices still rotating after each sweep from the third:";
+    for (int c : rotating) msg += " " + std::to_string(c);
+    msg += "; sizes of the unconverged ones:";
+    int shown = 0;
+    for (int q = 0; q < nj && shown < 16; ++q)
+      if (hflags[q]) {
+        msg += " " + std::to_string(live[q].p);
+        ++shown;
+      }
+    throw Error(ST_NUMERICAL_FAILURE, msg + ")");
+  }
   hipLaunchKernelGGL(column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, ctx.stream, dl);
   TTR_HIP(hipGetLastError());
 }

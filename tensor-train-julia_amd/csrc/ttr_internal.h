@@ -169,6 +169,8 @@ struct Ctx {
   int jacobi_max_sweeps = 40;
   // Jacobi rotation threshold: |G_ij| > scale * max(p, 16) eps sqrt(G_ii G_jj)
   double jacobi_tol_scale = 1.0;
+  // sort the columns by decreasing norm before every Jacobi sweep (de Rijk)
+  bool jacobi_sort = true;
   // partial sums of split-K GEMMs, reused by every launch on the stream
   double* splitk = nullptr;
   size_t splitk_elems = 0;

@@ -206,6 +206,7 @@ struct RandOpts
     forward::Int32                    # 0: reverse = true (right-to-left), 1: reverse = false
     blocks::Ptr{Ptr{Float64}}         # caller's blocks per core, or C_NULL (library draws)
     blocks_right::Ptr{Ptr{Float64}}   # stta: the right sketch's blocks
+    sketch_accum::Int32               # 1: only the blocks in sketch precision, contractions FP64
 end
 
 # sketch precision: Float64, Float32, or BF16 (BFloat16s.BFloat16, or the symbol :bf16)
@@ -215,9 +216,9 @@ _prec(T) = haskey(PREC, T) ? PREC[T] :
            throw(ArgumentError("sketch precision must be Float64, Float32 or BFloat16"))
 
 _opts(orthogonal, T, block_rks, seed; p=Ptr{Int64}(C_NULL), forward=false, blocks=Ptr{Ptr{Float64}}(C_NULL),
-      blocks_right=Ptr{Ptr{Float64}}(C_NULL)) =
+      blocks_right=Ptr{Ptr{Float64}}(C_NULL), accum64::Bool=false) =
     Ref(RandOpts(orthogonal ? 1 : 0, _prec(T), block_rks, UInt64(seed), p, 1.0, Int32(forward ? 1 : 0), blocks,
-                 blocks_right))
+                 blocks_right, Int32(accum64 ? 1 : 0)))
 
 # The reference's sketch blocks for `seed`: default_rng seeded, then
 # generate_sketch_blocks core by core in the reference's order (right to left for the
@@ -259,7 +260,7 @@ end
 # alpha_1 (A) y_1 + sum_{j>1} alpha_j y_j; with A given the product is never formed
 function rand_round(α::Vector{Float64}, A, y::Vector{TTvector{Float64,N}}, rks;
                     orthogonal=true, seed=1234, block_rks::Int=N, sketch_type=Float64, reference_draws::Bool=true,
-                    timer=nothing) where {N}
+                    sketch_accum64::Bool=false, timer=nothing) where {N}
     ids = [upload(t) for t in y]
     op = A === nothing ? -1 : upload(A)
     r = collect(Int64, rks)
@@ -273,7 +274,7 @@ function rand_round(α::Vector{Float64}, A, y::Vector{TTvector{Float64,N}}, rks;
     GC.@preserve α ids r B bptr begin
         item = Ref(RandItem(length(y), pointer(α), pointer(ids), op, pointer(r)))
         opts = _opts(orthogonal, sketch_type, block_rks, seed;
-                     blocks=reference_draws ? pointer(bptr) : Ptr{Ptr{Float64}}(C_NULL))
+                     blocks=reference_draws ? pointer(bptr) : Ptr{Ptr{Float64}}(C_NULL), accum64=sketch_accum64)
         check(ccall((:ttr_round_rand, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, Ref{Int64}),
                     handle(), 1, item, opts, out))
     end

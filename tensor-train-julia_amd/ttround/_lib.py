@@ -53,7 +53,7 @@ class RandItem(C.Structure):
 class RandOpts(C.Structure):
     _fields_ = [("orthogonal", C.c_int32), ("sketch_prec", C.c_int32), ("block_rks", i64), ("seed", C.c_uint64),
                 ("p", i64p), ("oversampling", C.c_double), ("forward", C.c_int32), ("blocks", f64pp),
-                ("blocks_right", f64pp)]
+                ("blocks_right", f64pp), ("sketch_accum", C.c_int32)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/ttround.h

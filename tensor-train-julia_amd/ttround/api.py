@@ -59,9 +59,12 @@ def _block_ptrs(blocks, keep):
 
 
 def _opts(orthogonal=True, sketch_prec="f64", block_rks=None, seed=1234, p=None, oversampling=1.0, forward=False,
-          blocks=None, blocks_right=None):
+          blocks=None, blocks_right=None, sketch_accum="sketch"):
     """``blocks``: optional per-core sketch blocks exactly as ``generate_sketch_blocks``
-    returns them (``src/sketches.jl:44-84``), used instead of the library's draws."""
+    returns them (``src/sketches.jl:44-84``), used instead of the library's draws.
+    ``sketch_accum``: ``"sketch"`` runs the sketch contractions in ``sketch_prec``
+    (FP32 accumulation for FP32/BF16); ``"f64"`` rounds only the blocks to
+    ``sketch_prec`` and contracts in FP64."""
     o = L.RandOpts()
     o.forward = 1 if forward else 0
     o.orthogonal = 1 if orthogonal else 0
@@ -78,6 +81,9 @@ def _opts(orthogonal=True, sketch_prec="f64", block_rks=None, seed=1234, p=None,
     o.oversampling = float(oversampling)
     o.blocks = _block_ptrs(blocks, keep)
     o.blocks_right = _block_ptrs(blocks_right, keep)
+    if sketch_accum not in ("sketch", "f64"):
+        raise ValueError('sketch_accum must be "sketch" or "f64"')
+    o.sketch_accum = 1 if sketch_accum == "f64" else 0
     return o, keep
 
 
@@ -290,7 +296,8 @@ def dot_operator_batch(psis: Sequence[DeviceTT], ops, phis: Sequence[DeviceTT]) 
 
 
 def round_rand_batch(items, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
-                     sketch_prec: str = "f64", p=None, oversampling: float = 1.0, blocks=None) -> List[DeviceTT]:
+                     sketch_prec: str = "f64", p=None, oversampling: float = 1.0, blocks=None,
+                     sketch_accum: str = "sketch") -> List[DeviceTT]:
     """Batched randomized rounding of linear combinations.
 
     ``items`` is a list of ``(alphas, [DeviceTT...], rks)`` triples, or of
@@ -313,7 +320,7 @@ def round_rand_batch(items, orthogonal: bool = True, seed: int = 1234, block_rks
         arr[i].term_ids = L.ptr_i64(ids)
         arr[i].op_id = -1 if op is None else op.id
         arr[i].rks = L.ptr_i64(r)
-    o, k2 = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling, blocks=blocks)
+    o, k2 = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling, blocks=blocks, sketch_accum=sketch_accum)
     out = np.zeros(len(items), np.int64)
     h.check(h.lib.ttr_round_rand(h.h, len(items), arr, C.byref(o), L.ptr_i64(out)))
     return [DeviceTT(h, t) for t in out]
@@ -668,9 +675,14 @@ def tt_rounding(x, tol: float = 1e-12, rmax: int = 2 ** 14, direction: str = "le
 
 
 def tt_rounding_(x: TTvector, tol: float = 1e-12, rmax: int = 2 ** 14, direction: str = "left") -> None:
-    """``tt_rounding!``: replaces x's cores, ranks and orthogonality in place."""
+    """``tt_rounding!`` (``src/tt_rounding.jl:158-167``): when ``x`` is already
+    orthogonal in the sweep's direction its cores, ranks and orthogonality are
+    replaced by the rounding; otherwise the reference rounds an orthogonalised copy
+    and ``x`` keeps its values."""
+    orth = is_leftorthogonal(x) if direction == "left" else is_rightorthogonal(x)
     r = tt_rounding(x, tol, rmax, direction)
-    x.ttv_vec, x.ttv_rks, x.ttv_ot = r.ttv_vec, r.ttv_rks, r.ttv_ot
+    if orth:
+        x.ttv_vec, x.ttv_rks, x.ttv_ot = r.ttv_vec, r.ttv_rks, r.ttv_ot
 
 
 def default_rank_heuristic(y) -> List[int]:
@@ -697,7 +709,7 @@ def _rks_arg(rks, N):
 
 
 def ttrand_rounding(*args, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
-                    sketch_prec: str = "f64", blocks=None) -> TTvector:
+                    sketch_prec: str = "f64", blocks=None, sketch_accum: str = "sketch") -> TTvector:
     """Randomized rounding (``src/tt_randtools.jl:112-361``).
 
     Forms: ``ttrand_rounding(y)``, ``ttrand_rounding(y, rks_or_rmax)``,
@@ -753,7 +765,7 @@ def ttrand_rounding(*args, orthogonal: bool = True, seed: int = 1234, block_rks:
     dop = DeviceTTO(op) if op is not None else None
     item = (alpha, dop, devs, rks) if dop is not None else (alpha, devs, rks)
     out = round_rand_batch([item], orthogonal=orthogonal, seed=seed, block_rks=block_rks,
-                           sketch_prec=sketch_prec, blocks=blocks)[0]
+                           sketch_prec=sketch_prec, blocks=blocks, sketch_accum=sketch_accum)[0]
     res = out.to_host()
     for d in devs + [out] + ([dop] if dop is not None else []):
         d.free()

@@ -162,3 +162,24 @@ def test_caller_blocks_need_equal_block_counts(T):
         T.round_rand_batch([([1.0], [a], r1), ([1.0], [a], r2)], orthogonal=False, block_rks=1, blocks=blocks)
     a.free()
     assert L.default_handle() is not None
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16"])
+def test_sketch_accum_f64_is_the_oracle_on_rounded_blocks(T, prec):
+    """sketch_accum="f64": only the blocks are rounded to the sketch precision, the
+    contractions run in FP64, which is exactly what the oracle computes on the
+    rounded blocks: FP64 agreement (1e-12) where FP32/BF16 accumulation does not."""
+    say(f"case: {prec} blocks with FP64 accumulation vs the oracle on the same rounded blocks")
+    dims = (4,) * 10
+    ys = [rand_tt(dims, r, 360 + r) for r in (12, 20, 9)]
+    al = [1.0, 0.5, 0.25]
+    rks = [1, 4, 16, 24, 24, 24, 24, 24, 16, 4, 1]  # below the true ranks
+    got = {}
+    for acc in ("f64", "sketch"):
+        got[acc] = to_oracle(T.ttrand_rounding(al, [to_api(t) for t in ys], rks, orthogonal=False, seed=77,
+                                               sketch_prec=prec, sketch_accum=acc))
+    blocks = {k: T.sketch_blocks(dims, rks, k, orthogonal=False, seed=77, sketch_prec=prec) for k in range(10)}
+    ref = O.ttrand_rounding_lincomb(al, ys, rks, blocks.__getitem__, False)
+    e64, es = O.rel_err(ref, got["f64"]), O.rel_err(ref, got["sketch"])
+    say(f"  rel err vs oracle: FP64 accumulation {e64:.3e}, {prec} accumulation {es:.3e}")
+    assert got["f64"].rks == ref.rks and e64 < 1e-12
