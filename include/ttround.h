@@ -234,10 +234,13 @@ int ttr_round_rand(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const 
 int ttr_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts, double* const* W_out,
                int64_t* sketch_rks);
 
-/* The reverse recursive sketch of H x without forming the product: W[k]
+/* The recursive sketch of H x without forming the product: W[k]
  * ((r_k R_k) x sketch_rks[k], column-major, row index x rank + r_k * H rank).
+ * opts->forward selects the left-to-right sketch (FP64; n_k R_k <= 64; as in the
+ * reference, opts->oversampling is not applied on that path).
  * Replaces: tt_recursive_sketch(T, A::TToperator, x, rks; reverse=true, ...)
- * (src/sketches.jl:317-389,456-459,463-472). */
+ * (src/sketches.jl:317-389,456-459,463-472) and its reverse=false form
+ * (:390-459, contract_sketch_core_forwards! for operators :1240-1309). */
 int ttr_sketch_op(ttr_handle_t h, int64_t op_id, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts,
                   double* const* W_out, int64_t* sketch_rks);
 
@@ -287,6 +290,16 @@ int ttr_stta(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const ttr_ra
 int ttr_stta_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts,
                     uint64_t seed_right, double* const* Omega_out, double* const* Psi_out, int64_t* sl,
                     int64_t* sr);
+
+/* stta_sketch(x, L, R) with explicit sketch TTs L (left) and R (right), all of
+ * equal dims: Omega_out[k] (rL_{k+1} x rR_{k+1}, column-major, k = 0..d-2) =
+ * the left partial contraction of (x, L) through core k, transposed, times the
+ * right one of (x, R) from core k+1, and Psi_out[k] (reference layout
+ * (n_k, rL_k, rR_{k+1}), k = 0..d-1) = core k of x contracted with both; NULL
+ * entries are skipped.  FP64.
+ * Replaces: stta_sketch(x, L, R) (src/sketches.jl:1112-1124). */
+int ttr_stta_sketch_tt(ttr_handle_t h, int64_t x_id, int64_t l_id, int64_t r_id, double* const* Omega_out,
+                       double* const* Psi_out);
 
 /* Raw N(0,1) draws number 0..count-1 of core k of the sketch stream. */
 int ttr_sketch_draws(ttr_handle_t h, int64_t k, int64_t count, uint64_t seed, double* out);

@@ -33,7 +33,8 @@ __all__ = [
     "sketch_structure", "generate_sketch_blocks", "contract_sketch_core_backwards",
     "contract_sketch_core_backwards_op", "tt_recursive_sketch",
     "tt_recursive_sketch_op", "tt_sketch", "tt_combined_sketch", "sketch_structure_forward",
-    "contract_sketch_core_forwards", "tt_recursive_sketch_forward", "stable_inverse", "stta_sketch",
+    "contract_sketch_core_forwards", "tt_recursive_sketch_forward", "contract_sketch_core_forwards_op",
+    "tt_recursive_sketch_forward_op", "stable_inverse", "stta_sketch",
     "stta_sketch_tt", "stta_solve", "stta", "ttrand_rounding", "ttrand_rounding_lincomb",
     "ttrand_rounding_op", "ttrand_rounding_residual", "rel_err", "tt_dot", "dot",
     "dot_par", "partial_contraction", "dot_operator",
@@ -814,6 +815,55 @@ def tt_recursive_sketch_forward(A: TT, rks, draws, orthogonal: bool = True, bloc
     out = []
     for k in range(N + 1):
         out.append(jreshape(W[k] / math.sqrt(p[k]), (A.rks[k], bv[k] * p[k])))
+    return out, [bv[k] * p[k] for k in range(N + 1)]
+
+
+def contract_sketch_core_forwards_op(A: np.ndarray, H: np.ndarray, S: np.ndarray, V: np.ndarray) -> np.ndarray:
+    """``W[a,b,c] = sum A[ze,al,a] H[z,ze,be,b] S[ga,z,c] V[al,be,ga]`` (the forward
+    operator contraction, ``src/sketches.jl:1240-1309``) with the reference's
+    cost-chosen order: ((V A) H) S or ((V S) H) A."""
+    ze, al, a = A.shape
+    z, _, be, b = H.shape
+    ga, _, c = S.shape
+    if ze * a * al * be * ga + a * ga * ze * be * z * b + a * b * ga * z * c < \
+            al * be * ga * z * c + ze * b * z * be * al * c + a * ze * al * b * c:
+        Ap = jreshape(np.transpose(A, (0, 2, 1)), (ze * a, al))                       # [ze, a, al]
+        VA = jreshape(Ap @ jreshape(V, (al, be * ga)), (ze, a, be, ga))
+        VAp = jreshape(np.transpose(VA, (1, 3, 0, 2)), (a * ga, ze * be))            # [a, ga, ze, be]
+        Bp = jreshape(np.transpose(H, (1, 2, 0, 3)), (ze * be, z * b))                # [ze, be, z, b]
+        VAB = jreshape(VAp @ Bp, (a, ga, z, b))
+        VABp = jreshape(np.transpose(VAB, (0, 3, 1, 2)), (a * b, ga * z))             # [a, b, ga, z]
+        return jreshape(VABp @ jreshape(S, (ga * z, c)), (a, b, c))
+    VS = jreshape(jreshape(V, (al * be, ga)) @ jreshape(S, (ga, z * c)), (al, be, z, c))
+    VSp = jreshape(np.transpose(VS, (2, 1, 0, 3)), (z * be, al * c))                  # [z, be, al, c]
+    Bp = jreshape(np.transpose(H, (1, 3, 0, 2)), (ze * b, z * be))                    # [ze, b, z, be]
+    VBS = jreshape(Bp @ VSp, (ze, b, al, c))
+    Ap = jreshape(np.transpose(A, (2, 0, 1)), (a, ze * al))                           # [a, ze, al]
+    VBSp = jreshape(np.transpose(VBS, (0, 2, 1, 3)), (ze * al, b * c))               # [ze, al, b, c]
+    return jreshape(Ap @ VBSp, (a, b, c))
+
+
+def tt_recursive_sketch_forward_op(H: TTO, A: TT, rks, draws, orthogonal: bool = True,
+                                   block_rks: Optional[int] = None, p=None, oversampling=1):
+    """Forward recursive sketch of ``H A`` without forming it,
+    ``tt_recursive_sketch(T, H, A, rks; reverse=false)`` (``src/sketches.jl:390-459``).
+    The block structure is the forward one, and, as in the reference, no
+    oversampling is applied on this path (``:399-402``).  Returns ``W[k]`` of shape
+    ``(A.rks[k], H.rks[k], bv_k p_k)`` divided by ``sqrt(p_k)`` and the sketch ranks."""
+    N = A.N
+    dims = A.dims
+    assert rks[0] == 1 and A.rks[0] == 1 and H.rks[0] == 1
+    bv, p = sketch_structure_forward(dims, rks, block_rks, p, 1)
+    W: List[Optional[np.ndarray]] = [None] * (N + 1)
+    W[0] = np.ones((1, 1, 1, 1))
+    for k in range(N):
+        S = _sketch_for(draws, k, bv[k], dims[k], bv[k + 1], p[k + 1], orthogonal)
+        Wk = np.zeros((A.rks[k + 1], H.rks[k + 1], bv[k + 1], p[k + 1]))
+        for j in range(p[k + 1]):
+            Wk[:, :, :, j] = contract_sketch_core_forwards_op(A.cores[k], H.cores[k], S[:, :, :, j],
+                                                              W[k][:, :, :, j if k > 0 else 0])
+        W[k + 1] = Wk
+    out = [jreshape(W[k] / math.sqrt(p[k]), (A.rks[k], H.rks[k], bv[k] * p[k])) for k in range(N + 1)]
     return out, [bv[k] * p[k] for k in range(N + 1)]
 
 

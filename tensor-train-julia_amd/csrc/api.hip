@@ -1269,9 +1269,11 @@ struct FwdPlan {
 // applied where W_k is used), k = 1..kend; W_0 = 1
 struct FwdTerm {
   DevTT* x = nullptr;
+  DevTTO* H = nullptr;     // operator term: the sketch of H x (never formed)
   double alpha = 1.0;
   std::vector<int64_t> p;  // block counts of its item
-  std::vector<double*> W;
+  std::vector<double*> W;  // W_k: (rc(k) x bv_k p_k), rows (x rank, H rank) with x rank fastest
+  int64_t rc(int k) const { return x->rks[k] * (H ? H->rks[k] : 1); }
 };
 
 // generate_sketch_blocks(rng, T, bv_k, n_k, bv_{k+1}, p_{k+1}, orthogonal) for cores
@@ -1328,25 +1330,58 @@ void sketch_phase_fwd(ttr_handle_s* h, const FwdPlan& fp, std::vector<FwdTerm>& 
   GemmBatch gb;
   for (auto& t : terms) {
     t.W.assign(size_t(fp.d) + 1, nullptr);
-    for (int k = 1; k <= kend; ++k) t.W[k] = ctx.ws.alloc_n<double>(size_t(t.x->rks[k] * fp.bv[k] * t.p[k]));
+    for (int k = 1; k <= kend; ++k) t.W[k] = ctx.ws.alloc_n<double>(size_t(t.rc(k) * fp.bv[k] * t.p[k]));
   }
   for (int k = 0; k < kend; ++k) {
     const int64_t n = fp.dims[k], bl = fp.bv[k], br = fp.bv[k + 1];
-    std::vector<double*> T1(terms.size(), nullptr);
+    std::vector<double*> T1(terms.size(), nullptr), T2(terms.size(), nullptr);
     if (k > 0) {
+      // T1 = W_k[:, block j]^T X_k summed over the x rank: rows (H rank, bl), columns (n, r1)
       for (size_t i = 0; i < terms.size(); ++i) {
         const DevTT* x = terms[i].x;
         const int64_t r = x->rks[k], r1 = x->rks[k + 1], pk1 = terms[i].p[k + 1];
-        T1[i] = ctx.ws.alloc_n<double>(size_t(bl * n * r1 * pk1));
+        const int64_t R = terms[i].H ? terms[i].H->rks[k] : 1;
+        T1[i] = ctx.ws.alloc_n<double>(size_t(R * bl * n * r1 * pk1));
         for (int64_t j = 0; j < pk1; ++j)
-          gb.add(int(bl), int(n * r1), int(r), 1.0, terms[i].W[k] + r * bl * j, r, 1, x->cores[k], 1, r, 0.0,
-                 T1[i] + bl * n * r1 * j, 1, bl);
+          gb.add(int(R * bl), int(n * r1), int(r), 1.0, terms[i].W[k] + r * R * bl * j, r, 1, x->cores[k], 1, r, 0.0,
+                 T1[i] + R * bl * n * r1 * j, 1, R * bl);
       }
       gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "fwd_t1");
     }
+    // operator terms (contract_sketch_core_forwards!, src/sketches.jl:1240-1309): the
+    // operator core acts on the (mode, left H bond) pairs of T1,
+    // T2[(g, z), (a, b)] = sum_{zeta, beta} H[z, zeta, beta, b] T1[(beta, g), (zeta, a)]
+    std::vector<OpApply> oj;
+    for (size_t i = 0; i < terms.size(); ++i) {
+      const DevTTO* H = terms[i].H;
+      if (!H) continue;
+      const DevTT* x = terms[i].x;
+      TTR_REQUIRE(!H->wide(k), ST_INVALID_ARG, "the forward operator sketch needs n_k * R_k <= 64");
+      const int64_t R = H->rks[k], R1 = H->rks[k + 1], r1 = x->rks[k + 1];
+      const int64_t nb = k == 0 ? 1 : terms[i].p[k + 1];  // W_0 = 1 serves every block of core 0
+      T2[i] = ctx.ws.alloc_n<double>(size_t(bl * n * r1 * R1 * nb));
+      for (int64_t j = 0; j < nb; ++j) {
+        OpApply o = op_job(H, k, false);
+        o.in = k == 0 ? static_cast<const double*>(x->cores[0]) : T1[i] + R * bl * n * r1 * j;
+        o.out = T2[i] + bl * n * r1 * R1 * j;
+        o.X = int(bl);
+        o.Y = int(r1);
+        o.is_b = 1; o.is_x = R; o.is_m = R * bl; o.is_y = R * bl * n;
+        o.os_x = 1; o.os_m = bl; o.os_y = bl * n; o.os_b = bl * n * r1;
+        oj.push_back(o);
+      }
+    }
+    op_apply_batched(ctx, oj, Prec::F64, Prec::F64);
     for (size_t i = 0; i < terms.size(); ++i) {
       const DevTT* x = terms[i].x;
       const int64_t r1 = x->rks[k + 1], pk1 = terms[i].p[k + 1];
+      if (terms[i].H) {  // W_{k+1}[(a, b), c] = T2^T S_j over (g, z)
+        const int64_t R1 = terms[i].H->rks[k + 1];
+        for (int64_t j = 0; j < pk1; ++j)
+          gb.add(int(r1 * R1), int(br), int(bl * n), 1.0, T2[i] + (k == 0 ? 0 : bl * n * r1 * R1 * j), bl * n, 1,
+                 fp.S[k] + bl * n * br * j, 1, bl * n, 0.0, terms[i].W[k + 1] + r1 * R1 * br * j, 1, r1 * R1);
+        continue;
+      }
       for (int64_t j = 0; j < pk1; ++j) {
         const double* A = k == 0 ? x->cores[0] : T1[i] + bl * n * r1 * j;  // (bl n) x r1
         gb.add(int(r1), int(br), int(bl * n), 1.0, A, bl * n, 1, fp.S[k] + bl * n * br * j, 1, bl * n, 0.0,
@@ -2390,18 +2425,24 @@ void sketch_one(ttr_handle_s* h, DevTT* x, DevTTO* H, const int64_t* rks, const 
 
 namespace {
 // forward recursive sketch of one TT into host arrays (W_0 = 1)
-void sketch_one_fwd(ttr_handle_s* h, DevTT* x, const int64_t* rks, const ttr_rand_opts* opts, double* const* W_out,
-                    int64_t* sketch_rks) {
+void sketch_one_fwd(ttr_handle_s* h, DevTT* x, DevTTO* H, const int64_t* rks, const ttr_rand_opts* opts,
+                    double* const* W_out, int64_t* sketch_rks) {
   TTR_REQUIRE(rks != nullptr, ST_INVALID_ARG, "rks must not be NULL");
   TTR_REQUIRE(to_prec(opts->sketch_prec) == Prec::F64, ST_INVALID_ARG, "the forward sketch is FP64");
   const int d = x->d;
   TTR_REQUIRE(rks[0] == 1 && x->rks[0] == 1, ST_INVALID_ARG, "rks[0] must be 1");
+  if (H) {
+    TTR_REQUIRE(H->dims == x->dims, ST_DIM_MISMATCH, "operator and TT dims differ");
+    TTR_REQUIRE(H->rks[0] == 1 && H->rks[d] == 1, ST_INVALID_ARG, "operator boundary ranks must be 1");
+  }
   FwdPlan fp;
   prepare_fwd(opts, d, x->dims, fp);
   std::vector<int64_t> r(rks, rks + d + 1);
   FwdTerm t;
   t.x = x;
-  t.p = block_counts_fwd(r, fp.bv, d, opts->p, opts->oversampling);
+  t.H = H;
+  // the operator form takes no oversampling (src/sketches.jl:399-402)
+  t.p = block_counts_fwd(r, fp.bv, d, opts->p, H ? 1.0 : opts->oversampling);
   fp.pmax = t.p;
   std::vector<FwdTerm> ts{t};
   make_sketch_fwd(h, fp, d);
@@ -2411,7 +2452,7 @@ void sketch_one_fwd(ttr_handle_s* h, DevTT* x, const int64_t* rks, const ttr_ran
     const int64_t s = fp.bv[k] * ts[0].p[k];
     if (sketch_rks) sketch_rks[k] = s;
     if (!W_out || !W_out[k]) continue;
-    const int64_t cnt = x->rks[k] * s;
+    const int64_t cnt = ts[0].rc(k) * s;
     const double scale = 1.0 / std::sqrt(double(ts[0].p[k]));
     if (k == 0) {
       for (int64_t e = 0; e < cnt; ++e) W_out[k][e] = scale;
@@ -2427,7 +2468,7 @@ int ttr_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr_rand
                int64_t* sketch_rks) {
   return guarded(h, [&] {
     TTR_REQUIRE(opts != nullptr, ST_INVALID_ARG, "options must not be NULL");
-    if (opts->forward) sketch_one_fwd(h, h->get(tt_id), rks, opts, W_out, sketch_rks);
+    if (opts->forward) sketch_one_fwd(h, h->get(tt_id), nullptr, rks, opts, W_out, sketch_rks);
     else sketch_one(h, h->get(tt_id), nullptr, rks, opts, W_out, sketch_rks);
   });
 }
@@ -2476,9 +2517,72 @@ int ttr_stta_sketch(ttr_handle_t h, int64_t tt_id, const int64_t* rks, const ttr
   });
 }
 
+// stta_sketch(x, L, R) with explicit sketch TTs (src/sketches.jl:1112-1124): the
+// left environments of (x, L) and the right ones of (x, R) (partial_contraction,
+// batched GEMMs), then Omega_k = WL_{k+1}^T WR_{k+1} and
+// Psi_k = WL_k^T X_k WR_{k+1}, all on the device.
+int ttr_stta_sketch_tt(ttr_handle_t h, int64_t x_id, int64_t l_id, int64_t r_id, double* const* Omega_out,
+                       double* const* Psi_out) {
+  return guarded(h, [&] {
+    DevTT* x = h->get(x_id);
+    DevTT* L = h->get(l_id);
+    DevTT* R = h->get(r_id);
+    TTR_REQUIRE(x->dims == L->dims && x->dims == R->dims, ST_DIM_MISMATCH, "TT dimensions are not compatible");
+    const int d = x->d;
+    Ctx& ctx = h->ctx;
+    std::vector<std::vector<double*>> WL(1, std::vector<double*>(size_t(d + 1))), WR = WL;
+    for (int k = 0; k <= d; ++k) {
+      WL[0][size_t(k)] = k == 0 ? h->one : ctx.ws.alloc_n<double>(size_t(x->rks[k] * L->rks[k]));
+      WR[0][size_t(k)] = k == d ? h->one : ctx.ws.alloc_n<double>(size_t(x->rks[k] * R->rks[k]));
+    }
+    pair_environments(h, {x}, {L}, false, WL);
+    pair_environments(h, {x}, {R}, true, WR);
+    std::vector<double*> Om(static_cast<size_t>(std::max(d - 1, 1))), Ps(static_cast<size_t>(d)),
+        T(static_cast<size_t>(d));
+    GemmBatch gb;
+    for (int k = 0; k + 1 < d; ++k) {  // Omega_k (rL_{k+1} x rR_{k+1})
+      const int64_t z = x->rks[k + 1], a = L->rks[k + 1], b = R->rks[k + 1];
+      Om[size_t(k)] = ctx.ws.alloc_n<double>(size_t(a * b));
+      gb.add(int(a), int(b), int(z), 1.0, WL[0][size_t(k + 1)], z, 1, WR[0][size_t(k + 1)], 1, z, 0.0, Om[size_t(k)], 1,
+             a);
+    }
+    for (int k = 0; k < d; ++k) {  // T = WL_k^T X_k: (rL_k) x (n r_{k+1})
+      const int64_t y = x->rks[k], n = x->dims[k], z1 = x->rks[k + 1], a = L->rks[k];
+      T[size_t(k)] = ctx.ws.alloc_n<double>(size_t(a * n * z1));
+      gb.add(int(a), int(n * z1), int(y), 1.0, WL[0][size_t(k)], y, 1, x->cores[k], 1, y, 0.0, T[size_t(k)], 1, a);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "stta_tt_a");
+    for (int k = 0; k < d; ++k) {  // Psi_k = T (rL_k n) x r_{k+1} times WR_{k+1}: device layout (rL_k, n, rR_{k+1})
+      const int64_t n = x->dims[k], z1 = x->rks[k + 1], a = L->rks[k], b = R->rks[k + 1];
+      Ps[size_t(k)] = ctx.ws.alloc_n<double>(size_t(a * n * b));
+      gb.add(int(a * n), int(b), int(z1), 1.0, T[size_t(k)], 1, a * n, WR[0][size_t(k + 1)], 1, z1, 0.0,
+             Ps[size_t(k)], 1, a * n);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "stta_tt_b");
+    ctx.sync();
+    for (int k = 0; k + 1 < d; ++k)
+      if (Omega_out && Omega_out[k])
+        TTR_HIP(hipMemcpy(Omega_out[k], Om[size_t(k)], size_t(L->rks[k + 1] * R->rks[k + 1]) * 8,
+                          hipMemcpyDeviceToHost));
+    for (int k = 0; k < d; ++k) {
+      if (!Psi_out || !Psi_out[k]) continue;
+      const int64_t a = L->rks[k], nk = x->dims[k], b = R->rks[k + 1];
+      std::vector<double> raw(size_t(a * nk * b));
+      TTR_HIP(hipMemcpy(raw.data(), Ps[size_t(k)], raw.size() * 8, hipMemcpyDeviceToHost));
+      for (int64_t bb = 0; bb < b; ++bb)  // device (a, n, b) -> reference (n, a, b)
+        for (int64_t zz = 0; zz < nk; ++zz)
+          for (int64_t aa = 0; aa < a; ++aa) Psi_out[k][zz + nk * (aa + a * bb)] = raw[size_t(aa + a * (zz + nk * bb))];
+    }
+  });
+}
+
 int ttr_sketch_op(ttr_handle_t h, int64_t op_id, int64_t tt_id, const int64_t* rks, const ttr_rand_opts* opts,
                   double* const* W_out, int64_t* sketch_rks) {
-  return guarded(h, [&] { sketch_one(h, h->get(tt_id), h->get_op(op_id), rks, opts, W_out, sketch_rks); });
+  return guarded(h, [&] {
+    TTR_REQUIRE(opts != nullptr, ST_INVALID_ARG, "options must not be NULL");
+    if (opts->forward) sketch_one_fwd(h, h->get(tt_id), h->get_op(op_id), rks, opts, W_out, sketch_rks);
+    else sketch_one(h, h->get(tt_id), h->get_op(op_id), rks, opts, W_out, sketch_rks);
+  });
 }
 
 // tt_sketch(T, [H,] A, s; reverse=true, ...) (src/sketches.jl:645-723, operator

@@ -320,10 +320,10 @@ function _recursive_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N
                            orthogonal=true, reverse=true,
                            seed=1234, block_rks::Int=N, p=0, oversampling=1, reference_draws::Bool=true,
                            timer=nothing) where {T,N}
-    reverse || H === nothing || throw(ArgumentError("the GPU forward sketch takes no operator"))
     bv = reverse ? _block_ranks(A.ttv_dims, block_rks, N) : _block_ranks_fwd(A.ttv_dims, block_rks, N)
     pp = p == 0 ? TensorTrains.compute_sketch_blocks_heuristic(rks, bv, N; reverse=reverse) : collect(Int, p)
-    if oversampling != 1
+    # the forward operator sketch takes no oversampling (src/sketches.jl:399-402)
+    if oversampling != 1 && (reverse || H === nothing)
         rng = reverse ? (1:N) : (2:N+1)
         pp[rng] .= ceil.(Int, oversampling .* pp[rng])
     end
@@ -375,9 +375,10 @@ tt_recursive_sketch(::Type{T}, A::TTvector{Float64,N}, rks_or_rmax; reverse=true
                     kwargs...) where {T<:Union{Float64,Float32},N} =
     _recursive_sketch(T, nothing, A, reverse ? _rks_of(rks_or_rmax, N) : _rks_of_fwd(rks_or_rmax, N);
                       reverse=reverse, kwargs...)
-tt_recursive_sketch(::Type{T}, H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax;
+tt_recursive_sketch(::Type{T}, H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax; reverse=true,
                     kwargs...) where {T<:Union{Float64,Float32},N} =
-    _recursive_sketch(T, H, A, _rks_of(rks_or_rmax, N); kwargs...)
+    _recursive_sketch(T, H, A, reverse ? _rks_of(rks_or_rmax, N) : _rks_of_fwd(rks_or_rmax, N);
+                      reverse=reverse, kwargs...)
 tt_recursive_sketch(A::TTvector{Float64,N}, rks_or_rmax; kwargs...) where {N} =
     tt_recursive_sketch(Float64, A, rks_or_rmax; kwargs...)
 tt_recursive_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax; kwargs...) where {N} =
@@ -567,6 +568,21 @@ function stta_sketch(x::TTvector{Float64,N}, rks::Vector{Int}; seed_left=1234, s
                     sl, sr))
     end
     free(id)
+    return Ω, Ψ
+end
+
+# stta_sketch(x, L, R) with explicit sketch TTs (src/sketches.jl:1112-1124)
+function stta_sketch(x::TTvector{Float64,N}, L::TTvector{Float64,N}, R::TTvector{Float64,N}) where {N}
+    (x.ttv_dims == L.ttv_dims == R.ttv_dims) || throw(DimensionMismatch("TT dimensions are not compatible"))
+    Ω = [zeros(Float64, L.ttv_rks[k + 1], R.ttv_rks[k + 1]) for k in 1:N-1]
+    Ψ = [zeros(Float64, x.ttv_dims[k], L.ttv_rks[k], R.ttv_rks[k + 1]) for k in 1:N]
+    a, l, r = upload(x), upload(L), upload(R)
+    GC.@preserve Ω Ψ begin
+        check(ccall((:ttr_stta_sketch_tt, lib), Cint,
+                    (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}),
+                    handle(), a, l, r, [pointer(o) for o in Ω], [pointer(q) for q in Ψ]))
+    end
+    free(a); free(l); free(r)
     return Ω, Ψ
 end
 

@@ -73,6 +73,7 @@ SIGNATURES = {
     "ttr_hadamard": (C.c_int, [C.c_void_p, i64, i64p, C.c_int32, i64p]),
     "ttr_orthogonalize": (C.c_int, [C.c_void_p, i64, i64p, i64, i64p]),
     "ttr_dot_noalloc": (C.c_int, [C.c_void_p, i64, i64p, i64p, f64p]),
+    "ttr_stta_sketch_tt": (C.c_int, [C.c_void_p, i64, i64, i64, f64pp, f64pp]),
     "ttr_round_svd": (C.c_int, [C.c_void_p, i64, i64p, C.c_double, i64, C.c_int, i64p]),
     "ttr_norm": (C.c_int, [C.c_void_p, i64, i64p, f64p]),
     "ttr_dot": (C.c_int, [C.c_void_p, i64, i64p, i64p, f64p]),

@@ -791,14 +791,12 @@ def tt_recursive_sketch(*args, orthogonal: bool = True, reverse: bool = True, se
         H, A, rks = args
     else:
         H, (A, rks) = None, args
-    if not reverse and H is not None:
-        raise NotImplementedError("the forward sketch is provided for TT vectors only")
     if isinstance(rks, (int, np.integer)):
         r = [int(rks)] * (A.N + 1)
         r[A.N if reverse else 0] = 1
         rks = r
     if not reverse:
-        return _forward_sketch(A, rks, orthogonal, seed, block_rks, p, oversampling, blocks)
+        return _forward_sketch(A, rks, orthogonal, seed, block_rks, p, oversampling, blocks, H)
     dev = A._dev()
     h = dev.handle
     dop = DeviceTTO(H, h) if H is not None else None
@@ -888,22 +886,35 @@ def tt_combined_sketch(*args, reverse: bool = True, orthogonal: bool = True, see
     return sketch, W, srk
 
 
-def _forward_sketch(A, rks, orthogonal, seed, block_rks, p, oversampling, blocks=None):
-    """``tt_recursive_sketch(A, rks; reverse=false)`` (``src/sketches.jl:241-303``), FP64."""
+def _forward_sketch(A, rks, orthogonal, seed, block_rks, p, oversampling, blocks=None, H=None):
+    """``tt_recursive_sketch([H,] A, rks; reverse=false)`` (``src/sketches.jl:241-303``;
+    operator form ``:390-459``, which applies no oversampling), FP64.  The operator
+    form returns ``W[k]`` of shape ``(r_k, R_k, s_k)``."""
     dev = A._dev()
     h = dev.handle
+    if H is not None:
+        oversampling = 1.0
+    dop = DeviceTTO(H, h) if H is not None else None
     o, keep = _opts(orthogonal, "f64", block_rks, seed, p, oversampling, forward=True, blocks=blocks)
     ra = L.as_i64(rks)
     bv, pp = block_structure_forward(A.ttv_dims, rks, block_rks, p, oversampling)
     srk = [bv[k] * pp[k] for k in range(A.N + 1)]
-    Ws = [np.zeros((A.ttv_rks[k], srk[k]), order="F") for k in range(A.N + 1)]
+    Hr = list(H.tto_rks) if H is not None else [1] * (A.N + 1)
+    Ws = [np.zeros((A.ttv_rks[k] * Hr[k], srk[k]), order="F") for k in range(A.N + 1)]
     arr = (L.f64p * (A.N + 1))(*[L.ptr_f64(w) for w in Ws])
     out_srk = np.zeros(A.N + 1, np.int64)
     try:
-        h.check(h.lib.ttr_sketch(h.h, dev.id, L.ptr_i64(ra), C.byref(o), arr, L.ptr_i64(out_srk)))
+        if dop is None:
+            h.check(h.lib.ttr_sketch(h.h, dev.id, L.ptr_i64(ra), C.byref(o), arr, L.ptr_i64(out_srk)))
+        else:
+            h.check(h.lib.ttr_sketch_op(h.h, dop.id, dev.id, L.ptr_i64(ra), C.byref(o), arr, L.ptr_i64(out_srk)))
     finally:
         dev.free()
+        if dop is not None:
+            dop.free()
     assert [int(v) for v in out_srk] == srk
+    if H is not None:
+        Ws = [np.reshape(w, (A.ttv_rks[k], Hr[k], srk[k]), order="F") for k, w in enumerate(Ws)]
     return Ws, srk
 
 
@@ -920,7 +931,11 @@ def stta_sketch(x: TTvector, rks, seed_left: int = 1234, seed_right: int = 5678,
                 block_rks: Optional[int] = None, blocks_left=None, blocks_right=None):
     """``stta_sketch(x, rks; seed_left, seed_right, orthogonal, block_rks)``
     (``src/sketches.jl:1126-1156``).  Returns ``(Omega, Psi)``: ``Omega[k]`` of shape
-    ``(sl_{k+1}, sr_{k+1})`` and ``Psi[k]`` of shape ``(n_k, sl_k, sr_{k+1})``."""
+    ``(sl_{k+1}, sr_{k+1})`` and ``Psi[k]`` of shape ``(n_k, sl_k, sr_{k+1})``.
+    ``stta_sketch(x, L, R)`` with explicit sketch TTs (``:1112-1124``) is the same
+    pair for the partial contractions of (x, L) and (x, R)."""
+    if isinstance(rks, TTvector):
+        return _stta_sketch_tt(x, rks, seed_left)
     N = x.N
     rks = _rks_stta(rks, N)
     bvl, pl = block_structure_forward(x.ttv_dims, rks, block_rks, None, 1.5)
@@ -942,6 +957,24 @@ def stta_sketch(x: TTvector, rks, seed_left: int = 1234, seed_right: int = 5678,
     finally:
         dev.free()
     assert [int(v) for v in osl] == sl and [int(v) for v in osr] == sr
+    return Om, Ps
+
+
+def _stta_sketch_tt(x: TTvector, Lt: TTvector, Rt: TTvector):
+    if x.ttv_dims != Lt.ttv_dims or x.ttv_dims != Rt.ttv_dims:
+        raise L.DimensionMismatch("TT dimensions are not compatible")
+    N = x.N
+    Om = [np.zeros((Lt.ttv_rks[k + 1], Rt.ttv_rks[k + 1]), order="F") for k in range(N - 1)]
+    Ps = [np.zeros((x.ttv_dims[k], Lt.ttv_rks[k], Rt.ttv_rks[k + 1]), order="F") for k in range(N)]
+    devs = [t._dev() for t in (x, Lt, Rt)]
+    h = devs[0].handle
+    om_arr = (L.f64p * max(N - 1, 1))(*[L.ptr_f64(w) for w in Om])
+    ps_arr = (L.f64p * N)(*[L.ptr_f64(w) for w in Ps])
+    try:
+        h.check(h.lib.ttr_stta_sketch_tt(h.h, devs[0].id, devs[1].id, devs[2].id, om_arr, ps_arr))
+    finally:
+        for d in devs:
+            d.free()
     return Om, Ps
 
 

@@ -248,3 +248,39 @@ def test_ttrand_rounding_op_truncating_matches_projection():
     Y = O.ttv_to_tensor(y)
     assert np.linalg.norm(Y - X) > 0.01 * np.linalg.norm(X)
     np.testing.assert_allclose(Y, ref, rtol=1e-10, atol=1e-10 * np.abs(X).max())
+
+
+# ---------------------------------------------------------------- forward operator sketch
+
+@pytest.mark.parametrize("shape", [(3, 2, 4, 2, 3, 2, 5), (2, 6, 2, 4, 1, 6, 2)])
+def test_contract_forwards_op_both_orders(shape):
+    """Both cost-chosen orders of the forward operator contraction equal the
+    direct sum W[a,b,c] = sum A[ze,al,a] H[z,ze,be,b] S[ga,z,c] V[al,be,ga]."""
+    n, al, a, be, b, ga, c = shape
+    g = np.random.default_rng(sum(shape))
+    A, H = g.standard_normal((n, al, a)), g.standard_normal((n, n, be, b))
+    S, V = g.standard_normal((ga, n, c)), g.standard_normal((al, be, ga))
+    ref = np.einsum("xpa,zxqb,gzc,pqg->abc", A, H, S, V)
+    np.testing.assert_allclose(O.contract_sketch_core_forwards_op(A, H, S, V), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_forward_op_sketch_is_the_sketch_of_the_product():
+    """tt_recursive_sketch(H, x; reverse=false) equals the forward sketch of the
+    explicit H x (src/tt_operations.jl:170-194, operator rank fastest in its bond)
+    with the bond index reordered to x rank fastest; the operator form applies no
+    oversampling (src/sketches.jl:399-402)."""
+    g = np.random.default_rng(1)
+    dims = (2, 3, 2, 3)
+    x = rand_tt(dims, [1, 2, 3, 2, 1], 7)
+    H = O.TTO([g.standard_normal((n, n, r0, r1)) for n, r0, r1 in zip(dims, [1, 2, 2, 3], [2, 2, 3, 1])], dims,
+              [1, 2, 2, 3, 1])
+    rks = [1, 3, 4, 3, 1]
+    bv, p = O.sketch_structure_forward(dims, rks, 2)
+    draws = [g.standard_normal(bv[k] * dims[k] * bv[k + 1] * p[k + 1]) for k in range(4)]
+    W, s = O.tt_recursive_sketch_forward_op(H, x, rks, draws, False, 2, oversampling=2)
+    We, se = O.tt_recursive_sketch_forward(O.mpo_apply(H, x), rks, draws, False, 2)
+    assert s == se
+    for k in range(5):
+        r, R = x.rks[k], H.rks[k]
+        np.testing.assert_allclose(W[k], We[k].reshape(R, r, -1, order="F").transpose(1, 0, 2), rtol=1e-12,
+                                   atol=1e-12 * np.abs(We[k]).max())
