@@ -101,7 +101,7 @@ def measured_traffic(args, kernel):
     except (OSError, ValueError):
         return None
     cfg = rec.get("config", {})
-    if any(cfg.get(k) != getattr(args, k) for k in ("items", "sketch_prec")):
+    if any(cfg.get(k) != getattr(args, k) for k in ("items", "sketch_prec", "sketch_accum")):
         return None
     k = rec.get("kernels", {}).get(kernel)
     return None if k is None else k.get("hbm_bytes_per_launch")

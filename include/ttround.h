@@ -317,6 +317,22 @@ int ttr_profile_count(ttr_handle_t h, int64_t* count);
 int ttr_profile_get(ttr_handle_t h, int64_t i, char* name, int64_t name_len, double* ms, int64_t* launches,
                     double* flops, double* bytes);
 
+/* HIP graphs of ttr_round_rand (no counterpart in the reference, whose sweeps
+ * run eagerly; the control exists for callers that repeat one call shape, such
+ * as a time-stepping loop).  With graphs on, per call key (item count, TT
+ * dimensions and ranks, target ranks, options and seed) the first call runs
+ * eagerly, the second is captured into a graph with its own workspace, and
+ * later calls replay it: only the host side runs, writing the call's
+ * descriptors (TT addresses, coefficients) into pinned staging that the graph
+ * copies to the device.  Results equal the eager ones bit for bit.  Off by
+ * default (or TTR_GRAPHS=1 at ttr_create); ttr_graph_enable(h, 0) frees every
+ * cached graph.  Calls with profiling on run eagerly.  At most 8 keys are
+ * cached per handle (least recently used evicted); each holds its workspace. */
+int ttr_graph_enable(ttr_handle_t h, int on);
+/* Counters since ttr_create: graphs captured, replays, replays that ran eagerly
+ * because their launches differed from the capture; graphs now cached. */
+int ttr_graph_stats(ttr_handle_t h, int64_t* captures, int64_t* replays, int64_t* fallbacks, int64_t* cached);
+
 #ifdef __cplusplus
 }
 #endif

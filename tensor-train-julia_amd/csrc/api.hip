@@ -140,6 +140,36 @@ struct ttr_handle_s {
   void* scratch = nullptr;  // this device's scratch array (set in ttr_create)
   void* stage = nullptr;    // pinned host staging of the non-allocating entry points
 
+  // HIP graphs of ttr_round_rand, one per call shape (see round_rand_graphed)
+  struct GraphEntry {
+    int state = 0;            // 0: run once eagerly, 1: captured, -1: never graphed
+    size_t ws_need = 0, pin_need = 0;
+    size_t split_elems = 0;   // split-K partial sums: the first call's largest request
+    void* dev = nullptr;      // fixed device workspace of the graph
+    void* host = nullptr;     // fixed pinned staging its copy nodes read
+    size_t dev_cap = 0, host_cap = 0;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uint64_t> sig;  // uploads and launches recorded at capture
+    uint64_t last_use = 0;
+  };
+  bool graphs = false;
+  std::map<std::string, GraphEntry> gcache;
+  uint64_t gclock = 0;
+  int64_t g_captures = 0, g_replays = 0, g_fallbacks = 0;
+  static void graph_free(GraphEntry& e) {
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    if (e.dev) (void)hipFree(e.dev);
+    if (e.host) (void)hipHostFree(e.host);
+    e.exec = nullptr;
+    e.dev = e.host = nullptr;
+    e.dev_cap = e.host_cap = 0;
+  }
+  void graph_clear() {
+    (void)hipStreamSynchronize(ctx.stream);
+    for (auto& kv : gcache) graph_free(kv.second);
+    gcache.clear();
+  }
+
   DevTT* get(int64_t id) {
     auto it = tts.find(id);
     TTR_REQUIRE(it != tts.end(), ST_NOT_FOUND, "unknown TT id " + std::to_string(id));
@@ -1224,6 +1254,162 @@ std::vector<RandItem> parse_items(ttr_handle_s* h, int64_t n, const ttr_rand_ite
   return out;
 }
 
+// Key of a ttr_round_rand call: what fixes its launches and its workspace layout
+// (shapes, ranks, options, seed).  Coefficients, TT contents and TT addresses are
+// not part of it: they reach the kernels only through uploaded descriptors.
+std::string rand_call_key(const std::vector<RandItem>& items, const ttr_rand_opts* o) {
+  std::string k;
+  auto put = [&k](int64_t v) { k.append(reinterpret_cast<const char*>(&v), sizeof(v)); };
+  put(int64_t(items.size()));
+  for (const auto& it : items) {
+    put(int64_t(it.terms.size()));
+    for (const auto& t : it.terms) {
+      put(t.x->d);
+      for (int64_t v : t.x->dims) put(v);
+      for (int64_t v : t.x->rks) put(v);
+      put(t.H ? 1 : 0);
+      if (t.H)
+        for (int64_t v : t.H->rks) put(v);
+    }
+    for (int64_t v : it.rks) put(v);
+  }
+  int64_t ov;
+  std::memcpy(&ov, &o->oversampling, sizeof(ov));
+  put(o->orthogonal);
+  put(o->sketch_prec);
+  put(o->block_rks);
+  put(int64_t(o->seed));
+  put(ov);
+  put(o->forward);
+  put(o->sketch_accum);
+  put(o->blocks ? 1 : 0);
+  put(o->p ? 1 : 0);
+  if (o->p)
+    for (int q = 0; q <= items[0].terms[0].x->d; ++q) put(o->p[q]);
+  return k;
+}
+
+// ttr_round_rand through HIP graphs (handle option, off by default).  Per call
+// key: the first call runs eagerly and records how much workspace and pinned
+// staging it used; the second is captured (stream capture, relaxed mode) into a
+// graph whose workspace and staging are fixed regions owned by the cache entry,
+// then launched; later calls rerun the host side only, which writes the call's
+// descriptors (TT core addresses, coefficients, GEMM problems) into the same
+// pinned staging that the graph's copy nodes read, and launch the graph.  Such a
+// replay must record the same uploads (sizes and addresses) and the same
+// launches (kernel, grid, block and argument bytes) as the capture; if it does
+// not, the call runs eagerly and the key is not graphed again.  The split-K
+// partial sums take one region of the fixed workspace sized for the largest
+// request of the first call, so every GEMM splits K as in the eager call and the
+// results are the eager ones bit for bit.
+std::vector<std::unique_ptr<DevTT>> round_rand_graphed(ttr_handle_s* h, std::vector<RandItem>& items,
+                                                       const ttr_rand_opts* o) {
+  Ctx& ctx = h->ctx;
+  if (!h->graphs || ctx.prof.on || !o) return round_rand_batch(h, items, o);
+  const std::string key = rand_call_key(items, o);
+  auto found = h->gcache.find(key);
+  if (found == h->gcache.end()) {
+    constexpr size_t kMaxGraphs = 8;
+    while (h->gcache.size() >= kMaxGraphs) {  // evict the least recently used shape
+      auto lru = h->gcache.begin();
+      for (auto it = h->gcache.begin(); it != h->gcache.end(); ++it)
+        if (it->second.last_use < lru->second.last_use) lru = it;
+      TTR_HIP(hipStreamSynchronize(ctx.stream));
+      ttr_handle_s::graph_free(lru->second);
+      h->gcache.erase(lru);
+    }
+    found = h->gcache.emplace(key, ttr_handle_s::GraphEntry{}).first;
+  }
+  ttr_handle_s::GraphEntry& e = found->second;
+  e.last_use = ++h->gclock;
+  if (e.state < 0) return round_rand_batch(h, items, o);
+  if (e.state == 0 && e.ws_need == 0) {
+    ctx.splitk_peak = 0;
+    auto outs = round_rand_batch(h, items, o);
+    e.ws_need = ctx.ws.used() + ((ctx.splitk_peak * sizeof(double) + 255) & ~size_t(255)) + 256;
+    e.pin_need = ctx.pin.used() + 256;
+    e.split_elems = ctx.splitk_peak;
+    return outs;
+  }
+  // fixed regions, graph mode and capture are undone whatever happens
+  struct Scope {
+    Ctx& c;
+    bool capturing = false;
+    explicit Scope(Ctx& cx) : c(cx) {}
+    void end() {
+      if (capturing) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c.stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        capturing = false;
+      }
+      c.ws.clear_fixed();
+      c.pin.clear_fixed();
+      c.fsplit = nullptr;
+      c.fsplit_elems = 0;
+      c.gmode = GraphMode::Off;
+    }
+    ~Scope() { end(); }
+  } scope(ctx);
+  auto enter = [&](GraphMode m) {
+    ctx.ws.set_fixed(e.dev, e.dev_cap);
+    ctx.pin.set_fixed(e.host, e.host_cap);
+    ctx.fsplit = e.split_elems ? ctx.ws.alloc_n<double>(e.split_elems) : nullptr;
+    ctx.fsplit_elems = e.split_elems;
+    ctx.gsig.clear();
+    ctx.gmode = m;
+  };
+  auto eager_again = [&](std::vector<std::unique_ptr<DevTT>>& outs) {
+    scope.end();
+    for (auto& t : outs)
+      if (t) h->free_tt(t.get());
+    e.state = -1;
+    ttr_handle_s::graph_free(e);
+    ++h->g_fallbacks;
+    return round_rand_batch(h, items, o);
+  };
+  if (e.state == 0) {
+    e.dev_cap = e.ws_need;
+    e.host_cap = e.pin_need;
+    TTR_HIP(hipMalloc(&e.dev, e.dev_cap));
+    TTR_HIP(hipHostMalloc(&e.host, e.host_cap, hipHostMallocDefault));
+    enter(GraphMode::Capture);
+    TTR_HIP(hipStreamBeginCapture(ctx.stream, hipStreamCaptureModeRelaxed));
+    scope.capturing = true;
+    std::vector<std::unique_ptr<DevTT>> outs;
+    hipGraph_t g = nullptr;
+    hipError_t st = hipSuccess;
+    try {
+      outs = round_rand_batch(h, items, o);
+    } catch (const Error&) {
+      st = hipErrorUnknown;
+    }
+    scope.capturing = false;
+    const hipError_t ec = hipStreamEndCapture(ctx.stream, &g);
+    if (st == hipSuccess) st = ec;
+    hipGraphExec_t ex = nullptr;
+    if (st == hipSuccess) st = hipGraphInstantiateWithFlags(&ex, g, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (st != hipSuccess) {
+      (void)hipGetLastError();
+      return eager_again(outs);
+    }
+    e.exec = ex;
+    e.sig = ctx.gsig;
+    e.state = 1;
+    TTR_HIP(hipGraphLaunch(e.exec, ctx.stream));
+    ++h->g_captures;
+    return outs;
+  }
+  enter(GraphMode::Replay);
+  auto outs = round_rand_batch(h, items, o);
+  if (ctx.gsig != e.sig) return eager_again(outs);
+  TTR_HIP(hipGraphLaunch(e.exec, ctx.stream));
+  ++h->g_replays;
+  return outs;
+}
+
 // ------------------------------------------------------------ forward sketch
 // Block ranks of the forward (left-to-right) sketch (src/sketches.jl:244-248):
 // bv_0 = 1, bv_{k+1} = min(block_rks, n_k bv_k)
@@ -1726,6 +1912,8 @@ int ttr_create(ttr_handle_t* out, int device) {
       TTR_REQUIRE(v >= 1e-3 && v <= 1.0, ST_INVALID_ARG, "TTR_JACOBI_TOL_SCALE must be in [1e-3, 1]");
       h->ctx.jacobi_tol_scale = v;
     }
+    // TTR_GRAPHS=1: ttr_round_rand through HIP graphs (ttr_graph_enable)
+    if (const char* c = std::getenv("TTR_GRAPHS")) h->graphs = c[0] == '1';
     // TTR_JACOBI_SORT=0: no column sorting between Jacobi sweeps
     if (const char* c = std::getenv("TTR_JACOBI_SORT")) h->ctx.jacobi_sort = c[0] != '0';
     // TTR_JACOBI_MAX_SWEEPS=s (1..40): Jacobi sweeps before a non-converged SVD fails
@@ -1748,6 +1936,7 @@ int ttr_destroy(ttr_handle_t h) {
   if (!h) return TTR_INVALID_ARG;
   (void)hipSetDevice(h->ctx.device);
   (void)hipStreamSynchronize(h->ctx.stream);
+  h->graph_clear();
   for (auto& kv : h->tts) h->free_tt(kv.second.get());
   h->tts.clear();
   h->ops.clear();  // DevTTO frees its device copies
@@ -2372,7 +2561,7 @@ int ttr_round_rand(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const 
   return guarded(h, [&] {
     TTR_REQUIRE(out_ids, ST_INVALID_ARG, "bad arguments");
     std::vector<RandItem> its = parse_items(h, n, items);
-    auto outs = round_rand_batch(h, its, opts);
+    auto outs = round_rand_graphed(h, its, opts);
     for (int64_t i = 0; i < n; ++i) out_ids[i] = h->add(std::move(outs[i]));
   });
 }
@@ -2701,6 +2890,26 @@ int ttr_sketch_blocks(ttr_handle_t h, int64_t d, const int64_t* dims, const int6
       shape4[1] = n;
       shape4[2] = bvl;
       shape4[3] = p;
+    }
+  });
+}
+
+int ttr_graph_enable(ttr_handle_t h, int on) {
+  return guarded(h, [&] {
+    h->graphs = on != 0;
+    if (!h->graphs) h->graph_clear();
+  });
+}
+
+int ttr_graph_stats(ttr_handle_t h, int64_t* captures, int64_t* replays, int64_t* fallbacks, int64_t* cached) {
+  return guarded(h, [&] {
+    if (captures) *captures = h->g_captures;
+    if (replays) *replays = h->g_replays;
+    if (fallbacks) *fallbacks = h->g_fallbacks;
+    if (cached) {
+      int64_t c = 0;
+      for (auto& kv : h->gcache) c += kv.second.state == 1;
+      *cached = c;
     }
   });
 }

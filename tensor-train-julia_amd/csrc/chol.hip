@@ -184,7 +184,7 @@ void chol_certify_batched(Ctx& ctx, const std::vector<CholJob>& jobs) {
   double fl = 0.0;
   for (const auto& j : jobs) fl += double(j.m) * j.m * j.m / 3.0;
   ProfScope ps(ctx, "svd_certify", fl, 0.0);
-  hipLaunchKernelGGL(chol_certify_kernel, dim3(unsigned(jobs.size())), dim3(CT), 0, ctx.stream, d);
+  TTR_LAUNCH(ctx, chol_certify_kernel, dim3(unsigned(jobs.size())), dim3(CT), 0, d);
   TTR_HIP(hipGetLastError());
 }
 

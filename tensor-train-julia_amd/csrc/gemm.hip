@@ -380,14 +380,14 @@ template <class TA, class TB, class TC, class MT>
 void launch_scalar(Ctx& ctx, int cfg, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
   dim3 grid(ntiles), block(256);
   switch (cfg) {
-    case 0: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 64, 16>), grid, block, 0, ctx.stream, dp, np, ds); break;
-    case 1: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 64, 16>), grid, block, 0, ctx.stream, dp, np, ds); break;
-    case 2: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 128, 16>), grid, block, 0, ctx.stream, dp, np, ds); break;
-    case 3: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 128, 16>), grid, block, 0, ctx.stream, dp, np, ds); break;
-    case 4: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 64, 8>), grid, block, 0, ctx.stream, dp, np, ds); break;
-    case 5: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 64, 8>), grid, block, 0, ctx.stream, dp, np, ds); break;
-    case 6: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 64, 128, 8>), grid, block, 0, ctx.stream, dp, np, ds); break;
-    default: hipLaunchKernelGGL((gemm_scalar_kernel<TA, TB, TC, MT, 128, 128, 8>), grid, block, 0, ctx.stream, dp, np, ds); break;
+    case 0: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 64, 16>), grid, block, 0, dp, np, ds); break;
+    case 1: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 64, 16>), grid, block, 0, dp, np, ds); break;
+    case 2: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 128, 16>), grid, block, 0, dp, np, ds); break;
+    case 3: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 128, 16>), grid, block, 0, dp, np, ds); break;
+    case 4: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 64, 8>), grid, block, 0, dp, np, ds); break;
+    case 5: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 64, 8>), grid, block, 0, dp, np, ds); break;
+    case 6: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 128, 8>), grid, block, 0, dp, np, ds); break;
+    default: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 128, 8>), grid, block, 0, dp, np, ds); break;
   }
 }
 
@@ -396,7 +396,7 @@ template <class TA, class TB, class TC>
 void launch_combo(Ctx& ctx, Prec math, int cfg, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
   if (math == Prec::F64) launch_scalar<TA, TB, TC, double>(ctx, cfg, ntiles, dp, np, ds);
   else if (math == Prec::F32) launch_scalar<TA, TB, TC, float>(ctx, cfg, ntiles, dp, np, ds);
-  else hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB, TC>), dim3(ntiles), dim3(256), 0, ctx.stream, dp, np, ds);
+  else TTR_LAUNCH(ctx, (gemm_bf16_kernel<TA, TB, TC>), dim3(ntiles), dim3(256), 0, dp, np, ds);
 }
 
 void launch(Ctx& ctx, Prec ta, Prec tb, Prec tc, Prec math, int cfg, int ntiles, const GemmProb* dp, int np,
@@ -550,9 +550,9 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     int64_t mx = 1;
     for (int32_t i : split_ids) mx = std::max<int64_t>(mx, int64_t(probs[size_t(i)].M) * probs[size_t(i)].N);
     dim3 grid(unsigned(std::min<int64_t>(256, (mx + 255) / 256)), unsigned(split_ids.size())), block(256);
-    if (tc == Prec::F64) hipLaunchKernelGGL(gemm_splitk_reduce<double>, grid, block, 0, ctx.stream, dp, di);
-    else if (tc == Prec::F32) hipLaunchKernelGGL(gemm_splitk_reduce<float>, grid, block, 0, ctx.stream, dp, di);
-    else hipLaunchKernelGGL(gemm_splitk_reduce<bf16s>, grid, block, 0, ctx.stream, dp, di);
+    if (tc == Prec::F64) TTR_LAUNCH(ctx, gemm_splitk_reduce<double>, grid, block, 0, dp, di);
+    else if (tc == Prec::F32) TTR_LAUNCH(ctx, gemm_splitk_reduce<float>, grid, block, 0, dp, di);
+    else TTR_LAUNCH(ctx, gemm_splitk_reduce<bf16s>, grid, block, 0, dp, di);
     TTR_HIP(hipGetLastError());
   };
   static const char* names[3] = {"gemm_f64", "gemm_f32", "gemm_bf16"};

@@ -339,7 +339,7 @@ void copy2d_batched(Ctx& ctx, const std::vector<Copy2D>& jobs) {
   TTR_REQUIRE(tiles < (int64_t(1) << 31), ST_INVALID_ARG, "copy too large");
   const Copy2DDev* d = ctx.upload(dj);
   const int32_t* pre = ctx.upload(prefix);
-  hipLaunchKernelGGL(copy2d_kernel, dim3(unsigned(tiles)), dim3(256), 0, ctx.stream, d, pre, int(dj.size()));
+  TTR_LAUNCH(ctx, copy2d_kernel, dim3(unsigned(tiles)), dim3(256), 0, d, pre, int(dj.size()));
   TTR_HIP(hipGetLastError());
 }
 
@@ -352,7 +352,7 @@ static void launch_y(Ctx& ctx, void (*kern)(const J*), const std::vector<J>& job
     int64_t bx = (max_elems + 255) / 256;
     if (bx > 1024) bx = 1024;
     if (bx < 1) bx = 1;
-    hipLaunchKernelGGL(kern, dim3(unsigned(bx), unsigned(cnt)), dim3(256), 0, ctx.stream, d);
+    TTR_LAUNCH(ctx, kern, dim3(unsigned(bx), unsigned(cnt)), dim3(256), 0, d);
     TTR_HIP(hipGetLastError());
   }
 }
@@ -395,7 +395,7 @@ void mpo_apply_batched(Ctx& ctx, const std::vector<MpoCore>& jobs) {
 namespace {
 template <int MAXO, class TI, class TO>
 void launch_op_apply(Ctx& ctx, const OpApply* d, int64_t njobs, int64_t bx) {
-  hipLaunchKernelGGL((op_apply_kernel<MAXO, TI, TO>), dim3(unsigned(bx), unsigned(njobs)), dim3(256), 0, ctx.stream, d);
+  TTR_LAUNCH(ctx, (op_apply_kernel<MAXO, TI, TO>), dim3(unsigned(bx), unsigned(njobs)), dim3(256), 0, d);
 }
 template <class TI, class TO>
 void launch_op_apply_o(Ctx& ctx, int maxo, const OpApply* d, int64_t njobs, int64_t bx) {
@@ -442,9 +442,9 @@ void pow2_normalize_batched(Ctx& ctx, const std::vector<Pow2Job>& jobs, Prec pre
   if (jobs.empty()) return;
   const Pow2Job* d = ctx.upload(jobs);
   dim3 grid(unsigned(jobs.size())), block(256);
-  if (prec == Prec::F64) hipLaunchKernelGGL(pow2_kernel<double>, grid, block, 0, ctx.stream, d);
-  else if (prec == Prec::F32) hipLaunchKernelGGL(pow2_kernel<float>, grid, block, 0, ctx.stream, d);
-  else hipLaunchKernelGGL(pow2_kernel<uint16_t>, grid, block, 0, ctx.stream, d);
+  if (prec == Prec::F64) TTR_LAUNCH(ctx, pow2_kernel<double>, grid, block, 0, d);
+  else if (prec == Prec::F32) TTR_LAUNCH(ctx, pow2_kernel<float>, grid, block, 0, d);
+  else TTR_LAUNCH(ctx, pow2_kernel<uint16_t>, grid, block, 0, d);
   TTR_HIP(hipGetLastError());
 }
 
@@ -454,15 +454,14 @@ void sketch_generate(Ctx& ctx, const std::vector<SketchGen>& jobs, uint64_t seed
   for (auto& j : jobs) mx = std::max<int64_t>(mx, int64_t(j.n) * j.bvr * j.bvl * j.p);
   const SketchGen* d = ctx.upload(jobs);
   int64_t bx = std::min<int64_t>(1024, std::max<int64_t>(1, (mx + 255) / 256));
-  hipLaunchKernelGGL(sketch_gen_kernel, dim3(unsigned(bx), unsigned(jobs.size())), dim3(256), 0, ctx.stream, d, seed,
-                     int(prec));
+  TTR_LAUNCH(ctx, sketch_gen_kernel, dim3(unsigned(bx), unsigned(jobs.size())), dim3(256), 0, d, seed, int(prec));
   TTR_HIP(hipGetLastError());
 }
 
 void sketch_draws(Ctx& ctx, double* out, int64_t count, int core, uint64_t seed) {
   if (count <= 0) return;
   int64_t bx = std::min<int64_t>(4096, (count + 255) / 256);
-  hipLaunchKernelGGL(sketch_draws_kernel, dim3(unsigned(bx)), dim3(256), 0, ctx.stream, out, count, core, seed);
+  TTR_LAUNCH(ctx, sketch_draws_kernel, dim3(unsigned(bx)), dim3(256), 0, out, count, core, seed);
   TTR_HIP(hipGetLastError());
 }
 
@@ -470,7 +469,7 @@ void randn_core(Ctx& ctx, double* out, int n, int r, int r2, int core, uint64_t 
   const int64_t tot = int64_t(n) * r * r2;
   if (tot <= 0) return;
   int64_t bx = std::min<int64_t>(8192, (tot + 255) / 256);
-  hipLaunchKernelGGL(randn_core_kernel, dim3(unsigned(bx)), dim3(256), 0, ctx.stream, out, n, r, r2, core, seed);
+  TTR_LAUNCH(ctx, randn_core_kernel, dim3(unsigned(bx)), dim3(256), 0, out, n, r, r2, core, seed);
   TTR_HIP(hipGetLastError());
 }
 
@@ -478,8 +477,7 @@ void sketch_pack(Ctx& ctx, const double* ref, void* out, int n, int bvr, int bvl
   const int64_t tot = int64_t(n) * bvr * bvl * p;
   if (tot <= 0) return;
   int64_t bx = std::min<int64_t>(4096, (tot + 255) / 256);
-  hipLaunchKernelGGL(sketch_pack_kernel, dim3(unsigned(bx)), dim3(256), 0, ctx.stream, ref, out, n, bvr, bvl, p, scale,
-                     int(prec));
+  TTR_LAUNCH(ctx, sketch_pack_kernel, dim3(unsigned(bx)), dim3(256), 0, ref, out, n, bvr, bvl, p, scale, int(prec));
   TTR_HIP(hipGetLastError());
 }
 
@@ -487,7 +485,7 @@ void sum_arrays(Ctx& ctx, double* out, const std::vector<const double*>& ins, in
   if (count <= 0) return;
   const double* const* d = ctx.upload(ins.data(), ins.size());
   int64_t bx = std::min<int64_t>(4096, (count + 255) / 256);
-  hipLaunchKernelGGL(sum_kernel, dim3(unsigned(bx)), dim3(256), 0, ctx.stream, out, d, int(ins.size()), count);
+  TTR_LAUNCH(ctx, sum_kernel, dim3(unsigned(bx)), dim3(256), 0, out, d, int(ins.size()), count);
   TTR_HIP(hipGetLastError());
 }
 
@@ -497,7 +495,7 @@ void zero_fill(Ctx& ctx, void* p, size_t bytes) {
   TTR_REQUIRE((reinterpret_cast<uintptr_t>(p) & 3) == 0 && (bytes & 3) == 0, ST_INVALID_ARG, "zero_fill alignment");
   const int64_t words = int64_t(bytes / 4);
   int64_t bx = std::min<int64_t>(4096, (words + 255) / 256);
-  hipLaunchKernelGGL(zero_kernel, dim3(unsigned(bx)), dim3(256), 0, ctx.stream, static_cast<uint32_t*>(p), words);
+  TTR_LAUNCH(ctx, zero_kernel, dim3(unsigned(bx)), dim3(256), 0, static_cast<uint32_t*>(p), words);
   TTR_HIP(hipGetLastError());
 }
 

@@ -54,6 +54,13 @@ void DevArena::reset() {
   }
 }
 
+size_t DevArena::used() const {
+  if (fixed_) return fixed_used_;
+  size_t u = 0;
+  for (auto& b : blocks_) u += b.used;
+  return u;
+}
+
 void DevArena::release() {
   for (auto& b : blocks_) (void)hipFree(b.p);
   blocks_.clear();
@@ -102,6 +109,13 @@ void PinnedArena::reset() {
   } else if (!blocks_.empty()) {
     blocks_[0].used = 0;
   }
+}
+
+size_t PinnedArena::used() const {
+  if (fixed_) return fixed_used_;
+  size_t u = 0;
+  for (auto& b : blocks_) u += b.used;
+  return u;
 }
 
 void PinnedArena::release() {
@@ -228,6 +242,7 @@ void Profiler::collect() {
 namespace ttr {
 
 double* Ctx::splitk_buffer(size_t elems) {
+  splitk_peak = std::max(splitk_peak, elems);
   if (ws.fixed()) {
     // one region reused by every launch of the call (they run in stream order)
     if (elems <= fsplit_elems) return fsplit;

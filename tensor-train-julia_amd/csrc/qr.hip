@@ -314,17 +314,17 @@ __global__ __launch_bounds__(NBO) void build_t_kernel(const TBuildJob* __restric
 
 void launch_panel(Ctx& ctx, const PanelJob* d, int njobs, int max_rows, bool reg) {
   if (!reg) {
-    hipLaunchKernelGGL(qr_panel_kernel, dim3(unsigned(njobs)), dim3(PT), 0, ctx.stream, d);
+    TTR_LAUNCH(ctx, qr_panel_kernel, dim3(unsigned(njobs)), dim3(PT), 0, d);
   } else {
     const int R = (max_rows + RT - 1) / RT;
     switch (R) {
       case 0:
-      case 1: hipLaunchKernelGGL(qr_panel_reg_kernel<1>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
-      case 2: hipLaunchKernelGGL(qr_panel_reg_kernel<2>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
-      case 3: hipLaunchKernelGGL(qr_panel_reg_kernel<3>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
-      case 4: hipLaunchKernelGGL(qr_panel_reg_kernel<4>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
-      case 5: hipLaunchKernelGGL(qr_panel_reg_kernel<5>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
-      default: hipLaunchKernelGGL(qr_panel_reg_kernel<6>, dim3(unsigned(njobs)), dim3(RT), 0, ctx.stream, d); break;
+      case 1: TTR_LAUNCH(ctx, qr_panel_reg_kernel<1>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+      case 2: TTR_LAUNCH(ctx, qr_panel_reg_kernel<2>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+      case 3: TTR_LAUNCH(ctx, qr_panel_reg_kernel<3>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+      case 4: TTR_LAUNCH(ctx, qr_panel_reg_kernel<4>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+      case 5: TTR_LAUNCH(ctx, qr_panel_reg_kernel<5>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+      default: TTR_LAUNCH(ctx, qr_panel_reg_kernel<6>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
     }
   }
   TTR_HIP(hipGetLastError());
@@ -443,7 +443,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
     if (!tb.empty()) {
       const TBuildJob* d = ctx.upload(tb);
-      hipLaunchKernelGGL(build_t_kernel, dim3(unsigned(tb.size())), dim3(NBO), 0, ctx.stream, d);
+      TTR_LAUNCH(ctx, build_t_kernel, dim3(unsigned(tb.size())), dim3(NBO), 0, d);
       TTR_HIP(hipGetLastError());
     }
     // trailing update A2 -= V (T^T (V^T A2))

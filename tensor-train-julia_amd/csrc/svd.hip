@@ -567,7 +567,7 @@ __global__ __launch_bounds__(256) void pinv_weights_kernel(const PinvWeights* __
 void pinv_weights_batched(Ctx& ctx, const std::vector<PinvWeights>& jobs) {
   if (jobs.empty()) return;
   const PinvWeights* d = ctx.upload(jobs);
-  hipLaunchKernelGGL(pinv_weights_kernel, dim3(unsigned(jobs.size())), dim3(256), 0, ctx.stream, d);
+  TTR_LAUNCH(ctx, pinv_weights_kernel, dim3(unsigned(jobs.size())), dim3(256), 0, d);
   TTR_HIP(hipGetLastError());
 }
 
@@ -627,7 +627,7 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
     dn = ctx.upload(nj_jobs);
   }
   auto sort_columns = [&]() {
-    hipLaunchKernelGGL(column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, ctx.stream, dn);
+    TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, dn);
     TTR_HIP(hipGetLastError());
     TTR_HIP(hipMemcpyAsync(hn, nrm, sizeof(double) * size_t(off[nj]), hipMemcpyDeviceToHost, ctx.stream));
     ctx.sync();
@@ -662,8 +662,7 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
       // timed per launch; the work depends on the sweeps taken, so the caller
       // adds it from the counters
       ProfScope ps(ctx, "jacobi_svd");
-      hipLaunchKernelGGL(jacobi_round_kernel<WB>, dim3(unsigned(slots)), dim3(Jac<WB>::JT), 0, ctx.stream, dj, nj, r,
-                         sweep, flags);
+      TTR_LAUNCH(ctx, jacobi_round_kernel<WB>, dim3(unsigned(slots)), dim3(Jac<WB>::JT), 0, dj, nj, r, sweep, flags);
       TTR_HIP(hipGetLastError());
     }
     // convergence check from the third sweep on (no matrix converges earlier in
@@ -696,7 +695,7 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
       }
     throw Error(ST_NUMERICAL_FAILURE, msg + ")");
   }
-  hipLaunchKernelGGL(column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, ctx.stream, dl);
+  TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, dl);
   TTR_HIP(hipGetLastError());
 }
 

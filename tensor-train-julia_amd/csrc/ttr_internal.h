@@ -77,6 +77,8 @@ class DevArena {
   void clear_fixed() { fixed_ = nullptr; fixed_cap_ = fixed_used_ = 0; }
   bool fixed() const { return fixed_ != nullptr; }
   size_t fixed_free() const { return fixed_cap_ - fixed_used_; }
+  // bytes handed out since the last reset (the size a fixed region needs for the same call)
+  size_t used() const;
 
  private:
   struct Block { char* p; size_t cap; size_t used; };
@@ -94,6 +96,7 @@ class PinnedArena {
   void release();
   void set_fixed(void* base, size_t cap) { fixed_ = static_cast<char*>(base); fixed_cap_ = cap; fixed_used_ = 0; }
   void clear_fixed() { fixed_ = nullptr; fixed_cap_ = fixed_used_ = 0; }
+  size_t used() const;
 
  private:
   struct Block { char* p; size_t cap; size_t used; };
@@ -150,6 +153,12 @@ class Profiler {
   hipEvent_t get_event();
 };
 
+// HIP graphs of whole calls: Capture records the launches and descriptor
+// uploads of a call into a graph; Replay reruns the host logic of a call of the
+// same shape, writing its descriptors into the same pinned staging (the graph's
+// copy nodes read them) without launching anything, then launches the graph.
+enum class GraphMode : int { Off = 0, Capture = 1, Replay = 2 };
+
 struct Ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -179,6 +188,12 @@ struct Ctx {
   // with the workspace in fixed mode: taken from it, or nullptr when it does
   // not fit (the GEMM then runs without splitting K)
   double* splitk_buffer(size_t elems);
+  size_t splitk_peak = 0;  // largest request of the current call (elements)
+  GraphMode gmode = GraphMode::Off;
+  bool launching() const { return gmode != GraphMode::Replay; }
+  // host-side record of a captured or replayed call: sizes of every upload and
+  // every launch, compared between the capture and each replay
+  std::vector<uint64_t> gsig;
   // copies a host array into fresh device workspace (asynchronously, through pinned memory)
   template <class T>
   T* upload(const T* host, size_t n) {
@@ -186,7 +201,11 @@ struct Ctx {
     void* h = pin.alloc(n * sizeof(T));
     std::memcpy(h, host, n * sizeof(T));
     T* d = ws.alloc_n<T>(n);
-    TTR_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, stream));
+    if (gmode != GraphMode::Off) {
+      gsig.push_back(uint64_t(n * sizeof(T)));
+      gsig.push_back(uint64_t(reinterpret_cast<uintptr_t>(h)) ^ (uint64_t(reinterpret_cast<uintptr_t>(d)) << 1));
+    }
+    if (launching()) TTR_HIP(hipMemcpyAsync(d, h, n * sizeof(T), hipMemcpyHostToDevice, stream));
     return d;
   }
   template <class T>
@@ -195,12 +214,44 @@ struct Ctx {
   void end_call() { sync(); if (prof.on) prof.collect(); ws.reset(); pin.reset(); }
 };
 
-// RAII timing scope around one or more launches on ctx.stream
+// FNV-1a hash of a launch (kernel, grid, block, LDS bytes, argument bytes), kept
+// in ctx.gsig while a call is captured or replayed: a replay whose launches
+// differ in any of these from the captured ones is not run from the graph
+template <class... A>
+inline void graph_note_launch(Ctx& ctx, const void* kernel, dim3 grid, dim3 block, size_t shmem, const A&... a) {
+  uint64_t x = 0xcbf29ce484222325ull;
+  auto mix = [&x](const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) x = (x ^ c[i]) * 0x100000001b3ull;
+  };
+  mix(&kernel, sizeof(kernel));
+  mix(&grid, sizeof(grid));
+  mix(&block, sizeof(block));
+  mix(&shmem, sizeof(shmem));
+  (mix(&a, sizeof(A)), ...);
+  ctx.gsig.push_back(x);
+}
+
+// Kernel launch on ctx.stream.  During graph capture or replay the launch is
+// recorded in ctx.gsig; in replay it is not enqueued (the graph holds it).
+#define TTR_LAUNCH(ctx, kernel, grid, block, shmem, ...)                                                   \
+  do {                                                                                                     \
+    if ((ctx).gmode != ::ttr::GraphMode::Off)                                                              \
+      ::ttr::graph_note_launch((ctx), reinterpret_cast<const void*>(kernel), dim3(grid), dim3(block),      \
+                               size_t(shmem), __VA_ARGS__);                                                \
+    if ((ctx).launching()) {                                                                               \
+      hipLaunchKernelGGL(kernel, grid, block, shmem, (ctx).stream, __VA_ARGS__);                           \
+      TTR_HIP(hipGetLastError());                                                                          \
+    }                                                                                                      \
+  } while (0)
+
+// RAII timing scope around one or more launches on ctx.stream (off inside graphs)
 struct ProfScope {
   Ctx& ctx;
   int tok;
   ProfScope(Ctx& c, const char* name, double flops = 0.0, double bytes = 0.0, const char* name2 = nullptr)
-      : ctx(c), tok(c.prof.on && name ? c.prof.start(c.stream, name, flops, bytes, name2) : -1) {}
+      : ctx(c),
+        tok(c.prof.on && name && c.gmode == GraphMode::Off ? c.prof.start(c.stream, name, flops, bytes, name2) : -1) {}
   ~ProfScope() { if (tok >= 0) ctx.prof.stop(ctx.stream, tok); }
 };
 

@@ -88,6 +88,8 @@ SIGNATURES = {
     "ttr_stta_sketch": (C.c_int, [C.c_void_p, i64, i64p, C.POINTER(RandOpts), C.c_uint64, f64pp, f64pp, i64p, i64p]),
     "ttr_sketch_draws": (C.c_int, [C.c_void_p, i64, i64, C.c_uint64, f64p]),
     "ttr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "ttr_graph_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "ttr_graph_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p]),
     "ttr_profile_reset": (C.c_int, [C.c_void_p]),
     "ttr_profile_count": (C.c_int, [C.c_void_p, i64p]),
     "ttr_profile_get": (C.c_int, [C.c_void_p, i64, C.c_char_p, i64, f64p, i64p, f64p, f64p]),
@@ -144,6 +146,15 @@ class Handle:
 
     def profile(self, on: bool = True):
         self.check(self.lib.ttr_profile_enable(self.h, 1 if on else 0))
+
+    def graphs(self, on: bool = True):
+        """ttr_round_rand through cached HIP graphs (off frees them)."""
+        self.check(self.lib.ttr_graph_enable(self.h, 1 if on else 0))
+
+    def graph_stats(self) -> dict:
+        v = [C.c_int64() for _ in range(4)]
+        self.check(self.lib.ttr_graph_stats(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(("captures", "replays", "fallbacks", "cached"), (x.value for x in v)))
 
     def profile_reset(self):
         self.check(self.lib.ttr_profile_reset(self.h))
