@@ -333,6 +333,18 @@ void launch_panel(Ctx& ctx, const PanelJob* d, int njobs, int max_rows, bool reg
 }  // namespace
 
 void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
+  {
+    int mm = 0;
+    size_t bytes = 0;
+    for (const auto& J : jobs) {
+      mm = std::max(mm, J.m);
+      bytes += size_t(std::max(J.m, 0)) * size_t(std::max(J.n, 0)) * sizeof(double);
+    }
+    if (mm <= QR_SMALL_MAX_M && bytes <= ctx.qr_small_bytes) {
+      qr_small_batched(ctx, jobs);
+      return;
+    }
+  }
   struct Aux {
     double* tau;
     double* T;   // inner panels: npanel * NBP * NBP

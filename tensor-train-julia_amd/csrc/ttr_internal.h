@@ -180,6 +180,10 @@ struct Ctx {
   double jacobi_tol_scale = 1.0;
   // sort the columns by decreasing norm before every Jacobi sweep (de Rijk)
   bool jacobi_sort = true;
+  // QR batches whose matrices have at most QR_SMALL_MAX_M rows and whose total
+  // size is at most this many bytes run one workgroup per matrix (qr_small_batched);
+  // larger batches use the blocked path whose updates spread over the chip
+  size_t qr_small_bytes = size_t(96) << 20;
   // partial sums of split-K GEMMs, reused by every launch on the stream
   double* splitk = nullptr;
   size_t splitk_elems = 0;
@@ -329,6 +333,10 @@ struct QRJob {
 };
 // factorizes every job; forms thin Q into job.Q when it is non-null
 void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs);
+// the same in one launch, one workgroup per matrix (m <= QR_SMALL_MAX_M); used by
+// qr_batched for batches whose matrices all fit (see Ctx::qr_small_bytes)
+constexpr int QR_SMALL_MAX_M = 1024;
+void qr_small_batched(Ctx& ctx, const std::vector<QRJob>& jobs);
 
 // ---------------------------------------------------------------------------
 // one-sided Jacobi SVD of square matrices, batched
