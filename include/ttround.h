@@ -225,6 +225,31 @@ typedef struct {
 int ttr_round_rand(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const ttr_rand_opts* opts,
                    int64_t* out_ids);
 
+/* Randomized rounding of n entrywise (Hadamard) products x_1 .* ... .* x_M
+ * without forming them.  Item i has nfactors[i] factors (1..8, equal dims,
+ * rks[0] = rks[d] = 1); factor_ids lists the items' factors one after the other;
+ * rks holds d+1 target sketch ranks per item.  The product's bond k carries
+ * the factors' rank indices with the first factor fastest.  The right-to-left
+ * sketch contracts one factor at a time per mode index (cost n_k times the sum
+ * over factors of the rank products, instead of n_k P_k P_{k+1} for the formed
+ * product core); the left-to-right sweep keeps the left part as
+ * (r_k n_k) x prod_m r_{m,k+1} and applies the next cores factor by factor.
+ * Output as for ttr_round_rand (left-orthogonal, untruncated ranks).
+ * Replaces: ttrand_rounding(y::NTuple{M,TTvector}, rks; orthogonal, seed,
+ * block_rks) (src/tt_randtools.jl:364-458) and its rmax form (:460-466). */
+int ttr_round_rand_tuple(ttr_handle_t h, int64_t n, const int64_t* nfactors, const int64_t* factor_ids,
+                         const int64_t* rks, const ttr_rand_opts* opts, int64_t* out_ids);
+
+/* The right-to-left recursive sketch of the entrywise product of m TTs, never
+ * formed: W_out[k] (prod_f r_{f,k} x sketch_rks[k], column-major, the first
+ * factor's rank index fastest; NULL entries skipped), sketch ranks into
+ * sketch_rks (d+1).  opts->forward must be 0.
+ * Replaces: tt_recursive_sketch(T, A::NTuple, rks; reverse=true, ...)
+ * (src/sketches.jl:474-535,600-604) with contract_sketch_core_kronecker_backwards!
+ * (src/sketches.jl:1735-1881). */
+int ttr_sketch_tuple(ttr_handle_t h, int64_t m, const int64_t* factor_ids, const int64_t* rks,
+                     const ttr_rand_opts* opts, double* const* W_out, int64_t* sketch_rks);
+
 /* The recursive sketch of one TT: W[k] (r_k x sketch_rks[k], column-major,
  * k = 0..d) into W_out[k] (NULL entries skipped), the sketch ranks into
  * sketch_rks (d+1).  opts->forward selects the left-to-right sketch (FP64 only).

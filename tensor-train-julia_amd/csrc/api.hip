@@ -4,6 +4,7 @@
 // sweep step over the whole batch is a handful of grouped launches (GEMM, QR
 // panels, Jacobi, copies) on the handle's stream.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <mutex>
 #include <numeric>
@@ -2553,6 +2554,410 @@ int ttr_dot_operator(ttr_handle_t h, int64_t n, const int64_t* psi_ids, const in
     dot_operator_batch(h, ps, Hs, fs, res);
     h->ctx.sync();
     TTR_HIP(hipMemcpy(out, res, size_t(n) * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+namespace {
+// ------------------------------------------------- entrywise (Hadamard) products
+// One NTuple item: the entrywise product x_1 .* ... .* x_M of TTs of equal dims,
+// never formed.  Its bond k carries the factors' rank indices with the first
+// factor fastest (P_k = prod_m r_mk), the order of the reference's NTuple
+// methods (src/tt_randtools.jl:378-382, src/sketches.jl:574-575).
+struct TupleItem {
+  std::vector<DevTT*> f;
+  std::vector<int64_t> rks, p, s;  // target sketch ranks, block counts, sketch ranks (d+1)
+  std::vector<void*> W;            // W_k: P_k x s_k in the working precision
+  int32_t* E = nullptr;            // per core exponents of W (d+1)
+  int64_t P(int k) const {
+    int64_t v = 1;
+    for (auto* x : f) v *= x->rks[k];
+    return v;
+  }
+};
+
+// T1[(a_1..a_M), i, c] = sum_al prod_m A_m[a_m, i, al_m] V[(al_1..al_M), c] for
+// every item (V: P_{k+1} x ncol, working precision; T1: P_k x n x ncol), one
+// factor at a time (contract_sketch_core_kronecker_backwards!, src/sketches.jl:
+// 1735-1881, whose cost is n * sum_m (prod of ranks) instead of the product core's
+// n * P_k * P_{k+1}): factor m contracts its right rank index for every mode
+// index i as one problem per (i, remaining column) of a grouped GEMM.
+// Intermediates alternate between U[0] and U[1] (item-sized regions).
+void tuple_apply_cores_right(Ctx& ctx, std::vector<TupleItem>& its, int k, const std::vector<const void*>& V,
+                             const std::vector<int64_t>& ncol, const std::vector<void*>& T1,
+                             const std::vector<std::array<void*, 2>>& U, Prec wp, const char* site) {
+  const size_t eb = prec_bytes(wp);
+  size_t maxM = 0;
+  for (auto& it : its) maxM = std::max(maxM, it.f.size());
+  GemmBatch gb;
+  for (size_t m = 0; m < maxM; ++m) {
+    for (size_t q = 0; q < its.size(); ++q) {
+      TupleItem& it = its[q];
+      const size_t M = it.f.size();
+      if (m >= M) continue;
+      const int64_t n = it.f[0]->dims[k];
+      const DevTT* x = it.f[m];
+      const int64_t L = x->rks[k], R = x->rks[k + 1];
+      int64_t pre = 1, rest = ncol[q];
+      for (size_t j = 0; j < m; ++j) pre *= it.f[j]->rks[k];
+      for (size_t j = m + 1; j < M; ++j) rest *= it.f[j]->rks[k + 1];
+      const int64_t Pk = it.P(k);
+      const bool last = m + 1 == M;
+      const char* src = static_cast<const char*>(m == 0 ? V[q] : U[q][(m - 1) & 1]);
+      char* dst = last ? static_cast<char*>(T1[q]) : static_cast<char*>(U[q][m & 1]);
+      for (int64_t i = 0; i < n; ++i) {
+        const double* Ai = x->cores[k] + L * i;  // A(a, al) = core[a + L (i + n al)]
+        if (m == 0) {
+          // U_1[i] (L x rest) = A_i (L x R) V (R x rest)
+          if (last) gb.add(int(L), int(rest), int(R), 1.0, Ai, 1, L * n, src, 1, R, 0.0, dst + size_t(L * i) * eb, 1,
+                           L * n);
+          else gb.add(int(L), int(rest), int(R), 1.0, Ai, 1, L * n, src, 1, R, 0.0,
+                      dst + size_t(i * L * rest) * eb, 1, L);
+          continue;
+        }
+        // per remaining column r: C^T (L x pre) = A_i (L x R) U_slab^T (R x pre)
+        const char* Ui = src + size_t(i * pre * R * rest) * eb;
+        for (int64_t r = 0; r < rest; ++r) {
+          const char* slab = Ui + size_t(r * pre * R) * eb;
+          if (last) gb.add(int(L), int(pre), int(R), 1.0, Ai, 1, L * n, slab, pre, 1, 0.0,
+                           dst + size_t(Pk * (i + n * r)) * eb, pre, 1);
+          else gb.add(int(L), int(pre), int(R), 1.0, Ai, 1, L * n, slab, pre, 1, 0.0,
+                      dst + size_t(i * pre * L * rest + r * pre * L) * eb, pre, 1);
+        }
+      }
+    }
+    gemm_run(ctx, gb, Prec::F64, wp, wp, wp, site);
+  }
+}
+
+// Right-to-left recursive sketch of every item (tt_recursive_sketch(T,
+// A::NTuple, rks; reverse=true), src/sketches.jl:474-535): W_k from W_{k+1} with
+// the factored core contraction, then the block products with S_k as in the
+// plain sketch; W_k is normalised to powers of two per core (exponents in E).
+void tuple_sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<TupleItem>& its, int kmin) {
+  Ctx& ctx = h->ctx;
+  const int d = sp.d;
+  const Prec wp = sp.wprec;
+  const size_t eb = prec_bytes(wp), es = prec_bytes(sp.prec);
+  std::vector<void*> T1(its.size());
+  std::vector<std::array<void*, 2>> U(its.size());
+  std::vector<double*> last(its.size());
+  for (size_t q = 0; q < its.size(); ++q) {
+    TupleItem& it = its[q];
+    it.W.assign(size_t(d + 1), nullptr);
+    it.E = ctx.ws.alloc_n<int32_t>(size_t(d + 1));
+    zero_fill(ctx, it.E, size_t(d + 1) * 4);
+    int64_t t1 = 1, u = 1;
+    for (int k = kmin; k < d; ++k) {
+      it.W[size_t(k)] = ctx.ws.alloc(size_t(it.P(k) * it.s[size_t(k)]) * eb);
+      if (k + 1 < d) {
+        const int64_t n = sp.dims[k], cols = it.s[size_t(k + 1)];
+        t1 = std::max(t1, it.P(k) * n * cols);
+        // intermediate after factor m: n * (prod_{j<=m} L_j) * (prod_{j>m} R_j) * cols
+        for (size_t m = 0; m + 1 < it.f.size(); ++m) {
+          int64_t v = n * cols;
+          for (size_t j = 0; j <= m; ++j) v *= it.f[j]->rks[k];
+          for (size_t j = m + 1; j < it.f.size(); ++j) v *= it.f[j]->rks[k + 1];
+          u = std::max(u, v);
+        }
+      }
+    }
+    T1[q] = ctx.ws.alloc(size_t(t1) * eb);
+    U[q] = {ctx.ws.alloc(size_t(u) * eb), ctx.ws.alloc(size_t(u) * eb)};
+    last[q] = ctx.ws.alloc_n<double>(size_t(it.P(d - 1) * sp.dims[d - 1]));
+  }
+  // the last core of each product, formed explicitly (P_{d-1} x n x 1)
+  std::vector<KronCore> kc;
+  for (size_t q = 0; q < its.size(); ++q) {
+    KronCore J{};
+    J.out = last[q];
+    J.m = int32_t(its[q].f.size());
+    J.n = int32_t(sp.dims[d - 1]);
+    J.RA = int32_t(its[q].P(d - 1));
+    J.RB = 1;
+    for (size_t f = 0; f < its[q].f.size(); ++f) {
+      J.in[f] = its[q].f[f]->cores[d - 1];
+      J.ra[f] = int32_t(its[q].f[f]->rks[d - 1]);
+      J.rb[f] = 1;
+    }
+    kc.push_back(J);
+  }
+  kron_cores_batched(ctx, kc);
+  GemmBatch gb;
+  for (int k = d - 1; k >= kmin; --k) {
+    const int64_t n = sp.dims[k], bvr = sp.bv[k + 1], bvl = sp.bv[k];
+    const char* S = static_cast<const char*>(sp.S[k]);
+    if (k == d - 1) {
+      for (size_t q = 0; q < its.size(); ++q) {
+        const int64_t Pk = its[q].P(k);
+        for (int64_t j = 0; j < its[q].p[size_t(k)]; ++j)
+          gb.add(int(Pk), int(bvl), int(n), 1.0, last[q], 1, Pk, S + size_t(n * bvl * j) * es, 1, n, 0.0,
+                 static_cast<char*>(its[q].W[size_t(k)]) + size_t(Pk * bvl * j) * eb, 1, Pk);
+      }
+      gemm_run(ctx, gb, Prec::F64, sp.prec, wp, wp, "tuple_sketch_last");
+    } else {
+      std::vector<const void*> V;
+      std::vector<int64_t> cols;
+      for (auto& it : its) {
+        V.push_back(it.W[size_t(k + 1)]);
+        cols.push_back(it.s[size_t(k + 1)]);
+      }
+      tuple_apply_cores_right(ctx, its, k, V, cols, T1, U, wp, "tuple_sketch_cores");
+      for (size_t q = 0; q < its.size(); ++q) {
+        const int64_t Pk = its[q].P(k);
+        const char* A = static_cast<const char*>(T1[q]);
+        for (int64_t j = 0; j < its[q].p[size_t(k)]; ++j)
+          gb.add(int(Pk), int(bvl), int(n * bvr), 1.0, A + size_t(Pk * n * bvr * j) * eb, 1, Pk,
+                 S + size_t(n * bvr * bvl * j) * es, 1, n * bvr, 0.0,
+                 static_cast<char*>(its[q].W[size_t(k)]) + size_t(Pk * bvl * j) * eb, 1, Pk);
+      }
+      gemm_run(ctx, gb, wp, sp.prec, wp, wp, "tuple_sketch_w");
+    }
+    std::vector<Pow2Job> pj;
+    for (auto& it : its)
+      pj.push_back(Pow2Job{it.W[size_t(k)], it.P(k), int(it.P(k)), int(it.s[size_t(k)]),
+                           k == d - 1 ? nullptr : it.E + k + 1, it.E + k});
+    pow2_normalize_batched(ctx, pj, wp);
+  }
+}
+
+// Sketch plan and items of a batch of NTuple items (sketch structure from the
+// target ranks, as for plain TTs)
+std::vector<TupleItem> tuple_items(ttr_handle_s* h, int64_t n, const int64_t* nfactors, const int64_t* ids,
+                                   const int64_t* rks, const ttr_rand_opts* o, SketchPlan& sp) {
+  TTR_REQUIRE(n >= 1 && nfactors && ids && rks, ST_INVALID_ARG, "bad arguments");
+  std::vector<TupleItem> its(static_cast<size_t>(n));
+  std::vector<RandItem> plan(static_cast<size_t>(n));
+  int64_t off = 0;
+  const DevTT* x0 = nullptr;
+  for (int64_t i = 0; i < n; ++i) {
+    TTR_REQUIRE(nfactors[i] >= 1 && nfactors[i] <= KRON_MAX, ST_INVALID_ARG, "an item needs 1..8 factors");
+    for (int64_t f = 0; f < nfactors[i]; ++f) {
+      DevTT* x = h->get(ids[off++]);
+      if (!x0) x0 = x;
+      check_dims_equal(x0, x);
+      TTR_REQUIRE(x->rks[0] == 1 && x->rks[size_t(x->d)] == 1, ST_INVALID_ARG, "factors need rks[0] = rks[d] = 1");
+      its[size_t(i)].f.push_back(x);
+    }
+  }
+  const int d = x0->d;
+  for (int64_t i = 0; i < n; ++i) {
+    plan[size_t(i)].rks.assign(rks + i * (d + 1), rks + (i + 1) * (d + 1));
+    for (int k = 0; k <= d; ++k)
+      TTR_REQUIRE(its[size_t(i)].P(k) < (int64_t(1) << 31), ST_INVALID_ARG, "product rank too large");
+  }
+  prepare_plan(h, o, d, x0->dims, plan, sp);
+  for (int64_t i = 0; i < n; ++i) {
+    its[size_t(i)].rks = plan[size_t(i)].rks;
+    its[size_t(i)].p = plan[size_t(i)].p;
+    its[size_t(i)].s = plan[size_t(i)].s;
+  }
+  return its;
+}
+
+// ttrand_rounding(y::NTuple{M,TTvector}, rks) (src/tt_randtools.jl:364-458),
+// batched: the factored right-to-left sketch, then per core k (FP64)
+//   Z_k = y_k W_{k+1}, Q_k = qr(Z_k), Qy = Q_k^T y_k,
+//   y_{k+1}[(rho, i), (w_1..w_M)] = sum_l Qy[rho, (l_1..l_M)] prod_m A_m[l_m, i, w_m]
+// where the left part y_k ((r_k n_k) x P_{k+1}) carries the factors' rank
+// indices and the next cores are applied one factor at a time.
+std::vector<std::unique_ptr<DevTT>> round_rand_tuple_batch(ttr_handle_s* h, std::vector<TupleItem>& its,
+                                                           SketchPlan& sp) {
+  Ctx& ctx = h->ctx;
+  const int d = sp.d;
+  std::vector<std::unique_ptr<DevTT>> outs(its.size());
+  std::vector<std::vector<int64_t>> orr(its.size());
+  for (size_t q = 0; q < its.size(); ++q) {
+    std::vector<int64_t> r(size_t(d + 1), 1);
+    for (int k = 0; k + 1 < d; ++k) r[size_t(k + 1)] = std::min(sp.dims[k] * r[size_t(k)], its[q].s[size_t(k + 1)]);
+    std::vector<int64_t> ot(size_t(d), 1);
+    ot[size_t(d - 1)] = 0;
+    outs[q] = h->make_tt(sp.dims, r, ot);
+    orr[q] = r;
+  }
+  if (d == 1) {  // the product itself (one core)
+    std::vector<KronCore> kc;
+    for (size_t q = 0; q < its.size(); ++q) {
+      KronCore J{};
+      J.out = outs[q]->cores[0];
+      J.m = int32_t(its[q].f.size());
+      J.n = int32_t(sp.dims[0]);
+      J.RA = J.RB = 1;
+      for (size_t f = 0; f < its[q].f.size(); ++f) {
+        J.in[f] = its[q].f[f]->cores[0];
+        J.ra[f] = J.rb[f] = 1;
+      }
+      kc.push_back(J);
+    }
+    kron_cores_batched(ctx, kc);
+    return outs;
+  }
+  {
+    ProfScope ps(ctx, "phase:reverse_sketch");
+    make_sketch(h, sp, 1);
+    tuple_sketch_phase(h, sp, its, 1);
+  }
+  ProfScope ps_orth(ctx, "phase:orthogonalization");
+  struct Buf {
+    double* y[2];
+    double* Z;
+    double* Qy;
+    double* V[2];
+  };
+  std::vector<Buf> bf(its.size());
+  for (size_t q = 0; q < its.size(); ++q) {
+    const TupleItem& it = its[q];
+    const size_t M = it.f.size();
+    int64_t ymax = sp.dims[0] * it.P(1), zmax = 1, qmax = 1, vmax = 1;
+    for (int k = 0; k + 1 < d; ++k) {
+      const int64_t r = orr[q][size_t(k)], r1 = orr[q][size_t(k + 1)], n = sp.dims[k], P1 = it.P(k + 1);
+      ymax = std::max(ymax, r * n * P1);
+      zmax = std::max(zmax, r * n * it.s[size_t(k + 1)]);
+      qmax = std::max(qmax, r1 * P1);
+      const int kk = k + 1, n1 = int(sp.dims[kk]);
+      for (size_t m = 0; m + 1 < M; ++m) {  // after factor m at core kk
+        int64_t v = int64_t(n1) * r1;
+        for (size_t j = 0; j <= m; ++j) v *= it.f[j]->rks[kk + 1];
+        for (size_t j = m + 1; j < M; ++j) v *= it.f[j]->rks[kk];
+        vmax = std::max(vmax, v);
+      }
+    }
+    bf[q].y[0] = ctx.ws.alloc_n<double>(size_t(ymax));
+    bf[q].y[1] = ctx.ws.alloc_n<double>(size_t(ymax));
+    bf[q].Z = ctx.ws.alloc_n<double>(size_t(zmax));
+    bf[q].Qy = ctx.ws.alloc_n<double>(size_t(qmax));
+    bf[q].V[0] = ctx.ws.alloc_n<double>(size_t(vmax));
+    bf[q].V[1] = ctx.ws.alloc_n<double>(size_t(vmax));
+  }
+  // y_0 = the first core of each product (1 x n_0 x P_1)
+  {
+    std::vector<KronCore> kc;
+    for (size_t q = 0; q < its.size(); ++q) {
+      KronCore J{};
+      J.out = bf[q].y[0];
+      J.m = int32_t(its[q].f.size());
+      J.n = int32_t(sp.dims[0]);
+      J.RA = 1;
+      J.RB = int32_t(its[q].P(1));
+      for (size_t f = 0; f < its[q].f.size(); ++f) {
+        J.in[f] = its[q].f[f]->cores[0];
+        J.ra[f] = 1;
+        J.rb[f] = int32_t(its[q].f[f]->rks[1]);
+      }
+      kc.push_back(J);
+    }
+    kron_cores_batched(ctx, kc);
+  }
+  GemmBatch gb;
+  int yb = 0;
+  for (int k = 0; k + 1 < d; ++k) {
+    const int64_t n = sp.dims[k];
+    for (size_t q = 0; q < its.size(); ++q) {
+      const int64_t r = orr[q][size_t(k)], s = its[q].s[size_t(k + 1)], P1 = its[q].P(k + 1);
+      gb.add(int(r * n), int(s), int(P1), 1.0, bf[q].y[yb], 1, r * n, its[q].W[size_t(k + 1)], 1, P1, 0.0, bf[q].Z,
+             1, r * n);
+    }
+    gemm_run(ctx, gb, Prec::F64, sp.wprec, Prec::F64, Prec::F64, "tuple_rr_Z");
+    std::vector<QRJob> qj;
+    for (size_t q = 0; q < its.size(); ++q) {
+      const int64_t r = orr[q][size_t(k)], s = its[q].s[size_t(k + 1)];
+      qj.push_back(QRJob{bf[q].Z, r * n, int(r * n), int(s), outs[q]->cores[k], r * n});
+    }
+    qr_batched(ctx, qj);
+    // Qy = Q^T y_k (r_{k+1} x P_{k+1})
+    for (size_t q = 0; q < its.size(); ++q) {
+      const int64_t r = orr[q][size_t(k)], r1 = orr[q][size_t(k + 1)], P1 = its[q].P(k + 1);
+      gb.add(int(r1), int(P1), int(r * n), 1.0, outs[q]->cores[k], r * n, 1, bf[q].y[yb], 1, r * n, 0.0, bf[q].Qy, 1,
+             r1);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "tuple_rr_P");
+    // y_{k+1} (or the last output core): the factors' cores at k+1 applied to Qy
+    const int kk = k + 1;
+    const int64_t n1 = sp.dims[kk];
+    size_t maxM = 0;
+    for (auto& it : its) maxM = std::max(maxM, it.f.size());
+    for (size_t m = 0; m < maxM; ++m) {
+      for (size_t q = 0; q < its.size(); ++q) {
+        const TupleItem& it = its[q];
+        const size_t M = it.f.size();
+        if (m >= M) continue;
+        const int64_t r1 = orr[q][size_t(kk)];
+        const DevTT* x = it.f[m];
+        const int64_t L = x->rks[kk], Wm = x->rks[kk + 1];
+        int64_t pre = r1, rest = 1;
+        for (size_t j = 0; j < m; ++j) pre *= it.f[j]->rks[kk + 1];
+        for (size_t j = m + 1; j < M; ++j) rest *= it.f[j]->rks[kk];
+        const bool lastf = m + 1 == M;
+        const double* src = m == 0 ? bf[q].Qy : bf[q].V[(m - 1) & 1];
+        double* dst = lastf ? (kk == d - 1 ? outs[q]->cores[kk] : bf[q].y[yb ^ 1]) : bf[q].V[m & 1];
+        for (int64_t i = 0; i < n1; ++i) {
+          const double* Bi = x->cores[kk] + L * i;  // B(l, w) = core[l + L (i + n1 w)]
+          const double* Ai = m == 0 ? src : src + i * pre * L * rest;
+          if (!lastf) {
+            for (int64_t c = 0; c < rest; ++c)
+              gb.add(int(pre), int(Wm), int(L), 1.0, Ai + c * pre * L, 1, pre, Bi, 1, L * n1, 0.0,
+                     dst + i * pre * Wm * rest + c * pre * Wm, 1, pre);
+          } else {
+            // rows (rho, w_1..w_{M-1}) split by the prefix t, columns w_M
+            const int64_t npre = pre / r1;
+            for (int64_t t = 0; t < npre; ++t)
+              gb.add(int(r1), int(Wm), int(L), 1.0, Ai + t * r1, 1, pre, Bi, 1, L * n1, 0.0,
+                     dst + r1 * i + r1 * n1 * t, 1, r1 * n1 * npre);
+          }
+        }
+      }
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "tuple_rr_Y");
+    }
+    yb ^= 1;
+  }
+  return outs;
+}
+}  // namespace
+
+int ttr_round_rand_tuple(ttr_handle_t h, int64_t n, const int64_t* nfactors, const int64_t* factor_ids,
+                         const int64_t* rks, const ttr_rand_opts* opts, int64_t* out_ids) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(out_ids && opts, ST_INVALID_ARG, "bad arguments");
+    SketchPlan sp;
+    auto its = tuple_items(h, n, nfactors, factor_ids, rks, opts, sp);
+    auto outs = round_rand_tuple_batch(h, its, sp);
+    for (int64_t i = 0; i < n; ++i) out_ids[i] = h->add(std::move(outs[size_t(i)]));
+  });
+}
+
+int ttr_sketch_tuple(ttr_handle_t h, int64_t m, const int64_t* factor_ids, const int64_t* rks,
+                     const ttr_rand_opts* opts, double* const* W_out, int64_t* sketch_rks) {
+  return guarded(h, [&] {
+    TTR_REQUIRE(opts, ST_INVALID_ARG, "options must not be NULL");
+    SketchPlan sp;
+    const int64_t nf = m;
+    auto its = tuple_items(h, 1, &nf, factor_ids, rks, opts, sp);
+    const int d = sp.d;
+    make_sketch(h, sp, 0);
+    tuple_sketch_phase(h, sp, its, 0);
+    h->ctx.sync();
+    const TupleItem& it = its[0];
+    std::vector<int32_t> E(static_cast<size_t>(d + 1));
+    TTR_HIP(hipMemcpy(E.data(), it.E, size_t(d + 1) * 4, hipMemcpyDeviceToHost));
+    const size_t eb = prec_bytes(sp.wprec);
+    for (int k = 0; k <= d; ++k) {
+      if (sketch_rks) sketch_rks[k] = it.s[size_t(k)];
+      if (!W_out || !W_out[k]) continue;
+      const int64_t cnt = it.P(k) * it.s[size_t(k)];
+      const double scale = std::ldexp(1.0, k == d ? 0 : E[size_t(k)]) / std::sqrt(double(it.p[size_t(k)]));
+      if (k == d) {
+        for (int64_t e = 0; e < cnt; ++e) W_out[k][e] = scale;
+        continue;
+      }
+      std::vector<char> raw(size_t(cnt) * eb);
+      TTR_HIP(hipMemcpy(raw.data(), it.W[size_t(k)], raw.size(), hipMemcpyDeviceToHost));
+      for (int64_t e = 0; e < cnt; ++e) {
+        double v;
+        if (sp.wprec == Prec::F64) std::memcpy(&v, raw.data() + 8 * e, 8);
+        else if (sp.wprec == Prec::F32) { float f; std::memcpy(&f, raw.data() + 4 * e, 4); v = f; }
+        else { uint16_t b; std::memcpy(&b, raw.data() + 2 * e, 2); v = bf16_bits_to_float(b); }
+        W_out[k][e] = v * scale;
+      }
+    }
   });
 }
 

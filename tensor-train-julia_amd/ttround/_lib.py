@@ -89,6 +89,8 @@ SIGNATURES = {
     "ttr_sketch_draws": (C.c_int, [C.c_void_p, i64, i64, C.c_uint64, f64p]),
     "ttr_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "ttr_graph_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "ttr_round_rand_tuple": (C.c_int, [C.c_void_p, i64, i64p, i64p, i64p, C.POINTER(RandOpts), i64p]),
+    "ttr_sketch_tuple": (C.c_int, [C.c_void_p, i64, i64p, i64p, C.POINTER(RandOpts), C.POINTER(f64p), i64p]),
     "ttr_graph_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p]),
     "ttr_profile_reset": (C.c_int, [C.c_void_p]),
     "ttr_profile_count": (C.c_int, [C.c_void_p, i64p]),

@@ -326,6 +326,53 @@ def round_rand_batch(items, orthogonal: bool = True, seed: int = 1234, block_rks
     return [DeviceTT(h, t) for t in out]
 
 
+def round_rand_tuple_batch(items, orthogonal: bool = True, seed: int = 1234, block_rks: Optional[int] = None,
+                           sketch_prec: str = "f64", p=None, oversampling: float = 1.0, blocks=None,
+                           sketch_accum: str = "sketch") -> List[DeviceTT]:
+    """Batched randomized rounding of entrywise products, never formed
+    (``ttr_round_rand_tuple``).  ``items`` is a list of ``([DeviceTT...], rks)``
+    pairs: the factors of one product and its target sketch ranks."""
+    h = items[0][0][0].handle
+    nf = L.as_i64([len(f) for f, _ in items])
+    ids = L.as_i64([t.id for f, _ in items for t in f])
+    rk = L.as_i64([int(v) for _, r in items for v in r])
+    o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling, blocks=blocks,
+                    sketch_accum=sketch_accum)
+    out = np.zeros(len(items), np.int64)
+    h.check(h.lib.ttr_round_rand_tuple(h.h, len(items), L.ptr_i64(nf), L.ptr_i64(ids), L.ptr_i64(rk), C.byref(o),
+                                       L.ptr_i64(out)))
+    return [DeviceTT(h, t) for t in out]
+
+
+def _tuple_sketch(factors, rks, orthogonal, seed, block_rks, p, oversampling, sketch_prec, blocks=None):
+    """Right-to-left recursive sketch of the entrywise product of ``factors``
+    (TTvectors), never formed (``ttr_sketch_tuple``): ``W[k]`` of shape
+    ``(r_1k, ..., r_Mk, s_k)``."""
+    factors = list(factors)
+    N = factors[0].N
+    if any(t.ttv_dims != factors[0].ttv_dims for t in factors):
+        raise L.DimensionMismatch("all TTs of the product must share dims")
+    devs = [t._dev() for t in factors]
+    h = devs[0].handle
+    try:
+        o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling, blocks=blocks)
+        ra = L.as_i64(rks)
+        bv, pp = block_structure(factors[0].ttv_dims, rks, block_rks, p, oversampling)
+        srk = [bv[k] * pp[k] for k in range(N + 1)]
+        fr = [[t.ttv_rks[k] for t in factors] for k in range(N + 1)]
+        Ws = [np.zeros((int(np.prod(fr[k])), srk[k]), order="F") for k in range(N + 1)]
+        arr = (L.f64p * (N + 1))(*[L.ptr_f64(w) for w in Ws])
+        out_srk = np.zeros(N + 1, np.int64)
+        ids = L.as_i64([d.id for d in devs])
+        h.check(h.lib.ttr_sketch_tuple(h.h, len(devs), L.ptr_i64(ids), L.ptr_i64(ra), C.byref(o), arr,
+                                       L.ptr_i64(out_srk)))
+    finally:
+        for d in devs:
+            d.free()
+    assert [int(v) for v in out_srk] == srk
+    return [w.reshape(tuple(fr[k]) + (srk[k],), order="F") for k, w in enumerate(Ws)], srk
+
+
 def block_structure(dims, rks, block_rks: Optional[int] = None, p=None, oversampling: float = 1.0):
     """Block ranks ``bv`` and block counts ``p`` of the reverse sketch
     (``src/sketches.jl:107-123,186-198``)."""
@@ -731,14 +778,22 @@ def ttrand_rounding(*args, orthogonal: bool = True, seed: int = 1234, block_rks:
             raise TypeError("ttrand_rounding(y::Tuple, rks_or_rmax): ranks are required")
         if len(args[0]) == 1:
             return ttrand_rounding(args[0][0], args[1], orthogonal=orthogonal, seed=seed, block_rks=block_rks,
-                                   sketch_prec=sketch_prec)
-        dev = _hadamard_dev(args[0])
-        rks = _rks_arg(args[1], dev.d)
-        out = round_rand_batch([([1.0], [dev], rks)], orthogonal=orthogonal, seed=seed, block_rks=block_rks,
-                               sketch_prec=sketch_prec)[0]
+                                   sketch_prec=sketch_prec, blocks=blocks, sketch_accum=sketch_accum)
+        ys = list(args[0])
+        if any(not isinstance(t, TTvector) for t in ys):
+            raise TypeError("a tuple argument must hold TTvectors")
+        if any(t.ttv_dims != ys[0].ttv_dims for t in ys):
+            raise L.DimensionMismatch("all TTs of the product must share dims")
+        rks = _rks_arg(args[1], ys[0].N)
+        devs = [t._dev() for t in ys]
+        try:
+            out = round_rand_tuple_batch([(devs, rks)], orthogonal=orthogonal, seed=seed, block_rks=block_rks,
+                                         sketch_prec=sketch_prec, blocks=blocks, sketch_accum=sketch_accum)[0]
+        finally:
+            for d in devs:
+                d.free()
         res = out.to_host()
-        for d in (dev, out):
-            d.free()
+        out.free()
         return res
     if isinstance(args[0], TTvector):
         y = args[0]
@@ -783,6 +838,14 @@ def tt_recursive_sketch(*args, orthogonal: bool = True, reverse: bool = True, se
     ``(r_k, R_k, sketch_rks[k])`` for the operator form, or
     ``(r_1k, ..., r_Mk, sketch_rks[k])`` for a tuple of M TTs."""
     if isinstance(args[0], tuple):
+        if reverse and len(args[0]) > 1:  # factored: the product is never formed
+            N = args[0][0].N
+            rks = args[1]
+            if isinstance(rks, (int, np.integer)):
+                rks = [int(rks)] * (N + 1)
+                rks[N] = 1
+            return _tuple_sketch(args[0], [int(v) for v in rks], orthogonal, seed, block_rks, p, oversampling,
+                                 sketch_prec, blocks)
         P = _Product(args[0])
         W, srk = tt_recursive_sketch(P, *args[1:], orthogonal=orthogonal, reverse=reverse, seed=seed,
                                      block_rks=block_rks, p=p, oversampling=oversampling, sketch_prec=sketch_prec)
@@ -834,10 +897,20 @@ def tt_sketch(*args, reverse: bool = True, orthogonal: bool = True, seed: int = 
     Returns ``(sketch, s_actual)``: the boundary sketch of length
     ``s_actual = bv_1 * cld(s, bv_1)``."""
     H, (A, s) = _split_op(args)
-    if isinstance(A, tuple):  # NTuple form (src/sketches.jl:819-909): the entrywise product
-        A = _Product(A)
     if not reverse:
         raise NotImplementedError("only the reverse (right-to-left) sketch is provided")
+    if isinstance(A, tuple) and len(A) > 1:
+        # NTuple form (src/sketches.jl:819-909): the recursive sketch of the
+        # product with p = cld(s, bv_1) blocks at every core, never formed
+        if H is not None:
+            raise TypeError("no operator form for a tuple of TTs")
+        N = A[0].N
+        bv, _ = block_structure(A[0].ttv_dims, [1] * (N + 1), block_rks)
+        pc = -(-int(s) // bv[0])
+        W, _ = _tuple_sketch(A, [1] * (N + 1), orthogonal, seed, block_rks, [pc] * N + [1], 1.0, sketch_prec)
+        return np.reshape(W[0], -1, order="F").copy(), bv[0] * pc
+    if isinstance(A, tuple):
+        A = A[0]
     if H is not None and tuple(H.tto_dims) != tuple(A.ttv_dims):
         raise L.DimensionMismatch("operator and TT dims differ")
     dev = A._dev()

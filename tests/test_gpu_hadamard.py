@@ -1,8 +1,10 @@
 """Entrywise (Hadamard) products on the device and the NTuple methods that
 round or sketch them, against the oracle (FP64: 1e-12 relative, products of
 two factors bit-exact).  The oracle's NTuple sketch and rounding work factor by
-factor, never forming the product; the device builds the product core by core
-(``ttr_hadamard``) and runs the plain TT path on it."""
+factor, never forming the product, and so does the device
+(``ttr_round_rand_tuple``, ``ttr_sketch_tuple``); ``ttr_hadamard`` forms the
+product core by core, and rounding that product with the same draws must give
+the factored result.  The forward tuple sketch runs on the formed product."""
 import sys
 
 import numpy as np
@@ -137,3 +139,82 @@ def test_hadamard_errors(T):
     with pytest.raises(Exception):
         T.hadamard_device([dx] * 9)
     dx.free()
+
+
+def _formed_round(T, xs, rks, **kw):
+    dev = [T.DeviceTT.upload(to_api(t)) for t in xs]
+    prod = T.hadamard_device(dev, first_fastest=True)
+    out = T.round_rand_batch([([1.0], [prod], rks)], **kw)[0]
+    res = to_oracle(out.to_host())
+    for d in dev + [prod, out]:
+        d.free()
+    return res
+
+
+def _factored_round(T, items, **kw):
+    devs = [[T.DeviceTT.upload(to_api(t)) for t in xs] for xs, _ in items]
+    outs = T.round_rand_tuple_batch([(d, r) for d, (_, r) in zip(devs, items)], **kw)
+    res = [to_oracle(o.to_host()) for o in outs]
+    for d in [x for ds in devs for x in ds] + outs:
+        d.free()
+    return res
+
+
+def test_factored_equals_formed_product(T):
+    """Two items in one batch (2 and 3 factors): the factored rounding equals the
+    rounding of the formed product on the same draws, and the factored oracle."""
+    dims = (3, 4, 3, 4, 3)
+    a = [rand_tt(dims, [1, 3, 9, 12, 3, 1], 360), rand_tt(dims, [1, 3, 8, 7, 3, 1], 361)]
+    b = [rand_tt(dims, [1, 2, 4, 5, 3, 1], 362 + j) for j in range(3)]
+    ra, rb = [1, 3, 12, 24, 9, 1], [1, 3, 10, 20, 8, 1]
+    for prec, acc in (("f64", "sketch"), ("f32", "f64")):
+        say(f"case: factored vs formed product, sketch {prec}, accumulation {acc}")
+        got = _factored_round(T, [(a, ra), (b, rb)], orthogonal=False, seed=21, sketch_prec=prec, sketch_accum=acc)
+        for xs, r, g in ((a, ra, got[0]), (b, rb, got[1])):
+            formed = _formed_round(T, xs, r, orthogonal=False, seed=21, sketch_prec=prec, sketch_accum=acc)
+            blocks = {k: T.sketch_blocks(dims, r, k, orthogonal=False, seed=21, sketch_prec=prec) for k in range(5)}
+            ref = O.ttrand_rounding_tuple(xs, r, blocks.__getitem__, False)
+            e1, e2 = O.rel_err(formed, g), O.rel_err(ref, g)
+            say(f"  {len(xs)} factors: vs formed product {e1:.3e}, vs factored oracle {e2:.3e}")
+            assert g.rks == formed.rks == ref.rks
+            assert e1 < 1e-12 and e2 < 1e-12
+
+
+def _entry(tt, idx):
+    v = np.ones((1, 1))
+    for k, i in enumerate(idx):
+        v = v @ tt.cores[k][i]
+    return float(v[0, 0])
+
+
+def test_product_too_large_to_form(T):
+    """Three factors with representation ranks 100 (each 100 scaled copies of one
+    rank-1 TT): the product has ranks 10^6, its cores far beyond 2^31 entries,
+    so ttr_hadamard refuses it; the factored rounding to ranks 2 recovers the
+    (rank-1) product, checked entry by entry at random multi-indices."""
+    say("case: NTuple rounding of a product with ranks 10^6")
+    dims = (8,) * 6
+    rng = np.random.default_rng(370)
+    facs = []
+    for f in range(3):
+        y = O.rand_tt(dims, 1, rng)
+        c = rng.uniform(0.5, 1.5, 100) / 100.0
+        x = O.tt_lincomb([y] * 100, list(c))
+        assert max(x.rks) == 100
+        facs.append(x)
+    dev = [T.DeviceTT.upload(to_api(t)) for t in facs]
+    with pytest.raises(ValueError):
+        T.hadamard_device(dev, first_fastest=True)
+    for d in dev:
+        d.free()
+    rks = [1] + [2] * 5 + [1]
+    out = _factored_round(T, [(facs, rks)], orthogonal=True, seed=4)[0]
+    bv, p = O.sketch_structure(dims, rks, None)
+    assert out.rks == [1] + [min(8 ** (k + 1), bv[k + 1] * p[k + 1]) for k in range(5)] + [1]
+    worst = 0.0
+    for _ in range(200):
+        idx = tuple(int(i) for i in rng.integers(0, 8, 6))
+        ref = np.prod([_entry(x, idx) for x in facs])
+        worst = max(worst, abs(_entry(out, idx) - ref) / max(abs(ref), 1e-300))
+    say(f"  max relative entry error over 200 samples: {worst:.3e}")
+    assert worst < 1e-10
