@@ -330,16 +330,24 @@ function _recursive_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N
     s = bv .* pp
     Hr = H === nothing ? ones(Int, N + 1) : collect(Int, H.tto_rks)
     W = [zeros(Float64, A.ttv_rks[k] * Hr[k], s[k]) for k in 1:N+1]
-    id = upload(A)
+    # the reverse sketch of a product runs factor by factor (ttr_sketch_tuple), the
+    # product never formed; the forward one sketches the product formed on the device
+    factored = A isa _Product && reverse && length(A.factors) > 1
+    id = factored ? -1 : upload(A)
+    fids = factored ? Int64[upload(t) for t in A.factors] : Int64[]
     op = H === nothing ? -1 : upload(H)
     r, pv, srk = collect(Int64, rks), collect(Int64, pp), zeros(Int64, N + 1)
     B = reference_draws ? _reference_blocks(T, A.ttv_dims, bv, pp, orthogonal, seed, reverse) : Array{Float64,4}[]
     bptr = [pointer(b) for b in B]
-    GC.@preserve W r pv B bptr begin
+    GC.@preserve W r pv B bptr fids begin
         ptrs = [pointer(w) for w in W]
         opts = _opts(orthogonal, T, block_rks, seed; p=pointer(pv), forward=!reverse,
                      blocks=reference_draws ? pointer(bptr) : Ptr{Ptr{Float64}}(C_NULL))
-        if H === nothing
+        if factored
+            check(ccall((:ttr_sketch_tuple, lib), Cint,
+                        (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ref{RandOpts}, Ptr{Ptr{Float64}}, Ptr{Int64}),
+                        handle(), length(fids), fids, r, opts, ptrs, srk))
+        elseif H === nothing
             check(ccall((:ttr_sketch, lib), Cint,
                         (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{RandOpts}, Ptr{Ptr{Float64}}, Ptr{Int64}),
                         handle(), id, r, opts, ptrs, srk))
@@ -349,7 +357,8 @@ function _recursive_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N
                         handle(), op, id, r, opts, ptrs, srk))
         end
     end
-    free(id)
+    id >= 0 && free(id)
+    foreach(free, fids)
     op >= 0 && free(op)
     H === nothing && return [reshape(W[k], _bond_shape(A, k)..., s[k]) for k in 1:N+1], collect(Int, srk)
     return [reshape(W[k], A.ttv_rks[k], Hr[k], s[k]) for k in 1:N+1], collect(Int, srk)
@@ -414,6 +423,18 @@ function _tt_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}}, s::
     return out, Int(sa[])
 end
 
+# the boundary sketch of a product: its factored recursive sketch with the same
+# block count p = cld(s, bv_1) at every core (src/sketches.jl:819-900)
+function _tt_sketch(::Type{T}, H::Nothing, A::_Product{N}, s::Int; reverse=true, orthogonal=true, seed=1234,
+                    block_rks::Int=N, reference_draws::Bool=true, timer=nothing) where {T,N}
+    reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
+    bv = _block_ranks(A.ttv_dims, block_rks, N)
+    pc = cld(s, bv[1])
+    W, _ = _recursive_sketch(T, nothing, A, ones(Int, N + 1); orthogonal=orthogonal, reverse=true, seed=seed,
+                             block_rks=block_rks, p=vcat(fill(pc, N), 1), reference_draws=reference_draws)
+    return vec(W[1]), bv[1] * pc
+end
+
 tt_sketch(::Type{T}, A::TTvector{Float64,N}, s::Int; kwargs...) where {T<:Union{Float64,Float32},N} =
     _tt_sketch(T, nothing, A, s; kwargs...)
 tt_sketch(::Type{T}, H::TToperator{Float64,N}, A::TTvector{Float64,N}, s::Int;
@@ -446,8 +467,9 @@ function _combined_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}
 end
 
 # NTuple forms (src/tt_randtools.jl:364-473, src/sketches.jl:474-615,819-909,1003-1040): the
-# entrywise product of the tuple, formed on the device with the first factor's rank index
-# fastest; target ranks are required, as in the reference
+# entrywise product of the tuple with the first factor's rank index fastest, never formed
+# (ttr_round_rand_tuple, ttr_sketch_tuple; the forward sketch forms it on the device);
+# target ranks are required, as in the reference
 function ttrand_rounding(y::NTuple{M,TTvector{Float64,N}}, rks::Vector{Int}; orthogonal=true, seed=1234,
                          block_rks::Int=N, sketch_type=Float64, reference_draws::Bool=true, timer=nothing) where {M,N}
     M == 1 && return ttrand_rounding(y[1], rks; orthogonal=orthogonal, seed=seed, block_rks=block_rks,
@@ -459,17 +481,17 @@ function ttrand_rounding(y::NTuple{M,TTvector{Float64,N}}, rks::Vector{Int}; ort
         B = _reference_blocks(Float64, y[1].ttv_dims, bv, pp, orthogonal, seed, true)
     end
     bptr = [pointer(b) for b in B]
-    id = upload(_Product(y))
-    α, ids, r = [1.0], Int64[id], collect(Int64, rks)
+    ids = Int64[upload(t) for t in y]
+    nf, r = Int64[M], collect(Int64, rks)
     out = Ref{Int64}(0)
-    GC.@preserve α ids r B bptr begin
-        item = Ref(RandItem(1, pointer(α), pointer(ids), -1, pointer(r)))
+    GC.@preserve ids nf r B bptr begin
         opts = _opts(orthogonal, sketch_type, block_rks, seed;
                      blocks=reference_draws ? pointer(bptr) : Ptr{Ptr{Float64}}(C_NULL))
-        check(ccall((:ttr_round_rand, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, Ref{Int64}),
-                    handle(), 1, item, opts, out))
+        check(ccall((:ttr_round_rand_tuple, lib), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ref{RandOpts}, Ref{Int64}),
+                    handle(), 1, nf, ids, r, opts, out))
     end
-    free(id)
+    foreach(free, ids)
     return download(out[], Val(N))
 end
 ttrand_rounding(y::NTuple{M,TTvector{Float64,N}}, rmax::Int; kwargs...) where {M,N} =
