@@ -14,6 +14,7 @@
 // MFMA GEMM over all matrices of the batch, so the trailing matrix is streamed
 // once per 64 columns.
 #include "ttr_internal.h"
+#include "qr_step.h"
 
 namespace ttr {
 
@@ -134,119 +135,16 @@ __global__ __launch_bounds__(PT) void qr_panel_kernel(const PanelJob* __restrict
 
 
 // Register-resident panel factorisation: panel width NBR = 16, 512 threads,
-// thread t holds panel rows t, t+512, ... (R of them), so a Householder step is
-// two workgroup reductions and register updates, with no global traffic.  The
-// column steps are instantiated per column (PanelStep<R, JJ>) so that every
-// register of the panel is addressed statically.
-constexpr int NBR = 16;
+// thread t holds panel rows t, t+512, ... (R of them); the column steps are those
+// of qr_step.h (one workgroup reduction per column, no global traffic).
+constexpr int NBR = qrstep::SNB;
 constexpr int RT = 512;
-constexpr int RW = RT / 64;
-
-struct PanelShared {
-  double red[RW][NBR + 1];
-  double bc[NBR + 1];
-  double Ts[NBR][NBR + 1];
-};
-
-template <int R, int JJ>
-struct PanelStep {
-  __device__ static __forceinline__ void run(double (&P)[R][NBR], PanelShared& sh, const PanelJob& J, int tid,
-                                             int lane, int wave, int m2) {
-    if (JJ < J.nb) {
-      // sigma = sum_{i > JJ} x_i^2 ; alpha = P[JJ][JJ] (held by thread JJ, q = 0)
-      double part = 0.0;
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int i = tid + RT * q;
-        if (i > JJ && i < m2) part += P[q][JJ] * P[q][JJ];
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
-      if (lane == 0) sh.red[wave][0] = part;
-      if (tid == JJ) sh.bc[NBR] = P[0][JJ];
-      __syncthreads();
-      double sigma = 0.0;
-#pragma unroll
-      for (int w = 0; w < RW; ++w) sigma += sh.red[w][0];
-      const double alpha = sh.bc[NBR];
-      double tau = 0.0, beta = alpha, scal = 1.0;
-      if (sigma != 0.0) {
-        beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
-        tau = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
-      }
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int i = tid + RT * q;
-        if (i > JJ && i < m2) P[q][JJ] *= scal;
-      }
-      if (tid == JJ) P[0][JJ] = beta;
-      if (tid == 0) J.tau[J.c0 + JJ] = tau;
-      // d_c = sum_{i >= JJ} v_i P[i][c] (v_JJ = 1) for every other column c
-      double acc[NBR];
-#pragma unroll
-      for (int c = 0; c < NBR; ++c) acc[c] = 0.0;
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int i = tid + RT * q;
-        if (i >= JJ && i < m2) {
-          const double v = (i == JJ) ? 1.0 : P[q][JJ];
-#pragma unroll
-          for (int c = 0; c < NBR; ++c)
-            if (c != JJ) acc[c] += v * P[q][c];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int c = 0; c < NBR; ++c) {
-        double sv = acc[c];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) sv += __shfl_xor(sv, off);
-        if (lane == 0) sh.red[wave][c] = sv;
-      }
-      __syncthreads();
-      if (tid < NBR) {
-        double sv = 0.0;
-#pragma unroll
-        for (int w = 0; w < RW; ++w) sv += sh.red[w][tid];
-        sh.bc[tid] = sv;
-      }
-      __syncthreads();
-      // apply H_JJ to columns c > JJ
-      if (tau != 0.0) {
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-          const int i = tid + RT * q;
-          if (i >= JJ && i < m2) {
-            const double v = (i == JJ) ? 1.0 : P[q][JJ];
-#pragma unroll
-            for (int c = JJ + 1; c < NBR; ++c) P[q][c] -= tau * v * sh.bc[c];
-          }
-        }
-      }
-      // T(0:JJ, JJ) = -tau T(0:JJ, 0:JJ) z,  z_c = d_c (c < JJ)
-      if (tid < JJ) {
-        double sv = 0.0;
-        for (int qq = tid; qq < JJ; ++qq) sv += sh.Ts[tid][qq] * sh.bc[qq];
-        sh.Ts[tid][JJ] = -tau * sv;
-      }
-      if (tid == 0) sh.Ts[JJ][JJ] = tau;
-      __syncthreads();
-    }
-    PanelStep<R, JJ + 1>::run(P, sh, J, tid, lane, wave, m2);
-  }
-};
-
-template <int R>
-struct PanelStep<R, NBR> {
-  __device__ static __forceinline__ void run(double (&)[R][NBR], PanelShared&, const PanelJob&, int, int, int, int) {}
-};
 
 template <int R>
 __global__ __launch_bounds__(RT) void qr_panel_reg_kernel(const PanelJob* __restrict__ jobs) {
   const PanelJob J = jobs[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  __shared__ PanelShared sh;
+  __shared__ qrstep::PanelShared16<RT> sh;
   double* A = J.A;
   const int64_t lda = J.lda;
   const int c0 = J.c0, nb = J.nb;
@@ -259,9 +157,9 @@ __global__ __launch_bounds__(RT) void qr_panel_reg_kernel(const PanelJob* __rest
     for (int c = 0; c < NBR; ++c)
       P[q][c] = (i < m2 && c < nb) ? A[(c0 + i) + lda * (c0 + c)] : 0.0;
   }
-  for (int idx = tid; idx < NBR * (NBR + 1); idx += RT) (&sh.Ts[0][0])[idx] = 0.0;
+  for (int idx = tid; idx < NBR * qrstep::LDV; idx += RT) sh.Ts[idx] = 0.0;
   __syncthreads();
-  PanelStep<R, 0>::run(P, sh, J, tid, lane, wave, m2);
+  qrstep::PanelStep16<R, RT, 0>::run(P, sh, J.tau + c0, nb, m2, tid, lane, wave);
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     const int i = tid + RT * q;
@@ -271,9 +169,10 @@ __global__ __launch_bounds__(RT) void qr_panel_reg_kernel(const PanelJob* __rest
         if (c < nb) A[(c0 + i) + lda * (c0 + c)] = P[q][c];
     }
   }
+  __syncthreads();
   for (int idx = tid; idx < NBP * NBP; idx += RT) {
     const int r = idx % NBP, c = idx / NBP;
-    J.T[r + NBP * c] = (r < nb && c < nb && r < NBR && c < NBR) ? sh.Ts[r][c] : 0.0;
+    J.T[r + NBP * c] = (r < nb && c < nb && r < NBR && c < NBR) ? sh.Ts[r * qrstep::LDV + c] : 0.0;
   }
 }
 

@@ -22,6 +22,7 @@
 //   v = (1, x/(alpha - beta)), tau = 0 when x == 0,
 // so Q and R agree with the reference's LAPACK factorisations column by column.
 #include "ttr_internal.h"
+#include "qr_step.h"
 
 namespace ttr {
 
@@ -31,8 +32,8 @@ constexpr int SQT = 512;       // threads: 2 waves per SIMD for the MFMA updates
 constexpr int SQW = SQT / 64;  // waves
 constexpr int SPT = 256;       // threads holding the panel (waves 0-3)
 constexpr int SPW = SPT / 64;
-constexpr int SNB = 16;        // panel width
-constexpr int LDV = SNB + 1;   // LDS row stride of V (odd: conflict-free column reads)
+using qrstep::SNB;
+using qrstep::LDV;
 
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
@@ -44,137 +45,6 @@ struct SmallQRJob {
   int64_t ldq;
   double* tau;    // min(m, n)
   double* T;      // one 16 x 16 T per panel (column-major), read back when forming Q
-};
-
-struct SmallShared {
-  double Ts[SNB * LDV];       // T of the current panel, Ts[r * LDV + c] = T(r, c)
-  double red[2][SPW][SNB];    // per-wave column sums (double-buffered by step parity)
-  double rowj[2][SNB];        // the pivot row of the panel before the step
-};
-
-__device__ __forceinline__ double shfl_x(double v, int mask) { return __shfl_xor(v, mask); }
-
-// The 16 panel sums of a wave: lane l ends with the wave total of column
-// (l >> 2) & 15 (lanes 4c .. 4c+3 all hold column c).
-__device__ __forceinline__ double butterfly16(const double (&acc)[SNB], int lane) {
-  double a8[8], a4[4], a2[2];
-  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
-#pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const double send = h5 ? acc[c] : acc[c + 8];
-    const double keep = h5 ? acc[c + 8] : acc[c];
-    a8[c] = keep + shfl_x(send, 32);
-  }
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const double send = h4 ? a8[c] : a8[c + 4];
-    const double keep = h4 ? a8[c + 4] : a8[c];
-    a4[c] = keep + shfl_x(send, 16);
-  }
-#pragma unroll
-  for (int c = 0; c < 2; ++c) {
-    const double send = h3 ? a4[c] : a4[c + 2];
-    const double keep = h3 ? a4[c + 2] : a4[c];
-    a2[c] = keep + shfl_x(send, 8);
-  }
-  double a1;
-  {
-    const double send = h2 ? a2[0] : a2[1];
-    const double keep = h2 ? a2[1] : a2[0];
-    a1 = keep + shfl_x(send, 4);
-  }
-  a1 += shfl_x(a1, 2);
-  a1 += shfl_x(a1, 1);
-  return a1;
-}
-
-// Householder step JJ of the panel held in P (rows tid + 256 q of the panel,
-// threads tid < 256; the other waves hold no rows and only meet the barriers).
-// The steps are instantiated per column so that every register of the panel is
-// addressed statically (a run-time column loop with selects measured slower).
-template <int R, int JJ>
-struct SmallStep {
-  __device__ static __forceinline__ void run(double (&P)[R][SNB], SmallShared& sh, double* tau_out, int nb,
-                                             int m2, int tid, int lane, int wave) {
-    if (JJ < nb) {
-      constexpr int buf = JJ & 1;
-      const bool pt = tid < SPT;
-      // s_c = sum_{i > JJ} x_i P[i][c] (s_JJ = sigma, the squared norm below the pivot)
-      double acc[SNB];
-#pragma unroll
-      for (int c = 0; c < SNB; ++c) acc[c] = 0.0;
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int i = tid + SPT * q;
-        if (pt && i > JJ && i < m2) {
-          const double x = P[q][JJ];
-#pragma unroll
-          for (int c = 0; c < SNB; ++c) acc[c] += x * P[q][c];
-        }
-      }
-      if (pt) {
-        const double ws = butterfly16(acc, lane);
-        if ((lane & 3) == 0) sh.red[buf][wave][lane >> 2] = ws;
-      }
-      if (tid == JJ) {
-#pragma unroll
-        for (int c = 0; c < SNB; ++c) sh.rowj[buf][c] = P[0][c];
-      }
-      __syncthreads();
-      double s[SNB], rj[SNB];
-#pragma unroll
-      for (int c = 0; c < SNB; ++c) {
-        double v = 0.0;
-#pragma unroll
-        for (int w = 0; w < SPW; ++w) v += sh.red[buf][w][c];
-        s[c] = v;
-        rj[c] = sh.rowj[buf][c];
-      }
-      const double sigma = s[JJ], alpha = rj[JJ];
-      double tau = 0.0, beta = alpha, scal = 1.0;
-      if (sigma != 0.0) {
-        beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
-        tau = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
-      }
-      // d_c = v^T A(:, c) over rows >= JJ (v_JJ = 1): for c > JJ the reflector's
-      // product with the remaining columns, for c < JJ the entries V^T v of T
-      double dd[SNB];
-#pragma unroll
-      for (int c = 0; c < SNB; ++c) dd[c] = rj[c] + scal * s[c];
-#pragma unroll
-      for (int q = 0; q < R; ++q) {
-        const int i = tid + SPT * q;
-        if (pt && i > JJ && i < m2) {
-          const double v = P[q][JJ] * scal;
-          P[q][JJ] = v;
-#pragma unroll
-          for (int c = JJ + 1; c < SNB; ++c) P[q][c] -= tau * v * dd[c];
-        }
-      }
-      if (tid == JJ) {
-        P[0][JJ] = beta;
-#pragma unroll
-        for (int c = JJ + 1; c < SNB; ++c) P[0][c] -= tau * dd[c];
-      }
-      if (tid == 0) tau_out[JJ] = tau;
-      // T(0:JJ, JJ) = -tau T(0:JJ, 0:JJ) d(0:JJ),  T(JJ, JJ) = tau
-      if (tid < JJ) {
-        double v = 0.0;
-#pragma unroll
-        for (int qq = 0; qq < JJ; ++qq)
-          if (qq >= tid) v += sh.Ts[tid * LDV + qq] * dd[qq];
-        sh.Ts[tid * LDV + JJ] = -tau * v;
-      }
-      if (tid == JJ) sh.Ts[JJ * LDV + JJ] = tau;
-    }
-    SmallStep<R, JJ + 1>::run(P, sh, tau_out, nb, m2, tid, lane, wave);
-  }
-};
-
-template <int R>
-struct SmallStep<R, SNB> {
-  __device__ static __forceinline__ void run(double (&)[R][SNB], SmallShared&, double*, int, int, int, int, int) {}
 };
 
 // C(rows 0..m2-1, 16 columns j0..) -= V (T^T or T) V^T C for the 16-column
@@ -289,7 +159,7 @@ template <int R>
 __global__ __launch_bounds__(SQT) void qr_small_kernel(const SmallQRJob* __restrict__ jobs) {
   const SmallQRJob J = jobs[blockIdx.x];
   __shared__ double Vs[SPT * R * LDV];
-  __shared__ SmallShared sh;
+  __shared__ qrstep::PanelShared16<SPT> sh;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m = J.m, n = J.n, kq = min(m, n);
   double* A = J.A;
@@ -309,7 +179,7 @@ __global__ __launch_bounds__(SQT) void qr_small_kernel(const SmallQRJob* __restr
     }
     for (int e = tid; e < SNB * LDV; e += SQT) sh.Ts[e] = 0.0;
     __syncthreads();
-    SmallStep<R, 0>::run(P, sh, J.tau + c0, nb, m2, tid, lane, wave);
+    qrstep::PanelStep16<R, SPT, 0>::run(P, sh, J.tau + c0, nb, m2, tid, lane, wave);
     // R and V back to A; V (unit lower, zero above, zero rows past m2) to LDS
     if (pt) {
 #pragma unroll

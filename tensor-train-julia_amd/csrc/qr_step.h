@@ -1,0 +1,168 @@
+// One Householder column step of a 16-column panel held in registers, shared by
+// the one-launch QR (qr_small.hip) and the panel kernel of the blocked QR
+// (qr.hip).  LAPACK conventions (dlarfg/dlarft):
+//   beta = -sign(alpha) ||(alpha, x)||, tau = (beta - alpha)/beta,
+//   v = (1, x/(alpha - beta)), tau = 0 when x == 0.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace ttr {
+namespace qrstep {
+
+constexpr int SNB = 16;        // panel width
+constexpr int LDV = SNB + 1;   // LDS row stride of 16-wide arrays (odd: conflict-free column reads)
+
+template <int NT>
+struct PanelShared16 {
+  double Ts[SNB * LDV];         // T of the current panel, Ts[r * LDV + c] = T(r, c)
+  double red[2][NT / 64][SNB];  // per-wave column sums (double-buffered by step parity)
+  double rowj[2][SNB];          // the pivot row of the panel before the step
+  double dd[SNB];               // v^T P(:, c) of the step
+  double sc[3];                 // tau, beta, 1 / (alpha - beta) of the step
+};
+
+__device__ __forceinline__ double shfl_x(double v, int mask) { return __shfl_xor(v, mask); }
+
+// The 16 panel sums of a wave: lane l ends with the wave total of column
+// (l >> 2) & 15 (lanes 4c .. 4c+3 all hold column c).
+__device__ __forceinline__ double butterfly16(const double (&acc)[SNB], int lane) {
+  double a8[8], a4[4], a2[2];
+  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const double send = h5 ? acc[c] : acc[c + 8];
+    const double keep = h5 ? acc[c + 8] : acc[c];
+    a8[c] = keep + shfl_x(send, 32);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double send = h4 ? a8[c] : a8[c + 4];
+    const double keep = h4 ? a8[c + 4] : a8[c];
+    a4[c] = keep + shfl_x(send, 16);
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const double send = h3 ? a4[c] : a4[c + 2];
+    const double keep = h3 ? a4[c + 2] : a4[c];
+    a2[c] = keep + shfl_x(send, 8);
+  }
+  double a1;
+  {
+    const double send = h2 ? a2[0] : a2[1];
+    const double keep = h2 ? a2[1] : a2[0];
+    a1 = keep + shfl_x(send, 4);
+  }
+  a1 += shfl_x(a1, 2);
+  a1 += shfl_x(a1, 1);
+  return a1;
+}
+
+// Householder step JJ of a 16-column panel held in registers: threads tid < NT
+// hold panel rows tid + NT q (R of them); threads past NT hold no rows and only
+// meet the barriers.  One workgroup reduction per column: the 16 sums
+// s_c = x^T P(:, c) of the column below the pivot (x) against every panel column
+// are reduced together (butterfly reduce-scatter, then one LDS pass), the
+// reflector's squared norm being s_JJ; with the pivot row broadcast through LDS
+// every thread then has v^T P(:, c) for all c, which both applies the reflector
+// (c > JJ) and gives the new column of T (c < JJ).  The steps are instantiated
+// per column so that every register of the panel is addressed statically (a
+// run-time column loop with selects measured slower).
+template <int R, int NT, int JJ>
+struct PanelStep16 {
+  __device__ static __forceinline__ void run(double (&P)[R][SNB], PanelShared16<NT>& sh, double* tau_out, int nb,
+                                             int m2, int tid, int lane, int wave) {
+    constexpr int NW = NT / 64;
+    if (JJ < nb) {
+      constexpr int buf = JJ & 1;
+      const bool pt = tid < NT;
+      // s_c = sum_{i > JJ} x_i P[i][c] (s_JJ = sigma, the squared norm below the pivot)
+      double acc[SNB];
+#pragma unroll
+      for (int c = 0; c < SNB; ++c) acc[c] = 0.0;
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = tid + NT * q;
+        if (pt && i > JJ && i < m2) {
+          const double x = P[q][JJ];
+#pragma unroll
+          for (int c = 0; c < SNB; ++c) acc[c] += x * P[q][c];
+        }
+      }
+      if (pt) {
+        const double ws = butterfly16(acc, lane);
+        if ((lane & 3) == 0) sh.red[buf][wave][lane >> 2] = ws;
+      }
+      if (tid == JJ) {
+#pragma unroll
+        for (int c = 0; c < SNB; ++c) sh.rowj[buf][c] = P[0][c];
+      }
+      __syncthreads();
+      // Threads c < 16 total column c over the waves and form d_c = v^T P(:, c)
+      // over rows >= JJ (v_JJ = 1): for c > JJ the reflector's product with the
+      // remaining columns, for c < JJ the entries V^T v of T.  A second barrier
+      // then hands every thread the 16 values and the reflector's scalars (each
+      // thread summing all waves itself would keep 16 x waves values live).
+      if (tid < SNB) {
+        double v = 0.0, sg = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+          v += sh.red[buf][w][tid];
+          sg += sh.red[buf][w][JJ];
+        }
+        const double al = sh.rowj[buf][JJ];
+        double t = 0.0, be = al, sl = 1.0;
+        if (sg != 0.0) {
+          be = -copysign(sqrt(al * al + sg), al);
+          t = (be - al) / be;
+          sl = 1.0 / (al - be);
+        }
+        sh.dd[tid] = sh.rowj[buf][tid] + sl * v;
+        if (tid == 0) {
+          sh.sc[0] = t;
+          sh.sc[1] = be;
+          sh.sc[2] = sl;
+        }
+      }
+      __syncthreads();
+      const double tau = sh.sc[0], beta = sh.sc[1], scal = sh.sc[2];
+      double dd[SNB];
+#pragma unroll
+      for (int c = 0; c < SNB; ++c) dd[c] = sh.dd[c];
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = tid + NT * q;
+        if (pt && i > JJ && i < m2) {
+          const double v = P[q][JJ] * scal;
+          P[q][JJ] = v;
+#pragma unroll
+          for (int c = JJ + 1; c < SNB; ++c) P[q][c] -= tau * v * dd[c];
+        }
+      }
+      if (tid == JJ) {
+        P[0][JJ] = beta;
+#pragma unroll
+        for (int c = JJ + 1; c < SNB; ++c) P[0][c] -= tau * dd[c];
+      }
+      if (tid == 0) tau_out[JJ] = tau;
+      // T(0:JJ, JJ) = -tau T(0:JJ, 0:JJ) d(0:JJ),  T(JJ, JJ) = tau
+      if (tid < JJ) {
+        double v = 0.0;
+#pragma unroll
+        for (int qq = 0; qq < JJ; ++qq)
+          if (qq >= tid) v += sh.Ts[tid * LDV + qq] * dd[qq];
+        sh.Ts[tid * LDV + JJ] = -tau * v;
+      }
+      if (tid == JJ) sh.Ts[JJ * LDV + JJ] = tau;
+    }
+    PanelStep16<R, NT, JJ + 1>::run(P, sh, tau_out, nb, m2, tid, lane, wave);
+  }
+};
+
+template <int R, int NT>
+struct PanelStep16<R, NT, SNB> {
+  __device__ static __forceinline__ void run(double (&)[R][SNB], PanelShared16<NT>&, double*, int, int, int, int,
+                                             int) {}
+};
+
+}  // namespace qrstep
+}  // namespace ttr
