@@ -9,7 +9,7 @@
 // Columns are processed in blocks of 64.  Inside a block, panels of 16 (32 for
 // very tall matrices) are factorised by one workgroup per matrix and applied to
 // the rest of the block; the block's reflectors are then merged into one
-// 64-wide compact WY factor (T from V^T V and tau, as dlarft), which updates the
+// 64-wide compact WY factor (T from V^T V and the panels' T's), which updates the
 // trailing matrix and, backwards, forms Q.  Every product is a grouped FP64
 // MFMA GEMM over all matrices of the batch, so the trailing matrix is streamed
 // once per 64 columns.
@@ -176,39 +176,53 @@ __global__ __launch_bounds__(RT) void qr_panel_reg_kernel(const PanelJob* __rest
   }
 }
 
-// T of a block of nb <= 64 reflectors from G = V^T V (64 x 64, ld 64) and tau:
-// T(0:j, j) = -tau_j T(0:j, 0:j) G(0:j, j), T(j, j) = tau_j (dlarft, forward,
-// columnwise).  One 64-thread workgroup per matrix.
+// T of a block of nb <= 64 reflectors (dlarft, forward, columnwise) assembled from
+// the T's its panels already have: with the block split into panels 1..q, the
+// columns of panel q are T(0:c0, q) = -T(0:c0, 0:c0) G(0:c0, q) T_qq (G = V^T V of
+// the block), T_qq the panel's own T.  One 256-thread workgroup per matrix, two
+// small products per panel instead of one sequential step per column.
 constexpr int NBO = 64;
 
-struct TBuildJob {
-  const double* G;
-  const double* tau;
-  double* T;  // 64 x 64, ld 64
-  int32_t nb;
+struct TMergeJob {
+  const double* G;   // 64 x 64, ld 64
+  const double* Tp;  // panel q's T at Tp + q * NBP * NBP, ld NBP
+  double* T;         // 64 x 64, ld 64
+  int32_t nb, w;     // columns of the block, panel width (16 or 32)
 };
 
-__global__ __launch_bounds__(NBO) void build_t_kernel(const TBuildJob* __restrict__ jobs) {
-  const TBuildJob J = jobs[blockIdx.x];
-  const int t = threadIdx.x, nb = J.nb;
-  __shared__ double Gs[NBO][NBO + 1];
+__global__ __launch_bounds__(256) void merge_t_kernel(const TMergeJob* __restrict__ jobs) {
+  const TMergeJob J = jobs[blockIdx.x];
+  const int t = threadIdx.x, nb = J.nb, w = J.w;
   __shared__ double Ts[NBO][NBO + 1];
-  for (int c = 0; c < NBO; ++c) {
-    Gs[t][c] = (t < nb && c < nb) ? J.G[t + NBO * c] : 0.0;
-    Ts[t][c] = 0.0;
+  __shared__ double Gs[NBO][NBO + 1];
+  __shared__ double X[NBO - 16][NBP + 1];
+  for (int e = t; e < NBO * NBO; e += 256) {
+    const int r = e % NBO, c = e / NBO;
+    Gs[r][c] = (r < nb && c < nb) ? J.G[r + NBO * c] : 0.0;
+    const int q = c / w;
+    Ts[r][c] = (r < nb && c < nb && r / w == q) ? J.Tp[size_t(q) * NBP * NBP + (r - q * w) + NBP * (c - q * w)] : 0.0;
   }
   __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    const double tau = J.tau[j];
-    if (t < j) {
-      double s = 0.0;
-      for (int q = t; q < j; ++q) s += Ts[t][q] * Gs[q][j];
-      Ts[t][j] = -tau * s;
+  for (int c0 = w; c0 < nb; c0 += w) {
+    const int wq = min(w, nb - c0);
+    // X (c0 x wq) = G(0:c0, c0:c0+wq) T_qq (T_qq upper triangular)
+    for (int e = t; e < c0 * wq; e += 256) {
+      const int r = e % c0, c = e / c0;
+      double x = 0.0;
+      for (int k = 0; k <= c; ++k) x += Gs[r][c0 + k] * Ts[c0 + k][c0 + c];
+      X[r][c] = x;
     }
-    if (t == j) Ts[j][j] = tau;
+    __syncthreads();
+    // T(0:c0, c0:c0+wq) = -T(0:c0, 0:c0) X (T upper triangular)
+    for (int e = t; e < c0 * wq; e += 256) {
+      const int r = e % c0, c = e / c0;
+      double y = 0.0;
+      for (int k = r; k < c0; ++k) y += Ts[r][k] * X[k][c];
+      Ts[r][c0 + c] = -y;
+    }
     __syncthreads();
   }
-  for (int c = 0; c < NBO; ++c) J.T[t + NBO * c] = Ts[t][c];
+  for (int e = t; e < NBO * NBO; e += 256) J.T[e] = Ts[e % NBO][e / NBO];
 }
 
 void launch_panel(Ctx& ctx, const PanelJob* d, int njobs, int max_rows, bool reg) {
@@ -342,20 +356,19 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     }
     // merge the block's reflectors: G = V^T V, T from (G, tau)
     copy_vx(b);
-    std::vector<TBuildJob> tb;
+    std::vector<TMergeJob> tb;
     for (size_t q = 0; q < jobs.size(); ++q) {
       const QRJob& J = jobs[q];
       const Aux& a = aux[q];
       if (b >= a.nblock) continue;
       const int c0 = b * NBO, m2 = J.m - c0, nbo = block_cols(a, b);
       gb.add(nbo, nbo, m2, 1.0, a.VX, m2, 1, a.VX, 1, m2, 0.0, a.G, 1, NBO);
-      tb.push_back(TBuildJob{a.G, a.tau + c0, a.TO + size_t(b) * NBO * NBO, nbo});
+      tb.push_back(TMergeJob{a.G, a.T + size_t(c0 / NB) * NBP * NBP, a.TO + size_t(b) * NBO * NBO, nbo, NB});
     }
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
     if (!tb.empty()) {
-      const TBuildJob* d = ctx.upload(tb);
-      TTR_LAUNCH(ctx, build_t_kernel, dim3(unsigned(tb.size())), dim3(NBO), 0, d);
-      TTR_HIP(hipGetLastError());
+      const TMergeJob* d = ctx.upload(tb);
+      TTR_LAUNCH(ctx, merge_t_kernel, dim3(unsigned(tb.size())), dim3(256), 0, d);
     }
     // trailing update A2 -= V (T^T (V^T A2))
     for (int stage = 0; stage < 3; ++stage) {
