@@ -80,18 +80,36 @@ constexpr int BM = 64, BN = 64;  // tile of the BF16 kernel
 // outputs.  The accumulation order of every output element is the K order whatever
 // the tile, and the split-K chunks are counted in k-tiles of KS = 16 for every BK,
 // so all tile configurations give bitwise identical results.
+//
+// Pipeline: two LDS buffers.  While the MFMAs of k-step i read one buffer, the
+// global loads of k-step i + 1 are in flight into registers and are then stored
+// to the other buffer, so one barrier per k-step orders everything.  The k-steps
+// of all segments of a problem form one sequence, so the pipeline does not drain
+// at segment boundaries.
 constexpr int KS = 16;
+constexpr int NXCD = 8;  // accelerator complex dies; workgroups go to them round-robin
+
+// Workgroup bid runs on die bid % 8.  Tile index given to it: each die gets one
+// contiguous range of the tile sequence, so the tiles that share operand panels
+// (neighbouring tiles of one problem, and the problems of one item) meet in the
+// same die's L2 instead of being spread over all eight.
+__device__ __forceinline__ int xcd_tile(int bid, int n) {
+  const int q = n / NXCD, r = n - q * NXCD;
+  const int x = bid % NXCD, j = bid / NXCD;
+  return x * q + min(x, r) + j;
+}
+
 template <class TA, class TB, class TC, class MT, int TM, int TN, int BK>
-__global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __restrict__ probs, int nprobs,
-                                                          const GemmSeg* __restrict__ segs) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM * TN >= 128 * 128 ? 2 : 1)))
+void gemm_scalar_kernel(const GemmProb* __restrict__ probs, int nprobs, const GemmSeg* __restrict__ segs) {
   static_assert(KS % BK == 0, "a split-K k-tile holds whole k-steps");
   constexpr int MI = TM / 32, NI = TN / 32;
   constexpr int NA = TM * BK / 256, NB = TN * BK / 256;  // tile elements per thread
-  __shared__ MT As[BK][TM + 1];
-  __shared__ MT Bs[BK][TN + 1];
+  __shared__ MT As[2][BK][TM + 1];
+  __shared__ MT Bs[2][BK][TN + 1];
   using acc_t = typename std::conditional<std::is_same<MT, double>::value, d4_t, f4_t>::type;
 
-  const int bid = blockIdx.x;
+  const int bid = xcd_tile(blockIdx.x, gridDim.x);
   const GemmProb P = probs[find_prob(probs, nprobs, bid)];
   const int t0 = bid - P.tile0;
   const int chunk = t0 / P.tiles_mn, t = t0 - chunk * P.tiles_mn;
@@ -112,6 +130,14 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
   for (int s = 0; s < P.nseg; ++s)
     if (segs[P.seg0 + s].exp) emax = max(emax, segs[P.seg0 + s].exp[0]);
 
+  // Element q of a thread's share of the A tile is (i, l) = (ai0, al0 + q * adl)
+  // when A is column-major in memory (i fastest across threads), else
+  // (ai0 + q * adi, al0) (l fastest); likewise (l, j) for B.  The thread's first
+  // element and the step between its elements are fixed per segment.
+  TA ra[NA];
+  TB rb[NB];
+  int buf = 0;
+  bool started = false;
   int kt_seg = 0;  // first k-tile of the segment in the problem's sequence
   for (int s = 0; s < P.nseg; ++s) {
     const GemmSeg S = segs[P.seg0 + s];
@@ -122,80 +148,83 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
     const int kend = min(ke, S.K);
     double scale = S.scale;
     if (S.exp) scale = ldexp(scale, S.exp[0] - emax);
-    const bool a_row_fast = (S.a_rs == 1);
-    const bool b_row_fast = (S.b_rs == 1);
-    // element q of a thread: A (i, l) with i fastest when A is column-major in
-    // memory, l fastest otherwise; likewise for B
-    auto a_idx = [&](int q, int& i, int& l) {
-      const int e = tid + 256 * q;
-      if (a_row_fast) { i = e % TM; l = e / TM; } else { l = e % BK; i = e / BK; }
-    };
-    auto b_idx = [&](int q, int& l, int& j) {
-      const int e = tid + 256 * q;
-      if (b_row_fast) { l = e % BK; j = e / BK; } else { j = e % TN; l = e / TN; }
-    };
-    // The k-tile k0 + BK is loaded into registers while the MFMAs of tile k0 run.
-    // The loads are raw (no arithmetic on the loaded values, so nothing waits
-    // for them before the MFMAs); the unit diagonal of the implicit-V modes and
-    // the segment scale are applied when the registers are stored to LDS.
-    TA ra[NA];
-    TB rb[NB];
-    // tiles entirely inside the operands (the common case) load without predicates
-    const bool a_full = row0 + TM <= P.M && S.a_mode == AM_PLAIN;
-    const bool b_full = col0 + TN <= P.N;
+    int ai0, al0, adi, adl, bl0, bj0, bdl, bdj;
+    if (S.a_rs == 1) { ai0 = tid % TM; al0 = tid / TM; adi = 0; adl = 256 / TM; }
+    else { al0 = tid % BK; ai0 = tid / BK; adl = 0; adi = 256 / BK; }
+    if (S.b_rs == 1) { bl0 = tid % BK; bj0 = tid / BK; bdl = 0; bdj = 256 / BK; }
+    else { bj0 = tid % TN; bl0 = tid / TN; bdj = 0; bdl = 256 / TN; }
+    const int64_t aoff = int64_t(row0 + ai0) * S.a_rs + int64_t(al0) * S.a_cs;
+    const int64_t adq = int64_t(adi) * S.a_rs + int64_t(adl) * S.a_cs;
+    const int64_t boff = int64_t(bl0) * S.b_rs + int64_t(col0 + bj0) * S.b_cs;
+    const int64_t bdq = int64_t(bdl) * S.b_rs + int64_t(bdj) * S.b_cs;
+    const bool a_in = row0 + TM <= P.M && S.a_mode == AM_PLAIN, b_in = col0 + TN <= P.N;
+
+    // Raw loads of k-step k0 into registers (no arithmetic on the loaded values,
+    // so nothing waits for them before the MFMAs); the unit diagonal of the
+    // implicit-V modes and the segment scale are applied when the registers are
+    // stored to LDS.  Tiles entirely inside the operands load without predicates.
     auto load_tiles = [&](int k0) {
       const bool k_full = k0 + BK <= S.K;
+      const int64_t ao = aoff + int64_t(k0) * S.a_cs, bo = boff + int64_t(k0) * S.b_rs;
+      if (a_in && k_full) {
 #pragma unroll
-      for (int q = 0; q < NA; ++q) {
-        int i, l;
-        a_idx(q, i, l);
-        const int64_t gi = row0 + i, gl = k0 + l;
-        if (a_full && k_full) {
-          ra[q] = ldg<TA>(S.A, gi * S.a_rs + gl * S.a_cs);
-        } else {
+        for (int q = 0; q < NA; ++q) ra[q] = ldg<TA>(S.A, ao + q * adq);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+          const int gi = row0 + ai0 + q * adi, gl = k0 + al0 + q * adl;
           const bool in = gi < P.M && gl < S.K &&
                           (S.a_mode == AM_PLAIN || (S.a_mode == AM_VT_UNIT ? gl > gi : gi > gl));
           ra[q] = TA{};
-          if (in) ra[q] = ldg<TA>(S.A, gi * S.a_rs + gl * S.a_cs);
+          if (in) ra[q] = ldg<TA>(S.A, ao + q * adq);
         }
       }
+      if (b_in && k_full) {
 #pragma unroll
-      for (int q = 0; q < NB; ++q) {
-        int l, j;
-        b_idx(q, l, j);
-        const int64_t gl = k0 + l, gj = col0 + j;
-        if (b_full && k_full) rb[q] = ldg<TB>(S.B, gl * S.b_rs + gj * S.b_cs);
-        else {
+        for (int q = 0; q < NB; ++q) rb[q] = ldg<TB>(S.B, bo + q * bdq);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+          const int gl = k0 + bl0 + q * bdl, gj = col0 + bj0 + q * bdj;
           rb[q] = TB{};
-          if (gl < S.K && gj < P.N) rb[q] = ldg<TB>(S.B, gl * S.b_rs + gj * S.b_cs);
+          if (gl < S.K && gj < P.N) rb[q] = ldg<TB>(S.B, bo + q * bdq);
         }
       }
     };
+    auto store_tiles = [&](int b, int k0) {
+      if (S.a_mode == AM_PLAIN) {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) As[b][al0 + q * adl][ai0 + q * adi] = MT(to_double(ra[q]) * scale);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+          const int i = ai0 + q * adi, l = al0 + q * adl;
+          double v = to_double(ra[q]);
+          if (row0 + i == k0 + l && row0 + i < P.M && k0 + l < S.K) v = 1.0;
+          As[b][l][i] = MT(v * scale);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NB; ++q) Bs[b][bl0 + q * bdl][bj0 + q * bdj] = MT(to_double(rb[q]));
+    };
+
+    // the first k-step of a segment goes to the buffer the previous segment's
+    // last MFMAs did not read
+    if (started) buf ^= 1;
+    started = true;
     load_tiles(kb);
+    store_tiles(buf, kb);
+    __syncthreads();
     for (int k0 = kb; k0 < kend; k0 += BK) {
-#pragma unroll
-      for (int q = 0; q < NA; ++q) {
-        int i, l;
-        a_idx(q, i, l);
-        double v = to_double(ra[q]);
-        if (S.a_mode != AM_PLAIN && row0 + i == k0 + l && row0 + i < P.M && k0 + l < S.K) v = 1.0;
-        As[l][i] = MT(v * scale);
-      }
-#pragma unroll
-      for (int q = 0; q < NB; ++q) {
-        int l, j;
-        b_idx(q, l, j);
-        Bs[l][j] = MT(to_double(rb[q]));
-      }
-      __syncthreads();
-      if (k0 + BK < kend) load_tiles(k0 + BK);
+      const bool more = k0 + BK < kend;
+      if (more) load_tiles(k0 + BK);  // in flight during this k-step's MFMAs
 #pragma unroll
       for (int kk = 0; kk < BK; kk += 4) {
         MT av[MI], bv[NI];
 #pragma unroll
-        for (int a = 0; a < MI; ++a) av[a] = As[kk + (lane >> 4)][wm * (TM / 2) + 16 * a + (lane & 15)];
+        for (int a = 0; a < MI; ++a) av[a] = As[buf][kk + (lane >> 4)][wm * (TM / 2) + 16 * a + (lane & 15)];
 #pragma unroll
-        for (int b = 0; b < NI; ++b) bv[b] = Bs[kk + (lane >> 4)][wn * (TN / 2) + 16 * b + (lane & 15)];
+        for (int b = 0; b < NI; ++b) bv[b] = Bs[buf][kk + (lane >> 4)][wn * (TN / 2) + 16 * b + (lane & 15)];
 #pragma unroll
         for (int a = 0; a < MI; ++a)
 #pragma unroll
@@ -206,7 +235,11 @@ __global__ __launch_bounds__(256) void gemm_scalar_kernel(const GemmProb* __rest
               acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
           }
       }
-      __syncthreads();
+      if (more) {
+        store_tiles(buf ^ 1, k0 + BK);
+        __syncthreads();
+        buf ^= 1;
+      }
     }
   }
   // epilogue: register r of a lane holds column lane%16 of each 16x16 block and
