@@ -40,6 +40,7 @@ __all__ = [
     "dot_par", "partial_contraction", "dot_operator",
     "hubbard_1d_mpo", "hubbard_1d_dense", "hadamard", "contract_sketch_core_kronecker_backwards",
     "tt_recursive_sketch_tuple", "tt_sketch_tuple", "ttrand_rounding_tuple",
+    "expand_basis", "sketched_rr", "sketched_rayleigh_ritz",
 ]
 
 
@@ -1180,6 +1181,200 @@ def rel_err(x: TT, y: TT) -> float:
     of ``examples/quantics_rounding_benchmark.jl:70-72``)."""
     nx = tt_norm(orthogonalize(x, i=x.N))
     return tt_norm(tt_sub(x, y)) / nx
+
+
+# ---------------------------------------------------------------------------
+# Sketched Krylov basis and sketched Rayleigh-Ritz: the solver-side callers of the
+# boundary sketches and the randomized rounding (src/tt_solvers.jl:330-689)
+# ---------------------------------------------------------------------------
+
+def _eig_sorted(M: np.ndarray, B: Optional[np.ndarray] = None):
+    """Julia's ``eigen(M)`` / ``eigen(M, B)``: eigenvalues sorted by (real, imag)
+    (``LinearAlgebra.eigsortby``), eigenvectors in the same order."""
+    if B is None:
+        lam, V = np.linalg.eig(M)
+    else:
+        import scipy.linalg
+        lam, V = scipy.linalg.eig(M, B)
+    order = sorted(range(len(lam)), key=lambda i: (lam[i].real, lam[i].imag))
+    return lam[order], V[:, order]
+
+
+def expand_basis(H: TTO, B_window: List[TT], W_B_window: List[List[np.ndarray]], B_sketch_window: List[np.ndarray],
+                 rmax: int, s: int, draws, draws_extra, orthogonal: bool = True, block_rks: Optional[int] = 8,
+                 k_trunc: Optional[int] = None):
+    """``expand_basis!`` (``src/tt_solvers.jl:354-445``), one sketch-orthogonal
+    Arnoldi step.  The combined sketch of ``H b_prev`` (ranks ``2 rmax``, boundary
+    sketch of size ``s``; ``draws`` / ``draws_extra`` are the draws of ``seed`` /
+    ``seed + 1``) gives the least-squares coefficients ``h`` against the window's
+    boundary sketches (``:376``); the left-to-right sweep over the ``1 + len(window)``
+    terms ``H b_prev, -h_i b_i`` with the precomputed sketches ``W_HB`` and
+    ``W_B_window`` (``:382-437``); ``tt_rounding(rmax)`` (``:439``); the combined
+    sketch of the result, normalised by the norm of its boundary sketch
+    (``:442-451``); the three windows mutated as a sliding window of ``k_trunc``
+    (``:453-455``).  Returns ``(b_new, sketch_b_new, sketch_Hb_prev)``."""
+    if k_trunc is None:
+        k_trunc = len(B_window)
+    b_prev = B_window[-1]
+    dims, N = b_prev.dims, b_prev.N
+    rks = [2 * rmax] * (N + 1)
+    rks[N] = 1
+    sketch_Hb_prev, WHb_raw, s_rks = tt_combined_sketch(b_prev, rks, s, draws, draws_extra, orthogonal, block_rks, H=H)
+    W_HB = [jreshape(WHb_raw[k], (b_prev.rks[k] * H.rks[k], s_rks[k])) for k in range(N + 1)]
+    # h = hcat(B_sketch_window...) \ sketch_Hb_prev (a tall least-squares solve)
+    h = np.linalg.lstsq(np.column_stack(B_sketch_window), sketch_Hb_prev, rcond=None)[0]
+    m = 1 + len(B_window)
+    vec_out: List[Optional[np.ndarray]] = [None] * N
+    out_rks = [1] * (N + 1)
+    ot = [0] * N
+    b1 = jreshape(b_prev.cores[0], (dims[0], b_prev.rks[1]))
+    H1 = jreshape(H.cores[0], (dims[0], dims[0], H.rks[1]))
+    Ay1 = np.einsum("ijb,ja->iab", H1, b1, optimize=True)  # (i1, alpha2, beta2)
+    Y = [jreshape(Ay1, (dims[0], 1, b_prev.rks[1] * H.rks[1]))]
+    for i, b in enumerate(B_window):
+        Y.append(-h[i] * b.cores[0])
+    for k in range(N - 1):
+        Z = np.tensordot(Y[0], W_HB[k + 1], axes=([2], [0]))
+        for i, W in enumerate(W_B_window):
+            Z = Z + np.tensordot(Y[1 + i], W[k + 1], axes=([2], [0]))
+        Q = _qr_thin(jreshape(Z, (dims[k] * out_rks[k], s_rks[k + 1])))
+        out_rks[k + 1] = Q.shape[1]
+        vec_out[k] = jreshape(Q, (dims[k], out_rks[k], out_rks[k + 1]))
+        ot[k] = 1
+        y1k = jreshape(Y[0], (dims[k], out_rks[k], b_prev.rks[k + 1], H.rks[k + 1]))
+        P = np.einsum("irab,irs->sab", y1k, vec_out[k], optimize=True)
+        t = _op_project(P, b_prev.cores[k + 1], H.cores[k + 1])
+        Ynew = [jreshape(t, (dims[k + 1], out_rks[k + 1], b_prev.rks[k + 2] * H.rks[k + 2]))]
+        for i, b in enumerate(B_window):
+            Pi = np.einsum("ira,irs->sa", Y[1 + i], vec_out[k], optimize=True)
+            Ynew.append(np.einsum("sa,iab->isb", Pi, b.cores[k + 1], optimize=True))
+        Y = Ynew
+    last = Y[0].copy()
+    for j in range(1, m):
+        last = last + Y[j]
+    vec_out[N - 1] = jreshape(last, (dims[N - 1], out_rks[N - 1], 1))
+    b_new = tt_rounding(TT(vec_out, dims, out_rks, ot), rmax=rmax)
+    sketch_b_new, W_B_new, _ = tt_combined_sketch(b_new, rks, s, draws, draws_extra, orthogonal, block_rks)
+    beta = float(np.linalg.norm(sketch_b_new))
+    sketch_b_new = sketch_b_new / beta
+    root = b_new.ot.index(0)  # W_B_new[i] ./= beta for i = 2:findfirst(ot == 0) (1-based)
+    for i in range(1, root + 1):
+        W_B_new[i] = W_B_new[i] / beta
+    b_new = tt_scale(1.0 / beta, b_new)
+    for win, item in ((B_window, b_new), (B_sketch_window, sketch_b_new), (W_B_window, W_B_new)):
+        win.append(item)
+        if len(win) > k_trunc:
+            win.pop(0)
+    return b_new, sketch_b_new, sketch_Hb_prev
+
+
+def sketched_rr(H: TTO, B: Sequence[TT], C: np.ndarray, D: np.ndarray, rmax: int, draws_round, draws_sample,
+                orthogonal: bool = True, block_rks: Optional[int] = 8, kappa_max: Optional[float] = None,
+                stable: bool = False):
+    """``sketched_rr`` (``src/tt_solvers.jl:466-514``), sketched Rayleigh-Ritz
+    (Tropp & Nakatsukasa 2022) on the basis ``B`` with sketches ``C = Omega B``,
+    ``D = Omega H B``: ``stable=false`` solves ``C M = D`` in the least-squares
+    sense (column-pivoted QR) and takes ``eigen(M)``; ``stable=true`` whitens with
+    the thin SVD of ``C``, drops directions with ``sigma/sigma_1 <= 1/kappa_max``
+    and solves ``U^T D V_k z = lambda diag(sigma_k) z``.  The Ritz vector is the
+    randomized rounding of ``sum_j y_j B_j`` to ranks ``4 rmax`` (``draws_round``),
+    SVD-rounded at ``tol = res_sketch^2/2`` and normalised; the sampled residual
+    compares the boundary sketches of ``H X`` and ``X`` of size ``2N`` drawn from
+    ``draws_sample``.  Returns ``(lambdas, X, res_sketch, res_sample)``."""
+    N = B[0].N
+    if kappa_max is None:
+        kappa_max = 1.0 / math.sqrt(np.finfo(np.float64).eps)
+    if stable:
+        U, sig, Vt = np.linalg.svd(C, full_matrices=False)
+        keep = [i for i in range(len(sig)) if sig[i] / sig[0] > 1.0 / kappa_max]
+        k = keep[-1] + 1 if keep else 1
+        Uk, Vk, sk = U[:, :k], Vt[:k].T, sig[:k]
+        M = Uk.T @ D @ Vk
+        lam, Z = _eig_sorted(M, np.diag(sk))
+        Yv = np.real(Vk @ Z)
+        for j in range(k):
+            Yv[:, j] = Yv[:, j] / np.linalg.norm(C @ Yv[:, j])
+    else:
+        M = np.linalg.lstsq(C, D, rcond=None)[0]
+        lam, Yv = _eig_sorted(M)
+        Yv = np.real(Yv)
+    lam = np.real(lam)
+    res_sketch = float(np.linalg.norm(D @ Yv[:, 0] - lam[0] * (C @ Yv[:, 0])))
+    y = Yv[:, 0] / np.linalg.norm(Yv[:, 0])
+    rks = [1] + [4 * rmax] * (N - 1) + [1]
+    X = ttrand_rounding_lincomb(list(y), list(B), rks, draws_round, orthogonal, block_rks)
+    X = tt_rounding(X, tol=res_sketch ** 2 / 2)
+    # X / norm(X): when res_sketch^2/2 exceeds norm(X) the rounding returned
+    # zeros_tt (src/tt_rounding.jl:176-178) and the reference's 1/0 = Inf turns
+    # the Ritz vector into NaNs; restated as such
+    with np.errstate(divide="ignore", invalid="ignore"):
+        X = tt_scale(np.float64(1.0) / np.float64(tt_norm(X)), X)
+        S1, _ = tt_sketch(X, 2 * N, draws_sample, orthogonal, block_rks, H=H)
+        S2, _ = tt_sketch(X, 2 * N, draws_sample, orthogonal, block_rks)
+        res_sample = float(np.linalg.norm(S1 - lam[0] * S2) / np.linalg.norm(S2))
+    return lam, X, res_sketch, res_sample
+
+
+def sketched_rayleigh_ritz(H: TTO, b0: TT, d: int, rmax: int, draws_for, orthogonal: bool = True,
+                           block_rks: Optional[int] = 8, seed: int = 1234, sketch_size: Optional[int] = None,
+                           k_trunc: Optional[int] = None, e_nuc: float = 0.0, sample_seeds=None):
+    """``sketched_rayleigh_ritz(H, b0, d, rmax)`` (``src/tt_solvers.jl:548-620``):
+    a ``d``-dimensional sketch-orthogonal Krylov basis from ``b0`` (normalised by
+    its boundary sketch), with a sketched Rayleigh-Ritz after every expansion and
+    a final one over all ``d`` vectors.  ``draws_for(seed)`` returns the draws of
+    every sketch made with that seed (per core, long enough for any block
+    structure; ``seed + 1`` feeds the combined sketches' extensions);
+    ``sample_seeds`` are the seeds the reference draws with ``rand(Int)`` for the
+    sampled residuals.  Returns ``(e_sketch, psi,
+    history)``; history entries are dicts with ``iter, e_sketch, e_true,
+    res_sketch, res_sample``."""
+    N = b0.N
+    if sketch_size is None:
+        sketch_size = 4 * d
+    if k_trunc is None:
+        k_trunc = d - 1
+    rks = [2 * rmax] * (N + 1)
+    rks[N] = 1
+    samp = list(sample_seeds) if sample_seeds is not None else [None] * d
+
+    def comb(x, Hx, s):
+        return tt_combined_sketch(x, rks, s, draws_for(seed), draws_for(seed + 1), orthogonal, block_rks, H=Hx)
+
+    sk1, W_B_1, _ = comb(b0, None, sketch_size)
+    s = len(sk1)
+    beta = float(np.linalg.norm(sk1))
+    sk1 = sk1 / beta
+    root = b0.ot.index(0)
+    for i in range(1, root + 1):
+        W_B_1[i] = W_B_1[i] / beta
+    B = [tt_scale(1.0 / beta, b0)]
+    sketch_b, sketch_Hb = [sk1], []
+    B_window, W_B_window, B_sketch_window = [B[0]], [W_B_1], [sk1]
+    history = []
+
+    def srr(Bs, sb, sHb, ss):
+        C = np.column_stack(sb)
+        D = np.column_stack(sHb)
+        return sketched_rr(H, Bs, C, D, rmax, draws_for(seed), draws_for(ss), orthogonal, block_rks)
+
+    for j in range(2, d + 1):
+        b_new, sk_new, sk_Hb = expand_basis(H, B_window, W_B_window, B_sketch_window, rmax, s,
+                                            draws_for(seed), draws_for(seed + 1), orthogonal, block_rks, k_trunc)
+        B.append(b_new)
+        sketch_b.append(sk_new)
+        sketch_Hb.append(sk_Hb)
+        lam, psi, r1, r2 = srr(B[:j - 1], sketch_b[:j - 1], sketch_Hb[:j - 1], samp[j - 2])
+        with np.errstate(invalid="ignore"):
+            e_true = dot_operator(psi, H, psi) + e_nuc
+        history.append(dict(iter=j - 1, e_sketch=lam[0] + e_nuc, e_true=e_true, res_sketch=r1, res_sample=r2))
+    skH, _, _ = comb(B[-1], H, s)
+    sketch_Hb.append(skH)
+    lam, psi, r1, r2 = srr(B, sketch_b, sketch_Hb, samp[d - 1])
+    e = lam[0] + e_nuc
+    with np.errstate(invalid="ignore"):
+        e_true = dot_operator(psi, H, psi) + e_nuc
+    history.append(dict(iter=d, e_sketch=e, e_true=e_true, res_sketch=r1, res_sample=r2))
+    return e, psi, history
 
 
 # ---------------------------------------------------------------------------

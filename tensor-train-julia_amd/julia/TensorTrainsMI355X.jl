@@ -8,10 +8,10 @@ GPU methods with the reference's names and keyword defaults
 src/sketches.jl:175-315,317-472,645-817,930-1001,1047-1070,1126-1156,
 src/tt_randtools.jl:552-638,
 src/tt_operations.jl:81-112,170-194,252-258,273-290,
-302-421, and the NTuple methods src/tt_randtools.jl:364-473, src/sketches.jl:474-615,
-819-909,1003-1040); the
-solver callers (tt_gmres, arnoldi, davidson, als/mals/dmrg via orthogonalize)
-need no change.  Every call uploads the cores (column-major (n, r, r') arrays,
+302-421, the NTuple methods src/tt_randtools.jl:364-473, src/sketches.jl:474-615,
+819-909,1003-1040, and expand_basis! src/tt_solvers.jl:354-455); the
+solver callers (tt_gmres, arnoldi, davidson, sketched_rr, sketched_rayleigh_ritz,
+als/mals/dmrg via orthogonalize) need no change.  Every call uploads the cores (column-major (n, r, r') arrays,
 exactly TTvector.ttv_vec), runs on the GPU and copies the result back into new
 Julia arrays.
 
@@ -33,8 +33,9 @@ module TensorTrainsMI355X
 
 using TensorTrains
 using Random
-import TensorTrains: tt_rounding, tt_rounding!, orthogonalize, ttrand_rounding, tt_recursive_sketch,
+import TensorTrains: tt_rounding, tt_rounding!, orthogonalize, ttrand_rounding, tt_recursive_sketch, expand_basis!,
                      tt_sketch, tt_combined_sketch, stta, stta_sketch, dot_par, dot_operator, partial_contraction
+import LinearAlgebra
 import LinearAlgebra: dot
 import Base: *
 
@@ -217,7 +218,7 @@ _prec(T) = haskey(PREC, T) ? PREC[T] :
 
 _opts(orthogonal, T, block_rks, seed; p=Ptr{Int64}(C_NULL), forward=false, blocks=Ptr{Ptr{Float64}}(C_NULL),
       blocks_right=Ptr{Ptr{Float64}}(C_NULL), accum64::Bool=false) =
-    Ref(RandOpts(orthogonal ? 1 : 0, _prec(T), block_rks, UInt64(seed), p, 1.0, Int32(forward ? 1 : 0), blocks,
+    Ref(RandOpts(orthogonal ? 1 : 0, _prec(T), block_rks, seed % UInt64, p, 1.0, Int32(forward ? 1 : 0), blocks,
                  blocks_right, Int32(accum64 ? 1 : 0)))
 
 # The reference's sketch blocks for `seed`: default_rng seeded, then
@@ -463,6 +464,9 @@ function _combined_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}
     end
     W[1] = H === nothing ? ones(Float64, ntuple(_ -> 1, length(_bond_shape(A, 1)) + 1)...) : ones(Float64, 1, 1, 1)
     sketch_rks[1] = 1
+    # a TT's sketches as Vector{Matrix{Float64}}, the type the reference's solvers keep
+    # them in (W_B_window::Vector{Vector{Matrix{T}}}, src/tt_solvers.jl:354-357)
+    (H === nothing && A isa TTvector) && return sketch, Matrix{Float64}[w for w in W], sketch_rks
     return sketch, W, sketch_rks
 end
 
@@ -550,7 +554,7 @@ function _stta(α::Vector{Float64}, y::Vector{TTvector{Float64,N}}, rks::Vector{
         opts = _opts(orthogonal, Float64, block_rks, seed_left; blocks=reference_draws ? pointer(pl) : nb,
                      blocks_right=reference_draws ? pointer(pr) : nb)
         check(ccall((:ttr_stta, lib), Cint, (Ptr{Cvoid}, Int64, Ref{RandItem}, Ref{RandOpts}, UInt64, Ref{Int64}),
-                    handle(), 1, item, opts, UInt64(seed_right), out))
+                    handle(), 1, item, opts, seed_right % UInt64, out))
     end
     foreach(free, ids)
     return download(out[], Val(N))
@@ -580,13 +584,13 @@ function stta_sketch(x::TTvector{Float64,N}, rks::Vector{Int}; seed_left=1234, s
     # the sketch ranks first (no outputs), then the arrays
     check(ccall((:ttr_stta_sketch, lib), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{RandOpts}, UInt64,
                 Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ptr{Int64}, Ptr{Int64}),
-                handle(), id, r, opts, UInt64(seed_right), nullp, nullp, sl, sr))
+                handle(), id, r, opts, seed_right % UInt64, nullp, nullp, sl, sr))
     Ω = [zeros(Float64, sl[k + 1], sr[k + 1]) for k in 1:N-1]
     Ψ = [zeros(Float64, x.ttv_dims[k], sl[k], sr[k + 1]) for k in 1:N]
     GC.@preserve Ω Ψ BL BR pl pr begin
         check(ccall((:ttr_stta_sketch, lib), Cint, (Ptr{Cvoid}, Int64, Ptr{Int64}, Ref{RandOpts}, UInt64,
                     Ptr{Ptr{Float64}}, Ptr{Ptr{Float64}}, Ptr{Int64}, Ptr{Int64}),
-                    handle(), id, r, opts, UInt64(seed_right), [pointer(o) for o in Ω], [pointer(q) for q in Ψ],
+                    handle(), id, r, opts, seed_right % UInt64, [pointer(o) for o in Ω], [pointer(q) for q in Ψ],
                     sl, sr))
     end
     free(id)
@@ -606,6 +610,41 @@ function stta_sketch(x::TTvector{Float64,N}, L::TTvector{Float64,N}, R::TTvector
     end
     free(a); free(l); free(r)
     return Ω, Ψ
+end
+
+# ------------------------------------------------------------------ sketched Krylov basis
+
+# expand_basis! (src/tt_solvers.jl:354-455).  Its left-to-right sweep over H b_prev and
+# -h_i b_i with the window's sketches (drawn from the same seed at ranks [2rmax, ..., 2rmax, 1])
+# is the operator form of the randomized rounding, so it runs as one device call (rand_round
+# sketches the terms itself from the reference's draws: the same sketches as the window's
+# stored ones up to rounding); h, the normalisation and the window bookkeeping stay on the
+# host.  The other solver-side callers, sketched_rr and sketched_rayleigh_ritz
+# (src/tt_solvers.jl:466-689), need no method here: they reach the device through
+# ttrand_rounding, tt_rounding, tt_sketch, tt_combined_sketch, dot_operator and this method.
+function expand_basis!(H::TToperator{Float64,N}, B_window::Vector{TTvector{Float64,N}},
+                       W_B_window::Vector{Vector{Matrix{Float64}}}, B_sketch_window::Vector{Vector{Float64}},
+                       rmax::Int, s::Int; orthogonal::Bool=true, block_rks=8, seed::Int=1234,
+                       k_trunc::Int=length(B_window)) where {N}
+    b_prev = B_window[end]
+    sketch_Hb_prev, _, _ = tt_combined_sketch(Float64, H, b_prev, 2rmax, s; orthogonal=orthogonal, reverse=true,
+                                              seed=seed, block_rks=block_rks)
+    h = hcat(B_sketch_window...) \ sketch_Hb_prev
+    y = rand_round(vcat(1.0, -h), H, vcat([b_prev], B_window), vcat(fill(2rmax, N), 1);
+                   orthogonal=orthogonal, seed=seed, block_rks=block_rks)
+    b_new = tt_rounding(y; rmax=rmax)
+    sketch_b_new, W_B_new, _ = tt_combined_sketch(Float64, b_new, 2rmax, s; orthogonal=orthogonal, reverse=true,
+                                                  seed=seed, block_rks=block_rks)
+    β = LinearAlgebra.norm(sketch_b_new)
+    sketch_b_new ./= β
+    for i = 2:findfirst(isequal(0), b_new.ttv_ot)
+        W_B_new[i] ./= β
+    end
+    b_new = b_new / β
+    push!(B_window, b_new);               length(B_window) > k_trunc && popfirst!(B_window)
+    push!(B_sketch_window, sketch_b_new); length(B_sketch_window) > k_trunc && popfirst!(B_sketch_window)
+    push!(W_B_window, W_B_new);           length(W_B_window) > k_trunc && popfirst!(W_B_window)
+    return b_new, sketch_b_new, sketch_Hb_prev
 end
 
 end # module

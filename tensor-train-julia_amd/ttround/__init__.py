@@ -14,7 +14,8 @@ from .api import (DeviceTT, DeviceTTO, TTvector, TToperator, default_rank_heuris
                   block_structure_forward,
                   tt_rounding_, tt_scale, ttrand_rounding, zeros_tt, is_leftorthogonal, is_rightorthogonal,
                   dot, dot_par, dot_operator, dot_batch, dot_operator_batch, partial_contraction,
-                  hadamard_device, round_rand_tuple_batch, json_to_mps, json_to_mpo, mps_to_json, mpo_to_json)
+                  hadamard_device, round_rand_tuple_batch, json_to_mps, json_to_mpo, mps_to_json, mpo_to_json,
+                  expand_basis_, sketched_rr, sketched_rayleigh_ritz)
 
 __version__ = "0.1.0"
 from .models import hubbard_1d_mpo  # noqa: F401,E402

@@ -1100,3 +1100,211 @@ def stta(*args, seed_left: int = 1234, seed_right: int = 5678, orthogonal: bool 
         for d in devs:
             d.free()
     return res
+
+
+# ------------------------------------------- sketched Krylov basis and Rayleigh-Ritz
+
+def _first_root(x: TTvector) -> int:
+    return list(x.ttv_ot).index(0)
+
+
+def expand_basis_(H: TToperator, B_window: List[TTvector], W_B_window: list, B_sketch_window: List[np.ndarray],
+                  rmax: int, s: int, orthogonal: bool = True, block_rks: Optional[int] = 8, seed: int = 1234,
+                  k_trunc: Optional[int] = None):
+    """``expand_basis!`` (``src/tt_solvers.jl:354-455``): one sketch-orthogonal
+    Arnoldi step over the sliding window, which it mutates.
+
+    The boundary sketch of ``H b_prev`` (``tt_combined_sketch``, ranks ``2 rmax``,
+    size ``s``) gives ``h``, the least-squares coefficients against the window's
+    boundary sketches.  The reference's left-to-right sweep over the terms
+    ``H b_prev, -h_i b_i`` with the sketches of the same seed and ranks is the
+    operator form of the randomized rounding, so it runs as one
+    ``ttrand_rounding([1, -h], H, [b_prev, window...], [2 rmax, ..., 1])`` on the
+    device (which sketches the terms itself: the same draws, so the same sketches
+    as the window's stored ones up to rounding); then ``tt_rounding(rmax)``, the
+    combined sketch of the result and the normalisation by its norm.  Returns
+    ``(b_new, sketch_b_new, sketch_Hb_prev)``."""
+    if k_trunc is None:
+        k_trunc = len(B_window)
+    b_prev = B_window[-1]
+    N = b_prev.N
+    kw = dict(orthogonal=orthogonal, seed=seed, block_rks=block_rks)
+    sketch_Hb_prev, _, _ = tt_combined_sketch(H, b_prev, 2 * rmax, s, **kw)
+    h = np.linalg.lstsq(np.column_stack(B_sketch_window), sketch_Hb_prev, rcond=None)[0]
+    rks = [2 * rmax] * (N + 1)
+    rks[N] = 1
+    y = ttrand_rounding([1.0] + [-float(v) for v in h], H, [b_prev] + list(B_window), rks, **kw)
+    b_new = tt_rounding(y, rmax=rmax)
+    sketch_b_new, W_B_new, _ = tt_combined_sketch(b_new, 2 * rmax, s, **kw)
+    beta = float(np.linalg.norm(sketch_b_new))
+    sketch_b_new = sketch_b_new / beta
+    for i in range(1, _first_root(b_new) + 1):  # W_B_new[i] ./= beta, i = 2:findfirst(ot .== 0)
+        W_B_new[i] = W_B_new[i] / beta
+    b_new = b_new / beta
+    for win, item in ((B_window, b_new), (B_sketch_window, sketch_b_new), (W_B_window, W_B_new)):
+        win.append(item)
+        if len(win) > k_trunc:
+            win.pop(0)
+    return b_new, sketch_b_new, sketch_Hb_prev
+
+
+def _eig_sorted(M: np.ndarray, B: Optional[np.ndarray] = None):
+    # Julia's eigen: eigenvalues sorted by (real, imag), vectors in the same order
+    if B is None:
+        lam, V = np.linalg.eig(M)
+    else:
+        import scipy.linalg
+        lam, V = scipy.linalg.eig(M, B)
+    order = sorted(range(len(lam)), key=lambda i: (lam[i].real, lam[i].imag))
+    return lam[order], V[:, order]
+
+
+def sketched_rr(H: TToperator, B: Sequence[TTvector], C: np.ndarray, D: np.ndarray, rmax: int,
+                orthogonal: bool = True, block_rks: Optional[int] = 8, seed: int = 1234,
+                kappa_max: Optional[float] = None, stable: bool = False, seed_samp: Optional[int] = None,
+                verbose: bool = True):
+    """Sketched Rayleigh-Ritz (``src/tt_solvers.jl:466-514``; Tropp & Nakatsukasa 2022).
+
+    ``C = Omega B`` and ``D = Omega H B`` (``s x d``).  ``stable=False`` solves
+    ``C M = D`` in the least-squares sense and takes ``eigen(M)``;
+    ``stable=True`` whitens with the thin SVD of ``C`` (directions with
+    ``sigma/sigma_1 <= 1/kappa_max`` dropped) and solves the generalized problem.
+    The Ritz vector ``sum_j y_j B_j`` is randomized-rounded to ranks ``4 rmax``
+    and SVD-rounded at ``tol = res_sketch^2/2`` on the device, then normalised
+    (NaN when that rounding returned zeros, as the reference's ``1/0``); the
+    sampled residual compares boundary sketches of ``H X`` and ``X`` of size ``2N``
+    drawn with ``seed_samp`` (random when None, the reference's ``rand(Int)``).
+    Returns ``(lambdas, X, res_sketch, res_sample)``."""
+    N = B[0].N
+    if kappa_max is None:
+        kappa_max = 1.0 / math.sqrt(np.finfo(np.float64).eps)
+    if stable:
+        U, sig, Vt = np.linalg.svd(C, full_matrices=False)
+        if verbose:
+            print(f"Sketch condition number κ(C) ≈ {sig[0] / sig[-1]:.4g}")
+        keep = [i for i in range(len(sig)) if sig[i] / sig[0] > 1.0 / kappa_max]
+        k = keep[-1] + 1 if keep else 1
+        if k < len(B) and verbose:
+            print(f"Warning: basis truncated from d={len(B)} to k={k} (ill-conditioned directions removed)")
+        Vk, sk = Vt[:k].T, sig[:k]
+        lam, Z = _eig_sorted(U[:, :k].T @ D @ Vk, np.diag(sk))
+        Y = np.real(Vk @ Z)
+        for j in range(k):
+            Y[:, j] = Y[:, j] / np.linalg.norm(C @ Y[:, j])
+    else:
+        lam, Y = _eig_sorted(np.linalg.lstsq(C, D, rcond=None)[0])
+        Y = np.real(Y)
+    lam = np.real(lam)
+    res_sketch = float(np.linalg.norm(D @ Y[:, 0] - lam[0] * (C @ Y[:, 0])))
+    y = Y[:, 0] / np.linalg.norm(Y[:, 0])
+    X = ttrand_rounding([float(v) for v in y], list(B), 4 * rmax, orthogonal=orthogonal, seed=seed,
+                        block_rks=block_rks)
+    X = tt_rounding(X, tol=res_sketch ** 2 / 2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        X = tt_scale(np.float64(1.0) / np.float64(norm(X)), X)
+    if seed_samp is None:
+        seed_samp = int(np.random.default_rng().integers(-(1 << 63), (1 << 63) - 1))
+    kw = dict(orthogonal=orthogonal, seed=seed_samp, block_rks=block_rks)
+    S1, _ = tt_sketch(H, X, 2 * N, **kw)
+    S2, _ = tt_sketch(X, 2 * N, **kw)
+    with np.errstate(invalid="ignore"):
+        res_sample = float(np.linalg.norm(S1 - lam[0] * S2) / np.linalg.norm(S2))
+    return lam, X, res_sketch, res_sample
+
+
+def sketched_rayleigh_ritz(H: TToperator, b0: TTvector, d, rmax: Optional[int] = None, orthogonal: bool = True,
+                           block_rks: Optional[int] = 8, seed: int = 1234, sketch_size: Optional[int] = None,
+                           k_trunc: Optional[int] = None, e_nuc: float = 0.0, validate_seed: bool = False,
+                           verbose: bool = True, sample_seeds=None):
+    """``sketched_rayleigh_ritz`` (``src/tt_solvers.jl:548-689``).
+
+    ``sketched_rayleigh_ritz(H, b0, d, rmax)``: a ``d``-dimensional
+    sketch-orthogonal Krylov basis from ``b0`` (normalised by its boundary sketch),
+    a sketched Rayleigh-Ritz after every expansion and a final one over all ``d``
+    vectors; returns ``(e_sketch, psi, history)`` with history entries
+    ``dict(iter, e_sketch, e_true, res_sketch, res_sample)``.
+
+    ``sketched_rayleigh_ritz(H, b0, schedule)`` with ``schedule`` a list of dicts
+    (``d``, ``rmax`` and optionally ``seed``, ``sketch_size``, ``k_trunc``,
+    ``label``): one stage per entry, each restarted from the previous stage's Ritz
+    vector rounded to the stage's ``rmax``; returns ``(e, psi, stages,
+    seeds_used)``.  ``sample_seeds`` (optional) fixes the seeds of the sampled
+    residuals, which the reference draws with ``rand(Int)``."""
+    if not isinstance(d, (int, np.integer)):
+        return _srr_schedule(H, b0, list(d), orthogonal, block_rks, seed, e_nuc, validate_seed, verbose)
+    d, rmax = int(d), int(rmax)
+    N = b0.N
+    if sketch_size is None:
+        sketch_size = 4 * d
+    if k_trunc is None:
+        k_trunc = d - 1
+    samp = list(sample_seeds) if sample_seeds is not None else [None] * d
+    kw = dict(orthogonal=orthogonal, seed=seed, block_rks=block_rks)
+    sk1, W_B_1, _ = tt_combined_sketch(b0, 2 * rmax, sketch_size, **kw)
+    s = len(sk1)
+    beta = float(np.linalg.norm(sk1))
+    sk1 = sk1 / beta
+    for i in range(1, _first_root(b0) + 1):
+        W_B_1[i] = W_B_1[i] / beta
+    B = [b0 / beta]
+    sketch_b, sketch_Hb = [sk1], []
+    B_window, W_B_window, B_sketch_window = [B[0]], [W_B_1], [sk1]
+    history = []
+
+    def srr(Bs, ss):
+        C = np.column_stack(sketch_b[:len(Bs)])
+        Dm = np.column_stack(sketch_Hb[:len(Bs)])
+        lam, psi, r1, r2 = sketched_rr(H, Bs, C, Dm, rmax, seed_samp=ss, verbose=verbose, **kw)
+        with np.errstate(invalid="ignore"):
+            e_true = dot_operator(psi, H, psi) + e_nuc
+        return lam[0] + e_nuc, psi, dict(iter=len(Bs), e_sketch=lam[0] + e_nuc, e_true=e_true,
+                                         res_sketch=r1, res_sample=r2)
+
+    for j in range(2, d + 1):
+        if verbose:
+            print(f"─── k = {j - 1} " + "─" * 44)
+        b_new, sk_new, sk_Hb = expand_basis_(H, B_window, W_B_window, B_sketch_window, rmax, s, k_trunc=k_trunc,
+                                             **kw)
+        B.append(b_new)
+        sketch_b.append(sk_new)
+        sketch_Hb.append(sk_Hb)
+        _, psi, rec = srr(B[:j - 1], samp[j - 2])
+        history.append(rec)
+        if verbose:
+            print(f"  Sketched Ritz value:    {rec['e_sketch']}\n  True Ritz value:        {rec['e_true']}\n"
+                  f"  Sketch residual:  {rec['res_sketch']}\n  Sampled residual: {rec['res_sample']}")
+    if verbose:
+        print(f"─── k = {d} (final) " + "─" * 45)
+    skH, _, _ = tt_combined_sketch(H, B[-1], 2 * rmax, s, **kw)
+    sketch_Hb.append(skH)
+    e, psi, rec = srr(B, samp[d - 1])
+    history.append(rec)
+    if verbose:
+        print(f"  Sketched Ritz:    {rec['e_sketch']}\n  True Ritz:        {rec['e_true']}\n"
+              f"  Sketch residual:  {rec['res_sketch']}\n  Sampled residual: {rec['res_sample']}")
+    return e, psi, history
+
+
+def _srr_schedule(H, b0, schedule, orthogonal, block_rks, seed, e_nuc, validate_seed, verbose):
+    # multi-stage form (src/tt_solvers.jl:622-689)
+    stages, seeds_used = [], []
+    psi, lam = b0, 0.0
+    rng = np.random.default_rng()
+    for i, stage in enumerate(schedule, start=1):
+        d, rmax = int(stage["d"]), int(stage["rmax"])
+        s = int(stage.get("seed", seed + i - 1))
+        sk = int(stage.get("sketch_size", 4 * d))
+        kt = int(stage.get("k_trunc", d - 1))
+        lbl = stage.get("label", f"d={d}, rmax={rmax}")
+        b_init = psi if i == 1 else tt_rounding(psi, rmax=rmax)
+        if validate_seed:
+            while abs(np.linalg.norm(tt_combined_sketch(b_init, 2 * rmax, sk, orthogonal=orthogonal, seed=s,
+                                                         block_rks=block_rks)[0]) ** 2 - 1) > 0.1:
+                s = int(rng.integers(-(1 << 63), (1 << 63) - 1))
+            if verbose:
+                print(f"Stage {i}: using seed {s}")
+        seeds_used.append(s)
+        lam, psi, hist = sketched_rayleigh_ritz(H, b_init, d, rmax, orthogonal=orthogonal, block_rks=block_rks,
+                                                seed=s, k_trunc=kt, sketch_size=sk, e_nuc=e_nuc, verbose=verbose)
+        stages.append((lbl, hist))
+    return lam, psi, stages, seeds_used
