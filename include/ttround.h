@@ -240,13 +240,16 @@ int ttr_round_rand(ttr_handle_t h, int64_t n, const ttr_rand_item* items, const 
 int ttr_round_rand_tuple(ttr_handle_t h, int64_t n, const int64_t* nfactors, const int64_t* factor_ids,
                          const int64_t* rks, const ttr_rand_opts* opts, int64_t* out_ids);
 
-/* The right-to-left recursive sketch of the entrywise product of m TTs, never
- * formed: W_out[k] (prod_f r_{f,k} x sketch_rks[k], column-major, the first
- * factor's rank index fastest; NULL entries skipped), sketch ranks into
- * sketch_rks (d+1).  opts->forward must be 0.
- * Replaces: tt_recursive_sketch(T, A::NTuple, rks; reverse=true, ...)
- * (src/sketches.jl:474-535,600-604) with contract_sketch_core_kronecker_backwards!
- * (src/sketches.jl:1735-1881). */
+/* The recursive sketch of the entrywise product of m TTs, never formed:
+ * W_out[k] (prod_f r_{f,k} x sketch_rks[k], column-major, the first factor's
+ * rank index fastest; NULL entries skipped), sketch ranks into sketch_rks (d+1).
+ * opts->forward = 0: right to left (any sketch precision); opts->forward = 1:
+ * left to right, FP64, blocks from the forward structure (opts->p and
+ * opts->oversampling as for ttr_sketch).
+ * Replaces: tt_recursive_sketch(T, A::NTuple, rks; reverse=true|false, ...)
+ * (src/sketches.jl:474-604) with contract_sketch_core_kronecker_backwards!
+ * (src/sketches.jl:1735-1881) and contract_sketch_core_kronecker_forwards!
+ * (src/sketches.jl:1332-1478). */
 int ttr_sketch_tuple(ttr_handle_t h, int64_t m, const int64_t* factor_ids, const int64_t* rks,
                      const ttr_rand_opts* opts, double* const* W_out, int64_t* sketch_rks);
 

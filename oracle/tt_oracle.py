@@ -41,6 +41,7 @@ __all__ = [
     "hubbard_1d_mpo", "hubbard_1d_dense", "hadamard", "contract_sketch_core_kronecker_backwards",
     "tt_recursive_sketch_tuple", "tt_sketch_tuple", "ttrand_rounding_tuple",
     "expand_basis", "sketched_rr", "sketched_rayleigh_ritz",
+    "contract_sketch_core_kronecker_forwards", "tt_recursive_sketch_tuple_forward",
 ]
 
 
@@ -700,16 +701,27 @@ def tt_recursive_sketch_op(H: TTO, A: TT, rks, draws, orthogonal: bool = True,
 
 
 def tt_sketch(A: TT, s: int, draws, orthogonal: bool = True, block_rks: Optional[int] = None,
-              H: Optional[TTO] = None):
+              H: Optional[TTO] = None, reverse: bool = True):
     """Boundary-only reverse sketch ``tt_sketch(T, [H,] A, s; reverse=true)``
     (``src/sketches.jl:645-723``, operator form ``:729-813``).
 
     The same block count ``p = cld(s, bv_1)`` is used at every core and only
     the boundary ``W_1`` of shape ``(1, bv_1, p)`` (``(1, 1, bv_1, p)`` with an
     operator) is kept, divided by ``sqrt(p)`` once.  Returns
-    ``(vec(W_1), bv_1 p)``."""
+    ``(vec(W_1), bv_1 p)``.  ``reverse=False`` (``:679-694,796-811``): the forward
+    sketch with ``p = cld(s, bv_{N+1})`` at every core, ``W_{N+1}`` kept; the same
+    blocks as the forward recursive sketch with ``p = [1, p, ..., p]``."""
     N = A.N
     dims = A.dims
+    if not reverse:
+        bvf, _ = sketch_structure_forward(dims, [1] * (N + 1), block_rks)
+        pc = -(-int(s) // bvf[N])
+        pv = [1] + [pc] * N
+        if H is None:
+            W, _ = tt_recursive_sketch_forward(A, [1] * (N + 1), draws, orthogonal, block_rks, p=pv)
+        else:
+            W, _ = tt_recursive_sketch_forward_op(H, A, [1] * (N + 1), draws, orthogonal, block_rks, p=pv)
+        return jreshape(W[N], -1), bvf[N] * pc
     bv, _ = sketch_structure(dims, [1] * (N + 1), block_rks)
     p = -(-int(s) // bv[0])
     W = np.ones((1, 1, 1)) if H is None else np.ones((1, 1, 1, 1))
@@ -730,27 +742,37 @@ def tt_sketch(A: TT, s: int, draws, orthogonal: bool = True, block_rks: Optional
 
 
 def tt_combined_sketch(A: TT, rks, s: int, draws, draws_extra, orthogonal: bool = True,
-                       block_rks: Optional[int] = None, H: Optional[TTO] = None):
+                       block_rks: Optional[int] = None, H: Optional[TTO] = None, reverse: bool = True):
     """``tt_combined_sketch([T,] [H,] A, rks, s; reverse=true)`` (``src/sketches.jl:930-953``,
     operator form ``:966-989``): the recursive sketch with the boundary rank raised
     to ``max(rks_1, min(s, rks_2))``; its boundary sketch, extended by a boundary-only
     sketch of size ``s - s_rec`` from the next seed (``draws_extra``) when ``s`` exceeds
     it, the two parts weighted by ``sqrt(len/s)``.  Returns ``(sketch, W, sketch_rks)``
-    with ``W[1]`` replaced by ones and ``sketch_rks[1] = 1``."""
+    with ``W[1]`` replaced by ones and ``sketch_rks[1] = 1``.  ``reverse=False``: the
+    same at the right boundary (``W[N+1]``, ``rks[N+1]`` raised to
+    ``max(rks[N+1], min(s, rks[N]))``) with the forward sketches."""
     rks = list(rks)
-    rks[0] = max(rks[0], min(int(s), rks[1]))
-    if H is None:
-        W, srk = tt_recursive_sketch(A, rks, draws, orthogonal, block_rks)
+    N = A.N
+    b, inner = (0, 1) if reverse else (N, N - 1)
+    rks[b] = max(rks[b], min(int(s), rks[inner]))
+    if reverse:
+        if H is None:
+            W, srk = tt_recursive_sketch(A, rks, draws, orthogonal, block_rks)
+        else:
+            W, srk = tt_recursive_sketch_op(H, A, rks, draws, orthogonal, block_rks)
     else:
-        W, srk = tt_recursive_sketch_op(H, A, rks, draws, orthogonal, block_rks)
-    sketch = jreshape(W[0], -1)
-    s_rec = srk[0]
+        if H is None:
+            W, srk = tt_recursive_sketch_forward(A, rks, draws, orthogonal, block_rks)
+        else:
+            W, srk = tt_recursive_sketch_forward_op(H, A, rks, draws, orthogonal, block_rks)
+    sketch = jreshape(W[b], -1)
+    s_rec = srk[b]
     if s > s_rec:
-        extra, _ = tt_sketch(A, s - s_rec, draws_extra, orthogonal, block_rks, H)
+        extra, _ = tt_sketch(A, s - s_rec, draws_extra, orthogonal, block_rks, H, reverse=reverse)
         s = s_rec + len(extra)
         sketch = np.concatenate([math.sqrt(len(sketch) / s) * sketch, math.sqrt(len(extra) / s) * extra])
-    W[0] = np.ones((1, 1)) if H is None else np.ones((1, 1, 1))
-    srk[0] = 1
+    W[b] = np.ones((1, 1)) if H is None else np.ones((1, 1, 1))
+    srk[b] = 1
     return sketch, W, srk
 
 
@@ -999,15 +1021,62 @@ def tt_recursive_sketch_tuple(ys: Sequence[TT], rks, draws, orthogonal: bool = T
     return out, [bv[k] * p[k] for k in range(N + 1)]
 
 
-def tt_sketch_tuple(ys: Sequence[TT], s: int, draws, orthogonal: bool = True, block_rks: Optional[int] = None):
-    """Boundary sketch of the entrywise product (``src/sketches.jl:819-900``):
-    the recursive sketch with ``p = cld(s, bv_1)`` blocks at every core."""
+def contract_sketch_core_kronecker_forwards(As: Sequence[np.ndarray], S: np.ndarray, V: np.ndarray) -> np.ndarray:
+    """``W[a_1..a_M, b] = sum_{z, al, be} prod_i A_i[z, al_i, a_i] S[be, z, b]
+    V[al_1..al_M, be]`` (``src/sketches.jl:1332-1478``), factor by factor per mode
+    index, the product core never formed."""
+    M = len(As)
+    out = None
+    for z in range(As[0].shape[0]):
+        T = V
+        for i in range(M):
+            T = np.moveaxis(np.tensordot(As[i][z], T, axes=([0], [i])), 0, i)
+        T = np.tensordot(T, S[:, z, :], axes=([M], [0]))
+        out = T if out is None else out + T
+    return out
+
+
+def tt_recursive_sketch_tuple_forward(ys: Sequence[TT], rks, draws, orthogonal: bool = True,
+                                      block_rks: Optional[int] = None, p=None, oversampling=1):
+    """Forward (left-to-right) sketch of the entrywise product of ``ys`` without
+    forming it (``src/sketches.jl:539-604``): ``W[k+1][..., j]`` contracts the
+    factors' cores ``k`` and block ``j`` with ``W[k][..., j]`` (``W[1] = 1``).
+    ``W[k]`` has shape ``(r_1k, ..., r_Mk, bv_k p_k)``, divided by ``sqrt(p_k)``."""
+    ys = list(ys)
+    M, N, dims = len(ys), ys[0].N, ys[0].dims
+    assert rks[0] == 1 and all(t.rks[0] == 1 for t in ys)
+    bv, p = sketch_structure_forward(dims, rks, block_rks, p, oversampling)
+    W: List[Optional[np.ndarray]] = [None] * (N + 1)
+    W[0] = np.ones((1,) * (M + 2))
+    for k in range(N):
+        S = _sketch_for(draws, k, bv[k], dims[k], bv[k + 1], p[k + 1], orthogonal)
+        Wk = np.zeros(tuple(t.rks[k + 1] for t in ys) + (bv[k + 1], p[k + 1]))
+        for j in range(p[k + 1]):
+            Vp = W[k][..., j if k > 0 else 0]
+            Wk[..., j] = contract_sketch_core_kronecker_forwards([t.cores[k] for t in ys], S[:, :, :, j], Vp)
+        W[k + 1] = Wk
+    out = []
+    for k in range(N + 1):
+        out.append(jreshape(W[k] / math.sqrt(p[k]), tuple(t.rks[k] for t in ys) + (bv[k] * p[k],)))
+    return out, [bv[k] * p[k] for k in range(N + 1)]
+
+
+def tt_sketch_tuple(ys: Sequence[TT], s: int, draws, orthogonal: bool = True, block_rks: Optional[int] = None,
+                    reverse: bool = True):
+    """Boundary sketch of the entrywise product (``src/sketches.jl:819-909``):
+    the recursive sketch with ``p = cld(s, bv_boundary)`` blocks at every core
+    (boundary ``W_1`` right to left, ``W_{N+1}`` left to right)."""
     ys = list(ys)
     N, dims = ys[0].N, ys[0].dims
-    bv, _ = sketch_structure(dims, [1] * (N + 1), block_rks)
-    pc = -(-int(s) // bv[0])
-    W, srk = tt_recursive_sketch_tuple(ys, [1] * (N + 1), draws, orthogonal, block_rks, p=[pc] * N + [1])
-    return np.reshape(W[0], -1, order="F"), bv[0] * pc
+    if reverse:
+        bv, _ = sketch_structure(dims, [1] * (N + 1), block_rks)
+        pc = -(-int(s) // bv[0])
+        W, srk = tt_recursive_sketch_tuple(ys, [1] * (N + 1), draws, orthogonal, block_rks, p=[pc] * N + [1])
+        return np.reshape(W[0], -1, order="F"), bv[0] * pc
+    bv, _ = sketch_structure_forward(dims, [1] * (N + 1), block_rks)
+    pc = -(-int(s) // bv[N])
+    W, srk = tt_recursive_sketch_tuple_forward(ys, [1] * (N + 1), draws, orthogonal, block_rks, p=[1] + [pc] * N)
+    return np.reshape(W[N], -1, order="F"), bv[N] * pc
 
 
 def ttrand_rounding_tuple(ys: Sequence[TT], rks, draws, orthogonal: bool = True,

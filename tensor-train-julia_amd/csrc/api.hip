@@ -2924,10 +2924,126 @@ int ttr_round_rand_tuple(ttr_handle_t h, int64_t n, const int64_t* nfactors, con
   });
 }
 
+namespace {
+// Left-to-right recursive sketch of the entrywise product of the factors f, never formed
+// (tt_recursive_sketch(T, A::NTuple, rks; reverse=false), src/sketches.jl:539-604), FP64, into
+// host arrays.  Per core k the blocks j < nb of W_k (rows: the factors' left rank indices,
+// first factor fastest) are contracted with the factors' cores one factor at a time
+// (contract_sketch_core_kronecker_forwards!, src/sketches.jl:1332-1478): factor 0 for every
+// (mode index z, block j) as one GEMM over all remaining columns, factor m >= 1 for every
+// (remaining column, z, j) as one GEMM of the already-contracted indices against A_m[z].
+// Intermediates are laid out [contracted, uncontracted, beta, z, j], so the last factor
+// leaves T_j = [(a_1..a_M), (beta, z)] with beta fastest, the row order of the forward
+// blocks S_j ((bv_k n_k) x bv_{k+1}), and W_{k+1}[:, block j] = T_j S_j is one GEMM per block.
+// W_0 = 1 serves every block of core 0.  Cost n_k sum_m (rank products) per block and core
+// instead of n_k P_k P_{k+1}.
+void sketch_tuple_fwd(ttr_handle_s* h, const std::vector<DevTT*>& f, const int64_t* rks, const ttr_rand_opts* opts,
+                      double* const* W_out, int64_t* sketch_rks) {
+  Ctx& ctx = h->ctx;
+  TTR_REQUIRE(rks != nullptr, ST_INVALID_ARG, "rks must not be NULL");
+  TTR_REQUIRE(to_prec(opts->sketch_prec) == Prec::F64, ST_INVALID_ARG, "the forward sketch is FP64");
+  const int d = f[0]->d;
+  const size_t M = f.size();
+  TTR_REQUIRE(rks[0] == 1, ST_INVALID_ARG, "rks[0] must be 1");
+  for (auto* x : f) TTR_REQUIRE(x->rks[0] == 1 && x->rks[size_t(d)] == 1, ST_INVALID_ARG,
+                                "factors need rks[0] = rks[d] = 1");
+  auto P = [&](int k) {
+    int64_t v = 1;
+    for (auto* x : f) v *= x->rks[size_t(k)];
+    return v;
+  };
+  for (int k = 0; k <= d; ++k) TTR_REQUIRE(P(k) < (int64_t(1) << 31), ST_INVALID_ARG, "product rank too large");
+  FwdPlan fp;
+  prepare_fwd(opts, d, f[0]->dims, fp);
+  const std::vector<int64_t> r(rks, rks + d + 1);
+  const std::vector<int64_t> p = block_counts_fwd(r, fp.bv, d, opts->p, opts->oversampling);
+  fp.pmax = p;
+  make_sketch_fwd(h, fp, d);
+  // |T^(m+1)| = (prod_{q<=m} R_q) (prod_{q>m} L_q) bv_k n_k nb: the largest over cores and factors
+  int64_t maxU = 1, maxT = 1;
+  for (int k = 0; k < d; ++k) {
+    const int64_t nb = k == 0 ? 1 : p[size_t(k) + 1], tail = fp.bv[size_t(k)] * fp.dims[size_t(k)] * nb;
+    for (size_t m = 0; m < M; ++m) {
+      int64_t v = tail;
+      for (size_t q = 0; q <= m; ++q) v *= f[q]->rks[size_t(k) + 1];
+      for (size_t q = m + 1; q < M; ++q) v *= f[q]->rks[size_t(k)];
+      if (m + 1 == M) maxT = std::max(maxT, v);
+      else maxU = std::max(maxU, v);
+    }
+  }
+  std::vector<double*> W(size_t(d) + 1, nullptr);
+  for (int k = 1; k <= d; ++k) W[size_t(k)] = ctx.ws.alloc_n<double>(size_t(P(k) * fp.bv[size_t(k)] * p[size_t(k)]));
+  double* one = ctx.ws.alloc_n<double>(1);
+  double* U[2] = {ctx.ws.alloc_n<double>(size_t(maxU)), ctx.ws.alloc_n<double>(size_t(maxU))};
+  double* TM = ctx.ws.alloc_n<double>(size_t(maxT));
+  fill2d_batched(ctx, {Fill2D{one, 1, 1, 1, 1.0}});
+  GemmBatch gb;
+  for (int k = 0; k < d; ++k) {
+    const int64_t n = fp.dims[size_t(k)], bl = fp.bv[size_t(k)], br = fp.bv[size_t(k) + 1];
+    const int64_t nb = k == 0 ? 1 : p[size_t(k) + 1];
+    const double* V = k == 0 ? one : W[size_t(k)];
+    const int64_t Pk = P(k), Pk1 = P(k + 1);
+    int64_t pre = 1;  // prod of the right ranks of the factors already contracted
+    for (size_t m = 0; m < M; ++m) {
+      const DevTT* x = f[m];
+      const int64_t L = x->rks[size_t(k)], R = x->rks[size_t(k) + 1];
+      int64_t mid = 1;  // left ranks of the factors not yet contracted
+      for (size_t q = m + 1; q < M; ++q) mid *= f[q]->rks[size_t(k)];
+      const int64_t MB = mid * bl;
+      const double* src = m == 0 ? V : U[(m - 1) & 1];
+      double* dst = m + 1 == M ? TM : U[m & 1];
+      for (int64_t j = 0; j < nb; ++j)
+        for (int64_t z = 0; z < n; ++z) {
+          const double* Az = x->cores[size_t(k)] + L * z;  // A_m[z](al, a) = core[al + L (z + n a)]
+          const int64_t zj = z + n * j;
+          if (m == 0) {  // [a_0, (mid, beta)] = A_0[z]^T V_j, V_j = L x MB (ld L)
+            gb.add(int(R), int(MB), int(L), 1.0, Az, L * n, 1, src + Pk * bl * j, 1, L, 0.0, dst + R * MB * zj, 1, R);
+            continue;
+          }
+          for (int64_t c = 0; c < MB; ++c)  // slab [pre, al_m] (ld pre) times A_m[z]
+            gb.add(int(pre), int(R), int(L), 1.0, src + pre * L * (c + MB * zj), 1, pre, Az, 1, L * n, 0.0,
+                   dst + pre * R * (c + MB * zj), 1, pre);
+        }
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "tuple_fwd_cores");
+      pre *= R;
+    }
+    // W_{k+1}[:, block j] = T_j S_j over (beta, z), T_j = TM block j (P_{k+1} x bv_k n)
+    for (int64_t j = 0; j < p[size_t(k) + 1]; ++j)
+      gb.add(int(Pk1), int(br), int(bl * n), 1.0, TM + Pk1 * bl * n * (k == 0 ? 0 : j), 1, Pk1,
+             fp.S[size_t(k)] + bl * n * br * j, 1, bl * n, 0.0, W[size_t(k) + 1] + Pk1 * br * j, 1, Pk1);
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "tuple_fwd_w");
+  }
+  ctx.sync();
+  for (int k = 0; k <= d; ++k) {
+    const int64_t sk = fp.bv[size_t(k)] * p[size_t(k)];
+    if (sketch_rks) sketch_rks[k] = sk;
+    if (!W_out || !W_out[k]) continue;
+    const int64_t cnt = P(k) * sk;
+    const double scale = 1.0 / std::sqrt(double(p[size_t(k)]));
+    if (k == 0) {
+      for (int64_t e = 0; e < cnt; ++e) W_out[k][e] = scale;
+      continue;
+    }
+    TTR_HIP(hipMemcpy(W_out[k], W[size_t(k)], size_t(cnt) * 8, hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < cnt; ++e) W_out[k][e] *= scale;
+  }
+}
+}  // namespace
+
 int ttr_sketch_tuple(ttr_handle_t h, int64_t m, const int64_t* factor_ids, const int64_t* rks,
                      const ttr_rand_opts* opts, double* const* W_out, int64_t* sketch_rks) {
   return guarded(h, [&] {
     TTR_REQUIRE(opts, ST_INVALID_ARG, "options must not be NULL");
+    if (opts->forward) {
+      TTR_REQUIRE(m >= 1 && m <= KRON_MAX && factor_ids, ST_INVALID_ARG, "a product needs 1..8 factors");
+      std::vector<DevTT*> f;
+      for (int64_t q = 0; q < m; ++q) {
+        f.push_back(h->get(factor_ids[q]));
+        check_dims_equal(f[0], f.back());
+      }
+      sketch_tuple_fwd(h, f, rks, opts, W_out, sketch_rks);
+      return;
+    }
     SketchPlan sp;
     const int64_t nf = m;
     auto its = tuple_items(h, 1, &nf, factor_ids, rks, opts, sp);

@@ -331,9 +331,9 @@ function _recursive_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N
     s = bv .* pp
     Hr = H === nothing ? ones(Int, N + 1) : collect(Int, H.tto_rks)
     W = [zeros(Float64, A.ttv_rks[k] * Hr[k], s[k]) for k in 1:N+1]
-    # the reverse sketch of a product runs factor by factor (ttr_sketch_tuple), the
-    # product never formed; the forward one sketches the product formed on the device
-    factored = A isa _Product && reverse && length(A.factors) > 1
+    # the sketch of a product runs factor by factor in either direction
+    # (ttr_sketch_tuple), the product never formed
+    factored = A isa _Product && length(A.factors) > 1
     id = factored ? -1 : upload(A)
     fids = factored ? Int64[upload(t) for t in A.factors] : Int64[]
     op = H === nothing ? -1 : upload(H)
@@ -380,6 +380,8 @@ _rks_of(rks::Vector{Int}, N) = rks
 
 _rks_of_fwd(rmax::Int, N) = (r = fill(rmax, N + 1); r[1] = 1; r)
 _rks_of_fwd(rks::Vector{Int}, N) = rks
+# the rank vector of an rmax in the direction the keywords ask for (src/sketches.jl:955-958)
+_rks_dir(rks_or_rmax, N, kw) = get(kw, :reverse, true) ? _rks_of(rks_or_rmax, N) : _rks_of_fwd(rks_or_rmax, N)
 
 tt_recursive_sketch(::Type{T}, A::TTvector{Float64,N}, rks_or_rmax; reverse=true,
                     kwargs...) where {T<:Union{Float64,Float32},N} =
@@ -397,7 +399,8 @@ tt_recursive_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rma
 # tt_sketch(T, [H,] A, s; reverse=true, ...) (src/sketches.jl:645-817): boundary sketch only
 function _tt_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}}, s::Int; reverse=true, orthogonal=true,
                     seed=1234, block_rks::Int=N, reference_draws::Bool=true, timer=nothing) where {T,N}
-    reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
+    reverse || return _tt_sketch_fwd(T, H, A, s; orthogonal=orthogonal, seed=seed, block_rks=block_rks,
+                                     reference_draws=reference_draws)
     # the same block count p = cld(s, bv_1) at every core (src/sketches.jl:665-668)
     bv = _block_ranks(A.ttv_dims, block_rks, N)
     pc = cld(s, bv[1])
@@ -428,12 +431,25 @@ end
 # block count p = cld(s, bv_1) at every core (src/sketches.jl:819-900)
 function _tt_sketch(::Type{T}, H::Nothing, A::_Product{N}, s::Int; reverse=true, orthogonal=true, seed=1234,
                     block_rks::Int=N, reference_draws::Bool=true, timer=nothing) where {T,N}
-    reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
+    reverse || return _tt_sketch_fwd(T, nothing, A, s; orthogonal=orthogonal, seed=seed, block_rks=block_rks,
+                                     reference_draws=reference_draws)
     bv = _block_ranks(A.ttv_dims, block_rks, N)
     pc = cld(s, bv[1])
     W, _ = _recursive_sketch(T, nothing, A, ones(Int, N + 1); orthogonal=orthogonal, reverse=true, seed=seed,
                              block_rks=block_rks, p=vcat(fill(pc, N), 1), reference_draws=reference_draws)
     return vec(W[1]), bv[1] * pc
+end
+
+# tt_sketch(...; reverse=false) (src/sketches.jl:679-694,796-811,894-900): the forward
+# recursive sketch with p = cld(s, bv_{N+1}) blocks at every core (the same draws), W_{N+1} kept
+function _tt_sketch_fwd(::Type{T}, H, A, s::Int; orthogonal=true, seed=1234, block_rks::Int=length(A.ttv_dims),
+                        reference_draws::Bool=true) where {T}
+    N = length(A.ttv_dims)
+    bv = _block_ranks_fwd(A.ttv_dims, block_rks, N)
+    pc = cld(s, bv[N + 1])
+    W, _ = _recursive_sketch(T, H, A, ones(Int, N + 1); orthogonal=orthogonal, reverse=false, seed=seed,
+                             block_rks=block_rks, p=vcat(1, fill(pc, N)), reference_draws=reference_draws)
+    return vec(W[N + 1]), bv[N + 1] * pc
 end
 
 tt_sketch(::Type{T}, A::TTvector{Float64,N}, s::Int; kwargs...) where {T<:Union{Float64,Float32},N} =
@@ -449,21 +465,21 @@ function _combined_sketch(::Type{T}, H, A::Union{TTvector{Float64,N},_Product{N}
                           reverse=true,
                           orthogonal=true, seed=1234, block_rks::Int=N, reference_draws::Bool=true,
                           timer=nothing) where {T,N}
-    reverse || throw(ArgumentError("the GPU sketch is the reverse (right-to-left) one"))
+    b, inner = reverse ? (1, 2) : (N + 1, N)
     rks = copy(rks)
-    rks[1] = max(rks[1], min(s, rks[2]))
-    W, sketch_rks = _recursive_sketch(T, H, A, rks; orthogonal=orthogonal, seed=seed, block_rks=block_rks,
-                                      reference_draws=reference_draws)
-    sketch = vec(W[1])
-    s_rec = sketch_rks[1]
+    rks[b] = max(rks[b], min(s, rks[inner]))
+    W, sketch_rks = _recursive_sketch(T, H, A, rks; orthogonal=orthogonal, reverse=reverse, seed=seed,
+                                      block_rks=block_rks, reference_draws=reference_draws)
+    sketch = vec(W[b])
+    s_rec = sketch_rks[b]
     if s > s_rec
-        extra, _ = _tt_sketch(T, H, A, s - s_rec; orthogonal=orthogonal, seed=seed + 1, block_rks=block_rks,
-                              reference_draws=reference_draws)
+        extra, _ = _tt_sketch(T, H, A, s - s_rec; reverse=reverse, orthogonal=orthogonal, seed=seed + 1,
+                              block_rks=block_rks, reference_draws=reference_draws)
         s = s_rec + length(extra)
         sketch = [sqrt(length(sketch) / s) .* sketch; sqrt(length(extra) / s) .* extra]
     end
-    W[1] = H === nothing ? ones(Float64, ntuple(_ -> 1, length(_bond_shape(A, 1)) + 1)...) : ones(Float64, 1, 1, 1)
-    sketch_rks[1] = 1
+    W[b] = H === nothing ? ones(Float64, ntuple(_ -> 1, length(_bond_shape(A, b)) + 1)...) : ones(Float64, 1, 1, 1)
+    sketch_rks[b] = 1
     # a TT's sketches as Vector{Matrix{Float64}}, the type the reference's solvers keep
     # them in (W_B_window::Vector{Vector{Matrix{T}}}, src/tt_solvers.jl:354-357)
     (H === nothing && A isa TTvector) && return sketch, Matrix{Float64}[w for w in W], sketch_rks
@@ -512,16 +528,16 @@ tt_sketch(::Type{T}, A::NTuple{M,TTvector{Float64,N}}, s::Int; kwargs...) where 
 tt_sketch(A::NTuple{M,TTvector{Float64,N}}, s::Int; kwargs...) where {M,N} = tt_sketch(Float64, A, s; kwargs...)
 tt_combined_sketch(::Type{T}, A::NTuple{M,TTvector{Float64,N}}, rks_or_rmax, s::Int;
                    kwargs...) where {T<:Union{Float64,Float32},M,N} =
-    _combined_sketch(T, nothing, _Product(A), _rks_of(rks_or_rmax, N), s; kwargs...)
+    _combined_sketch(T, nothing, _Product(A), _rks_dir(rks_or_rmax, N, kwargs), s; kwargs...)
 tt_combined_sketch(A::NTuple{M,TTvector{Float64,N}}, rks_or_rmax, s::Int; kwargs...) where {M,N} =
     tt_combined_sketch(Float64, A, rks_or_rmax, s; kwargs...)
 
 tt_combined_sketch(::Type{T}, A::TTvector{Float64,N}, rks_or_rmax, s::Int;
                    kwargs...) where {T<:Union{Float64,Float32},N} =
-    _combined_sketch(T, nothing, A, _rks_of(rks_or_rmax, N), s; kwargs...)
+    _combined_sketch(T, nothing, A, _rks_dir(rks_or_rmax, N, kwargs), s; kwargs...)
 tt_combined_sketch(::Type{T}, H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax, s::Int;
                    kwargs...) where {T<:Union{Float64,Float32},N} =
-    _combined_sketch(T, H, A, _rks_of(rks_or_rmax, N), s; kwargs...)
+    _combined_sketch(T, H, A, _rks_dir(rks_or_rmax, N, kwargs), s; kwargs...)
 tt_combined_sketch(A::TTvector{Float64,N}, rks_or_rmax, s::Int; kwargs...) where {N} =
     tt_combined_sketch(Float64, A, rks_or_rmax, s; kwargs...)
 tt_combined_sketch(H::TToperator{Float64,N}, A::TTvector{Float64,N}, rks_or_rmax, s::Int; kwargs...) where {N} =

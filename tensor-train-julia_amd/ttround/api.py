@@ -344,10 +344,11 @@ def round_rand_tuple_batch(items, orthogonal: bool = True, seed: int = 1234, blo
     return [DeviceTT(h, t) for t in out]
 
 
-def _tuple_sketch(factors, rks, orthogonal, seed, block_rks, p, oversampling, sketch_prec, blocks=None):
-    """Right-to-left recursive sketch of the entrywise product of ``factors``
-    (TTvectors), never formed (``ttr_sketch_tuple``): ``W[k]`` of shape
-    ``(r_1k, ..., r_Mk, s_k)``."""
+def _tuple_sketch(factors, rks, orthogonal, seed, block_rks, p, oversampling, sketch_prec, blocks=None,
+                  forward=False):
+    """Recursive sketch of the entrywise product of ``factors`` (TTvectors), never
+    formed (``ttr_sketch_tuple``), right to left or (``forward``, FP64) left to
+    right: ``W[k]`` of shape ``(r_1k, ..., r_Mk, s_k)``."""
     factors = list(factors)
     N = factors[0].N
     if any(t.ttv_dims != factors[0].ttv_dims for t in factors):
@@ -355,9 +356,10 @@ def _tuple_sketch(factors, rks, orthogonal, seed, block_rks, p, oversampling, sk
     devs = [t._dev() for t in factors]
     h = devs[0].handle
     try:
-        o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling, blocks=blocks)
+        o, keep = _opts(orthogonal, sketch_prec, block_rks, seed, p, oversampling, blocks=blocks, forward=forward)
         ra = L.as_i64(rks)
-        bv, pp = block_structure(factors[0].ttv_dims, rks, block_rks, p, oversampling)
+        structure = block_structure_forward if forward else block_structure
+        bv, pp = structure(factors[0].ttv_dims, rks, block_rks, p, oversampling)
         srk = [bv[k] * pp[k] for k in range(N + 1)]
         fr = [[t.ttv_rks[k] for t in factors] for k in range(N + 1)]
         Ws = [np.zeros((int(np.prod(fr[k])), srk[k]), order="F") for k in range(N + 1)]
@@ -838,14 +840,14 @@ def tt_recursive_sketch(*args, orthogonal: bool = True, reverse: bool = True, se
     ``(r_k, R_k, sketch_rks[k])`` for the operator form, or
     ``(r_1k, ..., r_Mk, sketch_rks[k])`` for a tuple of M TTs."""
     if isinstance(args[0], tuple):
-        if reverse and len(args[0]) > 1:  # factored: the product is never formed
+        if len(args[0]) > 1:  # factored: the product is never formed
             N = args[0][0].N
             rks = args[1]
             if isinstance(rks, (int, np.integer)):
                 rks = [int(rks)] * (N + 1)
-                rks[N] = 1
+                rks[N if reverse else 0] = 1
             return _tuple_sketch(args[0], [int(v) for v in rks], orthogonal, seed, block_rks, p, oversampling,
-                                 sketch_prec, blocks)
+                                 sketch_prec, blocks, forward=not reverse)
         P = _Product(args[0])
         W, srk = tt_recursive_sketch(P, *args[1:], orthogonal=orthogonal, reverse=reverse, seed=seed,
                                      block_rks=block_rks, p=p, oversampling=oversampling, sketch_prec=sketch_prec)
@@ -895,10 +897,12 @@ def tt_sketch(*args, reverse: bool = True, orthogonal: bool = True, seed: int = 
     (``src/sketches.jl:645-727``; operator form ``:729-817``, ``H A`` never formed).
 
     Returns ``(sketch, s_actual)``: the boundary sketch of length
-    ``s_actual = bv_1 * cld(s, bv_1)``."""
+    ``s_actual = bv_1 * cld(s, bv_1)``.  ``reverse=False`` (``:679-694,796-811,
+    894-900``, FP64): the forward recursive sketch with ``cld(s, bv_{N+1})`` blocks at
+    every core (the same draws), ``W_{N+1}`` returned."""
     H, (A, s) = _split_op(args)
     if not reverse:
-        raise NotImplementedError("only the reverse (right-to-left) sketch is provided")
+        return _tt_sketch_forward(H, A, int(s), orthogonal, seed, block_rks)
     if isinstance(A, tuple) and len(A) > 1:
         # NTuple form (src/sketches.jl:819-909): the recursive sketch of the
         # product with p = cld(s, bv_1) blocks at every core, never formed
@@ -925,6 +929,21 @@ def tt_sketch(*args, reverse: bool = True, orthogonal: bool = True, seed: int = 
     return out[:, 0].copy(), sa
 
 
+def _tt_sketch_forward(H, A, s, orthogonal, seed, block_rks):
+    if isinstance(A, tuple) and H is not None:
+        raise TypeError("no operator form for a tuple of TTs")
+    if isinstance(A, tuple) and len(A) == 1:
+        A = A[0]
+    first = A[0] if isinstance(A, tuple) else A
+    N = first.N
+    bv, _ = block_structure_forward(first.ttv_dims, [1] * (N + 1), block_rks)
+    pc = -(-int(s) // bv[N])
+    pre = [H] if H is not None else []
+    W, srk = tt_recursive_sketch(*pre, A, [1] * (N + 1), reverse=False, orthogonal=orthogonal, seed=seed,
+                                 block_rks=block_rks, p=[1] + [pc] * N)
+    return np.reshape(W[N], -1, order="F").copy(), bv[N] * pc
+
+
 def tt_combined_sketch(*args, reverse: bool = True, orthogonal: bool = True, seed: int = 1234,
                        block_rks: Optional[int] = None, sketch_prec: str = "f64"):
     """``tt_combined_sketch([H,] A, rks_or_rmax, s; reverse=true, ...)``
@@ -932,30 +951,33 @@ def tt_combined_sketch(*args, reverse: bool = True, orthogonal: bool = True, see
     raised to ``max(rks_1, min(s, rks_2))``, its boundary sketch extended to at
     least ``s`` entries by ``tt_sketch`` with ``seed + 1`` (each part weighted
     by ``sqrt(len/s)``).  Returns ``(sketch, W, sketch_rks)`` where ``W`` is the
-    recursive sketch with ``W[1]`` replaced by ones and ``sketch_rks[1] = 1``."""
+    recursive sketch with ``W[1]`` replaced by ones and ``sketch_rks[1] = 1``.
+    ``reverse=False`` (FP64): the same at the right boundary ``W[N+1]`` with the
+    forward sketches."""
     H, (A, rks, s) = _split_op(args)
-    if not reverse:
-        raise NotImplementedError("only the reverse (right-to-left) sketch is provided")
     M = len(A) if isinstance(A, tuple) else 0  # NTuple form (src/sketches.jl:1003-1040)
     if M and H is not None:
         raise TypeError("no operator form for a tuple of TTs")
     N = A[0].N if M else A.N
+    b, inner = (0, 1) if reverse else (N, N - 1)
     if isinstance(rks, (int, np.integer)):
         rks = [int(rks)] * (N + 1)
-        rks[N] = 1
+        rks[N if reverse else 0] = 1
     rks = [int(v) for v in rks]
-    rks[0] = max(rks[0], min(int(s), rks[1]))
+    rks[b] = max(rks[b], min(int(s), rks[inner]))
     pre = [H] if H is not None else []
     kw = dict(orthogonal=orthogonal, block_rks=block_rks, sketch_prec=sketch_prec)
-    W, srk = tt_recursive_sketch(*pre, A, rks, reverse=True, seed=seed, **kw)
-    sketch = np.reshape(W[0], -1, order="F").copy()
-    s_rec = srk[0]
+    if not reverse:
+        kw.pop("sketch_prec")
+    W, srk = tt_recursive_sketch(*pre, A, rks, reverse=reverse, seed=seed, **kw)
+    sketch = np.reshape(W[b], -1, order="F").copy()
+    s_rec = srk[b]
     if s > s_rec:
-        extra, _ = tt_sketch(*pre, A, int(s) - s_rec, seed=seed + 1, **kw)
+        extra, _ = tt_sketch(*pre, A, int(s) - s_rec, seed=seed + 1, reverse=reverse, **kw)
         s = s_rec + len(extra)
         sketch = np.concatenate([math.sqrt(len(sketch) / s) * sketch, math.sqrt(len(extra) / s) * extra])
-    W[0] = np.ones((1,) * (M + 1)) if M else (np.ones((1, 1)) if H is None else np.ones((1, 1, 1)))
-    srk[0] = 1
+    W[b] = np.ones((1,) * (M + 1)) if M else (np.ones((1, 1)) if H is None else np.ones((1, 1, 1)))
+    srk[b] = 1
     return sketch, W, srk
 
 

@@ -2,9 +2,9 @@
 round or sketch them, against the oracle (FP64: 1e-12 relative, products of
 two factors bit-exact).  The oracle's NTuple sketch and rounding work factor by
 factor, never forming the product, and so does the device
-(``ttr_round_rand_tuple``, ``ttr_sketch_tuple``); ``ttr_hadamard`` forms the
-product core by core, and rounding that product with the same draws must give
-the factored result.  The forward tuple sketch runs on the formed product."""
+(``ttr_round_rand_tuple``, ``ttr_sketch_tuple`` in both directions);
+``ttr_hadamard`` forms the product core by core, and rounding that product with
+the same draws must give the factored result."""
 import sys
 
 import numpy as np
@@ -107,22 +107,32 @@ def test_sketches_of_tuple_vs_oracle(T):
         assert Wc[k].shape == tuple(t.rks[k] for t in xs) + (sc[k],)
 
 
-def test_forward_sketch_of_tuple(T):
-    """tt_recursive_sketch((x, y), rks; reverse=false): the forward sketch of the
-    product in tuple order, rows reshaped to (r_xk, r_yk, s_k)."""
-    say("case: forward sketch of (x, y)")
+@pytest.mark.parametrize("orthogonal,three,p", [(True, False, None), (False, True, None), (True, True, [1, 6, 5, 3, 2])])
+def test_forward_sketch_of_tuple(T, orthogonal, three, p):
+    """tt_recursive_sketch((x, y[, z]), rks; reverse=false): factored on the device
+    (the product never formed), against the oracle's factored forward sketch and
+    the forward sketch of the explicit product in tuple order, rows reshaped to
+    (r_xk, r_yk[, r_zk], s_k)."""
+    say(f"case: forward sketch of a product of {3 if three else 2} factors, orthogonal={orthogonal}, p={p}")
     dims = (2, 3, 2, 3)
-    x = rand_tt(dims, [1, 2, 3, 2, 1], 340)
-    y = rand_tt(dims, [1, 2, 2, 3, 1], 341)
+    fs = [rand_tt(dims, [1, 2, 3, 2, 1], 340), rand_tt(dims, [1, 2, 2, 3, 1], 341)]
+    if three:
+        fs.append(rand_tt(dims, [1, 2, 3, 3, 1], 342))
     rks = [1, 4, 6, 5, 1]
-    W, s = T.tt_recursive_sketch((to_api(x), to_api(y)), rks, reverse=False, seed=12)
-    prov = device_blocks_provider(dims, rks, orthogonal=True, seed=12, forward=True)
-    Wr, sr = O.tt_recursive_sketch_forward(O.hadamard([x, y], first_fastest=True), rks, prov, True)
-    assert s == sr
+    W, s = T.tt_recursive_sketch(tuple(to_api(t) for t in fs), rks, reverse=False, seed=12, orthogonal=orthogonal,
+                                 p=p)
+    prov = device_blocks_provider(dims, rks, orthogonal=orthogonal, seed=12, forward=True, p=p)
+    Wf, sf = O.tt_recursive_sketch_tuple_forward(fs, rks, prov, orthogonal, p=p)
+    Wr, sr = O.tt_recursive_sketch_forward(O.hadamard(fs, first_fastest=True), rks, prov, orthogonal, p=p)
+    assert s == sr == sf
+    worst = 0.0
     for k in range(len(dims) + 1):
-        ref = Wr[k].reshape(x.rks[k], y.rks[k], -1, order="F")
-        assert W[k].shape == ref.shape
+        ref = Wr[k].reshape(tuple(t.rks[k] for t in fs) + (-1,), order="F")
+        assert W[k].shape == ref.shape == Wf[k].shape
         np.testing.assert_allclose(W[k], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+        worst = max(worst, np.linalg.norm(W[k] - Wf[k]) / np.linalg.norm(Wf[k]))
+    say(f"  sketch ranks {s}, worst rel err vs the factored oracle {worst:.3e}")
+    assert worst < 1e-12
 
 
 def test_hadamard_errors(T):

@@ -673,6 +673,101 @@ def test_hadamard_sketch_and_rounding_factored():
     np.testing.assert_allclose(st, se, rtol=1e-11, atol=1e-12 * np.abs(se).max())
 
 
+def _fwd_draws(dims, rks, block_rks=None, p=None, oversampling=1, g=None):
+    g = g or RNG
+    bv, pp = O.sketch_structure_forward(dims, rks, block_rks, p, oversampling)
+    return [g.standard_normal(bv[k] * dims[k] * bv[k + 1] * pp[k + 1]) for k in range(len(dims))]
+
+
+@pytest.mark.parametrize("orthogonal,oversampling", [(False, 1), (True, 1), (False, 1.5)])
+def test_forward_tuple_sketch_factored(orthogonal, oversampling):
+    """tt_recursive_sketch(NTuple; reverse=false) (sketches.jl:539-604, 1332-1478):
+    the factored forward sketch equals the forward sketch of the explicit product in
+    tuple order (first factor fastest), which is pinned by the dense forward sketch."""
+    dims = (2, 3, 2, 2)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 3, 2, 1], 81)
+    y = rand_tt(dims, [1, 2, 2, 3, 1], 82)
+    z = rand_tt(dims, [1, 1, 2, 2, 1], 83)
+    for fs in ([x, y], [x, y, z]):
+        xy = O.hadamard(fs, first_fastest=True)
+        rks = [1, 4, 6, 6, 4]
+        dr = _fwd_draws(dims, rks, None, oversampling=oversampling)
+        Wt, st = O.tt_recursive_sketch_tuple_forward(fs, rks, dr, orthogonal, oversampling=oversampling)
+        We, se = O.tt_recursive_sketch_forward(xy, rks, dr, orthogonal, oversampling=oversampling)
+        assert st == se
+        for k in range(N + 1):
+            assert Wt[k].shape == tuple(t.rks[k] for t in fs) + (se[k],)
+            ex = We[k].reshape(tuple(t.rks[k] for t in fs) + (-1,), order="F")
+            np.testing.assert_allclose(Wt[k], ex, rtol=1e-11, atol=1e-12 * np.abs(ex).max())
+
+
+def test_forward_boundary_sketches_dense():
+    """tt_sketch(...; reverse=false) (sketches.jl:679-694, 796-811, 894-900): p =
+    cld(s, bv_{N+1}) blocks at every core, W_{N+1} / sqrt(p) = vec(X)^T Omega_{1:N} /
+    sqrt(p) with the dense chain of blocks; the operator form equals the plain one
+    on the explicit H x, the tuple form equals the plain one on the product."""
+    dims = (2, 3, 2, 2)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 4, 3, 1], 84)
+    s = 7
+    bv, _ = O.sketch_structure_forward(dims, [1] * (N + 1))
+    pc = -(-s // bv[N])
+    for orthogonal in (False, True):
+        dr = [RNG.standard_normal(bv[k] * dims[k] * bv[k + 1] * pc) for k in range(N)]
+        got, sa = O.tt_sketch(x, s, dr, orthogonal, reverse=False)
+        assert sa == bv[N] * pc and sa >= s
+        pv = [1] + [pc] * N
+        Sb = [O.generate_sketch_blocks(dr[k], bv[k], dims[k], bv[k + 1], pc, orthogonal) for k in range(N)]
+        ref = (_left_part(x, N).T @ _dense_sketch_fwd(Sb, dims, pv, N))[0] / math.sqrt(pc)
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    H = O.hubbard_1d_mpo(3, 1.0, 4.0)
+    xh = rand_tt(H.dims, [1, 3, 4, 1], 85)
+    bvh, _ = O.sketch_structure_forward(H.dims, [1] * 4)
+    pch = -(-9 // bvh[3])
+    dr = [RNG.standard_normal(bvh[k] * H.dims[k] * bvh[k + 1] * pch) for k in range(3)]
+    got, _ = O.tt_sketch(xh, 9, dr, False, H=H, reverse=False)
+    ref, _ = O.tt_sketch(O.mpo_apply(H, xh), 9, dr, False, reverse=False)
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+    y = rand_tt(dims, [1, 2, 2, 2, 1], 86)
+    dr = [RNG.standard_normal(bv[k] * dims[k] * bv[k + 1] * pc) for k in range(N)]
+    got, sa = O.tt_sketch_tuple([x, y], s, dr, True, reverse=False)
+    ref, sb = O.tt_sketch(O.hadamard([x, y], first_fastest=True), s, dr, True, reverse=False)
+    assert sa == sb
+    np.testing.assert_allclose(got, ref, rtol=1e-11, atol=1e-12 * np.abs(ref).max())
+
+
+def test_forward_combined_sketch_parts():
+    """tt_combined_sketch(...; reverse=false) (sketches.jl:930-989): the right
+    boundary rank raised to max(rks_{N+1}, min(s, rks_N)); the forward recursive
+    sketch's W_{N+1} alone when it has s entries, else [sqrt(l/s) W_{N+1};
+    sqrt(le/s) tt_sketch(seed + 1)]; W_{N+1} replaced by ones."""
+    dims = (2, 3, 2, 2)
+    N = len(dims)
+    x = rand_tt(dims, [1, 2, 4, 3, 1], 87)
+    rks = [1, 3, 4, 3, 1]
+    rk1 = rks[:N] + [max(1, min(2, rks[N - 1]))]
+    dr = _fwd_draws(dims, rk1)
+    cs, W, srk = O.tt_combined_sketch(x, rks, 2, dr, None, False, reverse=False)
+    Wr, sr = O.tt_recursive_sketch_forward(x, rk1, dr, False)
+    np.testing.assert_array_equal(cs, O.jreshape(Wr[N], -1))
+    assert srk == sr[:N] + [1] and W[N].shape == (1, 1)
+    assert all(np.array_equal(a, b) for a, b in zip(W[:N], Wr[:N]))
+    s = 13
+    rk2 = rks[:N] + [max(1, min(s, rks[N - 1]))]
+    dr = _fwd_draws(dims, rk2)
+    Wr, sr = O.tt_recursive_sketch_forward(x, rk2, dr, False)
+    bvs, _ = O.sketch_structure_forward(dims, [1] * (N + 1))
+    pe = -(-(s - sr[N]) // bvs[N])
+    de = [RNG.standard_normal(bvs[k] * dims[k] * bvs[k + 1] * pe) for k in range(N)]
+    ex, le = O.tt_sketch(x, s - sr[N], de, False, reverse=False)
+    cs, W, srk = O.tt_combined_sketch(x, rks, s, dr, de, False, reverse=False)
+    tot = sr[N] + le
+    assert len(cs) == tot and tot >= s
+    np.testing.assert_allclose(cs[: sr[N]], math.sqrt(sr[N] / tot) * O.jreshape(Wr[N], -1), rtol=1e-15)
+    np.testing.assert_allclose(cs[sr[N]:], math.sqrt(le / tot) * ex, rtol=1e-15)
+
+
 # ---------------------------------------------------------------- RNG
 
 def test_philox_known_answers():
