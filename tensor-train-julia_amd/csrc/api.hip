@@ -1913,6 +1913,12 @@ int ttr_create(ttr_handle_t* out, int device) {
       TTR_REQUIRE(v >= 1e-3 && v <= 1.0, ST_INVALID_ARG, "TTR_JACOBI_TOL_SCALE must be in [1e-3, 1]");
       h->ctx.jacobi_tol_scale = v;
     }
+    // TTR_LAUNCH_LOG=path: one line per GEMM dispatch (site, math precision, tile
+    // configuration, workgroups, problems, algorithmic flops and bytes), appended to path
+    if (const char* c = std::getenv("TTR_LAUNCH_LOG")) {
+      h->ctx.launch_log = std::fopen(c, "a");
+      TTR_REQUIRE(h->ctx.launch_log != nullptr, ST_INVALID_ARG, "TTR_LAUNCH_LOG: cannot open the file");
+    }
     // TTR_GRAPHS=1: ttr_round_rand through HIP graphs (ttr_graph_enable)
     if (const char* c = std::getenv("TTR_GRAPHS")) h->graphs = c[0] == '1';
     // TTR_JACOBI_SORT=0: no column sorting between Jacobi sweeps
@@ -1942,6 +1948,7 @@ int ttr_destroy(ttr_handle_t h) {
   h->tts.clear();
   h->ops.clear();  // DevTTO frees its device copies
   (void)hipFree(h->one);
+  if (h->ctx.launch_log) std::fclose(h->ctx.launch_log);
   if (h->stage) (void)hipHostFree(h->stage);
   if (h->ctx.splitk) (void)hipFree(h->ctx.splitk);
   h->ctx.ws.release();

@@ -589,6 +589,11 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     TTR_HIP(hipGetLastError());
   };
   static const char* names[3] = {"gemm_f64", "gemm_f32", "gemm_bf16"};
+  auto log_launch = [&](int c, int64_t nt) {
+    if (ctx.launch_log)
+      std::fprintf(ctx.launch_log, "%s %s cfg=%d wgs=%lld probs=%d flops=%.0f bytes=%.0f\n", site ? site : "-",
+                   names[int(math)], c, (long long)nt, np, flops, bytes);
+  };
   const std::string site_name = site ? std::string("site:") + site : std::string();
   ProfScope ps(ctx, names[int(math)], flops, bytes, site ? site_name.c_str() : nullptr);
   if (tune) {
@@ -599,6 +604,7 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     int best_cfg = cfg;
     for (size_t ci = 0; ci < cands.size(); ++ci) {
       TTR_HIP(hipEventRecord(e0, ctx.stream));
+      log_launch(cands[ci], ntiles[ci]);
       launch(ctx, ta, tb, tc, math, cands[ci], int(ntiles[ci]), dprobs(ci), np, ds);
       TTR_HIP(hipEventRecord(e1, ctx.stream));
       TTR_HIP(hipEventSynchronize(e1));
@@ -615,6 +621,7 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     reduce(dprobs(0));
     return;
   }
+  log_launch(cfg, ntiles[0]);
   launch(ctx, ta, tb, tc, math, cfg, int(ntiles[0]), dprobs(0), np, ds);
   reduce(dprobs(0));
 }

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -171,6 +172,9 @@ struct Ctx {
   bool gemm_autotune = true;
   // GEMM tile configuration forced for every call (-1: choose per site)
   int gemm_force_cfg = -1;
+  // one line per GEMM dispatch, in submission order (TTR_LAUNCH_LOG=path; null: off), so
+  // per-dispatch profiler counters can be attributed to call sites
+  FILE* launch_log = nullptr;
   // skip the SVD of a bond whose singular values a Cholesky certificate shows
   // the floor rule keeps (see chol_certify_batched)
   bool svd_certify = true;
