@@ -21,39 +21,59 @@ struct PanelShared16 {
   double sc[3];                 // tau, beta, 1 / (alpha - beta) of the step
 };
 
-__device__ __forceinline__ double shfl_x(double v, int mask) { return __shfl_xor(v, mask); }
+// Cross-lane moves of a double on the VALU (no LDS crossbar round trip): DPP row and
+// quad permutations, and gfx950's lane-half swaps between two registers.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+constexpr int DPP_XOR1 = 0xB1;         // quad_perm [1, 0, 3, 2]
+constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2, 3, 0, 1]
+constexpr int DPP_XOR3 = 0x1B;         // quad_perm [3, 2, 1, 0]
+constexpr int DPP_HALF_MIRROR = 0x141; // lane l of each 8 <- lane 7 - l, i.e. l ^ 7
+constexpr int DPP_ROR8 = 0x128;        // row_ror:8, i.e. l ^ 8 within rows of 16
+
+// Lanes 0-31: x_l + x_{l+32}; lanes 32-63: y_{l-32} + y_l (v_permlane32_swap).
+__device__ __forceinline__ double swap32_add(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane32_swap(__double2loint(x), __double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap(__double2hiint(x), __double2hiint(y), false, false);
+  return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
+// Rows 0, 2 (of 16 lanes): x_l + x_{l+16}; rows 1, 3: y_{l-16} + y_l (v_permlane16_swap).
+__device__ __forceinline__ double swap16_add(double x, double y) {
+  const auto lo = __builtin_amdgcn_permlane16_swap(__double2loint(x), __double2loint(y), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap(__double2hiint(x), __double2hiint(y), false, false);
+  return __hiloint2double(hi[0], lo[0]) + __hiloint2double(hi[1], lo[1]);
+}
 
 // The 16 panel sums of a wave: lane l ends with the wave total of column
-// (l >> 2) & 15 (lanes 4c .. 4c+3 all hold column c).
+// (l >> 2) & 15 (lanes 4c .. 4c+3 all hold column c).  A reduce-scatter: at each
+// level a lane keeps the half of its columns its partner lane does not, and adds the
+// partner's values of that half (partners at lane distance 32, 16, 8, 4; then 2, 1
+// for the plain sum); the moves are the VALU ones above.
 __device__ __forceinline__ double butterfly16(const double (&acc)[SNB], int lane) {
   double a8[8], a4[4], a2[2];
-  const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
+  const bool h3 = lane & 8, h2 = lane & 4;
 #pragma unroll
-  for (int c = 0; c < 8; ++c) {
-    const double send = h5 ? acc[c] : acc[c + 8];
-    const double keep = h5 ? acc[c + 8] : acc[c];
-    a8[c] = keep + shfl_x(send, 32);
-  }
+  for (int c = 0; c < 8; ++c) a8[c] = swap32_add(acc[c], acc[c + 8]);
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const double send = h4 ? a8[c] : a8[c + 4];
-    const double keep = h4 ? a8[c + 4] : a8[c];
-    a4[c] = keep + shfl_x(send, 16);
-  }
+  for (int c = 0; c < 4; ++c) a4[c] = swap16_add(a8[c], a8[c + 4]);
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     const double send = h3 ? a4[c] : a4[c + 2];
     const double keep = h3 ? a4[c + 2] : a4[c];
-    a2[c] = keep + shfl_x(send, 8);
+    a2[c] = keep + dpp_d<DPP_ROR8>(send);
   }
   double a1;
   {
     const double send = h2 ? a2[0] : a2[1];
     const double keep = h2 ? a2[1] : a2[0];
-    a1 = keep + shfl_x(send, 4);
+    a1 = keep + dpp_d<DPP_XOR3>(dpp_d<DPP_HALF_MIRROR>(send));  // lane l ^ 4
   }
-  a1 += shfl_x(a1, 2);
-  a1 += shfl_x(a1, 1);
+  a1 += dpp_d<DPP_XOR2>(a1);
+  a1 += dpp_d<DPP_XOR1>(a1);
   return a1;
 }
 
