@@ -40,11 +40,12 @@ sys.path.insert(0, os.path.join(ROOT, "tensor-train-julia_amd"))
 import numpy as np  # noqa: E402
 
 PEAKS = {"gemm_f64": ("mfma", 78.6e12), "gemm_f32": ("mfma", 157.3e12), "gemm_bf16": ("mfma", 2516e12),
-         "qr_panel": ("hbm", 8.0e12), "jacobi_svd": ("mfma", 78.6e12), "op_apply": ("hbm", 8.0e12)}
+         "qr_panel": ("hbm", 8.0e12), "qr_block": ("hbm", 8.0e12), "jacobi_svd": ("mfma", 78.6e12),
+         "op_apply": ("hbm", 8.0e12)}
 # device kernel behind each profiled class (for matching rocprofv3 summaries)
 KERNEL_SYMBOL = {"gemm_f64": "gemm_scalar_kernel<double, double, double, double>",
                  "gemm_f32": "gemm_scalar_kernel<double, float, float, float>",
-                 "gemm_bf16": "gemm_bf16_kernel", "qr_panel": "qr_panel_reg_kernel",
+                 "gemm_bf16": "gemm_bf16_kernel", "qr_panel": "qr_panel_kernel", "qr_block": "qr_block_kernel",
                  "jacobi_svd": "jacobi_round_kernel", "op_apply": "op_apply_kernel"}
 
 WORKLOADS = {

@@ -6,13 +6,14 @@
 // column:
 //   beta = -sign(alpha) * ||(alpha, x)||,  tau = (beta - alpha)/beta,
 //   v = (1, x/(alpha - beta)),  tau = 0 when x == 0.
-// Columns are processed in blocks of 64.  Inside a block, panels of 16 (32 for
-// very tall matrices) are factorised by one workgroup per matrix and applied to
-// the rest of the block; the block's reflectors are then merged into one
-// 64-wide compact WY factor (T from V^T V and the panels' T's), which updates the
-// trailing matrix and, backwards, forms Q.  Every product is a grouped FP64
-// MFMA GEMM over all matrices of the batch, so the trailing matrix is streamed
-// once per 64 columns.
+// Columns are processed in blocks of 64.  Up to 3072 rows one workgroup per
+// matrix factors the whole block in one launch (panels of 16 in registers, each
+// applied on chip to the rest of the block); taller matrices take panels of 32
+// factorised by a streaming kernel and applied to the rest of the block by
+// grouped GEMMs.  The block's reflectors are then merged into one 64-wide
+// compact WY factor (T from V^T V and the panels' T's), which updates the
+// trailing matrix and, backwards, forms Q, as grouped FP64 MFMA GEMMs over all
+// matrices of the batch, so the trailing matrix is streamed once per 64 columns.
 #include "ttr_internal.h"
 #include "qr_step.h"
 
@@ -134,45 +135,202 @@ __global__ __launch_bounds__(PT) void qr_panel_kernel(const PanelJob* __restrict
 }
 
 
-// Register-resident panel factorisation: panel width NBR = 16, 512 threads,
-// thread t holds panel rows t, t+512, ... (R of them); the column steps are those
-// of qr_step.h (one workgroup reduction per column, no global traffic).
+// Register-resident block factorisation, one launch per block of NBO = 64
+// columns: one 512-thread workgroup per matrix factors the block's panels of
+// NBR = 16 columns in registers (thread t holds panel rows t, t+512, ...; the
+// column steps of qr_step.h) and applies each panel to the block's columns right
+// of it on chip, so the in-block updates (K = 16) are no grid-wide GEMMs.  Each
+// panel's explicit V (unit diagonal, zeros above) is written to the block's V
+// buffer, which the update reads back (L2) and the trailing update and the
+// formation of Q read afterwards.  The update splits the rows over the 8 waves
+// in multiples of 16 (v_mfma_f64_16x16x4):
+//   W = V^T C   (16 x n2: per-wave partial sums, added in LDS in wave order),
+//   W2 = T^T W, C -= V W2   (each wave on its own rows).
 constexpr int NBR = qrstep::SNB;
 constexpr int RT = 512;
+constexpr int BW = RT / 64;  // waves
+constexpr int NBO = 64;
+constexpr int BNS = (NBO - NBR) / NBR;  // slabs of 16 columns right of a panel (at most 3)
+constexpr int BU = 4;                   // k-steps (4 rows each) per batch of loads
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef const double __attribute__((address_space(1)))* gcd_t;
+typedef double __attribute__((address_space(1)))* gd_t;
+
+struct BlockJob {
+  double* A;
+  int64_t lda;
+  int32_t m, c0, nbo;  // block columns c0 .. c0 + nbo - 1 (nbo <= NBO)
+  double* tau;         // indexed by column
+  double* T;           // T of the panel at columns 16 q .. at T + q * NBP * NBP, ld NBP
+  double* VX;          // the block's explicit V: (m - c0) x nbo, ld m - c0
+};
 
 template <int R>
-__global__ __launch_bounds__(RT) void qr_panel_reg_kernel(const PanelJob* __restrict__ jobs) {
-  const PanelJob J = jobs[blockIdx.x];
+__global__ __launch_bounds__(RT) void qr_block_kernel(const BlockJob* __restrict__ jobs) {
+  const BlockJob J = jobs[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
   __shared__ qrstep::PanelShared16<RT> sh;
+  __shared__ double Wred[BW][BNS][NBR * NBR];
+  __shared__ double Ws[BNS][NBR * NBR];
   double* A = J.A;
   const int64_t lda = J.lda;
-  const int c0 = J.c0, nb = J.nb;
-  const int m2 = J.m - c0;
-  double P[R][NBR];
+  const int m = J.m, nbo = J.nbo;
+  const int64_t mb = m - J.c0;  // rows of the block, the ld of VX
+  for (int p = 0; NBR * p < nbo; ++p) {
+    const int pc = J.c0 + NBR * p, nb = min(NBR, nbo - NBR * p), m2 = m - pc;
+    double* A0 = A + pc + lda * pc;
+    double* V0 = J.VX + NBR * p + mb * (NBR * p);  // the panel's V: row i, column c at V0[i + mb c]
+    double P[R][NBR];
 #pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int i = tid + RT * q;
+    for (int q = 0; q < R; ++q) {
+      const int i = tid + RT * q;
 #pragma unroll
-    for (int c = 0; c < NBR; ++c)
-      P[q][c] = (i < m2 && c < nb) ? A[(c0 + i) + lda * (c0 + c)] : 0.0;
-  }
-  for (int idx = tid; idx < NBR * qrstep::LDV; idx += RT) sh.Ts[idx] = 0.0;
-  __syncthreads();
-  qrstep::PanelStep16<R, RT, 0>::run(P, sh, J.tau + c0, nb, m2, tid, lane, wave);
-#pragma unroll
-  for (int q = 0; q < R; ++q) {
-    const int i = tid + RT * q;
-    if (i < m2) {
-#pragma unroll
-      for (int c = 0; c < NBR; ++c)
-        if (c < nb) A[(c0 + i) + lda * (c0 + c)] = P[q][c];
+      for (int c = 0; c < NBR; ++c) P[q][c] = (i < m2 && c < nb) ? A0[i + lda * c] : 0.0;
     }
-  }
-  __syncthreads();
-  for (int idx = tid; idx < NBP * NBP; idx += RT) {
-    const int r = idx % NBP, c = idx / NBP;
-    J.T[r + NBP * c] = (r < nb && c < nb && r < NBR && c < NBR) ? sh.Ts[r * qrstep::LDV + c] : 0.0;
+    for (int idx = tid; idx < NBR * qrstep::LDV; idx += RT) sh.Ts[idx] = 0.0;
+    __syncthreads();
+    qrstep::PanelStep16<R, RT, 0>::run(P, sh, J.tau + pc, nb, m2, tid, lane, wave);
+    // R and V back to A; explicit V to VX (zero in the block's rows above the panel)
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = tid + RT * q;
+      if (i < m2) {
+#pragma unroll
+        for (int c = 0; c < NBR; ++c)
+          if (c < nb) {
+            A0[i + lda * c] = P[q][c];
+            V0[i + mb * c] = i > c ? P[q][c] : (i == c ? 1.0 : 0.0);
+          }
+      }
+    }
+    for (int e = tid; e < NBR * p * nb; e += RT) J.VX[e % (NBR * p) + mb * (NBR * p + e / (NBR * p))] = 0.0;
+    __syncthreads();
+    {
+      double* Tp = J.T + size_t(pc / NBR) * NBP * NBP;
+      for (int idx = tid; idx < NBP * NBP; idx += RT) {
+        const int r = idx % NBP, c = idx / NBP;
+        Tp[idx] = (r < nb && c < nb && r < NBR && c < NBR) ? sh.Ts[r * qrstep::LDV + c] : 0.0;
+      }
+    }
+    const int n2 = nbo - NBR * p - nb;  // block columns right of the panel
+    if (n2 > 0) {
+      const int ns = (n2 + NBR - 1) / NBR;
+      // global-address-space views: global (not flat) loads and stores, which LDS
+      // waits do not wait for
+      const gd_t C = (gd_t)(A0 + lda * nb);
+      const gcd_t Vg = (gcd_t)V0;
+      const int rpw = (m2 + NBR * BW - 1) / (NBR * BW) * NBR;
+      const int r0 = min(wave * rpw, m2), r1 = min(r0 + rpw, m2);
+      // W = V^T C over this wave's rows, batches of BU k-steps, the next batch's
+      // loads issued before the current batch's MFMAs
+      d4_t w[BNS];
+#pragma unroll
+      for (int s = 0; s < BNS; ++s) w[s] = d4_t{0.0, 0.0, 0.0, 0.0};
+      // rows past r1 load row r1 - 1 (always valid) and meet a zero V entry;
+      // columns past n2 load column n2 - 1 and give W columns nobody stores
+      auto load_w = [&](double (&a)[BU], double (&b)[BNS][BU], int ib) {
+#pragma unroll
+        for (int u = 0; u < BU; ++u) {
+          const int i = ib + 4 * u + lk;
+          const int ic = min(i, r1 - 1);
+          const double v = Vg[ic + mb * li];
+          a[u] = i < r1 ? v : 0.0;
+#pragma unroll
+          for (int s = 0; s < BNS; ++s) b[s][u] = C[ic + lda * min(NBR * s + li, n2 - 1)];
+        }
+      };
+      auto mma_w = [&](const double (&a)[BU], const double (&b)[BNS][BU]) {
+#pragma unroll
+        for (int u = 0; u < BU; ++u)
+#pragma unroll
+          for (int s = 0; s < BNS; ++s)
+            if (s < ns) w[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[s][u], w[s], 0, 0, 0);
+      };
+      if (r0 < r1) {
+        double a0[BU], b0[BNS][BU], a1[BU], b1[BNS][BU];
+        load_w(a0, b0, r0);
+        for (int ib = r0; ib < r1; ib += 8 * BU) {
+          load_w(a1, b1, ib + 4 * BU);
+          mma_w(a0, b0);
+          if (ib + 4 * BU >= r1) break;
+          load_w(a0, b0, ib + 8 * BU);
+          mma_w(a1, b1);
+        }
+      }
+      // register r of a 16 x 16 MFMA result holds row 4 r + lane / 16, column lane % 16
+#pragma unroll
+      for (int s = 0; s < BNS; ++s)
+        if (s < ns)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) Wred[wave][s][(4 * r + lk) * NBR + li] = w[s][r];
+      __syncthreads();
+      for (int e = tid; e < ns * NBR * NBR; e += RT) {
+        const int s = e / (NBR * NBR), o = e % (NBR * NBR);
+        double v = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < BW; ++ww) v += Wred[ww][s][o];
+        Ws[s][o] = v;
+      }
+      __syncthreads();
+      // W2 = T^T W (every wave, for its own update)
+      d4_t w2[BNS];
+#pragma unroll
+      for (int s = 0; s < BNS; ++s) {
+        w2[s] = d4_t{0.0, 0.0, 0.0, 0.0};
+        if (s < ns) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int k = 4 * r + lk;
+            w2[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(sh.Ts[k * qrstep::LDV + li], Ws[s][k * NBR + li], w2[s], 0, 0,
+                                                         0);
+          }
+        }
+      }
+      // C -= V W2 on this wave's rows, 16 at a time, the next 16 loaded before the
+      // current ones are updated (clamped rows as above: loaded, never stored)
+      auto load_u = [&](double (&a)[4], d4_t (&c)[BNS], int i0) {
+        const int ia = min(i0 + li, r1 - 1);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) a[r] = -Vg[ia + mb * (4 * r + lk)];
+#pragma unroll
+        for (int s = 0; s < BNS; ++s) {
+          const gcd_t Cs = C + lda * min(NBR * s + li, n2 - 1);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) c[s][r] = Cs[min(i0 + 4 * r + lk, r1 - 1)];
+        }
+      };
+      auto upd = [&](const double (&a)[4], d4_t (&c)[BNS], int i0) {
+#pragma unroll
+        for (int s = 0; s < BNS; ++s)
+          if (s < ns)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) c[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], w2[s][r], c[s], 0, 0, 0);
+#pragma unroll
+        for (int s = 0; s < BNS; ++s) {
+          const int j = NBR * s + li;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = i0 + 4 * r + lk;
+            if (i < r1 && j < n2) C[i + lda * j] = c[s][r];
+          }
+        }
+      };
+      if (r0 < r1) {
+        double aa[4], ab[4];
+        d4_t ca[BNS], cb[BNS];
+        load_u(aa, ca, r0);
+        for (int i0 = r0; i0 < r1; i0 += 2 * NBR) {
+          load_u(ab, cb, i0 + NBR);
+          upd(aa, ca, i0);
+          if (i0 + NBR >= r1) break;
+          load_u(aa, ca, i0 + 2 * NBR);
+          upd(ab, cb, i0 + NBR);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -181,8 +339,6 @@ __global__ __launch_bounds__(RT) void qr_panel_reg_kernel(const PanelJob* __rest
 // columns of panel q are T(0:c0, q) = -T(0:c0, 0:c0) G(0:c0, q) T_qq (G = V^T V of
 // the block), T_qq the panel's own T.  One 256-thread workgroup per matrix, two
 // small products per panel instead of one sequential step per column.
-constexpr int NBO = 64;
-
 struct TMergeJob {
   const double* G;   // 64 x 64, ld 64
   const double* Tp;  // panel q's T at Tp + q * NBP * NBP, ld NBP
@@ -225,20 +381,21 @@ __global__ __launch_bounds__(256) void merge_t_kernel(const TMergeJob* __restric
   for (int e = t; e < NBO * NBO; e += 256) J.T[e] = Ts[e % NBO][e / NBO];
 }
 
-void launch_panel(Ctx& ctx, const PanelJob* d, int njobs, int max_rows, bool reg) {
-  if (!reg) {
-    TTR_LAUNCH(ctx, qr_panel_kernel, dim3(unsigned(njobs)), dim3(PT), 0, d);
-  } else {
-    const int R = (max_rows + RT - 1) / RT;
-    switch (R) {
-      case 0:
-      case 1: TTR_LAUNCH(ctx, qr_panel_reg_kernel<1>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-      case 2: TTR_LAUNCH(ctx, qr_panel_reg_kernel<2>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-      case 3: TTR_LAUNCH(ctx, qr_panel_reg_kernel<3>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-      case 4: TTR_LAUNCH(ctx, qr_panel_reg_kernel<4>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-      case 5: TTR_LAUNCH(ctx, qr_panel_reg_kernel<5>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-      default: TTR_LAUNCH(ctx, qr_panel_reg_kernel<6>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-    }
+void launch_panel(Ctx& ctx, const PanelJob* d, int njobs) {
+  TTR_LAUNCH(ctx, qr_panel_kernel, dim3(unsigned(njobs)), dim3(PT), 0, d);
+  TTR_HIP(hipGetLastError());
+}
+
+void launch_block(Ctx& ctx, const BlockJob* d, int njobs, int max_rows) {
+  const int R = (max_rows + RT - 1) / RT;
+  switch (R) {
+    case 0:
+    case 1: TTR_LAUNCH(ctx, qr_block_kernel<1>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+    case 2: TTR_LAUNCH(ctx, qr_block_kernel<2>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+    case 3: TTR_LAUNCH(ctx, qr_block_kernel<3>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+    case 4: TTR_LAUNCH(ctx, qr_block_kernel<4>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+    case 5: TTR_LAUNCH(ctx, qr_block_kernel<5>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+    default: TTR_LAUNCH(ctx, qr_block_kernel<6>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
   }
   TTR_HIP(hipGetLastError());
 }
@@ -253,7 +410,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       mm = std::max(mm, J.m);
       bytes += size_t(std::max(J.m, 0)) * size_t(std::max(J.n, 0)) * sizeof(double);
     }
-    if (mm <= QR_SMALL_MAX_M && bytes <= ctx.qr_small_bytes) {
+    if (ctx.qr_path == 0 && mm <= QR_SMALL_MAX_M && bytes <= ctx.qr_small_bytes) {
       qr_small_batched(ctx, jobs);
       return;
     }
@@ -271,8 +428,8 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
   std::vector<Aux> aux(jobs.size());
   int max_blocks = 0, max_m = 0;
   for (const auto& J : jobs) max_m = std::max(max_m, J.m);
-  // register-resident panels (width 16) up to 6*512 rows, else the streaming panel (width 32)
-  const bool reg = max_m <= 6 * RT;
+  // register-resident blocks (panels of 16) up to 6*512 rows, else the streaming panel (width 32)
+  const bool reg = max_m <= 6 * RT && ctx.qr_path != 2;
   const int NB = reg ? NBR : NBP;
   for (size_t q = 0; q < jobs.size(); ++q) {
     const QRJob& J = jobs[q];
@@ -306,7 +463,30 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
   };
   // ---- factorisation
   for (int b = 0; b < max_blocks; ++b) {
-    for (int ip = 0; ip < NBO / NB; ++ip) {
+    if (reg) {  // the block's panels and in-block updates in one launch, V to VX
+      std::vector<BlockJob> bj;
+      double fl = 0.0, by = 0.0;
+      int max_rows = 0;
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const QRJob& J = jobs[q];
+        Aux& a = aux[q];
+        if (b >= a.nblock) continue;
+        const int c0 = b * NBO, nbo = block_cols(a, b), mb = J.m - c0;
+        bj.push_back(BlockJob{J.A, J.lda, J.m, c0, nbo, a.tau, a.T, a.VX});
+        max_rows = std::max(max_rows, mb);
+        for (int c = 0; c < nbo; c += NBR) {
+          const double m2 = J.m - (c0 + c), nb = std::min(NBR, nbo - c), n2 = nbo - c - nb;
+          fl += 4.0 * m2 * nb * nb + 4.0 * m2 * nb * n2;  // panel (2 m nb^2) + T, W = V^T C and C -= V W2
+        }
+        by += 24.0 * double(mb) * nbo;  // the block read and written, V written
+      }
+      if (!bj.empty()) {
+        const BlockJob* d = ctx.upload(bj);
+        ProfScope ps(ctx, "qr_block", fl, by);
+        launch_block(ctx, d, int(bj.size()), max_rows);
+      }
+    }
+    for (int ip = 0; !reg && ip < NBO / NB; ++ip) {
       std::vector<PanelJob> pj;
       for (size_t q = 0; q < jobs.size(); ++q) {
         const Aux& a = aux[q];
@@ -319,15 +499,13 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       if (pj.empty()) continue;
       const PanelJob* d = ctx.upload(pj);
       double pfl = 0.0, pby = 0.0;
-      int max_rows = 0;
       for (const auto& j : pj) {
         pfl += 4.0 * double(j.m - j.c0) * j.nb * j.nb;  // Householder panel (2 m nb^2) + T
         pby += 16.0 * double(j.m - j.c0) * j.nb;
-        max_rows = std::max(max_rows, j.m - j.c0);
       }
       {
         ProfScope ps(ctx, "qr_panel", pfl, pby);
-        launch_panel(ctx, d, int(pj.size()), max_rows, reg);
+        launch_panel(ctx, d, int(pj.size()));
       }
       // apply the panel to the rest of its block: A2 -= V (T^T (V^T A2)); the
       // triangular factor multiplies the short side (nb x n2), not V (m2 x nb)
@@ -355,7 +533,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       }
     }
     // merge the block's reflectors: G = V^T V, T from (G, tau)
-    copy_vx(b);
+    if (!reg) copy_vx(b);
     std::vector<TMergeJob> tb;
     for (size_t q = 0; q < jobs.size(); ++q) {
       const QRJob& J = jobs[q];

@@ -188,6 +188,9 @@ struct Ctx {
   // size is at most this many bytes run one workgroup per matrix (qr_small_batched);
   // larger batches use the blocked path whose updates spread over the chip
   size_t qr_small_bytes = size_t(96) << 20;
+  // 0: the QR path by size; 1: never the one-launch QR (blocked path); 2: the
+  // blocked path with streaming panels of 32 at every height
+  int qr_path = 0;
   // partial sums of split-K GEMMs, reused by every launch on the stream
   double* splitk = nullptr;
   size_t splitk_elems = 0;
