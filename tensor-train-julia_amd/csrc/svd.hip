@@ -32,6 +32,7 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 constexpr int RCH = 32;         // row chunk for Gram / apply
 constexpr int RING = 3;         // row chunks in flight (registers) ahead of the MFMAs
 constexpr int MAX_SWEEPS = 40;
+constexpr int JAC_MAX_SPLIT = 8;  // row ranges per visit in the split launches
 
 // Shape of a visit for column blocks of width WB_ (16 or 32)
 template <int WB_>
@@ -70,6 +71,7 @@ struct JacJob {
   int32_t* info;
   double tol;  // rotate a pair when |G_ij| > tol sqrt(G_ii G_jj)
   int p, nblk, pp, rounds, slot0;
+  int nsplit;   // row ranges of a visit's Gram and apply in the split launches
 };
 
 __host__ __device__ __forceinline__ int rr_player(int pos, int r, int pp) {
@@ -283,26 +285,33 @@ __device__ __forceinline__ void inner_sweep(double (*GA)[Jac<WB_>::GLD], double 
   __syncthreads();
 }
 
-template <int WB_>
-__global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_kernel(const JacJob* __restrict__ jobs,
-                                                                                  int njobs, int round, int sweep,
-                                                                                  int32_t* __restrict__ flags) {
+// MODE 0: a whole visit in one workgroup.  The split launches spread a visit over
+// several workgroups (a matrix's visits are few when the batch holds few
+// matrices): MODE 1 forms the Gram of one row range of the pair's columns into
+// a partial buffer (blockIdx.y = the range), MODE 2 adds the partials in range
+// order, runs the inner sweep and stores V and whether it rotated, MODE 3 applies
+// V to one row range of B and J.  With one range MODE 1-3 compute what MODE 0 does,
+// bit for bit.
+template <int WB_, int MODE>
+__global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_kernel(
+    const JacJob* __restrict__ jobs, int njobs, int round, int sweep, int32_t* __restrict__ flags,
+    double* __restrict__ gpart, int gstride, double* __restrict__ vbuf, int32_t* __restrict__ rotf) {
   using C = Jac<WB_>;
   constexpr int WB = C::WB, WC = C::WC, GLD = C::GLD, JT = C::JT, NW = C::NW, NT = C::NT;
-  const int bid = blockIdx.x;
+  const int bid = blockIdx.x, split = blockIdx.y;
   int lo = 0, hi = njobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (jobs[mid].slot0 <= bid) lo = mid; else hi = mid - 1;
   }
   const JacJob S = jobs[lo];
-  if (round >= S.rounds) return;
+  if (round >= S.rounds || split >= S.nsplit) return;
   if (sweep > 0 && flags[((sweep - 1) & 1) * njobs + lo] == 0) return;  // converged
   const int k = bid - S.slot0;
   const int p = S.p;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  if (k == 0 && round == 0 && tid == 0 && S.info) S.info[0] = sweep + 1;
+  if (MODE != 1 && MODE != 3 && k == 0 && round == 0 && tid == 0 && S.info) S.info[0] = sweep + 1;
   uint64_t t0 = 0, t1 = 0, t2 = 0;
   if (tid == 0) t0 = wall_clock64();
 
@@ -351,7 +360,29 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
       buf[e % RCH][e / RCH] = src[q];
     }
   };
-  {
+  if (MODE == 2) {
+    // the partial Grams of the row ranges, added in range order (upper-triangle tiles)
+    // every load issued before the first add (the sums are latency-bound)
+    const double* gp = gpart + size_t(bid) * gstride * WC * WC;
+    constexpr int EPT = WC * WC / JT;
+    double v[EPT][JAC_MAX_SPLIT];
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = tid + JT * u, i = e / WC, j = e % WC;
+#pragma unroll
+      for (int q = 0; q < JAC_MAX_SPLIT; ++q)
+        v[u][q] = (q < S.nsplit && i / 16 <= j / 16) ? gp[size_t(q) * WC * WC + e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = tid + JT * u, i = e / WC, j = e % WC;
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < JAC_MAX_SPLIT; ++q)
+        if (q < S.nsplit) acc += v[u][q];
+      if (i / 16 <= j / 16) G[i][j] = acc;
+    }
+  } else if (MODE != 3) {
     // the Gram's chunk is staged column-major ([column][row], pitch 2 mod 32
     // doubles): the MFMA operands are then 16 columns x 4 rows per read, which
     // fall in distinct banks; the staging stores are contiguous
@@ -371,22 +402,25 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
     d4v gacc[C::TPW];
 #pragma unroll
     for (int t = 0; t < C::TPW; ++t) gacc[t] = d4v{0, 0, 0, 0};
+    // chunks [gcb, gcb + gn) of the rows (all of them in MODE 0)
+    const int gcb = MODE == 1 ? split * nch / S.nsplit : 0;
+    const int gn = (MODE == 1 ? (split + 1) * nch / S.nsplit : nch) - gcb;
     double pre[RING][PER];
 #pragma unroll
     for (int u = 0; u < RING; ++u)
-      if (u < nch) load_chunk(S.B, S.ldb, u * RCH, pre[u]);
-    if (nch > 0) gstage(gbuf0, pre[0]);
-    if (RING < nch) load_chunk(S.B, S.ldb, RING * RCH, pre[0]);
+      if (u < gn) load_chunk(S.B, S.ldb, (gcb + u) * RCH, pre[u]);
+    if (gn > 0) gstage(gbuf0, pre[0]);
+    if (RING < gn) load_chunk(S.B, S.ldb, (gcb + RING) * RCH, pre[0]);
     __syncthreads();
-    for (int c0 = 0; c0 < nch; c0 += RING) {
+    for (int c0 = 0; c0 < gn; c0 += RING) {
 #pragma unroll
       for (int u = 0; u < RING; ++u) {
         const int c = c0 + u;
-        if (c < nch) {  // uniform over the workgroup
+        if (c < gn) {  // uniform over the workgroup
           const int un = (u + 1) % RING;
-          if (c + 1 < nch) {
+          if (c + 1 < gn) {
             gstage((c & 1) ? gbuf0 : gbuf1, pre[un]);
-            if (c + 1 + RING < nch) load_chunk(S.B, S.ldb, (c + 1 + RING) * RCH, pre[un]);
+            if (c + 1 + RING < gn) load_chunk(S.B, S.ldb, (gcb + c + 1 + RING) * RCH, pre[un]);
           }
           const double (*gch)[CP] = (c & 1) ? gbuf1 : gbuf0;
 #pragma unroll
@@ -406,18 +440,29 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
         }
       }
     }
+    double* gp = gpart + (size_t(bid) * gstride + split) * WC * WC;
 #pragma unroll
     for (int t = 0; t < C::TPW; ++t) {
       const int k = wave + NW * t;
       if (k < C::NGT) {
         const int bi = int(C::GTILES >> (4 * k)) & 3, bj = int(C::GTILES >> (4 * k + 2)) & 3;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) G[16 * bi + 4 * q + (lane >> 4)][16 * bj + (lane & 15)] = gacc[t][q];
+        for (int q = 0; q < 4; ++q) {
+          const int i = 16 * bi + 4 * q + (lane >> 4), j = 16 * bj + (lane & 15);
+          if (MODE == 1) gp[i * WC + j] = gacc[t][q];
+          else G[i][j] = gacc[t][q];
+        }
       }
     }
   }
+  if (MODE == 1) {
+    if (tid == 0 && S.info) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 1, unsigned(wall_clock64() - t0));
+    return;
+  }
   __syncthreads();
   if (tid == 0) t1 = wall_clock64();
+  double (*V)[GLD] = GA;
+  if (MODE != 3) {
   // scale G by a power of two (rotation angles are scale free) so that the
   // squares below cannot overflow for unnormalised inputs
   {
@@ -444,53 +489,71 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
   __shared__ int vrot;
   if (tid == 0) vrot = 0;
   if (!__syncthreads_or(need)) {
-    if (tid == 0 && S.info) {
-      atomicAdd(reinterpret_cast<unsigned*>(S.info) + 1, unsigned(t1 - t0));
-      atomicAdd(reinterpret_cast<unsigned*>(S.info) + 4, 1u);
+    if (tid == 0) {
+      if (MODE == 2) rotf[bid] = 0;
+      if (S.info) {
+        if (MODE == 0) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 1, unsigned(t1 - t0));
+        atomicAdd(reinterpret_cast<unsigned*>(S.info) + 4, 1u);
+      }
     }
     return;
   }
 
   inner_sweep<WB_>(GA, GB, tol2, tid, &vrot);
-  double (*V)[GLD] = GA;
   chunk = GB;
   if (tid == 0) {
     t2 = wall_clock64();
     if (S.info) {
-      atomicAdd(reinterpret_cast<unsigned*>(S.info) + 1, unsigned(t1 - t0));
+      if (MODE == 0) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 1, unsigned(t1 - t0));
       atomicAdd(reinterpret_cast<unsigned*>(S.info) + 2, unsigned(t2 - t1));
       atomicAdd(reinterpret_cast<unsigned*>(S.info) + 4, 1u);
     }
+    if (MODE == 2) rotf[bid] = vrot;
   }
   if (!vrot) return;
   if (tid == 0) {
     flags[(sweep & 1) * njobs + lo] = 1;
     if (S.info) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 5, 1u);
   }
+  if (MODE == 2) {  // V to the buffer the apply launch reads
+    double* vb = vbuf + size_t(bid) * WC * WC;
+    for (int e = tid; e < WC * WC; e += JT) vb[e] = V[e / WC][e % WC];
+    return;
+  }
+  } else {  // MODE 3
+    if (rotf[bid] == 0) return;
+    if (tid == 0) t2 = wall_clock64();
+  }
 
   // ---- apply V to the visited columns of B and J: C <- C V (wave w: row tile w / NT,
   // column tile w % NT); the chunks of B and then of J form one sequence, so the
   // loads of J's first chunks overlap B's last MFMAs
   {
-    const int tot = 2 * nch;
-    auto mat = [&](int c) { return c < nch ? S.B : S.J; };
-    auto mld = [&](int c) { return c < nch ? S.ldb : S.ldj; };
+    // chunks [acb, acb + tot) of the sequence (all 2 nch of them in MODE 0)
+    const int acb = MODE == 3 ? split * (2 * nch) / S.nsplit : 0;
+    const int tot = (MODE == 3 ? (split + 1) * (2 * nch) / S.nsplit : 2 * nch) - acb;
+    auto mat = [&](int c) { return acb + c < nch ? S.B : S.J; };
+    auto mld = [&](int c) { return acb + c < nch ? S.ldb : S.ldj; };
+    auto row0 = [&](int c) { return ((acb + c) % nch) * RCH; };
     double pre[RING][PER];
 #pragma unroll
     for (int u = 0; u < RING; ++u)
-      if (u < tot) load_chunk(mat(u), mld(u), (u % nch) * RCH, pre[u]);
+      if (u < tot) load_chunk(mat(u), mld(u), row0(u), pre[u]);
     // wave w: row tile bi = w / NT, column tile bj = w % NT of each chunk
     const int bi = wave / NT, bj = wave % NT;
     // this wave's MFMA B operands (column tile bj of V) for every k, read once;
     // V's buffer (GA) then becomes the second chunk buffer beside GB
     double vb[WC / 4];
+    const double* vg = vbuf + size_t(bid) * WC * WC;
 #pragma unroll
-    for (int kk = 0; kk < WC; kk += 4) vb[kk / 4] = V[kk + (lane >> 4)][16 * bj + (lane & 15)];
+    for (int kk = 0; kk < WC; kk += 4)
+      vb[kk / 4] = MODE == 3 ? vg[(kk + (lane >> 4)) * WC + 16 * bj + (lane & 15)]
+                             : V[kk + (lane >> 4)][16 * bj + (lane & 15)];
     __syncthreads();
     double(*abuf0)[GLD] = GB;
     double(*abuf1)[GLD] = GA;
     if (tot > 0) stage_chunk(abuf0, pre[0]);
-    if (RING < tot) load_chunk(mat(RING), mld(RING), (RING % nch) * RCH, pre[0]);
+    if (RING < tot) load_chunk(mat(RING), mld(RING), row0(RING), pre[0]);
     __syncthreads();
     const int gc = cols[16 * bj + (lane & 15)];
     for (int c0 = 0; c0 < tot; c0 += RING) {
@@ -502,12 +565,12 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
           if (c + 1 < tot) {
             stage_chunk((c & 1) ? abuf0 : abuf1, pre[un]);
             const int cn = c + 1 + RING;
-            if (cn < tot) load_chunk(mat(cn), mld(cn), (cn % nch) * RCH, pre[un]);
+            if (cn < tot) load_chunk(mat(cn), mld(cn), row0(cn), pre[un]);
           }
           const double(*ch)[GLD] = (c & 1) ? abuf1 : abuf0;
           double* M = mat(c);
           const int64_t ld = mld(c);
-          const int r0 = (c % nch) * RCH;
+          const int r0 = row0(c);
           d4v acc = {0, 0, 0, 0};
 #pragma unroll
           for (int kk = 0; kk < WC; kk += 4) {
@@ -580,7 +643,7 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   const int nj = int(live.size());
   std::vector<JacJob> jj(nj);
   std::vector<Fill2D> fills;
-  int slots = 0, max_rounds = 1;
+  int slots = 0, max_rounds = 1, max_split = 1;
   for (int q = 0; q < nj; ++q) {
     const SVDJob& s = live[q];
     JacJob& J = jj[q];
@@ -589,6 +652,11 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
     J.nblk = (s.p + WB - 1) / WB;
     J.pp = J.nblk + (J.nblk & 1);
     J.rounds = J.nblk == 1 ? 1 : J.pp - 1;
+    // row ranges of the split launches: about two 32-row chunks of Gram rows per
+    // workgroup, by the matrix's size alone (a matrix's result does not depend on
+    // the batch it comes in)
+    J.nsplit = std::max(1, std::min(JAC_MAX_SPLIT, ((s.p + RCH - 1) / RCH + 1) / 2));
+    max_split = std::max(max_split, J.nsplit);
     J.slot0 = slots;
     slots += J.nblk == 1 ? 1 : J.pp / 2;
     max_rounds = std::max(max_rounds, J.rounds);
@@ -596,6 +664,13 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   }
   const JacJob* dj = ctx.upload(jj);
   const SVDJob* dl = ctx.upload(live);
+  // split launches when some matrix has more than one row range: partial Grams,
+  // V and the rotate flag of every visit slot
+  const bool split = max_split > 1;
+  constexpr int WC = Jac<WB>::WC;
+  double* gpart = split ? ctx.ws.alloc_n<double>(size_t(slots) * max_split * WC * WC) : nullptr;
+  double* vbuf = split ? ctx.ws.alloc_n<double>(size_t(slots) * WC * WC) : nullptr;
+  int32_t* rotf = split ? ctx.ws.alloc_n<int32_t>(size_t(slots)) : nullptr;
   int32_t* flags = ctx.ws.alloc_n<int32_t>(size_t(2) * nj);
   int32_t* hflags = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * nj));
   fill2d_batched(ctx, fills);
@@ -662,7 +737,17 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
       // timed per launch; the work depends on the sweeps taken, so the caller
       // adds it from the counters
       ProfScope ps(ctx, "jacobi_svd");
-      TTR_LAUNCH(ctx, jacobi_round_kernel<WB>, dim3(unsigned(slots)), dim3(Jac<WB>::JT), 0, dj, nj, r, sweep, flags);
+      if (!split) {
+        TTR_LAUNCH(ctx, (jacobi_round_kernel<WB, 0>), dim3(unsigned(slots)), dim3(Jac<WB>::JT), 0, dj, nj, r, sweep,
+                   flags, gpart, max_split, vbuf, rotf);
+      } else {
+        TTR_LAUNCH(ctx, (jacobi_round_kernel<WB, 1>), dim3(unsigned(slots), unsigned(max_split)), dim3(Jac<WB>::JT),
+                   0, dj, nj, r, sweep, flags, gpart, max_split, vbuf, rotf);
+        TTR_LAUNCH(ctx, (jacobi_round_kernel<WB, 2>), dim3(unsigned(slots)), dim3(Jac<WB>::JT), 0, dj, nj, r, sweep,
+                   flags, gpart, max_split, vbuf, rotf);
+        TTR_LAUNCH(ctx, (jacobi_round_kernel<WB, 3>), dim3(unsigned(slots), unsigned(max_split)), dim3(Jac<WB>::JT),
+                   0, dj, nj, r, sweep, flags, gpart, max_split, vbuf, rotf);
+      }
       TTR_HIP(hipGetLastError());
     }
     // convergence check from the third sweep on (no matrix converges earlier in
