@@ -131,12 +131,11 @@ template <class T>
 __global__ __launch_bounds__(256) void pow2_kernel(const Pow2Job* __restrict__ jobs) {
   const Pow2Job J = jobs[blockIdx.x];
   T* X = static_cast<T*>(J.X);
-  const int64_t tot = int64_t(J.M) * J.N;
+  // one column per wave at a time (no 64-bit division per element)
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double m = 0.0;
-  for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int64_t i = e % J.M, c = e / J.M;
-    m = fmax(m, fabs(ld_as_d<T>(X, i + c * J.ld)));
-  }
+  for (int c = wave; c < J.N; c += 4)
+    for (int i = lane; i < J.M; i += 64) m = fmax(m, fabs(ld_as_d<T>(X, i + int64_t(c) * J.ld)));
   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
   __shared__ double red[4];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
@@ -145,10 +144,11 @@ __global__ __launch_bounds__(256) void pow2_kernel(const Pow2Job* __restrict__ j
   int e2 = 0;
   if (m > 0.0 && isfinite(m)) e2 = ilogb(m);
   const double s = ldexp(1.0, -e2);
-  for (int64_t e = threadIdx.x; e < tot; e += blockDim.x) {
-    const int64_t i = e % J.M, c = e / J.M;
-    st_from_d<T>(X, i + c * J.ld, ld_as_d<T>(X, i + c * J.ld) * s);
-  }
+  for (int c = wave; c < J.N; c += 4)
+    for (int i = lane; i < J.M; i += 64) {
+      const int64_t o = i + int64_t(c) * J.ld;
+      st_from_d<T>(X, o, ld_as_d<T>(X, o) * s);
+    }
   if (threadIdx.x == 0) J.exp_out[0] = (J.exp_in ? J.exp_in[0] : 0) + e2;
 }
 

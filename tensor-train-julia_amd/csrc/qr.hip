@@ -422,7 +422,10 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     double* G;   // NBO x NBO
     double* W;   // NBO x n
     double* W2;  // NBO x n: T W (or T^T W)
-    double* VX;  // m x NBO: the block's V with explicit unit diagonal and zeros above
+    double* VX;  // the blocks' V with explicit unit diagonal and zeros above, block b
+                 // ((m - 64 b) x 64, ld m - 64 b) at vx(b); one buffer for all blocks
+                 // when no Q is formed, else every block's kept for the formation of Q
+    bool vx_all;
     int kq, nblock;
   };
   std::vector<Aux> aux(jobs.size());
@@ -445,10 +448,16 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     a.G = ctx.ws.alloc_n<double>(size_t(NBO) * NBO);
     a.W = ctx.ws.alloc_n<double>(size_t(NBO) * std::max(1, J.n));
     a.W2 = ctx.ws.alloc_n<double>(size_t(NBO) * std::max(1, J.n));
-    a.VX = ctx.ws.alloc_n<double>(size_t(std::max(1, J.m)) * NBO);
+    a.vx_all = J.Q != nullptr;
+    a.VX = ctx.ws.alloc_n<double>(
+        a.vx_all ? size_t(NBO) * (size_t(a.nblock) * J.m - size_t(NBO / 2) * a.nblock * std::max(0, a.nblock - 1))
+                 : size_t(std::max(1, J.m)) * NBO);
   }
   GemmBatch gb;
   auto block_cols = [&](const Aux& a, int b) { return std::min(NBO, a.kq - b * NBO); };
+  auto vx = [&](const QRJob& J, const Aux& a, int b) {
+    return a.VX + (a.vx_all ? int64_t(NBO) * (int64_t(b) * J.m - int64_t(NBO / 2) * b * (b - 1)) : 0);
+  };
   // explicit V of block b (rows b*NBO.., ld m2) for every job that has it
   auto copy_vx = [&](int b) {
     std::vector<Copy2D> cp;
@@ -457,7 +466,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       const Aux& a = aux[q];
       if (b >= a.nblock) continue;
       const int c0 = b * NBO, m2 = J.m - c0;
-      cp.push_back(copy_job(J.A + c0 + J.lda * c0, 1, J.lda, a.VX, 1, m2, m2, block_cols(a, b), 1.0, 3));
+      cp.push_back(copy_job(J.A + c0 + J.lda * c0, 1, J.lda, vx(J, a, b), 1, m2, m2, block_cols(a, b), 1.0, 3));
     }
     copy2d_batched(ctx, cp);
   };
@@ -472,7 +481,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
         Aux& a = aux[q];
         if (b >= a.nblock) continue;
         const int c0 = b * NBO, nbo = block_cols(a, b), mb = J.m - c0;
-        bj.push_back(BlockJob{J.A, J.lda, J.m, c0, nbo, a.tau, a.T, a.VX});
+        bj.push_back(BlockJob{J.A, J.lda, J.m, c0, nbo, a.tau, a.T, vx(J, a, b)});
         max_rows = std::max(max_rows, mb);
         for (int c = 0; c < nbo; c += NBR) {
           const double m2 = J.m - (c0 + c), nb = std::min(NBR, nbo - c), n2 = nbo - c - nb;
@@ -540,7 +549,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       const Aux& a = aux[q];
       if (b >= a.nblock) continue;
       const int c0 = b * NBO, m2 = J.m - c0, nbo = block_cols(a, b);
-      gb.add(nbo, nbo, m2, 1.0, a.VX, m2, 1, a.VX, 1, m2, 0.0, a.G, 1, NBO);
+      gb.add(nbo, nbo, m2, 1.0, vx(J, a, b), m2, 1, vx(J, a, b), 1, m2, 0.0, a.G, 1, NBO);
       tb.push_back(TMergeJob{a.G, a.T + size_t(c0 / NB) * NBP * NBP, a.TO + size_t(b) * NBO * NBO, nbo, NB});
     }
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
@@ -560,11 +569,11 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
         double* A2 = J.A + c0 + J.lda * (c0 + nbo);
         const double* T = a.TO + size_t(b) * NBO * NBO;
         if (stage == 0)  // W = V^T A2  (nbo x n2)
-          gb.add(nbo, n2, m2, 1.0, a.VX, m2, 1, A2, 1, J.lda, 0.0, a.W, 1, NBO);
+          gb.add(nbo, n2, m2, 1.0, vx(J, a, b), m2, 1, A2, 1, J.lda, 0.0, a.W, 1, NBO);
         else if (stage == 1)  // W2 = T^T W  (nbo x n2)
           gb.add(nbo, n2, nbo, 1.0, T, NBO, 1, a.W, 1, NBO, 0.0, a.W2, 1, NBO);
         else  // A2 -= V W2
-          gb.add(m2, n2, nbo, -1.0, a.VX, 1, m2, a.W2, 1, NBO, 1.0, A2, 1, J.lda);
+          gb.add(m2, n2, nbo, -1.0, vx(J, a, b), 1, m2, a.W2, 1, NBO, 1.0, A2, 1, J.lda);
       }
       gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_trail");
     }
@@ -575,8 +584,8 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     if (jobs[q].Q && aux[q].kq > 0) fills.push_back(Fill2D{jobs[q].Q, jobs[q].ldq, jobs[q].m, aux[q].kq, 1.0});
   if (fills.empty()) return;
   fill2d_batched(ctx, fills);
+  // (every job here forms Q, so its blocks' V are all kept from the factorisation)
   for (int b = max_blocks - 1; b >= 0; --b) {
-    copy_vx(b);
     for (int stage = 0; stage < 3; ++stage) {
       for (size_t q = 0; q < jobs.size(); ++q) {
         const QRJob& J = jobs[q];
@@ -587,11 +596,11 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
         double* Q2 = J.Q + c0 + J.ldq * c0;
         const double* T = a.TO + size_t(b) * NBO * NBO;
         if (stage == 0)  // W = V^T Q2  (nbo x n2)
-          gb.add(nbo, n2, m2, 1.0, a.VX, m2, 1, Q2, 1, J.ldq, 0.0, a.W, 1, NBO);
+          gb.add(nbo, n2, m2, 1.0, vx(J, a, b), m2, 1, Q2, 1, J.ldq, 0.0, a.W, 1, NBO);
         else if (stage == 1)  // W2 = T W  (nbo x n2)
           gb.add(nbo, n2, nbo, 1.0, T, 1, NBO, a.W, 1, NBO, 0.0, a.W2, 1, NBO);
         else  // Q2 -= V W2
-          gb.add(m2, n2, nbo, -1.0, a.VX, 1, m2, a.W2, 1, NBO, 1.0, Q2, 1, J.ldq);
+          gb.add(m2, n2, nbo, -1.0, vx(J, a, b), 1, m2, a.W2, 1, NBO, 1.0, Q2, 1, J.ldq);
       }
       gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_formq");
     }

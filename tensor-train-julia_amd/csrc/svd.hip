@@ -590,18 +590,29 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
   if (tid == 0 && S.info) atomicAdd(reinterpret_cast<unsigned*>(S.info) + 3, unsigned(wall_clock64() - t2));
 }
 
-// singular values: column norms of B (one workgroup per matrix, one wave per column)
+// singular values: column norms of B, CN_COLS columns per workgroup (blockIdx.y),
+// four per wave at once: lane-strided partial sums, then a butterfly
+constexpr int CN_COLS = 16;
 __global__ __launch_bounds__(256) void column_norms_kernel(const SVDJob* __restrict__ jobs) {
   const SVDJob S = jobs[blockIdx.x];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int c = wave; c < S.p; c += 4) {
-    const double* x = S.B + int64_t(c) * S.ldb;
-    double s = 0.0;
-    for (int i = lane; i < S.p; i += 64) s += x[i] * x[i];
+  const int c0 = blockIdx.y * CN_COLS + 4 * wave;
+  if (c0 >= S.p) return;
+  const double* x[4];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-    if (lane == 0) S.sv[c] = sqrt(s);
-  }
+  for (int q = 0; q < 4; ++q) x[q] = S.B + int64_t(min(c0 + q, S.p - 1)) * S.ldb;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = lane; i < S.p; i += 64)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] += x[q][i] * x[q][i];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s[q] += __shfl_xor(s[q], off);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if (c0 + q < S.p) S.sv[c0 + q] = sqrt(s[q]);
 }
 
 // pseudo-inverse weights: one workgroup per matrix, the maximum singular value by
@@ -681,7 +692,12 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   // columns of similar size.  On graded matrices (terms whose norms differ by
   // many orders of magnitude) the unsorted iteration needed more than 40 sweeps.
   std::vector<int64_t> off(size_t(nj) + 1, 0);
-  for (int q = 0; q < nj; ++q) off[q + 1] = off[q] + live[q].p;
+  int max_p = 1;
+  for (int q = 0; q < nj; ++q) {
+    off[q + 1] = off[q] + live[q].p;
+    max_p = std::max(max_p, live[q].p);
+  }
+  const int cn_y = (max_p + CN_COLS - 1) / CN_COLS;  // column-norm workgroups per matrix
   double* nrm = nullptr;
   int32_t* perm = nullptr;
   double* hn = nullptr;
@@ -702,7 +718,7 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
     dn = ctx.upload(nj_jobs);
   }
   auto sort_columns = [&]() {
-    TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, dn);
+    TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj), unsigned(cn_y)), dim3(256), 0, dn);
     TTR_HIP(hipGetLastError());
     TTR_HIP(hipMemcpyAsync(hn, nrm, sizeof(double) * size_t(off[nj]), hipMemcpyDeviceToHost, ctx.stream));
     ctx.sync();
@@ -780,7 +796,7 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
       }
     throw Error(ST_NUMERICAL_FAILURE, msg + ")");
   }
-  TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj)), dim3(256), 0, dl);
+  TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj), unsigned(cn_y)), dim3(256), 0, dl);
   TTR_HIP(hipGetLastError());
 }
 
