@@ -256,6 +256,8 @@ def deterministic(items, op, wl, args):
     for _, terms, _ in items:
         for t in terms:
             t.free()
+    # the batch's workspace arena and freed storage go back to the runtime first
+    T.default_handle().trim()
     r = T.round_svd_batch(mats, **wl["cleanup"])
     for o in r:
         o.free()

@@ -361,6 +361,14 @@ int ttr_graph_enable(ttr_handle_t h, int on);
  * because their launches differed from the capture; graphs now cached. */
 int ttr_graph_stats(ttr_handle_t h, int64_t* captures, int64_t* replays, int64_t* fallbacks, int64_t* cached);
 
+/* Returns the handle's idle device memory to the runtime: the workspace arena
+ * (kept at its largest size so repeated calls allocate nothing) and the freed
+ * TT storage held for reuse.  Live TTs, operators and cached graphs are kept.
+ * No counterpart in the reference (Julia's GC owns its buffers); for callers
+ * that move from one workload to another of a different size, such as a large
+ * batch followed by materialised products. */
+int ttr_trim(ttr_handle_t h);
+
 #ifdef __cplusplus
 }
 #endif

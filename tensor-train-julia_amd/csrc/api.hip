@@ -3449,6 +3449,14 @@ int ttr_graph_stats(ttr_handle_t h, int64_t* captures, int64_t* replays, int64_t
   });
 }
 
+int ttr_trim(ttr_handle_t h) {
+  return guarded(h, [&] {
+    TTR_HIP(hipStreamSynchronize(h->ctx.stream));
+    h->ctx.ws.release();
+    h->ctx.cache.release_cached();
+  });
+}
+
 int ttr_profile_enable(ttr_handle_t h, int on) {
   return guarded(h, [&] { h->ctx.prof.on = on != 0; });
 }

@@ -19,13 +19,16 @@ void* DevArena::alloc(size_t bytes) {
                     " MiB of device scratch of the non-allocating entry points");
     void* p = fixed_ + fixed_used_;
     fixed_used_ += bytes;
+    fixed_peak_ = std::max(fixed_peak_, fixed_used_);
     return p;
   }
-  if (!blocks_.empty()) {
-    Block& b = blocks_.back();
+  // the current block, else a later one (blocks kept after a rewind), else a new one
+  for (; cur_ < blocks_.size(); ++cur_) {
+    Block& b = blocks_[cur_];
     if (b.used + bytes <= b.cap) {
       void* p = b.p + b.used;
       b.used += bytes;
+      peak_ = std::max(peak_, held());
       return p;
     }
   }
@@ -34,12 +37,39 @@ void* DevArena::alloc(size_t bytes) {
   char* p = nullptr;
   TTR_HIP(hipMalloc(&p, cap));
   blocks_.push_back({p, cap, bytes});
+  cur_ = blocks_.size() - 1;
+  peak_ = std::max(peak_, held());
   return p;
 }
 
+DevArena::Mark DevArena::mark() const {
+  if (fixed_) return {0, fixed_used_};
+  if (cur_ >= blocks_.size()) return {blocks_.size(), 0};
+  return {cur_, blocks_[cur_].used};
+}
+
+void DevArena::rewind(const Mark& m) {
+  if (fixed_) {
+    fixed_used_ = std::min(fixed_used_, m.off);
+    return;
+  }
+  for (size_t i = m.blk + 1; i < blocks_.size(); ++i) blocks_[i].used = 0;
+  if (m.blk < blocks_.size()) blocks_[m.blk].used = m.off;
+  cur_ = std::min(m.blk, blocks_.empty() ? size_t(0) : blocks_.size() - 1);
+}
+
+size_t DevArena::held() const {
+  size_t u = 0;
+  for (auto& b : blocks_) u += b.used;
+  return u;
+}
+
 void DevArena::reset() {
+  cur_ = 0;
+  peak_ = 0;
   if (fixed_) {
     fixed_used_ = 0;
+    fixed_peak_ = 0;
     return;
   }
   if (blocks_.size() > 1) {
@@ -54,16 +84,12 @@ void DevArena::reset() {
   }
 }
 
-size_t DevArena::used() const {
-  if (fixed_) return fixed_used_;
-  size_t u = 0;
-  for (auto& b : blocks_) u += b.used;
-  return u;
-}
+size_t DevArena::used() const { return fixed_ ? fixed_peak_ : peak_; }
 
 void DevArena::release() {
   for (auto& b : blocks_) (void)hipFree(b.p);
   blocks_.clear();
+  cur_ = 0;
 }
 
 PinnedArena::~PinnedArena() { release(); }

@@ -403,6 +403,7 @@ void launch_block(Ctx& ctx, const BlockJob* d, int njobs, int max_rows) {
 }  // namespace
 
 void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
+  WsScope scope(ctx);  // tau, T factors and the blocks' V are needed only here
   {
     int mm = 0;
     size_t bytes = 0;

@@ -647,6 +647,7 @@ void pinv_weights_batched(Ctx& ctx, const std::vector<PinvWeights>& jobs) {
 
 template <int WB>
 void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
+  WsScope scope(ctx);  // descriptors, flags, partial Grams and sort buffers are needed only here
   std::vector<SVDJob> live;
   for (auto& j : jobs)
     if (j.p > 0) live.push_back(j);

@@ -74,18 +74,29 @@ class DevArena {
   T* alloc_n(size_t n) { return static_cast<T*>(alloc(n * sizeof(T))); }
   void reset();  // caller must have synchronised the stream
   void release();
-  void set_fixed(void* base, size_t cap) { fixed_ = static_cast<char*>(base); fixed_cap_ = cap; fixed_used_ = 0; }
-  void clear_fixed() { fixed_ = nullptr; fixed_cap_ = fixed_used_ = 0; }
+  void set_fixed(void* base, size_t cap) {
+    fixed_ = static_cast<char*>(base);
+    fixed_cap_ = cap;
+    fixed_used_ = fixed_peak_ = 0;
+  }
+  void clear_fixed() { fixed_ = nullptr; fixed_cap_ = fixed_used_ = fixed_peak_ = 0; }
   bool fixed() const { return fixed_ != nullptr; }
   size_t fixed_free() const { return fixed_cap_ - fixed_used_; }
-  // bytes handed out since the last reset (the size a fixed region needs for the same call)
+  // most bytes held at once since the last reset (the size a fixed region needs for the same call)
   size_t used() const;
+  // A position in the arena; rewind(m) hands back everything allocated after it
+  // (the blocks stay, later allocations of the call reuse them).
+  struct Mark { size_t blk, off; };
+  Mark mark() const;
+  void rewind(const Mark& m);
 
  private:
   struct Block { char* p; size_t cap; size_t used; };
+  size_t held() const;
   std::vector<Block> blocks_;
+  size_t cur_ = 0, peak_ = 0;
   char* fixed_ = nullptr;
-  size_t fixed_cap_ = 0, fixed_used_ = 0;
+  size_t fixed_cap_ = 0, fixed_used_ = 0, fixed_peak_ = 0;
 };
 
 // Same scheme over pinned host memory (staging for descriptor arrays).
@@ -223,6 +234,30 @@ struct Ctx {
   T* upload(const std::vector<T>& v) { return upload(v.data(), v.size()); }
   void sync() { TTR_HIP(hipStreamSynchronize(stream)); }
   void end_call() { sync(); if (prof.on) prof.collect(); ws.reset(); pin.reset(); }
+};
+
+// Device workspace taken inside the scope goes back to the arena when the scope
+// ends, so a loop of calls (a QR per core step) reuses one region instead of
+// growing the arena by each call's share.  All device work runs on ctx.stream,
+// so the kernels that read the region are queued ahead of any that overwrite it.
+// Nothing allocated inside the scope may be used after it; a split-K buffer made
+// inside it is dropped too.
+class WsScope {
+ public:
+  explicit WsScope(Ctx& c) : c_(c), m_(c.ws.mark()), fs_(c.fsplit), fse_(c.fsplit_elems) {}
+  ~WsScope() {
+    c_.ws.rewind(m_);
+    c_.fsplit = fs_;
+    c_.fsplit_elems = fse_;
+  }
+  WsScope(const WsScope&) = delete;
+  WsScope& operator=(const WsScope&) = delete;
+
+ private:
+  Ctx& c_;
+  DevArena::Mark m_;
+  double* fs_;
+  size_t fse_;
 };
 
 // FNV-1a hash of a launch (kernel, grid, block, LDS bytes, argument bytes), kept

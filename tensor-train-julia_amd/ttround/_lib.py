@@ -93,6 +93,7 @@ SIGNATURES = {
     "ttr_sketch_tuple": (C.c_int, [C.c_void_p, i64, i64p, i64p, C.POINTER(RandOpts), C.POINTER(f64p), i64p]),
     "ttr_graph_stats": (C.c_int, [C.c_void_p, i64p, i64p, i64p, i64p]),
     "ttr_profile_reset": (C.c_int, [C.c_void_p]),
+    "ttr_trim": (C.c_int, [C.c_void_p]),
     "ttr_profile_count": (C.c_int, [C.c_void_p, i64p]),
     "ttr_profile_get": (C.c_int, [C.c_void_p, i64, C.c_char_p, i64, f64p, i64p, f64p, f64p]),
 }
@@ -157,6 +158,10 @@ class Handle:
         v = [C.c_int64() for _ in range(4)]
         self.check(self.lib.ttr_graph_stats(self.h, *[C.byref(x) for x in v]))
         return dict(zip(("captures", "replays", "fallbacks", "cached"), (x.value for x in v)))
+
+    def trim(self):
+        """Return the idle workspace arena and the freed TT storage to the runtime."""
+        self.check(self.lib.ttr_trim(self.h))
 
     def profile_reset(self):
         self.check(self.lib.ttr_profile_reset(self.h))

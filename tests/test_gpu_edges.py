@@ -76,3 +76,23 @@ def test_unit_modes_and_rank_one(T):
     s = to_oracle(T.tt_rounding(to_api(O.tt_add(one, one)), tol=1e-12))
     assert s.rks == [1, 1, 1, 1, 1]
     assert O.rel_err(O.tt_scale(2.0, one), s) < 1e-13
+
+
+def test_trim_keeps_live_objects(T):
+    """ttr_trim returns the idle workspace and freed storage; live TTs stay valid
+    and a call after it gives the same bits as before it."""
+    say("case: ttr_trim between two batched SVD roundings")
+    xs = [T.DeviceTT.randn((4,) * 6, [1, 4, 16, 12, 16, 4, 1], 900 + i) for i in range(3)]
+    s = T.lincomb_device([1.0, 0.5, 0.25], xs)
+    r1 = T.round_svd_batch([s, s], tol=1e-10)
+    c1 = [r1[0].core(k) for k in range(6)]
+    for o in r1:
+        o.free()
+    T.default_handle().trim()
+    r2 = T.round_svd_batch([s, s], tol=1e-10)
+    for k in range(6):
+        np.testing.assert_array_equal(r2[0].core(k), c1[k])
+        np.testing.assert_array_equal(r2[1].core(k), c1[k])
+    assert xs[0].core(2).shape == (4, 16, 12)
+    for o in r2 + xs + [s]:
+        o.free()
