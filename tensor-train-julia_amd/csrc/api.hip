@@ -1923,11 +1923,11 @@ int ttr_create(ttr_handle_t* out, int device) {
     if (const char* c = std::getenv("TTR_GRAPHS")) h->graphs = c[0] == '1';
     // TTR_JACOBI_SORT=0: no column sorting between Jacobi sweeps
     if (const char* c = std::getenv("TTR_JACOBI_SORT")) h->ctx.jacobi_sort = c[0] != '0';
-    // TTR_QR_PATH=p: 0 the QR path by size, 1 the blocked path at every size, 2 the
-    // blocked path with streaming panels (tests of each path)
+    // TTR_QR_PATH=p: 0 the QR path by size and batch, 1 the blocked path at every size, 2 the
+    // blocked path with streaming panels, 3 the one-launch QR wherever it fits (tests of each path)
     if (const char* c = std::getenv("TTR_QR_PATH")) {
       const int v = std::atoi(c);
-      TTR_REQUIRE(v >= 0 && v <= 2, ST_INVALID_ARG, "TTR_QR_PATH must be 0..2");
+      TTR_REQUIRE(v >= 0 && v <= 3, ST_INVALID_ARG, "TTR_QR_PATH must be 0..3");
       h->ctx.qr_path = v;
     }
     // TTR_JACOBI_MAX_SWEEPS=s (1..40): Jacobi sweeps before a non-converged SVD fails

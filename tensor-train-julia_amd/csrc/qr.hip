@@ -405,13 +405,19 @@ void launch_block(Ctx& ctx, const BlockJob* d, int njobs, int max_rows) {
 void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
   WsScope scope(ctx);  // tau, T factors and the blocks' V are needed only here
   {
-    int mm = 0;
+    int mm = 0, nn = 0;
     size_t bytes = 0;
     for (const auto& J : jobs) {
       mm = std::max(mm, J.m);
+      nn = std::max(nn, J.n);
       bytes += size_t(std::max(J.m, 0)) * size_t(std::max(J.n, 0)) * sizeof(double);
     }
-    if (ctx.qr_path == 0 && mm <= QR_SMALL_MAX_M && bytes <= ctx.qr_small_bytes) {
+    // One workgroup per matrix pays off only when the batch occupies the chip (or a
+    // matrix is a single panel): a few matrices run their column steps on a few CUs,
+    // while the blocked path spreads every update over all of them.
+    const bool fits = mm <= QR_SMALL_MAX_M && bytes <= ctx.qr_small_bytes;
+    const bool wide = jobs.size() >= QR_SMALL_MIN_BATCH || nn <= 16;
+    if ((ctx.qr_path == 0 && fits && wide) || (ctx.qr_path == 3 && mm <= QR_SMALL_MAX_M)) {
       qr_small_batched(ctx, jobs);
       return;
     }

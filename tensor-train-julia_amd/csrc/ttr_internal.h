@@ -199,8 +199,9 @@ struct Ctx {
   // size is at most this many bytes run one workgroup per matrix (qr_small_batched);
   // larger batches use the blocked path whose updates spread over the chip
   size_t qr_small_bytes = size_t(96) << 20;
-  // 0: the QR path by size; 1: never the one-launch QR (blocked path); 2: the
-  // blocked path with streaming panels of 32 at every height
+  // 0: the QR path by size and batch; 1: never the one-launch QR (blocked path); 2: the
+  // blocked path with streaming panels of 32 at every height; 3: the one-launch QR
+  // for every batch whose matrices have at most QR_SMALL_MAX_M rows
   int qr_path = 0;
   // partial sums of split-K GEMMs, reused by every launch on the stream
   double* splitk = nullptr;
@@ -378,6 +379,8 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs);
 // the same in one launch, one workgroup per matrix (m <= QR_SMALL_MAX_M); used by
 // qr_batched for batches whose matrices all fit (see Ctx::qr_small_bytes)
 constexpr int QR_SMALL_MAX_M = 1024;
+// and only for batches of at least this many matrices (one per CU), or of single panels
+constexpr size_t QR_SMALL_MIN_BATCH = 256;
 void qr_small_batched(Ctx& ctx, const std::vector<QRJob>& jobs);
 
 // ---------------------------------------------------------------------------

@@ -2,12 +2,13 @@
 orthogonalization and SVD sweeps, src/tt_randtools.jl:125-151 randomized
 projection, both through LAPACK's dgeqrf/dorgqr).
 
-The library picks a QR path by size: one launch per batch for matrices of at most
-1024 rows, else the blocked path (one launch per 64-column block with the 16-column
-panels and their in-block updates on chip, up to 3072 rows), else the blocked path
-with streaming 32-column panels and grouped-GEMM in-block updates.  A handle made
-with TTR_QR_PATH = 1 (blocked at every size) or 2 (streaming panels at every size)
-runs the same roundings through the other paths; each must equal the oracle to
+The library picks a QR path by size and batch: one launch per batch for batches of
+at least 256 matrices of at most 1024 rows (or of single 16-column panels), else the
+blocked path (one launch per 64-column block with the 16-column panels and their
+in-block updates on chip, up to 3072 rows), else the blocked path with streaming
+32-column panels and grouped-GEMM in-block updates.  A handle made with
+TTR_QR_PATH = 1 (blocked at every size), 2 (streaming panels at every size) or 3 (one
+launch wherever the rows fit) runs the same roundings through the other paths; each must equal the oracle to
 1e-12 (represented tensor, relative Frobenius norm), and the profile shows which
 kernel ran.  Shapes: unfoldings of 216 x 160 (three blocks, the last one two
 panels) and 960 x 72 (a last block of 8 columns); of 150 columns (a last block of
@@ -25,7 +26,7 @@ from helpers import device_blocks_provider, to_api, to_oracle
 
 pytestmark = pytest.mark.gpu
 
-PATH_KERNEL = {"0": None, "1": "qr_block", "2": "qr_panel"}
+PATH_KERNEL = {"0": None, "1": "qr_block", "2": "qr_panel", "3": "qr_small"}
 
 
 def say(*a):
@@ -58,7 +59,7 @@ def check_kernel(h, path):
         assert "qr_small" not in st and "qr_block" not in st
 
 
-@pytest.mark.parametrize("path", ["0", "1", "2"])
+@pytest.mark.parametrize("path", ["0", "1", "2", "3"])
 @pytest.mark.parametrize("dims,r1,r2,tol", [((6,) * 6, 100, 60, 1e-12), ((6,) * 6, 100, 60, 1e-4),
                                             ((5,) * 6, 100, 50, 1e-12), ((8,) * 6, 160, 100, 1e-8)])
 def test_tt_rounding_qr_paths(monkeypatch, path, dims, r1, r2, tol):
@@ -78,7 +79,7 @@ def test_tt_rounding_qr_paths(monkeypatch, path, dims, r1, r2, tol):
     assert err < 1e-12
 
 
-@pytest.mark.parametrize("path", ["0", "1", "2"])
+@pytest.mark.parametrize("path", ["0", "1", "2", "3"])
 def test_ttrand_rounding_qr_paths(monkeypatch, path):
     import ttround as T
     dims = (8,) * 6
