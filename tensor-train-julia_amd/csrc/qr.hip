@@ -163,7 +163,10 @@ struct BlockJob {
   int32_t m, c0, nbo;  // block columns c0 .. c0 + nbo - 1 (nbo <= NBO)
   double* tau;         // indexed by column
   double* T;           // T of the panel at columns 16 q .. at T + q * NBP * NBP, ld NBP
-  double* VX;          // the block's explicit V: (m - c0) x nbo, ld m - c0
+  double* VX;          // the block's explicit V: (m - c0) x nbo, ld vld
+  int64_t vld;
+  int32_t vz;          // rows above VX (of the same columns) set to zero: 0, or 64 for the
+                       // second block of a 128-column pair
 };
 
 template <int R>
@@ -177,7 +180,8 @@ __global__ __launch_bounds__(RT) void qr_block_kernel(const BlockJob* __restrict
   double* A = J.A;
   const int64_t lda = J.lda;
   const int m = J.m, nbo = J.nbo;
-  const int64_t mb = m - J.c0;  // rows of the block, the ld of VX
+  const int64_t mb = J.vld;  // the ld of VX
+  for (int e = tid; e < J.vz * nbo; e += RT) J.VX[e % J.vz - J.vz + mb * (e / J.vz)] = 0.0;
   for (int p = 0; NBR * p < nbo; ++p) {
     const int pc = J.c0 + NBR * p, nb = min(NBR, nbo - NBR * p), m2 = m - pc;
     double* A0 = A + pc + lda * pc;
@@ -340,10 +344,12 @@ __global__ __launch_bounds__(RT) void qr_block_kernel(const BlockJob* __restrict
 // the block), T_qq the panel's own T.  One 256-thread workgroup per matrix, two
 // small products per panel instead of one sequential step per column.
 struct TMergeJob {
-  const double* G;   // 64 x 64, ld 64
+  const double* G;   // 64 x 64, ld ldg
   const double* Tp;  // panel q's T at Tp + q * NBP * NBP, ld NBP
-  double* T;         // 64 x 64, ld 64
+  double* T;         // 64 x 64, ld ldt
   int32_t nb, w;     // columns of the block, panel width (16 or 32)
+  int32_t ldg, ldt;
+  int32_t zero_left; // 1: also zero the 64 x 64 block left of T (the lower-left block of a pair's T)
 };
 
 __global__ __launch_bounds__(256) void merge_t_kernel(const TMergeJob* __restrict__ jobs) {
@@ -354,7 +360,7 @@ __global__ __launch_bounds__(256) void merge_t_kernel(const TMergeJob* __restric
   __shared__ double X[NBO - 16][NBP + 1];
   for (int e = t; e < NBO * NBO; e += 256) {
     const int r = e % NBO, c = e / NBO;
-    Gs[r][c] = (r < nb && c < nb) ? J.G[r + NBO * c] : 0.0;
+    Gs[r][c] = (r < nb && c < nb) ? J.G[r + int64_t(J.ldg) * c] : 0.0;
     const int q = c / w;
     Ts[r][c] = (r < nb && c < nb && r / w == q) ? J.Tp[size_t(q) * NBP * NBP + (r - q * w) + NBP * (c - q * w)] : 0.0;
   }
@@ -378,7 +384,9 @@ __global__ __launch_bounds__(256) void merge_t_kernel(const TMergeJob* __restric
     }
     __syncthreads();
   }
-  for (int e = t; e < NBO * NBO; e += 256) J.T[e] = Ts[e % NBO][e / NBO];
+  for (int e = t; e < NBO * NBO; e += 256) J.T[e % NBO + int64_t(J.ldt) * (e / NBO)] = Ts[e % NBO][e / NBO];
+  if (J.zero_left)
+    for (int e = t; e < NBO * NBO; e += 256) J.T[e % NBO + int64_t(J.ldt) * (e / NBO - NBO)] = 0.0;
 }
 
 void launch_panel(Ctx& ctx, const PanelJob* d, int njobs) {
@@ -398,6 +406,197 @@ void launch_block(Ctx& ctx, const BlockJob* d, int njobs, int max_rows) {
     default: TTR_LAUNCH(ctx, qr_block_kernel<6>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
   }
   TTR_HIP(hipGetLastError());
+}
+
+// Blocked QR of matrices up to 6 * RT rows with the 64-column blocks taken in
+// pairs: block 2s is factored (its panels and in-block updates in one launch),
+// applied to block 2s + 1 only, block 2s + 1 is factored, and the pair's 128
+// reflectors, as one compact WY factor (V_s explicit, T_s = [T_0, T_01; 0, T_1]
+// with T_01 = -T_0 (V_0^T V_1) T_1), update the trailing columns and later form
+// Q.  The trailing updates and the formation of Q then multiply with K = 128,
+// so they pass over the trailing matrix half as often as with 64-column blocks.
+void qr_pairs(Ctx& ctx, std::vector<QRJob>& jobs) {
+  constexpr int NS = 2 * NBO;  // columns of a pair
+  struct Aux {
+    double *tau, *T, *TS, *G, *Y, *W, *W2, *VX;
+    bool vx_all;
+    int kq, nblock, npair;
+  };
+  std::vector<Aux> aux(jobs.size());
+  int max_pairs = 0;
+  for (size_t q = 0; q < jobs.size(); ++q) {
+    const QRJob& J = jobs[q];
+    TTR_REQUIRE(J.m >= 0 && J.n >= 0 && J.lda >= std::max(1, J.m), ST_INVALID_ARG, "qr: bad shape");
+    Aux& a = aux[q];
+    a.kq = std::min(J.m, J.n);
+    a.nblock = (a.kq + NBO - 1) / NBO;
+    a.npair = (a.kq + NS - 1) / NS;
+    max_pairs = std::max(max_pairs, a.npair);
+    const int npanel = (a.kq + NBR - 1) / NBR;
+    a.tau = ctx.ws.alloc_n<double>(std::max(1, J.n));
+    a.T = ctx.ws.alloc_n<double>(size_t(std::max(1, npanel)) * NBP * NBP);
+    a.TS = ctx.ws.alloc_n<double>(size_t(std::max(1, a.npair)) * NS * NS);
+    a.G = ctx.ws.alloc_n<double>(size_t(NS) * NBO);
+    a.Y = ctx.ws.alloc_n<double>(size_t(NBO) * NBO);
+    a.W = ctx.ws.alloc_n<double>(size_t(NS) * std::max(1, J.n));
+    a.W2 = ctx.ws.alloc_n<double>(size_t(NS) * std::max(1, J.n));
+    // pair s: (m - 128 s) x 128, ld m - 128 s; all pairs kept when Q is formed
+    a.vx_all = J.Q != nullptr;
+    a.VX = ctx.ws.alloc_n<double>(
+        a.vx_all ? size_t(NS) * (size_t(a.npair) * J.m - size_t(NS / 2) * a.npair * std::max(0, a.npair - 1))
+                 : size_t(std::max(1, J.m)) * NS);
+  }
+  auto blk_cols = [&](const Aux& a, int b) { return std::max(0, std::min(NBO, a.kq - b * NBO)); };
+  auto vxs = [&](const QRJob& J, const Aux& a, int s) {
+    return a.VX + (a.vx_all ? int64_t(NS) * (int64_t(s) * J.m - int64_t(NS / 2) * s * (s - 1)) : 0);
+  };
+  auto ts = [&](const Aux& a, int s) { return a.TS + size_t(s) * NS * NS; };
+  GemmBatch gb;
+  // one 64-column block of every job that has it: panels and in-block updates on chip
+  auto factor_block = [&](int s, int h) {
+    std::vector<BlockJob> bj;
+    double fl = 0.0, by = 0.0;
+    int max_rows = 0;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const QRJob& J = jobs[q];
+      Aux& a = aux[q];
+      const int b = 2 * s + h, nbo = blk_cols(a, b);
+      if (nbo <= 0) continue;
+      const int c0 = b * NBO, mb = J.m - c0;
+      const int64_t ld = J.m - int64_t(NS) * s;
+      bj.push_back(BlockJob{J.A, J.lda, J.m, c0, nbo, a.tau, a.T, vxs(J, a, s) + h * (NBO + NBO * ld), ld,
+                            h ? NBO : 0});
+      max_rows = std::max(max_rows, mb);
+      for (int c = 0; c < nbo; c += NBR) {
+        const double m2 = J.m - (c0 + c), nb = std::min(NBR, nbo - c), n2 = nbo - c - nb;
+        fl += 4.0 * m2 * nb * nb + 4.0 * m2 * nb * n2;  // panel (2 m nb^2) + T, W = V^T C and C -= V W2
+      }
+      by += 24.0 * double(mb) * nbo;  // the block read and written, V written
+    }
+    if (bj.empty()) return;
+    const BlockJob* d = ctx.upload(bj);
+    ProfScope ps(ctx, "qr_block", fl, by);
+    launch_block(ctx, d, int(bj.size()), max_rows);
+  };
+  // T of block h of pair s from its Gram (in G, ld 128) and its panels' T
+  auto merge_block = [&](int s, int h) {
+    std::vector<TMergeJob> tb;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const Aux& a = aux[q];
+      const int b = 2 * s + h, nbo = blk_cols(a, b);
+      if (nbo <= 0) continue;
+      tb.push_back(TMergeJob{a.G + h * NBO, a.T + size_t(b * NBO / NBR) * NBP * NBP,
+                             ts(a, s) + h * (NBO + NBO * NS), nbo, NBR, NS, NS, h});
+    }
+    if (tb.empty()) return;
+    const TMergeJob* d = ctx.upload(tb);
+    TTR_LAUNCH(ctx, merge_t_kernel, dim3(unsigned(tb.size())), dim3(256), 0, d);
+  };
+  // C -= V (T^T (V^T C)) (trans) or C -= V (T (V^T C)) for every job: V the first
+  // kv columns of pair s's V, C columns [j0, j1) of the job's A or Q from row 128 s
+  auto apply = [&](int s, bool trans, bool on_q, const char* site, auto kv_of, auto cols_of) {
+    for (int stage = 0; stage < 3; ++stage) {
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const QRJob& J = jobs[q];
+        const Aux& a = aux[q];
+        if (s >= a.npair || (on_q && !J.Q)) continue;
+        const int kv = kv_of(a, s);
+        int j0, j1;
+        cols_of(J, a, s, j0, j1);
+        const int r0 = NS * s, m2 = J.m - r0, n2 = j1 - j0;
+        if (kv <= 0 || n2 <= 0) continue;
+        const int64_t ld = J.m - int64_t(NS) * s, ldc = on_q ? J.ldq : J.lda;
+        double* C = (on_q ? J.Q : J.A) + r0 + ldc * j0;
+        const double* V = vxs(J, a, s);
+        const double* T = ts(a, s);
+        if (stage == 0)  // W = V^T C  (kv x n2)
+          gb.add(kv, n2, m2, 1.0, V, ld, 1, C, 1, ldc, 0.0, a.W, 1, NS);
+        else if (stage == 1)  // W2 = T^T W or T W  (kv x n2)
+          gb.add(kv, n2, kv, 1.0, T, trans ? NS : 1, trans ? 1 : NS, a.W, 1, NS, 0.0, a.W2, 1, NS);
+        else  // C -= V W2
+          gb.add(m2, n2, kv, -1.0, V, 1, ld, a.W2, 1, NS, 1.0, C, 1, ldc);
+      }
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, site);
+    }
+  };
+  auto pair_cols = [&](const Aux& a, int s) { return blk_cols(a, 2 * s) + blk_cols(a, 2 * s + 1); };
+  // ---- factorisation
+  for (int s = 0; s < max_pairs; ++s) {
+    factor_block(s, 0);
+    // G(0:64, 0:64) = V_0^T V_0, T_0
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const QRJob& J = jobs[q];
+      const Aux& a = aux[q];
+      const int nb0 = blk_cols(a, 2 * s);
+      if (nb0 <= 0) continue;
+      const int64_t ld = J.m - int64_t(NS) * s;
+      const double* V = vxs(J, a, s);
+      gb.add(nb0, nb0, int(ld), 1.0, V, ld, 1, V, 1, ld, 0.0, a.G, 1, NS);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
+    merge_block(s, 0);
+    // block 0 applied to block 1's columns
+    apply(s, true, false, "qr_pair", [&](const Aux& a, int ss) { return blk_cols(a, 2 * ss + 1) > 0 ? blk_cols(a, 2 * ss) : 0; },
+          [&](const QRJob&, const Aux& a, int ss, int& j0, int& j1) {
+            j0 = (2 * ss + 1) * NBO;
+            j1 = j0 + blk_cols(a, 2 * ss + 1);
+          });
+    factor_block(s, 1);
+    // G(0:128, 0:64) = V_s^T V_1: rows 0..63 hold V_0^T V_1, rows 64..127 V_1^T V_1
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const QRJob& J = jobs[q];
+      const Aux& a = aux[q];
+      const int nb1 = blk_cols(a, 2 * s + 1);
+      if (nb1 <= 0) continue;
+      const int64_t ld = J.m - int64_t(NS) * s;
+      const double* V = vxs(J, a, s);
+      gb.add(NBO + nb1, nb1, int(ld), 1.0, V, ld, 1, V + NBO * ld, 1, ld, 0.0, a.G, 1, NS);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
+    // T_1 from rows 64.. of G, then T_01 = -T_0 (V_0^T V_1) T_1
+    std::vector<TMergeJob> tb;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const Aux& a = aux[q];
+      const int nb1 = blk_cols(a, 2 * s + 1);
+      if (nb1 <= 0) continue;
+      tb.push_back(TMergeJob{a.G + NBO, a.T + size_t((2 * s + 1) * NBO / NBR) * NBP * NBP,
+                             ts(a, s) + NBO + NBO * NS, nb1, NBR, NS, NS, 1});
+    }
+    if (!tb.empty()) {
+      const TMergeJob* d = ctx.upload(tb);
+      TTR_LAUNCH(ctx, merge_t_kernel, dim3(unsigned(tb.size())), dim3(256), 0, d);
+    }
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const Aux& a = aux[q];
+      const int nb1 = blk_cols(a, 2 * s + 1);
+      if (nb1 <= 0) continue;
+      gb.add(NBO, nb1, nb1, 1.0, a.G, 1, NS, ts(a, s) + NBO + NBO * NS, 1, NS, 0.0, a.Y, 1, NBO);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_tmerge");
+    for (size_t q = 0; q < jobs.size(); ++q) {
+      const Aux& a = aux[q];
+      const int nb1 = blk_cols(a, 2 * s + 1);
+      if (nb1 <= 0) continue;
+      gb.add(NBO, nb1, NBO, -1.0, ts(a, s), 1, NS, a.Y, 1, NBO, 0.0, ts(a, s) + NBO * NS, 1, NS);
+    }
+    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_tmerge");
+    // the pair applied to the trailing columns
+    apply(s, true, false, "qr_trail", pair_cols, [&](const QRJob& J, const Aux& a, int ss, int& j0, int& j1) {
+      j0 = NS * ss + pair_cols(a, ss);
+      j1 = J.n;
+    });
+  }
+  // ---- thin Q by backward accumulation over the pairs: Q2 -= V_s (T_s (V_s^T Q2))
+  std::vector<Fill2D> fills;
+  for (size_t q = 0; q < jobs.size(); ++q)
+    if (jobs[q].Q && aux[q].kq > 0) fills.push_back(Fill2D{jobs[q].Q, jobs[q].ldq, jobs[q].m, aux[q].kq, 1.0});
+  if (fills.empty()) return;
+  fill2d_batched(ctx, fills);
+  for (int s = max_pairs - 1; s >= 0; --s)
+    apply(s, false, true, "qr_formq", pair_cols, [&](const QRJob&, const Aux& a, int ss, int& j0, int& j1) {
+      j0 = NS * ss;
+      j1 = a.kq;
+    });
 }
 
 }  // namespace
@@ -422,6 +621,16 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       return;
     }
   }
+  {
+    // register-resident blocks (panels of 16) up to 6*512 rows, taken in pairs; else
+    // 64-column blocks of streaming panels (width 32)
+    int max_m = 0;
+    for (const auto& J : jobs) max_m = std::max(max_m, J.m);
+    if (max_m <= 6 * RT && ctx.qr_path != 2) {
+      qr_pairs(ctx, jobs);
+      return;
+    }
+  }
   struct Aux {
     double* tau;
     double* T;   // inner panels: npanel * NBP * NBP
@@ -436,11 +645,8 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     int kq, nblock;
   };
   std::vector<Aux> aux(jobs.size());
-  int max_blocks = 0, max_m = 0;
-  for (const auto& J : jobs) max_m = std::max(max_m, J.m);
-  // register-resident blocks (panels of 16) up to 6*512 rows, else the streaming panel (width 32)
-  const bool reg = max_m <= 6 * RT && ctx.qr_path != 2;
-  const int NB = reg ? NBR : NBP;
+  int max_blocks = 0;
+  const int NB = NBP;
   for (size_t q = 0; q < jobs.size(); ++q) {
     const QRJob& J = jobs[q];
     TTR_REQUIRE(J.m >= 0 && J.n >= 0 && J.lda >= std::max(1, J.m), ST_INVALID_ARG, "qr: bad shape");
@@ -479,30 +685,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
   };
   // ---- factorisation
   for (int b = 0; b < max_blocks; ++b) {
-    if (reg) {  // the block's panels and in-block updates in one launch, V to VX
-      std::vector<BlockJob> bj;
-      double fl = 0.0, by = 0.0;
-      int max_rows = 0;
-      for (size_t q = 0; q < jobs.size(); ++q) {
-        const QRJob& J = jobs[q];
-        Aux& a = aux[q];
-        if (b >= a.nblock) continue;
-        const int c0 = b * NBO, nbo = block_cols(a, b), mb = J.m - c0;
-        bj.push_back(BlockJob{J.A, J.lda, J.m, c0, nbo, a.tau, a.T, vx(J, a, b)});
-        max_rows = std::max(max_rows, mb);
-        for (int c = 0; c < nbo; c += NBR) {
-          const double m2 = J.m - (c0 + c), nb = std::min(NBR, nbo - c), n2 = nbo - c - nb;
-          fl += 4.0 * m2 * nb * nb + 4.0 * m2 * nb * n2;  // panel (2 m nb^2) + T, W = V^T C and C -= V W2
-        }
-        by += 24.0 * double(mb) * nbo;  // the block read and written, V written
-      }
-      if (!bj.empty()) {
-        const BlockJob* d = ctx.upload(bj);
-        ProfScope ps(ctx, "qr_block", fl, by);
-        launch_block(ctx, d, int(bj.size()), max_rows);
-      }
-    }
-    for (int ip = 0; !reg && ip < NBO / NB; ++ip) {
+    for (int ip = 0; ip < NBO / NB; ++ip) {
       std::vector<PanelJob> pj;
       for (size_t q = 0; q < jobs.size(); ++q) {
         const Aux& a = aux[q];
@@ -549,7 +732,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       }
     }
     // merge the block's reflectors: G = V^T V, T from (G, tau)
-    if (!reg) copy_vx(b);
+    copy_vx(b);
     std::vector<TMergeJob> tb;
     for (size_t q = 0; q < jobs.size(); ++q) {
       const QRJob& J = jobs[q];
@@ -557,7 +740,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       if (b >= a.nblock) continue;
       const int c0 = b * NBO, m2 = J.m - c0, nbo = block_cols(a, b);
       gb.add(nbo, nbo, m2, 1.0, vx(J, a, b), m2, 1, vx(J, a, b), 1, m2, 0.0, a.G, 1, NBO);
-      tb.push_back(TMergeJob{a.G, a.T + size_t(c0 / NB) * NBP * NBP, a.TO + size_t(b) * NBO * NBO, nbo, NB});
+      tb.push_back(TMergeJob{a.G, a.T + size_t(c0 / NB) * NBP * NBP, a.TO + size_t(b) * NBO * NBO, nbo, NB, NBO, NBO, 0});
     }
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
     if (!tb.empty()) {
