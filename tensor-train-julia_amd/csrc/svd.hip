@@ -615,6 +615,68 @@ __global__ __launch_bounds__(256) void column_norms_kernel(const SVDJob* __restr
       if (c0 + q < S.p) S.sv[c0 + q] = sqrt(s[q]);
 }
 
+// Column order of a Jacobi matrix before a sweep: the stable order of decreasing
+// norm (equal norms keep their order), as the rank of each column, perm[rank] = column.
+// A NaN norm ranks below every number, so the ranks always form a permutation.
+// changed[q] is set when matrix q's order is not the identity.
+struct PermJob {
+  double *B, *J, *tB, *tJ;  // the matrix and its rotations; p x p scratch for both
+  int64_t ldb, ldj;
+  const double* nrm;
+  int32_t* perm;
+  int32_t p;
+};
+
+__device__ __forceinline__ double sort_key(double v) { return isnan(v) ? -INFINITY : v; }
+
+__global__ __launch_bounds__(256) void sort_perm_kernel(const PermJob* __restrict__ jobs, int32_t* __restrict__ changed) {
+  const PermJob P = jobs[blockIdx.x];
+  const int i = blockIdx.y * 256 + threadIdx.x;
+  __shared__ double v[256];
+  const double vi = i < P.p ? sort_key(P.nrm[i]) : 0.0;
+  int rank = 0;
+  for (int j0 = 0; j0 < P.p; j0 += 256) {
+    __syncthreads();
+    if (j0 + int(threadIdx.x) < P.p) v[threadIdx.x] = sort_key(P.nrm[j0 + threadIdx.x]);
+    __syncthreads();
+    const int n = min(256, P.p - j0);
+    for (int t = 0; t < n; ++t) {
+      const double vj = v[t];
+      rank += (vj > vi || (vj == vi && j0 + t < i)) ? 1 : 0;
+    }
+  }
+  if (i < P.p) {
+    P.perm[rank] = i;
+    if (rank != i) atomicOr(changed + blockIdx.x, 1);
+  }
+}
+
+// tB(:, c) = B(:, perm[c]) and tJ(:, c) = J(:, perm[c]) for the matrices whose order changed
+__global__ __launch_bounds__(256) void permute_gather_kernel(const PermJob* __restrict__ jobs,
+                                                             const int32_t* __restrict__ changed) {
+  if (!changed[blockIdx.y]) return;
+  const PermJob P = jobs[blockIdx.y];
+  const int64_t tot = int64_t(P.p) * P.p;
+  for (int64_t e = blockIdx.x * int64_t(256) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * 256) {
+    const int64_t r = e % P.p, c = e / P.p;
+    const int64_t src = min(max(P.perm[c], 0), P.p - 1);
+    P.tB[e] = P.B[r + P.ldb * src];
+    P.tJ[e] = P.J[r + P.ldj * src];
+  }
+}
+
+__global__ __launch_bounds__(256) void permute_back_kernel(const PermJob* __restrict__ jobs,
+                                                           const int32_t* __restrict__ changed) {
+  if (!changed[blockIdx.y]) return;
+  const PermJob P = jobs[blockIdx.y];
+  const int64_t tot = int64_t(P.p) * P.p;
+  for (int64_t e = blockIdx.x * int64_t(256) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * 256) {
+    const int64_t r = e % P.p, c = e / P.p;
+    P.B[r + P.ldb * c] = P.tB[e];
+    P.J[r + P.ldj * c] = P.tJ[e];
+  }
+}
+
 // pseudo-inverse weights: one workgroup per matrix, the maximum singular value by
 // a workgroup reduction, then column c of BJ scaled by 1/sv_c^2 or zeroed
 __global__ __launch_bounds__(256) void pinv_weights_kernel(const PinvWeights* __restrict__ jobs) {
@@ -684,7 +746,6 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   double* vbuf = split ? ctx.ws.alloc_n<double>(size_t(slots) * WC * WC) : nullptr;
   int32_t* rotf = split ? ctx.ws.alloc_n<int32_t>(size_t(slots)) : nullptr;
   int32_t* flags = ctx.ws.alloc_n<int32_t>(size_t(2) * nj);
-  int32_t* hflags = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * nj));
   fill2d_batched(ctx, fills);
   zero_fill(ctx, flags, sizeof(int32_t) * 2 * nj);
   const int max_sweeps = std::max(1, std::min(MAX_SWEEPS, ctx.jacobi_max_sweeps));
@@ -699,53 +760,61 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
     max_p = std::max(max_p, live[q].p);
   }
   const int cn_y = (max_p + CN_COLS - 1) / CN_COLS;  // column-norm workgroups per matrix
-  double* nrm = nullptr;
-  int32_t* perm = nullptr;
-  double* hn = nullptr;
-  int32_t* hp = nullptr;
   const SVDJob* dn = nullptr;
-  std::vector<double*> tB(nj), tJ(nj);
+  const PermJob* dperm = nullptr;
+  int32_t* changed = nullptr;
   if (ctx.jacobi_sort) {
-    nrm = ctx.ws.alloc_n<double>(size_t(off[nj]));
-    perm = ctx.ws.alloc_n<int32_t>(size_t(off[nj]));
-    hn = static_cast<double*>(ctx.pin.alloc(sizeof(double) * size_t(off[nj])));
-    hp = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * size_t(off[nj])));
+    double* nrm = ctx.ws.alloc_n<double>(size_t(off[nj]));
+    int32_t* perm = ctx.ws.alloc_n<int32_t>(size_t(off[nj]));
+    changed = ctx.ws.alloc_n<int32_t>(size_t(nj));
     std::vector<SVDJob> nj_jobs = live;
-    for (int q = 0; q < nj; ++q) {
-      nj_jobs[q].sv = nrm + off[q];
-      tB[q] = ctx.ws.alloc_n<double>(size_t(live[q].p) * live[q].p);
-      tJ[q] = ctx.ws.alloc_n<double>(size_t(live[q].p) * live[q].p);
-    }
-    dn = ctx.upload(nj_jobs);
-  }
-  auto sort_columns = [&]() {
-    TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj), unsigned(cn_y)), dim3(256), 0, dn);
-    TTR_HIP(hipGetLastError());
-    TTR_HIP(hipMemcpyAsync(hn, nrm, sizeof(double) * size_t(off[nj]), hipMemcpyDeviceToHost, ctx.stream));
-    ctx.sync();
-    std::vector<GatherCols> g;
-    std::vector<Copy2D> cp;
+    std::vector<PermJob> pj(static_cast<size_t>(nj));
     for (int q = 0; q < nj; ++q) {
       const int p = live[q].p;
-      int32_t* o = hp + off[q];
-      const double* v = hn + off[q];
-      for (int c = 0; c < p; ++c) o[c] = c;
-      std::stable_sort(o, o + p, [&](int32_t a, int32_t b) { return v[a] > v[b]; });
-      bool ident = true;
-      for (int c = 0; c < p && ident; ++c) ident = o[c] == c;
-      if (ident) continue;
-      g.push_back(GatherCols{live[q].B, tB[q], live[q].ldb, p, perm + off[q], p, p});
-      g.push_back(GatherCols{live[q].J, tJ[q], live[q].ldj, p, perm + off[q], p, p});
-      cp.push_back(copy_job(tB[q], 1, p, live[q].B, 1, live[q].ldb, p, p));
-      cp.push_back(copy_job(tJ[q], 1, p, live[q].J, 1, live[q].ldj, p, p));
+      nj_jobs[q].sv = nrm + off[q];
+      pj[q] = PermJob{live[q].B, live[q].J, ctx.ws.alloc_n<double>(size_t(p) * p), ctx.ws.alloc_n<double>(size_t(p) * p),
+                      live[q].ldb, live[q].ldj, nrm + off[q], perm + off[q], p};
     }
-    if (g.empty()) return;
-    TTR_HIP(hipMemcpyAsync(perm, hp, sizeof(int32_t) * size_t(off[nj]), hipMemcpyHostToDevice, ctx.stream));
-    gather_cols_batched(ctx, g);
-    copy2d_batched(ctx, cp);
+    dn = ctx.upload(nj_jobs);
+    dperm = ctx.upload(pj);
+  }
+  // the sort runs on the device (no host round trip per sweep)
+  const unsigned perm_x = unsigned(std::min<int64_t>(64, (int64_t(max_p) * max_p + 255) / 256));
+  auto sort_columns = [&]() {
+    TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj), unsigned(cn_y)), dim3(256), 0, dn);
+    zero_fill(ctx, changed, sizeof(int32_t) * size_t(nj));
+    TTR_LAUNCH(ctx, sort_perm_kernel, dim3(unsigned(nj), unsigned((max_p + 255) / 256)), dim3(256), 0, dperm, changed);
+    TTR_LAUNCH(ctx, permute_gather_kernel, dim3(perm_x, unsigned(nj)), dim3(256), 0, dperm,
+               static_cast<const int32_t*>(changed));
+    TTR_LAUNCH(ctx, permute_back_kernel, dim3(perm_x, unsigned(nj)), dim3(256), 0, dperm,
+               static_cast<const int32_t*>(changed));
+    TTR_HIP(hipGetLastError());
+  };
+  // The flags of sweep s are copied to pinned memory behind its launches and read
+  // while sweep s + 1 runs: the host never waits for the sweep it has just queued.
+  // The sweep after the last rotating one leaves every matrix unchanged (its
+  // workgroups return at once, its columns are already in order).
+  int32_t* hfl = static_cast<int32_t*>(ctx.pin.alloc(sizeof(int32_t) * 2 * size_t(nj)));
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (auto x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  for (auto& x : ev.e) TTR_HIP(hipEventCreateWithFlags(&x, hipEventDisableTiming));
+  std::vector<int> rotating;  // matrices still rotating after each checked sweep
+  int last_checked = -1;
+  auto still_rotating = [&](int sw) {  // matrices that rotated in sweep sw (copied flags)
+    TTR_HIP(hipEventSynchronize(ev.e[sw & 1]));
+    const int32_t* f = hfl + size_t(sw & 1) * nj;
+    int cnt = 0;
+    for (int q = 0; q < nj; ++q) cnt += f[q] != 0;
+    rotating.push_back(cnt);
+    last_checked = sw;
+    return cnt;
   };
   bool converged = false;
-  std::vector<int> rotating;  // matrices still rotating after each checked sweep
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     int32_t* cur = flags + size_t(sweep & 1) * nj;
     if (sweep >= 2) zero_fill(ctx, cur, sizeof(int32_t) * nj);
@@ -767,21 +836,19 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
       }
       TTR_HIP(hipGetLastError());
     }
-    // convergence check from the third sweep on (no matrix converges earlier in
-    // practice) and after the last sweep allowed
+    // convergence checked from the third sweep on (no matrix converges earlier in
+    // practice) and after the last sweep allowed, one sweep late
     if (sweep >= 2 || sweep == max_sweeps - 1) {
-      TTR_HIP(hipMemcpyAsync(hflags, cur, sizeof(int32_t) * nj, hipMemcpyDeviceToHost, ctx.stream));
-      ctx.sync();
-      int cnt = 0;
-      for (int q = 0; q < nj; ++q) cnt += hflags[q] != 0;
-      rotating.push_back(cnt);
-      const bool any = cnt > 0;
-      if (!any) {
-        converged = true;
-        break;
-      }
+      TTR_HIP(hipMemcpyAsync(hfl + size_t(sweep & 1) * nj, cur, sizeof(int32_t) * nj, hipMemcpyDeviceToHost,
+                             ctx.stream));
+      TTR_HIP(hipEventRecord(ev.e[sweep & 1], ctx.stream));
+    }
+    if (sweep - 1 >= 2 && still_rotating(sweep - 1) == 0) {
+      converged = true;
+      break;
     }
   }
+  if (!converged && last_checked != max_sweeps - 1) converged = still_rotating(max_sweeps - 1) == 0;
   // a matrix that still rotated in the last sweep has no converged SVD: report it
   // (the reference's LAPACK driver either converges or throws, src/tt_rounding.jl:191-195)
   if (!converged) {
@@ -790,8 +857,9 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
     for (int c : rotating) msg += " " + std::to_string(c);
     msg += "; sizes of the unconverged ones:";
     int shown = 0;
+    const int32_t* f = hfl + size_t(last_checked & 1) * nj;
     for (int q = 0; q < nj && shown < 16; ++q)
-      if (hflags[q]) {
+      if (f[q]) {
         msg += " " + std::to_string(live[q].p);
         ++shown;
       }
