@@ -879,8 +879,6 @@ struct RandTerm {
   DevTTO* H = nullptr;
   std::vector<void*> W;     // per core k: (rc(k) x s_k) in sketch precision
   int32_t* E;               // per core exponent (d+1 ints, device)
-  double* Y[2];
-  double* Pb;
   double* U = nullptr;      // operator terms: P x_{k+1} for every H bond
   int64_t rc(int k) const { return x->rks[k] * (H ? H->rks[k] : 1); }
 };
@@ -914,6 +912,18 @@ struct RandItem {
   std::vector<RandTerm> terms;
   std::vector<int64_t> rks, p, s;  // target ranks, block counts, sketch ranks (d+1)
   double* Z = nullptr;
+  // The terms' current cores side by side, [Y_1 | Y_2 | ...] ((r_k n_k) x sum_t rc_t(k+1),
+  // two buffers for the current and the next core), and their projections
+  // [P_1 | P_2 | ...] (r_{k+1} x sum_t rc_t(k+1)): one GEMM forms every P_t of an item
+  double* Yall[2] = {nullptr, nullptr};
+  double* Pall = nullptr;
+  // column of term t's block at bond k
+  int64_t col(size_t t, int k) const {
+    int64_t c = 0;
+    for (size_t u = 0; u < t; ++u) c += terms[u].rc(k);
+    return c;
+  }
+  int64_t cols(int k) const { return col(terms.size(), k); }
 };
 
 // right-to-left sketch phase of tt_recursive_sketch (src/sketches.jl:221-240);
@@ -1124,27 +1134,35 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
     int64_t zmax = 1;
     for (int k = 0; k + 1 < d; ++k) zmax = std::max(zmax, orr[i][k] * sp.dims[k] * items[i].s[k + 1]);
     items[i].Z = ctx.ws.alloc_n<double>(size_t(zmax));
+    int64_t ymax = 1, pmax = 1;
+    for (int k = 0; k + 1 < d; ++k) {
+      ymax = std::max(ymax, orr[i][k] * sp.dims[k] * items[i].cols(k + 1));
+      pmax = std::max(pmax, orr[i][k + 1] * items[i].cols(k + 1));
+    }
+    items[i].Yall[0] = ctx.ws.alloc_n<double>(size_t(ymax));
+    items[i].Yall[1] = ctx.ws.alloc_n<double>(size_t(ymax));
+    items[i].Pall = ctx.ws.alloc_n<double>(size_t(pmax));
     for (auto& t : items[i].terms) {
-      int64_t ymax = sp.dims[0] * t.rc(1), pmax = 1, umax = 1;
-      for (int k = 0; k + 1 < d; ++k) {
-        ymax = std::max(ymax, orr[i][k + 1] * sp.dims[k + 1] * t.rc(k + 2));
-        pmax = std::max(pmax, orr[i][k + 1] * t.rc(k + 1));
+      int64_t umax = 1;
+      for (int k = 0; k + 1 < d; ++k)
         if (t.H) umax = std::max(umax, orr[i][k + 1] * sp.dims[k + 1] * t.x->rks[k + 2] * t.H->rks[k + 1]);
-      }
-      t.Y[0] = ctx.ws.alloc_n<double>(size_t(ymax));
-      t.Y[1] = ctx.ws.alloc_n<double>(size_t(ymax));
-      t.Pb = ctx.ws.alloc_n<double>(size_t(pmax));
       if (t.H) t.U = ctx.ws.alloc_n<double>(size_t(umax));
     }
   }
+  // term tt's core at step k in buffer b, and its projection P at step k
+  auto Yat = [&](size_t i, size_t tt, int k, int b) {
+    return items[i].Yall[b] + orr[i][k] * sp.dims[k] * items[i].col(tt, k + 1);
+  };
+  auto Pat = [&](size_t i, size_t tt, int k) { return items[i].Pall + orr[i][k + 1] * items[i].col(tt, k + 1); };
   // operator terms: the first core of H x (1 x n x (r_1 R_1), x rank fastest)
   for (size_t i = 0; i < items.size(); ++i)
-    for (auto& t : items[i].terms) {
+    for (size_t tt = 0; tt < items[i].terms.size(); ++tt) {
+      const RandTerm& t = items[i].terms[tt];
       if (!t.H) continue;
       const int64_t n0 = sp.dims[0], r1 = t.x->rks[1];
       OpApply a = op_job(t.H, 0, false);
       a.in = t.x->cores[0];
-      a.out = t.Y[0];
+      a.out = Yat(i, tt, 0, 0);
       a.X = 1;
       a.Y = int(r1);
       a.is_m = 1; a.is_b = 0; a.is_x = 0; a.is_y = n0;
@@ -1156,14 +1174,18 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
   int yb = 0;
   for (int k = 0; k + 1 < d; ++k) {
     const int64_t n = sp.dims[k];
-    auto Ycur = [&](const RandTerm& t) -> const double* { return (k == 0 && !t.H) ? t.x->cores[0] : t.Y[yb]; };
+    auto Ycur = [&](size_t i, size_t tt) -> const double* {
+      const RandTerm& t = items[i].terms[tt];
+      return (k == 0 && !t.H) ? t.x->cores[0] : Yat(i, tt, k, yb);
+    };
     // Z_k = sum_t Y_t W_t (FP64 accumulate; terms rescaled by their sketch exponents)
     for (size_t i = 0; i < items.size(); ++i) {
       const int64_t r = orr[i][k], s = items[i].s[k + 1];
       gb.prob(items[i].Z, 1, r * n, int(r * n), int(s), 1.0, 0.0);
-      for (auto& t : items[i].terms) {
+      for (size_t tt = 0; tt < items[i].terms.size(); ++tt) {
+        const RandTerm& t = items[i].terms[tt];
         const int64_t R1 = t.rc(k + 1);
-        gb.seg(Ycur(t), 1, r * n, t.W[k + 1], 1, R1, int(R1), k == 0 ? t.alpha : 1.0, AM_PLAIN, t.E + k + 1);
+        gb.seg(Ycur(i, tt), 1, r * n, t.W[k + 1], 1, R1, int(R1), k == 0 ? t.alpha : 1.0, AM_PLAIN, t.E + k + 1);
       }
     }
     gemm_run(ctx, gb, Prec::F64, sp.wprec, Prec::F64, Prec::F64, "rr_Z");
@@ -1173,13 +1195,20 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
       qj.push_back(QRJob{items[i].Z, r * n, int(r * n), int(s), outs[i]->cores[k], r * n});
     }
     qr_batched(ctx, qj);
-    // P_t = Q^T Y_t
+    // P_t = Q^T Y_t: per term on the first core (the terms' own cores, scaled by
+    // alpha_t), then one product Q^T [Y_1 | Y_2 | ...] per item (same bits per entry)
     for (size_t i = 0; i < items.size(); ++i) {
       const int64_t r = orr[i][k], r1 = orr[i][k + 1];
-      for (auto& t : items[i].terms) {
+      if (k > 0) {
+        gb.add(int(r1), int(items[i].cols(k + 1)), int(r * n), 1.0, outs[i]->cores[k], r * n, 1, Yat(i, 0, k, yb), 1,
+               r * n, 0.0, items[i].Pall, 1, r1);
+        continue;
+      }
+      for (size_t tt = 0; tt < items[i].terms.size(); ++tt) {
+        const RandTerm& t = items[i].terms[tt];
         const int64_t R1 = t.rc(k + 1);
-        gb.prob(t.Pb, 1, r1, int(r1), int(R1), 1.0, 0.0);
-        gb.seg(outs[i]->cores[k], r * n, 1, Ycur(t), 1, r * n, int(r * n), k == 0 ? t.alpha : 1.0);
+        gb.prob(Pat(i, tt, k), 1, r1, int(r1), int(R1), 1.0, 0.0);
+        gb.seg(outs[i]->cores[k], r * n, 1, Ycur(i, tt), 1, r * n, int(r * n), k == 0 ? t.alpha : 1.0);
       }
     }
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "rr_P");
@@ -1191,16 +1220,18 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
     std::vector<char> has_op(items.size(), 0);
     for (size_t i = 0; i < items.size(); ++i) {
       const int64_t r1 = orr[i][k + 1];
-      for (auto& t : items[i].terms) {
+      for (size_t tt = 0; tt < items[i].terms.size(); ++tt) {
+        const RandTerm& t = items[i].terms[tt];
         if (!t.H) continue;
         has_op[i] = 1;
         const int64_t ry1 = t.x->rks[k + 1], ry2 = t.x->rks[k + 2], Hl = t.H->rks[k + 1];
+        const double* Pb = Pat(i, tt, k);
         for (int64_t b = 0; b < Hl; ++b)
-          gb.add(int(r1), int(n1 * ry2), int(ry1), 1.0, t.Pb + r1 * ry1 * b, 1, r1, t.x->cores[k + 1], 1, ry1, 0.0,
+          gb.add(int(r1), int(n1 * ry2), int(ry1), 1.0, Pb + r1 * ry1 * b, 1, r1, t.x->cores[k + 1], 1, ry1, 0.0,
                  t.U + r1 * n1 * ry2 * b, 1, r1);
         OpApply a = op_job(t.H, k + 1, false);
         a.in = t.U;
-        a.out = last ? outs[i]->cores[k + 1] : t.Y[yb ^ 1];
+        a.out = last ? outs[i]->cores[k + 1] : Yat(i, tt, k + 1, yb ^ 1);
         a.X = int(r1);
         a.Y = int(ry2);
         a.is_m = r1; a.is_b = r1 * n1 * ry2; a.is_x = 1; a.is_y = r1 * n1;
@@ -1214,19 +1245,22 @@ std::vector<std::unique_ptr<DevTT>> round_rand_batch(ttr_handle_s* h, std::vecto
     for (size_t i = 0; i < items.size(); ++i) {
       const int64_t r1 = orr[i][k + 1];
       if (!last) {
-        for (auto& t : items[i].terms) {
+        for (size_t tt = 0; tt < items[i].terms.size(); ++tt) {
+          const RandTerm& t = items[i].terms[tt];
           if (t.H) continue;
           const int64_t R1 = t.x->rks[k + 1], R2 = t.x->rks[k + 2];
-          gb.add(int(r1), int(n1 * R2), int(R1), 1.0, t.Pb, 1, r1, t.x->cores[k + 1], 1, R1, 0.0, t.Y[yb ^ 1], 1, r1);
+          gb.add(int(r1), int(n1 * R2), int(R1), 1.0, Pat(i, tt, k), 1, r1, t.x->cores[k + 1], 1, R1, 0.0,
+                 Yat(i, tt, k + 1, yb ^ 1), 1, r1);
         }
       } else {
         bool any = false;
-        for (auto& t : items[i].terms) {
+        for (size_t tt = 0; tt < items[i].terms.size(); ++tt) {
+          const RandTerm& t = items[i].terms[tt];
           if (t.H) continue;
           if (!any) gb.prob(outs[i]->cores[k + 1], 1, r1, int(r1), int(n1), 1.0, has_op[i] ? 1.0 : 0.0);
           any = true;
           const int64_t R1 = t.x->rks[k + 1];
-          gb.seg(t.Pb, 1, r1, t.x->cores[k + 1], 1, R1, int(R1), 1.0);
+          gb.seg(Pat(i, tt, k), 1, r1, t.x->cores[k + 1], 1, R1, int(R1), 1.0);
         }
       }
     }
