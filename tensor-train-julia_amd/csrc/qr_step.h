@@ -17,8 +17,6 @@ struct PanelShared16 {
   double Ts[SNB * LDV];         // T of the current panel, Ts[r * LDV + c] = T(r, c)
   double red[2][NT / 64][SNB];  // per-wave column sums (double-buffered by step parity)
   double rowj[2][SNB];          // the pivot row of the panel before the step
-  double dd[SNB];               // v^T P(:, c) of the step
-  double sc[3];                 // tau, beta, 1 / (alpha - beta) of the step
 };
 
 // Cross-lane moves of a double on the VALU (no LDS crossbar round trip): DPP row and
@@ -34,6 +32,11 @@ constexpr int DPP_XOR2 = 0x4E;         // quad_perm [2, 3, 0, 1]
 constexpr int DPP_XOR3 = 0x1B;         // quad_perm [3, 2, 1, 0]
 constexpr int DPP_HALF_MIRROR = 0x141; // lane l of each 8 <- lane 7 - l, i.e. l ^ 7
 constexpr int DPP_ROR8 = 0x128;        // row_ror:8, i.e. l ^ 8 within rows of 16
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
 
 // Lanes 0-31: x_l + x_{l+32}; lanes 32-63: y_{l-32} + y_l (v_permlane32_swap).
 __device__ __forceinline__ double swap32_add(double x, double y) {
@@ -79,19 +82,23 @@ __device__ __forceinline__ double butterfly16(const double (&acc)[SNB], int lane
 
 // Householder step JJ of a 16-column panel held in registers: threads tid < NT
 // hold panel rows tid + NT q (R of them); threads past NT hold no rows and only
-// meet the barriers.  One workgroup reduction per column: the 16 sums
+// meet the barrier.  One workgroup barrier per column: the 16 sums
 // s_c = x^T P(:, c) of the column below the pivot (x) against every panel column
-// are reduced together (butterfly reduce-scatter, then one LDS pass), the
-// reflector's squared norm being s_JJ; with the pivot row broadcast through LDS
-// every thread then has v^T P(:, c) for all c, which both applies the reflector
-// (c > JJ) and gives the new column of T (c < JJ).  The steps are instantiated
-// per column so that every register of the panel is addressed statically (a
-// run-time column loop with selects measured slower).
+// are reduced within each wave (butterfly reduce-scatter) and written to LDS;
+// after the barrier every wave adds the waves' partial sums itself (two values per
+// lane and two lane swaps, the same operations in every wave, so every wave has
+// the same bits), forms the reflector's scalars (the squared norm below the pivot
+// is s_JJ) and d_c = v^T P(:, c) over rows >= JJ (v_JJ = 1) in lane c, and reads
+// the 16 values into scalar registers: for c > JJ they apply the reflector to the
+// remaining columns, for c < JJ they are the entries V^T v of T.  The steps are
+// instantiated per column so that every register of the panel is addressed
+// statically (a run-time column loop with selects measured slower).
 template <int R, int NT, int JJ>
 struct PanelStep16 {
   __device__ static __forceinline__ void run(double (&P)[R][SNB], PanelShared16<NT>& sh, double* tau_out, int nb,
                                              int m2, int tid, int lane, int wave) {
     constexpr int NW = NT / 64;
+    static_assert(NW == 4 || NW == 8, "four or eight waves hold the panel");
     if (JJ < nb) {
       constexpr int buf = JJ & 1;
       const bool pt = tid < NT;
@@ -117,45 +124,33 @@ struct PanelStep16 {
         for (int c = 0; c < SNB; ++c) sh.rowj[buf][c] = P[0][c];
       }
       __syncthreads();
-      // Threads c < 16 total column c over the waves and form d_c = v^T P(:, c)
-      // over rows >= JJ (v_JJ = 1): for c > JJ the reflector's product with the
-      // remaining columns, for c < JJ the entries V^T v of T.  A second barrier
-      // then hands every thread the 16 values and the reflector's scalars (each
-      // thread summing all waves itself would keep 16 x waves values live).
-      if (tid < SNB) {
-        double v = 0.0, sg = 0.0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-          v += sh.red[buf][w][tid];
-          sg += sh.red[buf][w][JJ];
-        }
-        const double al = sh.rowj[buf][JJ];
-        double t = 0.0, be = al, sl = 1.0;
-        if (sg != 0.0) {
-          be = -copysign(sqrt(al * al + sg), al);
-          t = (be - al) / be;
-          sl = 1.0 / (al - be);
-        }
-        sh.dd[tid] = sh.rowj[buf][tid] + sl * v;
-        if (tid == 0) {
-          sh.sc[0] = t;
-          sh.sc[1] = be;
-          sh.sc[2] = sl;
-        }
+      // lane l holds column c = l % 16 of waves l / 16 (and l / 16 + 4), summed over
+      // the four lane groups: every lane ends with the total of its column
+      const int cl = lane & 15, g = lane >> 4;
+      double v = sh.red[buf][g][cl];
+      if (NW == 8) v += sh.red[buf][g + 4][cl];
+      v = swap32_add(v, v);
+      v = swap16_add(v, v);
+      const double sg = readlane_d(v, JJ);
+      const double al = sh.rowj[buf][JJ];
+      double tau = 0.0, beta = al, scal = 1.0;
+      if (sg != 0.0) {
+        beta = -copysign(sqrt(al * al + sg), al);
+        tau = (beta - al) / beta;
+        scal = 1.0 / (al - beta);
       }
-      __syncthreads();
-      const double tau = sh.sc[0], beta = sh.sc[1], scal = sh.sc[2];
+      const double dl = sh.rowj[buf][cl] + scal * v;
       double dd[SNB];
 #pragma unroll
-      for (int c = 0; c < SNB; ++c) dd[c] = sh.dd[c];
+      for (int c = 0; c < SNB; ++c) dd[c] = readlane_d(dl, c);
 #pragma unroll
       for (int q = 0; q < R; ++q) {
         const int i = tid + NT * q;
         if (pt && i > JJ && i < m2) {
-          const double v = P[q][JJ] * scal;
-          P[q][JJ] = v;
+          const double x = P[q][JJ] * scal;
+          P[q][JJ] = x;
 #pragma unroll
-          for (int c = JJ + 1; c < SNB; ++c) P[q][c] -= tau * v * dd[c];
+          for (int c = JJ + 1; c < SNB; ++c) P[q][c] -= tau * x * dd[c];
         }
       }
       if (tid == JJ) {
@@ -166,11 +161,11 @@ struct PanelStep16 {
       if (tid == 0) tau_out[JJ] = tau;
       // T(0:JJ, JJ) = -tau T(0:JJ, 0:JJ) d(0:JJ),  T(JJ, JJ) = tau
       if (tid < JJ) {
-        double v = 0.0;
+        double t = 0.0;
 #pragma unroll
         for (int qq = 0; qq < JJ; ++qq)
-          if (qq >= tid) v += sh.Ts[tid * LDV + qq] * dd[qq];
-        sh.Ts[tid * LDV + JJ] = -tau * v;
+          if (qq >= tid) t += sh.Ts[tid * LDV + qq] * dd[qq];
+        sh.Ts[tid * LDV + JJ] = -tau * t;
       }
       if (tid == JJ) sh.Ts[JJ * LDV + JJ] = tau;
     }
