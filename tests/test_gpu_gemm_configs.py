@@ -67,16 +67,20 @@ def test_gemm_configs_bitwise(monkeypatch):
     fixed.close()
 
 
+@pytest.mark.parametrize("qr_path", ["0", "2"])
 @pytest.mark.parametrize("cfg", range(1, 8))
-def test_each_gemm_config_bitwise(monkeypatch, cfg):
+def test_each_gemm_config_bitwise(monkeypatch, cfg, qr_path):
     """TTR_GEMM_CFG forces one tile configuration for every GEMM of a handle
-    (split-K batches and the unit-trapezoidal operand modes of the Householder
-    updates included): each must give the bits of configuration 0."""
+    (split-K batches included): each must give the bits of configuration 0.
+    TTR_QR_PATH=2 routes the Householder factorisations through the streaming
+    panels, whose in-block updates read V and V^T through the unit-trapezoidal
+    operand modes; the default path's paired blocks read an explicit V."""
     import ttround as T
     from ttround import _lib as L
 
-    say(f"case: GEMM tile configuration {cfg} forced vs configuration 0")
+    say(f"case: GEMM tile configuration {cfg} forced vs configuration 0, QR path {qr_path}")
     runs = {}
+    monkeypatch.setenv("TTR_QR_PATH", qr_path)
     for c in (0, cfg):
         monkeypatch.setenv("TTR_GEMM_CFG", str(c))
         h = L.Handle(0)
@@ -84,4 +88,5 @@ def test_each_gemm_config_bitwise(monkeypatch, cfg):
         runs[c] = _workload(T)
         h.close()
     monkeypatch.delenv("TTR_GEMM_CFG")
+    monkeypatch.delenv("TTR_QR_PATH")
     _same(runs[0], runs[cfg], f"configuration {cfg} vs 0")
