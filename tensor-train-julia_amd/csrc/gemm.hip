@@ -285,17 +285,24 @@ __global__ __launch_bounds__(256) void gemm_splitk_reduce(const GemmProb* __rest
 }
 
 // ------------------------------------------------------------- BF16 math
-// LDS holds A as [m][k] and B as [n][k] (k contiguous) so each lane reads its
-// 8 consecutive k values of the 16x16x32 operand as one 16-byte vector.
+// Tile 64 x 64, k-step 32, 256 threads as 2 x 2 waves of 32 x 32 (2 x 2 MFMA
+// 16x16x32 outputs).  LDS holds A as [m][k] and B as [n][k] (k contiguous) so each
+// lane reads its 8 consecutive k values of an MFMA operand as one 16-byte vector;
+// values are rounded to BF16 when they are stored to LDS (the FP64 core operand
+// of the first sketch contraction is read as FP64).  Pipeline as the scalar
+// kernel: two LDS buffers, the raw loads of k-step i + 1 in flight during the
+// MFMAs of k-step i, one barrier per k-step, unpredicated loads for tiles inside
+// the operands, die-contiguous tile order.  K is never split for BF16.
 template <class TA, class TB, class TC>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmProb* __restrict__ probs, int nprobs,
                                                         const GemmSeg* __restrict__ segs) {
   constexpr int BK = 32;
-  constexpr int KP = BK + 8;  // padded row (elements)
-  __shared__ __attribute__((aligned(16))) uint16_t As[BM][KP];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[BN][KP];
+  constexpr int KP = BK + 8;                              // padded row (elements)
+  constexpr int NA = BM * BK / 256, NB = BN * BK / 256;  // tile elements per thread
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][BM][KP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN][KP];
 
-  const int bid = blockIdx.x;
+  const int bid = xcd_tile(blockIdx.x, gridDim.x);
   const GemmProb P = probs[find_prob(probs, nprobs, bid)];
   const int t = bid - P.tile0;
   const int row0 = (t / P.tiles_n) * BM, col0 = (t % P.tiles_n) * BN;
@@ -303,57 +310,100 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmProb* __restri
   const int wm = wave & 1, wn = wave >> 1;
 
   f4_t acc[2][2];
+#pragma unroll
   for (int a = 0; a < 2; ++a)
+#pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = f4_t{0, 0, 0, 0};
 
   int emax = -(1 << 30);
   for (int s = 0; s < P.nseg; ++s)
     if (segs[P.seg0 + s].exp) emax = max(emax, segs[P.seg0 + s].exp[0]);
 
+  TA ra[NA];
+  TB rb[NB];
+  int buf = 0;
+  bool started = false;
   for (int s = 0; s < P.nseg; ++s) {
     const GemmSeg S = segs[P.seg0 + s];
+    if (S.K <= 0) continue;
     double scale = S.scale;
     if (S.exp) scale = ldexp(scale, S.exp[0] - emax);
-    const bool a_row_fast = (S.a_rs == 1);
-    const bool b_row_fast = (S.b_rs == 1);
-    for (int k0 = 0; k0 < S.K; k0 += BK) {
+    // element q of the thread's share: A (i, l) = (ai0 + q adi, al0 + q adl),
+    // B (l, j) = (bl0 + q bdl, bj0 + q bdj), the fast index along memory
+    int ai0, al0, adi, adl, bl0, bj0, bdl, bdj;
+    if (S.a_rs == 1) { ai0 = tid % BM; al0 = tid / BM; adi = 0; adl = 256 / BM; }
+    else { al0 = tid % BK; ai0 = tid / BK; adl = 0; adi = 256 / BK; }
+    if (S.b_rs == 1) { bl0 = tid % BK; bj0 = tid / BK; bdl = 0; bdj = 256 / BK; }
+    else { bj0 = tid % BN; bl0 = tid / BN; bdj = 0; bdl = 256 / BN; }
+    const int64_t aoff = int64_t(row0 + ai0) * S.a_rs + int64_t(al0) * S.a_cs;
+    const int64_t adq = int64_t(adi) * S.a_rs + int64_t(adl) * S.a_cs;
+    const int64_t boff = int64_t(bl0) * S.b_rs + int64_t(col0 + bj0) * S.b_cs;
+    const int64_t bdq = int64_t(bdl) * S.b_rs + int64_t(bdj) * S.b_cs;
+    const bool a_in = row0 + BM <= P.M && S.a_mode == AM_PLAIN, b_in = col0 + BN <= P.N;
+    auto load_tiles = [&](int k0) {
+      const bool k_full = k0 + BK <= S.K;
+      const int64_t ao = aoff + int64_t(k0) * S.a_cs, bo = boff + int64_t(k0) * S.b_rs;
+      if (a_in && k_full) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        int i, l;
-        if (a_row_fast) { i = tid & 63; l = (tid >> 6) + 4 * q; }
-        else { i = (tid >> 5) + 8 * q; l = tid & 31; }
-        const int64_t gi = row0 + i, gl = k0 + l;
-        double v = 0.0;
-        if (gi < P.M && gl < S.K) {
-          v = (S.a_mode == AM_PLAIN || (S.a_mode == AM_VT_UNIT ? gl > gi : gi > gl))
-                  ? load_d<TA>(S.A, gi * S.a_rs + gl * S.a_cs) : 0.0;
-          v = amode_value(S.a_mode, gi, gl, v) * scale;
+        for (int q = 0; q < NA; ++q) ra[q] = ldg<TA>(S.A, ao + q * adq);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+          const int gi = row0 + ai0 + q * adi, gl = k0 + al0 + q * adl;
+          const bool in = gi < P.M && gl < S.K &&
+                          (S.a_mode == AM_PLAIN || (S.a_mode == AM_VT_UNIT ? gl > gi : gi > gl));
+          ra[q] = TA{};
+          if (in) ra[q] = ldg<TA>(S.A, ao + q * adq);
         }
-        As[i][l] = f32_to_bf16(float(v));
+      }
+      if (b_in && k_full) {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) rb[q] = ldg<TB>(S.B, bo + q * bdq);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+          const int gl = k0 + bl0 + q * bdl, gj = col0 + bj0 + q * bdj;
+          rb[q] = TB{};
+          if (gl < S.K && gj < P.N) rb[q] = ldg<TB>(S.B, bo + q * bdq);
+        }
+      }
+    };
+    auto store_tiles = [&](int b, int k0) {
+#pragma unroll
+      for (int q = 0; q < NA; ++q) {
+        const int i = ai0 + q * adi, l = al0 + q * adl;
+        double v = to_double(ra[q]);
+        if (S.a_mode != AM_PLAIN && row0 + i == k0 + l && row0 + i < P.M && k0 + l < S.K) v = 1.0;
+        As[b][i][l] = f32_to_bf16(float(v * scale));
       }
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        int l, j;
-        if (b_row_fast) { l = tid & 31; j = (tid >> 5) + 8 * q; }
-        else { j = tid & 63; l = (tid >> 6) + 4 * q; }
-        const int64_t gl = k0 + l, gj = col0 + j;
-        double v = 0.0;
-        if (gl < S.K && gj < P.N) v = load_d<TB>(S.B, gl * S.b_rs + gj * S.b_cs);
-        Bs[j][l] = f32_to_bf16(float(v));
-      }
-      __syncthreads();
+      for (int q = 0; q < NB; ++q) Bs[b][bj0 + q * bdj][bl0 + q * bdl] = f32_to_bf16(float(to_double(rb[q])));
+    };
+    if (started) buf ^= 1;
+    started = true;
+    load_tiles(0);
+    store_tiles(buf, 0);
+    __syncthreads();
+    for (int k0 = 0; k0 < S.K; k0 += BK) {
+      const bool more = k0 + BK < S.K;
+      if (more) load_tiles(k0 + BK);  // in flight during this k-step's MFMAs
       const int kq = 8 * (lane >> 4);
-      bf8_t a0 = *reinterpret_cast<const bf8_t*>(&As[wm * 32 + (lane & 15)][kq]);
-      bf8_t a1 = *reinterpret_cast<const bf8_t*>(&As[wm * 32 + 16 + (lane & 15)][kq]);
-      bf8_t b0 = *reinterpret_cast<const bf8_t*>(&Bs[wn * 32 + (lane & 15)][kq]);
-      bf8_t b1 = *reinterpret_cast<const bf8_t*>(&Bs[wn * 32 + 16 + (lane & 15)][kq]);
+      const bf8_t a0 = *reinterpret_cast<const bf8_t*>(&As[buf][wm * 32 + (lane & 15)][kq]);
+      const bf8_t a1 = *reinterpret_cast<const bf8_t*>(&As[buf][wm * 32 + 16 + (lane & 15)][kq]);
+      const bf8_t b0 = *reinterpret_cast<const bf8_t*>(&Bs[buf][wn * 32 + (lane & 15)][kq]);
+      const bf8_t b1 = *reinterpret_cast<const bf8_t*>(&Bs[buf][wn * 32 + 16 + (lane & 15)][kq]);
       acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
       acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
       acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
       acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
-      __syncthreads();
+      if (more) {
+        store_tiles(buf ^ 1, k0 + BK);
+        __syncthreads();
+        buf ^= 1;
+      }
     }
   }
+  // register r of an f32 16x16 MFMA result holds row 4 (lane / 16) + r, column lane % 16
 #pragma unroll
   for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
