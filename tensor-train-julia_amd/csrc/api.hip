@@ -295,6 +295,7 @@ std::vector<std::unique_ptr<DevTT>> orthogonalize_batch(ttr_handle_s* h, const s
   }
   // workspace: FR / FL factors and a working matrix per item
   std::vector<double*> FR(nb), FL(nb), Abuf(nb), Qw(nb);
+  std::vector<Fill2D> fz;
   for (size_t q = 0; q < nb; ++q) {
     const DevTT* x = xs[q];
     int64_t mx_a = 1, mx_f = 1;
@@ -306,9 +307,10 @@ std::vector<std::unique_ptr<DevTT>> orthogonalize_batch(ttr_handle_s* h, const s
     FL[q] = ctx.ws.alloc_n<double>(size_t(mx_f));
     Abuf[q] = ctx.ws.alloc_n<double>(size_t(mx_a));
     Qw[q] = ctx.ws.alloc_n<double>(size_t(mx_a));
-    zero_fill(ctx, FR[q], 8);
-    zero_fill(ctx, FL[q], 8);
+    fz.push_back(Fill2D{FR[q], 1, 1, 1, 0.0});
+    fz.push_back(Fill2D{FL[q], 1, 1, 1, 0.0});
   }
+  fill2d_batched(ctx, fz);
   GemmBatch gb;
   // ---- left-to-right (cores 0..root-2): A = FR X_j, QR, core = Q, FR = R
   for (int j = 0; j < root - 1; ++j) {
@@ -938,11 +940,16 @@ void sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<RandItem>& items,
   const Prec wp = sp.wprec;
   // buffers
   std::vector<std::vector<void*>> T1(items.size()), T2(items.size());
+  // every term's sketch exponents in one array, cleared by one launch
+  size_t nterm = 0;
+  for (auto& it : items) nterm += it.terms.size();
+  int32_t* E_all = ctx.ws.alloc_n<int32_t>(nterm * size_t(d + 1));
+  zero_fill(ctx, E_all, nterm * size_t(d + 1) * 4);
+  size_t term_idx = 0;
   for (size_t i = 0; i < items.size(); ++i) {
     for (auto& t : items[i].terms) {
       t.W.assign(d + 1, nullptr);
-      t.E = ctx.ws.alloc_n<int32_t>(size_t(d + 1));
-      zero_fill(ctx, t.E, size_t(d + 1) * 4);
+      t.E = E_all + (term_idx++) * size_t(d + 1);
       int64_t mx = 1, mx2 = 1;
       for (int k = kmin; k < d; ++k) {
         t.W[k] = ctx.ws.alloc(size_t(t.rc(k) * items[i].s[k]) * eb);
@@ -2689,11 +2696,12 @@ void tuple_sketch_phase(ttr_handle_s* h, SketchPlan& sp, std::vector<TupleItem>&
   std::vector<void*> T1(its.size());
   std::vector<std::array<void*, 2>> U(its.size());
   std::vector<double*> last(its.size());
+  int32_t* E_all = ctx.ws.alloc_n<int32_t>(its.size() * size_t(d + 1));
+  zero_fill(ctx, E_all, its.size() * size_t(d + 1) * 4);
   for (size_t q = 0; q < its.size(); ++q) {
     TupleItem& it = its[q];
     it.W.assign(size_t(d + 1), nullptr);
-    it.E = ctx.ws.alloc_n<int32_t>(size_t(d + 1));
-    zero_fill(ctx, it.E, size_t(d + 1) * 4);
+    it.E = E_all + q * size_t(d + 1);
     int64_t t1 = 1, u = 1;
     for (int k = kmin; k < d; ++k) {
       it.W[size_t(k)] = ctx.ws.alloc(size_t(it.P(k) * it.s[size_t(k)]) * eb);
