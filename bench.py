@@ -288,6 +288,15 @@ def summarize(stats, args):
         st = stats.pop("jacobi_svd:" + ph, None)
         if st and jacobi:
             jacobi[ph] = st["flops"]
+    # executed MFMA work of the Jacobi launches (Gram + apply of the visits made) next to
+    # the algorithmic work of the same SVDs (21 m^3 each, Golub-Kahan-Reinsch count)
+    alg = stats.pop("jacobi_svd:alg_flops", None)
+    if jacobi and "jacobi_svd" in stats:
+        js = stats["jacobi_svd"]
+        jacobi["executed_gflop_per_step"] = round(js["flops"] / args.steps / 1e9, 3)
+        if alg:
+            jacobi["algorithmic_gflop_per_step"] = round(alg["flops"] / args.steps / 1e9, 3)
+            jacobi["algorithmic_tflops"] = round(alg["flops"] / max(js["ms"], 1e-9) / 1e9, 3)
     sites = {k[5:]: {"ms_per_step": round(v["ms"] / args.steps, 3), "launches_per_step": v["launches"] // args.steps,
                      "tflops": round(v["flops"] / max(v["ms"], 1e-9) / 1e9, 3)}
              for k, v in stats.items() if k.startswith("site:")}

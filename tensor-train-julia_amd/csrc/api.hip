@@ -601,6 +601,9 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
         ctx.prof.stats["jacobi_svd"].flops +=
             2.0 * m * (256.0 * gram_tiles * double(info[4]) + 2.0 * wc * wc * info[5]);
         ctx.prof.stats["jacobi_svd"].bytes += 8.0 * wc * m * (double(info[4]) + 4.0 * info[5]);
+        // algorithmic work of the same SVD: 21 m^3, the Golub-Kahan-Reinsch count for
+        // Sigma, U and V of a square matrix (Golub and Van Loan, Matrix Computations)
+        ctx.prof.stats["jacobi_svd:alg_flops"].flops += 21.0 * double(m) * m * m;
         ctx.prof.stats["jacobi_svd:ticks_gram"].flops += info[1];
         ctx.prof.stats["jacobi_svd:ticks_inner"].flops += info[2];
         ctx.prof.stats["jacobi_svd:ticks_apply"].flops += info[3];
