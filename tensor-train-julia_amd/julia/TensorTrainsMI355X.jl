@@ -55,6 +55,15 @@ function handle()
     return _handle[]
 end
 
+"""
+    trim()
+
+Return the handle's idle device workspace and its cached TT storage to the runtime
+(`ttr_trim`); live results stay valid.  For callers that move between workloads of very
+different sizes.  No counterpart in `TensorTrains` (Julia's GC owns its buffers).
+"""
+trim() = check(ccall((:ttr_trim, lib), Cint, (Ptr{Cvoid},), handle()))
+
 function check(st)
     st == TTR_OK && return
     msg = unsafe_string(ccall((:ttr_last_error, lib), Cstring, (Ptr{Cvoid},), handle()))
