@@ -242,6 +242,53 @@ def parity_and_cpu(items, specs, op, outs, raw, dims, wl, args):
     return max(errs), (max(errs_raw) if errs_raw else None), cpu, errs
 
 
+def input_errors(items, op, outs, raw, n):
+    """Relative Frobenius error of the first n results against the input they round
+    (sum_j alpha_j y_j, or H y for the operator workload), for the final result and
+    the randomized stage, from inner products on the device (one batched ttr_dot):
+    ||r - x||^2 = <r, r> - 2 sum_j alpha_j <r, y_j> + sum_jl alpha_j alpha_l <y_j, y_l>.
+    The expansion resolves errors down to ~1e-7 relative; the errors against the
+    input here are set by the truncation (1e-2 and above)."""
+    import ttround as T
+    n = min(n, len(items))
+    xs, ys, made = [], [], []
+    for i in range(n):
+        al, terms, _ = items[i]
+        base = [T.mpo_apply_device(op, terms[0])] if op is not None else list(terms)
+        if op is not None:
+            made += base
+        for j in range(len(base)):
+            for l in range(j, len(base)):
+                xs.append(base[j]); ys.append(base[l])
+        for res in (outs, raw):
+            if res is not None:
+                xs += [res[i]] * (len(base) + 1)
+                ys += base + [res[i]]
+    d = T.dot_batch(xs, ys)
+    for m in made:
+        m.free()
+    fin, rnd, p = [], [], 0
+    for i in range(n):
+        al = [1.0] if op is not None else items[i][0]
+        nb = len(al)
+        G = np.zeros((nb, nb))
+        for j in range(nb):
+            for l in range(j, nb):
+                G[j, l] = G[l, j] = d[p]
+                p += 1
+        xx = float(np.dot(al, G @ np.asarray(al)))
+        for res, dst in ((outs, fin), (raw, rnd)):
+            if res is None:
+                continue
+            rx = float(np.dot(al, d[p:p + nb]))
+            rr = float(d[p + nb])
+            p += nb + 1
+            dst.append(math.sqrt(max(rr - 2.0 * rx + xx, 0.0) / xx))
+    log(f"[bench] error against the input on {n} item(s): final {max(fin):.3e}"
+        + (f", randomized stage {max(rnd):.3e}" if rnd else ""))
+    return (max(fin) if fin else None), (max(rnd) if rnd else None)
+
+
 def deterministic(items, op, wl, args):
     """Plain FP64 SVD rounding (tt_rounding with the cleanup's tol/rmax) of the
     materialised sums / products of the first det_items items."""
@@ -481,6 +528,7 @@ def main():
     if args.cpu_items > 0:
         args.cpu_items = min(args.cpu_items, args.items)
         rel_err, rel_err_raw, cpu, errs = parity_and_cpu(items, specs, op, outs, raw, dims, wl, args)
+    err_in, err_in_raw = input_errors(items, op, outs, raw, max(args.cpu_items, 1))
     for o in outs + (raw or []):
         o.free()
     det = deterministic(items, op, wl, args) if args.det_items > 0 else None
@@ -506,6 +554,9 @@ def main():
         "rel_err_vs_oracle": rel_err,
         "rel_err_vs_oracle_randomized_stage": rel_err_raw,
         "rel_err_vs_oracle_items": None if errs is None else [float(f"{e:.3e}") for e in errs],
+        # approximation quality (largest over the same items): against the input itself
+        "rel_err_vs_input": err_in,
+        "rel_err_vs_input_randomized_stage": err_in_raw,
         "out_rank_max_item0": out_rks,
         "fp64_deterministic": det,
         "speedup_vs_fp64_deterministic": (round(value / det["rounded_tts_per_s"], 3) if det else None),
