@@ -1974,6 +1974,12 @@ int ttr_create(ttr_handle_t* out, int device) {
       TTR_REQUIRE(v >= 0 && v <= 3, ST_INVALID_ARG, "TTR_QR_PATH must be 0..3");
       h->ctx.qr_path = v;
     }
+    // TTR_QR_GROUP=g (1, 2 or 4): 64-column blocks per compact WY group of the blocked QR
+    if (const char* c = std::getenv("TTR_QR_GROUP")) {
+      const int v = std::atoi(c);
+      TTR_REQUIRE(v == 1 || v == 2 || v == 4, ST_INVALID_ARG, "TTR_QR_GROUP must be 1, 2 or 4");
+      h->ctx.qr_group = v;
+    }
     // TTR_JACOBI_MAX_SWEEPS=s (1..40): Jacobi sweeps before a non-converged SVD fails
     if (const char* c = std::getenv("TTR_JACOBI_MAX_SWEEPS")) {
       const int v = std::atoi(c);

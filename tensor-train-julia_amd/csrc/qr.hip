@@ -349,7 +349,7 @@ struct TMergeJob {
   double* T;         // 64 x 64, ld ldt
   int32_t nb, w;     // columns of the block, panel width (16 or 32)
   int32_t ldg, ldt;
-  int32_t zero_left; // 1: also zero the 64 x 64 block left of T (the lower-left block of a pair's T)
+  int32_t zero_left; // also zero this many 64-column blocks left of T (the rows of a group's T below the diagonal)
 };
 
 __global__ __launch_bounds__(256) void merge_t_kernel(const TMergeJob* __restrict__ jobs) {
@@ -385,8 +385,8 @@ __global__ __launch_bounds__(256) void merge_t_kernel(const TMergeJob* __restric
     __syncthreads();
   }
   for (int e = t; e < NBO * NBO; e += 256) J.T[e % NBO + int64_t(J.ldt) * (e / NBO)] = Ts[e % NBO][e / NBO];
-  if (J.zero_left)
-    for (int e = t; e < NBO * NBO; e += 256) J.T[e % NBO + int64_t(J.ldt) * (e / NBO - NBO)] = 0.0;
+  for (int e = t; e < J.zero_left * NBO * NBO; e += 256)
+    J.T[e % NBO + int64_t(J.ldt) * (e / NBO - J.zero_left * NBO)] = 0.0;
 }
 
 void launch_panel(Ctx& ctx, const PanelJob* d, int njobs) {
@@ -409,41 +409,42 @@ void launch_block(Ctx& ctx, const BlockJob* d, int njobs, int max_rows) {
 }
 
 // Blocked QR of matrices up to 6 * RT rows with the 64-column blocks taken in
-// pairs: block 2s is factored (its panels and in-block updates in one launch),
-// applied to block 2s + 1 only, block 2s + 1 is factored, and the pair's 128
-// reflectors, as one compact WY factor (V_s explicit, T_s = [T_0, T_01; 0, T_1]
-// with T_01 = -T_0 (V_0^T V_1) T_1), update the trailing columns and later form
-// Q.  The trailing updates and the formation of Q then multiply with K = 128,
-// so they pass over the trailing matrix half as often as with 64-column blocks.
-void qr_pairs(Ctx& ctx, std::vector<QRJob>& jobs) {
-  constexpr int NS = 2 * NBO;  // columns of a pair
+// groups of G: block h of group s is updated by the group's first h blocks,
+// factored (its panels and in-block updates in one launch, its explicit V written
+// into the group's V with zero rows above it), and the group's compact WY factor
+// grows by it: T_s = [T_acc, -T_acc (V_acc^T V_h) T_h; 0, T_h].  The group's 64 G
+// reflectors then update the trailing columns and later form Q with K = 64 G, so
+// those products pass over the trailing matrix once per 64 G columns (the update
+// A2 -= V W2 measured 18.6, 30.2 and 42.1 TF/s at K = 64, 128 and 256).  G = 2 by
+// default (Ctx::qr_group).
+void qr_groups(Ctx& ctx, std::vector<QRJob>& jobs, int G) {
+  const int NS = NBO * G;  // columns of a group
   struct Aux {
     double *tau, *T, *TS, *G, *Y, *W, *W2, *VX;
     bool vx_all;
-    int kq, nblock, npair;
+    int kq, ngrp;
   };
   std::vector<Aux> aux(jobs.size());
-  int max_pairs = 0;
+  int max_grp = 0;
   for (size_t q = 0; q < jobs.size(); ++q) {
     const QRJob& J = jobs[q];
     TTR_REQUIRE(J.m >= 0 && J.n >= 0 && J.lda >= std::max(1, J.m), ST_INVALID_ARG, "qr: bad shape");
     Aux& a = aux[q];
     a.kq = std::min(J.m, J.n);
-    a.nblock = (a.kq + NBO - 1) / NBO;
-    a.npair = (a.kq + NS - 1) / NS;
-    max_pairs = std::max(max_pairs, a.npair);
+    a.ngrp = (a.kq + NS - 1) / NS;
+    max_grp = std::max(max_grp, a.ngrp);
     const int npanel = (a.kq + NBR - 1) / NBR;
     a.tau = ctx.ws.alloc_n<double>(std::max(1, J.n));
     a.T = ctx.ws.alloc_n<double>(size_t(std::max(1, npanel)) * NBP * NBP);
-    a.TS = ctx.ws.alloc_n<double>(size_t(std::max(1, a.npair)) * NS * NS);
+    a.TS = ctx.ws.alloc_n<double>(size_t(std::max(1, a.ngrp)) * NS * NS);
     a.G = ctx.ws.alloc_n<double>(size_t(NS) * NBO);
-    a.Y = ctx.ws.alloc_n<double>(size_t(NBO) * NBO);
+    a.Y = ctx.ws.alloc_n<double>(size_t(NS) * NBO);
     a.W = ctx.ws.alloc_n<double>(size_t(NS) * std::max(1, J.n));
     a.W2 = ctx.ws.alloc_n<double>(size_t(NS) * std::max(1, J.n));
-    // pair s: (m - 128 s) x 128, ld m - 128 s; all pairs kept when Q is formed
+    // group s: (m - NS s) x NS, ld m - NS s; all groups kept when Q is formed
     a.vx_all = J.Q != nullptr;
     a.VX = ctx.ws.alloc_n<double>(
-        a.vx_all ? size_t(NS) * (size_t(a.npair) * J.m - size_t(NS / 2) * a.npair * std::max(0, a.npair - 1))
+        a.vx_all ? size_t(NS) * (size_t(a.ngrp) * J.m - size_t(NS / 2) * a.ngrp * std::max(0, a.ngrp - 1))
                  : size_t(std::max(1, J.m)) * NS);
   }
   auto blk_cols = [&](const Aux& a, int b) { return std::max(0, std::min(NBO, a.kq - b * NBO)); };
@@ -451,8 +452,9 @@ void qr_pairs(Ctx& ctx, std::vector<QRJob>& jobs) {
     return a.VX + (a.vx_all ? int64_t(NS) * (int64_t(s) * J.m - int64_t(NS / 2) * s * (s - 1)) : 0);
   };
   auto ts = [&](const Aux& a, int s) { return a.TS + size_t(s) * NS * NS; };
+  auto grp_cols = [&](const Aux& a, int s) { return std::max(0, std::min(NS, a.kq - s * NS)); };
   GemmBatch gb;
-  // one 64-column block of every job that has it: panels and in-block updates on chip
+  // block h of group s of every job that has it: panels and in-block updates on chip
   auto factor_block = [&](int s, int h) {
     std::vector<BlockJob> bj;
     double fl = 0.0, by = 0.0;
@@ -460,12 +462,12 @@ void qr_pairs(Ctx& ctx, std::vector<QRJob>& jobs) {
     for (size_t q = 0; q < jobs.size(); ++q) {
       const QRJob& J = jobs[q];
       Aux& a = aux[q];
-      const int b = 2 * s + h, nbo = blk_cols(a, b);
+      const int b = G * s + h, nbo = blk_cols(a, b);
       if (nbo <= 0) continue;
       const int c0 = b * NBO, mb = J.m - c0;
       const int64_t ld = J.m - int64_t(NS) * s;
-      bj.push_back(BlockJob{J.A, J.lda, J.m, c0, nbo, a.tau, a.T, vxs(J, a, s) + h * (NBO + NBO * ld), ld,
-                            h ? NBO : 0});
+      bj.push_back(BlockJob{J.A, J.lda, J.m, c0, nbo, a.tau, a.T, vxs(J, a, s) + int64_t(h) * NBO * (1 + ld), ld,
+                            h * NBO});
       max_rows = std::max(max_rows, mb);
       for (int c = 0; c < nbo; c += NBR) {
         const double m2 = J.m - (c0 + c), nb = std::min(NBR, nbo - c), n2 = nbo - c - nb;
@@ -478,28 +480,14 @@ void qr_pairs(Ctx& ctx, std::vector<QRJob>& jobs) {
     ProfScope ps(ctx, "qr_block", fl, by);
     launch_block(ctx, d, int(bj.size()), max_rows);
   };
-  // T of block h of pair s from its Gram (in G, ld 128) and its panels' T
-  auto merge_block = [&](int s, int h) {
-    std::vector<TMergeJob> tb;
-    for (size_t q = 0; q < jobs.size(); ++q) {
-      const Aux& a = aux[q];
-      const int b = 2 * s + h, nbo = blk_cols(a, b);
-      if (nbo <= 0) continue;
-      tb.push_back(TMergeJob{a.G + h * NBO, a.T + size_t(b * NBO / NBR) * NBP * NBP,
-                             ts(a, s) + h * (NBO + NBO * NS), nbo, NBR, NS, NS, h});
-    }
-    if (tb.empty()) return;
-    const TMergeJob* d = ctx.upload(tb);
-    TTR_LAUNCH(ctx, merge_t_kernel, dim3(unsigned(tb.size())), dim3(256), 0, d);
-  };
   // C -= V (T^T (V^T C)) (trans) or C -= V (T (V^T C)) for every job: V the first
-  // kv columns of pair s's V, C columns [j0, j1) of the job's A or Q from row 128 s
+  // kv columns of group s's V, C columns [j0, j1) of the job's A or Q from row NS s
   auto apply = [&](int s, bool trans, bool on_q, const char* site, auto kv_of, auto cols_of) {
     for (int stage = 0; stage < 3; ++stage) {
       for (size_t q = 0; q < jobs.size(); ++q) {
         const QRJob& J = jobs[q];
         const Aux& a = aux[q];
-        if (s >= a.npair || (on_q && !J.Q)) continue;
+        if (s >= a.ngrp || (on_q && !J.Q)) continue;
         const int kv = kv_of(a, s);
         int j0, j1;
         cols_of(J, a, s, j0, j1);
@@ -519,81 +507,73 @@ void qr_pairs(Ctx& ctx, std::vector<QRJob>& jobs) {
       gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, site);
     }
   };
-  auto pair_cols = [&](const Aux& a, int s) { return blk_cols(a, 2 * s) + blk_cols(a, 2 * s + 1); };
   // ---- factorisation
-  for (int s = 0; s < max_pairs; ++s) {
-    factor_block(s, 0);
-    // G(0:64, 0:64) = V_0^T V_0, T_0
-    for (size_t q = 0; q < jobs.size(); ++q) {
-      const QRJob& J = jobs[q];
-      const Aux& a = aux[q];
-      const int nb0 = blk_cols(a, 2 * s);
-      if (nb0 <= 0) continue;
-      const int64_t ld = J.m - int64_t(NS) * s;
-      const double* V = vxs(J, a, s);
-      gb.add(nb0, nb0, int(ld), 1.0, V, ld, 1, V, 1, ld, 0.0, a.G, 1, NS);
-    }
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
-    merge_block(s, 0);
-    // block 0 applied to block 1's columns
-    apply(s, true, false, "qr_pair", [&](const Aux& a, int ss) { return blk_cols(a, 2 * ss + 1) > 0 ? blk_cols(a, 2 * ss) : 0; },
-          [&](const QRJob&, const Aux& a, int ss, int& j0, int& j1) {
-            j0 = (2 * ss + 1) * NBO;
-            j1 = j0 + blk_cols(a, 2 * ss + 1);
-          });
-    factor_block(s, 1);
-    // G(0:128, 0:64) = V_s^T V_1: rows 0..63 hold V_0^T V_1, rows 64..127 V_1^T V_1
-    for (size_t q = 0; q < jobs.size(); ++q) {
-      const QRJob& J = jobs[q];
-      const Aux& a = aux[q];
-      const int nb1 = blk_cols(a, 2 * s + 1);
-      if (nb1 <= 0) continue;
-      const int64_t ld = J.m - int64_t(NS) * s;
-      const double* V = vxs(J, a, s);
-      gb.add(NBO + nb1, nb1, int(ld), 1.0, V, ld, 1, V + NBO * ld, 1, ld, 0.0, a.G, 1, NS);
-    }
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
-    // T_1 from rows 64.. of G, then T_01 = -T_0 (V_0^T V_1) T_1
-    std::vector<TMergeJob> tb;
-    for (size_t q = 0; q < jobs.size(); ++q) {
-      const Aux& a = aux[q];
-      const int nb1 = blk_cols(a, 2 * s + 1);
-      if (nb1 <= 0) continue;
-      tb.push_back(TMergeJob{a.G + NBO, a.T + size_t((2 * s + 1) * NBO / NBR) * NBP * NBP,
-                             ts(a, s) + NBO + NBO * NS, nb1, NBR, NS, NS, 1});
-    }
-    if (!tb.empty()) {
+  for (int s = 0; s < max_grp; ++s) {
+    for (int h = 0; h < G; ++h) {
+      // the group's first h blocks applied to block h's columns
+      if (h > 0)
+        apply(s, true, false, "qr_group", [&](const Aux& a, int ss) { return blk_cols(a, G * ss + h) > 0 ? h * NBO : 0; },
+              [&](const QRJob&, const Aux& a, int ss, int& j0, int& j1) {
+                j0 = (G * ss + h) * NBO;
+                j1 = j0 + blk_cols(a, G * ss + h);
+              });
+      factor_block(s, h);
+      // G(0:64h+nb, 0:nb) = V_s^T V_h: rows 0..64h-1 hold V_acc^T V_h, the rest V_h^T V_h
+      bool any = false;
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const QRJob& J = jobs[q];
+        const Aux& a = aux[q];
+        const int nb = blk_cols(a, G * s + h);
+        if (nb <= 0) continue;
+        const int64_t ld = J.m - int64_t(NS) * s;
+        const double* V = vxs(J, a, s);
+        gb.add(h * NBO + nb, nb, int(ld), 1.0, V, ld, 1, V + int64_t(h) * NBO * ld, 1, ld, 0.0, a.G, 1, NS);
+        any = true;
+      }
+      if (!any) break;
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
+      // T_h from rows 64h.. of G (and zeros left of it in its rows), then
+      // T(0:64h, 64h:) = -T_acc (V_acc^T V_h) T_h
+      std::vector<TMergeJob> tb;
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const Aux& a = aux[q];
+        const int nb = blk_cols(a, G * s + h);
+        if (nb <= 0) continue;
+        tb.push_back(TMergeJob{a.G + h * NBO, a.T + size_t((G * s + h) * NBO / NBR) * NBP * NBP,
+                               ts(a, s) + int64_t(h) * NBO * (1 + NS), nb, NBR, NS, NS, h});
+      }
       const TMergeJob* d = ctx.upload(tb);
       TTR_LAUNCH(ctx, merge_t_kernel, dim3(unsigned(tb.size())), dim3(256), 0, d);
+      if (h == 0) continue;
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const Aux& a = aux[q];
+        const int nb = blk_cols(a, G * s + h);
+        if (nb <= 0) continue;
+        gb.add(h * NBO, nb, nb, 1.0, a.G, 1, NS, ts(a, s) + int64_t(h) * NBO * (1 + NS), 1, NS, 0.0, a.Y, 1, NS);
+      }
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_tmerge");
+      for (size_t q = 0; q < jobs.size(); ++q) {
+        const Aux& a = aux[q];
+        const int nb = blk_cols(a, G * s + h);
+        if (nb <= 0) continue;
+        gb.add(h * NBO, nb, h * NBO, -1.0, ts(a, s), 1, NS, a.Y, 1, NS, 0.0, ts(a, s) + int64_t(h) * NBO * NS, 1, NS);
+      }
+      gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_tmerge");
     }
-    for (size_t q = 0; q < jobs.size(); ++q) {
-      const Aux& a = aux[q];
-      const int nb1 = blk_cols(a, 2 * s + 1);
-      if (nb1 <= 0) continue;
-      gb.add(NBO, nb1, nb1, 1.0, a.G, 1, NS, ts(a, s) + NBO + NBO * NS, 1, NS, 0.0, a.Y, 1, NBO);
-    }
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_tmerge");
-    for (size_t q = 0; q < jobs.size(); ++q) {
-      const Aux& a = aux[q];
-      const int nb1 = blk_cols(a, 2 * s + 1);
-      if (nb1 <= 0) continue;
-      gb.add(NBO, nb1, NBO, -1.0, ts(a, s), 1, NS, a.Y, 1, NBO, 0.0, ts(a, s) + NBO * NS, 1, NS);
-    }
-    gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_tmerge");
-    // the pair applied to the trailing columns
-    apply(s, true, false, "qr_trail", pair_cols, [&](const QRJob& J, const Aux& a, int ss, int& j0, int& j1) {
-      j0 = NS * ss + pair_cols(a, ss);
+    // the group applied to the trailing columns
+    apply(s, true, false, "qr_trail", grp_cols, [&](const QRJob& J, const Aux& a, int ss, int& j0, int& j1) {
+      j0 = NS * ss + grp_cols(a, ss);
       j1 = J.n;
     });
   }
-  // ---- thin Q by backward accumulation over the pairs: Q2 -= V_s (T_s (V_s^T Q2))
+  // ---- thin Q by backward accumulation over the groups: Q2 -= V_s (T_s (V_s^T Q2))
   std::vector<Fill2D> fills;
   for (size_t q = 0; q < jobs.size(); ++q)
     if (jobs[q].Q && aux[q].kq > 0) fills.push_back(Fill2D{jobs[q].Q, jobs[q].ldq, jobs[q].m, aux[q].kq, 1.0});
   if (fills.empty()) return;
   fill2d_batched(ctx, fills);
-  for (int s = max_pairs - 1; s >= 0; --s)
-    apply(s, false, true, "qr_formq", pair_cols, [&](const QRJob&, const Aux& a, int ss, int& j0, int& j1) {
+  for (int s = max_grp - 1; s >= 0; --s)
+    apply(s, false, true, "qr_formq", grp_cols, [&](const QRJob&, const Aux& a, int ss, int& j0, int& j1) {
       j0 = NS * ss;
       j1 = a.kq;
     });
@@ -622,12 +602,12 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
     }
   }
   {
-    // register-resident blocks (panels of 16) up to 6*512 rows, taken in pairs; else
+    // register-resident blocks (panels of 16) up to 6*512 rows, taken in groups; else
     // 64-column blocks of streaming panels (width 32)
     int max_m = 0;
     for (const auto& J : jobs) max_m = std::max(max_m, J.m);
     if (max_m <= 6 * RT && ctx.qr_path != 2) {
-      qr_pairs(ctx, jobs);
+      qr_groups(ctx, jobs, ctx.qr_group);
       return;
     }
   }

@@ -203,6 +203,11 @@ struct Ctx {
   // blocked path with streaming panels of 32 at every height; 3: the one-launch QR
   // for every batch whose matrices have at most QR_SMALL_MAX_M rows
   int qr_path = 0;
+  // 64-column blocks per compact WY group of the blocked QR (1, 2 or 4).  Measured on
+  // the C3 workload: 2 (2.13 s per step) beats 4 (2.17 s: faster trailing updates, but
+  // the wider group's V carries more zeros into the formation of Q, and the in-group
+  // updates and T merges grow)
+  int qr_group = 2;
   // partial sums of split-K GEMMs, reused by every launch on the stream
   double* splitk = nullptr;
   size_t splitk_elems = 0;
