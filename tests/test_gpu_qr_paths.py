@@ -100,3 +100,35 @@ def test_ttrand_rounding_qr_paths(monkeypatch, path):
     say(f"  ranks {y.rks}, rel err vs oracle {err:.3e}")
     assert y.rks == ref.rks
     assert err < 1e-12
+
+
+@pytest.mark.parametrize("group", ["1", "4"])
+def test_qr_groups_vs_oracle(monkeypatch, group):
+    """TTR_QR_GROUP sets how many 64-column blocks of the blocked QR form one compact
+    WY group (default 2): single blocks and groups of four, with the blocked path forced,
+    round to the oracle's result at 1e-12 (unfoldings of up to 2592 rows and 432
+    columns: seven blocks, a group of four and a partial group of three)."""
+    import ttround as T
+    dims = (6,) * 6
+    x = O.tt_add(rand_tt(dims, 216, 71), O.tt_scale(1e-3, rand_tt(dims, 216, 72)))
+    ys = [rand_tt(dims, r, 130 + j) for j, r in enumerate((150, 40, 64))]
+    al = [1.0, -0.5, 0.25]
+    rks = O.r_and_d_to_rks([216] * 7, dims, rmax=216)
+    say(f"case: QR groups of {group} blocks, tt_rounding ranks {x.rks} and ttrand_rounding sketch ranks {rks}")
+    ref = O.tt_rounding(x, tol=1e-10)
+    prov = device_blocks_provider(dims, rks, orthogonal=False, seed=44, sketch_prec="f64")
+    ref2 = O.ttrand_rounding_lincomb(al, ys, rks, prov, False)
+    monkeypatch.setenv("TTR_QR_GROUP", group)
+    h = handle(monkeypatch, "1")
+    monkeypatch.delenv("TTR_QR_GROUP")
+    try:
+        y = to_oracle(T.tt_rounding(to_api(x), tol=1e-10))
+        y2 = to_oracle(T.ttrand_rounding(al, [to_api(t) for t in ys], rks, orthogonal=False, seed=44,
+                                         sketch_prec="f64"))
+        check_kernel(h, "1")
+    finally:
+        h.close()
+    err, err2 = O.rel_err(ref, y), O.rel_err(ref2, y2)
+    say(f"  ranks {y.rks} / {y2.rks}, rel err vs oracle {err:.3e} / {err2:.3e}")
+    assert y.rks == ref.rks and y2.rks == ref2.rks
+    assert err < 1e-12 and err2 < 1e-12
