@@ -338,7 +338,10 @@ int ttr_sketch_draws(ttr_handle_t h, int64_t k, int64_t count, uint64_t seed, do
  * stream, with the algorithmic flops and bytes of each launch.  Phase names
  * ("phase:reverse_sketch", "phase:orthogonalization", ...) follow the
  * reference's TimerOutputs sections (src/tt_randtools.jl:113-153,
- * src/sketches.jl:176-304).  Statistics accumulate until ttr_profile_reset. */
+ * src/sketches.jl:176-304).  Statistics accumulate until ttr_profile_reset.
+ * on: 0 off, 1 every timed launch, 2 only the GEMM classes (with their call
+ * sites) and the phases, which keeps the host cost of the events off the other
+ * launches. */
 int ttr_profile_enable(ttr_handle_t h, int on);
 int ttr_profile_reset(ttr_handle_t h);
 int ttr_profile_count(ttr_handle_t h, int64_t* count);

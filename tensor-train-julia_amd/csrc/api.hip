@@ -585,7 +585,7 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
         continue;
       }
       const double* sv = sv_host.data() + sv_off[q];
-      if (ctx.prof.on) {
+      if (ctx.prof.on && ctx.prof.level == 1) {
         const uint32_t* info = info_host.data() + 8 * q;
         ProfStat& ps = ctx.prof.stats["jacobi_svd:sweeps"];
         ps.launches += 1;
@@ -3509,7 +3509,11 @@ int ttr_trim(ttr_handle_t h) {
 }
 
 int ttr_profile_enable(ttr_handle_t h, int on) {
-  return guarded(h, [&] { h->ctx.prof.on = on != 0; });
+  return guarded(h, [&] {
+    TTR_REQUIRE(on >= 0 && on <= 2, ST_INVALID_ARG, "profiling level must be 0, 1 or 2");
+    h->ctx.prof.on = on != 0;
+    h->ctx.prof.level = on == 2 ? 2 : 1;
+  });
 }
 
 int ttr_profile_reset(ttr_handle_t h) {

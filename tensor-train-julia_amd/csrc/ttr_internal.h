@@ -147,6 +147,13 @@ class Profiler {
  public:
   ~Profiler();
   bool on = false;
+  // 1: every timed launch; 2: only the GEMM classes and the sweep phases (two
+  // events per launch cost ~3 us of host time each, so a run that needs only the
+  // dominant kernel's times leaves the others out)
+  int level = 1;
+  bool records(const char* name) const {
+    return level == 1 || std::strncmp(name, "gemm_", 5) == 0 || std::strncmp(name, "phase:", 6) == 0;
+  }
   // returns a token; call stop(token) after enqueuing the launch
   // name2 (optional): the record is also added to that second statistic
   int start(hipStream_t s, const char* name, double flops, double bytes, const char* name2 = nullptr);
@@ -303,7 +310,9 @@ struct ProfScope {
   int tok;
   ProfScope(Ctx& c, const char* name, double flops = 0.0, double bytes = 0.0, const char* name2 = nullptr)
       : ctx(c),
-        tok(c.prof.on && name && c.gmode == GraphMode::Off ? c.prof.start(c.stream, name, flops, bytes, name2) : -1) {}
+        tok(c.prof.on && name && c.gmode == GraphMode::Off && c.prof.records(name)
+                ? c.prof.start(c.stream, name, flops, bytes, name2)
+                : -1) {}
   ~ProfScope() { if (tok >= 0) ctx.prof.stop(ctx.stream, tok); }
 };
 

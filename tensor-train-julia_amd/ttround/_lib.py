@@ -147,8 +147,10 @@ class Handle:
             msg = self.lib.ttr_last_error(self.h).decode(errors="replace")
             raise _EXC.get(st, DeviceError)(msg)
 
-    def profile(self, on: bool = True):
-        self.check(self.lib.ttr_profile_enable(self.h, 1 if on else 0))
+    def profile(self, on: bool = True, level: int = 1):
+        """hipEvent timing on the handle's stream: level 1 every timed launch, level 2
+        only the GEMM classes and the phases."""
+        self.check(self.lib.ttr_profile_enable(self.h, level if on else 0))
 
     def graphs(self, on: bool = True):
         """ttr_round_rand through cached HIP graphs (off frees them)."""
