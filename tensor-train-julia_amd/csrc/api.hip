@@ -578,10 +578,8 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     for (size_t q = 0; q < nb; ++q) {
       if (jcore[q] < 0) continue;
       const int64_t m = pdim[q];
-      if (cert[q]) {  // every column kept, in place
+      if (cert[q]) {  // every column kept, in place: B itself is (B J)_k, J = I
         K[q] = m;
-        order_off[q] = orders.size();
-        for (int64_t i = 0; i < m; ++i) orders.push_back(int32_t(i));
         continue;
       }
       const double* sv = sv_host.data() + sv_off[q];
@@ -635,20 +633,17 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
     // reorder columns of B*J and J in place order (into US / Mt scratch); the
     // second gather must not overwrite B before the first reads it: two launches
     std::vector<GatherCols> g1, g2;
-    std::vector<Fill2D> eye;
     for (size_t q = 0; q < nb; ++q) {
-      if (jcore[q] < 0) continue;
+      if (jcore[q] < 0 || cert[q]) continue;
       SweepState& s = st[q];
       const int64_t m = pdim[q], k = K[q];
       int32_t* didx = const_cast<int32_t*>(dorders) + order_off[q];
       g1.push_back(GatherCols{s.B, s.US, m, m, didx, int(m), int(k)});
-      if (cert[q]) eye.push_back(Fill2D{s.B, m, int(m), int(m), 1.0});  // J = I
-      else g2.push_back(GatherCols{s.J, s.B, m, m, didx, int(m), int(k)});
+      g2.push_back(GatherCols{s.J, s.B, m, m, didx, int(m), int(k)});
     }
     gather_cols_batched(ctx, g1);  // US <- (B J)[:, sel]        (m x k)
     // J[:, sel] goes to a fresh buffer (B is free now)
     gather_cols_batched(ctx, g2);  // B  <- J[:, sel]            (m x k)
-    fill2d_batched(ctx, eye);
     // The accumulated Jacobi rotations drift from orthogonality by about 64 eps per
     // applied visit (tens of visits per column); across a sweep of truncating bonds
     // that drift, not the SVD's accuracy, dominates the error.  One Newton-Schulz
@@ -674,8 +669,9 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       }
     }
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "svd_orth_fix");
-    // build the new cores
+    // build the new cores (a certified bond: (B J)_k = B, J_k = I)
     std::vector<Copy2D> cpo;
+    std::vector<Fill2D> eye;
     for (size_t q = 0; q < nb; ++q) {
       if (jcore[q] < 0) continue;
       SweepState& s = st[q];
@@ -683,8 +679,8 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       const int64_t p = P[q], qq = Qd[q], k = K[q], m = pdim[q];
       const bool t = tallcase[q];
       // (m x k): columns u_c s_c of the square factor, and the right rotations
-      const double* BJk = cert[q] ? s.US : s.USh;
-      const double* Jk = cert[q] ? s.B : s.Jh;
+      const double* BJk = cert[q] ? s.B : s.USh;
+      const double* Jk = s.Jh;
       if (left) {
         const int64_t n = s.y->dims[j];
         double* newj = h->core_alloc(s.out.get(), k * qq);
@@ -698,7 +694,8 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
           gb.add(int(rl), int(k), int(p), 1.0, s.cur[jl], 1, rl, BJk, 1, m, 0.0, dst, 1, rl);
         } else {
           // M = Q (R J) J^T: core_j = Jk^T, core_{j-1} <- core_{j-1} (Q (R J)_k)
-          cpo.push_back(copy_job(Jk, 1, m, newj, k, 1, int(m), int(k)));
+          if (cert[q]) eye.push_back(Fill2D{newj, k, int(k), int(m), 1.0});
+          else cpo.push_back(copy_job(Jk, 1, m, newj, k, 1, int(m), int(k)));
           gb.add(int(p), int(k), int(qq), 1.0, s.Qb, 1, p, BJk, 1, m, 0.0, s.Mt, 1, p);
         }
         (void)n;
@@ -716,7 +713,8 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
           gb.add(int(k), int(nr), int(qq), 1.0, BJk, m, 1, s.cur[jr], 1, qq, 0.0, dst, 1, k);
         } else {
           // M = J (Q (R J))^T: U = Jk, S V^T = (Q (B J)_k)^T
-          cpo.push_back(copy_job(Jk, 1, m, newj, 1, p, int(p), int(k)));
+          if (cert[q]) eye.push_back(Fill2D{newj, p, int(p), int(k), 1.0});
+          else cpo.push_back(copy_job(Jk, 1, m, newj, 1, p, int(p), int(k)));
           gb.add(int(k), int(qq), int(p), 1.0, BJk, m, 1, s.Qb, qq, 1, 0.0, s.Mt, 1, k);
         }
         s.cur[j] = newj;
@@ -724,6 +722,7 @@ void svd_sweep(ttr_handle_s* h, std::vector<SweepState>& st, double tol, int64_t
       }
     }
     copy2d_batched(ctx, cpo);
+    fill2d_batched(ctx, eye);
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "svd_sweep_a");
     // second-stage products of the tall-left / wide-right cases
     for (size_t q = 0; q < nb; ++q) {
