@@ -552,7 +552,9 @@ def main():
     if args.cpu_items > 0:
         args.cpu_items = min(args.cpu_items, args.items)
         rel_err, rel_err_raw, cpu, errs = parity_and_cpu(items, specs, op, outs, raw, dims, wl, args)
-    err_in, err_in_raw = input_errors(items, op, outs, raw, max(args.cpu_items, 1))
+    # with the parity leg (checked items only; --cpu-items 0 turns both off)
+    err_in, err_in_raw = (input_errors(items, op, outs, raw, args.cpu_items) if args.cpu_items > 0
+                          else (None, None))
     for o in outs + (raw or []):
         o.free()
     det = deterministic(items, op, wl, args) if args.det_items > 0 else None
