@@ -1369,6 +1369,12 @@ std::vector<std::unique_ptr<DevTT>> round_rand_graphed(ttr_handle_s* h, std::vec
   if (e.state < 0) return round_rand_batch(h, items, o);
   if (e.state == 0 && e.ws_need == 0) {
     ctx.splitk_peak = 0;
+    // uploads as in the capture, so the measured workspace fits it
+    struct UploadAll {
+      Ctx& c;
+      explicit UploadAll(Ctx& cx) : c(cx) { c.upload_all = true; }
+      ~UploadAll() { c.upload_all = false; }
+    } upload_all(ctx);
     auto outs = round_rand_batch(h, items, o);
     e.ws_need = ctx.ws.used() + ((ctx.splitk_peak * sizeof(double) + 255) & ~size_t(255)) + 256;
     e.pin_need = ctx.pin.used() + 256;

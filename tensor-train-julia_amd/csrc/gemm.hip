@@ -57,6 +57,20 @@ template <> __device__ __forceinline__ void store_c<double>(void* p, int64_t i, 
 template <> __device__ __forceinline__ void store_c<float>(void* p, int64_t i, double v) { static_cast<float*>(p)[i] = float(v); }
 template <> __device__ __forceinline__ void store_c<bf16s>(void* p, int64_t i, double v) { static_cast<uint16_t*>(p)[i] = f32_to_bf16(float(v)); }
 
+// The descriptors of a small batch travel in the kernel arguments (the kernel
+// argument segment holds up to 4 KiB), so such a launch needs no upload: one
+// copy dispatch and one dependent launch boundary less.  Larger batches, and
+// launches recorded into a graph (whose replays rewrite uploaded descriptors),
+// pass device pointers.
+constexpr int GEMM_INL_PROBS = 8, GEMM_INL_SEGS = 16;
+template <bool INL> struct GemmInl {};
+template <> struct GemmInl<true> {
+  GemmProb p[GEMM_INL_PROBS];
+  GemmSeg s[GEMM_INL_SEGS];
+  int32_t ids[GEMM_INL_PROBS];  // the problems whose K is split (for the reduction)
+};
+static_assert(sizeof(GemmInl<true>) <= 3072, "GEMM descriptors fit in the kernel arguments");
+
 __device__ __forceinline__ int find_prob(const GemmProb* probs, int nprobs, int bid) {
   int lo = 0, hi = nprobs - 1;
   while (lo < hi) {
@@ -99,10 +113,15 @@ __device__ __forceinline__ int xcd_tile(int bid, int n) {
   return x * q + min(x, r) + j;
 }
 
-template <class TA, class TB, class TC, class MT, int TM, int TN, int BK>
+template <class TA, class TB, class TC, class MT, int TM, int TN, int BK, bool INL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TM * TN >= 128 * 128 ? 2 : 1)))
-void gemm_scalar_kernel(const GemmProb* __restrict__ probs, int nprobs, const GemmSeg* __restrict__ segs) {
+void gemm_scalar_kernel(const GemmProb* __restrict__ probs, int nprobs, const GemmSeg* __restrict__ segs,
+                        GemmInl<INL> inl) {
   static_assert(KS % BK == 0, "a split-K k-tile holds whole k-steps");
+  if constexpr (INL) {
+    probs = inl.p;
+    segs = inl.s;
+  }
   constexpr int MI = TM / 32, NI = TN / 32;
   constexpr int NA = TM * BK / 256, NB = TN * BK / 256;  // tile elements per thread
   __shared__ MT As[2][BK][TM + 1];
@@ -268,9 +287,13 @@ void gemm_scalar_kernel(const GemmProb* __restrict__ probs, int nprobs, const Ge
 
 // C = alpha * (sum of the chunks in chunk order) + beta * C for the split problems
 // (blockIdx.y indexes the list of split problems)
-template <class TC>
+template <class TC, bool INL>
 __global__ __launch_bounds__(256) void gemm_splitk_reduce(const GemmProb* __restrict__ probs,
-                                                          const int32_t* __restrict__ split_ids) {
+                                                          const int32_t* __restrict__ split_ids, GemmInl<INL> inl) {
+  if constexpr (INL) {
+    probs = inl.p;
+    split_ids = inl.ids;
+  }
   const GemmProb P = probs[split_ids[blockIdx.y]];
   const int64_t mn = int64_t(P.M) * P.N;
   for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < mn; e += int64_t(gridDim.x) * blockDim.x) {
@@ -293,10 +316,14 @@ __global__ __launch_bounds__(256) void gemm_splitk_reduce(const GemmProb* __rest
 // kernel: two LDS buffers, the raw loads of k-step i + 1 in flight during the
 // MFMAs of k-step i, one barrier per k-step, unpredicated loads for tiles inside
 // the operands, die-contiguous tile order.  K is never split for BF16.
-template <class TA, class TB, class TC>
+template <class TA, class TB, class TC, bool INL>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(const GemmProb* __restrict__ probs, int nprobs,
-                                                        const GemmSeg* __restrict__ segs) {
+                                                        const GemmSeg* __restrict__ segs, GemmInl<INL> inl) {
   constexpr int BK = 32;
+  if constexpr (INL) {
+    probs = inl.p;
+    segs = inl.s;
+  }
   constexpr int KP = BK + 8;                              // padded row (elements)
   constexpr int NA = BM * BK / 256, NB = BN * BK / 256;  // tile elements per thread
   __shared__ __attribute__((aligned(16))) uint16_t As[2][BM][KP];
@@ -459,39 +486,42 @@ constexpr int NCFG = 8;
 constexpr int CFG_TM[NCFG] = {64, 128, 64, 128, 64, 128, 64, 128};
 constexpr int CFG_TN[NCFG] = {64, 64, 128, 128, 64, 64, 128, 128};
 
-template <class TA, class TB, class TC, class MT>
-void launch_scalar(Ctx& ctx, int cfg, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
+template <class TA, class TB, class TC, class MT, bool INL>
+void launch_scalar(Ctx& ctx, int cfg, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds,
+                   const GemmInl<INL>& inl) {
   dim3 grid(ntiles), block(256);
   switch (cfg) {
-    case 0: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 64, 16>), grid, block, 0, dp, np, ds); break;
-    case 1: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 64, 16>), grid, block, 0, dp, np, ds); break;
-    case 2: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 128, 16>), grid, block, 0, dp, np, ds); break;
-    case 3: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 128, 16>), grid, block, 0, dp, np, ds); break;
-    case 4: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 64, 8>), grid, block, 0, dp, np, ds); break;
-    case 5: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 64, 8>), grid, block, 0, dp, np, ds); break;
-    case 6: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 128, 8>), grid, block, 0, dp, np, ds); break;
-    default: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 128, 8>), grid, block, 0, dp, np, ds); break;
+    case 0: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 64, 16, INL>), grid, block, 0, dp, np, ds, inl); break;
+    case 1: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 64, 16, INL>), grid, block, 0, dp, np, ds, inl); break;
+    case 2: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 128, 16, INL>), grid, block, 0, dp, np, ds, inl); break;
+    case 3: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 128, 16, INL>), grid, block, 0, dp, np, ds, inl); break;
+    case 4: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 64, 8, INL>), grid, block, 0, dp, np, ds, inl); break;
+    case 5: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 64, 8, INL>), grid, block, 0, dp, np, ds, inl); break;
+    case 6: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 64, 128, 8, INL>), grid, block, 0, dp, np, ds, inl); break;
+    default: TTR_LAUNCH(ctx, (gemm_scalar_kernel<TA, TB, TC, MT, 128, 128, 8, INL>), grid, block, 0, dp, np, ds, inl); break;
   }
 }
 
 // the precision combinations the library uses: (A, B, C, math)
-template <class TA, class TB, class TC>
-void launch_combo(Ctx& ctx, Prec math, int cfg, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds) {
-  if (math == Prec::F64) launch_scalar<TA, TB, TC, double>(ctx, cfg, ntiles, dp, np, ds);
-  else if (math == Prec::F32) launch_scalar<TA, TB, TC, float>(ctx, cfg, ntiles, dp, np, ds);
-  else TTR_LAUNCH(ctx, (gemm_bf16_kernel<TA, TB, TC>), dim3(ntiles), dim3(256), 0, dp, np, ds);
+template <class TA, class TB, class TC, bool INL>
+void launch_combo(Ctx& ctx, Prec math, int cfg, int ntiles, const GemmProb* dp, int np, const GemmSeg* ds,
+                  const GemmInl<INL>& inl) {
+  if (math == Prec::F64) launch_scalar<TA, TB, TC, double>(ctx, cfg, ntiles, dp, np, ds, inl);
+  else if (math == Prec::F32) launch_scalar<TA, TB, TC, float>(ctx, cfg, ntiles, dp, np, ds, inl);
+  else TTR_LAUNCH(ctx, (gemm_bf16_kernel<TA, TB, TC, INL>), dim3(ntiles), dim3(256), 0, dp, np, ds, inl);
 }
 
+template <bool INL>
 void launch(Ctx& ctx, Prec ta, Prec tb, Prec tc, Prec math, int cfg, int ntiles, const GemmProb* dp, int np,
-            const GemmSeg* ds) {
+            const GemmSeg* ds, const GemmInl<INL>& inl) {
   using P = Prec;
-  if (ta == P::F64 && tb == P::F64 && tc == P::F64) launch_combo<double, double, double>(ctx, math, cfg, ntiles, dp, np, ds);
-  else if (ta == P::F64 && tb == P::F32 && tc == P::F32) launch_combo<double, float, float>(ctx, math, cfg, ntiles, dp, np, ds);
-  else if (ta == P::F64 && tb == P::BF16 && tc == P::BF16) launch_combo<double, bf16s, bf16s>(ctx, math, cfg, ntiles, dp, np, ds);
-  else if (ta == P::F32 && tb == P::F32 && tc == P::F32) launch_combo<float, float, float>(ctx, math, cfg, ntiles, dp, np, ds);
-  else if (ta == P::BF16 && tb == P::BF16 && tc == P::BF16) launch_combo<bf16s, bf16s, bf16s>(ctx, math, cfg, ntiles, dp, np, ds);
-  else if (ta == P::F64 && tb == P::F32 && tc == P::F64) launch_combo<double, float, double>(ctx, math, cfg, ntiles, dp, np, ds);
-  else if (ta == P::F64 && tb == P::BF16 && tc == P::F64) launch_combo<double, bf16s, double>(ctx, math, cfg, ntiles, dp, np, ds);
+  if (ta == P::F64 && tb == P::F64 && tc == P::F64) launch_combo<double, double, double>(ctx, math, cfg, ntiles, dp, np, ds, inl);
+  else if (ta == P::F64 && tb == P::F32 && tc == P::F32) launch_combo<double, float, float>(ctx, math, cfg, ntiles, dp, np, ds, inl);
+  else if (ta == P::F64 && tb == P::BF16 && tc == P::BF16) launch_combo<double, bf16s, bf16s>(ctx, math, cfg, ntiles, dp, np, ds, inl);
+  else if (ta == P::F32 && tb == P::F32 && tc == P::F32) launch_combo<float, float, float>(ctx, math, cfg, ntiles, dp, np, ds, inl);
+  else if (ta == P::BF16 && tb == P::BF16 && tc == P::BF16) launch_combo<bf16s, bf16s, bf16s>(ctx, math, cfg, ntiles, dp, np, ds, inl);
+  else if (ta == P::F64 && tb == P::F32 && tc == P::F64) launch_combo<double, float, double>(ctx, math, cfg, ntiles, dp, np, ds, inl);
+  else if (ta == P::F64 && tb == P::BF16 && tc == P::F64) launch_combo<double, bf16s, double>(ctx, math, cfg, ntiles, dp, np, ds, inl);
   else throw Error(ST_INVALID_ARG, "unsupported GEMM precision combination");
   TTR_HIP(hipGetLastError());
 }
@@ -624,18 +654,21 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     ntiles[ci] = nt;
     std::memcpy(pack.data() + seg_bytes + ids_bytes + prob_bytes * ci, probs.data(), probs.size() * sizeof(GemmProb));
   }
-  const char* dpack = ctx.upload(pack);
+  const bool inl = !tune && ctx.args_ok() &&
+                   probs.size() <= size_t(GEMM_INL_PROBS) && segs.size() <= size_t(GEMM_INL_SEGS);
+  const char* dpack = inl ? nullptr : ctx.upload(pack);
   const GemmSeg* ds = reinterpret_cast<const GemmSeg*>(dpack);
   const int32_t* di = reinterpret_cast<const int32_t*>(dpack + seg_bytes);
   auto dprobs = [&](size_t ci) { return reinterpret_cast<const GemmProb*>(dpack + seg_bytes + ids_bytes + prob_bytes * ci); };
-  auto reduce = [&](const GemmProb* dp) {
+  auto reduce = [&](const GemmProb* dp, const int32_t* dids, const auto& a) {
     if (split_ids.empty()) return;
+    constexpr bool I = std::is_same<std::decay_t<decltype(a)>, GemmInl<true>>::value;
     int64_t mx = 1;
     for (int32_t i : split_ids) mx = std::max<int64_t>(mx, int64_t(probs[size_t(i)].M) * probs[size_t(i)].N);
     dim3 grid(unsigned(std::min<int64_t>(256, (mx + 255) / 256)), unsigned(split_ids.size())), block(256);
-    if (tc == Prec::F64) TTR_LAUNCH(ctx, gemm_splitk_reduce<double>, grid, block, 0, dp, di);
-    else if (tc == Prec::F32) TTR_LAUNCH(ctx, gemm_splitk_reduce<float>, grid, block, 0, dp, di);
-    else TTR_LAUNCH(ctx, gemm_splitk_reduce<bf16s>, grid, block, 0, dp, di);
+    if (tc == Prec::F64) TTR_LAUNCH(ctx, (gemm_splitk_reduce<double, I>), grid, block, 0, dp, dids, a);
+    else if (tc == Prec::F32) TTR_LAUNCH(ctx, (gemm_splitk_reduce<float, I>), grid, block, 0, dp, dids, a);
+    else TTR_LAUNCH(ctx, (gemm_splitk_reduce<bf16s, I>), grid, block, 0, dp, dids, a);
     TTR_HIP(hipGetLastError());
   };
   static const char* names[3] = {"gemm_f64", "gemm_f32", "gemm_bf16"};
@@ -655,7 +688,7 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     for (size_t ci = 0; ci < cands.size(); ++ci) {
       TTR_HIP(hipEventRecord(e0, ctx.stream));
       log_launch(cands[ci], ntiles[ci]);
-      launch(ctx, ta, tb, tc, math, cands[ci], int(ntiles[ci]), dprobs(ci), np, ds);
+      launch(ctx, ta, tb, tc, math, cands[ci], int(ntiles[ci]), dprobs(ci), np, ds, GemmInl<false>{});
       TTR_HIP(hipEventRecord(e1, ctx.stream));
       TTR_HIP(hipEventSynchronize(e1));
       float ms = 0.0f;
@@ -668,12 +701,21 @@ void gemm_run(Ctx& ctx, GemmBatch& b, Prec ta, Prec tb, Prec tc, Prec math, cons
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ctx.gemm_cfg[key] = best_cfg;  // every candidate wrote the same C (or the same partial sums)
-    reduce(dprobs(0));
+    reduce(dprobs(0), di, GemmInl<false>{});
     return;
   }
   log_launch(cfg, ntiles[0]);
-  launch(ctx, ta, tb, tc, math, cfg, int(ntiles[0]), dprobs(0), np, ds);
-  reduce(dprobs(0));
+  if (inl) {
+    GemmInl<true> d{};
+    std::memcpy(d.p, probs.data(), probs.size() * sizeof(GemmProb));
+    std::memcpy(d.s, segs.data(), segs.size() * sizeof(GemmSeg));
+    std::copy(split_ids.begin(), split_ids.end(), d.ids);
+    launch(ctx, ta, tb, tc, math, cfg, int(ntiles[0]), nullptr, np, nullptr, d);
+    reduce(nullptr, nullptr, d);
+    return;
+  }
+  launch(ctx, ta, tb, tc, math, cfg, int(ntiles[0]), dprobs(0), np, ds, GemmInl<false>{});
+  reduce(dprobs(0), di, GemmInl<false>{});
 }
 
 }  // namespace ttr

@@ -169,8 +169,12 @@ struct BlockJob {
                        // second block of a 128-column pair
 };
 
-template <int R>
-__global__ __launch_bounds__(RT) void qr_block_kernel(const BlockJob* __restrict__ jobs) {
+constexpr int QR_ARG_JOBS = 4;  // jobs of a QR block or T-merge launch passed in its arguments
+
+template <int R, bool INL>
+__global__ __launch_bounds__(RT) void qr_block_kernel(const BlockJob* __restrict__ jobs,
+                                                      ArgJobs<BlockJob, QR_ARG_JOBS, INL> aj) {
+  if constexpr (INL) jobs = aj.v;
   const BlockJob J = jobs[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
@@ -352,7 +356,10 @@ struct TMergeJob {
   int32_t zero_left; // also zero this many 64-column blocks left of T (the rows of a group's T below the diagonal)
 };
 
-__global__ __launch_bounds__(256) void merge_t_kernel(const TMergeJob* __restrict__ jobs) {
+template <bool INL>
+__global__ __launch_bounds__(256) void merge_t_kernel(const TMergeJob* __restrict__ jobs,
+                                                      ArgJobs<TMergeJob, QR_ARG_JOBS, INL> aj) {
+  if constexpr (INL) jobs = aj.v;
   const TMergeJob J = jobs[blockIdx.x];
   const int t = threadIdx.x, nb = J.nb, w = J.w;
   __shared__ double Ts[NBO][NBO + 1];
@@ -394,18 +401,44 @@ void launch_panel(Ctx& ctx, const PanelJob* d, int njobs) {
   TTR_HIP(hipGetLastError());
 }
 
-void launch_block(Ctx& ctx, const BlockJob* d, int njobs, int max_rows) {
+template <bool INL>
+void launch_block_t(Ctx& ctx, const BlockJob* d, const ArgJobs<BlockJob, QR_ARG_JOBS, INL>& a, int njobs,
+                    int max_rows) {
   const int R = (max_rows + RT - 1) / RT;
+  const dim3 g{unsigned(njobs)}, b{unsigned(RT)};
   switch (R) {
     case 0:
-    case 1: TTR_LAUNCH(ctx, qr_block_kernel<1>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-    case 2: TTR_LAUNCH(ctx, qr_block_kernel<2>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-    case 3: TTR_LAUNCH(ctx, qr_block_kernel<3>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-    case 4: TTR_LAUNCH(ctx, qr_block_kernel<4>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-    case 5: TTR_LAUNCH(ctx, qr_block_kernel<5>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
-    default: TTR_LAUNCH(ctx, qr_block_kernel<6>, dim3(unsigned(njobs)), dim3(RT), 0, d); break;
+    case 1: TTR_LAUNCH(ctx, (qr_block_kernel<1, INL>), g, b, 0, d, a); break;
+    case 2: TTR_LAUNCH(ctx, (qr_block_kernel<2, INL>), g, b, 0, d, a); break;
+    case 3: TTR_LAUNCH(ctx, (qr_block_kernel<3, INL>), g, b, 0, d, a); break;
+    case 4: TTR_LAUNCH(ctx, (qr_block_kernel<4, INL>), g, b, 0, d, a); break;
+    case 5: TTR_LAUNCH(ctx, (qr_block_kernel<5, INL>), g, b, 0, d, a); break;
+    default: TTR_LAUNCH(ctx, (qr_block_kernel<6, INL>), g, b, 0, d, a); break;
   }
   TTR_HIP(hipGetLastError());
+}
+
+// one launch of qr_block_kernel over the jobs (timed as "qr_block")
+void launch_block(Ctx& ctx, const std::vector<BlockJob>& bj, int max_rows, double fl, double by) {
+  if (arg_jobs_fit<QR_ARG_JOBS>(ctx, bj)) {
+    ProfScope ps(ctx, "qr_block", fl, by);
+    launch_block_t(ctx, nullptr, arg_jobs<QR_ARG_JOBS>(bj), int(bj.size()), max_rows);
+    return;
+  }
+  const BlockJob* d = ctx.upload(bj);
+  ProfScope ps(ctx, "qr_block", fl, by);
+  launch_block_t(ctx, d, ArgJobs<BlockJob, QR_ARG_JOBS, false>{}, int(bj.size()), max_rows);
+}
+
+void launch_merge_t(Ctx& ctx, const std::vector<TMergeJob>& tb) {
+  if (tb.empty()) return;
+  const dim3 g{unsigned(tb.size())}, b{256u};
+  if (arg_jobs_fit<QR_ARG_JOBS>(ctx, tb)) {
+    TTR_LAUNCH(ctx, merge_t_kernel<true>, g, b, 0, nullptr, arg_jobs<QR_ARG_JOBS>(tb));
+  } else {
+    const TMergeJob* d = ctx.upload(tb);
+    TTR_LAUNCH(ctx, merge_t_kernel<false>, g, b, 0, d, (ArgJobs<TMergeJob, QR_ARG_JOBS, false>{}));
+  }
 }
 
 // Blocked QR of matrices up to 6 * RT rows with the 64-column blocks taken in
@@ -476,9 +509,7 @@ void qr_groups(Ctx& ctx, std::vector<QRJob>& jobs, int G) {
       by += 24.0 * double(mb) * nbo;  // the block read and written, V written
     }
     if (bj.empty()) return;
-    const BlockJob* d = ctx.upload(bj);
-    ProfScope ps(ctx, "qr_block", fl, by);
-    launch_block(ctx, d, int(bj.size()), max_rows);
+    launch_block(ctx, bj, max_rows, fl, by);
   };
   // C -= V (T^T (V^T C)) (trans) or C -= V (T (V^T C)) for every job: V the first
   // kv columns of group s's V, C columns [j0, j1) of the job's A or Q from row NS s
@@ -542,8 +573,7 @@ void qr_groups(Ctx& ctx, std::vector<QRJob>& jobs, int G) {
         tb.push_back(TMergeJob{a.G + h * NBO, a.T + size_t((G * s + h) * NBO / NBR) * NBP * NBP,
                                ts(a, s) + int64_t(h) * NBO * (1 + NS), nb, NBR, NS, NS, h});
       }
-      const TMergeJob* d = ctx.upload(tb);
-      TTR_LAUNCH(ctx, merge_t_kernel, dim3(unsigned(tb.size())), dim3(256), 0, d);
+      launch_merge_t(ctx, tb);
       if (h == 0) continue;
       for (size_t q = 0; q < jobs.size(); ++q) {
         const Aux& a = aux[q];
@@ -723,10 +753,7 @@ void qr_batched(Ctx& ctx, std::vector<QRJob>& jobs) {
       tb.push_back(TMergeJob{a.G, a.T + size_t(c0 / NB) * NBP * NBP, a.TO + size_t(b) * NBO * NBO, nbo, NB, NBO, NBO, 0});
     }
     gemm_run(ctx, gb, Prec::F64, Prec::F64, Prec::F64, Prec::F64, "qr_vtv");
-    if (!tb.empty()) {
-      const TMergeJob* d = ctx.upload(tb);
-      TTR_LAUNCH(ctx, merge_t_kernel, dim3(unsigned(tb.size())), dim3(256), 0, d);
-    }
+    launch_merge_t(ctx, tb);
     // trailing update A2 -= V (T^T (V^T A2))
     for (int stage = 0; stage < 3; ++stage) {
       for (size_t q = 0; q < jobs.size(); ++q) {

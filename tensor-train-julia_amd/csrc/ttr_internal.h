@@ -225,6 +225,11 @@ struct Ctx {
   double* splitk_buffer(size_t elems);
   size_t splitk_peak = 0;  // largest request of the current call (elements)
   GraphMode gmode = GraphMode::Off;
+  // every launch uploads its descriptors (no descriptors in kernel arguments): set
+  // while a call measures the workspace that its graph capture will need
+  bool upload_all = false;
+  // descriptors of a launch may travel in its kernel arguments
+  bool args_ok() const { return gmode == GraphMode::Off && !upload_all; }
   bool launching() const { return gmode != GraphMode::Replay; }
   // host-side record of a captured or replayed call: sizes of every upload and
   // every launch, compared between the capture and each replay
@@ -303,6 +308,24 @@ inline void graph_note_launch(Ctx& ctx, const void* kernel, dim3 grid, dim3 bloc
       TTR_HIP(hipGetLastError());                                                                          \
     }                                                                                                      \
   } while (0)
+
+// Job descriptors of a launch with at most N jobs, passed in the kernel
+// arguments instead of an upload: no copy dispatch and no dependent launch
+// boundary ahead of the kernel.  Launches with more jobs, and launches recorded
+// into a graph (whose replays rewrite uploaded descriptors in place), pass a
+// device array; the kernel then gets the empty ArgJobs<T, N, false>.
+template <class T, int N, bool INL> struct ArgJobs {};
+template <class T, int N> struct ArgJobs<T, N, true> { T v[N]; };
+template <int N, class T>
+inline bool arg_jobs_fit(const Ctx& ctx, const std::vector<T>& jobs) {
+  return ctx.args_ok() && !jobs.empty() && jobs.size() <= size_t(N);
+}
+template <int N, class T>
+inline ArgJobs<T, N, true> arg_jobs(const std::vector<T>& jobs) {
+  ArgJobs<T, N, true> a{};
+  std::copy(jobs.begin(), jobs.end(), a.v);
+  return a;
+}
 
 // RAII timing scope around one or more launches on ctx.stream (off inside graphs)
 struct ProfScope {
