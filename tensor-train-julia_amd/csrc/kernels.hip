@@ -73,10 +73,20 @@ struct Copy2DDev {
   int32_t tile0, tiles_n, tiles_m;
 };
 
+constexpr int SMALL_ARG_JOBS = 8;  // jobs of a copy, fill, gather or scaling launch passed in its arguments
+
+template <bool INL>
 __global__ __launch_bounds__(256) void copy2d_kernel(const Copy2DDev* __restrict__ jobs, const int32_t* __restrict__ prefix,
-                                                     int njobs) {
+                                                     int njobs, ArgJobs<Copy2DDev, SMALL_ARG_JOBS, INL> aj) {
   __shared__ double tile[32][33];
-  const int jid = find_job(prefix, njobs, blockIdx.x);
+  int jid = 0;
+  if constexpr (INL) {
+    jobs = aj.v;
+    for (int q = 1; q < njobs; ++q)
+      if (jobs[q].tile0 <= int(blockIdx.x)) jid = q;
+  } else {
+    jid = find_job(prefix, njobs, blockIdx.x);
+  }
   const Copy2DDev D = jobs[jid];
   const Copy2D& J = D.j;
   int t = blockIdx.x - D.tile0;
@@ -109,7 +119,9 @@ __global__ __launch_bounds__(256) void copy2d_kernel(const Copy2DDev* __restrict
   }
 }
 
-__global__ void fill2d_kernel(const Fill2D* __restrict__ jobs) {
+template <bool INL>
+__global__ void fill2d_kernel(const Fill2D* __restrict__ jobs, ArgJobs<Fill2D, SMALL_ARG_JOBS, INL> aj) {
+  if constexpr (INL) jobs = aj.v;
   const Fill2D J = jobs[blockIdx.y];
   const int64_t tot = int64_t(J.M) * J.N;
   for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
@@ -118,7 +130,9 @@ __global__ void fill2d_kernel(const Fill2D* __restrict__ jobs) {
   }
 }
 
-__global__ void gather_cols_kernel(const GatherCols* __restrict__ jobs) {
+template <bool INL>
+__global__ void gather_cols_kernel(const GatherCols* __restrict__ jobs, ArgJobs<GatherCols, SMALL_ARG_JOBS, INL> aj) {
+  if constexpr (INL) jobs = aj.v;
   const GatherCols J = jobs[blockIdx.y];
   const int64_t tot = int64_t(J.M) * J.ncol;
   for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < tot; e += int64_t(gridDim.x) * blockDim.x) {
@@ -127,8 +141,10 @@ __global__ void gather_cols_kernel(const GatherCols* __restrict__ jobs) {
   }
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void pow2_kernel(const Pow2Job* __restrict__ jobs) {
+template <class T, bool INL>
+__global__ __launch_bounds__(256) void pow2_kernel(const Pow2Job* __restrict__ jobs,
+                                                   ArgJobs<Pow2Job, SMALL_ARG_JOBS, INL> aj) {
+  if constexpr (INL) jobs = aj.v;
   const Pow2Job J = jobs[blockIdx.x];
   T* X = static_cast<T*>(J.X);
   // one column per wave at a time (no 64-bit division per element)
@@ -337,22 +353,50 @@ void copy2d_batched(Ctx& ctx, const std::vector<Copy2D>& jobs) {
   }
   if (dj.empty()) return;
   TTR_REQUIRE(tiles < (int64_t(1) << 31), ST_INVALID_ARG, "copy too large");
-  const Copy2DDev* d = ctx.upload(dj);
-  const int32_t* pre = ctx.upload(prefix);
-  TTR_LAUNCH(ctx, copy2d_kernel, dim3(unsigned(tiles)), dim3(256), 0, d, pre, int(dj.size()));
+  const dim3 g{unsigned(tiles)}, b{256u};
+  if (arg_jobs_fit<SMALL_ARG_JOBS>(ctx, dj)) {
+    TTR_LAUNCH(ctx, copy2d_kernel<true>, g, b, 0, nullptr, nullptr, int(dj.size()), arg_jobs<SMALL_ARG_JOBS>(dj));
+  } else {
+    const Copy2DDev* d = ctx.upload(dj);
+    const int32_t* pre = ctx.upload(prefix);
+    TTR_LAUNCH(ctx, copy2d_kernel<false>, g, b, 0, d, pre, int(dj.size()), (ArgJobs<Copy2DDev, SMALL_ARG_JOBS, false>{}));
+  }
   TTR_HIP(hipGetLastError());
 }
 
+// one workgroup row per job (blockIdx.y), up to 1024 workgroups along x
 template <class J>
 static void launch_y(Ctx& ctx, void (*kern)(const J*), const std::vector<J>& jobs, int64_t max_elems) {
-  if (jobs.empty()) return;
+  int64_t bx = (max_elems + 255) / 256;
+  if (bx > 1024) bx = 1024;
+  if (bx < 1) bx = 1;
   for (size_t s = 0; s < jobs.size(); s += 65535) {
     const size_t cnt = std::min<size_t>(65535, jobs.size() - s);
     const J* d = ctx.upload(jobs.data() + s, cnt);
-    int64_t bx = (max_elems + 255) / 256;
-    if (bx > 1024) bx = 1024;
-    if (bx < 1) bx = 1;
     TTR_LAUNCH(ctx, kern, dim3(unsigned(bx), unsigned(cnt)), dim3(256), 0, d);
+    TTR_HIP(hipGetLastError());
+  }
+}
+
+// the same for a kernel that also takes up to SMALL_ARG_JOBS jobs in its arguments
+template <class J>
+static void launch_y(Ctx& ctx, void (*kern)(const J*, ArgJobs<J, SMALL_ARG_JOBS, false>),
+                     void (*kern_args)(const J*, ArgJobs<J, SMALL_ARG_JOBS, true>), const std::vector<J>& jobs,
+                     int64_t max_elems) {
+  if (jobs.empty()) return;
+  int64_t bx = (max_elems + 255) / 256;
+  if (bx > 1024) bx = 1024;
+  if (bx < 1) bx = 1;
+  if (arg_jobs_fit<SMALL_ARG_JOBS>(ctx, jobs)) {
+    TTR_LAUNCH(ctx, kern_args, dim3(unsigned(bx), unsigned(jobs.size())), dim3(256), 0, nullptr,
+               arg_jobs<SMALL_ARG_JOBS>(jobs));
+    TTR_HIP(hipGetLastError());
+    return;
+  }
+  for (size_t s = 0; s < jobs.size(); s += 65535) {
+    const size_t cnt = std::min<size_t>(65535, jobs.size() - s);
+    const J* d = ctx.upload(jobs.data() + s, cnt);
+    TTR_LAUNCH(ctx, kern, dim3(unsigned(bx), unsigned(cnt)), dim3(256), 0, d, (ArgJobs<J, SMALL_ARG_JOBS, false>{}));
     TTR_HIP(hipGetLastError());
   }
 }
@@ -360,13 +404,13 @@ static void launch_y(Ctx& ctx, void (*kern)(const J*), const std::vector<J>& job
 void fill2d_batched(Ctx& ctx, const std::vector<Fill2D>& jobs) {
   int64_t mx = 0;
   for (auto& j : jobs) mx = std::max<int64_t>(mx, int64_t(j.M) * j.N);
-  launch_y(ctx, fill2d_kernel, jobs, mx);
+  launch_y(ctx, fill2d_kernel<false>, fill2d_kernel<true>, jobs, mx);
 }
 
 void gather_cols_batched(Ctx& ctx, const std::vector<GatherCols>& jobs) {
   int64_t mx = 0;
   for (auto& j : jobs) mx = std::max<int64_t>(mx, int64_t(j.M) * j.ncol);
-  launch_y(ctx, gather_cols_kernel, jobs, mx);
+  launch_y(ctx, gather_cols_kernel<false>, gather_cols_kernel<true>, jobs, mx);
 }
 
 void block_put_batched(Ctx& ctx, const std::vector<BlockPut>& jobs) {
@@ -440,11 +484,19 @@ void op_apply_batched(Ctx& ctx, const std::vector<OpApply>& jobs, Prec in, Prec 
 
 void pow2_normalize_batched(Ctx& ctx, const std::vector<Pow2Job>& jobs, Prec prec) {
   if (jobs.empty()) return;
-  const Pow2Job* d = ctx.upload(jobs);
   dim3 grid(unsigned(jobs.size())), block(256);
-  if (prec == Prec::F64) TTR_LAUNCH(ctx, pow2_kernel<double>, grid, block, 0, d);
-  else if (prec == Prec::F32) TTR_LAUNCH(ctx, pow2_kernel<float>, grid, block, 0, d);
-  else TTR_LAUNCH(ctx, pow2_kernel<uint16_t>, grid, block, 0, d);
+  if (arg_jobs_fit<SMALL_ARG_JOBS>(ctx, jobs)) {
+    const auto a = arg_jobs<SMALL_ARG_JOBS>(jobs);
+    if (prec == Prec::F64) TTR_LAUNCH(ctx, (pow2_kernel<double, true>), grid, block, 0, nullptr, a);
+    else if (prec == Prec::F32) TTR_LAUNCH(ctx, (pow2_kernel<float, true>), grid, block, 0, nullptr, a);
+    else TTR_LAUNCH(ctx, (pow2_kernel<uint16_t, true>), grid, block, 0, nullptr, a);
+  } else {
+    const Pow2Job* d = ctx.upload(jobs);
+    const ArgJobs<Pow2Job, SMALL_ARG_JOBS, false> a{};
+    if (prec == Prec::F64) TTR_LAUNCH(ctx, (pow2_kernel<double, false>), grid, block, 0, d, a);
+    else if (prec == Prec::F32) TTR_LAUNCH(ctx, (pow2_kernel<float, false>), grid, block, 0, d, a);
+    else TTR_LAUNCH(ctx, (pow2_kernel<uint16_t, false>), grid, block, 0, d, a);
+  }
   TTR_HIP(hipGetLastError());
 }
 
