@@ -593,7 +593,14 @@ __global__ __launch_bounds__(Jac<WB_>::JT, Jac<WB_>::MINB) void jacobi_round_ker
 // singular values: column norms of B, CN_COLS columns per workgroup (blockIdx.y),
 // four per wave at once: lane-strided partial sums, then a butterfly
 constexpr int CN_COLS = 16;
-__global__ __launch_bounds__(256) void column_norms_kernel(const SVDJob* __restrict__ jobs) {
+// Also clears the sort's change flag and, when given, the sweep's rotation flag
+// of each matrix (both read only by later launches on the stream).
+__global__ __launch_bounds__(256) void column_norms_kernel(const SVDJob* __restrict__ jobs, int32_t* changed,
+                                                           int32_t* rot_flags) {
+  if (blockIdx.y == 0 && threadIdx.x == 0) {
+    if (changed) changed[blockIdx.x] = 0;
+    if (rot_flags) rot_flags[blockIdx.x] = 0;
+  }
   const SVDJob S = jobs[blockIdx.x];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c0 = blockIdx.y * CN_COLS + 4 * wave;
@@ -736,8 +743,8 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
     max_rounds = std::max(max_rounds, J.rounds);
     fills.push_back(Fill2D{s.J, s.ldj, s.p, s.p, 1.0});
   }
-  const JacJob* dj = ctx.upload(jj);
-  const SVDJob* dl = ctx.upload(live);
+  const JacJob* dj = nullptr;
+  const SVDJob* dl = nullptr;
   // split launches when some matrix has more than one row range: partial Grams,
   // V and the rotate flag of every visit slot
   const bool split = max_split > 1;
@@ -746,8 +753,6 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   double* vbuf = split ? ctx.ws.alloc_n<double>(size_t(slots) * WC * WC) : nullptr;
   int32_t* rotf = split ? ctx.ws.alloc_n<int32_t>(size_t(slots)) : nullptr;
   int32_t* flags = ctx.ws.alloc_n<int32_t>(size_t(2) * nj);
-  fill2d_batched(ctx, fills);
-  zero_fill(ctx, flags, sizeof(int32_t) * 2 * nj);
   const int max_sweeps = std::max(1, std::min(MAX_SWEEPS, ctx.jacobi_max_sweeps));
   // de Rijk's pivoting: before every sweep the columns of B (and, to keep
   // B = B_in J, of J) are sorted by decreasing norm, so each column block holds
@@ -763,26 +768,44 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   const SVDJob* dn = nullptr;
   const PermJob* dperm = nullptr;
   int32_t* changed = nullptr;
+  std::vector<SVDJob> nj_jobs;
+  std::vector<PermJob> pj;
   if (ctx.jacobi_sort) {
     double* nrm = ctx.ws.alloc_n<double>(size_t(off[nj]));
     int32_t* perm = ctx.ws.alloc_n<int32_t>(size_t(off[nj]));
     changed = ctx.ws.alloc_n<int32_t>(size_t(nj));
-    std::vector<SVDJob> nj_jobs = live;
-    std::vector<PermJob> pj(static_cast<size_t>(nj));
+    nj_jobs = live;
+    pj.resize(static_cast<size_t>(nj));
     for (int q = 0; q < nj; ++q) {
       const int p = live[q].p;
       nj_jobs[q].sv = nrm + off[q];
       pj[q] = PermJob{live[q].B, live[q].J, ctx.ws.alloc_n<double>(size_t(p) * p), ctx.ws.alloc_n<double>(size_t(p) * p),
                       live[q].ldb, live[q].ldj, nrm + off[q], perm + off[q], p};
     }
-    dn = ctx.upload(nj_jobs);
-    dperm = ctx.upload(pj);
   }
+  {  // the run's descriptors in one upload: Jacobi jobs, SVD jobs, then the sort's jobs
+    auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
+    const size_t o1 = al(jj.size() * sizeof(JacJob)), o2 = o1 + al(live.size() * sizeof(SVDJob)),
+                 o3 = o2 + al(nj_jobs.size() * sizeof(SVDJob)), o4 = o3 + pj.size() * sizeof(PermJob);
+    std::vector<char> pack(o4);
+    std::memcpy(pack.data(), jj.data(), jj.size() * sizeof(JacJob));
+    std::memcpy(pack.data() + o1, live.data(), live.size() * sizeof(SVDJob));
+    if (!nj_jobs.empty()) std::memcpy(pack.data() + o2, nj_jobs.data(), nj_jobs.size() * sizeof(SVDJob));
+    if (!pj.empty()) std::memcpy(pack.data() + o3, pj.data(), pj.size() * sizeof(PermJob));
+    const char* d = ctx.upload(pack);
+    dj = reinterpret_cast<const JacJob*>(d);
+    dl = reinterpret_cast<const SVDJob*>(d + o1);
+    if (!nj_jobs.empty()) {
+      dn = reinterpret_cast<const SVDJob*>(d + o2);
+      dperm = reinterpret_cast<const PermJob*>(d + o3);
+    }
+  }
+  fill2d_batched(ctx, fills);
+  zero_fill(ctx, flags, sizeof(int32_t) * 2 * nj);
   // the sort runs on the device (no host round trip per sweep)
   const unsigned perm_x = unsigned(std::min<int64_t>(64, (int64_t(max_p) * max_p + 255) / 256));
-  auto sort_columns = [&]() {
-    TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj), unsigned(cn_y)), dim3(256), 0, dn);
-    zero_fill(ctx, changed, sizeof(int32_t) * size_t(nj));
+  auto sort_columns = [&](int32_t* rot_flags) {
+    TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj), unsigned(cn_y)), dim3(256), 0, dn, changed, rot_flags);
     TTR_LAUNCH(ctx, sort_perm_kernel, dim3(unsigned(nj), unsigned((max_p + 255) / 256)), dim3(256), 0, dperm, changed);
     TTR_LAUNCH(ctx, permute_gather_kernel, dim3(perm_x, unsigned(nj)), dim3(256), 0, dperm,
                static_cast<const int32_t*>(changed));
@@ -817,8 +840,9 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
   bool converged = false;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
     int32_t* cur = flags + size_t(sweep & 1) * nj;
-    if (sweep >= 2) zero_fill(ctx, cur, sizeof(int32_t) * nj);
-    if (ctx.jacobi_sort) sort_columns();
+    // the flags of sweep - 2 were copied out; they start at zero again
+    if (ctx.jacobi_sort) sort_columns(sweep >= 2 ? cur : nullptr);
+    else if (sweep >= 2) zero_fill(ctx, cur, sizeof(int32_t) * nj);
     for (int r = 0; r < max_rounds; ++r) {
       // timed per launch; the work depends on the sweeps taken, so the caller
       // adds it from the counters
@@ -865,7 +889,7 @@ void jacobi_run(Ctx& ctx, std::vector<SVDJob>& jobs) {
       }
     throw Error(ST_NUMERICAL_FAILURE, msg + ")");
   }
-  TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj), unsigned(cn_y)), dim3(256), 0, dl);
+  TTR_LAUNCH(ctx, column_norms_kernel, dim3(unsigned(nj), unsigned(cn_y)), dim3(256), 0, dl, nullptr, nullptr);
   TTR_HIP(hipGetLastError());
 }
 
