@@ -507,12 +507,15 @@ def main():
         for o in outs:
             o.free()
 
-    # The timed steps record hipEvents around the dominant kernel class's launches.
-    # Where that class is the FP64 GEMM (C3), only the GEMM classes and the phases are
-    # timed (each timed launch costs ~3 us of host time for its two events); the
-    # other classes come from one more, fully profiled, untimed step afterwards.
+    # Each timed launch costs ~3 us of host time for its two hipEvents.  Where the
+    # dominant kernel class is the FP64 GEMM (C3: long launches), the timed steps
+    # record events around the GEMM classes and the phases only; the other classes
+    # come from one more, fully profiled, untimed step afterwards.  The other
+    # workloads are chains of short dependent launches, where events on every launch
+    # would slow the steps themselves: their timed steps record no events and all
+    # kernel statistics come from that extra step.
     gemm_only = wl.get("gemm_dominant", False) and not args.no_events
-    h.profile(not args.no_events, level=2 if gemm_only else 1)
+    h.profile(gemm_only, level=2)
     h.profile_reset()
     t0 = time.perf_counter()
     for s in range(args.steps):
@@ -526,7 +529,7 @@ def main():
     h.profile(False)
     jacobi, sites, phases, roof, kernels, cleanup_stats = summarize(h.profile_stats(), args)
     kernel_source = "timed steps"
-    if gemm_only:
+    if not args.no_events:
         h.profile(True, level=1)
         h.profile_reset()
         extra_outs, _ = run_step(items, op, wl, args)
@@ -535,10 +538,14 @@ def main():
         h.profile(False)
         one = argparse.Namespace(**vars(args))
         one.steps = 1
-        jacobi, _, _, _, kx, cleanup_stats = summarize(h.profile_stats(), one)
-        kernels = {**{k: v for k, v in kx.items() if not k.startswith("gemm_")}, **kernels}
-        kernel_source = "GEMM classes: timed steps; other classes: one more fully profiled step after them"
-        if roof:
+        if not gemm_only:
+            jacobi, sites, phases, roof, kernels, cleanup_stats = summarize(h.profile_stats(), one)
+            kernel_source = "one fully profiled step after the timed steps (which record no events)"
+        else:
+            jacobi, _, _, _, kx, cleanup_stats = summarize(h.profile_stats(), one)
+            kernels = {**{k: v for k, v in kx.items() if not k.startswith("gemm_")}, **kernels}
+            kernel_source = "GEMM classes: timed steps; other classes: one more fully profiled step after them"
+        if roof and gemm_only:
             tot = sum(v["ms_per_step"] for v in kernels.values())
             roof["share_of_kernel_time"] = round(kernels[roof["kernel"]]["ms_per_step"] / tot, 4)
             top = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
